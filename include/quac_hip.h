@@ -1,0 +1,181 @@
+/*
+ * quac_hip.h -- the thin C-ABI between the QuaC-compatible C host layer
+ * (include/quac.h) and the hand-written gfx950 kernels of libquac.so.
+ *
+ * Plain pointers and sizes only.  Every entry point returns 0 on success or a
+ * negative QH_E* code; qh_last_error() returns a message for the last failure.
+ * Device objects (qh_mat, qh_vec, qh_batch) live in HBM on the current device
+ * and are driven on one HIP stream per process (qh_stream()).
+ *
+ * Each entry point names the reference interface whose work it replaces.
+ */
+#ifndef QUAC_HIP_H
+#define QUAC_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { QH_OK = 0, QH_EINVAL = -1, QH_ENOMEM = -2, QH_EHIP = -3, QH_ERANGE = -4, QH_ENODEV = -5 };
+
+/* ------------------------------------------------------------------ runtime */
+/* replaces the device side of QuaC_initialize/QuaC_finalize (src/quac.c:25-58, 89-141) */
+int qh_init(int device);
+int qh_finalize(void);
+const char *qh_last_error(void);
+int qh_device_count(void);
+int qh_synchronize(void);
+/* allocate / free raw device memory (plumbing for the host layer and tests) */
+int qh_dmalloc(void **p, size_t bytes);
+int qh_dfree(void *p);
+int qh_h2d(void *dst, const void *src, size_t bytes);
+int qh_d2h(void *dst, const void *src, size_t bytes);
+
+/* ----------------------------------------------------- superoperator assembly */
+/*
+ * A monomial operator for device assembly.  Every QuaC operator (and a VEC
+ * pair |p1><p2|) maps a basis level to at most one level, so it is described
+ * by a per-level table: next[l] (-1: no entry), prev[l] (inverse map), value.
+ * stride = n_after = total_levels / (levels * n_before) (kron.c:315).
+ */
+typedef struct { int64_t stride; int32_t levels; int32_t tab; } qh_op;
+typedef struct { int32_t next, prev; double re, im; } qh_level;
+/* term kinds = the dispatch of _add_ops_to_mat (kron.c:953-993) */
+enum { QH_HAM_DM = 0, QH_LIN_DM = 1, QH_HAM_ONLY = 2, QH_LIN_MCWF = 3 };
+typedef struct { int32_t kind, nops, op0, zero_a; double a_re, a_im; } qh_term;
+enum { QH_ASM_SCALE_MINUS_I = 1, QH_ASM_CONJ_A = 2, QH_ASM_FP32 = 4 };
+
+typedef struct qh_mat qh_mat;
+/*
+ * Build the CSR of construct_matrix (qsystem.c:593-676; kron.c:638-1036) on the
+ * device: rows in PETSc order with sorted columns, explicit zeros kept, every
+ * diagonal stored, duplicates summed in insertion order (term, row, part).
+ * dim = rows (N^2 for DM, N for WF/MCWF).  flags: QH_ASM_*.
+ */
+int qh_assemble(int64_t tot, int64_t dim, int nterms, const qh_term *terms, int nops, const qh_op *ops,
+                int nlev, const qh_level *levels, int flags, qh_mat **out);
+int qh_mat_destroy(qh_mat *m);
+int qh_mat_info(const qh_mat *m, int64_t *rows, int64_t *nnz);
+/* copy the CSR back (int64 like PetscInt; values complex128 interleaved) */
+int qh_mat_export(const qh_mat *m, int64_t *rowptr, int64_t *col, double *val);
+/* make / refresh the complex64 copy of the values used by the FP32 path */
+int qh_mat_make_fp32(qh_mat *m);
+/* block-diagonal concatenation of n matrices (batched parameter sweeps) */
+int qh_mat_blockdiag(int n, qh_mat *const *mats, qh_mat **out);
+/* select SpMV variant: lanes per row G in {2,4,8,16,32,64}; 0 = autotune */
+int qh_mat_set_lanes(qh_mat *m, int lanes64, int lanes32);
+int qh_mat_get_lanes(const qh_mat *m, int *lanes64, int *lanes32);
+
+/* ------------------------------------------------------------------ vectors */
+typedef struct qh_vec qh_vec;
+enum { QH_FP64 = 64, QH_FP32 = 32 };
+/* complex vector of n entries; FP32 vectors carry a compensation word */
+int qh_vec_create(int64_t n, int prec, qh_vec **out);
+int qh_vec_destroy(qh_vec *v);
+int qh_vec_size(const qh_vec *v, int64_t *n, int *prec);
+int qh_vec_set(qh_vec *v, const double *host_c128);        /* whole vector, complex128 */
+int qh_vec_get(const qh_vec *v, double *host_c128);
+int qh_vec_zero(qh_vec *v);
+int qh_vec_add_entry(qh_vec *v, int64_t loc, double re, double im);  /* VecSetValue(ADD_VALUES) */
+int qh_vec_get_entry(const qh_vec *v, int64_t loc, double *re, double *im);
+int qh_vec_copy(const qh_vec *src, qh_vec *dst);           /* any precision to any */
+int qh_vec_scale(qh_vec *v, double re, double im);
+/* raw device pointer (double2* for FP64, float2* hi for FP32) */
+void *qh_vec_data(qh_vec *v);
+
+/* y = A x (MatMult, used by qsystem.c:937,954,1310) */
+int qh_spmv(const qh_mat *A, const qh_vec *x, qh_vec *y);
+
+/* --------------------------------------------------------------- stepping */
+/*
+ * Fixed-step classic RK4 on dx/dt = A x (time_step_sys with -ts_rk_type 4
+ * -ts_adapt_type none, qsystem.c:895-1098).  Steps run as fused SpMV+stage
+ * kernels captured in a HIP graph; x (FP64 or FP32) is advanced in place.
+ * renorm_every > 0 (FP32 only): every that many steps the trace of the density
+ * matrix (dimension tot) is computed in FP64 and rho is rescaled to unit trace.
+ */
+int qh_rk4(qh_mat *A, qh_vec *x, double dt, int64_t nsteps, int64_t tot, int renorm_every);
+/* same steps launched eagerly with a HIP event pair around every stage kernel;
+ * the mean stage-kernel duration is then returned by qh_last_stage_ms */
+int qh_rk4_timed(qh_mat *A, qh_vec *x, double dt, int64_t nsteps, int64_t tot, int renorm_every);
+/*
+ * Time-dependent RHS (qsystem.c:1288-1330): y = A x + sum_k s_k(t) A_k x with
+ * s_k given per stage by the host.  Used by the host RK4 loop for td terms.
+ */
+int qh_spmv_td(const qh_mat *A, int ntd, qh_mat *const *Ak, const double *s_re_im,
+               const qh_vec *x, qh_vec *y);
+/* generic stage helpers for host-driven integrators (RK3BS, td RK4) */
+int qh_axpy_to(qh_vec *y, const qh_vec *x, double c_re, double c_im, const qh_vec *k); /* y = x + c k */
+int qh_lincomb(qh_vec *y, int n, const double *c_re_im, qh_vec *const *v);             /* y = sum c_i v_i */
+/* weighted RMS error of RK3BS: sqrt(mean(|e|^2/(atol+rtol max(|x|,|xn|))^2)) */
+int qh_rk3bs_err(const qh_vec *x, const qh_vec *xn, const qh_vec *k1, const qh_vec *k2,
+                 const qh_vec *k3, const qh_vec *k4, double h, double atol, double rtol, double *err);
+
+/* ------------------------------------------------------------- observables */
+/* Tr(ops rho) (qvec_utilities.c:1197-1281), FP64 accumulation on the device */
+int qh_expect_dm(const qh_vec *x, int64_t tot, int nops, const qh_op *ops, int nlev,
+                 const qh_level *levels, double *re, double *im);
+int qh_expect_wf(const qh_vec *x, int64_t tot, int nops, const qh_op *ops, int nlev,
+                 const qh_level *levels, double *re, double *im);
+/* trace of a DM (get_trace_qvec, qvec_utilities.c:378-392) */
+int qh_trace(const qh_vec *x, int64_t tot, double *re, double *im);
+/* <n_s> for every subsystem s at once: Re rho_ii weighted by level (populations) */
+int qh_populations(const qh_vec *x, int nsub, const int64_t *levels, const int64_t *strides, int dm,
+                   double *pops);
+
+/* ------------------------------------------------------------------- gates */
+/*
+ * rho <- U rho U^dagger in place (U on the row index, U* on the column index),
+ * or psi <- U psi for a WF.  nq = 1 or 2; u = 2x2 or 4x4 complex128 row-major;
+ * strides = n_after of the subsystems the gate acts on (first = most
+ * significant bit of the gate basis).  Replaces _apply_gate_sys
+ * (quantum_circuits.c:435-481) and _apply_gate2 (:696-772).
+ */
+int qh_gate(qh_vec *x, int64_t tot, int dm, int nq, const int64_t *strides, const double *u);
+
+/* ------------------------------------------------------- batched sweeps (FP32) */
+/*
+ * A batch of independent instances of uneven size advanced together: one
+ * block-diagonal CSR, one fused FP32 SpMV+stage kernel per RK stage over all
+ * instances, hipGraph-captured stepping, per-instance FP64 trace renormalisation
+ * and populations.  Replaces running time_step_sys once per instance.
+ */
+typedef struct qh_batch qh_batch;
+int qh_batch_create(int n, qh_mat *const *mats, const int64_t *tot, int prec, qh_batch **out);
+int qh_batch_destroy(qh_batch *b);
+int qh_batch_set_state(qh_batch *b, int i, const double *host_c128);
+int qh_batch_get_state(qh_batch *b, int i, double *host_c128);
+int qh_batch_rk4(qh_batch *b, double dt, int64_t nsteps, int renorm_every);
+/* populations of each instance: nsub[i] subsystems, levels/strides packed */
+int qh_batch_populations(qh_batch *b, const int *nsub, const int64_t *levels, const int64_t *strides,
+                         double *pops);
+int qh_batch_info(const qh_batch *b, int64_t *rows, int64_t *nnz);
+/* the batch's block-diagonal matrix, concatenated state, and instance offsets */
+qh_mat *qh_batch_matrix(qh_batch *b);
+qh_vec *qh_batch_state_vec(qh_batch *b);
+int64_t qh_batch_offset(qh_batch *b, int i);
+
+/* ------------------------------------------------------------ steady state */
+/*
+ * GMRES(restart) on A' x = e0 with A' = A + e0 (trace row), solver.c:33-220;
+ * FP64, CGS2 Gram-Schmidt, optional Jacobi right preconditioner; x holds the
+ * initial guess.  Stops at ||b - A'x|| <= tol or maxit iterations.
+ */
+int qh_gmres(const qh_mat *A, int64_t tot, qh_vec *x, int restart, double tol, int64_t maxit,
+             int jacobi, int64_t *iters, double *res);
+
+/* ------------------------------------------------------------- timing hooks */
+/* average duration (ms) of the last qh_rk4 stage kernels, measured with events */
+int qh_last_stage_ms(double *spmv_ms, int64_t *launches);
+/* set the number of RK steps captured per graph launch (default 8) */
+int qh_set_graph_steps(int steps);
+/* mean duration (ms) of reps plain SpMV launches y = A x on the chosen precision */
+int qh_bench_spmv(qh_mat *A, int prec, int reps, double *avg_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -639,8 +639,8 @@ static int gate_u4(int gate, cx u[4][4])
     u[0][0] = u[1][1] = cmk(1, 0); u[2][3] = u[3][2] = cmk(1, 0); break;
   case QO_CZ:
     u[0][0] = u[1][1] = u[2][2] = cmk(1, 0); u[3][3] = cmk(-1, 0); break;
-  case QO_CXZ: /* control X then Z on target: |11> -> -|10>, |10> -> |11> */
-    u[0][0] = u[1][1] = cmk(1, 0); u[2][3] = cmk(1, 0); u[3][2] = cmk(-1, 0); break;
+  case QO_CXZ: /* I2 (+) sig_x sig_z, quantum_circuits.c:1076-1080 */
+    u[0][0] = u[1][1] = cmk(1, 0); u[2][3] = cmk(-1, 0); u[3][2] = cmk(1, 0); break;
   default:
     return QO_ERR_ARG;
   }

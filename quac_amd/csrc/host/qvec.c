@@ -1,0 +1,316 @@
+/* qvec.c -- state containers and observables of the QuaC-compatible host layer
+ * (src/qvec_utilities.c; line numbers cited per function).  The data live in
+ * HBM; observables are reduced in FP64 on the device. */
+#include <math.h>
+#include <string.h>
+
+#include "quac_internal.h"
+
+static qvec new_qvec(qvec_type type, PetscInt n, PetscInt total_levels) {
+  qvec q = (qvec)calloc(1, sizeof(struct qvec));
+  q->my_type = type;
+  q->n = n;
+  q->Istart = 0;
+  q->Iend = n;
+  q->total_levels = total_levels;
+  q->n_ensemble = -1;
+  q->ens_i = -1;
+  q->ens_spawned = PETSC_FALSE;
+  if (qh_vec_create(n, QH_FP64, &q->data)) {
+    free(q);
+    quac_fail(9, "cannot allocate a state of %lld entries: %s", (long long)n, qh_last_error());
+    return NULL;
+  }
+  return q;
+}
+
+/* qvec_utilities.c:633-648 */
+void create_qvec_sys(qsystem qsys, qvec *new_qvec) {
+  if (qsys->dm_equations == PETSC_TRUE && qsys->mcwf_solver == PETSC_FALSE) create_dm_sys(qsys, new_qvec);
+  else create_wf_sys(qsys, new_qvec);
+}
+
+/* qvec_utilities.c:654-705 */
+void create_dm_sys(qsystem qsys, qvec *new_dm) {
+  if (quac_ensure_device()) return;
+  if (qsys->num_subsystems == 0) QUAC_FAIL(0, "You need to create operators before you construct a density matrix!");
+  qsys->dm_equations = PETSC_TRUE;
+  qsys->dim = qsys->total_levels * qsys->total_levels;
+  qsys->hspace_frozen = 1;
+  qvec q = new_qvec(DENSITY_MATRIX, qsys->dim, qsys->total_levels);
+  if (!q) return;
+  q->n_ops = qsys->num_subsystems;
+  q->ndims_hspace = 2 * qsys->num_subsystems;
+  q->hspace_dims = (PetscInt *)malloc(2 * qsys->num_subsystems * sizeof(PetscInt));
+  for (PetscInt i = 0; i < qsys->num_subsystems; i++) {
+    q->hspace_dims[i] = qsys->subsystem_list[i]->my_levels;
+    q->hspace_dims[i + qsys->num_subsystems] = qsys->subsystem_list[i]->my_levels;
+  }
+  *new_dm = q;
+}
+
+/* qvec_utilities.c:710-760 */
+void create_wf_sys(qsystem qsys, qvec *new_wf) {
+  if (quac_ensure_device()) return;
+  if (qsys->num_subsystems == 0) QUAC_FAIL(0, "You need to create operators before you construct a wavefunction!");
+  if (qsys->dm_equations == PETSC_TRUE && qsys->mcwf_solver == PETSC_FALSE)
+    QUAC_FAIL(0, "Must use density matrices if Lindblad terms are used without using mcwf_solver.");
+  qsys->dim = qsys->total_levels;
+  qsys->hspace_frozen = 1;
+  qvec q = new_qvec(WAVEFUNCTION, qsys->dim, qsys->total_levels);
+  if (!q) return;
+  q->n_ops = qsys->num_subsystems;
+  q->ndims_hspace = qsys->num_subsystems;
+  q->hspace_dims = (PetscInt *)malloc(qsys->num_subsystems * sizeof(PetscInt));
+  for (PetscInt i = 0; i < qsys->num_subsystems; i++) q->hspace_dims[i] = qsys->subsystem_list[i]->my_levels;
+  *new_wf = q;
+}
+
+void create_arb_qvec(qvec *out, PetscInt n, qvec_type type) {
+  if (quac_ensure_device()) return;
+  PetscInt tl = n;
+  if (type == DENSITY_MATRIX) {
+    tl = (PetscInt)llround(sqrt((double)n));
+    if (tl * tl != n) QUAC_FAIL(9, "n for DENSITY_MATRIX must be perfect square!");
+  }
+  qvec q = new_qvec(type, n, tl);
+  if (!q) return;
+  q->n_ops = 1;
+  q->ndims_hspace = type == DENSITY_MATRIX ? 2 : 1;
+  q->hspace_dims = (PetscInt *)malloc(2 * sizeof(PetscInt));
+  q->hspace_dims[0] = q->hspace_dims[1] = tl;
+  *out = q;
+}
+
+/* qvec_utilities.c:1076-1090 */
+void destroy_qvec(qvec *state) {
+  if (!state || !*state) return;
+  free((*state)->hspace_dims);
+  qh_vec_destroy((*state)->data);
+  free(*state);
+  *state = NULL;
+}
+
+/* qvec_utilities.c:879-891 */
+void add_to_qvec_loc(qvec state, PetscScalar alpha, PetscInt loc) {
+  if (loc > state->n || loc < 0) QUAC_FAIL(0, "Adding to a qvec location that is out of range!");
+  if (loc == state->n) QUAC_FAIL(0, "Adding to a qvec location that is out of range!");
+  QUAC_HIPCHK(qh_vec_add_entry(state->data, loc, creal(alpha), cimag(alpha)));
+}
+
+/* qvec_utilities.c:919-965: DM takes (i, j) with loc = sqrt(n)*i + j */
+void add_to_qvec(qvec state, PetscScalar alpha, ...) {
+  va_list ap;
+  va_start(ap, alpha);
+  PetscInt loc = va_arg(ap, PetscInt);
+  if (state->my_type == DENSITY_MATRIX) {
+    PetscInt loc2 = va_arg(ap, PetscInt);
+    loc = state->total_levels * loc + loc2;
+  }
+  va_end(ap);
+  add_to_qvec_loc(state, alpha, loc);
+}
+
+/* qvec_utilities.c:996-1033 (keeps the reference's n_before-weighted location) */
+void get_qvec_loc_fock_op_list(qvec state, PetscInt *loc, PetscInt num_ops, operator *ops, PetscInt *fock) {
+  *loc = 0;
+  for (PetscInt i = 0; i < num_ops; i++) {
+    if (fock[i] >= ops[i]->my_levels || fock[i] < 0)
+      QUAC_FAIL(0, "The Fock State must be less than the number of levels and greater than 0!");
+    *loc += fock[i] * ops[i]->n_before;
+  }
+  if (state->my_type == DENSITY_MATRIX) *loc = *loc * state->total_levels + *loc;
+}
+
+void add_to_qvec_fock_op_list(PetscScalar alpha, qvec state, PetscInt num_ops, operator *ops, PetscInt *fock) {
+  PetscInt loc = 0;
+  quac_clear_error();
+  get_qvec_loc_fock_op_list(state, &loc, num_ops, ops, fock);
+  if (quac_last_error()[0]) return;
+  add_to_qvec_loc(state, alpha, loc);
+}
+
+/* qvec_utilities.c:817-842 */
+void add_to_qvec_fock_op(PetscScalar alpha, qvec state, PetscInt num_ops, ...) {
+  va_list ap;
+  operator *ops = (operator *)malloc(sizeof(operator) * (num_ops > 0 ? num_ops : 1));
+  PetscInt *fock = (PetscInt *)malloc(sizeof(PetscInt) * (num_ops > 0 ? num_ops : 1));
+  va_start(ap, num_ops);
+  for (PetscInt i = 0; i < num_ops; i++) {
+    ops[i] = va_arg(ap, operator);
+    fock[i] = va_arg(ap, PetscInt);
+  }
+  va_end(ap);
+  add_to_qvec_fock_op_list(alpha, state, num_ops, ops, fock);
+  free(ops);
+  free(fock);
+}
+
+/* qvec_utilities.c:1047-1064: device writes are already in place; order them */
+void assemble_qvec(qvec state) { (void)state; }
+
+/* qvec_utilities.c:2540-2566 */
+void set_qvec_from_init_excited_op(qsystem qsys, qvec state) {
+  PetscInt init = 0;
+  for (PetscInt i = 0; i < qsys->num_subsystems; i++) {
+    PetscInt n_after = qsys->total_levels / (qsys->subsystem_list[i]->my_levels * qsys->subsystem_list[i]->n_before);
+    init += qsys->subsystem_list[i]->initial_exc * n_after;
+  }
+  if (qsys->dm_equations == PETSC_TRUE) init = qsys->total_levels * init + init;
+  add_to_qvec_loc(state, 1.0, init);
+}
+
+/* qsystem.c:1340-1349 */
+void set_init_excited_op(operator op, PetscInt initial_exc) {
+  if (initial_exc >= op->my_levels && op->my_op_type != VEC)
+    QUAC_FAIL(0, "The initial excitation level must be less than the number of levels and can't be used for VEC_OP!");
+  op->initial_exc = initial_exc;
+}
+
+/* qvec_utilities.c:1176-1195, 1197-1281 */
+void get_expectation_value_qvec_list(qvec state, PetscScalar *trace_val, PetscInt num_ops, operator *ops) {
+  qc_chain c;
+  qc_chain_init(&c);
+  if (qc_chain_append(&c, state->total_levels, num_ops, ops) < 0) {
+    qc_chain_free(&c);
+    return;
+  }
+  double re = 0.0, im = 0.0;
+  int rc;
+  if (state->my_type == DENSITY_MATRIX)
+    rc = qh_expect_dm(state->data, state->total_levels, c.nops, c.ops, c.nlev, c.lev, &re, &im);
+  else
+    rc = qh_expect_wf(state->data, state->total_levels, c.nops, c.ops, c.nlev, c.lev, &re, &im);
+  qc_chain_free(&c);
+  if (rc) QUAC_FAIL(9, "expectation value failed: %s", qh_last_error());
+  *trace_val = re + im * PETSC_i;
+}
+
+/* qvec_utilities.c:1133-1151 */
+void get_expectation_value_qvec(qvec state, PetscScalar *trace_val, PetscInt num_ops, ...) {
+  va_list ap;
+  va_start(ap, num_ops);
+  operator *ops = (operator *)malloc(sizeof(operator) * (num_ops > 0 ? num_ops : 1));
+  for (PetscInt i = 0; i < num_ops; i++) ops[i] = va_arg(ap, operator);
+  va_end(ap);
+  get_expectation_value_qvec_list(state, trace_val, num_ops, ops);
+  free(ops);
+}
+
+/* qvec_utilities.c:378-392 */
+void get_trace_qvec(qvec dm, PetscScalar *val) {
+  if (dm->my_type != DENSITY_MATRIX) QUAC_FAIL(0, "Can only take the trace of density matrices!");
+  double re, im;
+  QUAC_HIPCHK(qh_trace(dm->data, dm->total_levels, &re, &im));
+  *val = re + im * PETSC_i;
+}
+
+/* qvec_utilities.c:322-337: loc = sqrt(n)*col + row */
+void get_dm_element_qvec(qvec dm, PetscInt row, PetscInt col, PetscScalar *val) {
+  double re, im;
+  QUAC_HIPCHK(qh_vec_get_entry(dm->data, dm->total_levels * col + row, &re, &im));
+  *val = re + im * PETSC_i;
+}
+
+void get_wf_element_qvec(qvec wf, PetscInt i, PetscScalar *val) {
+  double re, im;
+  QUAC_HIPCHK(qh_vec_get_entry(wf->data, i, &re, &im));
+  *val = re + im * PETSC_i;
+}
+
+/* qvec_utilities.c:1682-1717 (DM: Re diag(rho); WF: |psi_i|^2), arrays malloc'd */
+void get_bitstring_probs(qvec state, PetscInt *nloc, PetscReal **probs, PetscReal **vars) {
+  PetscInt N = state->total_levels;
+  double *h = (double *)malloc(sizeof(double) * 2 * state->n);
+  if (qh_vec_get(state->data, h)) {
+    free(h);
+    QUAC_FAIL(9, "get_bitstring_probs: %s", qh_last_error());
+  }
+  *nloc = N;
+  *probs = (PetscReal *)malloc(sizeof(PetscReal) * N);
+  *vars = (PetscReal *)calloc(N, sizeof(PetscReal));
+  for (PetscInt i = 0; i < N; i++) {
+    if (state->my_type == DENSITY_MATRIX) (*probs)[i] = h[2 * (N * i + i)];
+    else (*probs)[i] = h[2 * i] * h[2 * i] + h[2 * i + 1] * h[2 * i + 1];
+  }
+  free(h);
+}
+
+/* PETSc binary Vec reader (qvec_utilities.c:97-166): {1211214, N} + N complex128, big endian */
+static uint64_t be64(const unsigned char *p) {
+  uint64_t v = 0;
+  for (int i = 0; i < 8; i++) v = (v << 8) | p[i];
+  return v;
+}
+
+static void read_binary(qvec *out, const char *fname, qvec_type type) {
+  FILE *f = fopen(fname, "rb");
+  if (!f) QUAC_FAIL(9, "cannot open %s", fname);
+  unsigned char hdr[16];
+  if (fread(hdr, 1, 16, f) != 16 || be64(hdr) != 1211214) {
+    fclose(f);
+    QUAC_FAIL(9, "%s is not a PETSc binary Vec", fname);
+  }
+  PetscInt n = (PetscInt)be64(hdr + 8);
+  unsigned char *buf = (unsigned char *)malloc((size_t)n * 16);
+  size_t got = fread(buf, 1, (size_t)n * 16, f);
+  fclose(f);
+  if (got != (size_t)n * 16) {
+    free(buf);
+    QUAC_FAIL(9, "%s is truncated", fname);
+  }
+  double *h = (double *)malloc(sizeof(double) * 2 * n);
+  for (PetscInt i = 0; i < 2 * n; i++) {
+    uint64_t v = be64(buf + 8 * i);
+    memcpy(&h[i], &v, 8);
+  }
+  free(buf);
+  quac_clear_error();
+  create_arb_qvec(out, n, type);
+  if (!quac_last_error()[0] && *out) {
+    if (qh_vec_set((*out)->data, h)) quac_fail(9, "%s", qh_last_error());
+  }
+  free(h);
+}
+
+void read_qvec_dm_binary(qvec *out, const char fname[]) { read_binary(out, fname, DENSITY_MATRIX); }
+void read_qvec_wf_binary(qvec *out, const char fname[]) { read_binary(out, fname, WAVEFUNCTION); }
+
+void copy_qvec(qvec src, qvec dst) {
+  if (src->n != dst->n) QUAC_FAIL(9, "copy_qvec: sizes differ");
+  QUAC_HIPCHK(qh_vec_copy(src->data, dst->data));
+}
+
+void print_qvec(qvec state) {
+  double *h = (double *)malloc(sizeof(double) * 2 * state->n);
+  if (qh_vec_get(state->data, h)) {
+    free(h);
+    QUAC_FAIL(9, "print_qvec: %s", qh_last_error());
+  }
+  if (state->my_type == DENSITY_MATRIX) {
+    PetscInt N = state->total_levels;
+    for (PetscInt i = 0; i < N; i++) {
+      for (PetscInt j = 0; j < N; j++) printf("%4.3e + %4.3ei  ", h[2 * (N * j + i)], h[2 * (N * j + i) + 1]);
+      printf("\n");
+    }
+  } else {
+    for (PetscInt i = 0; i < state->n; i++) printf("%4.3e + %4.3ei\n", h[2 * i], h[2 * i + 1]);
+  }
+  printf("\n");
+  free(h);
+}
+
+/* accessors for FFI callers */
+Mat quac_system_matrix(qsystem s) { return s->mat_allocated ? s->mat_A : NULL; }
+Mat quac_system_td_matrix(qsystem s, PetscInt k) {
+  return (k >= 0 && k < s->num_time_dep) ? s->time_dep[k].mat_A : NULL;
+}
+PetscInt quac_system_total_levels(qsystem s) { return s->total_levels; }
+PetscInt quac_system_dim(qsystem s) { return s->dim; }
+PetscBool quac_system_dm(qsystem s) { return s->dm_equations; }
+Vec quac_qvec_data(qvec q) { return q->data; }
+void quac_qvec_info(qvec q, PetscInt *n, PetscInt *tl, qvec_type *type) {
+  if (n) *n = q->n;
+  if (tl) *tl = q->total_levels;
+  if (type) *type = q->my_type;
+}

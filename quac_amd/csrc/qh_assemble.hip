@@ -1,0 +1,453 @@
+// qh_assemble.hip -- construct_matrix on the device, straight into CSR.
+//
+// Reference semantics (qsystem.c:593-676, kron.c:638-1036): for every term and
+// every iteration row i the reference calls MatSetValue(ADD_VALUES) for up to
+// three entries; PETSc keeps each row sorted, sums duplicates in insertion
+// order, keeps explicit zeros, and construct_matrix finally adds 0 to every
+// diagonal.  The transposed scatter of the G^T x I part (kron.c:721-725) and
+// the diagonal L^dag L parts (kron.c:1012-1024) write into rows other than i.
+//
+// Design: every QuaC operator chain is a partial bijection of basis indices, so
+// each target row r can *gather* its own contributions instead of having them
+// scattered to it: the I x G and G* x G parts come from iteration r (forward
+// chain), the G^T x I and diagonal parts from the unique iteration
+// i = chain^{-1}(r) (inverse chain).  Ordering the <= 3 contributions of a term
+// by (iteration, part) reproduces PETSc's insertion order exactly, so one
+// thread per row rebuilds the row with PETSc's own insert/merge rule -- no
+// global sort, no atomics, deterministic, bit-identical values.
+//
+// Passes: count (upper bound per row) -> scan -> fill (sorted insert/merge in a
+// per-row scratch segment) -> scan of exact lengths -> compact (+ -i scale,
+// + complex64 copy).
+#include <hipcub/hipcub.hpp>
+
+#include "qh_common.h"
+
+#pragma clang fp contract(off)
+
+namespace {
+
+struct Ctx {
+  const qh_term *terms;
+  int nterms;
+  const qh_op *ops;
+  const qh_level *tab;
+  uint32_t tot;
+  int conj_a;
+};
+
+__device__ inline cxd cmul(cxd a, cxd b) { return cxd_make(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re); }
+__device__ inline cxd cconj(cxd a) { return cxd_make(a.re, -a.im); }
+
+enum { M_IG = 0, M_GI = 1, M_GG = 2 };
+
+// _get_val_j_ops (kron.c:161-207) with the per-op rule of kron.c:294-538:
+// j = i + (next(sub) - sub) * n_after with sub = i / n_after % levels.
+__device__ bool chain_fwd(const Ctx &c, const qh_op *ops, int n, uint32_t i, int mode, uint32_t &j, cxd &v) {
+  int64_t cur = i;
+  cxd val = cxd_make(1.0, 0.0);
+  for (int k = 0; k < n; k++) {
+    const qh_op op = ops[k];
+    const int64_t L = op.levels, s = op.stride;
+    if (mode == M_GG) {
+      int64_t i1 = cur / c.tot, i2 = cur - i1 * c.tot;
+      int64_t s1 = (i1 / s) % L, s2 = (i2 / s) % L;
+      const qh_level e1 = c.tab[op.tab + s1], e2 = c.tab[op.tab + s2];
+      if (e1.next < 0 || e2.next < 0) return false;
+      i1 += (e1.next - s1) * s;
+      i2 += (e2.next - s2) * s;
+      cur = i1 * c.tot + i2;
+      cxd t = cmul(cxd_make(e1.re, -e1.im), cxd_make(e2.re, e2.im));
+      val = cmul(t, val);
+    } else {
+      const int64_t st = (mode == M_GI) ? s * c.tot : s;
+      const int64_t sub = (cur / st) % L;
+      const qh_level e = c.tab[op.tab + sub];
+      if (e.next < 0) return false;
+      cur += (e.next - sub) * st;
+      cxd ev = cxd_make(e.re, mode == M_GI ? -e.im : e.im);
+      val = cmul(ev, val);
+    }
+  }
+  j = (uint32_t)cur;
+  v = val;
+  return true;
+}
+
+// inverse of the forward chain (IG or GI): the unique i with chain(i) = r
+__device__ bool chain_inv(const Ctx &c, const qh_op *ops, int n, uint32_t r, int mode, uint32_t &i) {
+  int64_t cur = r;
+  for (int k = n - 1; k >= 0; k--) {
+    const qh_op op = ops[k];
+    const int64_t L = op.levels;
+    const int64_t st = (mode == M_GI) ? op.stride * (int64_t)c.tot : op.stride;
+    const int64_t sub = (cur / st) % L;
+    const int32_t p = c.tab[op.tab + sub].prev;
+    if (p < 0) return false;
+    cur += (p - sub) * st;
+  }
+  i = (uint32_t)cur;
+  return true;
+}
+
+struct Contrib {
+  uint64_t key;  // iteration * 4 + part: PETSc insertion order within a term
+  uint32_t col;
+  cxd v;
+};
+
+// contributions of one term to target row r, in insertion order; returns count
+__device__ int term_contribs(const Ctx &c, const qh_term &t, uint32_t r, Contrib out[3]) {
+  const qh_op *ops = c.ops + t.op0;
+  const int n = t.nops;
+  cxd a = t.zero_a ? cxd_make(0.0, 0.0) : cxd_make(t.a_re, t.a_im);
+  const cxd I = cxd_make(0.0, 1.0);
+  int m = 0;
+  uint32_t j, i;
+  cxd v;
+  switch (t.kind) {
+    case QH_HAM_ONLY:  // kron.c:638-657: A[i][j] += a*val
+      if (chain_fwd(c, ops, n, r, M_IG, j, v)) out[m++] = {(uint64_t)r * 4, j, cmul(a, v)};
+      break;
+    case QH_LIN_MCWF:  // kron.c:664-700: A[i][j] += -i*a/2*val (ops list already doubled)
+      if (chain_fwd(c, ops, n, r, M_IG, j, v)) {
+        cxd s = cmul(cxd_make(-0.0, -1.0), a);
+        s = cxd_make(s.re / 2, s.im / 2);
+        out[m++] = {(uint64_t)r * 4, j, cmul(s, v)};
+      }
+      break;
+    case QH_HAM_DM:  // kron.c:702-729
+      if (chain_fwd(c, ops, n, r, M_IG, j, v)) {
+        cxd na = cxd_make(-a.re, -a.im);
+        out[m++] = {(uint64_t)r * 4 + 0, j, cmul(cmul(na, I), v)};
+      }
+      if (chain_inv(c, ops, n, r, M_GI, i) && chain_fwd(c, ops, n, i, M_GI, j, v) && j == r) {
+        cxd aa = c.conj_a ? cconj(a) : a;
+        out[m++] = {(uint64_t)i * 4 + 1, i, cmul(cmul(aa, I), cconj(v))};
+      }
+      break;
+    case QH_LIN_DM: {  // kron.c:995-1036
+      cxd h = cxd_make(-0.5 * a.re, -0.5 * a.im);
+      if (chain_inv(c, ops, n, r, M_IG, i) && chain_fwd(c, ops, n, i, M_IG, j, v) && j == r)
+        out[m++] = {(uint64_t)i * 4 + 0, r, cmul(cmul(h, cconj(v)), v)};
+      if (chain_inv(c, ops, n, r, M_GI, i) && chain_fwd(c, ops, n, i, M_GI, j, v) && j == r)
+        out[m++] = {(uint64_t)i * 4 + 1, r, cmul(h, cconj(cmul(cconj(v), v)))};
+      if (chain_fwd(c, ops, n, r, M_GG, j, v)) out[m++] = {(uint64_t)r * 4 + 2, j, cmul(a, v)};
+      break;
+    }
+    default:
+      break;
+  }
+  // tiny insertion sort by key (at most 3 entries)
+  for (int x = 1; x < m; x++) {
+    Contrib y = out[x];
+    int z = x - 1;
+    while (z >= 0 && out[z].key > y.key) {
+      out[z + 1] = out[z];
+      z--;
+    }
+    out[z + 1] = y;
+  }
+  return m;
+}
+
+__global__ void __launch_bounds__(256) k_count(Ctx c, uint32_t rows, int64_t *cnt) {
+  uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= rows) return;
+  int64_t n = 1;  // the explicit diagonal zero (qsystem.c:656-662)
+  Contrib tmp[3];
+  for (int t = 0; t < c.nterms; t++) n += term_contribs(c, c.terms[t], r, tmp);
+  cnt[r] = n;
+}
+
+// MatSetValues_SeqAIJ rule: an existing entry is added to, a new one is set.
+__device__ inline void seg_insert(uint32_t *col, cxd *val, int &len, uint32_t cidx, cxd v) {
+  int p = len;
+  while (p > 0 && col[p - 1] > cidx) p--;
+  if (p > 0 && col[p - 1] == cidx) {
+    val[p - 1] = cxd_make(val[p - 1].re + v.re, val[p - 1].im + v.im);
+    return;
+  }
+  for (int q = len; q > p; q--) {
+    col[q] = col[q - 1];
+    val[q] = val[q - 1];
+  }
+  col[p] = cidx;
+  val[p] = v;
+  len++;
+}
+
+__global__ void __launch_bounds__(256) k_fill(Ctx c, uint32_t rows, const int64_t *off, uint32_t *tcol, cxd *tval,
+                                              int64_t *len_out) {
+  uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= rows) return;
+  uint32_t *col = tcol + off[r];
+  cxd *val = tval + off[r];
+  int len = 0;
+  Contrib tmp[3];
+  for (int t = 0; t < c.nterms; t++) {
+    int m = term_contribs(c, c.terms[t], r, tmp);
+    for (int x = 0; x < m; x++) seg_insert(col, val, len, tmp[x].col, tmp[x].v);
+  }
+  seg_insert(col, val, len, r, cxd_make(0.0, 0.0));
+  len_out[r] = len;
+}
+
+__global__ void __launch_bounds__(256) k_compact(uint32_t rows, const int64_t *off, const int64_t *len,
+                                                 const int64_t *rowstart, const uint32_t *tcol, const cxd *tval,
+                                                 int32_t *rowptr, int32_t *colout, double2 *valout, float2 *val32,
+                                                 int scale) {
+  uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= rows) return;
+  int64_t s = off[r], d = rowstart[r], n = len[r];
+  rowptr[r] = (int32_t)d;
+  if (r == rows - 1) rowptr[rows] = (int32_t)(d + n);
+  for (int64_t k = 0; k < n; k++) {
+    cxd v = tval[s + k];
+    if (scale) v = cmul(cxd_make(-0.0, -1.0), v);  // MatScale(A,-i), qsystem.c:670-673
+    colout[d + k] = (int32_t)tcol[s + k];
+    valout[d + k] = make_double2(v.re, v.im);
+    if (val32) val32[d + k] = make_float2((float)v.re, (float)v.im);
+  }
+}
+
+int exclusive_scan(const int64_t *in, int64_t *out, int64_t n, int64_t *total) {
+  void *tmp = nullptr;
+  size_t tmpb = 0;
+  QH_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, tmpb, in, out, (int)n, qh_stream()));
+  int rc = qh_alloc(&tmp, tmpb);
+  if (rc) return rc;
+  QH_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, tmpb, in, out, (int)n, qh_stream()));
+  int64_t last_in = 0, last_out = 0;
+  QH_HIP(hipMemcpyAsync(&last_in, in + n - 1, 8, hipMemcpyDeviceToHost, qh_stream()));
+  QH_HIP(hipMemcpyAsync(&last_out, out + n - 1, 8, hipMemcpyDeviceToHost, qh_stream()));
+  QH_HIP(hipStreamSynchronize(qh_stream()));
+  qh_free_ptr(tmp);
+  *total = last_in + last_out;
+  return QH_OK;
+}
+
+}  // namespace
+
+extern "C" int qh_assemble(int64_t tot, int64_t dim, int nterms, const qh_term *terms, int nops, const qh_op *ops,
+                           int nlev, const qh_level *levels, int flags, qh_mat **out) {
+  int rc = qh_ensure_init();
+  if (rc) return rc;
+  QH_REQUIRE(out && tot > 0 && dim > 0, QH_EINVAL, "qh_assemble: bad arguments");
+  QH_REQUIRE(dim < (int64_t)INT32_MAX && tot * tot <= (int64_t)INT32_MAX * 2, QH_ERANGE,
+             "qh_assemble: dimension %lld exceeds the 32-bit CSR index range", (long long)dim);
+  QH_REQUIRE(nterms >= 0 && nops >= 0 && nlev >= 0, QH_EINVAL, "qh_assemble: negative counts");
+  for (int t = 0; t < nterms; t++) {
+    QH_REQUIRE(terms[t].op0 >= 0 && terms[t].nops >= 0 && terms[t].op0 + terms[t].nops <= nops, QH_EINVAL,
+               "qh_assemble: term %d op range out of bounds", t);
+    QH_REQUIRE(terms[t].kind >= QH_HAM_DM && terms[t].kind <= QH_LIN_MCWF, QH_EINVAL, "qh_assemble: bad kind");
+  }
+  for (int k = 0; k < nops; k++) {
+    QH_REQUIRE(ops[k].levels > 0 && ops[k].stride > 0 && ops[k].tab >= 0 && ops[k].tab + ops[k].levels <= nlev,
+               QH_EINVAL, "qh_assemble: op %d table out of bounds", k);
+    QH_REQUIRE(tot % (ops[k].stride * ops[k].levels) == 0, QH_EINVAL, "qh_assemble: op %d stride inconsistent", k);
+  }
+  for (int l = 0; l < nlev; l++) {
+    // next/prev must stay inside the op's level range; checked against levels of the owner below
+    QH_REQUIRE(levels[l].next >= -1 && levels[l].prev >= -1, QH_EINVAL, "qh_assemble: bad level entry");
+  }
+  for (int k = 0; k < nops; k++)
+    for (int l = 0; l < ops[k].levels; l++) {
+      const qh_level &e = levels[ops[k].tab + l];
+      QH_REQUIRE(e.next < ops[k].levels && e.prev < ops[k].levels, QH_EINVAL,
+                 "qh_assemble: op %d level map leaves the subsystem", k);
+    }
+
+  hipStream_t s = qh_stream();
+  const uint32_t rows = (uint32_t)dim;
+  qh_term *d_terms = nullptr;
+  qh_op *d_ops = nullptr;
+  qh_level *d_lev = nullptr;
+  int64_t *cnt = nullptr, *off = nullptr, *len = nullptr, *start = nullptr;
+  uint32_t *tcol = nullptr;
+  cxd *tval = nullptr;
+  qh_mat *m = nullptr;
+  int64_t total = 0, nnz = 0;
+  Ctx c;
+
+#define QH_TRY(x)          \
+  do {                     \
+    rc = (x);              \
+    if (rc) goto cleanup;  \
+  } while (0)
+#define QH_TRYHIP(x)                                                                  \
+  do {                                                                                \
+    hipError_t e_ = (x);                                                              \
+    if (e_ != hipSuccess) {                                                           \
+      qh_set_error("qh_assemble: %s: %s", #x, hipGetErrorString(e_));                 \
+      rc = QH_EHIP;                                                                   \
+      goto cleanup;                                                                   \
+    }                                                                                 \
+  } while (0)
+
+  QH_TRY(qh_alloc_t(&d_terms, nterms > 0 ? nterms : 1));
+  QH_TRY(qh_alloc_t(&d_ops, nops > 0 ? nops : 1));
+  QH_TRY(qh_alloc_t(&d_lev, nlev > 0 ? nlev : 1));
+  if (nterms) QH_TRYHIP(hipMemcpyAsync(d_terms, terms, sizeof(qh_term) * nterms, hipMemcpyHostToDevice, s));
+  if (nops) QH_TRYHIP(hipMemcpyAsync(d_ops, ops, sizeof(qh_op) * nops, hipMemcpyHostToDevice, s));
+  if (nlev) QH_TRYHIP(hipMemcpyAsync(d_lev, levels, sizeof(qh_level) * nlev, hipMemcpyHostToDevice, s));
+  c.terms = d_terms;
+  c.nterms = nterms;
+  c.ops = d_ops;
+  c.tab = d_lev;
+  c.tot = (uint32_t)tot;
+  c.conj_a = (flags & QH_ASM_CONJ_A) ? 1 : 0;
+
+  QH_TRY(qh_alloc_t(&cnt, rows));
+  QH_TRY(qh_alloc_t(&off, rows));
+  hipLaunchKernelGGL(k_count, dim3(qh_blocks(rows, 256)), dim3(256), 0, s, c, rows, cnt);
+  QH_TRYHIP(hipGetLastError());
+  QH_TRY(exclusive_scan(cnt, off, rows, &total));
+  QH_TRY(qh_alloc_t(&tcol, (size_t)total));
+  QH_TRY(qh_alloc_t(&tval, (size_t)total));
+  QH_TRY(qh_alloc_t(&len, rows));
+  hipLaunchKernelGGL(k_fill, dim3(qh_blocks(rows, 256)), dim3(256), 0, s, c, rows, off, tcol, tval, len);
+  QH_TRYHIP(hipGetLastError());
+  start = cnt;  // reuse
+  QH_TRY(exclusive_scan(len, start, rows, &nnz));
+  if (nnz >= (int64_t)INT32_MAX) {
+    qh_set_error("qh_assemble: nnz %lld exceeds the 32-bit CSR range", (long long)nnz);
+    rc = QH_ERANGE;
+    goto cleanup;
+  }
+  m = new qh_mat();
+  m->rows = dim;
+  m->nnz = nnz;
+  QH_TRY(qh_alloc_t(&m->rowptr, (size_t)rows + 1));
+  QH_TRY(qh_alloc_t(&m->col, (size_t)nnz));
+  QH_TRY(qh_alloc_t(&m->val, (size_t)nnz));
+  if (flags & QH_ASM_FP32) QH_TRY(qh_alloc_t(&m->val32, (size_t)nnz));
+  hipLaunchKernelGGL(k_compact, dim3(qh_blocks(rows, 256)), dim3(256), 0, s, rows, off, len, start, tcol, tval,
+                     m->rowptr, m->col, m->val, m->val32, (flags & QH_ASM_SCALE_MINUS_I) ? 1 : 0);
+  QH_TRYHIP(hipGetLastError());
+  QH_TRYHIP(hipStreamSynchronize(s));
+  *out = m;
+  m = nullptr;
+cleanup:
+  if (m) qh_mat_destroy(m);
+  qh_free_ptr(d_terms);
+  qh_free_ptr(d_ops);
+  qh_free_ptr(d_lev);
+  qh_free_ptr(cnt);
+  qh_free_ptr(off);
+  qh_free_ptr(len);
+  qh_free_ptr(tcol);
+  qh_free_ptr(tval);
+  return rc;
+#undef QH_TRY
+#undef QH_TRYHIP
+}
+
+extern "C" int qh_mat_destroy(qh_mat *m) {
+  if (!m) return QH_OK;
+  qh_free_ptr(m->rowptr);
+  qh_free_ptr(m->col);
+  qh_free_ptr(m->val);
+  qh_free_ptr(m->val32);
+  delete m;
+  return QH_OK;
+}
+
+extern "C" int qh_mat_info(const qh_mat *m, int64_t *rows, int64_t *nnz) {
+  QH_REQUIRE(m, QH_EINVAL, "null matrix");
+  if (rows) *rows = m->rows;
+  if (nnz) *nnz = m->nnz;
+  return QH_OK;
+}
+
+__global__ void k_widen_i32(const int32_t *a, int64_t *b, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    b[i] = a[i];
+}
+
+extern "C" int qh_mat_export(const qh_mat *m, int64_t *rowptr, int64_t *col, double *val) {
+  QH_REQUIRE(m, QH_EINVAL, "null matrix");
+  hipStream_t s = qh_stream();
+  int64_t *tmp = nullptr;
+  int64_t big = m->rows + 1 > m->nnz ? m->rows + 1 : m->nnz;
+  int rc = qh_alloc_t(&tmp, (size_t)big);
+  if (rc) return rc;
+  if (rowptr) {
+    hipLaunchKernelGGL(k_widen_i32, dim3(qh_blocks(m->rows + 1, 256)), dim3(256), 0, s, m->rowptr, tmp, m->rows + 1);
+    QH_HIP(hipMemcpyAsync(rowptr, tmp, 8 * (m->rows + 1), hipMemcpyDeviceToHost, s));
+    QH_HIP(hipStreamSynchronize(s));
+  }
+  if (col && m->nnz) {
+    hipLaunchKernelGGL(k_widen_i32, dim3(qh_blocks(m->nnz, 256)), dim3(256), 0, s, m->col, tmp, m->nnz);
+    QH_HIP(hipMemcpyAsync(col, tmp, 8 * m->nnz, hipMemcpyDeviceToHost, s));
+    QH_HIP(hipStreamSynchronize(s));
+  }
+  if (val && m->nnz) QH_HIP(hipMemcpyAsync(val, m->val, 16 * m->nnz, hipMemcpyDeviceToHost, s));
+  QH_HIP(hipStreamSynchronize(s));
+  qh_free_ptr(tmp);
+  return QH_OK;
+}
+
+__global__ void k_narrow_vals(const double2 *a, float2 *b, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    b[i] = make_float2((float)a[i].x, (float)a[i].y);
+}
+
+extern "C" int qh_mat_make_fp32(qh_mat *m) {
+  QH_REQUIRE(m, QH_EINVAL, "null matrix");
+  if (!m->val32) {
+    int rc = qh_alloc_t(&m->val32, (size_t)(m->nnz ? m->nnz : 1));
+    if (rc) return rc;
+  }
+  if (m->nnz)
+    hipLaunchKernelGGL(k_narrow_vals, dim3(qh_blocks(m->nnz, 256)), dim3(256), 0, qh_stream(), m->val, m->val32,
+                       m->nnz);
+  QH_HIP(hipGetLastError());
+  return QH_OK;
+}
+
+__global__ void k_bd_rows(const int32_t *rp, int64_t rows, int32_t row0, int32_t nz0, int32_t *out) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i <= rows; i += (int64_t)gridDim.x * blockDim.x)
+    out[row0 + i] = rp[i] + nz0;
+}
+__global__ void k_bd_cols(const int32_t *c, int64_t nnz, int32_t row0, int32_t *out) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nnz; i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = c[i] + row0;
+}
+
+extern "C" int qh_mat_blockdiag(int n, qh_mat *const *mats, qh_mat **out) {
+  QH_REQUIRE(n > 0 && mats && out, QH_EINVAL, "qh_mat_blockdiag: bad arguments");
+  int64_t rows = 0, nnz = 0;
+  for (int i = 0; i < n; i++) {
+    QH_REQUIRE(mats[i], QH_EINVAL, "qh_mat_blockdiag: null matrix %d", i);
+    rows += mats[i]->rows;
+    nnz += mats[i]->nnz;
+  }
+  QH_REQUIRE(rows < INT32_MAX && nnz < INT32_MAX, QH_ERANGE, "qh_mat_blockdiag: too large for 32-bit CSR");
+  qh_mat *m = new qh_mat();
+  m->rows = rows;
+  m->nnz = nnz;
+  int rc = qh_alloc_t(&m->rowptr, (size_t)rows + 1);
+  if (!rc) rc = qh_alloc_t(&m->col, (size_t)nnz);
+  if (!rc) rc = qh_alloc_t(&m->val, (size_t)nnz);
+  if (rc) {
+    qh_mat_destroy(m);
+    return rc;
+  }
+  hipStream_t s = qh_stream();
+  int64_t r0 = 0, z0 = 0;
+  for (int i = 0; i < n; i++) {
+    const qh_mat *a = mats[i];
+    hipLaunchKernelGGL(k_bd_rows, dim3(qh_blocks(a->rows + 1, 256)), dim3(256), 0, s, a->rowptr, a->rows,
+                       (int32_t)r0, (int32_t)z0, m->rowptr);
+    hipLaunchKernelGGL(k_bd_cols, dim3(qh_blocks(a->nnz, 256)), dim3(256), 0, s, a->col, a->nnz, (int32_t)r0,
+                       m->col + z0);
+    QH_HIP(hipMemcpyAsync(m->val + z0, a->val, 16 * a->nnz, hipMemcpyDeviceToDevice, s));
+    r0 += a->rows;
+    z0 += a->nnz;
+  }
+  QH_HIP(hipGetLastError());
+  QH_HIP(hipStreamSynchronize(s));
+  *out = m;
+  return QH_OK;
+}

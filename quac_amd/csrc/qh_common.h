@@ -1,0 +1,96 @@
+// qh_common.h -- internal definitions shared by the gfx950 kernels of libquac.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+
+#include "../../include/quac_hip.h"
+
+// ----------------------------------------------------------------- errors
+void qh_set_error(const char *fmt, ...);
+
+#define QH_HIP(call)                                                                 \
+  do {                                                                               \
+    hipError_t e_ = (call);                                                          \
+    if (e_ != hipSuccess) {                                                          \
+      qh_set_error("%s:%d %s: %s", __FILE__, __LINE__, #call, hipGetErrorString(e_)); \
+      return QH_EHIP;                                                                \
+    }                                                                                \
+  } while (0)
+
+#define QH_REQUIRE(cond, code, ...)   \
+  do {                                \
+    if (!(cond)) {                    \
+      qh_set_error(__VA_ARGS__);      \
+      return (code);                  \
+    }                                 \
+  } while (0)
+
+hipStream_t qh_stream();
+int qh_ensure_init();
+int qh_num_cus();
+
+// ----------------------------------------------------------------- objects
+struct qh_mat {
+  int64_t rows = 0, nnz = 0;
+  int32_t *rowptr = nullptr;   // rows+1
+  int32_t *col = nullptr;      // nnz
+  double2 *val = nullptr;      // nnz complex128
+  float2 *val32 = nullptr;     // nnz complex64 (FP32 path), made on demand
+  int lanes64 = 0, lanes32 = 0;  // SpMV lanes per row (0: not tuned yet)
+  int32_t max_row = 0;         // longest row
+};
+
+struct qh_vec {
+  int64_t n = 0;
+  int prec = QH_FP64;
+  void *d = nullptr;           // double2* (FP64) or float2* high part (FP32)
+  float2 *lo = nullptr;        // FP32 compensation word (low part), zero-initialised
+};
+
+// device allocation helpers (return QH_* codes)
+int qh_alloc(void **p, size_t bytes);
+void qh_free_ptr(void *p);
+
+template <typename T>
+static inline int qh_alloc_t(T **p, size_t count) {
+  return qh_alloc(reinterpret_cast<void **>(p), count * sizeof(T));
+}
+
+// --------------------------------------------------------- complex helpers
+// Products are written out exactly as C99 complex multiplication, and the
+// assembly kernels are compiled with contraction off so that every bit of the
+// assembled values matches the host-side C arithmetic.
+struct cxd {
+  double re, im;
+};
+__host__ __device__ static inline cxd cxd_make(double r, double i) {
+  cxd z;
+  z.re = r;
+  z.im = i;
+  return z;
+}
+
+static inline unsigned qh_blocks(int64_t work, int per_block) {
+  int64_t b = (work + per_block - 1) / per_block;
+  if (b < 1) b = 1;
+  return (unsigned)b;
+}
+
+// Remap a linear block index so that each of the 8 XCDs of an MI355X (which
+// receive blocks round-robin) works on one contiguous range of the index space:
+// neighbouring rows share an L2 slice, which keeps the near-diagonal gathers of
+// x local to one XCD.
+__device__ static inline unsigned qh_xcd_block(unsigned b, unsigned nb) {
+  const unsigned nx = 8;
+  if (nb < nx * 2) return b;
+  unsigned per = nb / nx, rem = nb % nx;
+  unsigned x = b % nx, k = b / nx;
+  // XCD x owns logical blocks [x*per + min(x,rem), ...) of size per + (x<rem)
+  unsigned base = x * per + (x < rem ? x : rem);
+  unsigned size = per + (x < rem ? 1u : 0u);
+  if (k < size) return base + k;
+  return b;  // unreachable for a full round-robin distribution
+}

@@ -1,0 +1,158 @@
+// qh_gates.hip -- in-place gate application rho <- U rho U^dagger (or psi <- U psi).
+//
+// The reference builds a fresh dim x dim AIJ gate superoperator U* x U for every
+// gate and multiplies (quantum_circuits.c:435-481), or runs QuEST-style pair
+// updates twice over vec(rho) (:696-772).  Here one pass does both sides: each
+// thread owns the 2x2 (1-qubit) or 4x4 (2-qubit) block of rho addressed by the
+// gate's row bits and column bits, loads it once, applies U on the left and U^dagger on
+// the right in registers, and stores it back: 2 * rows * sizeof(entry) bytes per
+// gate, the minimum for an in-place update.
+#include <algorithm>
+
+#include "qh_common.h"
+
+namespace {
+
+struct U4 {
+  double re[16], im[16];
+};
+
+// insert a zero digit of a 2-level subsystem at stride s
+__device__ inline int64_t ins0(int64_t t, int64_t s) { return (t / s) * (2 * s) + t % s; }
+
+__device__ inline double2 ldv(const void *d, const float2 *lo, int64_t i) {
+  if (!lo) return ((const double2 *)d)[i];
+  float2 h = ((const float2 *)d)[i], l = lo[i];
+  return make_double2((double)h.x + (double)l.x, (double)h.y + (double)l.y);
+}
+__device__ inline void stv(void *d, float2 *lo, int64_t i, double2 v) {
+  if (!lo) {
+    ((double2 *)d)[i] = v;
+    return;
+  }
+  float hr = (float)v.x, hi = (float)v.y;
+  ((float2 *)d)[i] = make_float2(hr, hi);
+  lo[i] = make_float2((float)(v.x - (double)hr), (float)(v.y - (double)hi));
+}
+
+// K = 2 (1 qubit) or 4 (2 qubits); DM: X' = U X U^dagger on a KxK block; WF: psi' = U psi
+template <int K, bool DM>
+__global__ void __launch_bounds__(256) k_gate(void *x, float2 *lo, int64_t nthreads, int nins, int64_t s0,
+                                              int64_t s1, int64_t s2, int64_t s3, int64_t bs0, int64_t bs1,
+                                              int64_t cs0, int64_t cs1, U4 u) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= nthreads) return;
+  // insert the gate's zero digits, lowest stride first
+  int64_t base = t;
+  const int64_t ss[4] = {s0, s1, s2, s3};
+  for (int q = 0; q < nins; q++) base = ins0(base, ss[q]);
+  // offsets of the K basis states of the gate (first qubit = most significant)
+  int64_t ro[K], co[K];
+#pragma unroll
+  for (int a = 0; a < K; a++) {
+    if (K == 2) {
+      ro[a] = a * bs0;
+      co[a] = a * cs0;
+    } else {
+      ro[a] = ((a >> 1) & 1) * bs0 + (a & 1) * bs1;
+      co[a] = ((a >> 1) & 1) * cs0 + (a & 1) * cs1;
+    }
+  }
+  if (!DM) {
+    double2 p[K];
+#pragma unroll
+    for (int a = 0; a < K; a++) p[a] = ldv(x, lo, base + ro[a]);
+#pragma unroll
+    for (int a = 0; a < K; a++) {
+      double r = 0.0, i = 0.0;
+#pragma unroll
+      for (int c = 0; c < K; c++) {
+        r += u.re[a * K + c] * p[c].x - u.im[a * K + c] * p[c].y;
+        i += u.re[a * K + c] * p[c].y + u.im[a * K + c] * p[c].x;
+      }
+      stv(x, lo, base + ro[a], make_double2(r, i));
+    }
+    return;
+  }
+  // X[c][d] = rho[row c, col d] at base + ro[c] + co[d]
+  double2 X[K][K];
+#pragma unroll
+  for (int c = 0; c < K; c++)
+#pragma unroll
+    for (int d = 0; d < K; d++) X[c][d] = ldv(x, lo, base + ro[c] + co[d]);
+  // Y = U X
+  double2 Y[K][K];
+#pragma unroll
+  for (int a = 0; a < K; a++)
+#pragma unroll
+    for (int d = 0; d < K; d++) {
+      double r = 0.0, i = 0.0;
+#pragma unroll
+      for (int c = 0; c < K; c++) {
+        r += u.re[a * K + c] * X[c][d].x - u.im[a * K + c] * X[c][d].y;
+        i += u.re[a * K + c] * X[c][d].y + u.im[a * K + c] * X[c][d].x;
+      }
+      Y[a][d] = make_double2(r, i);
+    }
+  // Z = Y U^dagger: Z[a][b] = sum_d Y[a][d] conj(U[b][d])
+#pragma unroll
+  for (int a = 0; a < K; a++)
+#pragma unroll
+    for (int b = 0; b < K; b++) {
+      double r = 0.0, i = 0.0;
+#pragma unroll
+      for (int d = 0; d < K; d++) {
+        const double ur = u.re[b * K + d], ui = -u.im[b * K + d];
+        r += Y[a][d].x * ur - Y[a][d].y * ui;
+        i += Y[a][d].x * ui + Y[a][d].y * ur;
+      }
+      stv(x, lo, base + ro[a] + co[b], make_double2(r, i));
+    }
+}
+
+}  // namespace
+
+extern "C" int qh_gate(qh_vec *x, int64_t tot, int dm, int nq, const int64_t *strides, const double *u) {
+  int rc = qh_ensure_init();
+  if (rc) return rc;
+  QH_REQUIRE(x && strides && u && (nq == 1 || nq == 2), QH_EINVAL, "qh_gate: bad arguments");
+  QH_REQUIRE(dm ? tot * tot == x->n : tot == x->n, QH_EINVAL, "qh_gate: vector size mismatch");
+  for (int q = 0; q < nq; q++)
+    QH_REQUIRE(strides[q] > 0 && tot % (2 * strides[q]) == 0, QH_EINVAL,
+               "qh_gate: stride %lld is not a 2-level subsystem of dimension %lld", (long long)strides[q],
+               (long long)tot);
+  QH_REQUIRE(nq == 1 || strides[0] != strides[1], QH_EINVAL, "qh_gate: both qubits are the same subsystem");
+  U4 m;
+  const int K = nq == 1 ? 2 : 4;
+  for (int i = 0; i < K * K; i++) {
+    m.re[i] = u[2 * i];
+    m.im[i] = u[2 * i + 1];
+  }
+  int64_t ins[4];
+  int nins = 0;
+  for (int q = 0; q < nq; q++) ins[nins++] = strides[q];
+  if (dm)
+    for (int q = 0; q < nq; q++) ins[nins++] = strides[q] * tot;
+  std::sort(ins, ins + nins);
+  for (int q = nins; q < 4; q++) ins[q] = 1;
+  const int64_t total = dm ? tot * tot : tot;
+  const int64_t nthreads = total >> nins;
+  const int64_t bs0 = strides[0], bs1 = nq == 2 ? strides[1] : 0;
+  const int64_t cs0 = dm ? strides[0] * tot : 0, cs1 = (dm && nq == 2) ? strides[1] * tot : 0;
+  const dim3 grid(qh_blocks(nthreads, 256));
+  hipStream_t s = qh_stream();
+  if (nq == 1 && dm)
+    hipLaunchKernelGGL((k_gate<2, true>), grid, dim3(256), 0, s, x->d, x->lo, nthreads, nins, ins[0], ins[1], ins[2],
+                       ins[3], bs0, bs1, cs0, cs1, m);
+  else if (nq == 1)
+    hipLaunchKernelGGL((k_gate<2, false>), grid, dim3(256), 0, s, x->d, x->lo, nthreads, nins, ins[0], ins[1],
+                       ins[2], ins[3], bs0, bs1, cs0, cs1, m);
+  else if (dm)
+    hipLaunchKernelGGL((k_gate<4, true>), grid, dim3(256), 0, s, x->d, x->lo, nthreads, nins, ins[0], ins[1], ins[2],
+                       ins[3], bs0, bs1, cs0, cs1, m);
+  else
+    hipLaunchKernelGGL((k_gate<4, false>), grid, dim3(256), 0, s, x->d, x->lo, nthreads, nins, ins[0], ins[1],
+                       ins[2], ins[3], bs0, bs1, cs0, cs1, m);
+  QH_HIP(hipGetLastError());
+  return QH_OK;
+}

@@ -1,0 +1,337 @@
+// qh_gmres.hip -- steady state by restarted GMRES in FP64 on the device.
+//
+// Semantics of steady_state (solver.c:33-220): the trace row is added to row 0
+// of the Liouvillian (row 0 += 1 at the diagonal columns i*(N+1)) so that the
+// solution of A' x = e0 has unit trace; GMRES with restart 100.  The Krylov
+// basis lives in HBM (column-major, one contiguous vector per basis column);
+// classical Gram-Schmidt with one re-orthogonalisation (CGS2) is done with two
+// streaming kernels per pass (all dots of a pass in one launch, all updates in
+// one launch), the Hessenberg column stays on the device until the single
+// device-to-host copy per iteration; Givens rotations and the triangular solve
+// run on the host (they are O(restart^2)).  An optional Jacobi right
+// preconditioner scales by the inverse diagonal.
+#include <cmath>
+#include <complex>
+#include <vector>
+
+#include "qh_common.h"
+
+namespace {
+
+typedef std::complex<double> cd;
+
+__global__ void k_diag_inv(const int32_t *rp, const int32_t *col, const double2 *val, int64_t n, double2 *dinv) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double dr = 0.0, di = 0.0;
+    for (int32_t k = rp[i]; k < rp[i + 1]; k++)
+      if (col[k] == i) {
+        dr = val[k].x;
+        di = val[k].y;
+      }
+    if (i == 0) dr += 1.0;  // stabilisation entry at (0,0)
+    const double dd = dr * dr + di * di;
+    dinv[i] = dd == 0.0 ? make_double2(1.0, 0.0) : make_double2(dr / dd, -di / dd);
+  }
+}
+
+// z = dinv .* v
+__global__ void k_cmul_vec(const double2 *a, const double2 *v, double2 *z, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double2 p = a[i], q = v[i];
+    z[i] = make_double2(p.x * q.x - p.y * q.y, p.x * q.y + p.y * q.x);
+  }
+}
+// z = v ./ dinv  (x -> u for the initial guess)
+__global__ void k_cdiv_vec(const double2 *a, const double2 *v, double2 *z, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double2 p = a[i], q = v[i];
+    double dd = p.x * p.x + p.y * p.y;
+    z[i] = make_double2((q.x * p.x + q.y * p.y) / dd, (q.y * p.x - q.x * p.y) / dd);
+  }
+}
+
+// y[0] += sum_i z[i*(N+1)]   (single block)
+__global__ void k_add_trace_row0(const double2 *z, int64_t tot, double2 *y) {
+  __shared__ double sr[256], si[256];
+  double ar = 0.0, ai = 0.0;
+  for (int64_t i = threadIdx.x; i < tot; i += 256) {
+    double2 v = z[i * (tot + 1)];
+    ar += v.x;
+    ai += v.y;
+  }
+  sr[threadIdx.x] = ar;
+  si[threadIdx.x] = ai;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      sr[threadIdx.x] += sr[threadIdx.x + o];
+      si[threadIdx.x] += si[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) y[0] = make_double2(y[0].x + sr[0], y[0].y + si[0]);
+}
+
+// r = b - r with b = e0
+__global__ void k_resid(double2 *r, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double2 v = r[i];
+    r[i] = make_double2((i == 0 ? 1.0 : 0.0) - v.x, -v.y);
+  }
+}
+
+const int DOT_BLOCKS = 256;
+
+// partial[j*DOT_BLOCKS + bx] = sum over the block's rows of conj(V_j) w
+__global__ void __launch_bounds__(256) k_multidot(const double2 *V, int64_t ld, const double2 *w, int64_t n,
+                                                  double2 *partial) {
+  const int j = blockIdx.y;
+  const double2 *v = V + ld * j;
+  double ar = 0.0, ai = 0.0;
+  for (int64_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    double2 a = v[i], b = w[i];
+    ar += a.x * b.x + a.y * b.y;
+    ai += a.x * b.y - a.y * b.x;
+  }
+  __shared__ double sr[256], si[256];
+  sr[threadIdx.x] = ar;
+  si[threadIdx.x] = ai;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      sr[threadIdx.x] += sr[threadIdx.x + o];
+      si[threadIdx.x] += si[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[(int64_t)j * gridDim.x + blockIdx.x] = make_double2(sr[0], si[0]);
+}
+
+// h[j] = sum_bx partial[j][bx]  (one block per j)
+__global__ void k_reduce_rows(const double2 *partial, int nb, double2 *h) {
+  const int j = blockIdx.x;
+  __shared__ double sr[256], si[256];
+  double ar = 0.0, ai = 0.0;
+  for (int b = threadIdx.x; b < nb; b += 256) {
+    double2 v = partial[(int64_t)j * nb + b];
+    ar += v.x;
+    ai += v.y;
+  }
+  sr[threadIdx.x] = ar;
+  si[threadIdx.x] = ai;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      sr[threadIdx.x] += sr[threadIdx.x + o];
+      si[threadIdx.x] += si[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) h[j] = make_double2(sr[0], si[0]);
+}
+
+// w -= sum_j h_j V_j
+__global__ void __launch_bounds__(256) k_multiaxpy(const double2 *V, int64_t ld, int m, const double2 *h, double2 *w,
+                                                   int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double2 acc = w[i];
+    for (int j = 0; j < m; j++) {
+      const double2 hj = h[j], v = V[ld * j + i];
+      acc.x -= hj.x * v.x - hj.y * v.y;
+      acc.y -= hj.x * v.y + hj.y * v.x;
+    }
+    w[i] = acc;
+  }
+}
+
+// v = w / ||w||  with the norm read from device memory (|.|^2 in h.x)
+__global__ void k_normalize(const double2 *w, const double2 *nrm2, double2 *v, int64_t n) {
+  const double s = nrm2[0].x > 0.0 ? 1.0 / sqrt(nrm2[0].x) : 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    v[i] = make_double2(w[i].x * s, w[i].y * s);
+}
+
+// x += sum_j y_j V_j
+__global__ void k_update(double2 *x, const double2 *V, int64_t ld, int m, const double2 *y, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double2 acc = x[i];
+    for (int j = 0; j < m; j++) {
+      const double2 c = y[j], v = V[ld * j + i];
+      acc.x += c.x * v.x - c.y * v.y;
+      acc.y += c.x * v.y + c.y * v.x;
+    }
+    x[i] = acc;
+  }
+}
+
+unsigned gr(int64_t n) {
+  int64_t b = (n + 255) / 256;
+  if (b > 4096) b = 4096;
+  if (b < 1) b = 1;
+  return (unsigned)b;
+}
+
+}  // namespace
+
+extern "C" int qh_gmres(const qh_mat *A, int64_t tot, qh_vec *x, int restart, double tol, int64_t maxit, int jacobi,
+                        int64_t *iters, double *res) {
+  int rc = qh_ensure_init();
+  if (rc) return rc;
+  QH_REQUIRE(A && x && x->prec == QH_FP64 && x->n == A->rows && tot * tot == A->rows, QH_EINVAL,
+             "qh_gmres: FP64 density-matrix vector of the matrix size required");
+  QH_REQUIRE(restart >= 1 && restart <= 1000, QH_EINVAL, "qh_gmres: restart in [1,1000]");
+  const int64_t n = A->rows;
+  const int R = restart;
+  hipStream_t s = qh_stream();
+  double2 *V = nullptr, *w = nullptr, *z = nullptr, *u = nullptr, *dinv = nullptr, *part = nullptr, *h = nullptr,
+          *yv = nullptr;
+  qh_vec zv, wv;
+  rc = qh_alloc_t(&V, (size_t)n * (R + 1));
+  if (!rc) rc = qh_alloc_t(&w, n);
+  if (!rc) rc = qh_alloc_t(&z, n);
+  if (!rc) rc = qh_alloc_t(&u, n);
+  if (!rc) rc = qh_alloc_t(&part, (size_t)DOT_BLOCKS * (R + 2));
+  if (!rc) rc = qh_alloc_t(&h, 2 * (R + 2));
+  if (!rc) rc = qh_alloc_t(&yv, R + 1);
+  if (!rc && jacobi) rc = qh_alloc_t(&dinv, n);
+  if (rc) goto done;
+  {
+    zv.n = n;
+    zv.prec = QH_FP64;
+    zv.d = z;
+    wv.n = n;
+    wv.prec = QH_FP64;
+    wv.d = w;
+    if (jacobi) {
+      hipLaunchKernelGGL(k_diag_inv, dim3(gr(n)), dim3(256), 0, s, A->rowptr, A->col, A->val, n, dinv);
+      hipLaunchKernelGGL(k_cdiv_vec, dim3(gr(n)), dim3(256), 0, s, dinv, (const double2 *)x->d, u, n);
+    } else {
+      QH_HIP(hipMemcpyAsync(u, x->d, 16 * n, hipMemcpyDeviceToDevice, s));
+    }
+    // y = A' M^-1 v into out (uses z as scratch)
+    auto matvec = [&](const double2 *v, double2 *out) -> int {
+      const double2 *src = v;
+      if (jacobi) {
+        hipLaunchKernelGGL(k_cmul_vec, dim3(gr(n)), dim3(256), 0, s, dinv, v, z, n);
+        src = z;
+      }
+      qh_vec iv, ov;
+      iv.n = ov.n = n;
+      iv.prec = ov.prec = QH_FP64;
+      iv.d = const_cast<double2 *>(src);
+      ov.d = out;
+      int r2 = qh_spmv(A, &iv, &ov);
+      iv.d = ov.d = nullptr;
+      if (r2) return r2;
+      hipLaunchKernelGGL(k_add_trace_row0, dim3(1), dim3(256), 0, s, src, tot, out);
+      QH_HIP(hipGetLastError());
+      return QH_OK;
+    };
+    auto dots = [&](int m, const double2 *vec, double2 *hout) {
+      hipLaunchKernelGGL(k_multidot, dim3(DOT_BLOCKS, m), dim3(256), 0, s, V, n, vec, n, part);
+      hipLaunchKernelGGL(k_reduce_rows, dim3(m), dim3(256), 0, s, part, DOT_BLOCKS, hout);
+    };
+    auto norm2 = [&](const double2 *vec, double2 *hout) {
+      // ||vec||^2 via the multidot kernel with V := vec
+      hipLaunchKernelGGL(k_multidot, dim3(DOT_BLOCKS, 1), dim3(256), 0, s, vec, n, vec, n, part);
+      hipLaunchKernelGGL(k_reduce_rows, dim3(1), dim3(256), 0, s, part, DOT_BLOCKS, hout);
+    };
+
+    int64_t it = 0;
+    double rn = 0.0;
+    std::vector<cd> H((size_t)(R + 1) * R), cs(R), sn(R), g(R + 1), y(R);
+    std::vector<double2> hh(2 * (R + 2));
+    for (;;) {
+      rc = matvec(u, w);
+      if (rc) break;
+      hipLaunchKernelGGL(k_resid, dim3(gr(n)), dim3(256), 0, s, w, n);
+      norm2(w, h);
+      double2 hn2;
+      QH_HIP(hipMemcpyAsync(&hn2, h, 16, hipMemcpyDeviceToHost, s));
+      QH_HIP(hipStreamSynchronize(s));
+      rn = sqrt(hn2.x);
+      if (rn <= tol || it >= maxit) break;
+      hipLaunchKernelGGL(k_normalize, dim3(gr(n)), dim3(256), 0, s, w, h, V, n);
+      std::fill(H.begin(), H.end(), cd(0.0, 0.0));
+      std::fill(g.begin(), g.end(), cd(0.0, 0.0));
+      g[0] = rn;
+      int k = 0;
+      bool conv = false;
+      for (; k < R && it < maxit; k++, it++) {
+        rc = matvec(V + n * k, w);
+        if (rc) break;
+        // CGS2: two passes of (all dots, all updates); h[0..k] and h[R+2 .. R+2+k]
+        dots(k + 1, w, h);
+        hipLaunchKernelGGL(k_multiaxpy, dim3(gr(n)), dim3(256), 0, s, V, n, k + 1, h, w, n);
+        dots(k + 1, w, h + (R + 2));
+        hipLaunchKernelGGL(k_multiaxpy, dim3(gr(n)), dim3(256), 0, s, V, n, k + 1, h + (R + 2), w, n);
+        norm2(w, h + (k + 1));
+        hipLaunchKernelGGL(k_normalize, dim3(gr(n)), dim3(256), 0, s, w, h + (k + 1), V + n * (k + 1), n);
+        QH_HIP(hipMemcpyAsync(hh.data(), h, sizeof(double2) * 2 * (R + 2), hipMemcpyDeviceToHost, s));
+        QH_HIP(hipStreamSynchronize(s));
+        for (int jj = 0; jj <= k; jj++)
+          H[(size_t)jj * R + k] = cd(hh[jj].x + hh[R + 2 + jj].x, hh[jj].y + hh[R + 2 + jj].y);
+        H[(size_t)(k + 1) * R + k] = cd(sqrt(hh[k + 1].x), 0.0);
+        for (int jj = 0; jj < k; jj++) {
+          cd a = H[(size_t)jj * R + k], b = H[(size_t)(jj + 1) * R + k];
+          H[(size_t)jj * R + k] = std::conj(cs[jj]) * a + std::conj(sn[jj]) * b;
+          H[(size_t)(jj + 1) * R + k] = -sn[jj] * a + cs[jj] * b;
+        }
+        cd a = H[(size_t)k * R + k], b = H[(size_t)(k + 1) * R + k];
+        double den = sqrt(std::norm(a) + std::norm(b));
+        if (den == 0.0) {
+          cs[k] = 1.0;
+          sn[k] = 0.0;
+        } else {
+          cs[k] = a / den;
+          sn[k] = b / den;
+        }
+        cd gk = g[k];
+        g[k] = std::conj(cs[k]) * gk;
+        g[k + 1] = -sn[k] * gk;
+        H[(size_t)k * R + k] = std::conj(cs[k]) * a + std::conj(sn[k]) * b;
+        H[(size_t)(k + 1) * R + k] = 0.0;
+        if (std::abs(g[k + 1]) <= tol) {
+          k++;
+          it++;
+          conv = true;
+          break;
+        }
+      }
+      if (rc) break;
+      (void)conv;
+      for (int i = k - 1; i >= 0; i--) {
+        cd sacc = g[i];
+        for (int jj = i + 1; jj < k; jj++) sacc -= H[(size_t)i * R + jj] * y[jj];
+        y[i] = sacc / H[(size_t)i * R + i];
+      }
+      std::vector<double2> yh(k > 0 ? k : 1);
+      for (int i = 0; i < k; i++) yh[i] = make_double2(y[i].real(), y[i].imag());
+      if (k > 0) {
+        QH_HIP(hipMemcpyAsync(yv, yh.data(), sizeof(double2) * k, hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_update, dim3(gr(n)), dim3(256), 0, s, u, V, n, k, yv, n);
+      }
+    }
+    if (!rc) {
+      if (jacobi)
+        hipLaunchKernelGGL(k_cmul_vec, dim3(gr(n)), dim3(256), 0, s, dinv, u, (double2 *)x->d, n);
+      else
+        QH_HIP(hipMemcpyAsync(x->d, u, 16 * n, hipMemcpyDeviceToDevice, s));
+      QH_HIP(hipStreamSynchronize(s));
+      if (iters) *iters = it;
+      if (res) *res = rn;
+    }
+    zv.d = nullptr;
+    wv.d = nullptr;
+  }
+done:
+  qh_free_ptr(V);
+  qh_free_ptr(w);
+  qh_free_ptr(z);
+  qh_free_ptr(u);
+  qh_free_ptr(dinv);
+  qh_free_ptr(part);
+  qh_free_ptr(h);
+  qh_free_ptr(yv);
+  return rc;
+}

@@ -1,0 +1,307 @@
+// qh_runtime.hip -- device, stream, memory and complex-vector plumbing.
+#include <cstdarg>
+#include <cstring>
+#include <mutex>
+
+#include "qh_common.h"
+
+static thread_local char g_err[1024] = "";
+static hipStream_t g_stream = nullptr;
+static int g_init = 0;
+static int g_device = 0;
+static int g_cus = 256;
+
+void qh_set_error(const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+
+extern "C" const char *qh_last_error(void) { return g_err; }
+
+extern "C" int qh_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+extern "C" int qh_init(int device) {
+  if (g_init) return QH_OK;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+    qh_set_error("no HIP device available");
+    return QH_ENODEV;
+  }
+  QH_REQUIRE(device >= 0 && device < n, QH_EINVAL, "device %d out of range (%d devices)", device, n);
+  QH_HIP(hipSetDevice(device));
+  // default priority stream; every kernel of the library is ordered on it
+  QH_HIP(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+  hipDeviceProp_t prop;
+  QH_HIP(hipGetDeviceProperties(&prop, device));
+  g_cus = prop.multiProcessorCount;
+  g_device = device;
+  g_init = 1;
+  return QH_OK;
+}
+
+int qh_ensure_init() { return g_init ? QH_OK : qh_init(0); }
+hipStream_t qh_stream() { return g_stream; }
+int qh_num_cus() { return g_cus; }
+
+extern "C" int qh_finalize(void) {
+  if (!g_init) return QH_OK;
+  QH_HIP(hipStreamSynchronize(g_stream));
+  QH_HIP(hipStreamDestroy(g_stream));
+  g_stream = nullptr;
+  g_init = 0;
+  return QH_OK;
+}
+
+extern "C" int qh_synchronize(void) {
+  int rc = qh_ensure_init();
+  if (rc) return rc;
+  QH_HIP(hipStreamSynchronize(g_stream));
+  return QH_OK;
+}
+
+int qh_alloc(void **p, size_t bytes) {
+  int rc = qh_ensure_init();
+  if (rc) return rc;
+  *p = nullptr;
+  if (bytes == 0) bytes = 256;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    qh_set_error("hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    return QH_ENOMEM;
+  }
+  return QH_OK;
+}
+
+void qh_free_ptr(void *p) {
+  if (p) (void)hipFree(p);
+}
+
+extern "C" int qh_dmalloc(void **p, size_t bytes) { return qh_alloc(p, bytes); }
+extern "C" int qh_dfree(void *p) {
+  if (p) QH_HIP(hipFree(p));
+  return QH_OK;
+}
+extern "C" int qh_h2d(void *dst, const void *src, size_t bytes) {
+  int rc = qh_ensure_init();
+  if (rc) return rc;
+  QH_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, g_stream));
+  QH_HIP(hipStreamSynchronize(g_stream));
+  return QH_OK;
+}
+extern "C" int qh_d2h(void *dst, const void *src, size_t bytes) {
+  int rc = qh_ensure_init();
+  if (rc) return rc;
+  QH_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, g_stream));
+  QH_HIP(hipStreamSynchronize(g_stream));
+  return QH_OK;
+}
+
+// ------------------------------------------------------------------ vectors
+__global__ void k_fill_zero_f2(float2 *a, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    a[i] = make_float2(0.f, 0.f);
+}
+__global__ void k_fill_zero_d2(double2 *a, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    a[i] = make_double2(0.0, 0.0);
+}
+// split complex128 into (hi, lo) complex64 pairs: hi = fl(x), lo = fl(x - hi)
+__global__ void k_split_d2(const double2 *src, float2 *hi, float2 *lo, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double2 v = src[i];
+    float hr = (float)v.x, hi_ = (float)v.y;
+    hi[i] = make_float2(hr, hi_);
+    lo[i] = make_float2((float)(v.x - (double)hr), (float)(v.y - (double)hi_));
+  }
+}
+__global__ void k_join_f2(const float2 *hi, const float2 *lo, double2 *dst, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    float2 h = hi[i], l = lo[i];
+    dst[i] = make_double2((double)h.x + (double)l.x, (double)h.y + (double)l.y);
+  }
+}
+__global__ void k_scale_d2(double2 *a, int64_t n, double sr, double si) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double2 v = a[i];
+    a[i] = make_double2(sr * v.x - si * v.y, sr * v.y + si * v.x);
+  }
+}
+__global__ void k_scale_f2(float2 *hi, float2 *lo, int64_t n, double sr, double si) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double xr = (double)hi[i].x + (double)lo[i].x, xi = (double)hi[i].y + (double)lo[i].y;
+    double yr = sr * xr - si * xi, yi = sr * xi + si * xr;
+    float hr = (float)yr, hm = (float)yi;
+    hi[i] = make_float2(hr, hm);
+    lo[i] = make_float2((float)(yr - (double)hr), (float)(yi - (double)hm));
+  }
+}
+__global__ void k_add_entry_d2(double2 *a, int64_t loc, double re, double im) {
+  a[loc] = make_double2(a[loc].x + re, a[loc].y + im);
+}
+__global__ void k_add_entry_f2(float2 *hi, float2 *lo, int64_t loc, double re, double im) {
+  double xr = (double)hi[loc].x + (double)lo[loc].x + re, xi = (double)hi[loc].y + (double)lo[loc].y + im;
+  float hr = (float)xr, hm = (float)xi;
+  hi[loc] = make_float2(hr, hm);
+  lo[loc] = make_float2((float)(xr - (double)hr), (float)(xi - (double)hm));
+}
+
+static unsigned grid_for(int64_t n) {
+  int64_t b = (n + 255) / 256;
+  int64_t cap = (int64_t)qh_num_cus() * 32;
+  if (b > cap) b = cap;
+  if (b < 1) b = 1;
+  return (unsigned)b;
+}
+
+extern "C" int qh_vec_create(int64_t n, int prec, qh_vec **out) {
+  QH_REQUIRE(n > 0 && out, QH_EINVAL, "qh_vec_create: bad arguments");
+  QH_REQUIRE(prec == QH_FP64 || prec == QH_FP32, QH_EINVAL, "qh_vec_create: prec must be 64 or 32");
+  qh_vec *v = new qh_vec();
+  v->n = n;
+  v->prec = prec;
+  int rc = qh_alloc(&v->d, (size_t)n * (prec == QH_FP64 ? sizeof(double2) : sizeof(float2)));
+  if (!rc && prec == QH_FP32) rc = qh_alloc_t(&v->lo, (size_t)n);
+  if (rc) {
+    qh_free_ptr(v->d);
+    qh_free_ptr(v->lo);
+    delete v;
+    return rc;
+  }
+  *out = v;
+  return qh_vec_zero(v);
+}
+
+extern "C" int qh_vec_destroy(qh_vec *v) {
+  if (!v) return QH_OK;
+  qh_free_ptr(v->d);
+  qh_free_ptr(v->lo);
+  delete v;
+  return QH_OK;
+}
+
+extern "C" int qh_vec_size(const qh_vec *v, int64_t *n, int *prec) {
+  QH_REQUIRE(v, QH_EINVAL, "null vector");
+  if (n) *n = v->n;
+  if (prec) *prec = v->prec;
+  return QH_OK;
+}
+
+extern "C" void *qh_vec_data(qh_vec *v) { return v ? v->d : nullptr; }
+
+extern "C" int qh_vec_zero(qh_vec *v) {
+  QH_REQUIRE(v, QH_EINVAL, "null vector");
+  if (v->prec == QH_FP64) {
+    hipLaunchKernelGGL(k_fill_zero_d2, dim3(grid_for(v->n)), dim3(256), 0, qh_stream(), (double2 *)v->d, v->n);
+  } else {
+    hipLaunchKernelGGL(k_fill_zero_f2, dim3(grid_for(v->n)), dim3(256), 0, qh_stream(), (float2 *)v->d, v->n);
+    hipLaunchKernelGGL(k_fill_zero_f2, dim3(grid_for(v->n)), dim3(256), 0, qh_stream(), v->lo, v->n);
+  }
+  QH_HIP(hipGetLastError());
+  return QH_OK;
+}
+
+extern "C" int qh_vec_set(qh_vec *v, const double *host) {
+  QH_REQUIRE(v && host, QH_EINVAL, "qh_vec_set: null argument");
+  if (v->prec == QH_FP64) {
+    QH_HIP(hipMemcpyAsync(v->d, host, (size_t)v->n * 16, hipMemcpyHostToDevice, qh_stream()));
+    QH_HIP(hipStreamSynchronize(qh_stream()));
+    return QH_OK;
+  }
+  double2 *tmp = nullptr;
+  int rc = qh_alloc_t(&tmp, (size_t)v->n);
+  if (rc) return rc;
+  QH_HIP(hipMemcpyAsync(tmp, host, (size_t)v->n * 16, hipMemcpyHostToDevice, qh_stream()));
+  hipLaunchKernelGGL(k_split_d2, dim3(grid_for(v->n)), dim3(256), 0, qh_stream(), tmp, (float2 *)v->d, v->lo, v->n);
+  QH_HIP(hipStreamSynchronize(qh_stream()));
+  qh_free_ptr(tmp);
+  return QH_OK;
+}
+
+extern "C" int qh_vec_get(const qh_vec *v, double *host) {
+  QH_REQUIRE(v && host, QH_EINVAL, "qh_vec_get: null argument");
+  if (v->prec == QH_FP64) {
+    QH_HIP(hipMemcpyAsync(host, v->d, (size_t)v->n * 16, hipMemcpyDeviceToHost, qh_stream()));
+    QH_HIP(hipStreamSynchronize(qh_stream()));
+    return QH_OK;
+  }
+  double2 *tmp = nullptr;
+  int rc = qh_alloc_t(&tmp, (size_t)v->n);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_join_f2, dim3(grid_for(v->n)), dim3(256), 0, qh_stream(), (const float2 *)v->d, v->lo, tmp, v->n);
+  QH_HIP(hipMemcpyAsync(host, tmp, (size_t)v->n * 16, hipMemcpyDeviceToHost, qh_stream()));
+  QH_HIP(hipStreamSynchronize(qh_stream()));
+  qh_free_ptr(tmp);
+  return QH_OK;
+}
+
+extern "C" int qh_vec_add_entry(qh_vec *v, int64_t loc, double re, double im) {
+  QH_REQUIRE(v && loc >= 0 && loc < v->n, QH_ERANGE, "qh_vec_add_entry: location %lld out of range",
+             (long long)loc);
+  if (v->prec == QH_FP64)
+    hipLaunchKernelGGL(k_add_entry_d2, dim3(1), dim3(1), 0, qh_stream(), (double2 *)v->d, loc, re, im);
+  else
+    hipLaunchKernelGGL(k_add_entry_f2, dim3(1), dim3(1), 0, qh_stream(), (float2 *)v->d, v->lo, loc, re, im);
+  QH_HIP(hipGetLastError());
+  return QH_OK;
+}
+
+extern "C" int qh_vec_get_entry(const qh_vec *v, int64_t loc, double *re, double *im) {
+  QH_REQUIRE(v && loc >= 0 && loc < v->n, QH_ERANGE, "qh_vec_get_entry: location out of range");
+  if (v->prec == QH_FP64) {
+    double2 h;
+    QH_HIP(hipMemcpyAsync(&h, (const double2 *)v->d + loc, 16, hipMemcpyDeviceToHost, qh_stream()));
+    QH_HIP(hipStreamSynchronize(qh_stream()));
+    *re = h.x;
+    *im = h.y;
+  } else {
+    float2 h, l;
+    QH_HIP(hipMemcpyAsync(&h, (const float2 *)v->d + loc, 8, hipMemcpyDeviceToHost, qh_stream()));
+    QH_HIP(hipMemcpyAsync(&l, v->lo + loc, 8, hipMemcpyDeviceToHost, qh_stream()));
+    QH_HIP(hipStreamSynchronize(qh_stream()));
+    *re = (double)h.x + (double)l.x;
+    *im = (double)h.y + (double)l.y;
+  }
+  return QH_OK;
+}
+
+__global__ void k_copy_f2_to_d2(const float2 *hi, const float2 *lo, double2 *dst, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    float2 h = hi[i], l = lo[i];
+    dst[i] = make_double2((double)h.x + (double)l.x, (double)h.y + (double)l.y);
+  }
+}
+
+extern "C" int qh_vec_copy(const qh_vec *src, qh_vec *dst) {
+  QH_REQUIRE(src && dst && src->n == dst->n, QH_EINVAL, "qh_vec_copy: size mismatch");
+  hipStream_t s = qh_stream();
+  if (src->prec == dst->prec) {
+    size_t b = (size_t)src->n * (src->prec == QH_FP64 ? 16 : 8);
+    QH_HIP(hipMemcpyAsync(dst->d, src->d, b, hipMemcpyDeviceToDevice, s));
+    if (src->prec == QH_FP32) QH_HIP(hipMemcpyAsync(dst->lo, src->lo, b, hipMemcpyDeviceToDevice, s));
+  } else if (src->prec == QH_FP64) {
+    hipLaunchKernelGGL(k_split_d2, dim3(grid_for(src->n)), dim3(256), 0, s, (const double2 *)src->d, (float2 *)dst->d,
+                       dst->lo, src->n);
+  } else {
+    hipLaunchKernelGGL(k_copy_f2_to_d2, dim3(grid_for(src->n)), dim3(256), 0, s, (const float2 *)src->d, src->lo,
+                       (double2 *)dst->d, src->n);
+  }
+  QH_HIP(hipGetLastError());
+  return QH_OK;
+}
+
+extern "C" int qh_vec_scale(qh_vec *v, double re, double im) {
+  QH_REQUIRE(v, QH_EINVAL, "null vector");
+  if (v->prec == QH_FP64)
+    hipLaunchKernelGGL(k_scale_d2, dim3(grid_for(v->n)), dim3(256), 0, qh_stream(), (double2 *)v->d, v->n, re, im);
+  else
+    hipLaunchKernelGGL(k_scale_f2, dim3(grid_for(v->n)), dim3(256), 0, qh_stream(), (float2 *)v->d, v->lo, v->n, re,
+                       im);
+  QH_HIP(hipGetLastError());
+  return QH_OK;
+}

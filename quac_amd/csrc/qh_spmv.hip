@@ -1,0 +1,902 @@
+// qh_spmv.hip -- the hot path: complex CSR SpMV fused with the RK4 stage updates.
+//
+// Each RK4 step of dx/dt = A x is four launches of one kernel template:
+//   S1: K1 = A x
+//   S2: K2 = A (x + h/2 K1)        (stage input formed on the fly in the gather)
+//   S3: K3 = A (x + h/2 K2)
+//   S4: x' = x + h/6 (K1 + 2 K2 + 2 K3 + A (x + h K3))   (update fused in the epilogue)
+// so the stage vectors Y_s are never materialised: per step the matrix is
+// streamed 4 times and 14 vector passes are made (4 gathers of x, 3 of K, 3 K
+// writes, 3 K reads + x read + x' write in the epilogue).
+//
+// Layout and mapping (gfx950, wave64): a group of G lanes owns one row (64/G rows
+// per wavefront, G autotuned per matrix); lanes stride through the row so every
+// load instruction of a wave reads contiguous col/val bytes; the matrix streams
+// use non-temporal loads so they do not evict x from L2/MALL; the group reduces
+// with cross-lane shuffles and lane 0 runs the fused epilogue.  Block indices
+// are remapped so each XCD sweeps one contiguous band of rows.
+//
+// FP64 path: complex128 values, FP64 accumulation (the drop-in path).
+// FP32 path: complex64 values and vectors; each lane accumulates with TwoSum
+// compensation, the state carries a complex64 compensation word (x = hi + lo)
+// so that the RK update does not lose the low bits of x.
+#include <algorithm>
+#include <vector>
+
+#include "qh_common.h"
+
+namespace {
+
+template <typename R>
+struct C2;
+template <>
+struct C2<double> {
+  typedef double2 T;
+  typedef double __attribute__((ext_vector_type(2))) V;
+};
+template <>
+struct C2<float> {
+  typedef float2 T;
+  typedef float __attribute__((ext_vector_type(2))) V;
+};
+
+template <typename R>
+__device__ inline typename C2<R>::T ld_stream(const typename C2<R>::T *p) {
+  typename C2<R>::V v = __builtin_nontemporal_load(reinterpret_cast<const typename C2<R>::V *>(p));
+  typename C2<R>::T t;
+  t.x = v.x;
+  t.y = v.y;
+  return t;
+}
+
+// error-free sum: s + e == a + b exactly
+__device__ inline void two_sum(float a, float b, float &s, float &e) {
+  s = a + b;
+  float bb = s - a;
+  e = (a - (s - bb)) + (b - bb);
+}
+
+template <typename R>
+struct Acc;
+
+template <>
+struct Acc<double> {
+  double re = 0.0, im = 0.0;
+  __device__ inline void add(double pr, double pi) {
+    re += pr;
+    im += pi;
+  }
+  template <int G>
+  __device__ inline void reduce() {
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1) {
+      re += __shfl_xor(re, off, G);
+      im += __shfl_xor(im, off, G);
+    }
+  }
+  __device__ inline double2 value() const { return make_double2(re, im); }
+};
+
+template <>
+struct Acc<float> {
+  float re = 0.f, im = 0.f, cre = 0.f, cim = 0.f;
+  __device__ inline void add(float pr, float pi) {
+    float e;
+    two_sum(re, pr, re, e);
+    cre += e;
+    two_sum(im, pi, im, e);
+    cim += e;
+  }
+  template <int G>
+  __device__ inline void reduce() {
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1) {
+      float ore = __shfl_xor(re, off, G), oim = __shfl_xor(im, off, G);
+      float ocr = __shfl_xor(cre, off, G), oci = __shfl_xor(cim, off, G);
+      float e;
+      two_sum(re, ore, re, e);
+      cre += e + ocr;
+      two_sum(im, oim, im, e);
+      cim += e + oci;
+    }
+  }
+  __device__ inline float2 value() const { return make_float2(re + cre, im + cim); }
+};
+
+enum { MODE_SPMV = 0, MODE_STAGE = 1, MODE_FINAL = 2 };
+
+template <typename R, int G, int MODE>
+__global__ void __launch_bounds__(256) k_rk_stage(
+    int32_t nrows, const int32_t *__restrict__ rp, const int32_t *__restrict__ col,
+    const typename C2<R>::T *__restrict__ val, const typename C2<R>::T *__restrict__ x,
+    const typename C2<R>::T *__restrict__ kin, R ck, typename C2<R>::T *__restrict__ kout,
+    const typename C2<R>::T *__restrict__ k1, const typename C2<R>::T *__restrict__ k2,
+    const typename C2<R>::T *__restrict__ k3, typename C2<R>::T *__restrict__ xout, R h,
+    float2 *__restrict__ xlo) {
+  typedef typename C2<R>::T T;
+  const unsigned b = qh_xcd_block(blockIdx.x, gridDim.x);
+  const int64_t tid = (int64_t)b * 256 + threadIdx.x;
+  const int32_t row = (int32_t)(tid / G);
+  const int lane = (int)(tid % G);
+  if (row >= nrows) return;  // G divides 256: a group never straddles the bound
+  const int32_t beg = rp[row], end = rp[row + 1];
+  Acc<R> acc;
+  int32_t k = beg + lane;
+  // two independent loads in flight per lane
+  for (; k + G < end; k += 2 * G) {
+    const int32_t c0 = __builtin_nontemporal_load(col + k);
+    const int32_t c1 = __builtin_nontemporal_load(col + k + G);
+    const T a0 = ld_stream<R>(val + k);
+    const T a1 = ld_stream<R>(val + k + G);
+    T x0 = x[c0], x1 = x[c1];
+    if (MODE != MODE_SPMV) {
+      const T q0 = kin[c0], q1 = kin[c1];
+      x0.x += ck * q0.x;
+      x0.y += ck * q0.y;
+      x1.x += ck * q1.x;
+      x1.y += ck * q1.y;
+    }
+    acc.add(a0.x * x0.x - a0.y * x0.y, a0.x * x0.y + a0.y * x0.x);
+    acc.add(a1.x * x1.x - a1.y * x1.y, a1.x * x1.y + a1.y * x1.x);
+  }
+  if (k < end) {
+    const int32_t c0 = __builtin_nontemporal_load(col + k);
+    const T a0 = ld_stream<R>(val + k);
+    T x0 = x[c0];
+    if (MODE != MODE_SPMV) {
+      const T q0 = kin[c0];
+      x0.x += ck * q0.x;
+      x0.y += ck * q0.y;
+    }
+    acc.add(a0.x * x0.x - a0.y * x0.y, a0.x * x0.y + a0.y * x0.x);
+  }
+  acc.template reduce<G>();
+  if (lane != 0) return;
+  const T y = acc.value();
+  if (MODE != MODE_FINAL) {
+    kout[row] = y;
+    return;
+  }
+  // x' = x + h/6 (K1 + 2K2 + 2K3 + K4)
+  const T a = k1[row], bq = k2[row], cq = k3[row];
+  const R s6 = h * (R)(1.0 / 6.0), s3 = h * (R)(1.0 / 3.0);
+  const R dre = s6 * a.x + s3 * bq.x + s3 * cq.x + s6 * y.x;
+  const R dim = s6 * a.y + s3 * bq.y + s3 * cq.y + s6 * y.y;
+  const T xo = x[row];
+  if (sizeof(R) == 8) {
+    T r;
+    r.x = xo.x + dre;
+    r.y = xo.y + dim;
+    xout[row] = r;
+  } else {
+    // compensated update of (hi, lo): hi' + lo' = hi + lo + d to ~2^-48
+    float2 lo = xlo[row];
+    float sr, er, si, ei;
+    two_sum((float)xo.x, (float)dre, sr, er);
+    two_sum((float)xo.y, (float)dim, si, ei);
+    float lr = lo.x + er, li = lo.y + ei;
+    float hr = sr + lr, hi = si + li;
+    lr = lr - (hr - sr);
+    li = li - (hi - si);
+    T r;
+    r.x = hr;
+    r.y = hi;
+    xout[row] = r;
+    xlo[row] = make_float2(lr, li);
+  }
+}
+
+template <typename R, int MODE>
+int launch_stage_g(int G, hipStream_t s, const qh_mat *A, const typename C2<R>::T *val, const void *x,
+                   const void *kin, R ck, void *kout, const void *k1, const void *k2, const void *k3, void *xout,
+                   R h, float2 *xlo) {
+  typedef typename C2<R>::T T;
+  const int32_t n = (int32_t)A->rows;
+  const unsigned grid = qh_blocks((int64_t)n * G, 256);
+#define QH_L(GG)                                                                                                 \
+  hipLaunchKernelGGL((k_rk_stage<R, GG, MODE>), dim3(grid), dim3(256), 0, s, n, A->rowptr, A->col, val,          \
+                     (const T *)x, (const T *)kin, ck, (T *)kout, (const T *)k1, (const T *)k2, (const T *)k3,   \
+                     (T *)xout, h, xlo)
+  switch (G) {
+    case 1: QH_L(1); break;
+    case 2: QH_L(2); break;
+    case 4: QH_L(4); break;
+    case 8: QH_L(8); break;
+    case 16: QH_L(16); break;
+    case 32: QH_L(32); break;
+    case 64: QH_L(64); break;
+    default:
+      qh_set_error("unsupported lanes-per-row %d", G);
+      return QH_EINVAL;
+  }
+#undef QH_L
+  return QH_OK;
+}
+
+// ------------------------------------------------------------ autotuner
+// Time one SpMV for each lanes-per-row choice on the real matrix and keep the
+// fastest (rows-per-wavefront = 64/G).
+template <typename R>
+int autotune(qh_mat *A, int *out_g) {
+  typedef typename C2<R>::T T;
+  const typename C2<R>::T *val = sizeof(R) == 8 ? (const T *)A->val : (const T *)A->val32;
+  void *x = nullptr, *y = nullptr;
+  int rc = qh_alloc(&x, (size_t)A->rows * sizeof(T));
+  if (!rc) rc = qh_alloc(&y, (size_t)A->rows * sizeof(T));
+  if (rc) {
+    qh_free_ptr(x);
+    return rc;
+  }
+  hipStream_t s = qh_stream();
+  QH_HIP(hipMemsetAsync(x, 0, (size_t)A->rows * sizeof(T), s));
+  hipEvent_t e0, e1;
+  QH_HIP(hipEventCreate(&e0));
+  QH_HIP(hipEventCreate(&e1));
+  const double avg = (double)A->nnz / (double)(A->rows > 0 ? A->rows : 1);
+  std::vector<int> cands;
+  for (int g : {2, 4, 8, 16, 32, 64})
+    if (g <= 4 * avg + 4) cands.push_back(g);
+  if (cands.empty()) cands.push_back(2);
+  float best = 1e30f;
+  int bestg = cands[0];
+  for (int g : cands) {
+    for (int w = 0; w < 2; w++)
+      launch_stage_g<R, MODE_SPMV>(g, s, A, val, x, nullptr, (R)0, y, nullptr, nullptr, nullptr, nullptr, (R)0,
+                                   nullptr);
+    QH_HIP(hipEventRecord(e0, s));
+    const int reps = 5;
+    for (int w = 0; w < reps; w++)
+      launch_stage_g<R, MODE_SPMV>(g, s, A, val, x, nullptr, (R)0, y, nullptr, nullptr, nullptr, nullptr, (R)0,
+                                   nullptr);
+    QH_HIP(hipEventRecord(e1, s));
+    QH_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    QH_HIP(hipEventElapsedTime(&ms, e0, e1));
+    if (ms < best) {
+      best = ms;
+      bestg = g;
+    }
+  }
+  QH_HIP(hipEventDestroy(e0));
+  QH_HIP(hipEventDestroy(e1));
+  qh_free_ptr(x);
+  qh_free_ptr(y);
+  *out_g = bestg;
+  return QH_OK;
+}
+
+int lanes_for(qh_mat *A, int prec, int *g) {
+  int rc = QH_OK;
+  if (prec == QH_FP64) {
+    if (!A->lanes64) rc = autotune<double>(A, &A->lanes64);
+    *g = A->lanes64;
+  } else {
+    if (!A->val32) rc = qh_mat_make_fp32(A);
+    if (!rc && !A->lanes32) rc = autotune<float>(A, &A->lanes32);
+    *g = A->lanes32;
+  }
+  return rc;
+}
+
+// ------------------------------------------------------------ trace renorm
+// FP64 trace of a complex64 (hi+lo) density matrix: per-block partial sums,
+// then one block sums them and writes 1/Tr; the scale kernel reads it.
+__global__ void __launch_bounds__(256) k_trace_partial_f32(const float2 *hi, const float2 *lo, int64_t tot,
+                                                           int64_t base, double *partial) {
+  __shared__ double sh[256];
+  double acc = 0.0;
+  for (int64_t i = blockIdx.x * 256 + threadIdx.x; i < tot; i += (int64_t)gridDim.x * 256) {
+    int64_t loc = base + i * (tot + 1);
+    acc += (double)hi[loc].x + (double)lo[loc].x;
+  }
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+}
+
+__global__ void k_trace_final(const double *partial, int n, double *inv) {
+  __shared__ double sh[256];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) acc += partial[i];
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) inv[0] = (sh[0] != 0.0) ? 1.0 / sh[0] : 1.0;
+}
+
+__global__ void k_scale_by_dev(float2 *hi, float2 *lo, int64_t n, const double *inv) {
+  const double s = inv[0];
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double xr = ((double)hi[i].x + (double)lo[i].x) * s, xi = ((double)hi[i].y + (double)lo[i].y) * s;
+    float hr = (float)xr, hm = (float)xi;
+    hi[i] = make_float2(hr, hm);
+    lo[i] = make_float2((float)(xr - (double)hr), (float)(xi - (double)hm));
+  }
+}
+
+}  // namespace
+
+// ------------------------------------------------------------ stepper state
+struct Stepper {
+  const qh_mat *A = nullptr;
+  int prec = 0;
+  int64_t n = 0;
+  void *k[3] = {nullptr, nullptr, nullptr};
+  void *xalt = nullptr;  // double buffer for the state (hi part for FP32)
+  double *partial = nullptr, *inv = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipGraph_t graph = nullptr;
+  // parameters the graph was captured with
+  void *gx = nullptr;
+  float2 *glo = nullptr;
+  double gdt = 0;
+  int gsteps = 0, grenorm = 0, gG = 0;
+  int64_t gtot = 0;
+};
+
+static Stepper g_st;
+static int g_graph_steps = 8;
+static double g_last_stage_ms = 0.0;
+static int64_t g_last_launches = 0;
+
+static void stepper_release() {
+  if (g_st.exec) (void)hipGraphExecDestroy(g_st.exec);
+  if (g_st.graph) (void)hipGraphDestroy(g_st.graph);
+  for (auto &p : g_st.k) qh_free_ptr(p);
+  qh_free_ptr(g_st.xalt);
+  qh_free_ptr(g_st.partial);
+  qh_free_ptr(g_st.inv);
+  g_st = Stepper();
+}
+
+static int stepper_prepare(const qh_mat *A, int prec, int64_t n) {
+  if (g_st.A == A && g_st.prec == prec && g_st.n == n && g_st.k[0]) return QH_OK;
+  (void)hipStreamSynchronize(qh_stream());
+  stepper_release();
+  size_t es = prec == QH_FP64 ? 16 : 8;
+  int rc = QH_OK;
+  for (auto &p : g_st.k)
+    if (!rc) rc = qh_alloc(&p, (size_t)n * es);
+  if (!rc) rc = qh_alloc(&g_st.xalt, (size_t)n * es);
+  if (!rc) rc = qh_alloc_t(&g_st.partial, 1024);
+  if (!rc) rc = qh_alloc_t(&g_st.inv, 1);
+  if (rc) {
+    stepper_release();
+    return rc;
+  }
+  g_st.A = A;
+  g_st.prec = prec;
+  g_st.n = n;
+  return QH_OK;
+}
+
+// one RK4 step from xa into xb (xb != xa), launched on stream s
+template <typename R>
+static int rk4_step(hipStream_t s, const qh_mat *A, int G, const void *xa, void *xb, float2 *xlo, R h,
+                    hipEvent_t *ev) {
+  typedef typename C2<R>::T T;
+  const T *val = sizeof(R) == 8 ? (const T *)A->val : (const T *)A->val32;
+  void **k = g_st.k;
+  int rc;
+  if (ev) QH_HIP(hipEventRecord(ev[0], s));
+  rc = launch_stage_g<R, MODE_SPMV>(G, s, A, val, xa, nullptr, (R)0, k[0], nullptr, nullptr, nullptr, nullptr, h,
+                                    xlo);
+  if (ev) QH_HIP(hipEventRecord(ev[1], s));
+  if (!rc)
+    rc = launch_stage_g<R, MODE_STAGE>(G, s, A, val, xa, k[0], h / 2, k[1], nullptr, nullptr, nullptr, nullptr, h,
+                                       xlo);
+  if (ev) QH_HIP(hipEventRecord(ev[2], s));
+  if (!rc)
+    rc = launch_stage_g<R, MODE_STAGE>(G, s, A, val, xa, k[1], h / 2, k[2], nullptr, nullptr, nullptr, nullptr, h,
+                                       xlo);
+  if (ev) QH_HIP(hipEventRecord(ev[3], s));
+  if (!rc)
+    rc = launch_stage_g<R, MODE_FINAL>(G, s, A, val, xa, k[2], h, nullptr, k[0], k[1], k[2], xb, h, xlo);
+  if (ev) QH_HIP(hipEventRecord(ev[4], s));
+  return rc;
+}
+
+static int renorm_launch(hipStream_t s, qh_vec *x, void *hi, int64_t tot, int64_t base) {
+  const int nb = 512;
+  hipLaunchKernelGGL(k_trace_partial_f32, dim3(nb), dim3(256), 0, s, (const float2 *)hi, x->lo, tot, base,
+                     g_st.partial);
+  hipLaunchKernelGGL(k_trace_final, dim3(1), dim3(256), 0, s, g_st.partial, nb, g_st.inv);
+  hipLaunchKernelGGL(k_scale_by_dev, dim3(std::min<int64_t>(qh_blocks(x->n, 256), 8192)), dim3(256), 0, s,
+                     (float2 *)hi, x->lo, x->n, g_st.inv);
+  QH_HIP(hipGetLastError());
+  return QH_OK;
+}
+
+extern "C" int qh_set_graph_steps(int steps) {
+  QH_REQUIRE(steps >= 1 && steps <= 1024, QH_EINVAL, "graph steps must be in [1,1024]");
+  g_graph_steps = steps;
+  return QH_OK;
+}
+
+extern "C" int qh_last_stage_ms(double *spmv_ms, int64_t *launches) {
+  if (spmv_ms) *spmv_ms = g_last_stage_ms;
+  if (launches) *launches = g_last_launches;
+  return QH_OK;
+}
+
+// Advance x in place by nsteps RK4 steps.  Steps are grouped in pairs (x -> alt
+// -> x) so a captured graph always starts and ends on the caller's buffer.
+static int rk4_impl(qh_mat *A, qh_vec *x, double dt, int64_t nsteps, int64_t tot, int renorm_every, int timed) {
+  int rc = qh_ensure_init();
+  if (rc) return rc;
+  QH_REQUIRE(A && x && x->n == A->rows, QH_EINVAL, "qh_rk4: vector size %lld != matrix rows %lld",
+             x ? (long long)x->n : -1LL, A ? (long long)A->rows : -1LL);
+  QH_REQUIRE(nsteps >= 0, QH_EINVAL, "qh_rk4: negative step count");
+  QH_REQUIRE(renorm_every == 0 || (x->prec == QH_FP32 && tot > 0 && tot * tot == x->n), QH_EINVAL,
+             "qh_rk4: trace renormalisation needs an FP32 density matrix of dimension tot");
+  int G = 0;
+  rc = lanes_for(A, x->prec, &G);
+  if (rc) return rc;
+  rc = stepper_prepare(A, x->prec, x->n);
+  if (rc) return rc;
+  hipStream_t s = qh_stream();
+  if (nsteps == 0) return QH_OK;
+
+  if (timed) {
+    // eager launches with events around every stage kernel: per-launch durations
+    std::vector<hipEvent_t> ev(5 * nsteps);
+    for (auto &e : ev) QH_HIP(hipEventCreate(&e));
+    void *bufs[2] = {x->d, g_st.xalt};
+    for (int64_t st = 0; st < nsteps; st++) {
+      void *xa = bufs[st & 1], *xb = bufs[(st + 1) & 1];
+      rc = x->prec == QH_FP64 ? rk4_step<double>(s, A, G, xa, xb, x->lo, dt, &ev[5 * st])
+                              : rk4_step<float>(s, A, G, xa, xb, x->lo, (float)dt, &ev[5 * st]);
+      if (rc) return rc;
+      if (renorm_every > 0 && (st + 1) % renorm_every == 0) renorm_launch(s, x, xb, tot, 0);
+    }
+    if (nsteps & 1) QH_HIP(hipMemcpyAsync(x->d, g_st.xalt, x->n * (x->prec == QH_FP64 ? 16 : 8),
+                                          hipMemcpyDeviceToDevice, s));
+    QH_HIP(hipStreamSynchronize(s));
+    double tot_ms = 0.0;
+    for (int64_t st = 0; st < nsteps; st++)
+      for (int q = 0; q < 4; q++) {
+        float ms = 0.f;
+        QH_HIP(hipEventElapsedTime(&ms, ev[5 * st + q], ev[5 * st + q + 1]));
+        tot_ms += ms;
+      }
+    g_last_stage_ms = tot_ms / (4.0 * (double)nsteps);
+    g_last_launches = 4 * nsteps;
+    for (auto &e : ev) QH_HIP(hipEventDestroy(e));
+    return QH_OK;
+  }
+
+  // graph path: capture 2*P steps (P pairs) once, replay
+  int pair_steps = g_graph_steps & ~1;
+  if (pair_steps < 2) pair_steps = 2;
+  if (renorm_every > 0) {
+    // renormalise exactly every renorm_every steps: capture that many (rounded to pairs)
+    pair_steps = renorm_every + (renorm_every & 1);
+  }
+  const int64_t full = nsteps / pair_steps;
+  if (full > 0) {
+    const bool recapture = !g_st.exec || g_st.gx != x->d || g_st.glo != x->lo || g_st.gdt != dt ||
+                           g_st.gsteps != pair_steps || g_st.grenorm != renorm_every || g_st.gG != G ||
+                           g_st.gtot != tot;
+    if (recapture) {
+      if (g_st.exec) (void)hipGraphExecDestroy(g_st.exec);
+      if (g_st.graph) (void)hipGraphDestroy(g_st.graph);
+      g_st.exec = nullptr;
+      g_st.graph = nullptr;
+      QH_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      int crc = QH_OK;
+      for (int st = 0; st < pair_steps && !crc; st++) {
+        void *xa = (st & 1) ? g_st.xalt : x->d, *xb = (st & 1) ? x->d : g_st.xalt;
+        crc = x->prec == QH_FP64 ? rk4_step<double>(s, A, G, xa, xb, x->lo, dt, nullptr)
+                                 : rk4_step<float>(s, A, G, xa, xb, x->lo, (float)dt, nullptr);
+      }
+      if (!crc && renorm_every > 0) crc = renorm_launch(s, x, x->d, tot, 0);
+      hipGraph_t g = nullptr;
+      hipError_t ce = hipStreamEndCapture(s, &g);
+      if (crc) {
+        if (g) (void)hipGraphDestroy(g);
+        return crc;
+      }
+      QH_HIP(ce);
+      g_st.graph = g;
+      QH_HIP(hipGraphInstantiate(&g_st.exec, g, nullptr, nullptr, 0));
+      g_st.gx = x->d;
+      g_st.glo = x->lo;
+      g_st.gdt = dt;
+      g_st.gsteps = pair_steps;
+      g_st.grenorm = renorm_every;
+      g_st.gG = G;
+      g_st.gtot = tot;
+    }
+    for (int64_t r = 0; r < full; r++) QH_HIP(hipGraphLaunch(g_st.exec, s));
+  }
+  // remainder: eager launches
+  const int64_t rem = nsteps - full * pair_steps;
+  for (int64_t st = 0; st < rem; st++) {
+    void *xa = (st & 1) ? g_st.xalt : x->d, *xb = (st & 1) ? x->d : g_st.xalt;
+    rc = x->prec == QH_FP64 ? rk4_step<double>(s, A, G, xa, xb, x->lo, dt, nullptr)
+                            : rk4_step<float>(s, A, G, xa, xb, x->lo, (float)dt, nullptr);
+    if (rc) return rc;
+  }
+  if (rem & 1)
+    QH_HIP(hipMemcpyAsync(x->d, g_st.xalt, x->n * (x->prec == QH_FP64 ? 16 : 8), hipMemcpyDeviceToDevice, s));
+  QH_HIP(hipGetLastError());
+  return QH_OK;
+}
+
+extern "C" int qh_rk4(qh_mat *A, qh_vec *x, double dt, int64_t nsteps, int64_t tot, int renorm_every) {
+  return rk4_impl(A, x, dt, nsteps, tot, renorm_every, 0);
+}
+
+extern "C" int qh_rk4_timed(qh_mat *A, qh_vec *x, double dt, int64_t nsteps, int64_t tot, int renorm_every) {
+  return rk4_impl(A, x, dt, nsteps, tot, renorm_every, 1);
+}
+
+extern "C" int qh_mat_set_lanes(qh_mat *m, int l64, int l32) {
+  QH_REQUIRE(m, QH_EINVAL, "null matrix");
+  auto ok = [](int g) { return g == 0 || g == 1 || g == 2 || g == 4 || g == 8 || g == 16 || g == 32 || g == 64; };
+  QH_REQUIRE(ok(l64) && ok(l32), QH_EINVAL, "lanes per row must be a power of two <= 64");
+  m->lanes64 = l64;
+  m->lanes32 = l32;
+  if (g_st.A == m) g_st.gG = -1;  // force a re-capture
+  return QH_OK;
+}
+
+extern "C" int qh_mat_get_lanes(const qh_mat *m, int *l64, int *l32) {
+  QH_REQUIRE(m, QH_EINVAL, "null matrix");
+  if (l64) *l64 = m->lanes64;
+  if (l32) *l32 = m->lanes32;
+  return QH_OK;
+}
+
+// mean duration of `reps` plain SpMV launches (y = A x) measured with events
+extern "C" int qh_bench_spmv(qh_mat *A, int prec, int reps, double *avg_ms) {
+  int rc = qh_ensure_init();
+  if (rc) return rc;
+  QH_REQUIRE(A && avg_ms && reps > 0 && (prec == QH_FP64 || prec == QH_FP32), QH_EINVAL, "qh_bench_spmv: bad args");
+  int G = 0;
+  rc = lanes_for(A, prec, &G);
+  if (rc) return rc;
+  const size_t es = prec == QH_FP64 ? 16 : 8;
+  void *x = nullptr, *y = nullptr;
+  rc = qh_alloc(&x, (size_t)A->rows * es);
+  if (!rc) rc = qh_alloc(&y, (size_t)A->rows * es);
+  if (rc) {
+    qh_free_ptr(x);
+    return rc;
+  }
+  hipStream_t s = qh_stream();
+  QH_HIP(hipMemsetAsync(x, 0, (size_t)A->rows * es, s));
+  std::vector<hipEvent_t> ev(reps + 1);
+  for (auto &e : ev) QH_HIP(hipEventCreate(&e));
+  for (int r = 0; r < reps && !rc; r++) {
+    QH_HIP(hipEventRecord(ev[r], s));
+    rc = prec == QH_FP64 ? launch_stage_g<double, MODE_SPMV>(G, s, A, A->val, x, nullptr, 0.0, y, nullptr, nullptr,
+                                                             nullptr, nullptr, 0.0, nullptr)
+                         : launch_stage_g<float, MODE_SPMV>(G, s, A, A->val32, x, nullptr, 0.f, y, nullptr, nullptr,
+                                                            nullptr, nullptr, 0.f, nullptr);
+  }
+  QH_HIP(hipEventRecord(ev[reps], s));
+  QH_HIP(hipEventSynchronize(ev[reps]));
+  double tot = 0.0;
+  for (int r = 0; r < reps; r++) {
+    float ms = 0.f;
+    QH_HIP(hipEventElapsedTime(&ms, ev[r], ev[r + 1]));
+    tot += ms;
+  }
+  *avg_ms = tot / reps;
+  for (auto &e : ev) QH_HIP(hipEventDestroy(e));
+  qh_free_ptr(x);
+  qh_free_ptr(y);
+  return rc;
+}
+
+extern "C" int qh_spmv(const qh_mat *A_, const qh_vec *x, qh_vec *y) {
+  qh_mat *A = const_cast<qh_mat *>(A_);
+  QH_REQUIRE(A && x && y && x->n == A->rows && y->n == A->rows && x->prec == y->prec && x->d != y->d, QH_EINVAL,
+             "qh_spmv: bad arguments");
+  int G = 0;
+  int rc = lanes_for(A, x->prec, &G);
+  if (rc) return rc;
+  hipStream_t s = qh_stream();
+  if (x->prec == QH_FP64)
+    rc = launch_stage_g<double, MODE_SPMV>(G, s, A, A->val, x->d, nullptr, 0.0, y->d, nullptr, nullptr, nullptr,
+                                           nullptr, 0.0, nullptr);
+  else {
+    rc = launch_stage_g<float, MODE_SPMV>(G, s, A, A->val32, x->d, nullptr, 0.f, y->d, nullptr, nullptr, nullptr,
+                                          nullptr, 0.f, nullptr);
+    if (!rc) QH_HIP(hipMemsetAsync(y->lo, 0, (size_t)y->n * 8, s));
+  }
+  QH_HIP(hipGetLastError());
+  return rc;
+}
+
+// ------------------------------------------------ generic vector helpers (FP64)
+namespace {
+__global__ void k_axpy_to(double2 *y, const double2 *x, double cr, double ci, const double2 *k, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double2 a = x[i], b = k[i];
+    y[i] = make_double2(a.x + cr * b.x - ci * b.y, a.y + cr * b.y + ci * b.x);
+  }
+}
+struct LinArgs {
+  const double2 *v[8];
+  double cr[8], ci[8];
+  int n;
+};
+__global__ void k_lincomb(double2 *y, LinArgs a, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double re = 0.0, im = 0.0;
+    for (int j = 0; j < a.n; j++) {
+      double2 b = a.v[j][i];
+      re += a.cr[j] * b.x - a.ci[j] * b.y;
+      im += a.cr[j] * b.y + a.ci[j] * b.x;
+    }
+    y[i] = make_double2(re, im);
+  }
+}
+__global__ void k_add_scaled(double2 *y, const double2 *w, double sr, double si, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double2 a = y[i], b = w[i];
+    y[i] = make_double2(a.x + sr * b.x - si * b.y, a.y + sr * b.y + si * b.x);
+  }
+}
+__global__ void __launch_bounds__(256) k_rk3bs_err(const double2 *x, const double2 *xn, const double2 *k1,
+                                                   const double2 *k2, const double2 *k3, const double2 *k4,
+                                                   double h, double atol, double rtol, int64_t n, double *partial) {
+  __shared__ double sh[256];
+  double acc = 0.0;
+  const double c1 = 2.0 / 9.0 - 7.0 / 24.0, c2 = 1.0 / 3.0 - 1.0 / 4.0, c3 = 4.0 / 9.0 - 1.0 / 3.0, c4 = -1.0 / 8.0;
+  for (int64_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    double er = h * (c1 * k1[i].x + c2 * k2[i].x + c3 * k3[i].x + c4 * k4[i].x);
+    double ei = h * (c1 * k1[i].y + c2 * k2[i].y + c3 * k3[i].y + c4 * k4[i].y);
+    double ax = sqrt(x[i].x * x[i].x + x[i].y * x[i].y), an = sqrt(xn[i].x * xn[i].x + xn[i].y * xn[i].y);
+    double sc = atol + rtol * (ax > an ? ax : an);
+    acc += (er * er + ei * ei) / (sc * sc);
+  }
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+}
+unsigned gridn(int64_t n) { return (unsigned)std::min<int64_t>(qh_blocks(n, 256), 8192); }
+}  // namespace
+
+extern "C" int qh_axpy_to(qh_vec *y, const qh_vec *x, double cr, double ci, const qh_vec *k) {
+  QH_REQUIRE(y && x && k && y->n == x->n && k->n == x->n && x->prec == QH_FP64 && y->prec == QH_FP64 &&
+                 k->prec == QH_FP64,
+             QH_EINVAL, "qh_axpy_to: FP64 vectors of equal size required");
+  hipLaunchKernelGGL(k_axpy_to, dim3(gridn(x->n)), dim3(256), 0, qh_stream(), (double2 *)y->d,
+                     (const double2 *)x->d, cr, ci, (const double2 *)k->d, x->n);
+  QH_HIP(hipGetLastError());
+  return QH_OK;
+}
+
+extern "C" int qh_lincomb(qh_vec *y, int n, const double *c, qh_vec *const *v) {
+  QH_REQUIRE(y && n >= 1 && n <= 8 && y->prec == QH_FP64, QH_EINVAL, "qh_lincomb: 1..8 FP64 vectors");
+  LinArgs a;
+  a.n = n;
+  for (int j = 0; j < n; j++) {
+    QH_REQUIRE(v[j] && v[j]->n == y->n && v[j]->prec == QH_FP64, QH_EINVAL, "qh_lincomb: size mismatch");
+    a.v[j] = (const double2 *)v[j]->d;
+    a.cr[j] = c[2 * j];
+    a.ci[j] = c[2 * j + 1];
+  }
+  hipLaunchKernelGGL(k_lincomb, dim3(gridn(y->n)), dim3(256), 0, qh_stream(), (double2 *)y->d, a, y->n);
+  QH_HIP(hipGetLastError());
+  return QH_OK;
+}
+
+extern "C" int qh_spmv_td(const qh_mat *A, int ntd, qh_mat *const *Ak, const double *s, const qh_vec *x,
+                          qh_vec *y) {
+  QH_REQUIRE(x && y && x->prec == QH_FP64 && y->prec == QH_FP64, QH_EINVAL, "qh_spmv_td: FP64 only");
+  int rc = qh_spmv(A, x, y);
+  if (rc || ntd == 0) return rc;
+  qh_vec *w = nullptr;
+  rc = qh_vec_create(x->n, QH_FP64, &w);
+  if (rc) return rc;
+  for (int k = 0; k < ntd && !rc; k++) {
+    // only terms above min_time_dep (qsystem.c:34,1320)
+    if (sqrt(s[2 * k] * s[2 * k] + s[2 * k + 1] * s[2 * k + 1]) <= 1e-10) continue;
+    rc = qh_spmv(Ak[k], x, w);
+    if (!rc)
+      hipLaunchKernelGGL(k_add_scaled, dim3(gridn(x->n)), dim3(256), 0, qh_stream(), (double2 *)y->d,
+                         (const double2 *)w->d, s[2 * k], s[2 * k + 1], x->n);
+  }
+  qh_vec_destroy(w);
+  QH_HIP(hipGetLastError());
+  return rc;
+}
+
+extern "C" int qh_rk3bs_err(const qh_vec *x, const qh_vec *xn, const qh_vec *k1, const qh_vec *k2,
+                            const qh_vec *k3, const qh_vec *k4, double h, double atol, double rtol, double *err) {
+  QH_REQUIRE(x && xn && k1 && k2 && k3 && k4 && err && x->prec == QH_FP64, QH_EINVAL, "qh_rk3bs_err: bad args");
+  const int nb = 512;
+  double *partial = nullptr;
+  int rc = qh_alloc_t(&partial, nb);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_rk3bs_err, dim3(nb), dim3(256), 0, qh_stream(), (const double2 *)x->d,
+                     (const double2 *)xn->d, (const double2 *)k1->d, (const double2 *)k2->d,
+                     (const double2 *)k3->d, (const double2 *)k4->d, h, atol, rtol, x->n, partial);
+  double hp[nb];
+  QH_HIP(hipMemcpyAsync(hp, partial, sizeof(hp), hipMemcpyDeviceToHost, qh_stream()));
+  QH_HIP(hipStreamSynchronize(qh_stream()));
+  qh_free_ptr(partial);
+  double sum = 0.0;
+  for (int i = 0; i < nb; i++) sum += hp[i];
+  *err = sqrt(sum / (double)x->n);
+  return QH_OK;
+}
+
+// ------------------------------------------------------------------ batches
+struct qh_batch {
+  qh_mat *A = nullptr;            // block-diagonal matrix of all instances
+  qh_vec *x = nullptr;            // concatenated states
+  int n = 0;
+  std::vector<int64_t> off, rows, tot;
+  double *partial = nullptr;      // per-instance trace scratch
+  double *inv = nullptr;
+  int64_t *doff = nullptr, *dtot = nullptr;
+};
+
+namespace {
+// per-instance FP64 trace (one block per instance) and rescale
+__global__ void __launch_bounds__(256) k_batch_trace(const float2 *hi, const float2 *lo, const int64_t *off,
+                                                     const int64_t *tot, double *inv) {
+  __shared__ double sh[256];
+  const int b = blockIdx.x;
+  const int64_t N = tot[b], base = off[b];
+  double acc = 0.0;
+  for (int64_t i = threadIdx.x; i < N; i += 256) {
+    int64_t loc = base + i * (N + 1);
+    acc += (double)hi[loc].x + (double)lo[loc].x;
+  }
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) inv[b] = sh[0] != 0.0 ? 1.0 / sh[0] : 1.0;
+}
+__global__ void k_batch_scale(float2 *hi, float2 *lo, const int64_t *off, const double *inv, int n, int64_t total) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+    // binary search of the instance owning entry i
+    int lo_i = 0, hi_i = n - 1;
+    while (lo_i < hi_i) {
+      int mid = (lo_i + hi_i + 1) / 2;
+      if (off[mid] <= i) lo_i = mid;
+      else hi_i = mid - 1;
+    }
+    const double s = inv[lo_i];
+    double xr = ((double)hi[i].x + (double)lo[i].x) * s, xi = ((double)hi[i].y + (double)lo[i].y) * s;
+    float hr = (float)xr, hm = (float)xi;
+    hi[i] = make_float2(hr, hm);
+    lo[i] = make_float2((float)(xr - (double)hr), (float)(xi - (double)hm));
+  }
+}
+}  // namespace
+
+extern "C" int qh_batch_create(int n, qh_mat *const *mats, const int64_t *tot, int prec, qh_batch **out) {
+  QH_REQUIRE(n > 0 && mats && tot && out, QH_EINVAL, "qh_batch_create: bad arguments");
+  qh_batch *b = new qh_batch();
+  b->n = n;
+  int64_t o = 0;
+  for (int i = 0; i < n; i++) {
+    b->off.push_back(o);
+    b->rows.push_back(mats[i]->rows);
+    b->tot.push_back(tot[i]);
+    if (tot[i] * tot[i] != mats[i]->rows) {
+      delete b;
+      qh_set_error("qh_batch_create: instance %d is not a density-matrix superoperator of dimension %lld", i,
+                   (long long)tot[i]);
+      return QH_EINVAL;
+    }
+    o += mats[i]->rows;
+  }
+  int rc = qh_mat_blockdiag(n, mats, &b->A);
+  if (!rc && prec == QH_FP32) rc = qh_mat_make_fp32(b->A);
+  if (!rc) rc = qh_vec_create(o, prec, &b->x);
+  if (!rc) rc = qh_alloc_t(&b->inv, n);
+  if (!rc) rc = qh_alloc_t(&b->doff, n);
+  if (!rc) rc = qh_alloc_t(&b->dtot, n);
+  if (!rc) rc = qh_h2d(b->doff, b->off.data(), 8 * n);
+  if (!rc) rc = qh_h2d(b->dtot, b->tot.data(), 8 * n);
+  if (rc) {
+    qh_batch_destroy(b);
+    return rc;
+  }
+  *out = b;
+  return QH_OK;
+}
+
+extern "C" int qh_batch_destroy(qh_batch *b) {
+  if (!b) return QH_OK;
+  if (g_st.A == b->A) {
+    (void)hipStreamSynchronize(qh_stream());
+    stepper_release();
+  }
+  qh_mat_destroy(b->A);
+  qh_vec_destroy(b->x);
+  qh_free_ptr(b->inv);
+  qh_free_ptr(b->doff);
+  qh_free_ptr(b->dtot);
+  delete b;
+  return QH_OK;
+}
+
+extern "C" int qh_batch_info(const qh_batch *b, int64_t *rows, int64_t *nnz) { return qh_mat_info(b->A, rows, nnz); }
+
+extern "C" int qh_batch_set_state(qh_batch *b, int i, const double *host) {
+  QH_REQUIRE(b && i >= 0 && i < b->n, QH_EINVAL, "qh_batch_set_state: bad instance");
+  // set through a temporary FP64 vector and copy into the slice
+  qh_vec *t = nullptr;
+  int rc = qh_vec_create(b->rows[i], QH_FP64, &t);
+  if (rc) return rc;
+  rc = qh_vec_set(t, host);
+  if (!rc) {
+    qh_vec slice;
+    slice.n = b->rows[i];
+    slice.prec = b->x->prec;
+    slice.d = (char *)b->x->d + b->off[i] * (b->x->prec == QH_FP64 ? 16 : 8);
+    slice.lo = b->x->lo ? b->x->lo + b->off[i] : nullptr;
+    rc = qh_vec_copy(t, &slice);
+    slice.d = nullptr;
+    slice.lo = nullptr;
+  }
+  QH_HIP(hipStreamSynchronize(qh_stream()));
+  qh_vec_destroy(t);
+  return rc;
+}
+
+extern "C" int qh_batch_get_state(qh_batch *b, int i, double *host) {
+  QH_REQUIRE(b && i >= 0 && i < b->n, QH_EINVAL, "qh_batch_get_state: bad instance");
+  qh_vec *t = nullptr;
+  int rc = qh_vec_create(b->rows[i], QH_FP64, &t);
+  if (rc) return rc;
+  qh_vec slice;
+  slice.n = b->rows[i];
+  slice.prec = b->x->prec;
+  slice.d = (char *)b->x->d + b->off[i] * (b->x->prec == QH_FP64 ? 16 : 8);
+  slice.lo = b->x->lo ? b->x->lo + b->off[i] : nullptr;
+  rc = qh_vec_copy(&slice, t);
+  slice.d = nullptr;
+  slice.lo = nullptr;
+  if (!rc) rc = qh_vec_get(t, host);
+  qh_vec_destroy(t);
+  return rc;
+}
+
+extern "C" int qh_batch_rk4(qh_batch *b, double dt, int64_t nsteps, int renorm_every) {
+  QH_REQUIRE(b, QH_EINVAL, "null batch");
+  QH_REQUIRE(renorm_every == 0 || b->x->prec == QH_FP32, QH_EINVAL, "batch renormalisation needs FP32");
+  if (renorm_every == 0) return rk4_impl(b->A, b->x, dt, nsteps, 0, 0, 0);
+  int64_t done = 0;
+  while (done < nsteps) {
+    int64_t chunk = std::min<int64_t>(renorm_every, nsteps - done);
+    int rc = rk4_impl(b->A, b->x, dt, chunk, 0, 0, 0);
+    if (rc) return rc;
+    if (chunk == renorm_every) {
+      hipLaunchKernelGGL(k_batch_trace, dim3(b->n), dim3(256), 0, qh_stream(), (const float2 *)b->x->d, b->x->lo,
+                         b->doff, b->dtot, b->inv);
+      hipLaunchKernelGGL(k_batch_scale, dim3(gridn(b->x->n)), dim3(256), 0, qh_stream(), (float2 *)b->x->d,
+                         b->x->lo, b->doff, b->inv, b->n, b->x->n);
+      QH_HIP(hipGetLastError());
+    }
+    done += chunk;
+  }
+  return QH_OK;
+}
+
+extern "C" qh_vec *qh_batch_state_vec(qh_batch *b) { return b ? b->x : nullptr; }
+extern "C" qh_mat *qh_batch_matrix(qh_batch *b) { return b ? b->A : nullptr; }
+extern "C" int64_t qh_batch_offset(qh_batch *b, int i) { return (b && i >= 0 && i < b->n) ? b->off[i] : -1; }

@@ -1,0 +1,164 @@
+"""Gates (both qubit conventions), observables, PETSc-binary state input, and the
+steady-state GMRES on the device, against the oracle and closed forms (1e-12
+max-abs for gates/observables in FP64)."""
+import math
+import os
+
+import numpy as np
+import pytest
+
+import dense
+import oracle
+import petsc_io
+import quac_amd as qa
+import recipes
+from qbuild import build_system
+
+pytestmark = pytest.mark.gpu
+FX = os.path.join(os.path.dirname(__file__), "golden", "pristine_matrices")
+RNG = np.random.default_rng(7)
+
+
+def _rand_rho(N):
+    a = RNG.normal(size=(N, N)) + 1j * RNG.normal(size=(N, N))
+    rho = a @ a.conj().T
+    return rho / np.trace(rho)
+
+
+GATECASES = [("H", 1, (0,), 0.0), ("H", 1, (3,), 0.0), ("RZ", 8, (1,), 0.7), ("RX", 6, (2,), 1.3),
+             ("SIGMAX", 2, (3,), 0.0), ("SIGMAY", 3, (0,), 0.0), ("CNOT", -1, (0, 1), 0.0),
+             ("CNOT", -1, (3, 1), 0.0), ("CZ", -3, (2, 0), 0.0), ("CXZ", -2, (1, 3), 0.0)]
+
+
+@pytest.mark.parametrize("name,code,qubits,theta", GATECASES)
+def test_gate_superop_convention_vs_oracle(name, code, qubits, theta):
+    nq = 4
+    r = {"subs": [("op", 2)] * nq, "terms": [("lin", 0.0, [(0, "n")])], "mcwf": False}
+    qs, _, _ = build_system(r, construct=False)
+    x = qs.create_qvec()
+    rho = _rand_rho(2 ** nq)
+    x.set(dense.vec(rho))
+    qs.apply_gate(x, name, list(qubits), theta=theta)
+    ref = oracle.gate_superop([2] * nq, [1, 2, 4, 8], True, code, qubits, dense.vec(rho), theta=theta)
+    assert np.abs(x.get() - ref).max() < 1e-12
+    qs.destroy()
+
+
+@pytest.mark.parametrize("name,code,qubits,theta", GATECASES)
+def test_gate_inplace_convention_vs_oracle(name, code, qubits, theta):
+    nq = 4
+    r = {"subs": [("op", 2)] * nq, "terms": [("lin", 0.0, [(0, "n")])], "mcwf": False}
+    qs, _, _ = build_system(r, construct=False)
+    x = qs.create_qvec()
+    rho = _rand_rho(2 ** nq)
+    x.set(dense.vec(rho))
+    x.apply_gate2(name, list(qubits), theta=theta)
+    ref = oracle.gate_inplace([2] * nq, True, code, qubits, dense.vec(rho), theta=theta)
+    assert np.abs(x.get() - ref).max() < 1e-12
+    qs.destroy()
+
+
+def test_gate_on_wavefunction_and_mixed_levels():
+    # qubit embedded between a 3-level system and a 2-level system (strides 2 and 1)
+    qs = qa.System()
+    a = qs.create_op(3)
+    b = qs.create_op(2)
+    c = qs.create_op(2)
+    qs.add_ham_term(1.0, a.n)
+    x = qs.create_wf()
+    psi = RNG.normal(size=12) + 1j * RNG.normal(size=12)
+    x.set(psi)
+    qs.apply_gate(x, "H", [1])
+    ref = oracle.gate_superop([3, 2, 2], [1, 3, 6], False, 1, (1,), psi)
+    assert np.abs(x.get() - ref).max() < 1e-12
+    qs.destroy()
+
+
+def test_apply_circuit_to_qvec_and_layers():
+    nq = 3
+    r = {"subs": [("op", 2)] * nq, "terms": [("lin", 0.0, [(0, "n")])], "mcwf": False}
+    qs, _, _ = build_system(r, construct=False)
+    x = qs.create_qvec()
+    rho = _rand_rho(8)
+    x.set(dense.vec(rho))
+    circ = qa.Circuit(10)
+    gates = [("H", 1, (0,), 0), ("CNOT", -1, (0, 1), 0), ("RZ", 8, (2,), 0.4), ("RX", 6, (1,), 2.0),
+             ("CNOT", -1, (2, 0), 0)]
+    for g in gates:
+        circ.add_gate(1.0, g[0], list(g[2]), theta=g[3])
+    qs.apply_circuit_to_qvec(circ, x)
+    ref = dense.vec(rho)
+    for g in gates:
+        ref = oracle.gate_superop([2] * nq, [1, 2, 4], True, g[1], g[2], ref, theta=g[3])
+    assert np.abs(x.get() - ref).max() < 1e-12
+    # greedy layering (quantum_circuits.c:316-404): H(0) | CNOT(0,1), RZ(2) | RX(1), CNOT(2,0)
+    qs.schedule(circ)
+    assert circ.num_layers == 3
+    assert circ.layer_times() == [1.0, 2.0, 3.0]
+    circ.destroy()
+    qs.destroy()
+
+
+def test_expectation_trace_populations_vs_oracle():
+    r = {"subs": [("op", 2), ("op", 3), ("vec", 3)], "terms": [("lin", 0.0, [(0, "n")])], "mcwf": False}
+    qs, subs, op = build_system(r, construct=False)
+    tot = 18
+    rho = _rand_rho(tot)
+    x = qs.create_qvec()
+    x.set(dense.vec(rho))
+    for refs in ([(0, "lower"), (1, "raise")], [(1, "n")], [(0, "sx"), (0, "sy")],
+                 [(2, ("vec", 1)), (2, ("vec", 2)), (1, "lower")]):
+        got = x.expectation(*[op(q) for q in refs])
+        _, terms = recipes.oracle_terms({"subs": r["subs"], "terms": [("ham", 1.0, refs)]})
+        assert abs(got - oracle.expect_dm(tot, dense.vec(rho), terms[0][2])) < 1e-12
+    assert abs(x.trace() - np.trace(rho)) < 1e-12
+    assert abs(x.element(3, 5) - rho[3, 5]) < 1e-15
+    assert np.abs(x.bitstring_probs() - np.real(np.diag(rho))).max() < 1e-15
+    qs.destroy()
+
+
+def test_read_dm_binary_fixture():
+    """qvec_utilities_tests.c:788-870 reads rand_dm.dat; device copy must equal the file."""
+    v = petsc_io.read_vec(os.path.join(FX, "rand_dm.dat"))
+    x = qa.read_qvec_dm_binary(os.path.join(FX, "rand_dm.dat"))
+    assert x.total_levels == 4 and x.is_dm
+    assert np.array_equal(x.get(), v)
+    assert abs(x.trace() - 1.0) < 1e-12
+    x.destroy()
+
+
+@pytest.mark.parametrize("omega,delta,gamma", [(1.0, 0.0, 0.5), (0.4, 0.3, 1.0)])
+def test_steady_state_two_level_analytic(omega, delta, gamma):
+    qs = qa.System()
+    o = qs.create_op(2)
+    qs.add_ham_term(omega / 2, o.sig_x)
+    qs.add_ham_term(delta, o.n)
+    qs.add_lin_term(gamma, o)
+    qs.construct_matrix()
+    x = qs.create_qvec()
+    for jac in (False, True):
+        x.set(np.zeros(4))
+        it, res = qs.steady_state(x, tol=1e-12, jacobi=jac)
+        rho = dense.unvec(x.get(), 2)
+        ree = (omega ** 2 / 4) / (delta ** 2 + gamma ** 2 / 4 + omega ** 2 / 2)
+        assert res < 1e-11 and abs(rho[1, 1].real - ree) < 1e-10 and abs(np.trace(rho) - 1) < 1e-10
+    qs.destroy()
+
+
+def test_steady_state_kerr_small_vs_oracle():
+    p = recipes.kerr_params(4, 1)[0]
+    r = recipes.kerr_recipe(p, levels=4)
+    tot, terms = recipes.oracle_terms(r)
+    m = oracle.construct(tot, terms, dm=True)
+    xo, ito, reso = m.gmres(tot, tol=1e-10, jacobi=True)
+    qs, subs, op = build_system(r)
+    x = qs.create_qvec()
+    it, res = qs.steady_state(x, tol=1e-10, jacobi=True)
+    assert res <= 1e-10
+    got = x.get()
+    assert np.abs(got - xo).max() < 1e-7
+    # the solution is a fixed point: A rho ~ 0, unit trace, Hermitian
+    assert np.abs(m.spmv(got)).max() < 1e-8
+    rho = dense.unvec(got, tot)
+    assert abs(np.trace(rho) - 1) < 1e-10 and np.abs(rho - rho.conj().T).max() < 1e-8
+    qs.destroy()

@@ -1,0 +1,207 @@
+"""Time stepping on the device against the CPU oracle and closed forms.
+
+Tolerances (written here, not hidden): the FP64 drop-in path must match the
+oracle's FP64 RK4 to 1e-10 in every population (and 1e-10 max-abs on the
+whole density matrix for the small cases); the FP32 bulk path must keep every
+population within 1e-6 of the FP64 path."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+import quac_amd as qa
+import recipes
+from qbuild import build_system, populations_np, subsystem_layout
+
+pytestmark = pytest.mark.gpu
+
+
+def _fixed_rk4():
+    qa.set_option("ts_rk_type", "4")
+    qa.set_option("ts_adapt_type", "none")
+
+
+def _adaptive():
+    qa.set_option("ts_rk_type", "3bs")
+    qa.set_option("ts_adapt_type", "basic")
+
+
+def _oracle_run(recipe, x0, dt, steps):
+    tot, terms = recipes.oracle_terms(recipe)
+    m = oracle.construct(tot, terms, dm=True)
+    return m.rk4(x0, dt, steps, nthreads=4)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_jc_rk4_vs_oracle(prec):
+    """Jaynes-Cummings 2x10 (400-row superoperator), dt = 0.01, 2000 steps."""
+    _fixed_rk4()
+    qa.set_option("quac_precision", prec)
+    try:
+        r = recipes.jc_recipe()
+        N, loc = recipes.jc_init()
+        qs, subs, op = build_system(r)
+        x = qs.create_qvec()
+        x.add_loc(1.0, loc)
+        steps, tend = qs.time_step(x, 0.0, 20.0, 0.01, 100000)
+        assert steps == 2000 and abs(tend - 20.0) < 1e-9
+        got = x.get()
+        x0 = np.zeros(N * N, np.complex128)
+        x0[loc] = 1
+        ref = _oracle_run(r, x0, 0.01, 2000)
+        pg, pr = populations_np(got, r), populations_np(ref, r)
+        if prec == "fp64":
+            assert np.abs(got - ref).max() < 1e-10
+            assert np.abs(pg - pr).max() < 1e-10
+        else:
+            assert np.abs(pg - pr).max() < 1e-6
+        # device observables agree with numpy on the same state
+        e = x.expectation(subs[1][1].n)
+        assert abs(e.real - pr[1]) < (1e-10 if prec == "fp64" else 1e-6)
+        qs.destroy()
+    finally:
+        qa.set_option("quac_precision", "fp64")
+
+
+def test_spin_chain6_fp64_and_fp32_vs_oracle():
+    _fixed_rk4()
+    r = recipes.spin_chain_recipe(6, seed=1)
+    N, loc = recipes.spin_chain_init(6)
+    x0 = np.zeros(N * N, np.complex128)
+    x0[loc] = 1
+    ref = _oracle_run(r, x0, 0.005, 400)
+    pr = populations_np(ref, r)
+    for prec, tol in (("fp64", 1e-10), ("fp32", 1e-6)):
+        qa.set_option("quac_precision", prec)
+        qs, subs, op = build_system(r)
+        x = qs.create_qvec()
+        x.add_loc(1.0, loc)
+        qs.time_step(x, 0.0, 2.0, 0.005, 100000)
+        pg = x.populations([L for L, _ in subsystem_layout(r)], [s for _, s in subsystem_layout(r)])
+        assert np.abs(pg - pr).max() < tol, (prec, np.abs(pg - pr).max())
+        assert abs(x.trace() - 1.0) < (1e-12 if prec == "fp64" else 1e-6)
+        qs.destroy()
+    qa.set_option("quac_precision", "fp64")
+
+
+def test_rk4_device_direct_matches_oracle_and_lanes():
+    """qh_rk4 on the assembled matrix for every lanes-per-row variant."""
+    r = recipes.spin_chain_recipe(5, seed=3)
+    N, loc = recipes.spin_chain_init(5)
+    qs, _, _ = build_system(r)
+    A = qs.matrix()
+    x0 = np.zeros(N * N, np.complex128)
+    x0[loc] = 1
+    ref = _oracle_run(r, x0, 0.01, 37)
+    for g in (1, 2, 4, 8, 16, 32, 64):
+        A.set_lanes(g, g)
+        v = qa.DeviceVec(N * N, qa.FP64)
+        v.set(x0)
+        qa.rk4_device(A, v, 0.01, 37)
+        assert np.abs(v.get() - ref).max() < 1e-12, g
+    A.set_lanes(0, 0)
+    qs.destroy()
+
+
+def test_spmv_matches_oracle():
+    r = recipes.kerr_recipe(recipes.kerr_params(4, 1)[0], levels=4)
+    tot, terms = recipes.oracle_terms(r)
+    m = oracle.construct(tot, terms, dm=True)
+    rng = np.random.default_rng(0)
+    x = rng.normal(size=tot * tot) + 1j * rng.normal(size=tot * tot)
+    qs, _, _ = build_system(r)
+    xv, yv = qa.DeviceVec(tot * tot), qa.DeviceVec(tot * tot)
+    xv.set(x)
+    qa._qh(qa.lib().qh_spmv(qs.matrix().h, xv.h, yv.h))
+    assert np.abs(yv.get() - m.spmv(x)).max() < 1e-12
+    qs.destroy()
+
+
+@pytest.mark.parametrize("vec", (False, True))
+def test_t1_decay_adaptive_default(vec):
+    """time_step_tests.c:24-101: T1 decay with the default adaptive RK3BS, <n>(1)=1/e +- 1e-4."""
+    _adaptive()
+    qs = qa.System()
+    if vec:
+        v = qs.create_vec_op(2)
+        qs.add_lin_term(1.0, v[0], v[1])
+    else:
+        o = qs.create_op(2)
+        qs.add_lin_term(1.0, o)
+    x = qs.create_qvec()
+    x.add(1.0, 1, 1)
+    qs.construct_matrix()
+    steps, tend = qs.time_step(x, 0.0, 1.0, 0.05, 200)
+    assert abs(tend - 1.0) < 1e-9
+    e = x.expectation(v[1], v[1]) if vec else x.expectation(o.n)
+    assert abs(e.real - math.exp(-1)) < 1e-4
+    qs.destroy()
+
+
+def test_cascade_and_circuit_decay():
+    """time_step_tests.c:337-396: SIGMAX on qubit 0 at t=0.1 hits the last-created
+    subsystem (in-place convention), qubit 1 at 0.2; <n> = e^-1 and e^-0.9 at t=1.1."""
+    _adaptive()
+    qs = qa.System()
+    o2 = qs.create_op(2)
+    o3 = qs.create_op(2)
+    qs.add_ham_term(1.0, o2.n)
+    qs.add_lin_term(1.0, o2)
+    qs.add_lin_term(1.0, o3)
+    x = qs.create_qvec()
+    x.add(1.0, 0, 0)
+    qs.construct_matrix()
+    c = qa.Circuit(5)
+    c.add_gate(0.0, "SIGMAX", [0])
+    c.add_gate(0.1, "SIGMAX", [1])
+    qs.apply_circuit(c, 0.1)
+    steps, tend = qs.time_step(x, 0.0, 1.1, 0.05, 200)
+    assert abs(x.expectation(o2.n).real - math.exp(-0.9)) < 5e-4
+    assert abs(x.expectation(o3.n).real - math.exp(-1.0)) < 5e-4
+    qs.destroy()
+
+
+def test_time_dependent_rabi_drive():
+    """time_dep_tests.c:46-64,142-213: H(t) = Omega(t)/2 sigma_x with
+    Omega(t) = Omega0 e^{-alpha t}: <sigma_z>(t) = cos(Omega0 (1 - e^{-alpha t})/alpha)."""
+    _fixed_rk4()
+    om, al = 2.0, 0.5
+    qs = qa.System()
+    o = qs.create_op(2)
+    qs.add_ham_term_time_dep(1.0, lambda t: om / 2 * math.exp(-al * t), o.sig_x)
+    qs.add_lin_term(0.0, o)
+    x = qs.create_qvec()
+    x.add(1.0, 0, 0)
+    qs.construct_matrix()
+    qs.time_step(x, 0.0, 2.0, 0.001, 100000)
+    want = math.cos(om * (1 - math.exp(-al * 2.0)) / al)
+    assert abs(x.expectation(o.sig_z).real - want) < 1e-6
+    qs.destroy()
+
+
+def test_monitor_called_every_step():
+    _fixed_rk4()
+    r = recipes.jc_recipe(levels=4)
+    qs, subs, op = build_system(r)
+    N = 8
+    x = qs.create_qvec()
+    x.add_loc(1.0, 4 * N + 4)
+    seen = []
+    qs.set_monitor(lambda step, t, v: seen.append((step, round(t, 9))))
+    qs.time_step(x, 0.0, 0.1, 0.01, 1000)
+    qs.set_monitor(None)
+    assert [s for s, _ in seen] == list(range(11))
+    assert abs(seen[-1][1] - 0.1) < 1e-9
+    qs.destroy()
+
+
+def test_steps_max_stops_early():
+    """'Did not reach final time' path: steps_max smaller than tmax/dt."""
+    _fixed_rk4()
+    qs, _, _ = build_system(recipes.jc_recipe(levels=3))
+    x = qs.create_qvec()
+    x.add_loc(1.0, 0)
+    steps, tend = qs.time_step(x, 0.0, 1.0, 0.01, 7)
+    assert steps == 7 and abs(tend - 0.07) < 1e-12
+    qs.destroy()
