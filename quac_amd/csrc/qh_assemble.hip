@@ -349,6 +349,10 @@ extern "C" int qh_mat_destroy(qh_mat *m) {
   qh_free_ptr(m->col);
   qh_free_ptr(m->val);
   qh_free_ptr(m->val32);
+  qh_free_ptr(m->ks_moff);
+  qh_free_ptr(m->ks_meta);
+  qh_free_ptr(m->ks_val);
+  qh_free_ptr(m->ks_val32);
   delete m;
   return QH_OK;
 }
@@ -402,6 +406,9 @@ extern "C" int qh_mat_make_fp32(qh_mat *m) {
   if (m->nnz)
     hipLaunchKernelGGL(k_narrow_vals, dim3(qh_blocks(m->nnz, 256)), dim3(256), 0, qh_stream(), m->val, m->val32,
                        m->nnz);
+  if (m->ks_val32 && m->nnz)  // refresh the sliced-layout copy too
+    hipLaunchKernelGGL(k_narrow_vals, dim3(qh_blocks(m->nnz, 256)), dim3(256), 0, qh_stream(), m->ks_val,
+                       m->ks_val32, m->nnz);
   QH_HIP(hipGetLastError());
   return QH_OK;
 }

@@ -33,15 +33,36 @@ int qh_ensure_init();
 int qh_num_cus();
 
 // ----------------------------------------------------------------- objects
+// Slot metadata of the slot-aligned sliced-ELL SpMV layout (one slice = 64
+// consecutive rows = one wavefront).  Every entry of the slice whose column is
+// row +- ad (and that is the rank-th such entry of its row) sits in one slot;
+// mask marks the lanes (rows) that have it, sign the lanes whose column is
+// row - ad, and base is the index of the slot's first value in the compacted
+// value array (values of active lanes are stored consecutively in lane order).
+struct qh_kmeta {
+  uint64_t mask, sign;
+  uint32_t ad, base;
+};
+
 struct qh_mat {
   int64_t rows = 0, nnz = 0;
   int32_t *rowptr = nullptr;   // rows+1
   int32_t *col = nullptr;      // nnz
   double2 *val = nullptr;      // nnz complex128
   float2 *val32 = nullptr;     // nnz complex64 (FP32 path), made on demand
-  int lanes64 = 0, lanes32 = 0;  // SpMV lanes per row (0: not tuned yet)
+  int lanes64 = 0, lanes32 = 0;  // SpMV variant: lanes per row of the CSR kernel, or QH_KSELL
   int32_t max_row = 0;         // longest row
+  // slot-aligned sliced ELL (built on demand from the CSR)
+  int ks_state = 0;            // 0 not built, 1 built, -1 not applicable (a row longer than 64)
+  int32_t nslices = 0;
+  int32_t *ks_moff = nullptr;  // nslices+1 slot offsets
+  qh_kmeta *ks_meta = nullptr;
+  double2 *ks_val = nullptr;   // nnz, slice/slot/lane order
+  float2 *ks_val32 = nullptr;
+  int64_t ks_slots = 0;
 };
+// lanes-per-row code selecting the sliced-ELL kernel
+#define QH_KSELL 128
 
 struct qh_vec {
   int64_t n = 0;

@@ -186,12 +186,275 @@ __global__ void __launch_bounds__(256) k_rk_stage(
   }
 }
 
+// ------------------------------------------------- slot-aligned sliced ELL
+// One wavefront owns a slice of 64 consecutive rows, one lane per row.  Slot k
+// of the slice holds, for every lane that has it, the entry at column row +- ad_k;
+// the per-slot metadata is wave-uniform (scalar loads), the values of the active
+// lanes are compacted (no padding, no column-index stream), and the x gathers of
+// a slot are contiguous across the wavefront.
+template <typename R, int MODE>
+__global__ void __launch_bounds__(256) k_ks_stage(
+    int32_t nrows, int32_t nslices, const int32_t *__restrict__ moff, const qh_kmeta *__restrict__ meta,
+    const typename C2<R>::T *__restrict__ val, const typename C2<R>::T *__restrict__ x,
+    const typename C2<R>::T *__restrict__ kin, R ck, typename C2<R>::T *__restrict__ kout,
+    const typename C2<R>::T *__restrict__ k1, const typename C2<R>::T *__restrict__ k2,
+    const typename C2<R>::T *__restrict__ k3, typename C2<R>::T *__restrict__ xout, R h,
+    float2 *__restrict__ xlo) {
+  typedef typename C2<R>::T T;
+  const unsigned b = qh_xcd_block(blockIdx.x, gridDim.x);
+  const int s = (int)(b * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6));
+  if (s >= nslices) return;  // whole wavefront
+  const int lane = threadIdx.x & 63;
+  const int32_t r = s * 64 + lane;
+  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  const int32_t k0 = moff[s], kend = moff[s + 1];
+  Acc<R> acc;
+  int32_t k = k0;
+  for (; k + 1 < kend; k += 2) {
+    const qh_kmeta m0 = meta[k], m1 = meta[k + 1];
+    const bool a0 = (m0.mask >> lane) & 1, a1 = (m1.mask >> lane) & 1;
+    T v0, v1, x0, x1;
+    if (a0) {
+      const int32_t c0 = ((m0.sign >> lane) & 1) ? r - (int32_t)m0.ad : r + (int32_t)m0.ad;
+      v0 = ld_stream<R>(val + m0.base + __popcll(m0.mask & lt));
+      x0 = x[c0];
+      if (MODE != MODE_SPMV) {
+        const T q = kin[c0];
+        x0.x += ck * q.x;
+        x0.y += ck * q.y;
+      }
+    }
+    if (a1) {
+      const int32_t c1 = ((m1.sign >> lane) & 1) ? r - (int32_t)m1.ad : r + (int32_t)m1.ad;
+      v1 = ld_stream<R>(val + m1.base + __popcll(m1.mask & lt));
+      x1 = x[c1];
+      if (MODE != MODE_SPMV) {
+        const T q = kin[c1];
+        x1.x += ck * q.x;
+        x1.y += ck * q.y;
+      }
+    }
+    if (a0) acc.add(v0.x * x0.x - v0.y * x0.y, v0.x * x0.y + v0.y * x0.x);
+    if (a1) acc.add(v1.x * x1.x - v1.y * x1.y, v1.x * x1.y + v1.y * x1.x);
+  }
+  if (k < kend) {
+    const qh_kmeta m0 = meta[k];
+    if ((m0.mask >> lane) & 1) {
+      const int32_t c0 = ((m0.sign >> lane) & 1) ? r - (int32_t)m0.ad : r + (int32_t)m0.ad;
+      const T v0 = ld_stream<R>(val + m0.base + __popcll(m0.mask & lt));
+      T x0 = x[c0];
+      if (MODE != MODE_SPMV) {
+        const T q = kin[c0];
+        x0.x += ck * q.x;
+        x0.y += ck * q.y;
+      }
+      acc.add(v0.x * x0.x - v0.y * x0.y, v0.x * x0.y + v0.y * x0.x);
+    }
+  }
+  if (r >= nrows) return;
+  const T y = acc.value();
+  if (MODE != MODE_FINAL) {
+    kout[r] = y;
+    return;
+  }
+  const T a = k1[r], bq = k2[r], cq = k3[r];
+  const R s6 = h * (R)(1.0 / 6.0), s3 = h * (R)(1.0 / 3.0);
+  const R dre = s6 * a.x + s3 * bq.x + s3 * cq.x + s6 * y.x;
+  const R dim = s6 * a.y + s3 * bq.y + s3 * cq.y + s6 * y.y;
+  const T xo = x[r];
+  if (sizeof(R) == 8) {
+    T o;
+    o.x = xo.x + dre;
+    o.y = xo.y + dim;
+    xout[r] = o;
+  } else {
+    float2 lo = xlo[r];
+    float sr, er, si, ei;
+    two_sum((float)xo.x, (float)dre, sr, er);
+    two_sum((float)xo.y, (float)dim, si, ei);
+    float lr = lo.x + er, li = lo.y + ei;
+    float hr = sr + lr, hi = si + li;
+    lr = lr - (hr - sr);
+    li = li - (hi - si);
+    T o;
+    o.x = hr;
+    o.y = hi;
+    xout[r] = o;
+    xlo[r] = make_float2(lr, li);
+  }
+}
+
+// Build of the sliced layout from the CSR: one 64-lane block per slice.
+// Each lane sorts its row's entries by key = (|col-row| << 1 | rank) where rank
+// separates the two entries row-ad and row+ad of one row; the slice's slots are
+// the union of the lanes' keys in increasing order (a wave-wide min merge).
+const int KS_MAXROW = 64;
+
+__device__ inline uint64_t wave_min_u64(uint64_t v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const uint64_t o = __shfl_xor(v, off, 64);
+    v = o < v ? o : v;
+  }
+  return v;
+}
+
+template <bool FILL>
+__global__ void __launch_bounds__(64) k_ks_build(int32_t nrows, const int32_t *__restrict__ rp,
+                                                 const int32_t *__restrict__ col, const double2 *__restrict__ val,
+                                                 int32_t *width, const int32_t *moff, qh_kmeta *meta,
+                                                 double2 *kval, int *overflow) {
+  __shared__ uint64_t keys[64][KS_MAXROW];
+  const int s = blockIdx.x, lane = threadIdx.x;
+  const int32_t r = s * 64 + lane;
+  int len = 0, beg = 0;
+  if (r < nrows) {
+    beg = rp[r];
+    len = rp[r + 1] - beg;
+    if (len > KS_MAXROW) {
+      overflow[0] = 1;
+      len = 0;
+    }
+  }
+  uint64_t *kk = keys[lane];
+  // key0 = ad << 1 | (d > 0), packed with the entry index in the low 6 bits
+  for (int e = 0; e < len; e++) {
+    const int32_t d = col[beg + e] - r;
+    const uint64_t ad = (uint64_t)(d < 0 ? -(int64_t)d : (int64_t)d);
+    const uint64_t key0 = (ad << 1) | (d > 0 ? 1u : 0u);
+    uint64_t v = (key0 << 6) | (uint64_t)e;
+    int p = e;
+    while (p > 0 && kk[p - 1] > v) {
+      kk[p] = kk[p - 1];
+      p--;
+    }
+    kk[p] = v;
+  }
+  // rank: the second entry with the same |d| gets rank 1
+  uint64_t prev_ad = ~0ull;
+  for (int e = 0; e < len; e++) {
+    const uint64_t ad = kk[e] >> 7;
+    const uint64_t rank = (ad == prev_ad) ? 1u : 0u;
+    kk[e] = (((ad << 1) | rank) << 6) | (kk[e] & 63u);
+    prev_ad = ad;
+  }
+  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  int p = 0, slot = 0;
+  int32_t base = (s * 64 < nrows) ? rp[s * 64] : 0;
+  int32_t mo = FILL ? moff[s] : 0;
+  for (;;) {
+    const uint64_t cur = p < len ? (kk[p] >> 6) : ~0ull;
+    const uint64_t m = wave_min_u64(cur);
+    if (m == ~0ull) break;
+    const bool act = (cur == m);
+    const uint64_t mask = __ballot(act);
+    if (FILL) {
+      const int e = act ? (int)(kk[p] & 63u) : 0;
+      const bool neg = act && (col[beg + e] < r);
+      const uint64_t sign = __ballot(neg);
+      if (lane == 0) {
+        qh_kmeta md;
+        md.mask = mask;
+        md.sign = sign;
+        md.ad = (uint32_t)(m >> 1);
+        md.base = (uint32_t)base;
+        meta[mo + slot] = md;
+      }
+      if (act) kval[base + __popcll(mask & lt)] = val[beg + e];
+    }
+    base += __popcll(mask);
+    if (act) p++;
+    slot++;
+  }
+  if (!FILL && lane == 0) width[s] = slot;
+}
+
+__global__ void k_narrow_ks(const double2 *a, float2 *b, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    b[i] = make_float2((float)a[i].x, (float)a[i].y);
+}
+
+int ks_build(qh_mat *A) {
+  if (A->ks_state) return QH_OK;
+  hipStream_t s = qh_stream();
+  const int32_t ns = (int32_t)((A->rows + 63) / 64);
+  int32_t *width = nullptr;
+  int *ovf = nullptr;
+  int rc = qh_alloc_t(&width, (size_t)ns + 1);
+  if (!rc) rc = qh_alloc_t(&ovf, 1);
+  if (!rc) rc = qh_alloc_t(&A->ks_moff, (size_t)ns + 1);
+  if (rc) {
+    qh_free_ptr(width);
+    qh_free_ptr(ovf);
+    return rc;
+  }
+  QH_HIP(hipMemsetAsync(ovf, 0, sizeof(int), s));
+  hipLaunchKernelGGL((k_ks_build<false>), dim3(ns), dim3(64), 0, s, (int32_t)A->rows, A->rowptr, A->col, A->val,
+                     width, nullptr, nullptr, nullptr, ovf);
+  QH_HIP(hipGetLastError());
+  int hovf = 0;
+  std::vector<int32_t> hw(ns);
+  QH_HIP(hipMemcpyAsync(&hovf, ovf, sizeof(int), hipMemcpyDeviceToHost, s));
+  QH_HIP(hipMemcpyAsync(hw.data(), width, sizeof(int32_t) * ns, hipMemcpyDeviceToHost, s));
+  QH_HIP(hipStreamSynchronize(s));
+  if (hovf) {
+    A->ks_state = -1;
+    qh_free_ptr(width);
+    qh_free_ptr(ovf);
+    qh_free_ptr(A->ks_moff);
+    A->ks_moff = nullptr;
+    return QH_OK;
+  }
+  std::vector<int32_t> ho(ns + 1);
+  int64_t tot = 0;
+  for (int32_t i = 0; i < ns; i++) {
+    ho[i] = (int32_t)tot;
+    tot += hw[i];
+  }
+  ho[ns] = (int32_t)tot;
+  QH_HIP(hipMemcpyAsync(A->ks_moff, ho.data(), sizeof(int32_t) * (ns + 1), hipMemcpyHostToDevice, s));
+  rc = qh_alloc_t(&A->ks_meta, (size_t)(tot > 0 ? tot : 1));
+  if (!rc) rc = qh_alloc_t(&A->ks_val, (size_t)(A->nnz > 0 ? A->nnz : 1));
+  if (rc) {
+    qh_free_ptr(width);
+    qh_free_ptr(ovf);
+    return rc;
+  }
+  hipLaunchKernelGGL((k_ks_build<true>), dim3(ns), dim3(64), 0, s, (int32_t)A->rows, A->rowptr, A->col, A->val,
+                     width, A->ks_moff, A->ks_meta, A->ks_val, ovf);
+  QH_HIP(hipGetLastError());
+  QH_HIP(hipStreamSynchronize(s));
+  qh_free_ptr(width);
+  qh_free_ptr(ovf);
+  A->nslices = ns;
+  A->ks_slots = tot;
+  A->ks_state = 1;
+  return QH_OK;
+}
+
+int ks_make_fp32(qh_mat *A) {
+  if (A->ks_state != 1 || A->ks_val32) return QH_OK;
+  int rc = qh_alloc_t(&A->ks_val32, (size_t)(A->nnz > 0 ? A->nnz : 1));
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_narrow_ks, dim3(qh_blocks(A->nnz, 256)), dim3(256), 0, qh_stream(), A->ks_val, A->ks_val32,
+                     A->nnz);
+  QH_HIP(hipGetLastError());
+  return QH_OK;
+}
+
 template <typename R, int MODE>
 int launch_stage_g(int G, hipStream_t s, const qh_mat *A, const typename C2<R>::T *val, const void *x,
                    const void *kin, R ck, void *kout, const void *k1, const void *k2, const void *k3, void *xout,
                    R h, float2 *xlo) {
   typedef typename C2<R>::T T;
   const int32_t n = (int32_t)A->rows;
+  if (G == QH_KSELL) {
+    const T *kv = sizeof(R) == 8 ? (const T *)A->ks_val : (const T *)A->ks_val32;
+    hipLaunchKernelGGL((k_ks_stage<R, MODE>), dim3(qh_blocks(A->nslices, 4)), dim3(256), 0, s, n, A->nslices,
+                       A->ks_moff, A->ks_meta, kv, (const T *)x, (const T *)kin, ck, (T *)kout, (const T *)k1,
+                       (const T *)k2, (const T *)k3, (T *)xout, h, xlo);
+    return QH_OK;
+  }
   const unsigned grid = qh_blocks((int64_t)n * G, 256);
 #define QH_L(GG)                                                                                                 \
   hipLaunchKernelGGL((k_rk_stage<R, GG, MODE>), dim3(grid), dim3(256), 0, s, n, A->rowptr, A->col, val,          \
@@ -234,6 +497,12 @@ int autotune(qh_mat *A, int *out_g) {
   QH_HIP(hipEventCreate(&e1));
   const double avg = (double)A->nnz / (double)(A->rows > 0 ? A->rows : 1);
   std::vector<int> cands;
+  {
+    int brc = ks_build(A);
+    if (!brc && A->ks_state == 1 && sizeof(R) == 4) brc = ks_make_fp32(A);
+    if (brc) return brc;
+    if (A->ks_state == 1) cands.push_back(QH_KSELL);
+  }
   for (int g : {2, 4, 8, 16, 32, 64})
     if (g <= 4 * avg + 4) cands.push_back(g);
   if (cands.empty()) cands.push_back(2);
@@ -269,10 +538,15 @@ int lanes_for(qh_mat *A, int prec, int *g) {
   int rc = QH_OK;
   if (prec == QH_FP64) {
     if (!A->lanes64) rc = autotune<double>(A, &A->lanes64);
+    if (!rc && A->lanes64 == QH_KSELL) rc = ks_build(A);
+    if (!rc && A->lanes64 == QH_KSELL && A->ks_state != 1) A->lanes64 = 8;
     *g = A->lanes64;
   } else {
     if (!A->val32) rc = qh_mat_make_fp32(A);
     if (!rc && !A->lanes32) rc = autotune<float>(A, &A->lanes32);
+    if (!rc && A->lanes32 == QH_KSELL) rc = ks_build(A);
+    if (!rc && A->lanes32 == QH_KSELL) rc = ks_make_fp32(A);
+    if (!rc && A->lanes32 == QH_KSELL && A->ks_state != 1) A->lanes32 = 8;
     *g = A->lanes32;
   }
   return rc;
@@ -540,8 +814,10 @@ extern "C" int qh_rk4_timed(qh_mat *A, qh_vec *x, double dt, int64_t nsteps, int
 
 extern "C" int qh_mat_set_lanes(qh_mat *m, int l64, int l32) {
   QH_REQUIRE(m, QH_EINVAL, "null matrix");
-  auto ok = [](int g) { return g == 0 || g == 1 || g == 2 || g == 4 || g == 8 || g == 16 || g == 32 || g == 64; };
-  QH_REQUIRE(ok(l64) && ok(l32), QH_EINVAL, "lanes per row must be a power of two <= 64");
+  auto ok = [](int g) {
+    return g == 0 || g == 1 || g == 2 || g == 4 || g == 8 || g == 16 || g == 32 || g == 64 || g == QH_KSELL;
+  };
+  QH_REQUIRE(ok(l64) && ok(l32), QH_EINVAL, "lanes per row must be a power of two <= 64, or QH_KSELL (128)");
   m->lanes64 = l64;
   m->lanes32 = l32;
   if (g_st.A == m) g_st.gG = -1;  // force a re-capture

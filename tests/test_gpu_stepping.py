@@ -94,14 +94,54 @@ def test_rk4_device_direct_matches_oracle_and_lanes():
     x0 = np.zeros(N * N, np.complex128)
     x0[loc] = 1
     ref = _oracle_run(r, x0, 0.01, 37)
-    for g in (1, 2, 4, 8, 16, 32, 64):
+    for g in (1, 2, 4, 8, 16, 32, 64, 128):  # 128 = slot-aligned sliced ELL
         A.set_lanes(g, g)
         v = qa.DeviceVec(N * N, qa.FP64)
         v.set(x0)
         qa.rk4_device(A, v, 0.01, 37)
         assert np.abs(v.get() - ref).max() < 1e-12, g
+        w = qa.DeviceVec(N * N, qa.FP32)
+        w.set(x0)
+        qa.rk4_device(A, w, 0.01, 37)
+        assert np.abs(w.get() - ref).max() < 1e-6, g
     A.set_lanes(0, 0)
     qs.destroy()
+
+
+@pytest.mark.parametrize("name", ["chain10", "tc_batch"])
+def test_sliced_ell_spmv_matches_csr(name):
+    """the sliced layout (slot union, compacted values, sign masks) against the CSR kernel"""
+    rng = np.random.default_rng(5)
+    if name == "chain10":
+        qs, _, _ = build_system(recipes.spin_chain_recipe(10, seed=1))
+        A, keep = qs.matrix(), [qs]
+    else:
+        keep, mats, tots = [], [], []
+        for p in recipes.tc_instances(40, 3):
+            q2, _, _ = build_system(recipes.tc_recipe(*p))
+            keep.append(q2)
+            mats.append(q2.matrix())
+            tots.append(recipes.tc_init(p[0], p[1])[0])
+        b = qa.Batch(mats, tots, prec=qa.FP64)
+        A = b.matrix()
+        keep.append(b)
+    n = A.info()[0]
+    x = rng.normal(size=n) + 1j * rng.normal(size=n)
+    xv = qa.DeviceVec(n)
+    xv.set(x)
+    outs = []
+    for g in (8, 128):
+        A.set_lanes(g, g)
+        yv = qa.DeviceVec(n)
+        qa._qh(qa.lib().qh_spmv(A.h, xv.h, yv.h))
+        outs.append(yv.get())
+    assert np.abs(outs[0] - outs[1]).max() <= 1e-12 * np.abs(outs[0]).max()
+    rp, col, val = A.export()
+    ref = np.zeros(n, np.complex128)
+    np.add.at(ref, np.repeat(np.arange(n), np.diff(rp)), val * x[col])
+    assert np.abs(outs[1] - ref).max() <= 1e-12 * np.abs(ref).max()
+    for k in keep:
+        k.destroy()
 
 
 def test_spmv_matches_oracle():
