@@ -58,6 +58,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--renorm", type=int, default=100)
+    ap.add_argument("--workload", default="chain10", choices=["chain10", "tc512", "circuit12", "jc"])
+    ap.add_argument("--instances", type=int, default=512)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on exactly one GPU: this build targets a single MI355X "
@@ -67,6 +69,12 @@ def main():
     import quac_amd as qa
     import recipes
     from qbuild import build_system, subsystem_layout
+
+    if args.workload != "chain10":
+        import bench_workloads
+        out = getattr(bench_workloads, args.workload)(args)
+        print(json.dumps(out), flush=True)
+        return
 
     qa.set_option("ts_rk_type", "4")
     qa.set_option("ts_adapt_type", "none")
@@ -125,6 +133,11 @@ def main():
     spmv_ms32 = qa.bench_spmv(A, qa.FP32, 30)
     spmv_ms64 = qa.bench_spmv(A, qa.FP64, 30)
     lanes = A.lanes()
+    layout = A.layout_info()
+    # bytes the sliced-layout step actually has to move: 4 x (stored values + slot
+    # metadata) + 17 vector passes (4 gathers, 3 K and 3 Y writes, 3 x reads for
+    # the Y's, K1..K3 + x reads and the x' write in the last epilogue)
+    ks_step32 = 4 * (layout["stored_vals"] * 8 + layout["meta_bytes"]) + 17 * rows * 8
 
     b_step32 = rk4_step_bytes(nnz, rows, 8)
     b_step64 = rk4_step_bytes(nnz, rows, 16)
@@ -188,6 +201,8 @@ def main():
                  "fp64_gbs": round(spmv_gbs64, 1), "fp64_frac": round(spmv_gbs64 / HBM_PEAK_GBS, 4),
                  "fp32_ms": round(spmv_ms32, 5), "fp64_ms": round(spmv_ms64, 5), "lanes_per_row": lanes},
         "assembly_s": round(assemble_s, 4),
+        "sliced_layout": dict(layout, step_bytes_fp32=ks_step32,
+                              step_gbs_fp32=round(ks_step32 / (el32 / K) / 1e9, 1)),
     }
     print(json.dumps(out), flush=True)
     qs.destroy()

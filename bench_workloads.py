@@ -1,0 +1,229 @@
+"""Secondary workloads of bench.py (selected with --workload); each returns the
+dict that bench.py prints as its JSON line.
+
+tc512     : batched parameter sweep of Tavis-Cummings instances of uneven size
+            (rows 100 .. 230 400), FP32 batch (one block-diagonal sliced matrix,
+            fused stages, hipGraph) vs the plain FP64 path run one instance at a
+            time; instance-steps/s and max population error.
+circuit12 : 12-qubit noisy circuit on rho (4096 x 4096): 300 seeded gates in
+            greedy layers, RK4 (dt = 0.01, 10 steps) between layers; FP32 bulk vs
+            FP64, per-qubit populations and trace.
+jc        : Jaynes-Cummings 2 x 10 (400-row superoperator), RK4 dt = 0.01 to
+            t = 100 with populations every 100 steps (latency-bound).
+"""
+import ctypes
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+import quac_amd as qa  # noqa: E402
+import recipes  # noqa: E402
+from qbuild import build_system, subsystem_layout  # noqa: E402
+
+
+def _pops(vec, subs):
+    """FP64 populations of a device DM vector (copied to FP64 first if FP32)"""
+    lv = np.array([L for L, _ in subs], np.int64)
+    st = np.array([s for _, s in subs], np.int64)
+    out = np.zeros(len(subs))
+    src = vec
+    if vec.prec != qa.FP64:
+        src = qa.DeviceVec(vec.n, qa.FP64)
+        qa._qh(qa.lib().qh_vec_copy(vec.h, src.h))
+    qa._qh(qa.lib().qh_populations(src.h, len(subs), lv.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                   st.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), 1,
+                                   out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+    return out
+
+
+def _trace(vec, N):
+    src = vec
+    if vec.prec != qa.FP64:
+        src = qa.DeviceVec(vec.n, qa.FP64)
+        qa._qh(qa.lib().qh_vec_copy(vec.h, src.h))
+    re, im = ctypes.c_double(), ctypes.c_double()
+    qa._qh(qa.lib().qh_trace(src.h, N, ctypes.byref(re), ctypes.byref(im)))
+    return complex(re.value, im.value)
+
+
+def tc512(args):
+    n_inst = args.instances
+    K = args.steps if args.steps != 2000 else 1000
+    W = min(args.warmup, 10)
+    dt = 0.01
+    insts = recipes.tc_instances(n_inst, 2)
+    t0 = time.perf_counter()
+    systems, mats, tots, subs, x0s = [], [], [], [], []
+    for p in insts:
+        r = recipes.tc_recipe(*p)
+        qs, _, _ = build_system(r)
+        N, loc = recipes.tc_init(p[0], p[1])
+        x0 = np.zeros(N * N, np.complex128)
+        x0[loc] = 1.0
+        systems.append(qs)
+        mats.append(qs.matrix())
+        tots.append(N)
+        subs.append(subsystem_layout(r))
+        x0s.append(x0)
+    qa.synchronize()
+    asm_s = time.perf_counter() - t0
+    rows = sum(t * t for t in tots)
+    nnz = sum(m.info()[1] for m in mats)
+
+    # batched FP32: warmup W steps, then K timed steps
+    b = qa.Batch(mats, tots, prec=qa.FP32)
+    for i, x0 in enumerate(x0s):
+        b.set_state(i, x0)
+    b.rk4(dt, W, renorm_every=0)
+    qa.synchronize()
+    t = time.perf_counter()
+    b.rk4(dt, K, renorm_every=100)
+    qa.synchronize()
+    el32 = time.perf_counter() - t
+    pops32 = b.populations(subs)
+    bl = b.matrix().lanes()
+
+    # plain FP64 path, one instance at a time (same W + K steps; K timed)
+    el64 = 0.0
+    worst = 0.0
+    for i, m in enumerate(mats):
+        v = qa.DeviceVec(tots[i] ** 2, qa.FP64)
+        v.set(x0s[i])
+        qa.rk4_device(m, v, dt, W)
+        qa.synchronize()
+        t = time.perf_counter()
+        qa.rk4_device(m, v, dt, K)
+        qa.synchronize()
+        el64 += time.perf_counter() - t
+        worst = max(worst, float(np.abs(_pops(v, subs[i]) - pops32[i]).max()))
+    b.destroy()
+    for qs in systems:
+        qs.destroy()
+    v32 = n_inst * K / el32
+    v64 = n_inst * K / el64
+    return {"metric": "batched Lindblad RK4 steps/s at 1e-6 pop err vs plain FP64; SpMV % HBM peak",
+            "value": round(v32, 2), "unit": "instance RK4 steps/s", "n_gpus": 1, "steps": K, "warmup": W,
+            "ms_per_step": round(1e3 * el32 / K, 5), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "fp32 batch vs fp64 one-at-a-time", "data": "synthetic",
+            "config": {"workload": f"{n_inst} Tavis-Cummings instances (seed 2), rows {min(t*t for t in tots)}.."
+                                   f"{max(t*t for t in tots)}, total {rows} rows {nnz} nnz, RK4 dt={dt}",
+                       "batch_lanes": bl},
+            "fp64_one_at_a_time_steps_per_s": round(v64, 2), "batched_over_plain": round(v32 / v64, 3),
+            "max_pop_err_fp32_batch_vs_fp64": worst, "assembly_s": round(asm_s, 3)}
+
+
+def _gate_u(name, theta):
+    s = 2 ** -0.5
+    if name == "H":
+        return np.array([[s, s], [s, -s]], np.complex128)
+    if name == "RX":
+        return np.array([[math.cos(theta / 2), -1j * math.sin(theta / 2)],
+                         [-1j * math.sin(theta / 2), math.cos(theta / 2)]])
+    if name == "RZ":
+        return np.diag([1.0, np.exp(1j * theta)])
+    if name == "CNOT":
+        return np.array([[1, 0, 0, 0], [0, 1, 0, 0], [0, 0, 0, 1], [0, 0, 1, 0]], np.complex128)
+    raise ValueError(name)
+
+
+def _layers(gates, nq):
+    """greedy ASAP layering (quantum_circuits.c:316-404 semantics)"""
+    cur = [0] * nq
+    layers = []
+    for g in gates:
+        lv = max(cur[q] for q in g[1])
+        while len(layers) <= lv:
+            layers.append([])
+        layers[lv].append(g)
+        for q in g[1]:
+            cur[q] = lv + 1
+    return layers
+
+
+def circuit12(args):
+    nq = 12
+    N = 2 ** nq
+    recipe = recipes.noisy_circuit_recipe(nq)
+    qs, _, _ = build_system(recipe)
+    A = qs.matrix()
+    gates = recipes.random_circuit(nq, 300, 3)
+    layers = _layers(gates, nq)
+    subs = subsystem_layout(recipe)
+    x0 = np.zeros(N * N, np.complex128)
+    x0[0] = 1.0
+    strides = [N // (2 ** (q + 1)) for q in range(nq)]  # qubit q = subsystem q (MSB first)
+    res = {}
+    for prec in (qa.FP64, qa.FP32):
+        v = qa.DeviceVec(N * N, prec)
+        v.set(x0)
+        qa.rk4_device(A, v, 0.01, 1)  # autotune + graph capture outside the timed region
+        v.set(x0)
+        qa.synchronize()
+        t = time.perf_counter()
+        for layer in layers:
+            for (name, qb, th) in layer:
+                u = _gate_u(name, th)
+                st = np.array([strides[q] for q in qb], np.int64)
+                uu = np.ascontiguousarray(u.reshape(-1)).view(np.float64)
+                qa._qh(qa.lib().qh_gate(v.h, N, 1, len(qb), st.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                        uu.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+            qa.rk4_device(A, v, 0.01, 10, tot=N, renorm_every=10 if prec == qa.FP32 else 0)
+        qa.synchronize()
+        el = time.perf_counter() - t
+        res[prec] = (el, _pops(v, subs), _trace(v, N))
+    qs.destroy()
+    el64, p64, t64 = res[qa.FP64]
+    el32, p32, t32 = res[qa.FP32]
+    steps = 10 * len(layers)
+    return {"metric": "batched Lindblad RK4 steps/s at 1e-6 pop err vs plain FP64; SpMV % HBM peak",
+            "value": round(steps / el32, 2), "unit": "RK4 steps/s (with gates)", "n_gpus": 1, "steps": steps,
+            "warmup": 1, "ms_per_step": round(1e3 * el32 / steps, 5), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "fp32 bulk vs fp64", "data": "synthetic",
+            "config": {"workload": f"12-qubit noisy circuit, rho 4096x4096, {len(gates)} gates (seed 3) in "
+                                   f"{len(layers)} layers, RK4 10 x 0.01 between layers, T1 1e-3, dephasing 2e-3"},
+            "fp64_steps_per_s": round(steps / el64, 2), "fp32_over_fp64": round(el64 / el32, 3),
+            "wall_s_fp32": round(el32, 4), "wall_s_fp64": round(el64, 4),
+            "max_pop_err_fp32_vs_fp64": float(np.abs(p32 - p64).max()),
+            "trace_fp32": abs(t32), "trace_fp64": abs(t64), "populations_fp64": [round(x, 10) for x in p64]}
+
+
+def jc(args):
+    recipe = recipes.jc_recipe()
+    N, loc = recipes.jc_init()
+    qs, _, _ = build_system(recipe)
+    A = qs.matrix()
+    subs = subsystem_layout(recipe)
+    x0 = np.zeros(N * N, np.complex128)
+    x0[loc] = 1.0
+    out = {}
+    for prec in (qa.FP64, qa.FP32):
+        v = qa.DeviceVec(N * N, prec)
+        v.set(x0)
+        qa.rk4_device(A, v, 0.01, 2)
+        v.set(x0)
+        qa.synchronize()
+        t = time.perf_counter()
+        trace_pts = []
+        for chunk in range(100):   # t = 100: 10 000 steps, populations every 100 steps
+            qa.rk4_device(A, v, 0.01, 100)
+            trace_pts.append(_pops(v, subs))
+        qa.synchronize()
+        out[prec] = (time.perf_counter() - t, np.array(trace_pts))
+    qs.destroy()
+    el64, p64 = out[qa.FP64]
+    el32, p32 = out[qa.FP32]
+    return {"metric": "batched Lindblad RK4 steps/s at 1e-6 pop err vs plain FP64; SpMV % HBM peak",
+            "value": round(10000 / el64, 2), "unit": "RK4 steps/s (FP64, incl. 100 population readbacks)",
+            "n_gpus": 1, "steps": 10000, "warmup": 2, "ms_per_step": round(1e3 * el64 / 10000, 5),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp64",
+            "data": "synthetic",
+            "config": {"workload": "Jaynes-Cummings 2x10 (400-row superoperator), RK4 dt=0.01 to t=100"},
+            "fp32_steps_per_s": round(10000 / el32, 2),
+            "max_pop_err_fp32_vs_fp64": float(np.abs(p32 - p64).max()),
+            "final_populations_fp64": [float(x) for x in p64[-1]]}

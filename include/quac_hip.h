@@ -65,9 +65,13 @@ int qh_mat_export(const qh_mat *m, int64_t *rowptr, int64_t *col, double *val);
 int qh_mat_make_fp32(qh_mat *m);
 /* block-diagonal concatenation of n matrices (batched parameter sweeps) */
 int qh_mat_blockdiag(int n, qh_mat *const *mats, qh_mat **out);
-/* select SpMV variant: lanes per row G in {2,4,8,16,32,64}; 0 = autotune */
+/* select SpMV variant: lanes per row G in {1,2,4,8,16,32,64} of the CSR kernel,
+ * 128 = slot-aligned sliced ELL (one wavefront per 64 rows); 0 = autotune */
 int qh_mat_set_lanes(qh_mat *m, int lanes64, int lanes32);
 int qh_mat_get_lanes(const qh_mat *m, int *lanes64, int *lanes32);
+/* sliced layout statistics (built on demand): slices, slots, stored values
+ * (values of non-uniform slots), and the metadata bytes */
+int qh_mat_layout_info(qh_mat *m, int64_t *slices, int64_t *slots, int64_t *stored_vals, int64_t *meta_bytes);
 
 /* ------------------------------------------------------------------ vectors */
 typedef struct qh_vec qh_vec;

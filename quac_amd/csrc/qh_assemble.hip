@@ -406,9 +406,9 @@ extern "C" int qh_mat_make_fp32(qh_mat *m) {
   if (m->nnz)
     hipLaunchKernelGGL(k_narrow_vals, dim3(qh_blocks(m->nnz, 256)), dim3(256), 0, qh_stream(), m->val, m->val32,
                        m->nnz);
-  if (m->ks_val32 && m->nnz)  // refresh the sliced-layout copy too
-    hipLaunchKernelGGL(k_narrow_vals, dim3(qh_blocks(m->nnz, 256)), dim3(256), 0, qh_stream(), m->ks_val,
-                       m->ks_val32, m->nnz);
+  if (m->ks_val32 && m->ks_nval)  // refresh the sliced-layout copy too
+    hipLaunchKernelGGL(k_narrow_vals, dim3(qh_blocks(m->ks_nval, 256)), dim3(256), 0, qh_stream(), m->ks_val,
+                       m->ks_val32, m->ks_nval);
   QH_HIP(hipGetLastError());
   return QH_OK;
 }

@@ -39,10 +39,15 @@ int qh_num_cus();
 // mask marks the lanes (rows) that have it, sign the lanes whose column is
 // row - ad, and base is the index of the slot's first value in the compacted
 // value array (values of active lanes are stored consecutively in lane order).
+// When every active lane of the slot holds the bit-identical value, the value
+// is kept in the metadata (vre, vim) and base is QH_KS_UNIFORM: no per-lane
+// value is stored or loaded for that slot.
 struct qh_kmeta {
   uint64_t mask, sign;
   uint32_t ad, base;
+  double vre, vim;
 };
+#define QH_KS_UNIFORM 0xFFFFFFFFu
 
 struct qh_mat {
   int64_t rows = 0, nnz = 0;
@@ -60,9 +65,13 @@ struct qh_mat {
   double2 *ks_val = nullptr;   // nnz, slice/slot/lane order
   float2 *ks_val32 = nullptr;
   int64_t ks_slots = 0;
+  int64_t ks_nval = 0;         // stored (non-uniform) values
 };
-// lanes-per-row code selecting the sliced-ELL kernel
+// variant codes selecting the sliced-ELL kernel: 128 + log2(slot unroll) - 1,
+// i.e. 128 / 129 / 130 process 2 / 4 / 8 slots per pass
 #define QH_KSELL 128
+#define QH_IS_KSELL(g) ((g) >= 128 && (g) <= 130)
+#define QH_KSELL_UNROLL(g) (2 << ((g) - 128))
 
 struct qh_vec {
   int64_t n = 0;

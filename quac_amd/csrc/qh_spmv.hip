@@ -192,14 +192,36 @@ __global__ void __launch_bounds__(256) k_rk_stage(
 // the per-slot metadata is wave-uniform (scalar loads), the values of the active
 // lanes are compacted (no padding, no column-index stream), and the x gathers of
 // a slot are contiguous across the wavefront.
-template <typename R, int MODE>
+// value of an active lane in a slot: from the wave-uniform metadata when the
+// slot is uniform, else from the compacted per-lane value stream
+template <typename R>
+__device__ inline typename C2<R>::T ks_value(const qh_kmeta &m, const typename C2<R>::T *val, uint64_t lt) {
+  typename C2<R>::T a;
+  if (m.base == QH_KS_UNIFORM) {
+    a.x = (R)m.vre;
+    a.y = (R)m.vim;
+    return a;
+  }
+  return ld_stream<R>(val + m.base + __popcll(m.mask & lt));
+}
+
+// Stage kernel on the sliced layout.  The stage input Y_s = x + c_s K_{s-1} is
+// materialised by the previous stage's epilogue, so every stage gathers one
+// vector:
+//   MODE_SPMV : kout = A xin
+//   MODE_STAGE: kout = A xin;  ynext = x + cn * kout          (stages 1-3)
+//   MODE_FINAL: xout = x + h/6 (K1 + 2 K2 + 2 K3 + A xin)    (stage 4)
+// U slots are processed per pass (all metadata loads, then all value loads and
+// gathers, then the FMAs); U trades memory-level parallelism against the SGPRs
+// the metadata occupies (occupancy), and is chosen by the autotuner.
+template <typename R, int MODE, int U>
 __global__ void __launch_bounds__(256) k_ks_stage(
     int32_t nrows, int32_t nslices, const int32_t *__restrict__ moff, const qh_kmeta *__restrict__ meta,
-    const typename C2<R>::T *__restrict__ val, const typename C2<R>::T *__restrict__ x,
-    const typename C2<R>::T *__restrict__ kin, R ck, typename C2<R>::T *__restrict__ kout,
-    const typename C2<R>::T *__restrict__ k1, const typename C2<R>::T *__restrict__ k2,
-    const typename C2<R>::T *__restrict__ k3, typename C2<R>::T *__restrict__ xout, R h,
-    float2 *__restrict__ xlo) {
+    const typename C2<R>::T *__restrict__ val, const typename C2<R>::T *__restrict__ xin,
+    typename C2<R>::T *__restrict__ kout, typename C2<R>::T *__restrict__ ynext, R cn,
+    const typename C2<R>::T *__restrict__ x, const typename C2<R>::T *__restrict__ k1,
+    const typename C2<R>::T *__restrict__ k2, const typename C2<R>::T *__restrict__ k3,
+    typename C2<R>::T *__restrict__ xout, R h, float2 *__restrict__ xlo) {
   typedef typename C2<R>::T T;
   const unsigned b = qh_xcd_block(blockIdx.x, gridDim.x);
   const int s = (int)(b * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6));
@@ -209,59 +231,51 @@ __global__ void __launch_bounds__(256) k_ks_stage(
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   const int32_t k0 = moff[s], kend = moff[s + 1];
   Acc<R> acc;
-  int32_t k = k0;
-  for (; k + 1 < kend; k += 2) {
-    const qh_kmeta m0 = meta[k], m1 = meta[k + 1];
-    const bool a0 = (m0.mask >> lane) & 1, a1 = (m1.mask >> lane) & 1;
-    T v0, v1, x0, x1;
-    if (a0) {
-      const int32_t c0 = ((m0.sign >> lane) & 1) ? r - (int32_t)m0.ad : r + (int32_t)m0.ad;
-      v0 = ld_stream<R>(val + m0.base + __popcll(m0.mask & lt));
-      x0 = x[c0];
-      if (MODE != MODE_SPMV) {
-        const T q = kin[c0];
-        x0.x += ck * q.x;
-        x0.y += ck * q.y;
+  for (int32_t k = k0; k < kend; k += U) {
+    qh_kmeta m[U];
+#pragma unroll
+    for (int j = 0; j < U; j++)
+      if (k + j < kend) m[j] = meta[k + j];
+    bool act[U];
+    T v[U], xv[U];
+#pragma unroll
+    for (int j = 0; j < U; j++) {
+      act[j] = (k + j < kend) && ((m[j].mask >> lane) & 1);
+      if (act[j]) {
+        const int32_t c = ((m[j].sign >> lane) & 1) ? r - (int32_t)m[j].ad : r + (int32_t)m[j].ad;
+        v[j] = ks_value<R>(m[j], val, lt);
+        xv[j] = xin[c];
       }
     }
-    if (a1) {
-      const int32_t c1 = ((m1.sign >> lane) & 1) ? r - (int32_t)m1.ad : r + (int32_t)m1.ad;
-      v1 = ld_stream<R>(val + m1.base + __popcll(m1.mask & lt));
-      x1 = x[c1];
-      if (MODE != MODE_SPMV) {
-        const T q = kin[c1];
-        x1.x += ck * q.x;
-        x1.y += ck * q.y;
-      }
-    }
-    if (a0) acc.add(v0.x * x0.x - v0.y * x0.y, v0.x * x0.y + v0.y * x0.x);
-    if (a1) acc.add(v1.x * x1.x - v1.y * x1.y, v1.x * x1.y + v1.y * x1.x);
-  }
-  if (k < kend) {
-    const qh_kmeta m0 = meta[k];
-    if ((m0.mask >> lane) & 1) {
-      const int32_t c0 = ((m0.sign >> lane) & 1) ? r - (int32_t)m0.ad : r + (int32_t)m0.ad;
-      const T v0 = ld_stream<R>(val + m0.base + __popcll(m0.mask & lt));
-      T x0 = x[c0];
-      if (MODE != MODE_SPMV) {
-        const T q = kin[c0];
-        x0.x += ck * q.x;
-        x0.y += ck * q.y;
-      }
-      acc.add(v0.x * x0.x - v0.y * x0.y, v0.x * x0.y + v0.y * x0.x);
-    }
+#pragma unroll
+    for (int j = 0; j < U; j++)
+      if (act[j]) acc.add(v[j].x * xv[j].x - v[j].y * xv[j].y, v[j].x * xv[j].y + v[j].y * xv[j].x);
   }
   if (r >= nrows) return;
   const T y = acc.value();
-  if (MODE != MODE_FINAL) {
+  if (MODE == MODE_SPMV) {
     kout[r] = y;
+    return;
+  }
+  const T xo = x[r];
+  if (MODE == MODE_STAGE) {
+    kout[r] = y;
+    T yn;
+    if (sizeof(R) == 8) {
+      yn.x = xo.x + cn * y.x;
+      yn.y = xo.y + cn * y.y;
+    } else {
+      const float2 lo = xlo[r];
+      yn.x = xo.x + (lo.x + cn * y.x);
+      yn.y = xo.y + (lo.y + cn * y.y);
+    }
+    ynext[r] = yn;
     return;
   }
   const T a = k1[r], bq = k2[r], cq = k3[r];
   const R s6 = h * (R)(1.0 / 6.0), s3 = h * (R)(1.0 / 3.0);
   const R dre = s6 * a.x + s3 * bq.x + s3 * cq.x + s6 * y.x;
   const R dim = s6 * a.y + s3 * bq.y + s3 * cq.y + s6 * y.y;
-  const T xo = x[r];
   if (sizeof(R) == 8) {
     T o;
     o.x = xo.x + dre;
@@ -302,8 +316,9 @@ __device__ inline uint64_t wave_min_u64(uint64_t v) {
 template <bool FILL>
 __global__ void __launch_bounds__(64) k_ks_build(int32_t nrows, const int32_t *__restrict__ rp,
                                                  const int32_t *__restrict__ col, const double2 *__restrict__ val,
-                                                 int32_t *width, const int32_t *moff, qh_kmeta *meta,
-                                                 double2 *kval, int *overflow) {
+                                                 int32_t *width, int32_t *nvals, const int32_t *moff,
+                                                 const int32_t *voff, qh_kmeta *meta, double2 *kval,
+                                                 int *overflow) {
   __shared__ uint64_t keys[64][KS_MAXROW];
   const int s = blockIdx.x, lane = threadIdx.x;
   const int32_t r = s * 64 + lane;
@@ -340,7 +355,7 @@ __global__ void __launch_bounds__(64) k_ks_build(int32_t nrows, const int32_t *_
   }
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   int p = 0, slot = 0;
-  int32_t base = (s * 64 < nrows) ? rp[s * 64] : 0;
+  int32_t base = FILL ? voff[s] : 0;  // compacted non-uniform values of this slice start here
   int32_t mo = FILL ? moff[s] : 0;
   for (;;) {
     const uint64_t cur = p < len ? (kk[p] >> 6) : ~0ull;
@@ -348,8 +363,15 @@ __global__ void __launch_bounds__(64) k_ks_build(int32_t nrows, const int32_t *_
     if (m == ~0ull) break;
     const bool act = (cur == m);
     const uint64_t mask = __ballot(act);
+    const int e = act ? (int)(kk[p] & 63u) : 0;
+    const double2 v = act ? val[beg + e] : make_double2(0.0, 0.0);
+    // uniform slot: every active lane holds the bit-identical value of the first one
+    const int first = __ffsll((unsigned long long)mask) - 1;
+    const double fre = __shfl(v.x, first, 64), fim = __shfl(v.y, first, 64);
+    const bool same = !act || (__double_as_longlong(v.x) == __double_as_longlong(fre) &&
+                               __double_as_longlong(v.y) == __double_as_longlong(fim));
+    const bool uni = __ballot(same) == ~0ull;
     if (FILL) {
-      const int e = act ? (int)(kk[p] & 63u) : 0;
       const bool neg = act && (col[beg + e] < r);
       const uint64_t sign = __ballot(neg);
       if (lane == 0) {
@@ -357,16 +379,21 @@ __global__ void __launch_bounds__(64) k_ks_build(int32_t nrows, const int32_t *_
         md.mask = mask;
         md.sign = sign;
         md.ad = (uint32_t)(m >> 1);
-        md.base = (uint32_t)base;
+        md.base = uni ? QH_KS_UNIFORM : (uint32_t)base;
+        md.vre = uni ? fre : 0.0;
+        md.vim = uni ? fim : 0.0;
         meta[mo + slot] = md;
       }
-      if (act) kval[base + __popcll(mask & lt)] = val[beg + e];
+      if (act && !uni) kval[base + __popcll(mask & lt)] = v;
     }
-    base += __popcll(mask);
+    if (!uni) base += __popcll(mask);
     if (act) p++;
     slot++;
   }
-  if (!FILL && lane == 0) width[s] = slot;
+  if (!FILL && lane == 0) {
+    width[s] = slot;
+    nvals[s] = base;
+  }
 }
 
 __global__ void k_narrow_ks(const double2 *a, float2 *b, int64_t n) {
@@ -378,67 +405,102 @@ int ks_build(qh_mat *A) {
   if (A->ks_state) return QH_OK;
   hipStream_t s = qh_stream();
   const int32_t ns = (int32_t)((A->rows + 63) / 64);
-  int32_t *width = nullptr;
+  int32_t *width = nullptr, *nvals = nullptr, *voff = nullptr;
   int *ovf = nullptr;
   int rc = qh_alloc_t(&width, (size_t)ns + 1);
+  if (!rc) rc = qh_alloc_t(&nvals, (size_t)ns + 1);
+  if (!rc) rc = qh_alloc_t(&voff, (size_t)ns + 1);
   if (!rc) rc = qh_alloc_t(&ovf, 1);
   if (!rc) rc = qh_alloc_t(&A->ks_moff, (size_t)ns + 1);
-  if (rc) {
+  auto cleanup = [&]() {
     qh_free_ptr(width);
+    qh_free_ptr(nvals);
+    qh_free_ptr(voff);
     qh_free_ptr(ovf);
+  };
+  if (rc) {
+    cleanup();
     return rc;
   }
   QH_HIP(hipMemsetAsync(ovf, 0, sizeof(int), s));
   hipLaunchKernelGGL((k_ks_build<false>), dim3(ns), dim3(64), 0, s, (int32_t)A->rows, A->rowptr, A->col, A->val,
-                     width, nullptr, nullptr, nullptr, ovf);
+                     width, nvals, nullptr, nullptr, nullptr, nullptr, ovf);
   QH_HIP(hipGetLastError());
   int hovf = 0;
-  std::vector<int32_t> hw(ns);
+  std::vector<int32_t> hw(ns), hn(ns);
   QH_HIP(hipMemcpyAsync(&hovf, ovf, sizeof(int), hipMemcpyDeviceToHost, s));
   QH_HIP(hipMemcpyAsync(hw.data(), width, sizeof(int32_t) * ns, hipMemcpyDeviceToHost, s));
+  QH_HIP(hipMemcpyAsync(hn.data(), nvals, sizeof(int32_t) * ns, hipMemcpyDeviceToHost, s));
   QH_HIP(hipStreamSynchronize(s));
-  if (hovf) {
+  if (hovf) {  // a row longer than the slice builder handles: keep the CSR kernels
     A->ks_state = -1;
-    qh_free_ptr(width);
-    qh_free_ptr(ovf);
+    cleanup();
     qh_free_ptr(A->ks_moff);
     A->ks_moff = nullptr;
     return QH_OK;
   }
-  std::vector<int32_t> ho(ns + 1);
-  int64_t tot = 0;
+  std::vector<int32_t> ho(ns + 1), hv(ns + 1);
+  int64_t tot = 0, totv = 0;
   for (int32_t i = 0; i < ns; i++) {
     ho[i] = (int32_t)tot;
+    hv[i] = (int32_t)totv;
     tot += hw[i];
+    totv += hn[i];
   }
   ho[ns] = (int32_t)tot;
+  hv[ns] = (int32_t)totv;
   QH_HIP(hipMemcpyAsync(A->ks_moff, ho.data(), sizeof(int32_t) * (ns + 1), hipMemcpyHostToDevice, s));
+  QH_HIP(hipMemcpyAsync(voff, hv.data(), sizeof(int32_t) * (ns + 1), hipMemcpyHostToDevice, s));
   rc = qh_alloc_t(&A->ks_meta, (size_t)(tot > 0 ? tot : 1));
-  if (!rc) rc = qh_alloc_t(&A->ks_val, (size_t)(A->nnz > 0 ? A->nnz : 1));
+  if (!rc) rc = qh_alloc_t(&A->ks_val, (size_t)(totv > 0 ? totv : 1));
   if (rc) {
-    qh_free_ptr(width);
-    qh_free_ptr(ovf);
+    cleanup();
     return rc;
   }
   hipLaunchKernelGGL((k_ks_build<true>), dim3(ns), dim3(64), 0, s, (int32_t)A->rows, A->rowptr, A->col, A->val,
-                     width, A->ks_moff, A->ks_meta, A->ks_val, ovf);
+                     width, nvals, A->ks_moff, voff, A->ks_meta, A->ks_val, ovf);
   QH_HIP(hipGetLastError());
   QH_HIP(hipStreamSynchronize(s));
-  qh_free_ptr(width);
-  qh_free_ptr(ovf);
+  cleanup();
   A->nslices = ns;
   A->ks_slots = tot;
+  A->ks_nval = totv;
   A->ks_state = 1;
   return QH_OK;
 }
 
 int ks_make_fp32(qh_mat *A) {
   if (A->ks_state != 1 || A->ks_val32) return QH_OK;
-  int rc = qh_alloc_t(&A->ks_val32, (size_t)(A->nnz > 0 ? A->nnz : 1));
+  int rc = qh_alloc_t(&A->ks_val32, (size_t)(A->ks_nval > 0 ? A->ks_nval : 1));
   if (rc) return rc;
-  hipLaunchKernelGGL(k_narrow_ks, dim3(qh_blocks(A->nnz, 256)), dim3(256), 0, qh_stream(), A->ks_val, A->ks_val32,
-                     A->nnz);
+  if (A->ks_nval)
+    hipLaunchKernelGGL(k_narrow_ks, dim3(qh_blocks(A->ks_nval, 256)), dim3(256), 0, qh_stream(), A->ks_val,
+                       A->ks_val32, A->ks_nval);
   QH_HIP(hipGetLastError());
+  return QH_OK;
+}
+
+// launch the sliced-layout kernel with the slot unroll encoded in the variant code G
+template <typename R, int MODE>
+int launch_ks(int G, hipStream_t s, const qh_mat *A, const void *xin, void *kout, void *ynext, R cn, const void *x,
+              const void *k1, const void *k2, const void *k3, void *xout, R h, float2 *xlo) {
+  typedef typename C2<R>::T T;
+  const T *kv = sizeof(R) == 8 ? (const T *)A->ks_val : (const T *)A->ks_val32;
+  const dim3 grid(qh_blocks(A->nslices, 4)), blk(256);
+  const int32_t n = (int32_t)A->rows, ns = A->nslices;
+#define QH_LK(UU)                                                                                           \
+  hipLaunchKernelGGL((k_ks_stage<R, MODE, UU>), grid, blk, 0, s, n, ns, A->ks_moff, A->ks_meta, kv,         \
+                     (const T *)xin, (T *)kout, (T *)ynext, cn, (const T *)x, (const T *)k1, (const T *)k2, \
+                     (const T *)k3, (T *)xout, h, xlo)
+  switch (QH_KSELL_UNROLL(G)) {
+    case 2: QH_LK(2); break;
+    case 4: QH_LK(4); break;
+    case 8: QH_LK(8); break;
+    default:
+      qh_set_error("unsupported sliced-layout variant %d", G);
+      return QH_EINVAL;
+  }
+#undef QH_LK
   return QH_OK;
 }
 
@@ -448,12 +510,13 @@ int launch_stage_g(int G, hipStream_t s, const qh_mat *A, const typename C2<R>::
                    R h, float2 *xlo) {
   typedef typename C2<R>::T T;
   const int32_t n = (int32_t)A->rows;
-  if (G == QH_KSELL) {
-    const T *kv = sizeof(R) == 8 ? (const T *)A->ks_val : (const T *)A->ks_val32;
-    hipLaunchKernelGGL((k_ks_stage<R, MODE>), dim3(qh_blocks(A->nslices, 4)), dim3(256), 0, s, n, A->nslices,
-                       A->ks_moff, A->ks_meta, kv, (const T *)x, (const T *)kin, ck, (T *)kout, (const T *)k1,
-                       (const T *)k2, (const T *)k3, (T *)xout, h, xlo);
-    return QH_OK;
+  if (QH_IS_KSELL(G)) {  // plain y = A x on the sliced layout (stages go through launch_ks)
+    if (MODE != MODE_SPMV) {
+      qh_set_error("sliced-layout stages go through launch_ks");
+      return QH_EINVAL;
+    }
+    return launch_ks<R, MODE_SPMV>(G, s, A, x, kout, nullptr, (R)0, nullptr, nullptr, nullptr, nullptr, nullptr,
+                                   (R)0, nullptr);
   }
   const unsigned grid = qh_blocks((int64_t)n * G, 256);
 #define QH_L(GG)                                                                                                 \
@@ -501,7 +564,8 @@ int autotune(qh_mat *A, int *out_g) {
     int brc = ks_build(A);
     if (!brc && A->ks_state == 1 && sizeof(R) == 4) brc = ks_make_fp32(A);
     if (brc) return brc;
-    if (A->ks_state == 1) cands.push_back(QH_KSELL);
+    if (A->ks_state == 1)
+      for (int u = 128; u <= 130; u++) cands.push_back(u);
   }
   for (int g : {2, 4, 8, 16, 32, 64})
     if (g <= 4 * avg + 4) cands.push_back(g);
@@ -538,15 +602,15 @@ int lanes_for(qh_mat *A, int prec, int *g) {
   int rc = QH_OK;
   if (prec == QH_FP64) {
     if (!A->lanes64) rc = autotune<double>(A, &A->lanes64);
-    if (!rc && A->lanes64 == QH_KSELL) rc = ks_build(A);
-    if (!rc && A->lanes64 == QH_KSELL && A->ks_state != 1) A->lanes64 = 8;
+    if (!rc && QH_IS_KSELL(A->lanes64)) rc = ks_build(A);
+    if (!rc && QH_IS_KSELL(A->lanes64) && A->ks_state != 1) A->lanes64 = 8;
     *g = A->lanes64;
   } else {
     if (!A->val32) rc = qh_mat_make_fp32(A);
     if (!rc && !A->lanes32) rc = autotune<float>(A, &A->lanes32);
-    if (!rc && A->lanes32 == QH_KSELL) rc = ks_build(A);
-    if (!rc && A->lanes32 == QH_KSELL) rc = ks_make_fp32(A);
-    if (!rc && A->lanes32 == QH_KSELL && A->ks_state != 1) A->lanes32 = 8;
+    if (!rc && QH_IS_KSELL(A->lanes32)) rc = ks_build(A);
+    if (!rc && QH_IS_KSELL(A->lanes32)) rc = ks_make_fp32(A);
+    if (!rc && QH_IS_KSELL(A->lanes32) && A->ks_state != 1) A->lanes32 = 8;
     *g = A->lanes32;
   }
   return rc;
@@ -603,6 +667,7 @@ struct Stepper {
   int prec = 0;
   int64_t n = 0;
   void *k[3] = {nullptr, nullptr, nullptr};
+  void *y[2] = {nullptr, nullptr};  // materialised stage inputs (sliced-layout path)
   void *xalt = nullptr;  // double buffer for the state (hi part for FP32)
   double *partial = nullptr, *inv = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -624,6 +689,7 @@ static void stepper_release() {
   if (g_st.exec) (void)hipGraphExecDestroy(g_st.exec);
   if (g_st.graph) (void)hipGraphDestroy(g_st.graph);
   for (auto &p : g_st.k) qh_free_ptr(p);
+  for (auto &p : g_st.y) qh_free_ptr(p);
   qh_free_ptr(g_st.xalt);
   qh_free_ptr(g_st.partial);
   qh_free_ptr(g_st.inv);
@@ -637,6 +703,8 @@ static int stepper_prepare(const qh_mat *A, int prec, int64_t n) {
   size_t es = prec == QH_FP64 ? 16 : 8;
   int rc = QH_OK;
   for (auto &p : g_st.k)
+    if (!rc) rc = qh_alloc(&p, (size_t)n * es);
+  for (auto &p : g_st.y)
     if (!rc) rc = qh_alloc(&p, (size_t)n * es);
   if (!rc) rc = qh_alloc(&g_st.xalt, (size_t)n * es);
   if (!rc) rc = qh_alloc_t(&g_st.partial, 1024);
@@ -659,6 +727,23 @@ static int rk4_step(hipStream_t s, const qh_mat *A, int G, const void *xa, void 
   const T *val = sizeof(R) == 8 ? (const T *)A->val : (const T *)A->val32;
   void **k = g_st.k;
   int rc;
+  if (QH_IS_KSELL(G)) {
+    // stage inputs materialised in the epilogues: x -> Ya -> Yb -> Ya
+    void *Ya = g_st.y[0], *Yb = g_st.y[1];
+    const void *ins[4] = {xa, Ya, Yb, Ya};
+    void *outs[3] = {Ya, Yb, Ya};
+    const R cns[3] = {h / 2, h / 2, h};
+    for (int st = 0; st < 3; st++) {
+      if (ev) QH_HIP(hipEventRecord(ev[st], s));
+      rc = launch_ks<R, MODE_STAGE>(G, s, A, ins[st], k[st], outs[st], cns[st], xa, nullptr, nullptr, nullptr,
+                                    nullptr, h, xlo);
+      if (rc) return rc;
+    }
+    if (ev) QH_HIP(hipEventRecord(ev[3], s));
+    rc = launch_ks<R, MODE_FINAL>(G, s, A, ins[3], nullptr, nullptr, (R)0, xa, k[0], k[1], k[2], xb, h, xlo);
+    if (ev) QH_HIP(hipEventRecord(ev[4], s));
+    return rc;
+  }
   if (ev) QH_HIP(hipEventRecord(ev[0], s));
   rc = launch_stage_g<R, MODE_SPMV>(G, s, A, val, xa, nullptr, (R)0, k[0], nullptr, nullptr, nullptr, nullptr, h,
                                     xlo);
@@ -815,12 +900,24 @@ extern "C" int qh_rk4_timed(qh_mat *A, qh_vec *x, double dt, int64_t nsteps, int
 extern "C" int qh_mat_set_lanes(qh_mat *m, int l64, int l32) {
   QH_REQUIRE(m, QH_EINVAL, "null matrix");
   auto ok = [](int g) {
-    return g == 0 || g == 1 || g == 2 || g == 4 || g == 8 || g == 16 || g == 32 || g == 64 || g == QH_KSELL;
+    return g == 0 || g == 1 || g == 2 || g == 4 || g == 8 || g == 16 || g == 32 || g == 64 || QH_IS_KSELL(g);
   };
-  QH_REQUIRE(ok(l64) && ok(l32), QH_EINVAL, "lanes per row must be a power of two <= 64, or QH_KSELL (128)");
+  QH_REQUIRE(ok(l64) && ok(l32), QH_EINVAL, "lanes per row must be a power of two <= 64, or 128..130 (sliced layout)");
   m->lanes64 = l64;
   m->lanes32 = l32;
   if (g_st.A == m) g_st.gG = -1;  // force a re-capture
+  return QH_OK;
+}
+
+extern "C" int qh_mat_layout_info(qh_mat *m, int64_t *slices, int64_t *slots, int64_t *stored, int64_t *mbytes) {
+  QH_REQUIRE(m, QH_EINVAL, "null matrix");
+  int rc = ks_build(m);
+  if (rc) return rc;
+  const bool ok = m->ks_state == 1;
+  if (slices) *slices = ok ? m->nslices : 0;
+  if (slots) *slots = ok ? m->ks_slots : 0;
+  if (stored) *stored = ok ? m->ks_nval : 0;
+  if (mbytes) *mbytes = ok ? (int64_t)(m->ks_slots * sizeof(qh_kmeta) + 4 * (m->nslices + 1)) : 0;
   return QH_OK;
 }
 
