@@ -39,6 +39,25 @@ def rk4_step_bytes(nnz, rows, S):
     return 4 * (nnz * (S + 4) + 4 * (rows + 1)) + 14 * rows * S
 
 
+def kernel_name(variant, real):
+    """name of the stage kernel a lanes-per-row variant code selects"""
+    if 128 <= variant <= 130:
+        return f"k_ks_stage<{real}, MODE, {2 << (variant - 128)}> (sliced layout)"
+    return f"k_rk_stage<{real}, {variant}> (CSR, {variant} lanes per row)"
+
+
+def read_traffic(path):
+    """HBM bytes per stage launch from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE + WRITE_SIZE, each scaled by its copy-kernel factor); None when absent"""
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch"):
+                return float(line.split("=", 1)[1]), os.path.relpath(path, ROOT)
+    except OSError:
+        pass
+    return None, None
+
+
 def cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -58,7 +77,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--renorm", type=int, default=100)
-    ap.add_argument("--workload", default="chain10", choices=["chain10", "tc512", "circuit12", "jc"])
+    ap.add_argument("--workload", default="chain10", choices=["chain10", "tc512", "circuit12", "jc", "copy"])
     ap.add_argument("--instances", type=int, default=512)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -95,15 +114,19 @@ def main():
     x0 = np.zeros(rows, np.complex128)
     x0[loc] = 1.0
 
+    dev_ms = {}
+
     def run(prec, renorm):
         v = qa.DeviceVec(rows, prec)
         v.set(x0)
         qa.rk4_device(A, v, dt, W, tot=N, renorm_every=renorm)   # warmup (autotune + graph capture)
         qa.synchronize()
         t = time.perf_counter()
-        qa.rk4_device(A, v, dt, K, tot=N, renorm_every=renorm)
+        with qa.StreamTimer() as tm:   # HIP events on the stream the stage kernels run on
+            qa.rk4_device(A, v, dt, K, tot=N, renorm_every=renorm)
         qa.synchronize()
         el = time.perf_counter() - t
+        dev_ms[prec] = tm.ms
         dv = qa.DeviceVec(rows, qa.FP64)
         qa._qh(qa.lib().qh_vec_copy(v.h, dv.h))
         pops = np.zeros(len(levels))
@@ -139,12 +162,20 @@ def main():
     # the Y's, K1..K3 + x reads and the x' write in the last epilogue)
     ks_step32 = 4 * (layout["stored_vals"] * 8 + layout["meta_bytes"]) + 17 * rows * 8
 
+    # mean stage-kernel duration over the timed FP32 region: its device time
+    # (event pair) over its 4K stage launches (includes the renormalisation
+    # kernels every renorm steps and the gaps between graph nodes)
+    launch_ms32 = dev_ms[qa.FP32] / (4 * K)
+    launch_ms64 = dev_ms[qa.FP64] / (4 * K)
     b_step32 = rk4_step_bytes(nnz, rows, 8)
     b_step64 = rk4_step_bytes(nnz, rows, 16)
-    ach_stage32 = (b_step32 / 4) / (stage_ms32 * 1e-3) / 1e9
-    ach_stage64 = (b_step64 / 4) / (stage_ms64 * 1e-3) / 1e9
+    ach_stage32 = (b_step32 / 4) / (launch_ms32 * 1e-3) / 1e9
+    ach_stage64 = (b_step64 / 4) / (launch_ms64 * 1e-3) / 1e9
     spmv_gbs32 = spmv_bytes(nnz, rows, 8) / (spmv_ms32 * 1e-3) / 1e9
     spmv_gbs64 = spmv_bytes(nnz, rows, 16) / (spmv_ms64 * 1e-3) / 1e9
+
+    kname = kernel_name(lanes[1], "float")
+    traffic, traffic_src = read_traffic(os.path.join(ROOT, "profiles", "k_ks_stage_chain10_traffic.txt"))
 
     steps_s32 = K / el32
     steps_s64 = K / el64
@@ -187,16 +218,20 @@ def main():
                                f"{nnz} nnz, RK4 dt={dt}, h_i,J_i~U[0.5,1.5] seed 1, gamma1=1e-3, gamma2=5e-3",
                    "parallelism": "single GPU", "renorm_every": args.renorm},
         "roofline": {"bound": "hbm", "achieved": round(ach_stage32, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(ach_stage32 / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_rk_stage<float> (mean of the 4 fused RK4 stage launches)",
-                     "bytes_per_launch": b_step32 / 4, "launch_ms": round(stage_ms32, 5)},
+                     "frac": round(ach_stage32 / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": f"{kname} (mean of the 4 fused RK4 stage launches)",
+                     "bytes_per_launch": b_step32 / 4, "launch_ms": round(launch_ms32, 5),
+                     "launch_ms_per_kernel_events": round(stage_ms32, 5),
+                     "format_bytes_per_launch": ks_step32 / 4,
+                     "format_gbs": round((ks_step32 / 4) / (launch_ms32 * 1e-3) / 1e9, 1),
+                     "traffic_source": traffic_src},
         "cpu_baseline": cpu,
         "fp64_steps_per_s": round(steps_s64, 3),
         "fp32_over_fp64": round(steps_s32 / steps_s64, 3),
         "max_pop_err_fp32_vs_fp64": pop_err,
         "trace_fp32": tr32, "trace_fp64": tr64,
         "fp64_stage": {"achieved_gbs": round(ach_stage64, 1), "frac": round(ach_stage64 / HBM_PEAK_GBS, 4),
-                       "launch_ms": round(stage_ms64, 5)},
+                       "launch_ms": round(launch_ms64, 5), "launch_ms_per_kernel_events": round(stage_ms64, 5)},
         "spmv": {"fp32_gbs": round(spmv_gbs32, 1), "fp32_frac": round(spmv_gbs32 / HBM_PEAK_GBS, 4),
                  "fp64_gbs": round(spmv_gbs64, 1), "fp64_frac": round(spmv_gbs64 / HBM_PEAK_GBS, 4),
                  "fp32_ms": round(spmv_ms32, 5), "fp64_ms": round(spmv_ms64, 5), "lanes_per_row": lanes},

@@ -10,6 +10,9 @@ circuit12 : 12-qubit noisy circuit on rho (4096 x 4096): 300 seeded gates in
             FP64, per-qubit populations and trace.
 jc        : Jaynes-Cummings 2 x 10 (400-row superoperator), RK4 dt = 0.01 to
             t = 100 with populations every 100 steps (latency-bound).
+copy      : plain streaming copy of known size (complex128 -> complex64 hi/lo
+            split: 16 B read + 16 B written per element), the calibration run
+            for the HBM byte counters (FETCH_SIZE / WRITE_SIZE).
 """
 import ctypes
 import math
@@ -227,3 +230,22 @@ def jc(args):
             "fp32_steps_per_s": round(10000 / el32, 2),
             "max_pop_err_fp32_vs_fp64": float(np.abs(p32 - p64).max()),
             "final_populations_fp64": [float(x) for x in p64[-1]]}
+
+
+def copy(args):
+    n = 1 << 24                        # 16 777 216 elements: 256 MiB read, 256 MiB written
+    reps = 20
+    src = qa.DeviceVec(n, qa.FP64)
+    dst = qa.DeviceVec(n, qa.FP32)
+    qa._qh(qa.lib().qh_vec_copy(src.h, dst.h))
+    qa.synchronize()
+    with qa.StreamTimer() as tm:
+        for _ in range(reps):
+            qa._qh(qa.lib().qh_vec_copy(src.h, dst.h))
+    ms = tm.ms / reps
+    rd = wr = 16 * n
+    return {"metric": "HBM copy calibration", "value": round((rd + wr) / (ms * 1e-3) / 1e9, 1), "unit": "GB/s",
+            "n_gpus": 1, "steps": reps, "warmup": 1, "ms_per_step": round(ms, 5), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f64 -> f32 pair", "data": "zeros",
+            "config": {"workload": "k_split_d2 over 16 777 216 complex128 elements"},
+            "read_bytes_per_launch": rd, "write_bytes_per_launch": wr}

@@ -178,6 +178,10 @@ int qh_last_stage_ms(double *spmv_ms, int64_t *launches);
 int qh_set_graph_steps(int steps);
 /* mean duration (ms) of reps plain SpMV launches y = A x on the chosen precision */
 int qh_bench_spmv(qh_mat *A, int prec, int reps, double *avg_ms);
+/* HIP event pair on the library stream around a caller's region; qh_timer_end
+ * waits for the region to finish and returns its device time in ms */
+int qh_timer_begin(void);
+int qh_timer_end(double *ms);
 
 #ifdef __cplusplus
 }

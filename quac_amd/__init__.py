@@ -174,6 +174,8 @@ def _setup(L):
     sig("qh_last_stage_ms", i32, P(dbl), P(i64))
     sig("qh_set_graph_steps", i32, i32)
     sig("qh_bench_spmv", i32, vp, i32, i32, P(dbl))
+    sig("qh_timer_begin", i32)
+    sig("qh_timer_end", i32, P(dbl))
     sig("qh_trace", i32, vp, i64, P(dbl), P(dbl))
     sig("qh_populations", i32, vp, i32, P(i64), P(i64), i32, P(dbl))
     sig("qh_gate", i32, vp, i64, i32, i32, P(i64), P(dbl))
@@ -711,3 +713,18 @@ def bench_spmv(mat, prec, reps=20):
     ms = ctypes.c_double()
     _qh(lib().qh_bench_spmv(mat.h, prec, reps, ctypes.byref(ms)))
     return ms.value
+
+
+class StreamTimer:
+    """HIP event pair on the library stream around a region (device ms in .ms)"""
+
+    def __enter__(self):
+        _qh(lib().qh_timer_begin())
+        return self
+
+    def __exit__(self, *exc):
+        if exc[0] is None:
+            ms = ctypes.c_double()
+            _qh(lib().qh_timer_end(ctypes.byref(ms)))
+            self.ms = ms.value
+        return False

@@ -350,6 +350,7 @@ extern "C" int qh_mat_destroy(qh_mat *m) {
   qh_free_ptr(m->val);
   qh_free_ptr(m->val32);
   qh_free_ptr(m->ks_moff);
+  qh_free_ptr(m->ks_umid);
   qh_free_ptr(m->ks_meta);
   qh_free_ptr(m->ks_val);
   qh_free_ptr(m->ks_val32);

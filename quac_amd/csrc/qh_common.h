@@ -41,7 +41,8 @@ int qh_num_cus();
 // value array (values of active lanes are stored consecutively in lane order).
 // When every active lane of the slot holds the bit-identical value, the value
 // is kept in the metadata (vre, vim) and base is QH_KS_UNIFORM: no per-lane
-// value is stored or loaded for that slot.
+// value is stored or loaded for that slot.  A slice lists its uniform slots
+// first, then the others, each group in increasing (ad, rank) order.
 struct qh_kmeta {
   uint64_t mask, sign;
   uint32_t ad, base;
@@ -61,6 +62,7 @@ struct qh_mat {
   int ks_state = 0;            // 0 not built, 1 built, -1 not applicable (a row longer than 64)
   int32_t nslices = 0;
   int32_t *ks_moff = nullptr;  // nslices+1 slot offsets
+  int32_t *ks_umid = nullptr;  // nslices: end of the slice's uniform slots (they come first)
   qh_kmeta *ks_meta = nullptr;
   double2 *ks_val = nullptr;   // nnz, slice/slot/lane order
   float2 *ks_val32 = nullptr;

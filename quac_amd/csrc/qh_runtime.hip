@@ -49,9 +49,17 @@ int qh_ensure_init() { return g_init ? QH_OK : qh_init(0); }
 hipStream_t qh_stream() { return g_stream; }
 int qh_num_cus() { return g_cus; }
 
+// Event pair on the library stream bracketing a caller's timed region.
+static hipEvent_t g_timer[2] = {nullptr, nullptr};
+
 extern "C" int qh_finalize(void) {
   if (!g_init) return QH_OK;
   QH_HIP(hipStreamSynchronize(g_stream));
+  for (int i = 0; i < 2; ++i)
+    if (g_timer[i]) {
+      QH_HIP(hipEventDestroy(g_timer[i]));
+      g_timer[i] = nullptr;
+    }
   QH_HIP(hipStreamDestroy(g_stream));
   g_stream = nullptr;
   g_init = 0;
@@ -62,6 +70,25 @@ extern "C" int qh_synchronize(void) {
   int rc = qh_ensure_init();
   if (rc) return rc;
   QH_HIP(hipStreamSynchronize(g_stream));
+  return QH_OK;
+}
+
+extern "C" int qh_timer_begin(void) {
+  int rc = qh_ensure_init();
+  if (rc) return rc;
+  for (int i = 0; i < 2; ++i)
+    if (!g_timer[i]) QH_HIP(hipEventCreate(&g_timer[i]));
+  QH_HIP(hipEventRecord(g_timer[0], g_stream));
+  return QH_OK;
+}
+
+extern "C" int qh_timer_end(double *ms) {
+  QH_REQUIRE(ms && g_timer[0], QH_EINVAL, "qh_timer_end without qh_timer_begin");
+  QH_HIP(hipEventRecord(g_timer[1], g_stream));
+  QH_HIP(hipEventSynchronize(g_timer[1]));
+  float f = 0.f;
+  QH_HIP(hipEventElapsedTime(&f, g_timer[0], g_timer[1]));
+  *ms = f;
   return QH_OK;
 }
 

@@ -192,17 +192,60 @@ __global__ void __launch_bounds__(256) k_rk_stage(
 // the per-slot metadata is wave-uniform (scalar loads), the values of the active
 // lanes are compacted (no padding, no column-index stream), and the x gathers of
 // a slot are contiguous across the wavefront.
-// value of an active lane in a slot: from the wave-uniform metadata when the
-// slot is uniform, else from the compacted per-lane value stream
+//
+// The slot loop is branch-free: the metadata's 64-bit lane masks select per lane
+// directly (inverse ballot), inactive lanes gather their own row and contribute
+// zero, so the U gathers of a pass are all in flight together.  Products and
+// sums are formed in FP64 for both storage precisions (FP64 FMA issues at the
+// FP32 rate on CDNA4).
+
+// gather with a 32-bit byte offset (vectors are < 4 GiB: checked when the layout is built)
 template <typename R>
-__device__ inline typename C2<R>::T ks_value(const qh_kmeta &m, const typename C2<R>::T *val, uint64_t lt) {
-  typename C2<R>::T a;
-  if (m.base == QH_KS_UNIFORM) {
-    a.x = (R)m.vre;
-    a.y = (R)m.vim;
-    return a;
+__device__ inline typename C2<R>::T ld_gather(const typename C2<R>::T *__restrict__ p, int32_t i) {
+  return *reinterpret_cast<const typename C2<R>::T *>(reinterpret_cast<const char *>(p) +
+                                                      (uint32_t)i * (uint32_t)sizeof(typename C2<R>::T));
+}
+
+// One pass over U slots of a slice starting at mp (cnt valid slots when TAIL).
+// VALS: the slots carry per-lane values in the compacted stream, else the
+// slot's uniform value is in its metadata.
+template <typename R, bool VALS, bool TAIL, int U>
+__device__ __forceinline__ void ks_pass(const qh_kmeta *__restrict__ mp, int cnt, int32_t rr, uint64_t lt,
+                                        const typename C2<R>::T *__restrict__ xin,
+                                        const typename C2<R>::T *__restrict__ val, double &are, double &aim) {
+  typedef typename C2<R>::T T;
+  qh_kmeta m[U];
+#pragma unroll
+  for (int j = 0; j < U; j++) m[j] = mp[TAIL ? (j < cnt ? j : cnt - 1) : j];
+  T xv[U], vv[U];
+#pragma unroll
+  for (int j = 0; j < U; j++) {
+    const bool on = __builtin_amdgcn_inverse_ballot_w64((!TAIL || j < cnt) ? m[j].mask : 0ull);
+    const bool neg = __builtin_amdgcn_inverse_ballot_w64(m[j].sign);
+    int32_t c = neg ? rr - (int32_t)m[j].ad : rr + (int32_t)m[j].ad;
+    c = on ? c : rr;
+    xv[j] = ld_gather<R>(xin, c);
+    if (VALS) {
+      uint32_t vi = m[j].base + (uint32_t)__popcll(m[j].mask & lt);
+      vi = on ? vi : m[j].base;
+      vv[j] = ld_stream<R>(val + vi);
+    }
   }
-  return ld_stream<R>(val + m.base + __popcll(m.mask & lt));
+#pragma unroll
+  for (int j = 0; j < U; j++) {
+    const bool on = __builtin_amdgcn_inverse_ballot_w64((!TAIL || j < cnt) ? m[j].mask : 0ull);
+    const double xr = (double)(on ? xv[j].x : (R)0), xi = (double)(on ? xv[j].y : (R)0);
+    double ar, ai;
+    if (VALS) {
+      ar = (double)vv[j].x;
+      ai = (double)vv[j].y;
+    } else {
+      ar = m[j].vre;
+      ai = m[j].vim;
+    }
+    are = fma(ar, xr, fma(-ai, xi, are));
+    aim = fma(ar, xi, fma(ai, xr, aim));
+  }
 }
 
 // Stage kernel on the sliced layout.  The stage input Y_s = x + c_s K_{s-1} is
@@ -211,48 +254,38 @@ __device__ inline typename C2<R>::T ks_value(const qh_kmeta &m, const typename C
 //   MODE_SPMV : kout = A xin
 //   MODE_STAGE: kout = A xin;  ynext = x + cn * kout          (stages 1-3)
 //   MODE_FINAL: xout = x + h/6 (K1 + 2 K2 + 2 K3 + A xin)    (stage 4)
-// U slots are processed per pass (all metadata loads, then all value loads and
-// gathers, then the FMAs); U trades memory-level parallelism against the SGPRs
-// the metadata occupies (occupancy), and is chosen by the autotuner.
+// U slots are processed per pass (all metadata loads, then all gathers, then
+// the FMAs); U trades memory-level parallelism against the SGPRs the metadata
+// occupies, and is chosen by the autotuner.  The FP32 state is x = hi + lo
+// (complex64 pair); its updates are formed in FP64 and split again.
 template <typename R, int MODE, int U>
 __global__ void __launch_bounds__(256) k_ks_stage(
-    int32_t nrows, int32_t nslices, const int32_t *__restrict__ moff, const qh_kmeta *__restrict__ meta,
-    const typename C2<R>::T *__restrict__ val, const typename C2<R>::T *__restrict__ xin,
-    typename C2<R>::T *__restrict__ kout, typename C2<R>::T *__restrict__ ynext, R cn,
-    const typename C2<R>::T *__restrict__ x, const typename C2<R>::T *__restrict__ k1,
-    const typename C2<R>::T *__restrict__ k2, const typename C2<R>::T *__restrict__ k3,
-    typename C2<R>::T *__restrict__ xout, R h, float2 *__restrict__ xlo) {
+    int32_t nrows, int32_t nslices, const int32_t *__restrict__ moff, const int32_t *__restrict__ umid,
+    const qh_kmeta *__restrict__ meta, const typename C2<R>::T *__restrict__ val,
+    const typename C2<R>::T *__restrict__ xin, typename C2<R>::T *__restrict__ kout,
+    typename C2<R>::T *__restrict__ ynext, R cn, const typename C2<R>::T *__restrict__ x,
+    const typename C2<R>::T *__restrict__ k1, const typename C2<R>::T *__restrict__ k2,
+    const typename C2<R>::T *__restrict__ k3, typename C2<R>::T *__restrict__ xout, R h,
+    float2 *__restrict__ xlo) {
   typedef typename C2<R>::T T;
   const unsigned b = qh_xcd_block(blockIdx.x, gridDim.x);
   const int s = (int)(b * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6));
   if (s >= nslices) return;  // whole wavefront
   const int lane = threadIdx.x & 63;
   const int32_t r = s * 64 + lane;
+  const int32_t rr = r < nrows ? r : nrows - 1;  // inactive lanes of the last slice gather a valid row
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  const int32_t k0 = moff[s], kend = moff[s + 1];
-  Acc<R> acc;
-  for (int32_t k = k0; k < kend; k += U) {
-    qh_kmeta m[U];
-#pragma unroll
-    for (int j = 0; j < U; j++)
-      if (k + j < kend) m[j] = meta[k + j];
-    bool act[U];
-    T v[U], xv[U];
-#pragma unroll
-    for (int j = 0; j < U; j++) {
-      act[j] = (k + j < kend) && ((m[j].mask >> lane) & 1);
-      if (act[j]) {
-        const int32_t c = ((m[j].sign >> lane) & 1) ? r - (int32_t)m[j].ad : r + (int32_t)m[j].ad;
-        v[j] = ks_value<R>(m[j], val, lt);
-        xv[j] = xin[c];
-      }
-    }
-#pragma unroll
-    for (int j = 0; j < U; j++)
-      if (act[j]) acc.add(v[j].x * xv[j].x - v[j].y * xv[j].y, v[j].x * xv[j].y + v[j].y * xv[j].x);
-  }
+  const int32_t k0 = moff[s], km = umid[s], kend = moff[s + 1];
+  double are = 0.0, aim = 0.0;
+  int32_t k = k0;
+  for (; k + U <= km; k += U) ks_pass<R, false, false, U>(meta + k, U, rr, lt, xin, val, are, aim);
+  if (k < km) ks_pass<R, false, true, U>(meta + k, km - k, rr, lt, xin, val, are, aim);
+  for (k = km; k + U <= kend; k += U) ks_pass<R, true, false, U>(meta + k, U, rr, lt, xin, val, are, aim);
+  if (k < kend) ks_pass<R, true, true, U>(meta + k, kend - k, rr, lt, xin, val, are, aim);
   if (r >= nrows) return;
-  const T y = acc.value();
+  T y;
+  y.x = (R)are;
+  y.y = (R)aim;
   if (MODE == MODE_SPMV) {
     kout[r] = y;
     return;
@@ -266,35 +299,33 @@ __global__ void __launch_bounds__(256) k_ks_stage(
       yn.y = xo.y + cn * y.y;
     } else {
       const float2 lo = xlo[r];
-      yn.x = xo.x + (lo.x + cn * y.x);
-      yn.y = xo.y + (lo.y + cn * y.y);
+      yn.x = (R)((double)xo.x + (double)lo.x + (double)cn * are);
+      yn.y = (R)((double)xo.y + (double)lo.y + (double)cn * aim);
     }
     ynext[r] = yn;
     return;
   }
   const T a = k1[r], bq = k2[r], cq = k3[r];
-  const R s6 = h * (R)(1.0 / 6.0), s3 = h * (R)(1.0 / 3.0);
-  const R dre = s6 * a.x + s3 * bq.x + s3 * cq.x + s6 * y.x;
-  const R dim = s6 * a.y + s3 * bq.y + s3 * cq.y + s6 * y.y;
   if (sizeof(R) == 8) {
+    const R s6 = h * (R)(1.0 / 6.0), s3 = h * (R)(1.0 / 3.0);
+    const R dre = s6 * a.x + s3 * bq.x + s3 * cq.x + s6 * y.x;
+    const R dim = s6 * a.y + s3 * bq.y + s3 * cq.y + s6 * y.y;
     T o;
     o.x = xo.x + dre;
     o.y = xo.y + dim;
     xout[r] = o;
   } else {
-    float2 lo = xlo[r];
-    float sr, er, si, ei;
-    two_sum((float)xo.x, (float)dre, sr, er);
-    two_sum((float)xo.y, (float)dim, si, ei);
-    float lr = lo.x + er, li = lo.y + ei;
-    float hr = sr + lr, hi = si + li;
-    lr = lr - (hr - sr);
-    li = li - (hi - si);
+    // x' = hi + lo + d in FP64, split again into (hi', lo')
+    const double s6 = (double)h * (1.0 / 6.0), s3 = (double)h * (1.0 / 3.0);
+    const double dre = s6 * (double)a.x + s3 * (double)bq.x + s3 * (double)cq.x + s6 * are;
+    const double dim = s6 * (double)a.y + s3 * (double)bq.y + s3 * (double)cq.y + s6 * aim;
+    const float2 lo = xlo[r];
+    const double nr = ((double)xo.x + (double)lo.x) + dre, ni = ((double)xo.y + (double)lo.y) + dim;
     T o;
-    o.x = hr;
-    o.y = hi;
+    o.x = (R)nr;
+    o.y = (R)ni;
     xout[r] = o;
-    xlo[r] = make_float2(lr, li);
+    xlo[r] = make_float2((float)(nr - (double)o.x), (float)(ni - (double)o.y));
   }
 }
 
@@ -316,9 +347,9 @@ __device__ inline uint64_t wave_min_u64(uint64_t v) {
 template <bool FILL>
 __global__ void __launch_bounds__(64) k_ks_build(int32_t nrows, const int32_t *__restrict__ rp,
                                                  const int32_t *__restrict__ col, const double2 *__restrict__ val,
-                                                 int32_t *width, int32_t *nvals, const int32_t *moff,
-                                                 const int32_t *voff, qh_kmeta *meta, double2 *kval,
-                                                 int *overflow) {
+                                                 int32_t *width, int32_t *nvals, int32_t *nuni,
+                                                 const int32_t *moff, const int32_t *voff, qh_kmeta *meta,
+                                                 double2 *kval, int *overflow) {
   __shared__ uint64_t keys[64][KS_MAXROW];
   const int s = blockIdx.x, lane = threadIdx.x;
   const int32_t r = s * 64 + lane;
@@ -354,9 +385,10 @@ __global__ void __launch_bounds__(64) k_ks_build(int32_t nrows, const int32_t *_
     prev_ad = ad;
   }
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  int p = 0, slot = 0;
+  int p = 0, slot = 0, uslot = 0, nslot = 0;
   int32_t base = FILL ? voff[s] : 0;  // compacted non-uniform values of this slice start here
-  int32_t mo = FILL ? moff[s] : 0;
+  const int32_t mo = FILL ? moff[s] : 0;
+  const int32_t mn = FILL ? moff[s] + nuni[s] : 0;  // the non-uniform slots follow the uniform ones
   for (;;) {
     const uint64_t cur = p < len ? (kk[p] >> 6) : ~0ull;
     const uint64_t m = wave_min_u64(cur);
@@ -382,17 +414,22 @@ __global__ void __launch_bounds__(64) k_ks_build(int32_t nrows, const int32_t *_
         md.base = uni ? QH_KS_UNIFORM : (uint32_t)base;
         md.vre = uni ? fre : 0.0;
         md.vim = uni ? fim : 0.0;
-        meta[mo + slot] = md;
+        meta[uni ? mo + uslot : mn + nslot] = md;
       }
       if (act && !uni) kval[base + __popcll(mask & lt)] = v;
     }
     if (!uni) base += __popcll(mask);
     if (act) p++;
     slot++;
+    if (uni)
+      uslot++;
+    else
+      nslot++;
   }
   if (!FILL && lane == 0) {
     width[s] = slot;
     nvals[s] = base;
+    nuni[s] = uslot;
   }
 }
 
@@ -405,16 +442,24 @@ int ks_build(qh_mat *A) {
   if (A->ks_state) return QH_OK;
   hipStream_t s = qh_stream();
   const int32_t ns = (int32_t)((A->rows + 63) / 64);
-  int32_t *width = nullptr, *nvals = nullptr, *voff = nullptr;
+  // the gathers address vectors with 32-bit byte offsets
+  if (A->rows * (int64_t)sizeof(double2) >= ((int64_t)1 << 32)) {
+    A->ks_state = -1;
+    return QH_OK;
+  }
+  int32_t *width = nullptr, *nvals = nullptr, *nuni = nullptr, *voff = nullptr;
   int *ovf = nullptr;
   int rc = qh_alloc_t(&width, (size_t)ns + 1);
   if (!rc) rc = qh_alloc_t(&nvals, (size_t)ns + 1);
+  if (!rc) rc = qh_alloc_t(&nuni, (size_t)ns + 1);
   if (!rc) rc = qh_alloc_t(&voff, (size_t)ns + 1);
   if (!rc) rc = qh_alloc_t(&ovf, 1);
   if (!rc) rc = qh_alloc_t(&A->ks_moff, (size_t)ns + 1);
+  if (!rc) rc = qh_alloc_t(&A->ks_umid, (size_t)ns);
   auto cleanup = [&]() {
     qh_free_ptr(width);
     qh_free_ptr(nvals);
+    qh_free_ptr(nuni);
     qh_free_ptr(voff);
     qh_free_ptr(ovf);
   };
@@ -424,32 +469,37 @@ int ks_build(qh_mat *A) {
   }
   QH_HIP(hipMemsetAsync(ovf, 0, sizeof(int), s));
   hipLaunchKernelGGL((k_ks_build<false>), dim3(ns), dim3(64), 0, s, (int32_t)A->rows, A->rowptr, A->col, A->val,
-                     width, nvals, nullptr, nullptr, nullptr, nullptr, ovf);
+                     width, nvals, nuni, nullptr, nullptr, nullptr, nullptr, ovf);
   QH_HIP(hipGetLastError());
   int hovf = 0;
-  std::vector<int32_t> hw(ns), hn(ns);
+  std::vector<int32_t> hw(ns), hn(ns), hu(ns);
   QH_HIP(hipMemcpyAsync(&hovf, ovf, sizeof(int), hipMemcpyDeviceToHost, s));
   QH_HIP(hipMemcpyAsync(hw.data(), width, sizeof(int32_t) * ns, hipMemcpyDeviceToHost, s));
   QH_HIP(hipMemcpyAsync(hn.data(), nvals, sizeof(int32_t) * ns, hipMemcpyDeviceToHost, s));
+  QH_HIP(hipMemcpyAsync(hu.data(), nuni, sizeof(int32_t) * ns, hipMemcpyDeviceToHost, s));
   QH_HIP(hipStreamSynchronize(s));
   if (hovf) {  // a row longer than the slice builder handles: keep the CSR kernels
     A->ks_state = -1;
     cleanup();
     qh_free_ptr(A->ks_moff);
+    qh_free_ptr(A->ks_umid);
     A->ks_moff = nullptr;
+    A->ks_umid = nullptr;
     return QH_OK;
   }
-  std::vector<int32_t> ho(ns + 1), hv(ns + 1);
+  std::vector<int32_t> ho(ns + 1), hv(ns + 1), hm(ns);
   int64_t tot = 0, totv = 0;
   for (int32_t i = 0; i < ns; i++) {
     ho[i] = (int32_t)tot;
     hv[i] = (int32_t)totv;
+    hm[i] = (int32_t)(tot + hu[i]);
     tot += hw[i];
     totv += hn[i];
   }
   ho[ns] = (int32_t)tot;
   hv[ns] = (int32_t)totv;
   QH_HIP(hipMemcpyAsync(A->ks_moff, ho.data(), sizeof(int32_t) * (ns + 1), hipMemcpyHostToDevice, s));
+  QH_HIP(hipMemcpyAsync(A->ks_umid, hm.data(), sizeof(int32_t) * ns, hipMemcpyHostToDevice, s));
   QH_HIP(hipMemcpyAsync(voff, hv.data(), sizeof(int32_t) * (ns + 1), hipMemcpyHostToDevice, s));
   rc = qh_alloc_t(&A->ks_meta, (size_t)(tot > 0 ? tot : 1));
   if (!rc) rc = qh_alloc_t(&A->ks_val, (size_t)(totv > 0 ? totv : 1));
@@ -458,7 +508,7 @@ int ks_build(qh_mat *A) {
     return rc;
   }
   hipLaunchKernelGGL((k_ks_build<true>), dim3(ns), dim3(64), 0, s, (int32_t)A->rows, A->rowptr, A->col, A->val,
-                     width, nvals, A->ks_moff, voff, A->ks_meta, A->ks_val, ovf);
+                     width, nvals, nuni, A->ks_moff, voff, A->ks_meta, A->ks_val, ovf);
   QH_HIP(hipGetLastError());
   QH_HIP(hipStreamSynchronize(s));
   cleanup();
@@ -489,7 +539,7 @@ int launch_ks(int G, hipStream_t s, const qh_mat *A, const void *xin, void *kout
   const dim3 grid(qh_blocks(A->nslices, 4)), blk(256);
   const int32_t n = (int32_t)A->rows, ns = A->nslices;
 #define QH_LK(UU)                                                                                           \
-  hipLaunchKernelGGL((k_ks_stage<R, MODE, UU>), grid, blk, 0, s, n, ns, A->ks_moff, A->ks_meta, kv,         \
+  hipLaunchKernelGGL((k_ks_stage<R, MODE, UU>), grid, blk, 0, s, n, ns, A->ks_moff, A->ks_umid, A->ks_meta, kv, \
                      (const T *)xin, (T *)kout, (T *)ynext, cn, (const T *)x, (const T *)k1, (const T *)k2, \
                      (const T *)k3, (T *)xout, h, xlo)
   switch (QH_KSELL_UNROLL(G)) {
