@@ -49,6 +49,9 @@ struct qh_kmeta {
   double vre, vim;
 };
 #define QH_KS_UNIFORM 0xFFFFFFFFu
+// empty slots after the last one, so that a pass of up to 8 slots may read
+// its metadata without a bounds check
+#define QH_KS_PAD 16
 
 struct qh_mat {
   int64_t rows = 0, nnz = 0;

@@ -21,6 +21,7 @@
 // compensation, the state carries a complex64 compensation word (x = hi + lo)
 // so that the RK update does not lose the low bits of x.
 #include <algorithm>
+#include <cstdlib>
 #include <vector>
 
 #include "qh_common.h"
@@ -194,47 +195,67 @@ __global__ void __launch_bounds__(256) k_rk_stage(
 // a slot are contiguous across the wavefront.
 //
 // The slot loop is branch-free: the metadata's 64-bit lane masks select per lane
-// directly (inverse ballot), inactive lanes gather their own row and contribute
-// zero, so the U gathers of a pass are all in flight together.  Products and
-// sums are formed in FP64 for both storage precisions (FP64 FMA issues at the
-// FP32 rate on CDNA4).
+// directly (inverse ballot), inactive lanes load through an out-of-range buffer
+// offset and read exact zeros, so the U gathers of a pass are all in flight
+// together.  Products and sums are formed in FP64 for both storage precisions
+// (FP64 FMA issues at the FP32 rate on CDNA4).
 
-// gather with a 32-bit byte offset (vectors are < 4 GiB: checked when the layout is built)
-template <typename R>
-__device__ inline typename C2<R>::T ld_gather(const typename C2<R>::T *__restrict__ p, int32_t i) {
-  return *reinterpret_cast<const typename C2<R>::T *>(reinterpret_cast<const char *>(p) +
-                                                      (uint32_t)i * (uint32_t)sizeof(typename C2<R>::T));
+// Raw buffer resource over n elements (gfx9 descriptor word 3: 32-bit data
+// format).  A buffer load whose byte offset is at or beyond n * sizeof(T)
+// returns zero instead of touching memory: inactive lanes load with offset
+// QH_KS_OOB and so contribute exact zeros without any per-lane select.
+#define QH_KS_OOB 0x80000000u
+__device__ inline __amdgpu_buffer_rsrc_t ks_rsrc(const void *p, int64_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes, 0x00020000);
 }
 
-// One pass over U slots of a slice starting at mp (cnt valid slots when TAIL).
+template <typename R>
+__device__ inline typename C2<R>::T ks_load(__amdgpu_buffer_rsrc_t rs, uint32_t off);
+template <>
+__device__ inline float2 ks_load<float>(__amdgpu_buffer_rsrc_t rs, uint32_t off) {
+  return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(rs, off, 0, 0));
+}
+template <>
+__device__ inline double2 ks_load<double>(__amdgpu_buffer_rsrc_t rs, uint32_t off) {
+  return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0));
+}
+
+// One pass over U slots of a slice starting at mp; cnt valid slots when TAIL
+// (the metadata array carries QH_KS_PAD empty slots past its end, so the reads
+// of a tail pass never leave it; slots past cnt get an empty lane mask).
 // VALS: the slots carry per-lane values in the compacted stream, else the
 // slot's uniform value is in its metadata.
 template <typename R, bool VALS, bool TAIL, int U>
-__device__ __forceinline__ void ks_pass(const qh_kmeta *__restrict__ mp, int cnt, int32_t rr, uint64_t lt,
-                                        const typename C2<R>::T *__restrict__ xin,
-                                        const typename C2<R>::T *__restrict__ val, double &are, double &aim) {
+__device__ __forceinline__ void ks_pass(const qh_kmeta *__restrict__ mp, int32_t cnt, uint32_t r8,
+                                        __amdgpu_buffer_rsrc_t xr, __amdgpu_buffer_rsrc_t vr, double &are,
+                                        double &aim) {
   typedef typename C2<R>::T T;
   qh_kmeta m[U];
+  uint64_t mask[U];
 #pragma unroll
-  for (int j = 0; j < U; j++) m[j] = mp[TAIL ? (j < cnt ? j : cnt - 1) : j];
+  for (int j = 0; j < U; j++) {
+    m[j] = mp[j];
+    mask[j] = (!TAIL || j < cnt) ? m[j].mask : 0ull;
+  }
   T xv[U], vv[U];
 #pragma unroll
   for (int j = 0; j < U; j++) {
-    const bool on = __builtin_amdgcn_inverse_ballot_w64((!TAIL || j < cnt) ? m[j].mask : 0ull);
+    const bool on = __builtin_amdgcn_inverse_ballot_w64(mask[j]);
     const bool neg = __builtin_amdgcn_inverse_ballot_w64(m[j].sign);
-    int32_t c = neg ? rr - (int32_t)m[j].ad : rr + (int32_t)m[j].ad;
-    c = on ? c : rr;
-    xv[j] = ld_gather<R>(xin, c);
+    const uint32_t ad8 = m[j].ad * (uint32_t)sizeof(T);
+    uint32_t off = neg ? r8 - ad8 : r8 + ad8;
+    off = on ? off : QH_KS_OOB;
+    xv[j] = ks_load<R>(xr, off);
     if (VALS) {
-      uint32_t vi = m[j].base + (uint32_t)__popcll(m[j].mask & lt);
-      vi = on ? vi : m[j].base;
-      vv[j] = ld_stream<R>(val + vi);
+      // rank of the lane among the slot's active lanes
+      const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask[j] >> 32),
+                                                      __builtin_amdgcn_mbcnt_lo((uint32_t)mask[j], 0u));
+      vv[j] = ks_load<R>(vr, (m[j].base + rank) * (uint32_t)sizeof(T));
     }
   }
 #pragma unroll
   for (int j = 0; j < U; j++) {
-    const bool on = __builtin_amdgcn_inverse_ballot_w64((!TAIL || j < cnt) ? m[j].mask : 0ull);
-    const double xr = (double)(on ? xv[j].x : (R)0), xi = (double)(on ? xv[j].y : (R)0);
+    const double xre = (double)xv[j].x, xim = (double)xv[j].y;
     double ar, ai;
     if (VALS) {
       ar = (double)vv[j].x;
@@ -243,9 +264,21 @@ __device__ __forceinline__ void ks_pass(const qh_kmeta *__restrict__ mp, int cnt
       ar = m[j].vre;
       ai = m[j].vim;
     }
-    are = fma(ar, xr, fma(-ai, xi, are));
-    aim = fma(ar, xi, fma(ai, xr, aim));
+    are = fma(ar, xre, fma(-ai, xim, are));
+    aim = fma(ar, xim, fma(ai, xre, aim));
   }
+}
+
+// all slots [k, kend) of one group (uniform or not) of a slice
+template <typename R, bool VALS, int U>
+__device__ __forceinline__ void ks_group(const qh_kmeta *__restrict__ meta, int32_t k, int32_t kend, uint32_t r8,
+                                         __amdgpu_buffer_rsrc_t xr, __amdgpu_buffer_rsrc_t vr, double &are,
+                                         double &aim) {
+  static_assert(U - 1 <= QH_KS_PAD, "tail pass reads past the padding");
+  const qh_kmeta *mp = meta + k;
+  const int32_t n = kend - k;
+  for (int32_t p = 0; p < n / U; p++, mp += U) ks_pass<R, VALS, false, U>(mp, U, r8, xr, vr, are, aim);
+  if (n % U) ks_pass<R, VALS, true, U>(mp, n % U, r8, xr, vr, are, aim);
 }
 
 // Stage kernel on the sliced layout.  The stage input Y_s = x + c_s K_{s-1} is
@@ -261,7 +294,7 @@ __device__ __forceinline__ void ks_pass(const qh_kmeta *__restrict__ mp, int cnt
 template <typename R, int MODE, int U>
 __global__ void __launch_bounds__(256) k_ks_stage(
     int32_t nrows, int32_t nslices, const int32_t *__restrict__ moff, const int32_t *__restrict__ umid,
-    const qh_kmeta *__restrict__ meta, const typename C2<R>::T *__restrict__ val,
+    const qh_kmeta *__restrict__ meta, int32_t nval, const typename C2<R>::T *__restrict__ val,
     const typename C2<R>::T *__restrict__ xin, typename C2<R>::T *__restrict__ kout,
     typename C2<R>::T *__restrict__ ynext, R cn, const typename C2<R>::T *__restrict__ x,
     const typename C2<R>::T *__restrict__ k1, const typename C2<R>::T *__restrict__ k2,
@@ -273,15 +306,13 @@ __global__ void __launch_bounds__(256) k_ks_stage(
   if (s >= nslices) return;  // whole wavefront
   const int lane = threadIdx.x & 63;
   const int32_t r = s * 64 + lane;
-  const int32_t rr = r < nrows ? r : nrows - 1;  // inactive lanes of the last slice gather a valid row
-  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   const int32_t k0 = moff[s], km = umid[s], kend = moff[s + 1];
+  const __amdgpu_buffer_rsrc_t xr = ks_rsrc(xin, (int64_t)nrows * (int64_t)sizeof(T));
+  const __amdgpu_buffer_rsrc_t vr = ks_rsrc(val, (int64_t)nval * (int64_t)sizeof(T));
+  const uint32_t r8 = (uint32_t)r * (uint32_t)sizeof(T);
   double are = 0.0, aim = 0.0;
-  int32_t k = k0;
-  for (; k + U <= km; k += U) ks_pass<R, false, false, U>(meta + k, U, rr, lt, xin, val, are, aim);
-  if (k < km) ks_pass<R, false, true, U>(meta + k, km - k, rr, lt, xin, val, are, aim);
-  for (k = km; k + U <= kend; k += U) ks_pass<R, true, false, U>(meta + k, U, rr, lt, xin, val, are, aim);
-  if (k < kend) ks_pass<R, true, true, U>(meta + k, kend - k, rr, lt, xin, val, are, aim);
+  ks_group<R, false, U>(meta, k0, km, r8, xr, vr, are, aim);
+  ks_group<R, true, U>(meta, km, kend, r8, xr, vr, are, aim);
   if (r >= nrows) return;
   T y;
   y.x = (R)are;
@@ -442,8 +473,10 @@ int ks_build(qh_mat *A) {
   if (A->ks_state) return QH_OK;
   hipStream_t s = qh_stream();
   const int32_t ns = (int32_t)((A->rows + 63) / 64);
-  // the gathers address vectors with 32-bit byte offsets
-  if (A->rows * (int64_t)sizeof(double2) >= ((int64_t)1 << 32)) {
+  // the stage kernels address vectors and the value stream through buffer
+  // resources: byte offsets must stay below 2^31 (the value stream is checked
+  // once its size is known)
+  if (A->rows * (int64_t)sizeof(double2) >= ((int64_t)1 << 31)) {
     A->ks_state = -1;
     return QH_OK;
   }
@@ -478,7 +511,10 @@ int ks_build(qh_mat *A) {
   QH_HIP(hipMemcpyAsync(hn.data(), nvals, sizeof(int32_t) * ns, hipMemcpyDeviceToHost, s));
   QH_HIP(hipMemcpyAsync(hu.data(), nuni, sizeof(int32_t) * ns, hipMemcpyDeviceToHost, s));
   QH_HIP(hipStreamSynchronize(s));
-  if (hovf) {  // a row longer than the slice builder handles: keep the CSR kernels
+  int64_t nstored = 0;
+  for (int32_t i = 0; i < ns; i++) nstored += hn[i];
+  if (nstored * (int64_t)sizeof(double2) >= ((int64_t)1 << 31)) hovf = 1;
+  if (hovf) {  // a row longer than the slice builder handles, or too many stored values: keep the CSR kernels
     A->ks_state = -1;
     cleanup();
     qh_free_ptr(A->ks_moff);
@@ -501,7 +537,8 @@ int ks_build(qh_mat *A) {
   QH_HIP(hipMemcpyAsync(A->ks_moff, ho.data(), sizeof(int32_t) * (ns + 1), hipMemcpyHostToDevice, s));
   QH_HIP(hipMemcpyAsync(A->ks_umid, hm.data(), sizeof(int32_t) * ns, hipMemcpyHostToDevice, s));
   QH_HIP(hipMemcpyAsync(voff, hv.data(), sizeof(int32_t) * (ns + 1), hipMemcpyHostToDevice, s));
-  rc = qh_alloc_t(&A->ks_meta, (size_t)(tot > 0 ? tot : 1));
+  rc = qh_alloc_t(&A->ks_meta, (size_t)tot + QH_KS_PAD);
+  if (!rc) QH_HIP(hipMemsetAsync(A->ks_meta + tot, 0, sizeof(qh_kmeta) * QH_KS_PAD, s));
   if (!rc) rc = qh_alloc_t(&A->ks_val, (size_t)(totv > 0 ? totv : 1));
   if (rc) {
     cleanup();
@@ -539,7 +576,8 @@ int launch_ks(int G, hipStream_t s, const qh_mat *A, const void *xin, void *kout
   const dim3 grid(qh_blocks(A->nslices, 4)), blk(256);
   const int32_t n = (int32_t)A->rows, ns = A->nslices;
 #define QH_LK(UU)                                                                                           \
-  hipLaunchKernelGGL((k_ks_stage<R, MODE, UU>), grid, blk, 0, s, n, ns, A->ks_moff, A->ks_umid, A->ks_meta, kv, \
+  hipLaunchKernelGGL((k_ks_stage<R, MODE, UU>), grid, blk, 0, s, n, ns, A->ks_moff, A->ks_umid, A->ks_meta,     \
+                     (int32_t)A->ks_nval, kv,                                                               \
                      (const T *)xin, (T *)kout, (T *)ynext, cn, (const T *)x, (const T *)k1, (const T *)k2, \
                      (const T *)k3, (T *)xout, h, xlo)
   switch (QH_KSELL_UNROLL(G)) {
@@ -635,6 +673,9 @@ int autotune(qh_mat *A, int *out_g) {
     QH_HIP(hipEventSynchronize(e1));
     float ms = 0.f;
     QH_HIP(hipEventElapsedTime(&ms, e0, e1));
+    if (getenv("QH_AUTOTUNE_VERBOSE"))
+      fprintf(stderr, "qh autotune: %s rows %lld nnz %lld variant %d: %.2f us per SpMV\n",
+              sizeof(R) == 8 ? "fp64" : "fp32", (long long)A->rows, (long long)A->nnz, g, 1e3 * ms / reps);
     if (ms < best) {
       best = ms;
       bestg = g;
