@@ -209,7 +209,14 @@ void apply_gate2(qvec, gate_type, ...);                         /* quantum_circu
 /* steady state of a qsystem (solver.c:33-220 semantics; the reference has it
  * only for the legacy global API).  x receives rho_ss. */
 void steady_state_sys(qsystem, qvec);
-void steady_state_sys_opts(qsystem, qvec, PetscInt restart, PetscReal tol, PetscInt maxit, PetscBool jacobi,
+/* explicit solver settings: GMRES(restart) to ||b - A'x|| <= tol or maxit
+ * iterations with the right preconditioner precond (QUAC_SS_PC_*); x holds the
+ * initial guess on entry */
+enum { QUAC_SS_PC_NONE = 0, QUAC_SS_PC_JACOBI = 1, QUAC_SS_PC_HBASIS = 2 };
+/* the N x N effective Hamiltonian H - (i/2) sum a C^dag C of the time-independent
+ * terms as a device CSR (destroy with qh_mat_destroy); returns 0 on success */
+int quac_assemble_heff(qsystem, Mat *out);
+void steady_state_sys_opts(qsystem, qvec, PetscInt restart, PetscReal tol, PetscInt maxit, PetscInt precond,
                            PetscInt *iters, PetscReal *residual);
 
 /* ------------------------------------------------------ new in this library */

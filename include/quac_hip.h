@@ -170,6 +170,26 @@ int64_t qh_batch_offset(qh_batch *b, int i);
  */
 int qh_gmres(const qh_mat *A, int64_t tot, qh_vec *x, int restart, double tol, int64_t maxit,
              int jacobi, int64_t *iters, double *res);
+/*
+ * Hamiltonian-eigenbasis preconditioner of the steady-state solve (replaces the
+ * role of PETSc's PCASM/ILU(0) at solver.c:182-183; no operator-level
+ * counterpart in the reference).  heff is the N x N effective Hamiltonian
+ * H - (i/2) sum_k a_k C_k^dag C_k, assembled by qh_assemble with QH_HAM_ONLY
+ * (Hamiltonian) and QH_LIN_MCWF (Lindblad) terms and no flags.  M^-1 divides
+ * the Liouvillian's no-jump part in the eigenbasis of the Hermitian part of heff.
+ */
+typedef struct qh_pc qh_pc;
+int qh_pc_create_hbasis(const qh_mat *heff, qh_pc **out);
+/* out = M^-1 in, both FP64 vectors of N*N entries (vec layout of an N x N matrix) */
+int qh_pc_apply(const qh_pc *pc, const qh_vec *in, qh_vec *out);
+int qh_pc_destroy(qh_pc *pc);
+/* qh_gmres with a chosen right preconditioner: pc_kind 0 none, 1 Jacobi,
+ * 2 the Hamiltonian-eigenbasis operator pc */
+#define QH_PC_NONE 0
+#define QH_PC_JACOBI 1
+#define QH_PC_HBASIS 2
+int qh_gmres_pc(const qh_mat *A, int64_t tot, qh_vec *x, int restart, double tol, int64_t maxit,
+                int pc_kind, const qh_pc *pc, int64_t *iters, double *res);
 
 /* ------------------------------------------------------------- timing hooks */
 /* average duration (ms) of the last qh_rk4 stage kernels, measured with events */

@@ -141,7 +141,7 @@ def _setup(L):
     sig("apply_circuit_to_qvec", None, vp, CircuitStruct, vp)
     sig("apply_gate2_list", None, vp, i32, P(i64), dbl, dbl, dbl, vp)
     sig("apply_gate_sys_list", None, vp, vp, i32, P(i64), dbl, dbl, dbl, vp)
-    sig("steady_state_sys_opts", None, vp, vp, i64, dbl, i64, i32, P(i64), P(dbl))
+    sig("steady_state_sys_opts", None, vp, vp, i64, dbl, i64, i64, P(i64), P(dbl))
     sig("quac_system_matrix", vp, vp)
     sig("quac_system_td_matrix", vp, vp, i64)
     sig("quac_system_total_levels", i64, vp)
@@ -174,6 +174,11 @@ def _setup(L):
     sig("qh_last_stage_ms", i32, P(dbl), P(i64))
     sig("qh_set_graph_steps", i32, i32)
     sig("qh_bench_spmv", i32, vp, i32, i32, P(dbl))
+    sig("qh_pc_create_hbasis", i32, vp, P(vp))
+    sig("qh_pc_apply", i32, vp, vp, vp)
+    sig("qh_pc_destroy", i32, vp)
+    sig("qh_gmres_pc", i32, vp, i64, vp, i32, dbl, i64, i32, vp, P(i64), P(dbl))
+    sig("quac_assemble_heff", i32, vp, P(vp))
     sig("qh_timer_begin", i32)
     sig("qh_timer_end", i32, P(dbl))
     sig("qh_trace", i32, vp, i64, P(dbl), P(dbl))
@@ -600,10 +605,14 @@ class System:
         lib().quac_ts_stats(ctypes.byref(s), ctypes.byref(t))
         return s.value, t.value
 
-    def steady_state(self, x, restart=100, tol=1e-11, maxit=10000, jacobi=True):
+    def steady_state(self, x, restart=100, tol=1e-11, maxit=10000, precond=None, jacobi=None):
+        """GMRES steady state into x (x is the initial guess); precond: "hbasis"
+        (default), "jacobi" or "none"; jacobi=True/False is the older spelling"""
+        if precond is None:
+            precond = "hbasis" if jacobi is None else ("jacobi" if jacobi else "none")
+        pc = {"none": 0, "jacobi": 1, "hbasis": 2}[precond]
         it, res = ctypes.c_int64(), ctypes.c_double()
-        lib().steady_state_sys_opts(self.h, x.h, restart, tol, maxit, int(jacobi), ctypes.byref(it),
-                                    ctypes.byref(res))
+        lib().steady_state_sys_opts(self.h, x.h, restart, tol, maxit, pc, ctypes.byref(it), ctypes.byref(res))
         _check()
         return it.value, res.value
 
@@ -728,3 +737,28 @@ class StreamTimer:
             _qh(lib().qh_timer_end(ctypes.byref(ms)))
             self.ms = ms.value
         return False
+
+
+class HBasisPC:
+    """Hamiltonian-eigenbasis preconditioner of a system's steady-state solve"""
+
+    def __init__(self, system):
+        m = ctypes.c_void_p()
+        rc = lib().quac_assemble_heff(system.h, ctypes.byref(m))
+        _check()
+        if rc:
+            raise QuacError(lib().quac_last_error().decode())
+        try:
+            pc = ctypes.c_void_p()
+            _qh(lib().qh_pc_create_hbasis(m, ctypes.byref(pc)))
+            self.h = pc
+        finally:
+            lib().qh_mat_destroy(m)
+
+    def apply(self, vin, vout):
+        _qh(lib().qh_pc_apply(self.h, vin.h, vout.h))
+
+    def destroy(self):
+        if self.h:
+            lib().qh_pc_destroy(self.h)
+            self.h = None

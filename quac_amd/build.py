@@ -58,7 +58,7 @@ def build(verbose=False, jobs=None):
             if verbose and out.strip():
                 print(out)
     tmp = LIB + ".tmp"
-    _compile([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs + ["-lm"], "link")
+    _compile([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs + ["-lrocsolver", "-lrocblas", "-lm"], "link")
     os.replace(tmp, LIB)
     return LIB
 

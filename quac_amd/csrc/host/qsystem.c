@@ -306,9 +306,15 @@ static int term_kind(qsystem s, mat_term_type t) {
 }
 
 /* append one term's descriptor (MCWF Lindblad: reversed daggers then ops, kron.c:675-686) */
+static int push_term_kind(qsystem s, qc_chain *c, qh_term **terms, int *nt, int *cap, mat_term *mt, int zero_a,
+                          double are, double aim, int kind);
 static int push_term(qsystem s, qc_chain *c, qh_term **terms, int *nt, int *cap, mat_term *mt, int zero_a,
                      double are, double aim) {
-  int kind = term_kind(s, mt->my_term_type);
+  return push_term_kind(s, c, terms, nt, cap, mt, zero_a, are, aim, term_kind(s, mt->my_term_type));
+}
+
+static int push_term_kind(qsystem s, qc_chain *c, qh_term **terms, int *nt, int *cap, mat_term *mt, int zero_a,
+                          double are, double aim, int kind) {
   if (kind < 0) return quac_fail(0, "Wrong mat_term_type in construct_matrix - ham!"), -1;
   int op0 = c->nops;
   int n;
@@ -360,6 +366,29 @@ static int assemble(qsystem s, int td_index, Mat *out) {
     if (quac_opt_int("quac_kron_conj_a", 0)) flags |= QH_ASM_CONJ_A;
     int r = qh_assemble(s->total_levels, s->dim, nt, terms, c.nops, c.ops, c.nlev, c.lev, flags, out);
     if (r) rc = quac_fail(9, "device assembly failed: %s", qh_last_error());
+  }
+  free(terms);
+  qc_chain_free(&c);
+  return rc;
+}
+
+/* The N x N effective Hamiltonian H - (i/2) sum a C^dag C of the time-independent
+ * terms (the MCWF assembly of kron.c:638-700 without the -i scale), used by the
+ * steady-state preconditioner. */
+int quac_assemble_heff(qsystem s, Mat *out) {
+  qc_chain c;
+  qc_chain_init(&c);
+  qh_term *terms = NULL;
+  int nt = 0, cap = 0, rc = 0;
+  for (PetscInt i = 0; i < s->num_time_indep && !rc; i++) {
+    mat_term *mt = &s->time_indep[i];
+    int lin = (mt->my_term_type == LINDBLAD || mt->my_term_type == TD_LINDBLAD);
+    rc = push_term_kind(s, &c, &terms, &nt, &cap, mt, 0, creal(mt->a), cimag(mt->a), lin ? QH_LIN_MCWF : QH_HAM_ONLY);
+  }
+  if (!rc && nt == 0) rc = quac_fail(0, "no time-independent terms for the effective Hamiltonian");
+  if (!rc) {
+    int r = qh_assemble(s->total_levels, s->total_levels, nt, terms, c.nops, c.ops, c.nlev, c.lev, 0, out);
+    if (r) rc = quac_fail(9, "device assembly of the effective Hamiltonian failed: %s", qh_last_error());
   }
   free(terms);
   qc_chain_free(&c);

@@ -4,30 +4,52 @@
  * API; here it works on a qsystem: the trace row is added to row 0 of the
  * Liouvillian, b = e0, GMRES(restart) with rtol 1e-11 by default
  * (solver.c:13-15), on the device in FP64.  The initial guess is whatever the
- * qvec holds (zero for a fresh qvec, matching the reference's effective x=0). */
+ * qvec holds (zero for a fresh qvec, matching the reference's effective x=0).
+ * The reference preconditions with PCASM/ILU(0) (solver.c:182-183); here the
+ * default is the Hamiltonian-eigenbasis operator of qh_precond.hip (Jacobi and
+ * none are selectable). */
 #include <math.h>
 
 #include "quac_internal.h"
 
-void steady_state_sys_opts(qsystem qsys, qvec x, PetscInt restart, PetscReal tol, PetscInt maxit, PetscBool jacobi,
+void steady_state_sys_opts(qsystem qsys, qvec x, PetscInt restart, PetscReal tol, PetscInt maxit, PetscInt precond,
                            PetscInt *iters, PetscReal *residual) {
   if (!qsys->mat_allocated) QUAC_FAIL(0, "Matrix does not seem to be constructed! You need to call construct_matrix!");
   if (!qsys->dm_equations || qsys->mcwf_solver)
     QUAC_FAIL(0, "Steady state is only supported for density-matrix (Lindblad) systems");
   if (x->my_type != DENSITY_MATRIX || x->n != qsys->dim) QUAC_FAIL(0, "steady_state_sys needs a density matrix");
+  if (precond < QUAC_SS_PC_NONE || precond > QUAC_SS_PC_HBASIS) QUAC_FAIL(0, "unknown steady-state preconditioner");
   int64_t it = 0;
   double res = 0.0;
-  int rc = qh_gmres(qsys->mat_A, qsys->total_levels, x->data, (int)restart, tol, maxit, jacobi ? 1 : 0, &it, &res);
+  int rc = 0;
+  if (precond == QUAC_SS_PC_HBASIS) {
+    /* the eigenbasis of the effective Hamiltonian's Hermitian part (rebuilt per
+     * solve: the terms may have changed since the last one) */
+    Mat heff = NULL;
+    qh_pc *pc = NULL;
+    if (quac_assemble_heff(qsys, &heff)) return;
+    rc = qh_pc_create_hbasis(heff, &pc);
+    qh_mat_destroy(heff);
+    if (!rc) rc = qh_gmres_pc(qsys->mat_A, qsys->total_levels, x->data, (int)restart, tol, maxit, QH_PC_HBASIS, pc,
+                              &it, &res);
+    qh_pc_destroy(pc);
+  } else {
+    rc = qh_gmres_pc(qsys->mat_A, qsys->total_levels, x->data, (int)restart, tol, maxit, (int)precond, NULL, &it,
+                     &res);
+  }
   if (rc) QUAC_FAIL(9, "steady state solve failed: %s", qh_last_error());
   if (iters) *iters = it;
   if (residual) *residual = res;
   if (quac_opt_int("quac_verbose", 0)) PetscPrintf(0, "Iterations %lld residual %e\n", (long long)it, res);
 }
 
+/* defaults: restart 100, rtol 1e-11, at most 10000 iterations (solver.c:13-15,
+ * 186-187); the preconditioner is the Hamiltonian-eigenbasis operator unless
+ * -quac_ss_pc selects 0 (none) or 1 (Jacobi) */
 void steady_state_sys(qsystem qsys, qvec x) {
   PetscInt restart = quac_opt_int("ksp_gmres_restart", 100);
   PetscReal tol = quac_opt_double("ksp_rtol", 1e-11);
   PetscInt maxit = quac_opt_int("ksp_max_it", 10000);
-  PetscBool jac = quac_opt_int("quac_jacobi", 1) ? PETSC_TRUE : PETSC_FALSE;
-  steady_state_sys_opts(qsys, x, restart, tol, maxit, jac, NULL, NULL);
+  PetscInt pc = quac_opt_int("quac_ss_pc", QUAC_SS_PC_HBASIS);
+  steady_state_sys_opts(qsys, x, restart, tol, maxit, pc, NULL, NULL);
 }

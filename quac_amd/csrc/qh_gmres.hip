@@ -8,8 +8,9 @@
 // streaming kernels per pass (all dots of a pass in one launch, all updates in
 // one launch), the Hessenberg column stays on the device until the single
 // device-to-host copy per iteration; Givens rotations and the triangular solve
-// run on the host (they are O(restart^2)).  An optional Jacobi right
-// preconditioner scales by the inverse diagonal.
+// run on the host (they are O(restart^2)).  Right preconditioners: none, the
+// inverse diagonal (Jacobi), or the Hamiltonian-eigenbasis operator of
+// qh_precond.hip.
 #include <cmath>
 #include <complex>
 #include <vector>
@@ -164,6 +165,12 @@ __global__ void k_update(double2 *x, const double2 *V, int64_t ld, int m, const 
   }
 }
 
+// y += x
+__global__ void k_axpy1(const double2 *x, double2 *y, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    y[i] = make_double2(y[i].x + x[i].x, y[i].y + x[i].y);
+}
+
 unsigned gr(int64_t n) {
   int64_t b = (n + 255) / 256;
   if (b > 4096) b = 4096;
@@ -173,57 +180,63 @@ unsigned gr(int64_t n) {
 
 }  // namespace
 
-extern "C" int qh_gmres(const qh_mat *A, int64_t tot, qh_vec *x, int restart, double tol, int64_t maxit, int jacobi,
-                        int64_t *iters, double *res) {
+int qh_pc_apply_dev(const qh_pc *pc, const double2 *v, double2 *z);
+int64_t qh_pc_dim(const qh_pc *pc);
+
+// Restarted right-preconditioned GMRES on A' M^-1 (M: none, Jacobi or the
+// Hamiltonian-eigenbasis operator).  Each cycle starts from the true residual
+// r = e0 - A'x of the caller's x, builds the Krylov basis of A' M^-1 from
+// r/||r||, and ends with x += M^-1 (V y).
+static int gmres_impl(const qh_mat *A, int64_t tot, qh_vec *x, int restart, double tol, int64_t maxit, int pck,
+                      const qh_pc *pc, int64_t *iters, double *res) {
   int rc = qh_ensure_init();
   if (rc) return rc;
   QH_REQUIRE(A && x && x->prec == QH_FP64 && x->n == A->rows && tot * tot == A->rows, QH_EINVAL,
              "qh_gmres: FP64 density-matrix vector of the matrix size required");
   QH_REQUIRE(restart >= 1 && restart <= 1000, QH_EINVAL, "qh_gmres: restart in [1,1000]");
+  QH_REQUIRE(pck >= 0 && pck <= 2, QH_EINVAL, "qh_gmres: preconditioner kind must be 0, 1 or 2");
+  QH_REQUIRE(pck != 2 || (pc && qh_pc_dim(pc) == tot), QH_EINVAL,
+             "qh_gmres: the Hamiltonian-eigenbasis preconditioner must match the Hilbert dimension");
   const int64_t n = A->rows;
   const int R = restart;
   hipStream_t s = qh_stream();
-  double2 *V = nullptr, *w = nullptr, *z = nullptr, *u = nullptr, *dinv = nullptr, *part = nullptr, *h = nullptr,
+  double2 *V = nullptr, *w = nullptr, *z = nullptr, *t = nullptr, *dinv = nullptr, *part = nullptr, *h = nullptr,
           *yv = nullptr;
-  qh_vec zv, wv;
+  double2 *xd = (double2 *)x->d;
   rc = qh_alloc_t(&V, (size_t)n * (R + 1));
   if (!rc) rc = qh_alloc_t(&w, n);
   if (!rc) rc = qh_alloc_t(&z, n);
-  if (!rc) rc = qh_alloc_t(&u, n);
+  if (!rc) rc = qh_alloc_t(&t, n);
   if (!rc) rc = qh_alloc_t(&part, (size_t)DOT_BLOCKS * (R + 2));
   if (!rc) rc = qh_alloc_t(&h, 2 * (R + 2));
   if (!rc) rc = qh_alloc_t(&yv, R + 1);
-  if (!rc && jacobi) rc = qh_alloc_t(&dinv, n);
+  if (!rc && pck == 1) rc = qh_alloc_t(&dinv, n);
   if (rc) goto done;
   {
-    zv.n = n;
-    zv.prec = QH_FP64;
-    zv.d = z;
-    wv.n = n;
-    wv.prec = QH_FP64;
-    wv.d = w;
-    if (jacobi) {
+    if (pck == 1)
       hipLaunchKernelGGL(k_diag_inv, dim3(gr(n)), dim3(256), 0, s, A->rowptr, A->col, A->val, n, dinv);
-      hipLaunchKernelGGL(k_cdiv_vec, dim3(gr(n)), dim3(256), 0, s, dinv, (const double2 *)x->d, u, n);
-    } else {
-      QH_HIP(hipMemcpyAsync(u, x->d, 16 * n, hipMemcpyDeviceToDevice, s));
-    }
-    // y = A' M^-1 v into out (uses z as scratch)
-    auto matvec = [&](const double2 *v, double2 *out) -> int {
-      const double2 *src = v;
-      if (jacobi) {
-        hipLaunchKernelGGL(k_cmul_vec, dim3(gr(n)), dim3(256), 0, s, dinv, v, z, n);
-        src = z;
+    // out = M^-1 v
+    auto precond = [&](const double2 *v, double2 *out) -> int {
+      if (pck == 1) {
+        hipLaunchKernelGGL(k_cmul_vec, dim3(gr(n)), dim3(256), 0, s, dinv, v, out, n);
+        QH_HIP(hipGetLastError());
+        return QH_OK;
       }
+      if (pck == 2) return qh_pc_apply_dev(pc, v, out);
+      QH_HIP(hipMemcpyAsync(out, v, 16 * n, hipMemcpyDeviceToDevice, s));
+      return QH_OK;
+    };
+    // out = A' v  (A plus the trace functional on row 0)
+    auto apply_a = [&](const double2 *v, double2 *out) -> int {
       qh_vec iv, ov;
       iv.n = ov.n = n;
       iv.prec = ov.prec = QH_FP64;
-      iv.d = const_cast<double2 *>(src);
+      iv.d = const_cast<double2 *>(v);
       ov.d = out;
       int r2 = qh_spmv(A, &iv, &ov);
       iv.d = ov.d = nullptr;
       if (r2) return r2;
-      hipLaunchKernelGGL(k_add_trace_row0, dim3(1), dim3(256), 0, s, src, tot, out);
+      hipLaunchKernelGGL(k_add_trace_row0, dim3(1), dim3(256), 0, s, v, tot, out);
       QH_HIP(hipGetLastError());
       return QH_OK;
     };
@@ -242,7 +255,7 @@ extern "C" int qh_gmres(const qh_mat *A, int64_t tot, qh_vec *x, int restart, do
     std::vector<cd> H((size_t)(R + 1) * R), cs(R), sn(R), g(R + 1), y(R);
     std::vector<double2> hh(2 * (R + 2));
     for (;;) {
-      rc = matvec(u, w);
+      rc = apply_a(xd, w);
       if (rc) break;
       hipLaunchKernelGGL(k_resid, dim3(gr(n)), dim3(256), 0, s, w, n);
       norm2(w, h);
@@ -256,9 +269,9 @@ extern "C" int qh_gmres(const qh_mat *A, int64_t tot, qh_vec *x, int restart, do
       std::fill(g.begin(), g.end(), cd(0.0, 0.0));
       g[0] = rn;
       int k = 0;
-      bool conv = false;
       for (; k < R && it < maxit; k++, it++) {
-        rc = matvec(V + n * k, w);
+        rc = precond(V + n * k, z);
+        if (!rc) rc = apply_a(z, w);
         if (rc) break;
         // CGS2: two passes of (all dots, all updates); h[0..k] and h[R+2 .. R+2+k]
         dots(k + 1, w, h);
@@ -294,44 +307,52 @@ extern "C" int qh_gmres(const qh_mat *A, int64_t tot, qh_vec *x, int restart, do
         if (std::abs(g[k + 1]) <= tol) {
           k++;
           it++;
-          conv = true;
           break;
         }
       }
       if (rc) break;
-      (void)conv;
       for (int i = k - 1; i >= 0; i--) {
         cd sacc = g[i];
         for (int jj = i + 1; jj < k; jj++) sacc -= H[(size_t)i * R + jj] * y[jj];
         y[i] = sacc / H[(size_t)i * R + i];
       }
-      std::vector<double2> yh(k > 0 ? k : 1);
-      for (int i = 0; i < k; i++) yh[i] = make_double2(y[i].real(), y[i].imag());
       if (k > 0) {
+        // x += M^-1 (V y)
+        std::vector<double2> yh(k);
+        for (int i = 0; i < k; i++) yh[i] = make_double2(y[i].real(), y[i].imag());
         QH_HIP(hipMemcpyAsync(yv, yh.data(), sizeof(double2) * k, hipMemcpyHostToDevice, s));
-        hipLaunchKernelGGL(k_update, dim3(gr(n)), dim3(256), 0, s, u, V, n, k, yv, n);
+        QH_HIP(hipMemsetAsync(t, 0, 16 * n, s));
+        hipLaunchKernelGGL(k_update, dim3(gr(n)), dim3(256), 0, s, t, V, n, k, yv, n);
+        rc = precond(t, z);
+        if (rc) break;
+        hipLaunchKernelGGL(k_axpy1, dim3(gr(n)), dim3(256), 0, s, z, xd, n);
+        QH_HIP(hipStreamSynchronize(s));
       }
     }
     if (!rc) {
-      if (jacobi)
-        hipLaunchKernelGGL(k_cmul_vec, dim3(gr(n)), dim3(256), 0, s, dinv, u, (double2 *)x->d, n);
-      else
-        QH_HIP(hipMemcpyAsync(x->d, u, 16 * n, hipMemcpyDeviceToDevice, s));
       QH_HIP(hipStreamSynchronize(s));
       if (iters) *iters = it;
       if (res) *res = rn;
     }
-    zv.d = nullptr;
-    wv.d = nullptr;
   }
 done:
   qh_free_ptr(V);
   qh_free_ptr(w);
   qh_free_ptr(z);
-  qh_free_ptr(u);
+  qh_free_ptr(t);
   qh_free_ptr(dinv);
   qh_free_ptr(part);
   qh_free_ptr(h);
   qh_free_ptr(yv);
   return rc;
+}
+
+extern "C" int qh_gmres(const qh_mat *A, int64_t tot, qh_vec *x, int restart, double tol, int64_t maxit, int jacobi,
+                        int64_t *iters, double *res) {
+  return gmres_impl(A, tot, x, restart, tol, maxit, jacobi ? 1 : 0, nullptr, iters, res);
+}
+
+extern "C" int qh_gmres_pc(const qh_mat *A, int64_t tot, qh_vec *x, int restart, double tol, int64_t maxit,
+                           int pc_kind, const qh_pc *pc, int64_t *iters, double *res) {
+  return gmres_impl(A, tot, x, restart, tol, maxit, pc_kind, pc, iters, res);
 }

@@ -1,0 +1,311 @@
+// qh_precond.hip -- the Hamiltonian-eigenbasis preconditioner of the
+// steady-state GMRES solve.
+//
+// The Liouvillian of a density-matrix system splits as L = L0 + J, where
+//   L0 rho = -i Heff rho + i rho Heff^dag,   Heff = H - (i/2) sum_k a_k C_k^dag C_k
+// is the no-jump part and J rho = sum_k a_k C_k rho C_k^dag the jumps.  In the
+// column-major vec layout L0 is the Kronecker sum I (x) (-i Heff) + conj(Heff) (x) I,
+// so with Heff ~= W (E + i diag(g)) W^dag, W the eigenvectors of the Hermitian part
+// of Heff and g_i = <w_i| (Heff - Heff^dag)/(2i) |w_i>, the operator
+//   M rho = W [ (W^dag rho W) .* Dm ] W^dag,   Dm_ij = (g_i + g_j) - i (E_i - E_j)
+// approximates L0 and is inverted exactly by dividing by Dm instead.  GMRES is
+// then run on L' M^-1 (right preconditioning): the jump part and the dropped
+// off-diagonal decay couplings are what is left for the Krylov iteration.
+//
+// Set-up: Heff is assembled on the device as an N x N CSR by the caller,
+// densified, split into Hermitian / anti-Hermitian parts, and the Hermitian part
+// diagonalised with rocSOLVER (zheevd).  Each application costs four N^3
+// complex GEMMs (rocBLAS) and one elementwise pass.
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "qh_common.h"
+
+struct qh_pc {
+  int64_t n = 0;             // Hilbert dimension N (vectors have N*N entries)
+  double2 *W = nullptr;      // eigenvectors, N x N column-major
+  double2 *D = nullptr;      // denominators Dm, N x N column-major
+  double2 *T = nullptr;      // workspace N x N
+  rocblas_handle blas = nullptr;
+};
+
+namespace {
+
+#define QH_RB(call)                                                                   \
+  do {                                                                                \
+    rocblas_status st_ = (call);                                                      \
+    if (st_ != rocblas_status_success) {                                              \
+      qh_set_error("%s:%d %s: rocBLAS status %d", __FILE__, __LINE__, #call, (int)st_); \
+      return QH_EHIP;                                                                 \
+    }                                                                                 \
+  } while (0)
+
+unsigned grid1(int64_t n) {
+  int64_t b = (n + 255) / 256;
+  if (b > 8192) b = 8192;
+  return (unsigned)(b < 1 ? 1 : b);
+}
+
+__global__ void k_fill0(double2 *a, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    a[i] = make_double2(0.0, 0.0);
+}
+
+// dense column-major copy of an N x N CSR matrix
+__global__ void k_densify(const int32_t *rp, const int32_t *col, const double2 *val, int64_t n, double2 *A) {
+  const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  for (int32_t k = rp[r]; k < rp[r + 1]; k++) A[r + n * (int64_t)col[k]] = val[k];
+}
+
+// Hermitian part (A + A^dag)/2 into Hh, anti-Hermitian part (A - A^dag)/(2i) into Ha
+__global__ void k_split_herm(const double2 *A, int64_t n, double2 *Hh, double2 *Ha) {
+  const int64_t total = n * n;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = t % n, j = t / n;
+    const double2 a = A[i + n * j], b = A[j + n * i];  // b^* = (A^dag)_ij
+    Hh[t] = make_double2(0.5 * (a.x + b.x), 0.5 * (a.y - b.y));
+    // (a - conj(b)) / (2i) = ((a.y + b.y) - i (a.x - b.x)) / 2
+    Ha[t] = make_double2(0.5 * (a.y + b.y), -0.5 * (a.x - b.x));
+  }
+}
+
+// g_i = Re sum_r conj(W[r,i]) T[r,i]   (T = Ha W); one block per column
+__global__ void __launch_bounds__(256) k_coldot(const double2 *W, const double2 *T, int64_t n, double *g) {
+  __shared__ double red[256];
+  const int64_t i = blockIdx.x;
+  double acc = 0.0;
+  for (int64_t r = threadIdx.x; r < n; r += 256) {
+    const double2 w = W[r + n * i], t = T[r + n * i];
+    acc += w.x * t.x + w.y * t.y;
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) g[i] = red[0];
+}
+
+// Dm_ij = (g_i + g_j) - i (E_i - E_j), with |Dm| floored at floor_ (kept on the
+// stable side: a non-positive real part)
+__global__ void k_denoms(const double *E, const double *g, int64_t n, double floor_, double2 *D) {
+  const int64_t total = n * n;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = t % n, j = t / n;
+    double re = g[i] + g[j], im = -(E[i] - E[j]);
+    if (re * re + im * im < floor_ * floor_) re = -floor_;
+    D[t] = make_double2(re, im);
+  }
+}
+
+__global__ void k_cdiv_inplace(double2 *a, const double2 *d, int64_t total) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+    const double2 x = a[t], y = d[t];
+    const double den = y.x * y.x + y.y * y.y;
+    a[t] = make_double2((x.x * y.x + x.y * y.y) / den, (x.y * y.x - x.x * y.y) / den);
+  }
+}
+
+// flag[0] = 1 if any eigenvalue or eigenvector entry is not finite
+__global__ void k_nonfinite(const double2 *W, const double *E, int64_t n, int *flag) {
+  const int64_t total = n * n;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+    const double2 w = W[t];
+    if (!isfinite(w.x) || !isfinite(w.y) || (t < n && !isfinite(E[t]))) flag[0] = 1;
+  }
+}
+
+// flag[0] = 1 if W^dag W (in T) departs from the identity by more than tol
+__global__ void k_orth_check(const double2 *T, int64_t n, double tol, int *flag) {
+  const int64_t total = n * n;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+    const double2 v = T[t];
+    const double re = v.x - ((t % n) == (t / n) ? 1.0 : 0.0);
+    if (!(fabs(re) <= tol && fabs(v.y) <= tol)) flag[0] = 1;
+  }
+}
+
+const rocblas_double_complex ONE(1.0, 0.0), ZERO(0.0, 0.0);
+
+}  // namespace
+
+extern "C" int qh_pc_destroy(qh_pc *pc) {
+  if (!pc) return QH_OK;
+  qh_free_ptr(pc->W);
+  qh_free_ptr(pc->D);
+  qh_free_ptr(pc->T);
+  if (pc->blas) rocblas_destroy_handle(pc->blas);
+  delete pc;
+  return QH_OK;
+}
+
+extern "C" int qh_pc_create_hbasis(const qh_mat *heff, qh_pc **out) {
+  int rc = qh_ensure_init();
+  if (rc) return rc;
+  QH_REQUIRE(heff && out && heff->rows > 0 && heff->rows <= 16384, QH_EINVAL,
+             "qh_pc_create_hbasis: N x N effective Hamiltonian with N in [1, 16384] required");
+  *out = nullptr;
+  const int64_t n = heff->rows, nn = n * n;
+  hipStream_t s = qh_stream();
+  qh_pc *pc = new qh_pc();
+  pc->n = n;
+  double2 *A = nullptr, *Ha = nullptr;
+  double *E = nullptr, *work = nullptr, *g = nullptr;
+  rocblas_int *info = nullptr;
+  rc = qh_alloc_t(&pc->W, nn);
+  if (!rc) rc = qh_alloc_t(&pc->D, nn);
+  if (!rc) rc = qh_alloc_t(&pc->T, nn);
+  if (!rc) rc = qh_alloc_t(&A, nn);
+  if (!rc) rc = qh_alloc_t(&Ha, nn);
+  if (!rc) rc = qh_alloc_t(&E, n);
+  if (!rc) rc = qh_alloc_t(&work, n);
+  if (!rc) rc = qh_alloc_t(&g, n);
+  if (!rc) rc = qh_alloc_t(&info, 2);
+  auto cleanup = [&]() {
+    qh_free_ptr(A);
+    qh_free_ptr(Ha);
+    qh_free_ptr(E);
+    qh_free_ptr(work);
+    qh_free_ptr(g);
+    qh_free_ptr(info);
+  };
+  if (rc) {
+    cleanup();
+    qh_pc_destroy(pc);
+    return rc;
+  }
+  auto fail = [&](int code) {
+    cleanup();
+    qh_pc_destroy(pc);
+    return code;
+  };
+  if (rocblas_create_handle(&pc->blas) != rocblas_status_success) {
+    qh_set_error("rocblas_create_handle failed");
+    return fail(QH_EHIP);
+  }
+  if (rocblas_set_stream(pc->blas, s) != rocblas_status_success) {
+    qh_set_error("rocblas_set_stream failed");
+    return fail(QH_EHIP);
+  }
+  hipLaunchKernelGGL(k_fill0, dim3(grid1(nn)), dim3(256), 0, s, A, nn);
+  hipLaunchKernelGGL(k_densify, dim3(grid1(n)), dim3(256), 0, s, heff->rowptr, heff->col, heff->val, n, A);
+  hipLaunchKernelGGL(k_split_herm, dim3(grid1(nn)), dim3(256), 0, s, A, n, pc->W, Ha);
+  if (hipGetLastError() != hipSuccess) {
+    qh_set_error("qh_pc_create_hbasis: set-up kernel launch failed");
+    return fail(QH_EHIP);
+  }
+  // Hermitian eigensolver: divide and conquer first; if it reports failure or
+  // returns non-finite values, the Jacobi and then the QR-iteration solvers
+  // are tried on a fresh copy of the Hermitian part (kept in A)
+  {
+    int *flag = nullptr;
+    if (qh_alloc_t(&flag, 1)) return fail(QH_ENOMEM);
+    QH_HIP(hipMemcpyAsync(A, pc->W, sizeof(double2) * nn, hipMemcpyDeviceToDevice, s));
+    int ok = 0;
+    for (int method = 0; method < 3 && !ok; method++) {
+      if (method > 0) QH_HIP(hipMemcpyAsync(pc->W, A, sizeof(double2) * nn, hipMemcpyDeviceToDevice, s));
+      rocblas_status st;
+      rocblas_double_complex *Wd = (rocblas_double_complex *)pc->W;
+      if (method == 0) {
+        st = rocsolver_zheevd(pc->blas, rocblas_evect_original, rocblas_fill_lower, (rocblas_int)n, Wd,
+                              (rocblas_int)n, E, work, info);
+      } else if (method == 1) {
+        rocblas_int sweeps = 0;
+        st = rocsolver_zheevj(pc->blas, rocblas_esort_ascending, rocblas_evect_original, rocblas_fill_lower,
+                              (rocblas_int)n, Wd, (rocblas_int)n, 0.0, work, 100, info + 1, E, info);
+        (void)sweeps;
+      } else {
+        st = rocsolver_zheev(pc->blas, rocblas_evect_original, rocblas_fill_lower, (rocblas_int)n, Wd,
+                             (rocblas_int)n, E, work, info);
+      }
+      if (st != rocblas_status_success) continue;
+      int hinfo0 = 0, hflag = 0;
+      QH_HIP(hipMemsetAsync(flag, 0, sizeof(int), s));
+      hipLaunchKernelGGL(k_nonfinite, dim3(grid1(nn)), dim3(256), 0, s, pc->W, E, n, flag);
+      if (rocblas_zgemm(pc->blas, rocblas_operation_conjugate_transpose, rocblas_operation_none, (rocblas_int)n,
+                        (rocblas_int)n, (rocblas_int)n, &ONE, Wd, (rocblas_int)n, Wd, (rocblas_int)n, &ZERO,
+                        (rocblas_double_complex *)pc->T, (rocblas_int)n) != rocblas_status_success)
+        continue;
+      hipLaunchKernelGGL(k_orth_check, dim3(grid1(nn)), dim3(256), 0, s, pc->T, n, 1e-8, flag);
+      QH_HIP(hipMemcpyAsync(&hinfo0, info, sizeof(int), hipMemcpyDeviceToHost, s));
+      QH_HIP(hipMemcpyAsync(&hflag, flag, sizeof(int), hipMemcpyDeviceToHost, s));
+      QH_HIP(hipStreamSynchronize(s));
+      ok = (hinfo0 == 0 && hflag == 0);
+    }
+    qh_free_ptr(flag);
+    if (!ok) {
+      qh_set_error("qh_pc_create_hbasis: no Hermitian eigensolver produced a finite eigenbasis");
+      return fail(QH_EHIP);
+    }
+  }
+  // g_i = <w_i| Ha |w_i>
+  if (rocblas_zgemm(pc->blas, rocblas_operation_none, rocblas_operation_none, (rocblas_int)n, (rocblas_int)n,
+                    (rocblas_int)n, &ONE, (const rocblas_double_complex *)Ha, (rocblas_int)n,
+                    (const rocblas_double_complex *)pc->W, (rocblas_int)n, &ZERO, (rocblas_double_complex *)A,
+                    (rocblas_int)n) != rocblas_status_success) {
+    qh_set_error("rocblas_zgemm failed");
+    return fail(QH_EHIP);
+  }
+  hipLaunchKernelGGL(k_coldot, dim3((unsigned)n), dim3(256), 0, s, pc->W, A, n, g);
+  int hinfo = 0;
+  std::vector<double> hE(n), hg(n);
+  if (hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipMemcpyAsync(hE.data(), E, sizeof(double) * n, hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipMemcpyAsync(hg.data(), g, sizeof(double) * n, hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess) {
+    qh_set_error("qh_pc_create_hbasis: device copy failed");
+    return fail(QH_EHIP);
+  }
+  if (hinfo != 0) {
+    qh_set_error("rocsolver_zheevd did not converge (info %d)", hinfo);
+    return fail(QH_EHIP);
+  }
+  // floor for |Dm|: a small fraction of the operator's scale
+  double scale = 0.0;
+  for (int64_t i = 0; i < n; i++) scale = std::max(scale, std::max(std::fabs(hE[i]), std::fabs(hg[i])));
+  const double floor_ = (scale > 0.0 ? scale : 1.0) * 1e-10;
+  hipLaunchKernelGGL(k_denoms, dim3(grid1(nn)), dim3(256), 0, s, E, g, n, floor_, pc->D);
+  if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s) != hipSuccess) {
+    qh_set_error("qh_pc_create_hbasis: denominator kernel failed");
+    return fail(QH_EHIP);
+  }
+  cleanup();
+  *out = pc;
+  return QH_OK;
+}
+
+// z = M^-1 v for vec layouts of N x N matrices (column-major, loc = N*col + row)
+int qh_pc_apply_dev(const qh_pc *pc, const double2 *v, double2 *z) {
+  const rocblas_int n = (rocblas_int)pc->n;
+  hipStream_t s = qh_stream();
+  const rocblas_double_complex *W = (const rocblas_double_complex *)pc->W;
+  rocblas_double_complex *T = (rocblas_double_complex *)pc->T;
+  rocblas_double_complex *Z = (rocblas_double_complex *)z;
+  // T = W^dag V ; Z = T W ; Z ./= Dm ; T = W Z ; Z = T W^dag
+  QH_RB(rocblas_zgemm(pc->blas, rocblas_operation_conjugate_transpose, rocblas_operation_none, n, n, n, &ONE, W, n,
+                      (const rocblas_double_complex *)v, n, &ZERO, T, n));
+  QH_RB(rocblas_zgemm(pc->blas, rocblas_operation_none, rocblas_operation_none, n, n, n, &ONE, T, n, W, n, &ZERO, Z,
+                      n));
+  hipLaunchKernelGGL(k_cdiv_inplace, dim3(grid1(pc->n * pc->n)), dim3(256), 0, s, z, pc->D, pc->n * pc->n);
+  QH_RB(rocblas_zgemm(pc->blas, rocblas_operation_none, rocblas_operation_none, n, n, n, &ONE, W, n, Z, n, &ZERO, T,
+                      n));
+  QH_RB(rocblas_zgemm(pc->blas, rocblas_operation_none, rocblas_operation_conjugate_transpose, n, n, n, &ONE, T, n, W,
+                      n, &ZERO, Z, n));
+  QH_HIP(hipGetLastError());
+  return QH_OK;
+}
+
+int64_t qh_pc_dim(const qh_pc *pc) { return pc ? pc->n : 0; }
+
+extern "C" int qh_pc_apply(const qh_pc *pc, const qh_vec *in, qh_vec *out) {
+  QH_REQUIRE(pc && in && out && in != out && in->prec == QH_FP64 && out->prec == QH_FP64 &&
+                 in->n == pc->n * pc->n && out->n == in->n,
+             QH_EINVAL, "qh_pc_apply: two distinct FP64 vectors of N*N entries required");
+  return qh_pc_apply_dev(pc, (const double2 *)in->d, (double2 *)out->d);
+}
