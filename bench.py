@@ -77,8 +77,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--renorm", type=int, default=100)
-    ap.add_argument("--workload", default="chain10", choices=["chain10", "tc512", "circuit12", "jc", "copy"])
+    ap.add_argument("--workload", default="chain10", choices=["chain10", "tc512", "circuit12", "jc", "copy", "kerr"])
     ap.add_argument("--instances", type=int, default=512)
+    ap.add_argument("--levels", type=int, default=8)
+    ap.add_argument("--train", type=int, default=2000)
+    ap.add_argument("--test", type=int, default=200)
+    ap.add_argument("--rank", type=int, default=128)
+    ap.add_argument("--epochs", type=int, default=2000)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on exactly one GPU: this build targets a single MI355X "

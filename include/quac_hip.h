@@ -191,6 +191,28 @@ int qh_pc_destroy(qh_pc *pc);
 int qh_gmres_pc(const qh_mat *A, int64_t tot, qh_vec *x, int restart, double tol, int64_t maxit,
                 int pc_kind, const qh_pc *pc, int64_t *iters, double *res);
 
+/* --------------------------------------------------- warm-start network */
+/*
+ * The learned warm start of the steady-state solve: a fully connected network
+ * x -> [Linear -> tanh] x (nlayers-1) -> Linear -> y in FP32 with hand-written
+ * forward, backward and Adam kernels.  No counterpart in the reference (QuaC has
+ * no learned component).  dims: nlayers+1 widths; parameters are one FP32 array
+ * W_0 (dims[1] x dims[0], row-major), b_0, W_1, b_1, ...  x, t, y are device
+ * arrays of batch x width, row-major.  Loss: mean over the batch of
+ * sum_j 0.5 w_j (y_j - t_j)^2 with per-output weights w (default 1).
+ */
+typedef struct qh_mlp qh_mlp;
+int qh_mlp_create(int nlayers, const int *dims, int max_batch, qh_mlp **out);
+int qh_mlp_destroy(qh_mlp *net);
+int qh_mlp_num_params(const qh_mlp *net, int64_t *n);
+int qh_mlp_set_params(qh_mlp *net, const float *host);     /* also resets the Adam state */
+int qh_mlp_get_params(const qh_mlp *net, float *host);
+int qh_mlp_set_output_weights(qh_mlp *net, const float *host);
+int qh_mlp_forward(qh_mlp *net, const float *x, int batch, float *y);
+int qh_mlp_loss_grad(qh_mlp *net, const float *x, const float *t, int batch, float *loss, float *grad_host);
+int qh_mlp_train_step(qh_mlp *net, const float *x, const float *t, int batch, float lr, float beta1, float beta2,
+                      float eps, float *loss);
+
 /* ------------------------------------------------------------- timing hooks */
 /* average duration (ms) of the last qh_rk4 stage kernels, measured with events */
 int qh_last_stage_ms(double *spmv_ms, int64_t *launches);

@@ -89,6 +89,7 @@ _lib = None
 
 def _setup(L):
     P, vp, i64, dbl, i32 = ctypes.POINTER, ctypes.c_void_p, ctypes.c_int64, ctypes.c_double, ctypes.c_int
+    flt = ctypes.c_float
 
     def sig(name, res, *args):
         f = getattr(L, name)
@@ -179,6 +180,15 @@ def _setup(L):
     sig("qh_pc_destroy", i32, vp)
     sig("qh_gmres_pc", i32, vp, i64, vp, i32, dbl, i64, i32, vp, P(i64), P(dbl))
     sig("quac_assemble_heff", i32, vp, P(vp))
+    sig("qh_mlp_create", i32, i32, P(i32), i32, P(vp))
+    sig("qh_mlp_destroy", i32, vp)
+    sig("qh_mlp_num_params", i32, vp, P(i64))
+    sig("qh_mlp_set_params", i32, vp, P(flt))
+    sig("qh_mlp_get_params", i32, vp, P(flt))
+    sig("qh_mlp_set_output_weights", i32, vp, P(flt))
+    sig("qh_mlp_forward", i32, vp, vp, i32, vp)
+    sig("qh_mlp_loss_grad", i32, vp, vp, vp, i32, P(flt), P(flt))
+    sig("qh_mlp_train_step", i32, vp, vp, vp, i32, flt, flt, flt, flt, P(flt))
     sig("qh_timer_begin", i32)
     sig("qh_timer_end", i32, P(dbl))
     sig("qh_trace", i32, vp, i64, P(dbl), P(dbl))
@@ -761,4 +771,83 @@ class HBasisPC:
     def destroy(self):
         if self.h:
             lib().qh_pc_destroy(self.h)
+            self.h = None
+
+
+class DeviceF32:
+    """raw FP32 device array (plumbing for the warm-start network)"""
+
+    def __init__(self, n):
+        self.n = int(n)
+        p = ctypes.c_void_p()
+        _qh(lib().qh_dmalloc(ctypes.byref(p), max(4 * self.n, 4)))
+        self.h = p
+
+    def set(self, a):
+        a = np.ascontiguousarray(a, np.float32).reshape(-1)
+        assert a.size == self.n
+        _qh(lib().qh_h2d(self.h, a.ctypes.data_as(ctypes.c_void_p), 4 * self.n))
+
+    def get(self):
+        out = np.empty(self.n, np.float32)
+        _qh(lib().qh_d2h(out.ctypes.data_as(ctypes.c_void_p), self.h, 4 * self.n))
+        return out
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().qh_dfree(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+class MLP:
+    """the warm-start network on the device: x -> [Linear -> tanh]* -> Linear"""
+
+    def __init__(self, dims, max_batch):
+        self.dims = [int(d) for d in dims]
+        arr = (ctypes.c_int * len(self.dims))(*self.dims)
+        h = ctypes.c_void_p()
+        _qh(lib().qh_mlp_create(len(self.dims) - 1, arr, int(max_batch), ctypes.byref(h)))
+        self.h = h
+        n = ctypes.c_int64()
+        _qh(lib().qh_mlp_num_params(h, ctypes.byref(n)))
+        self.nparam = n.value
+        self.max_batch = int(max_batch)
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == self.nparam
+        _qh(lib().qh_mlp_set_params(self.h, p.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+
+    def get_params(self):
+        p = np.empty(self.nparam, np.float32)
+        _qh(lib().qh_mlp_get_params(self.h, p.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+        return p
+
+    def set_output_weights(self, w):
+        w = np.ascontiguousarray(w, np.float32)
+        assert w.size == self.dims[-1]
+        _qh(lib().qh_mlp_set_output_weights(self.h, w.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+
+    def forward(self, x_dev, batch, y_dev):
+        _qh(lib().qh_mlp_forward(self.h, x_dev.h, int(batch), y_dev.h))
+
+    def loss_grad(self, x_dev, t_dev, batch):
+        loss = ctypes.c_float()
+        g = np.empty(self.nparam, np.float32)
+        _qh(lib().qh_mlp_loss_grad(self.h, x_dev.h, t_dev.h, int(batch), ctypes.byref(loss),
+                                   g.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+        return loss.value, g
+
+    def train_step(self, x_dev, t_dev, batch, lr, beta1=0.9, beta2=0.999, eps=1e-8, want_loss=False):
+        loss = ctypes.c_float()
+        _qh(lib().qh_mlp_train_step(self.h, x_dev.h, t_dev.h, int(batch), lr, beta1, beta2, eps,
+                                    ctypes.byref(loss) if want_loss else None))
+        return loss.value if want_loss else None
+
+    def destroy(self):
+        if self.h:
+            lib().qh_mlp_destroy(self.h)
             self.h = None

@@ -268,6 +268,30 @@ def kerr_params(seed, count):
     return out
 
 
+def kerr_sweep_params(seed, count):
+    """parameter sets of the learned steady-state warm start: detunings in
+    U[-1, 1], Kerr nonlinearities in U[0.05, 0.5] and drives in U[0, 1] per
+    cavity, hopping in U[0.1, 0.5], one cavity loss in U[0.05, 0.2] shared by
+    the three cavities"""
+    rng = SplitMix(seed)
+    out = []
+    for _ in range(count):
+        d = [rng.uniform(-1, 1) for _ in range(3)]
+        K = [rng.uniform(0.05, 0.5) for _ in range(3)]
+        F = [rng.uniform(0, 1) for _ in range(3)]
+        J = rng.uniform(0.1, 0.5)
+        kap = rng.uniform(0.05, 0.2)
+        out.append({"delta": d, "K": K, "F": F, "J": J, "kappa": [kap] * 3})
+    return out
+
+
+def kerr_features(p):
+    """the 10 network inputs: every Hamiltonian parameter in units of the loss
+    (the steady state of L = -i[H, .] + kappa sum D[a_k] depends on H / kappa only)"""
+    k = p["kappa"][0]
+    return [x / k for x in p["delta"]] + [x / k for x in p["K"]] + [x / k for x in p["F"]] + [p["J"] / k]
+
+
 def kerr_recipe(p, levels=8):
     """H = sum d_k n_k + K_k a^dag a^dag a a + F_k (a_k + a_k^dag) + J sum (a_k^dag a_{k+1} + h.c.);
     L: kappa_k L[a_k]."""

@@ -1,0 +1,65 @@
+"""The learned steady-state warm start end to end at a small size (3 Kerr
+cavities x 4 levels): training data from the device solver, basis, network
+trained on the device, then held-out solves with and without the warm start.
+
+Checked: every seeded solve reaches the same residual as the plain solve (GMRES
+finishes it), the warm starts are Hermitian with unit trace, training lowers
+the loss, and the network's warm start is closer to the true steady state than
+the trivial guess (the mean of the training solutions).  The iteration counts
+of both runs are printed; the full-size comparison (8 levels, 2000 / 200 sets)
+is bench.py --workload kerr."""
+import numpy as np
+import pytest
+
+import quac_amd as qa
+import recipes
+from qbuild import build_system
+from quac_amd.warmstart import WarmStart, hermitian_unit_trace
+
+pytestmark = pytest.mark.gpu
+
+L = 4
+N = L ** 3
+
+
+def _solve(p, x0=None, tol=1e-8):
+    qs, _, _ = build_system(recipes.kerr_recipe(p, levels=L))
+    x = qs.create_qvec()
+    if x0 is not None:
+        x.set(x0)
+    it, res = qs.steady_state(x, restart=100, tol=tol, maxit=3000, precond="hbasis")
+    sol = x.get()
+    qs.destroy()
+    return sol, it, res
+
+
+def test_warm_start_pipeline_small():
+    train = recipes.kerr_sweep_params(4, 300)
+    feats, sols = [], []
+    for p in train:
+        sol, it, res = _solve(p, tol=1e-10)
+        assert res <= 1e-10
+        feats.append(recipes.kerr_features(p))
+        sols.append(sol)
+    sols = np.array(sols)
+    ws = WarmStart(N, rank=48, hidden=(128, 128, 128), seed=1)
+    stats = ws.fit(np.array(feats), sols, epochs=400, batch=100, lr=3e-3)
+    assert stats["losses"][-1][1] < stats["losses"][0][1] / 5
+    trivial = hermitian_unit_trace(sols.mean(0), N)
+    cold, warm, e_net, e_triv = [], [], [], []
+    for p in recipes.kerr_sweep_params(5, 24):
+        sol, it_c, res_c = _solve(p)
+        x0 = ws.predict([recipes.kerr_features(p)])[0]
+        rho = x0.reshape(N, N, order="F")
+        assert abs(np.trace(rho) - 1) < 1e-10 and np.abs(rho - rho.conj().T).max() < 1e-12
+        _, it_w, res_w = _solve(p, x0=x0)
+        assert res_c <= 1e-8 and res_w <= 1e-8
+        cold.append(it_c)
+        warm.append(it_w)
+        n = np.linalg.norm(sol)
+        e_net.append(np.linalg.norm(x0 - sol) / n)
+        e_triv.append(np.linalg.norm(trivial - sol) / n)
+    ws.destroy()
+    print(f"Kerr 3x{L}: cold {np.mean(cold):.1f} warm {np.mean(warm):.1f} iterations; "
+          f"warm-start error {np.mean(e_net):.3f} (trivial guess {np.mean(e_triv):.3f})")
+    assert np.mean(e_net) < 0.95 * np.mean(e_triv)
