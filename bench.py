@@ -217,7 +217,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32 (complex64 SpMV/stages, TwoSum-compensated; fp64 trace/observables)",
+        "dtype": "fp32 (complex64 matrix values and stage vectors, products and row sums in fp64, "
+                 "hi+lo complex64 state; fp64 trace renormalisation and observables)",
         "data": "synthetic",
         "config": {"workload": f"{args.nq}-qubit dissipative spin chain, rho {N}x{N}, superoperator {rows} rows "
                                f"{nnz} nnz, RK4 dt={dt}, h_i,J_i~U[0.5,1.5] seed 1, gamma1=1e-3, gamma2=5e-3",
@@ -229,7 +230,10 @@ def main():
                      "launch_ms_per_kernel_events": round(stage_ms32, 5),
                      "format_bytes_per_launch": ks_step32 / 4,
                      "format_gbs": round((ks_step32 / 4) / (launch_ms32 * 1e-3) / 1e9, 1),
-                     "traffic_source": traffic_src},
+                     "traffic_source": traffic_src,
+                     "traffic_gbs": round(traffic / (launch_ms32 * 1e-3) / 1e9, 1) if traffic else None,
+                     "traffic_frac": round(traffic / (launch_ms32 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+                     if traffic else None},
         "cpu_baseline": cpu,
         "fp64_steps_per_s": round(steps_s64, 3),
         "fp32_over_fp64": round(steps_s32 / steps_s64, 3),
