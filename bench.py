@@ -41,8 +41,9 @@ def rk4_step_bytes(nnz, rows, S):
 
 def kernel_name(variant, real):
     """name of the stage kernel a lanes-per-row variant code selects"""
-    if 128 <= variant <= 130:
-        return f"k_ks_stage<{real}, MODE, {2 << (variant - 128)}> (sliced layout)"
+    if 128 <= variant <= 132:
+        u, vv = {128: (2, 0), 129: (4, 0), 130: (8, 0), 131: (4, 1), 132: (8, 1)}[variant]
+        return f"k_ks_stage<{real}, MODE, {u}, {'true' if vv else 'false'}> (sliced layout)"
     return f"k_rk_stage<{real}, {variant}> (CSR, {variant} lanes per row)"
 
 

@@ -140,6 +140,14 @@ typedef struct qsystem {
   void *ts_ctx;
   PetscInt num_circuits, circuit_list_size, current_circuit;
   circuit *circuit_list;
+  /* MCWF state (after the fields above; apps do not read these): one threshold
+   * per trajectory (qsystem.c:1085-1091), the jump operators C_k as N x N
+   * matrices with their coefficients, the uniform generator's state */
+  PetscReal *rand_number;
+  Mat *mcwf_jump;
+  PetscScalar *mcwf_jump_a;
+  PetscInt mcwf_njump;
+  unsigned long long mcwf_rng[4];
 } *qsystem;
 
 /* ------------------------------------------------------------ src/quac.h */
@@ -177,6 +185,7 @@ void add_lin_term_time_dep_list(qsystem, PetscScalar, void *, quac_tdfunc_ri, Pe
 void create_qvec_sys(qsystem, qvec *);                  /* qvec_utilities.c:633-648 */
 void create_dm_sys(qsystem, qvec *);                    /* :654-705 */
 void create_wf_sys(qsystem, qvec *);                    /* :710-760 */
+void create_wf_ensemble_sys(qsystem, qvec *);           /* :765-817 */
 void create_arb_qvec(qvec *, PetscInt, qvec_type);
 void destroy_qvec(qvec *);                              /* :1076-1090 */
 void add_to_qvec(qvec, PetscScalar, ...);               /* :919-965 */

@@ -66,7 +66,8 @@ int qh_mat_make_fp32(qh_mat *m);
 /* block-diagonal concatenation of n matrices (batched parameter sweeps) */
 int qh_mat_blockdiag(int n, qh_mat *const *mats, qh_mat **out);
 /* select SpMV variant: lanes per row G in {1,2,4,8,16,32,64} of the CSR kernel,
- * 128 = slot-aligned sliced ELL (one wavefront per 64 rows); 0 = autotune */
+ * 128..132 = slot-aligned sliced ELL (one wavefront per 64 rows; 2/4/8 slots per
+ * pass, 131/132 with vector-loaded uniform values); 0 = autotune */
 int qh_mat_set_lanes(qh_mat *m, int lanes64, int lanes32);
 int qh_mat_get_lanes(const qh_mat *m, int *lanes64, int *lanes32);
 /* sliced layout statistics (built on demand): slices, slots, stored values
@@ -190,6 +191,21 @@ int qh_pc_destroy(qh_pc *pc);
 #define QH_PC_HBASIS 2
 int qh_gmres_pc(const qh_mat *A, int64_t tot, qh_vec *x, int restart, double tol, int64_t maxit,
                 int pc_kind, const qh_pc *pc, int64_t *iters, double *res);
+
+/* ---------------------------------------------------------- MCWF ensembles */
+/*
+ * Monte Carlo wavefunction trajectories (time_step_sys under use_mcwf_solver,
+ * qsystem.c:1024-1079; norm test and jumps qsystem.c:1138-1235).  An ensemble
+ * is one FP64 vector of M trajectories x N amplitudes, trajectory i at entries
+ * [i*N, (i+1)*N).  qh_vec_view aliases a range of an FP64 vector (destroying the
+ * view does not free the storage).
+ */
+int qh_vec_view(qh_vec *base, int64_t offset, int64_t n, qh_vec **out);
+int qh_ens_broadcast(const qh_vec *init, qh_vec *ens, int64_t M);
+/* nsteps classic RK4 steps of size dt of every trajectory under the N x N generator A */
+int qh_ens_rk4(const qh_mat *A, qh_vec *ens, int64_t M, double dt, int64_t nsteps);
+/* squared norm of each trajectory into norms2[M] (host) */
+int qh_ens_norms(const qh_vec *ens, int64_t M, double *norms2);
 
 /* --------------------------------------------------- warm-start network */
 /*

@@ -1,0 +1,199 @@
+/* mcwf.c -- Monte Carlo wavefunction trajectories of a qsystem under
+ * use_mcwf_solver (qsystem.c:358-403, 1024-1079, 1138-1235).
+ *
+ * All trajectories advance together on the device (qh_ens_rk4): one RK4 step of
+ * d psi/dt = -i Heff psi for the whole ensemble, then the squared norms of all
+ * trajectories are compared with their thresholds r_i.  A trajectory whose norm
+ * fell below r_i jumps at the end of that step: operator k is chosen with
+ * probability a_k ||C_k psi||^2 / sum_j a_j ||C_j psi||^2, psi <- sqrt(a_k) C_k psi /
+ * sqrt(a_k ||C_k psi||^2), and a new threshold is drawn.  The reference locates
+ * the crossing inside the step with PETSc's event handling; here the jump time
+ * is resolved to the step (first order in dt), and every trajectory uses the
+ * user's dt with classic RK4.
+ *
+ * Random numbers: SPRNG (the reference's generator) is not available; uniforms
+ * in [0, 1) come from xoshiro256** seeded through SplitMix64 with the seed given
+ * to use_mcwf_solver (0: a seed from the clock, as make_sprng_seed does).  Results
+ * agree with the reference statistically, not trajectory by trajectory. */
+#include <complex.h>
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <time.h>
+
+#include "quac_internal.h"
+
+static uint64_t rotl64(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+
+void quac_rng_seed(qsystem s, uint64_t seed) {
+  if (seed == 0) seed = (uint64_t)time(NULL) ^ 0x5DEECE66DULL;
+  uint64_t z = seed;
+  for (int i = 0; i < 4; i++) {
+    z += 0x9E3779B97F4A7C15ULL;
+    uint64_t t = z;
+    t = (t ^ (t >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    t = (t ^ (t >> 27)) * 0x94D049BB133111EBULL;
+    s->mcwf_rng[i] = t ^ (t >> 31);
+  }
+}
+
+double quac_rng_uniform(qsystem s) {
+  unsigned long long *q = s->mcwf_rng;
+  const uint64_t r = rotl64(q[1] * 5, 7) * 9;
+  const uint64_t t = q[1] << 17;
+  q[2] ^= q[0];
+  q[3] ^= q[1];
+  q[1] ^= q[2];
+  q[0] ^= q[3];
+  q[2] ^= t;
+  q[3] = rotl64(q[3], 45);
+  return (double)(r >> 11) * (1.0 / 9007199254740992.0);
+}
+
+/* jump operators of the time-independent Lindblad terms (qsystem.c:393-399) */
+static int build_jumps(qsystem s) {
+  if (s->mcwf_jump) return 0;
+  PetscInt n = 0;
+  for (PetscInt i = 0; i < s->num_time_indep; i++)
+    if (s->time_indep[i].my_term_type == LINDBLAD) n++;
+  if (n == 0) return quac_fail(9, "use_mcwf_solver needs at least one time-independent Lindblad term");
+  s->mcwf_jump = (Mat *)calloc((size_t)n, sizeof(Mat));
+  s->mcwf_jump_a = (PetscScalar *)calloc((size_t)n, sizeof(PetscScalar));
+  s->mcwf_njump = 0;
+  for (PetscInt i = 0; i < s->num_time_indep; i++) {
+    mat_term *t = &s->time_indep[i];
+    if (t->my_term_type != LINDBLAD) continue;
+    Mat m = NULL;
+    construct_op_matrix_wf_list(s, 1.0, &m, t->num_ops, t->ops);
+    if (!m) return 9;
+    s->mcwf_jump[s->mcwf_njump] = m;
+    s->mcwf_jump_a[s->mcwf_njump] = t->a;
+    s->mcwf_njump++;
+  }
+  return 0;
+}
+
+void quac_mcwf_free(qsystem s) {
+  for (PetscInt j = 0; j < s->mcwf_njump; j++) qh_mat_destroy(s->mcwf_jump[j]);
+  free(s->mcwf_jump);
+  free(s->mcwf_jump_a);
+  free(s->rand_number);
+  s->mcwf_jump = NULL;
+  s->mcwf_jump_a = NULL;
+  s->rand_number = NULL;
+  s->mcwf_njump = 0;
+}
+
+/* copy the initial condition into every trajectory and draw the thresholds
+ * (qsystem.c:977-989) */
+static int spawn(qsystem s, qvec x) {
+  if (x->ens_spawned) return 0;
+  const int64_t M = s->num_local_trajs, N = x->n;
+  Vec block = NULL;
+  if (qh_vec_create(M * N, QH_FP64, &block)) return quac_fail(9, "cannot allocate %lld trajectories: %s",
+                                                              (long long)M, qh_last_error());
+  if (qh_ens_broadcast(x->data, block, M)) {
+    qh_vec_destroy(block);
+    return quac_fail(9, "ensemble set-up failed: %s", qh_last_error());
+  }
+  x->ens_datas = (Vec *)malloc(sizeof(Vec));
+  x->ens_datas[0] = block;
+  x->n_ensemble = M;
+  free(s->rand_number);
+  s->rand_number = (PetscReal *)malloc(sizeof(PetscReal) * (size_t)M);
+  for (int64_t i = 0; i < M; i++) s->rand_number[i] = quac_rng_uniform(s);
+  x->ens_spawned = PETSC_TRUE;
+  return 0;
+}
+
+/* the jump of trajectory i (qsystem.c:1159-1235) */
+static int jump(qsystem s, Vec block, int64_t i, int64_t N, Vec work) {
+  Vec v = NULL;
+  if (qh_vec_view(block, i * N, N, &v)) return quac_fail(9, "%s", qh_last_error());
+  double *p = (double *)malloc(sizeof(double) * (size_t)s->mcwf_njump);
+  double tot = 0.0;
+  int rc = 0;
+  for (PetscInt j = 0; j < s->mcwf_njump && !rc; j++) {
+    double n2 = 0.0;
+    rc = qh_spmv(s->mcwf_jump[j], v, work);
+    if (!rc) rc = qh_ens_norms(work, 1, &n2);
+    p[j] = creal(s->mcwf_jump_a[j] * n2);
+    tot += p[j];
+  }
+  if (!rc && tot > 0.0) {
+    const double u = quac_rng_uniform(s);
+    double acc = 0.0;
+    PetscInt k = s->mcwf_njump - 1;
+    for (PetscInt j = 0; j < s->mcwf_njump; j++) {
+      acc += p[j] / tot;
+      if (acc >= u) {
+        k = j;
+        break;
+      }
+    }
+    double complex sc = csqrt(s->mcwf_jump_a[k]) / sqrt(p[k]);
+    rc = qh_spmv(s->mcwf_jump[k], v, work);
+    if (!rc) rc = qh_vec_scale(work, creal(sc), cimag(sc));
+    if (!rc) rc = qh_vec_copy(work, v);
+  }
+  s->rand_number[i] = quac_rng_uniform(s);
+  free(p);
+  qh_vec_destroy(v);
+  if (rc) return quac_fail(9, "MCWF jump failed: %s", qh_last_error());
+  return 0;
+}
+
+int quac_mcwf_time_step(qsystem s, qvec x, PetscReal t0, PetscReal tmax, PetscReal dt, PetscInt steps_max,
+                        int (*monitor)(qsystem, PetscInt, PetscReal, qvec)) {
+  if (x->my_type != WF_ENSEMBLE) return quac_fail(0, "time_step_sys with use_mcwf_solver needs a qvec from create_qvec_sys");
+  if (build_jumps(s) || spawn(s, x)) return 9;
+  const int64_t M = x->n_ensemble, N = x->n;
+  Vec block = x->ens_datas[0];
+  Vec work = NULL;
+  if (qh_vec_create(N, QH_FP64, &work)) return quac_fail(9, "%s", qh_last_error());
+  double *nrm = (double *)malloc(sizeof(double) * (size_t)M);
+  PetscReal t = t0;
+  PetscInt step = 0;
+  const PetscReal eps = 1e-12 * (fabs(tmax) + fabs(dt) + 1.0);
+  int rc = 0;
+  if (monitor) rc = monitor(s, 0, t, x);
+  while (!rc && t < tmax - eps && step < steps_max) {
+    PetscReal h = dt;
+    if (t + h > tmax - eps) h = tmax - t;
+    if (qh_ens_rk4(s->mat_A, block, M, h, 1) || qh_ens_norms(block, M, nrm)) {
+      rc = quac_fail(9, "MCWF step failed: %s", qh_last_error());
+      break;
+    }
+    for (int64_t i = 0; i < M && !rc; i++)
+      if (nrm[i] < s->rand_number[i]) rc = jump(s, block, i, N, work);
+    t += h;
+    step++;
+    if (!rc && monitor) rc = monitor(s, step, t, x);
+  }
+  free(nrm);
+  qh_vec_destroy(work);
+  s->current_time = t;
+  return rc;
+}
+
+/* ensemble average of <psi_i|O|psi_i> / <psi_i|psi_i> (qvec_utilities.c:1353-1420) */
+int quac_mcwf_expect(qvec x, qc_chain *c, double *re, double *im) {
+  const int64_t M = x->n_ensemble, N = x->n;
+  Vec block = x->ens_datas[0];
+  double *nrm = (double *)malloc(sizeof(double) * (size_t)M);
+  int rc = qh_ens_norms(block, M, nrm);
+  double sr = 0.0, si = 0.0;
+  for (int64_t i = 0; i < M && !rc; i++) {
+    Vec v = NULL;
+    double er = 0.0, ei = 0.0;
+    rc = qh_vec_view(block, i * N, N, &v);
+    if (!rc) rc = qh_expect_wf(v, x->total_levels, c->nops, c->ops, c->nlev, c->lev, &er, &ei);
+    qh_vec_destroy(v);
+    sr += er / nrm[i];
+    si += ei / nrm[i];
+  }
+  free(nrm);
+  *re = sr / (double)M;
+  *im = si / (double)M;
+  return rc;
+}

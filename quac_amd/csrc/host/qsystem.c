@@ -41,6 +41,7 @@ void initialize_system(qsystem *qsys) {
 /* ----------------------------------------------------------- qsystem.c:67 */
 void destroy_system(qsystem *qsys) {
   if (!qsys || !*qsys) return;
+  quac_mcwf_free(*qsys);
   clear_mat_terms_sys(*qsys);
   free((*qsys)->subsystem_list);
   free((*qsys)->time_dep);
@@ -182,6 +183,7 @@ void use_mcwf_solver(qsystem qsys, PetscInt num_tot_trajs, PetscInt seed) {
   qsys->seed = seed;
   qsys->num_tot_trajs = num_tot_trajs;
   qsys->num_local_trajs = num_tot_trajs;
+  quac_rng_seed(qsys, (uint64_t)seed);
 }
 
 /* --------------------------------------------------------- term registry */
@@ -470,6 +472,15 @@ static int call_monitor(qsystem s, struct quac_ts *ts, qvec x) {
   return s->ts_monitor(ts, ts->step, ts->t, x->data, s->ts_ctx);
 }
 
+/* monitor of the MCWF ensemble: called once per ensemble step with the
+ * ensemble's storage as the Vec (all trajectories, trajectory i at [i*N, (i+1)*N)) */
+static int mcwf_monitor(qsystem s, PetscInt step, PetscReal t, qvec x) {
+  struct quac_ts ts = {s, t, step};
+  ts.t = t;
+  ts.step = step;
+  return s->ts_monitor(&ts, step, t, x->ens_spawned ? x->ens_datas[0] : x->data, s->ts_ctx);
+}
+
 /* time-dependent RHS y = A x + sum_k s_k(t) A_k x (qsystem.c:1300-1330) */
 static int td_rhs(qsystem s, PetscReal t, Vec x, Vec y) {
   int ntd = (int)s->num_time_dep;
@@ -559,8 +570,13 @@ void time_step_sys(qsystem qsys, qvec x, PetscReal init_time, PetscReal time_max
   if (!x || !x->data) QUAC_FAIL(0, "time_step_sys: state not created");
   if (x->n != qsys->dim) QUAC_FAIL(0, "time_step_sys: state size %lld does not match the system dimension %lld",
                                    (long long)x->n, (long long)qsys->dim);
-  if (qsys->mcwf_solver) QUAC_FAIL(9, "time_step_sys: the MCWF trajectory solver is not provided by this library");
   if (dt <= 0) QUAC_FAIL(0, "time_step_sys: dt must be positive");
+  if (qsys->mcwf_solver) {
+    /* trajectories of the ensemble (qsystem.c:1024-1079) */
+    if (quac_mcwf_time_step(qsys, x, init_time, time_max, dt, steps_max, qsys->ts_monitor ? mcwf_monitor : NULL))
+      QUAC_FAIL(9, "time_step_sys (MCWF) failed");
+    return;
+  }
 
   /* integrator: PETSc default here is TSRK3BS adaptive (qsystem.c:1003-1005);
    * -ts_rk_type 4 -ts_adapt_type none selects fixed-step classic RK4 */

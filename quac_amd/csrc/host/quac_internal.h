@@ -3,6 +3,7 @@
 #define QUAC_INTERNAL_H
 
 #include <stdarg.h>
+#include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
 
@@ -36,6 +37,13 @@ int quac_errors_return(void);
 
 int quac_ensure_device(void);
 
+/* MCWF trajectories (mcwf.c) */
+void quac_rng_seed(qsystem s, uint64_t seed);
+double quac_rng_uniform(qsystem s);
+void quac_mcwf_free(qsystem s);
+int quac_mcwf_time_step(qsystem s, qvec x, PetscReal t0, PetscReal tmax, PetscReal dt, PetscInt steps_max,
+                        int (*monitor)(qsystem, PetscInt, PetscReal, qvec));
+
 /* options */
 int quac_opt_int(const char *name, int dflt);
 double quac_opt_double(const char *name, double dflt);
@@ -50,6 +58,7 @@ typedef struct {
 } qc_chain;
 void qc_chain_init(qc_chain *c);
 void qc_chain_free(qc_chain *c);
+int quac_mcwf_expect(qvec x, qc_chain *c, double *re, double *im);
 /* Append the ops of a QuaC operator product to c (a consecutive pair of VEC
  * operators becomes one |p1><p2| entry).  Returns number of appended entries,
  * or -1 on error (message recorded).  tot = total_levels of the space. */

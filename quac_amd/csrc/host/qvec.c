@@ -27,7 +27,29 @@ static qvec new_qvec(qvec_type type, PetscInt n, PetscInt total_levels) {
 /* qvec_utilities.c:633-648 */
 void create_qvec_sys(qsystem qsys, qvec *new_qvec) {
   if (qsys->dm_equations == PETSC_TRUE && qsys->mcwf_solver == PETSC_FALSE) create_dm_sys(qsys, new_qvec);
+  else if (qsys->mcwf_solver == PETSC_TRUE) create_wf_ensemble_sys(qsys, new_qvec);
   else create_wf_sys(qsys, new_qvec);
+}
+
+/* qvec_utilities.c:765-817: the initial condition lives in data; the
+ * trajectories are spawned from it by the first time_step_sys, into one device
+ * block of n_ensemble x N (ens_datas[0]) */
+void create_wf_ensemble_sys(qsystem qsys, qvec *new_wf) {
+  if (quac_ensure_device()) return;
+  if (qsys->num_subsystems == 0) QUAC_FAIL(0, "You need to create operators before you construct a wavefunction!");
+  if (qsys->dm_equations == PETSC_TRUE && qsys->mcwf_solver == PETSC_FALSE)
+    QUAC_FAIL(0, "Must use density matrices if Lindblad terms are used without using mcwf_solver.");
+  qsys->dim = qsys->total_levels;
+  qsys->hspace_frozen = 1;
+  qvec q = new_qvec(WF_ENSEMBLE, qsys->dim, qsys->total_levels);
+  if (!q) return;
+  q->n_ops = qsys->num_subsystems;
+  q->ndims_hspace = qsys->num_subsystems;
+  q->hspace_dims = (PetscInt *)malloc(qsys->num_subsystems * sizeof(PetscInt));
+  for (PetscInt i = 0; i < qsys->num_subsystems; i++) q->hspace_dims[i] = qsys->subsystem_list[i]->my_levels;
+  q->n_ensemble = qsys->num_local_trajs;
+  q->ens_spawned = PETSC_FALSE;
+  *new_wf = q;
 }
 
 /* qvec_utilities.c:654-705 */
@@ -86,6 +108,10 @@ void create_arb_qvec(qvec *out, PetscInt n, qvec_type type) {
 void destroy_qvec(qvec *state) {
   if (!state || !*state) return;
   free((*state)->hspace_dims);
+  if ((*state)->ens_datas) {
+    qh_vec_destroy((*state)->ens_datas[0]);
+    free((*state)->ens_datas);
+  }
   qh_vec_destroy((*state)->data);
   free(*state);
   *state = NULL;
@@ -179,6 +205,8 @@ void get_expectation_value_qvec_list(qvec state, PetscScalar *trace_val, PetscIn
   int rc;
   if (state->my_type == DENSITY_MATRIX)
     rc = qh_expect_dm(state->data, state->total_levels, c.nops, c.ops, c.nlev, c.lev, &re, &im);
+  else if (state->my_type == WF_ENSEMBLE && state->ens_spawned)
+    rc = quac_mcwf_expect(state, &c, &re, &im);
   else
     rc = qh_expect_wf(state->data, state->total_levels, c.nops, c.ops, c.nlev, c.lev, &re, &im);
   qc_chain_free(&c);

@@ -72,17 +72,18 @@ struct qh_mat {
   int64_t ks_slots = 0;
   int64_t ks_nval = 0;         // stored (non-uniform) values
 };
-// variant codes selecting the sliced-ELL kernel: 128 + log2(slot unroll) - 1,
-// i.e. 128 / 129 / 130 process 2 / 4 / 8 slots per pass
+// variant codes selecting the sliced-ELL kernel: 128 / 129 / 130 process 2 / 4 / 8
+// slots per pass; 131 / 132 process 4 / 8 slots per pass and read the uniform
+// slot values with vector loads
 #define QH_KSELL 128
-#define QH_IS_KSELL(g) ((g) >= 128 && (g) <= 130)
-#define QH_KSELL_UNROLL(g) (2 << ((g) - 128))
+#define QH_IS_KSELL(g) ((g) >= 128 && (g) <= 132)
 
 struct qh_vec {
   int64_t n = 0;
   int prec = QH_FP64;
   void *d = nullptr;           // double2* (FP64) or float2* high part (FP32)
   float2 *lo = nullptr;        // FP32 compensation word (low part), zero-initialised
+  int view = 0;                // 1: d points into another vector's storage (not freed here)
 };
 
 // device allocation helpers (return QH_* codes)

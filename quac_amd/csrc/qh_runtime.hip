@@ -206,9 +206,24 @@ extern "C" int qh_vec_create(int64_t n, int prec, qh_vec **out) {
 
 extern "C" int qh_vec_destroy(qh_vec *v) {
   if (!v) return QH_OK;
-  qh_free_ptr(v->d);
-  qh_free_ptr(v->lo);
+  if (!v->view) {
+    qh_free_ptr(v->d);
+    qh_free_ptr(v->lo);
+  }
   delete v;
+  return QH_OK;
+}
+
+// a vector of n entries aliasing entries [offset, offset + n) of an FP64 vector
+extern "C" int qh_vec_view(qh_vec *base, int64_t offset, int64_t n, qh_vec **out) {
+  QH_REQUIRE(base && out && base->prec == QH_FP64 && offset >= 0 && n > 0 && offset + n <= base->n, QH_EINVAL,
+             "qh_vec_view: range outside an FP64 vector");
+  qh_vec *v = new qh_vec();
+  v->n = n;
+  v->prec = QH_FP64;
+  v->d = (double2 *)base->d + offset;
+  v->view = 1;
+  *out = v;
   return QH_OK;
 }
 

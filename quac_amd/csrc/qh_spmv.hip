@@ -21,6 +21,7 @@
 // compensation, the state carries a complex64 compensation word (x = hi + lo)
 // so that the RK update does not lose the low bits of x.
 #include <algorithm>
+#include <cstddef>
 #include <cstdlib>
 #include <vector>
 
@@ -225,10 +226,14 @@ __device__ inline double2 ks_load<double>(__amdgpu_buffer_rsrc_t rs, uint32_t of
 // of a tail pass never leave it; slots past cnt get an empty lane mask).
 // VALS: the slots carry per-lane values in the compacted stream, else the
 // slot's uniform value is in its metadata.
-template <typename R, bool VALS, bool TAIL, int U>
+// VV: the uniform value of a slot is read with a vector load (broadcast to all
+// lanes) instead of a scalar load, which keeps the metadata held in SGPRs at six
+// dwords per slot and lets a pass carry 8 slots; mr covers the metadata array
+// and k is the slot index of mp[0].
+template <typename R, bool VALS, bool TAIL, int U, bool VV>
 __device__ __forceinline__ void ks_pass(const qh_kmeta *__restrict__ mp, int32_t cnt, uint32_t r8,
-                                        __amdgpu_buffer_rsrc_t xr, __amdgpu_buffer_rsrc_t vr, double &are,
-                                        double &aim) {
+                                        __amdgpu_buffer_rsrc_t xr, __amdgpu_buffer_rsrc_t vr,
+                                        __amdgpu_buffer_rsrc_t mr, int32_t k, double &are, double &aim) {
   typedef typename C2<R>::T T;
   qh_kmeta m[U];
   uint64_t mask[U];
@@ -238,6 +243,14 @@ __device__ __forceinline__ void ks_pass(const qh_kmeta *__restrict__ mp, int32_t
     mask[j] = (!TAIL || j < cnt) ? m[j].mask : 0ull;
   }
   T xv[U], vv[U];
+  double2 uv[U];
+  if (!VALS && VV) {
+#pragma unroll
+    for (int j = 0; j < U; j++)
+      uv[j] = __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(
+                                              mr, (uint32_t)(k + j) * (uint32_t)sizeof(qh_kmeta) +
+                                                      (uint32_t)offsetof(qh_kmeta, vre), 0, 0));
+  }
 #pragma unroll
   for (int j = 0; j < U; j++) {
     const bool on = __builtin_amdgcn_inverse_ballot_w64(mask[j]);
@@ -260,6 +273,9 @@ __device__ __forceinline__ void ks_pass(const qh_kmeta *__restrict__ mp, int32_t
     if (VALS) {
       ar = (double)vv[j].x;
       ai = (double)vv[j].y;
+    } else if (VV) {
+      ar = uv[j].x;
+      ai = uv[j].y;
     } else {
       ar = m[j].vre;
       ai = m[j].vim;
@@ -270,15 +286,16 @@ __device__ __forceinline__ void ks_pass(const qh_kmeta *__restrict__ mp, int32_t
 }
 
 // all slots [k, kend) of one group (uniform or not) of a slice
-template <typename R, bool VALS, int U>
+template <typename R, bool VALS, int U, bool VV>
 __device__ __forceinline__ void ks_group(const qh_kmeta *__restrict__ meta, int32_t k, int32_t kend, uint32_t r8,
-                                         __amdgpu_buffer_rsrc_t xr, __amdgpu_buffer_rsrc_t vr, double &are,
-                                         double &aim) {
+                                         __amdgpu_buffer_rsrc_t xr, __amdgpu_buffer_rsrc_t vr,
+                                         __amdgpu_buffer_rsrc_t mr, double &are, double &aim) {
   static_assert(U - 1 <= QH_KS_PAD, "tail pass reads past the padding");
   const qh_kmeta *mp = meta + k;
   const int32_t n = kend - k;
-  for (int32_t p = 0; p < n / U; p++, mp += U) ks_pass<R, VALS, false, U>(mp, U, r8, xr, vr, are, aim);
-  if (n % U) ks_pass<R, VALS, true, U>(mp, n % U, r8, xr, vr, are, aim);
+  for (int32_t p = 0; p < n / U; p++, mp += U, k += U)
+    ks_pass<R, VALS, false, U, VV>(mp, U, r8, xr, vr, mr, k, are, aim);
+  if (n % U) ks_pass<R, VALS, true, U, VV>(mp, n % U, r8, xr, vr, mr, k, are, aim);
 }
 
 // Stage kernel on the sliced layout.  The stage input Y_s = x + c_s K_{s-1} is
@@ -291,10 +308,10 @@ __device__ __forceinline__ void ks_group(const qh_kmeta *__restrict__ meta, int3
 // the FMAs); U trades memory-level parallelism against the SGPRs the metadata
 // occupies, and is chosen by the autotuner.  The FP32 state is x = hi + lo
 // (complex64 pair); its updates are formed in FP64 and split again.
-template <typename R, int MODE, int U>
+template <typename R, int MODE, int U, bool VV>
 __global__ void __launch_bounds__(256) k_ks_stage(
     int32_t nrows, int32_t nslices, const int32_t *__restrict__ moff, const int32_t *__restrict__ umid,
-    const qh_kmeta *__restrict__ meta, int32_t nval, const typename C2<R>::T *__restrict__ val,
+    const qh_kmeta *__restrict__ meta, int32_t nmeta, int32_t nval, const typename C2<R>::T *__restrict__ val,
     const typename C2<R>::T *__restrict__ xin, typename C2<R>::T *__restrict__ kout,
     typename C2<R>::T *__restrict__ ynext, R cn, const typename C2<R>::T *__restrict__ x,
     const typename C2<R>::T *__restrict__ k1, const typename C2<R>::T *__restrict__ k2,
@@ -309,10 +326,11 @@ __global__ void __launch_bounds__(256) k_ks_stage(
   const int32_t k0 = moff[s], km = umid[s], kend = moff[s + 1];
   const __amdgpu_buffer_rsrc_t xr = ks_rsrc(xin, (int64_t)nrows * (int64_t)sizeof(T));
   const __amdgpu_buffer_rsrc_t vr = ks_rsrc(val, (int64_t)nval * (int64_t)sizeof(T));
+  const __amdgpu_buffer_rsrc_t mr = ks_rsrc(meta, (int64_t)nmeta * (int64_t)sizeof(qh_kmeta));
   const uint32_t r8 = (uint32_t)r * (uint32_t)sizeof(T);
   double are = 0.0, aim = 0.0;
-  ks_group<R, false, U>(meta, k0, km, r8, xr, vr, are, aim);
-  ks_group<R, true, U>(meta, km, kend, r8, xr, vr, are, aim);
+  ks_group<R, false, U, VV>(meta, k0, km, r8, xr, vr, mr, are, aim);
+  ks_group<R, true, U, VV>(meta, km, kend, r8, xr, vr, mr, are, aim);
   if (r >= nrows) return;
   T y;
   y.x = (R)are;
@@ -575,15 +593,17 @@ int launch_ks(int G, hipStream_t s, const qh_mat *A, const void *xin, void *kout
   const T *kv = sizeof(R) == 8 ? (const T *)A->ks_val : (const T *)A->ks_val32;
   const dim3 grid(qh_blocks(A->nslices, 4)), blk(256);
   const int32_t n = (int32_t)A->rows, ns = A->nslices;
-#define QH_LK(UU)                                                                                           \
-  hipLaunchKernelGGL((k_ks_stage<R, MODE, UU>), grid, blk, 0, s, n, ns, A->ks_moff, A->ks_umid, A->ks_meta,     \
-                     (int32_t)A->ks_nval, kv,                                                               \
+#define QH_LK(UU, VV)                                                                                       \
+  hipLaunchKernelGGL((k_ks_stage<R, MODE, UU, VV>), grid, blk, 0, s, n, ns, A->ks_moff, A->ks_umid, A->ks_meta, \
+                     (int32_t)(A->ks_slots + QH_KS_PAD), (int32_t)A->ks_nval, kv,                           \
                      (const T *)xin, (T *)kout, (T *)ynext, cn, (const T *)x, (const T *)k1, (const T *)k2, \
                      (const T *)k3, (T *)xout, h, xlo)
-  switch (QH_KSELL_UNROLL(G)) {
-    case 2: QH_LK(2); break;
-    case 4: QH_LK(4); break;
-    case 8: QH_LK(8); break;
+  switch (G) {
+    case 128: QH_LK(2, false); break;
+    case 129: QH_LK(4, false); break;
+    case 130: QH_LK(8, false); break;
+    case 131: QH_LK(4, true); break;
+    case 132: QH_LK(8, true); break;
     default:
       qh_set_error("unsupported sliced-layout variant %d", G);
       return QH_EINVAL;
@@ -653,7 +673,7 @@ int autotune(qh_mat *A, int *out_g) {
     if (!brc && A->ks_state == 1 && sizeof(R) == 4) brc = ks_make_fp32(A);
     if (brc) return brc;
     if (A->ks_state == 1)
-      for (int u = 128; u <= 130; u++) cands.push_back(u);
+      for (int u = 128; u <= 132; u++) cands.push_back(u);
   }
   for (int g : {2, 4, 8, 16, 32, 64})
     if (g <= 4 * avg + 4) cands.push_back(g);
@@ -993,7 +1013,7 @@ extern "C" int qh_mat_set_lanes(qh_mat *m, int l64, int l32) {
   auto ok = [](int g) {
     return g == 0 || g == 1 || g == 2 || g == 4 || g == 8 || g == 16 || g == 32 || g == 64 || QH_IS_KSELL(g);
   };
-  QH_REQUIRE(ok(l64) && ok(l32), QH_EINVAL, "lanes per row must be a power of two <= 64, or 128..130 (sliced layout)");
+  QH_REQUIRE(ok(l64) && ok(l32), QH_EINVAL, "lanes per row must be a power of two <= 64, or 128..132 (sliced layout)");
   m->lanes64 = l64;
   m->lanes32 = l32;
   if (g_st.A == m) g_st.gG = -1;  // force a re-capture

@@ -94,7 +94,7 @@ def test_rk4_device_direct_matches_oracle_and_lanes():
     x0 = np.zeros(N * N, np.complex128)
     x0[loc] = 1
     ref = _oracle_run(r, x0, 0.01, 37)
-    for g in (1, 2, 4, 8, 16, 32, 64, 128, 129, 130):  # 128..130 = sliced layout, 2/4/8 slots per pass
+    for g in (1, 2, 4, 8, 16, 32, 64, 128, 129, 130, 131, 132):  # 128..132 = sliced layout variants
         A.set_lanes(g, g)
         v = qa.DeviceVec(N * N, qa.FP64)
         v.set(x0)
@@ -130,7 +130,7 @@ def test_sliced_ell_spmv_matches_csr(name):
     xv = qa.DeviceVec(n)
     xv.set(x)
     outs = []
-    for g in (8, 128, 129, 130):
+    for g in (8, 128, 129, 130, 131, 132):
         A.set_lanes(g, g)
         yv = qa.DeviceVec(n)
         qa._qh(qa.lib().qh_spmv(A.h, xv.h, yv.h))
