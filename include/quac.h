@@ -204,6 +204,13 @@ void read_qvec_dm_binary(qvec *, const char[]);         /* :132-166 */
 void read_qvec_wf_binary(qvec *, const char[]);         /* :97-126 */
 void copy_qvec(qvec, qvec);
 void print_qvec(qvec);
+void print_qvec_file(qvec, const char[]);               /* :203-211 */
+void print_dm_qvec_file(qvec, const char[]);            /* :213-241 */
+void print_wf_qvec_file(qvec, const char[]);            /* :246-268 */
+/* PETSc binary writers (MatView / VecView on a binary viewer, the fixture
+ * format of tests/t_helpers.c:7-13); return 0 on success */
+int quac_write_mat_petsc_binary(Mat, const char[]);
+int quac_write_qvec_petsc_binary(qvec, const char[]);
 
 /* ------------------------------------------ src/quantum_circuits.h/gates.h */
 void create_circuit(circuit *, PetscInt);                       /* quantum_gates.c:610-628 */

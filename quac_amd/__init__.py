@@ -132,6 +132,9 @@ def _setup(L):
     sig("get_bitstring_probs", None, vp, P(i64), P(P(dbl)), P(P(dbl)))
     sig("read_qvec_dm_binary", None, P(vp), ctypes.c_char_p)
     sig("read_qvec_wf_binary", None, P(vp), ctypes.c_char_p)
+    sig("quac_write_mat_petsc_binary", i32, vp, ctypes.c_char_p)
+    sig("quac_write_qvec_petsc_binary", i32, vp, ctypes.c_char_p)
+    sig("print_qvec_file", None, vp, ctypes.c_char_p)
     sig("set_init_excited_op", None, OpPtr, i64)
     sig("set_qvec_from_init_excited_op", None, vp, vp)
     sig("create_circuit", None, P(CircuitStruct), i64)
@@ -301,6 +304,12 @@ class Matrix:
         _qh(lib().qh_mat_info(self.h, ctypes.byref(r), ctypes.byref(z)))
         return r.value, z.value
 
+    def write_petsc_binary(self, path):
+        rc = lib().quac_write_mat_petsc_binary(self.h, str(path).encode())
+        _check()
+        if rc:
+            raise QuacError(lib().quac_last_error().decode())
+
     def export(self):
         """(rowptr int64, col int64, val complex128) copied from the device."""
         rows, nnz = self.info()
@@ -375,6 +384,16 @@ class QVec:
     @property
     def is_dm(self):
         return self.type == DENSITY_MATRIX
+
+    def write_petsc_binary(self, path):
+        rc = lib().quac_write_qvec_petsc_binary(self.h, str(path).encode())
+        _check()
+        if rc:
+            raise QuacError(lib().quac_last_error().decode())
+
+    def print_file(self, path):
+        lib().print_qvec_file(self.h, str(path).encode())
+        _check()
 
     def data(self):
         return DeviceVec(handle=lib().quac_qvec_data(self.h), owned=False)

@@ -342,3 +342,102 @@ void quac_qvec_info(qvec q, PetscInt *n, PetscInt *tl, qvec_type *type) {
   if (tl) *tl = q->total_levels;
   if (type) *type = q->my_type;
 }
+
+/* ------------------------------------------------------- file writers */
+/* text dumps (qvec_utilities.c:203-275): "%40.30e  %40.30e" per element, a DM
+ * row per line followed by an empty line */
+void print_dm_qvec_file(qvec dm, const char filename[]) {
+  FILE *fp = fopen(filename, "w");
+  if (!fp) QUAC_FAIL(1, "Cannot open file in print_dm_qvec_file= %s", filename);
+  double *h = (double *)malloc(sizeof(double) * 2 * dm->n);
+  if (qh_vec_get(dm->data, h)) {
+    free(h);
+    fclose(fp);
+    QUAC_FAIL(9, "print_dm_qvec_file: %s", qh_last_error());
+  }
+  PetscInt N = (PetscInt)llround(sqrt((double)dm->n));
+  for (PetscInt i = 0; i < N; i++) {
+    for (PetscInt j = 0; j < N; j++) fprintf(fp, "%40.30e  %40.30e ", h[2 * (N * j + i)], h[2 * (N * j + i) + 1]);
+    fprintf(fp, "\n");
+  }
+  fprintf(fp, "\n");
+  free(h);
+  fclose(fp);
+}
+
+void print_wf_qvec_file(qvec state, const char filename[]) {
+  FILE *fp = fopen(filename, "w");
+  if (!fp) QUAC_FAIL(1, "Cannot open file in print_dm_qvec_file= %s", filename);
+  double *h = (double *)malloc(sizeof(double) * 2 * state->n);
+  if (qh_vec_get(state->data, h)) {
+    free(h);
+    fclose(fp);
+    QUAC_FAIL(9, "print_wf_qvec_file: %s", qh_last_error());
+  }
+  for (PetscInt i = 0; i < state->n; i++) fprintf(fp, "%40.30e  %40.30e\n", h[2 * i], h[2 * i + 1]);
+  free(h);
+  fclose(fp);
+}
+
+void print_qvec_file(qvec state, const char filename[]) {
+  if (state->my_type == DENSITY_MATRIX)
+    print_dm_qvec_file(state, filename);
+  else
+    print_wf_qvec_file(state, filename);
+}
+
+/* PETSc binary formats as written by MatView/VecView on a binary viewer with
+ * 64-bit PetscInt and complex scalars (the format of the reference's fixtures,
+ * tests/t_helpers.c:7-13): big-endian int64 {1211216, M, N, nz}, M row lengths,
+ * nz column indices, nz complex128 values; vectors {1211214, N} + N complex128. */
+static void put_be64(FILE *fp, uint64_t v) {
+  unsigned char b[8];
+  for (int i = 0; i < 8; i++) b[i] = (unsigned char)(v >> (56 - 8 * i));
+  fwrite(b, 1, 8, fp);
+}
+static void put_be_double(FILE *fp, double d) {
+  uint64_t u;
+  memcpy(&u, &d, 8);
+  put_be64(fp, u);
+}
+
+int quac_write_mat_petsc_binary(Mat A, const char filename[]) {
+  int64_t rows = 0, nnz = 0;
+  if (!A || qh_mat_info(A, &rows, &nnz)) return quac_fail(9, "quac_write_mat_petsc_binary: no matrix");
+  int64_t *rp = (int64_t *)malloc(sizeof(int64_t) * (rows + 1));
+  int64_t *col = (int64_t *)malloc(sizeof(int64_t) * (nnz > 0 ? nnz : 1));
+  double *val = (double *)malloc(sizeof(double) * 2 * (nnz > 0 ? nnz : 1));
+  int rc = qh_mat_export(A, rp, col, val);
+  FILE *fp = rc ? NULL : fopen(filename, "wb");
+  if (!rc && !fp) rc = quac_fail(1, "cannot open %s for writing", filename);
+  if (fp) {
+    put_be64(fp, 1211216);
+    put_be64(fp, (uint64_t)rows);
+    put_be64(fp, (uint64_t)rows);
+    put_be64(fp, (uint64_t)nnz);
+    for (int64_t r = 0; r < rows; r++) put_be64(fp, (uint64_t)(rp[r + 1] - rp[r]));
+    for (int64_t k = 0; k < nnz; k++) put_be64(fp, (uint64_t)col[k]);
+    for (int64_t k = 0; k < 2 * nnz; k++) put_be_double(fp, val[k]);
+    if (fclose(fp)) rc = quac_fail(1, "write error on %s", filename);
+  }
+  free(rp);
+  free(col);
+  free(val);
+  return rc;
+}
+
+int quac_write_qvec_petsc_binary(qvec state, const char filename[]) {
+  double *h = (double *)malloc(sizeof(double) * 2 * state->n);
+  int rc = qh_vec_get(state->data, h);
+  FILE *fp = rc ? NULL : fopen(filename, "wb");
+  if (rc) rc = quac_fail(9, "quac_write_qvec_petsc_binary: %s", qh_last_error());
+  else if (!fp) rc = quac_fail(1, "cannot open %s for writing", filename);
+  if (fp) {
+    put_be64(fp, 1211214);
+    put_be64(fp, (uint64_t)state->n);
+    for (PetscInt i = 0; i < 2 * state->n; i++) put_be_double(fp, h[i]);
+    if (fclose(fp)) rc = quac_fail(1, "write error on %s", filename);
+  }
+  free(h);
+  return rc;
+}
