@@ -60,6 +60,7 @@ struct qh_mat {
   double2 *val = nullptr;      // nnz complex128
   float2 *val32 = nullptr;     // nnz complex64 (FP32 path), made on demand
   int lanes64 = 0, lanes32 = 0;  // SpMV variant: lanes per row of the CSR kernel, or QH_KSELL
+  int lanes_forced = 0;          // set by qh_mat_set_lanes (not by the autotuner)
   int32_t max_row = 0;         // longest row
   // slot-aligned sliced ELL (built on demand from the CSR)
   int ks_state = 0;            // 0 not built, 1 built, -1 not applicable (a row longer than 64)

@@ -898,6 +898,179 @@ extern "C" int qh_last_stage_ms(double *spmv_ms, int64_t *launches) {
 
 // Advance x in place by nsteps RK4 steps.  Steps are grouped in pairs (x -> alt
 // -> x) so a captured graph always starts and ends on the caller's buffer.
+// ------------------------------------------------ small systems in one workgroup
+// For a generator of at most QH_SMALL_ROWS rows the whole RK4 integration runs
+// in one workgroup with the state and the three stage vectors in LDS (5 x 16 B
+// per row): every stage is one pass over the CSR rows followed by a barrier, so
+// a step costs four barriers instead of four kernel launches.  The state is kept
+// in FP64 in LDS for both storage precisions; the matrix values are FP64.
+#define QH_SMALL_ROWS 800
+
+// MAXNZ > 0: every thread owns at most one row and keeps its (at most MAXNZ)
+// entries in registers; MAXNZ == 0: rows are read from the CSR in memory.
+template <bool F32, int MAXNZ>
+__global__ void __launch_bounds__(1024) k_rk4_small(int32_t n, const int32_t *__restrict__ rp,
+                                                    const int32_t *__restrict__ col, const double2 *__restrict__ val,
+                                                    void *xg, float2 *xlo, double h, int64_t nsteps, int64_t tot,
+                                                    int renorm_every) {
+  extern __shared__ double2 sm[];
+  double2 *xa = sm, *xb = sm + n, *k1 = sm + 2 * n, *k2 = sm + 3 * n, *k3 = sm + 4 * n;
+  __shared__ double2 scale;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  for (int r = tid; r < n; r += nt) {
+    if (F32) {
+      const float2 hi = ((const float2 *)xg)[r], lo = xlo[r];
+      xa[r] = make_double2((double)hi.x + (double)lo.x, (double)hi.y + (double)lo.y);
+    } else {
+      xa[r] = ((const double2 *)xg)[r];
+    }
+  }
+  __syncthreads();
+  // register copy of this thread's row (MAXNZ > 0: n <= blockDim.x, one row per thread)
+  int32_t rc_[MAXNZ > 0 ? MAXNZ : 1];
+  double2 rv_[MAXNZ > 0 ? MAXNZ : 1];
+  int rlen = 0;
+  if (MAXNZ > 0 && tid < n) {
+    const int32_t b = rp[tid];
+    rlen = rp[tid + 1] - b;
+#pragma unroll
+    for (int k = 0; k < (MAXNZ > 0 ? MAXNZ : 1); k++) {
+      rc_[k] = k < rlen ? col[b + k] : 0;
+      rv_[k] = k < rlen ? val[b + k] : make_double2(0.0, 0.0);
+    }
+  }
+  auto rowdot = [&](int r, const double2 *xs, const double2 *kin, double c) {
+    double are = 0.0, aim = 0.0;
+    if (MAXNZ > 0) {
+#pragma unroll
+      for (int k = 0; k < (MAXNZ > 0 ? MAXNZ : 1); k++) {
+        if (k < rlen) {
+          double2 v = xs[rc_[k]];
+          if (kin) {
+            v.x += c * kin[rc_[k]].x;
+            v.y += c * kin[rc_[k]].y;
+          }
+          const double2 a = rv_[k];
+          are = fma(a.x, v.x, fma(-a.y, v.y, are));
+          aim = fma(a.x, v.y, fma(a.y, v.x, aim));
+        }
+      }
+      return make_double2(are, aim);
+    }
+    for (int32_t k = rp[r]; k < rp[r + 1]; k++) {
+      const int32_t cc = col[k];
+      double2 v = xs[cc];
+      if (kin) {
+        v.x += c * kin[cc].x;
+        v.y += c * kin[cc].y;
+      }
+      const double2 a = val[k];
+      are = fma(a.x, v.x, fma(-a.y, v.y, are));
+      aim = fma(a.x, v.y, fma(a.y, v.x, aim));
+    }
+    return make_double2(are, aim);
+  };
+  const double s6 = h * (1.0 / 6.0), s3 = h * (1.0 / 3.0);
+  for (int64_t st = 0; st < nsteps; st++) {
+    for (int r = tid; r < n; r += nt) k1[r] = rowdot(r, xa, nullptr, 0.0);
+    __syncthreads();
+    for (int r = tid; r < n; r += nt) k2[r] = rowdot(r, xa, k1, 0.5 * h);
+    __syncthreads();
+    for (int r = tid; r < n; r += nt) k3[r] = rowdot(r, xa, k2, 0.5 * h);
+    __syncthreads();
+    for (int r = tid; r < n; r += nt) {
+      const double2 k4 = rowdot(r, xa, k3, h), a = k1[r], b = k2[r], c = k3[r], xo = xa[r];
+      xb[r] = make_double2(xo.x + s6 * a.x + s3 * b.x + s3 * c.x + s6 * k4.x,
+                           xo.y + s6 * a.y + s3 * b.y + s3 * c.y + s6 * k4.y);
+    }
+    __syncthreads();
+    double2 *t = xa;
+    xa = xb;
+    xb = t;
+    if (renorm_every > 0 && (st + 1) % renorm_every == 0) {
+      if (tid == 0) {
+        double tr = 0.0, ti = 0.0;
+        for (int64_t i = 0; i < tot; i++) {
+          tr += xa[i * (tot + 1)].x;
+          ti += xa[i * (tot + 1)].y;
+        }
+        const double d = tr * tr + ti * ti;
+        scale = d > 0.0 ? make_double2(tr / d, -ti / d) : make_double2(1.0, 0.0);
+      }
+      __syncthreads();
+      const double2 sc = scale;
+      for (int r = tid; r < n; r += nt) {
+        const double2 v = xa[r];
+        xa[r] = make_double2(sc.x * v.x - sc.y * v.y, sc.x * v.y + sc.y * v.x);
+      }
+      __syncthreads();
+    }
+  }
+  for (int r = tid; r < n; r += nt) {
+    const double2 v = xa[r];
+    if (F32) {
+      const float hr = (float)v.x, hm = (float)v.y;
+      ((float2 *)xg)[r] = make_float2(hr, hm);
+      xlo[r] = make_float2((float)(v.x - (double)hr), (float)(v.y - (double)hm));
+    } else {
+      ((double2 *)xg)[r] = v;
+    }
+  }
+}
+
+// longest row of a CSR (one block, grid-stride)
+__global__ void k_max_row(const int32_t *__restrict__ rp, int64_t n, int32_t *out) {
+  __shared__ int32_t red[256];
+  int32_t m = 0;
+  for (int64_t r = threadIdx.x; r < n; r += 256) m = max(m, rp[r + 1] - rp[r]);
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] = max(red[threadIdx.x], red[threadIdx.x + o]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = red[0];
+}
+
+static int ensure_max_row(qh_mat *A) {
+  if (A->max_row > 0) return QH_OK;
+  int32_t *d = nullptr;
+  int rc = qh_alloc_t(&d, 1);
+  if (rc) return rc;
+  hipStream_t s = qh_stream();
+  hipLaunchKernelGGL(k_max_row, dim3(1), dim3(256), 0, s, A->rowptr, A->rows, d);
+  int32_t h = 0;
+  QH_HIP(hipMemcpyAsync(&h, d, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  QH_HIP(hipStreamSynchronize(s));
+  qh_free_ptr(d);
+  A->max_row = h > 0 ? h : 1;
+  return QH_OK;
+}
+
+static int rk4_small(qh_mat *A, qh_vec *x, double dt, int64_t nsteps, int64_t tot, int renorm_every) {
+  int rc0 = ensure_max_row(A);
+  if (rc0) return rc0;
+  const int32_t n = (int32_t)A->rows;
+  int bt = ((n + 63) / 64) * 64;
+  if (bt > 1024) bt = 1024;
+  const size_t lds = (size_t)5 * n * sizeof(double2);
+  hipStream_t s = qh_stream();
+  const int mz = A->max_row <= 8 ? 8 : A->max_row <= 16 ? 16 : A->max_row <= 32 ? 32 : 0;
+#define QH_LS(F, Z)                                                                                             \
+  hipLaunchKernelGGL((k_rk4_small<F, Z>), dim3(1), dim3(bt), lds, s, n, A->rowptr, A->col, A->val, x->d, x->lo, \
+                     dt, nsteps, tot, renorm_every)
+  const bool f32 = x->prec == QH_FP32;
+  switch (mz) {
+    case 8: if (f32) QH_LS(true, 8); else QH_LS(false, 8); break;
+    case 16: if (f32) QH_LS(true, 16); else QH_LS(false, 16); break;
+    case 32: if (f32) QH_LS(true, 32); else QH_LS(false, 32); break;
+    default: if (f32) QH_LS(true, 0); else QH_LS(false, 0); break;
+  }
+#undef QH_LS
+  QH_HIP(hipGetLastError());
+  return QH_OK;
+}
+
 static int rk4_impl(qh_mat *A, qh_vec *x, double dt, int64_t nsteps, int64_t tot, int renorm_every, int timed) {
   int rc = qh_ensure_init();
   if (rc) return rc;
@@ -906,6 +1079,11 @@ static int rk4_impl(qh_mat *A, qh_vec *x, double dt, int64_t nsteps, int64_t tot
   QH_REQUIRE(nsteps >= 0, QH_EINVAL, "qh_rk4: negative step count");
   QH_REQUIRE(renorm_every == 0 || (x->prec == QH_FP32 && tot > 0 && tot * tot == x->n), QH_EINVAL,
              "qh_rk4: trace renormalisation needs an FP32 density matrix of dimension tot");
+  // small generators (and no variant forced by qh_mat_set_lanes): one workgroup, LDS-resident
+  if (!timed && A->rows <= QH_SMALL_ROWS && !A->lanes_forced) {
+    if (nsteps == 0) return QH_OK;
+    return rk4_small(A, x, dt, nsteps, tot, renorm_every);
+  }
   int G = 0;
   rc = lanes_for(A, x->prec, &G);
   if (rc) return rc;
@@ -1016,6 +1194,7 @@ extern "C" int qh_mat_set_lanes(qh_mat *m, int l64, int l32) {
   QH_REQUIRE(ok(l64) && ok(l32), QH_EINVAL, "lanes per row must be a power of two <= 64, or 128..132 (sliced layout)");
   m->lanes64 = l64;
   m->lanes32 = l32;
+  m->lanes_forced = (l64 != 0 || l32 != 0);
   if (g_st.A == m) g_st.gG = -1;  // force a re-capture
   return QH_OK;
 }

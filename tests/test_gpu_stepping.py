@@ -245,3 +245,30 @@ def test_steps_max_stops_early():
     steps, tend = qs.time_step(x, 0.0, 1.0, 0.01, 7)
     assert steps == 7 and abs(tend - 0.07) < 1e-12
     qs.destroy()
+
+
+def test_small_system_single_workgroup_matches_regular_kernels():
+    """generators of <= 800 rows run all steps in one LDS-resident workgroup;
+    same result as the multi-launch kernels and the oracle"""
+    r = recipes.jc_recipe()
+    N, loc = recipes.jc_init()
+    qs, _, _ = build_system(r)
+    A = qs.matrix()
+    x0 = np.zeros(N * N, np.complex128)
+    x0[loc] = 1
+    ref = _oracle_run(r, x0, 0.01, 300)
+    v = qa.DeviceVec(N * N, qa.FP64)
+    v.set(x0)
+    qa.rk4_device(A, v, 0.01, 300)
+    assert np.abs(v.get() - ref).max() < 1e-12
+    A.set_lanes(8, 8)
+    w = qa.DeviceVec(N * N, qa.FP64)
+    w.set(x0)
+    qa.rk4_device(A, w, 0.01, 300)
+    assert np.abs(w.get() - ref).max() < 1e-12
+    A.set_lanes(0, 0)
+    f = qa.DeviceVec(N * N, qa.FP32)
+    f.set(x0)
+    qa.rk4_device(A, f, 0.01, 300, tot=N, renorm_every=50)
+    assert np.abs(populations_np(f.get(), r) - populations_np(ref, r)).max() < 1e-6
+    qs.destroy()
