@@ -78,7 +78,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--renorm", type=int, default=100)
-    ap.add_argument("--workload", default="chain10", choices=["chain10", "tc512", "circuit12", "jc", "copy", "kerr"])
+    ap.add_argument("--workload", default="chain10", choices=["chain10", "tc512", "circuit12", "jc", "copy", "kerr", "mcwf"])
     ap.add_argument("--instances", type=int, default=512)
     ap.add_argument("--levels", type=int, default=8)
     ap.add_argument("--train", type=int, default=2000)

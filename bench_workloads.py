@@ -15,6 +15,9 @@ kerr      : learned steady-state warm start on 3 driven Kerr cavities x 8
             parameter sets from the device GMRES, low-rank basis, network
             trained on the device, then GMRES to 1e-8 on held-out sets with and
             without the network's warm start.
+mcwf      : Monte Carlo wavefunction ensemble of the Jaynes-Cummings 2 x 10 system
+            (10 000 trajectories, RK4 dt = 0.01 to t = 10) against the density-matrix
+            RK4 of the same system: trajectory-steps/s and the ensemble error.
 copy      : plain streaming copy of known size (complex128 -> complex64 hi/lo
             split: 16 B read + 16 B written per element), the calibration run
             for the HBM byte counters (FETCH_SIZE / WRITE_SIZE).
@@ -343,3 +346,47 @@ def kerr(args):
             "training": {"solved": len(feats), "skipped": skipped, "mean_iterations": float(np.mean(tr_its)),
                          "basis_err": stats["train_basis_err"], "network_err": stats["train_net_err"],
                          "loss_first_last": [stats["losses"][0][1], stats["losses"][-1][1]]}}
+
+
+def mcwf(args):
+    M, dt, T = 10000, 0.01, 10.0
+    steps = int(round(T / dt))
+    qa.set_option("ts_rk_type", "4")
+    qa.set_option("ts_adapt_type", "none")
+    recipe = recipes.jc_recipe()
+    N, loc = recipes.jc_init()
+    # density-matrix reference on the device (FP64 drop-in path)
+    qd, _, _ = build_system(recipe)
+    xd = qd.create_qvec()
+    xd.add_loc(1.0, loc)
+    qd.time_step(xd, 0.0, T, dt, 10 ** 7)
+    subs = subsystem_layout(recipe)
+    ref = _pops(xd.data(), subs)
+    qd.destroy()
+    # trajectories
+    qs = qa.System()
+    ops = []
+    for (kind, L) in recipe["subs"]:
+        ops.append(qs.create_op(L))
+    for (k, a, o) in recipe["terms"]:
+        plist = [{"lower": ops[s_], "raise": ops[s_].dag, "n": ops[s_].n}[nm] for (s_, nm) in o]
+        (qs.add_ham_term if k == "ham" else qs.add_lin_term)(a, *plist)
+    qs.use_mcwf_solver(M, 5)
+    qs.construct_matrix()
+    x = qs.create_qvec()
+    x.add(1.0, loc // N)      # |e,0> as a wavefunction: Hilbert index of the DM diagonal entry
+    qa.synchronize()
+    t0 = time.perf_counter()
+    qs.time_step(x, 0.0, T, dt, 10 ** 7)
+    qa.synchronize()
+    el = time.perf_counter() - t0
+    got = [x.expectation(ops[0].n).real, x.expectation(ops[1].n).real]
+    qs.destroy()
+    return {"metric": "MCWF trajectory RK4 steps/s (ensemble average vs density matrix)",
+            "value": round(M * steps / el, 1), "unit": "trajectory RK4 steps/s", "n_gpus": 1, "steps": steps,
+            "warmup": 0, "ms_per_step": round(1e3 * el / steps, 4), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "fp64", "data": "synthetic",
+            "config": {"workload": f"Jaynes-Cummings 2x10 wavefunctions, {M} trajectories, RK4 dt={dt} to t={T}"},
+            "populations_mcwf": got, "populations_dm": [float(v) for v in ref],
+            "max_abs_err": float(max(abs(a - b) for a, b in zip(got, ref))),
+            "sem_bound": 4 * 4.5 / math.sqrt(M)}

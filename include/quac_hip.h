@@ -206,6 +206,9 @@ int qh_ens_broadcast(const qh_vec *init, qh_vec *ens, int64_t M);
 int qh_ens_rk4(const qh_mat *A, qh_vec *ens, int64_t M, double dt, int64_t nsteps);
 /* squared norm of each trajectory into norms2[M] (host) */
 int qh_ens_norms(const qh_vec *ens, int64_t M, double *norms2);
+/* mean over the trajectories of <psi|O|psi>/<psi|psi> (qvec_utilities.c:1353-1420) */
+int qh_ens_expect(const qh_vec *ens, int64_t M, int64_t tot, int nops, const qh_op *ops, int nlev,
+                  const qh_level *levels, double *re, double *im);
 
 /* --------------------------------------------------- warm-start network */
 /*

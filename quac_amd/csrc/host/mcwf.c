@@ -178,22 +178,5 @@ int quac_mcwf_time_step(qsystem s, qvec x, PetscReal t0, PetscReal tmax, PetscRe
 
 /* ensemble average of <psi_i|O|psi_i> / <psi_i|psi_i> (qvec_utilities.c:1353-1420) */
 int quac_mcwf_expect(qvec x, qc_chain *c, double *re, double *im) {
-  const int64_t M = x->n_ensemble, N = x->n;
-  Vec block = x->ens_datas[0];
-  double *nrm = (double *)malloc(sizeof(double) * (size_t)M);
-  int rc = qh_ens_norms(block, M, nrm);
-  double sr = 0.0, si = 0.0;
-  for (int64_t i = 0; i < M && !rc; i++) {
-    Vec v = NULL;
-    double er = 0.0, ei = 0.0;
-    rc = qh_vec_view(block, i * N, N, &v);
-    if (!rc) rc = qh_expect_wf(v, x->total_levels, c->nops, c->ops, c->nlev, c->lev, &er, &ei);
-    qh_vec_destroy(v);
-    sr += er / nrm[i];
-    si += ei / nrm[i];
-  }
-  free(nrm);
-  *re = sr / (double)M;
-  *im = si / (double)M;
-  return rc;
+  return qh_ens_expect(x->ens_datas[0], x->n_ensemble, x->n, c->nops, c->ops, c->nlev, c->lev, re, im);
 }

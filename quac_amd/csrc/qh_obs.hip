@@ -80,6 +80,39 @@ __global__ void __launch_bounds__(256) k_expect_wf(const void *x, const float2 *
   block_sum2(ar, ai, partial);
 }
 
+// ensemble of M wavefunctions (trajectory b at [b*tot, (b+1)*tot)): block b writes
+// <psi_b|O|psi_b> / <psi_b|psi_b> (qvec_utilities.c:1353-1420)
+__global__ void __launch_bounds__(256) k_expect_ens(const double2 *x, int64_t tot, const qh_op *ops, int nops,
+                                                    const qh_level *tab, double2 *out) {
+  const double2 *p = x + (int64_t)blockIdx.x * tot;
+  double ar = 0.0, ai = 0.0, nn = 0.0;
+  for (int64_t i = threadIdx.x; i < tot; i += 256) {
+    const double2 pi = p[i];
+    nn += pi.x * pi.x + pi.y * pi.y;
+    int64_t k;
+    double2 v;
+    if (!chain_ig(ops, nops, tab, i, k, v)) continue;
+    const double2 pk = p[k];
+    const double tr = v.x * pk.x - v.y * pk.y, ti = v.x * pk.y + v.y * pk.x;
+    ar += pi.x * tr + pi.y * ti;
+    ai += pi.x * ti - pi.y * tr;
+  }
+  __shared__ double sa[256], sb[256], sn[256];
+  sa[threadIdx.x] = ar;
+  sb[threadIdx.x] = ai;
+  sn[threadIdx.x] = nn;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      sa[threadIdx.x] += sa[threadIdx.x + o];
+      sb[threadIdx.x] += sb[threadIdx.x + o];
+      sn[threadIdx.x] += sn[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = make_double2(sa[0] / sn[0], sb[0] / sn[0]);
+}
+
 __global__ void __launch_bounds__(256) k_trace(const void *x, const float2 *lo, int64_t tot, double2 *partial) {
   double ar = 0.0, ai = 0.0;
   for (int64_t i = blockIdx.x * 256 + threadIdx.x; i < tot; i += (int64_t)gridDim.x * 256) {
@@ -316,5 +349,41 @@ extern "C" int qh_batch_populations(qh_batch *b, const int *nsub, const int64_t 
   qh_free_ptr(d_lev);
   qh_free_ptr(d_str);
   qh_free_ptr(d_out);
+  return rc;
+}
+
+// ensemble average of the normalised per-trajectory expectation values
+extern "C" int qh_ens_expect(const qh_vec *ens, int64_t M, int64_t tot, int nops, const qh_op *ops, int nlev,
+                             const qh_level *levels, double *re, double *im) {
+  int rc = qh_ensure_init();
+  if (rc) return rc;
+  QH_REQUIRE(ens && re && im && ens->prec == QH_FP64 && M >= 1 && M <= (1 << 30) && ens->n == M * tot, QH_EINVAL,
+             "qh_ens_expect: FP64 ensemble of M trajectories of dimension tot required");
+  qh_op *dops = nullptr;
+  qh_level *dlev = nullptr;
+  double2 *out = nullptr;
+  rc = upload_chain(nops, ops, nlev, levels, &dops, &dlev);
+  if (!rc) rc = qh_alloc_t(&out, (size_t)M);
+  if (!rc) {
+    hipLaunchKernelGGL(k_expect_ens, dim3((unsigned)M), dim3(256), 0, qh_stream(), (const double2 *)ens->d, tot,
+                       dops, nops, dlev, out);
+    std::vector<double2> h((size_t)M);
+    if (hipMemcpyAsync(h.data(), out, sizeof(double2) * M, hipMemcpyDeviceToHost, qh_stream()) != hipSuccess ||
+        hipStreamSynchronize(qh_stream()) != hipSuccess) {
+      qh_set_error("qh_ens_expect: device copy failed");
+      rc = QH_EHIP;
+    } else {
+      double sr = 0.0, si = 0.0;
+      for (int64_t b = 0; b < M; b++) {
+        sr += h[b].x;
+        si += h[b].y;
+      }
+      *re = sr / (double)M;
+      *im = si / (double)M;
+    }
+  }
+  qh_free_ptr(dops);
+  qh_free_ptr(dlev);
+  qh_free_ptr(out);
   return rc;
 }
