@@ -354,6 +354,7 @@ extern "C" int qh_mat_destroy(qh_mat *m) {
   qh_free_ptr(m->ks_meta);
   qh_free_ptr(m->ks_val);
   qh_free_ptr(m->ks_val32);
+  qh_free_ptr(m->ks_meta32);
   delete m;
   return QH_OK;
 }

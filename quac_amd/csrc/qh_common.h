@@ -49,6 +49,15 @@ struct qh_kmeta {
   double vre, vim;
 };
 #define QH_KS_UNIFORM 0xFFFFFFFFu
+// The FP32 path's copy of the slot metadata, 32 B per slot (one scalar load):
+// the byte offset ad8 = 8 |d| of a complex64 gather, and in `base` the
+// compacted-value index of a non-uniform slot or, for a uniform slot, the
+// negated offset -ad8; the uniform value in FP32.
+struct qh_kmeta32 {
+  uint64_t mask, sign;
+  uint32_t ad8, base;
+  float vre, vim;
+};
 // empty slots after the last one, so that a pass of up to 8 slots may read
 // its metadata without a bounds check
 #define QH_KS_PAD 16
@@ -70,6 +79,7 @@ struct qh_mat {
   qh_kmeta *ks_meta = nullptr;
   double2 *ks_val = nullptr;   // nnz, slice/slot/lane order
   float2 *ks_val32 = nullptr;
+  qh_kmeta32 *ks_meta32 = nullptr;  // FP32 metadata (ks_slots + QH_KS_PAD entries)
   int64_t ks_slots = 0;
   int64_t ks_nval = 0;         // stored (non-uniform) values
 };

@@ -285,6 +285,67 @@ __device__ __forceinline__ void ks_pass(const qh_kmeta *__restrict__ mp, int32_t
   }
 }
 
+// FP32 pass over U slots of the 32-byte metadata: one scalar load per slot, the
+// offsets pre-scaled, products summed in FP32 within the pass and the pass sum
+// added to the FP64 row sum.
+template <bool VALS, bool TAIL, int U>
+__device__ __forceinline__ void ks_pass32(const qh_kmeta32 *__restrict__ mp, int32_t cnt, uint32_t r8,
+                                          __amdgpu_buffer_rsrc_t xr, __amdgpu_buffer_rsrc_t vr, double &are,
+                                          double &aim) {
+  qh_kmeta32 m[U];
+  uint64_t mask[U];
+#pragma unroll
+  for (int j = 0; j < U; j++) {
+    m[j] = mp[j];
+    mask[j] = (!TAIL || j < cnt) ? m[j].mask : 0ull;
+  }
+  float2 xv[U], vv[U];
+#pragma unroll
+  for (int j = 0; j < U; j++) {
+    const bool on = __builtin_amdgcn_inverse_ballot_w64(mask[j]);
+    const bool neg = __builtin_amdgcn_inverse_ballot_w64(m[j].sign);
+    uint32_t off;
+    if (VALS)
+      off = neg ? r8 - m[j].ad8 : r8 + m[j].ad8;
+    else
+      off = r8 + (neg ? m[j].base : m[j].ad8);  // base holds -ad8 for uniform slots
+    off = on ? off : QH_KS_OOB;
+    xv[j] = ks_load<float>(xr, off);
+    if (VALS) {
+      const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask[j] >> 32),
+                                                      __builtin_amdgcn_mbcnt_lo((uint32_t)mask[j], 0u));
+      vv[j] = ks_load<float>(vr, (m[j].base + rank) * 8u);
+    }
+  }
+  float pr = 0.f, pi = 0.f;
+#pragma unroll
+  for (int j = 0; j < U; j++) {
+    float ar, ai;
+    if (VALS) {
+      ar = vv[j].x;
+      ai = vv[j].y;
+    } else {
+      ar = m[j].vre;
+      ai = m[j].vim;
+    }
+    pr = fmaf(ar, xv[j].x, fmaf(-ai, xv[j].y, pr));
+    pi = fmaf(ar, xv[j].y, fmaf(ai, xv[j].x, pi));
+  }
+  are += (double)pr;
+  aim += (double)pi;
+}
+
+template <bool VALS, int U>
+__device__ __forceinline__ void ks_group32(const qh_kmeta32 *__restrict__ meta, int32_t k, int32_t kend,
+                                           uint32_t r8, __amdgpu_buffer_rsrc_t xr, __amdgpu_buffer_rsrc_t vr,
+                                           double &are, double &aim) {
+  static_assert(U - 1 <= QH_KS_PAD, "tail pass reads past the padding");
+  const qh_kmeta32 *mp = meta + k;
+  const int32_t n = kend - k;
+  for (int32_t p = 0; p < n / U; p++, mp += U) ks_pass32<VALS, false, U>(mp, U, r8, xr, vr, are, aim);
+  if (n % U) ks_pass32<VALS, true, U>(mp, n % U, r8, xr, vr, are, aim);
+}
+
 // all slots [k, kend) of one group (uniform or not) of a slice
 template <typename R, bool VALS, int U, bool VV>
 __device__ __forceinline__ void ks_group(const qh_kmeta *__restrict__ meta, int32_t k, int32_t kend, uint32_t r8,
@@ -311,7 +372,8 @@ __device__ __forceinline__ void ks_group(const qh_kmeta *__restrict__ meta, int3
 template <typename R, int MODE, int U, bool VV>
 __global__ void __launch_bounds__(256) k_ks_stage(
     int32_t nrows, int32_t nslices, const int32_t *__restrict__ moff, const int32_t *__restrict__ umid,
-    const qh_kmeta *__restrict__ meta, int32_t nmeta, int32_t nval, const typename C2<R>::T *__restrict__ val,
+    const qh_kmeta *__restrict__ meta, const qh_kmeta32 *__restrict__ meta32, int32_t nmeta, int32_t nval,
+    const typename C2<R>::T *__restrict__ val,
     const typename C2<R>::T *__restrict__ xin, typename C2<R>::T *__restrict__ kout,
     typename C2<R>::T *__restrict__ ynext, R cn, const typename C2<R>::T *__restrict__ x,
     const typename C2<R>::T *__restrict__ k1, const typename C2<R>::T *__restrict__ k2,
@@ -329,8 +391,13 @@ __global__ void __launch_bounds__(256) k_ks_stage(
   const __amdgpu_buffer_rsrc_t mr = ks_rsrc(meta, (int64_t)nmeta * (int64_t)sizeof(qh_kmeta));
   const uint32_t r8 = (uint32_t)r * (uint32_t)sizeof(T);
   double are = 0.0, aim = 0.0;
-  ks_group<R, false, U, VV>(meta, k0, km, r8, xr, vr, mr, are, aim);
-  ks_group<R, true, U, VV>(meta, km, kend, r8, xr, vr, mr, are, aim);
+  if (sizeof(R) == 4 && !VV) {
+    ks_group32<false, U>(meta32, k0, km, r8, xr, vr, are, aim);
+    ks_group32<true, U>(meta32, km, kend, r8, xr, vr, are, aim);
+  } else {
+    ks_group<R, false, U, VV>(meta, k0, km, r8, xr, vr, mr, are, aim);
+    ks_group<R, true, U, VV>(meta, km, kend, r8, xr, vr, mr, are, aim);
+  }
   if (r >= nrows) return;
   T y;
   y.x = (R)are;
@@ -574,13 +641,37 @@ int ks_build(qh_mat *A) {
   return QH_OK;
 }
 
+__global__ void k_make_meta32(const qh_kmeta *m, int64_t n, qh_kmeta32 *o) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const qh_kmeta a = m[i];
+    qh_kmeta32 b;
+    b.mask = a.mask;
+    b.sign = a.sign;
+    b.ad8 = a.ad * 8u;
+    b.base = a.base == QH_KS_UNIFORM ? (uint32_t)(-(int32_t)(a.ad * 8u)) : a.base;
+    b.vre = (float)a.vre;
+    b.vim = (float)a.vim;
+    o[i] = b;
+  }
+}
+
 int ks_make_fp32(qh_mat *A) {
-  if (A->ks_state != 1 || A->ks_val32) return QH_OK;
-  int rc = qh_alloc_t(&A->ks_val32, (size_t)(A->ks_nval > 0 ? A->ks_nval : 1));
-  if (rc) return rc;
-  if (A->ks_nval)
-    hipLaunchKernelGGL(k_narrow_ks, dim3(qh_blocks(A->ks_nval, 256)), dim3(256), 0, qh_stream(), A->ks_val,
-                       A->ks_val32, A->ks_nval);
+  if (A->ks_state != 1 || (A->ks_val32 && A->ks_meta32)) return QH_OK;
+  int rc = QH_OK;
+  if (!A->ks_val32) {
+    rc = qh_alloc_t(&A->ks_val32, (size_t)(A->ks_nval > 0 ? A->ks_nval : 1));
+    if (rc) return rc;
+    if (A->ks_nval)
+      hipLaunchKernelGGL(k_narrow_ks, dim3(qh_blocks(A->ks_nval, 256)), dim3(256), 0, qh_stream(), A->ks_val,
+                         A->ks_val32, A->ks_nval);
+  }
+  if (!A->ks_meta32) {
+    const int64_t nm = A->ks_slots + QH_KS_PAD;
+    rc = qh_alloc_t(&A->ks_meta32, (size_t)nm);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_make_meta32, dim3(qh_blocks(nm, 256)), dim3(256), 0, qh_stream(), A->ks_meta, nm,
+                       A->ks_meta32);
+  }
   QH_HIP(hipGetLastError());
   return QH_OK;
 }
@@ -595,7 +686,7 @@ int launch_ks(int G, hipStream_t s, const qh_mat *A, const void *xin, void *kout
   const int32_t n = (int32_t)A->rows, ns = A->nslices;
 #define QH_LK(UU, VV)                                                                                       \
   hipLaunchKernelGGL((k_ks_stage<R, MODE, UU, VV>), grid, blk, 0, s, n, ns, A->ks_moff, A->ks_umid, A->ks_meta, \
-                     (int32_t)(A->ks_slots + QH_KS_PAD), (int32_t)A->ks_nval, kv,                           \
+                     A->ks_meta32, (int32_t)(A->ks_slots + QH_KS_PAD), (int32_t)A->ks_nval, kv,             \
                      (const T *)xin, (T *)kout, (T *)ynext, cn, (const T *)x, (const T *)k1, (const T *)k2, \
                      (const T *)k3, (T *)xout, h, xlo)
   switch (G) {
