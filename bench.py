@@ -166,7 +166,8 @@ def main():
     # bytes the sliced-layout step actually has to move: 4 x (stored values + slot
     # metadata) + 17 vector passes (4 gathers, 3 K and 3 Y writes, 3 x reads for
     # the Y's, K1..K3 + x reads and the x' write in the last epilogue)
-    ks_step32 = 4 * (layout["stored_vals"] * 8 + layout["meta_bytes"]) + 17 * rows * 8
+    meta32 = layout["slots"] * 32 + 12 * layout["slices"]
+    ks_step32 = 4 * (layout["stored_vals"] * 8 + layout["stored_reals"] * 4 + meta32) + 17 * rows * 8
 
     # mean stage-kernel duration over the timed FP32 region: its device time
     # (event pair) over its 4K stage launches (includes the renormalisation

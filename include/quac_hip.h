@@ -70,9 +70,11 @@ int qh_mat_blockdiag(int n, qh_mat *const *mats, qh_mat **out);
  * pass, 131/132 with vector-loaded uniform values); 0 = autotune */
 int qh_mat_set_lanes(qh_mat *m, int lanes64, int lanes32);
 int qh_mat_get_lanes(const qh_mat *m, int *lanes64, int *lanes32);
-/* sliced layout statistics (built on demand): slices, slots, stored values
- * (values of non-uniform slots), and the metadata bytes */
-int qh_mat_layout_info(qh_mat *m, int64_t *slices, int64_t *slots, int64_t *stored_vals, int64_t *meta_bytes);
+/* sliced layout statistics (built on demand): slices, slots, stored complex
+ * values (of complex slots), stored real values (of real-or-imaginary slots),
+ * and the metadata bytes */
+int qh_mat_layout_info(qh_mat *m, int64_t *slices, int64_t *slots, int64_t *stored_vals, int64_t *stored_reals,
+                       int64_t *meta_bytes);
 
 /* ------------------------------------------------------------------ vectors */
 typedef struct qh_vec qh_vec;

@@ -351,6 +351,9 @@ extern "C" int qh_mat_destroy(qh_mat *m) {
   qh_free_ptr(m->val32);
   qh_free_ptr(m->ks_moff);
   qh_free_ptr(m->ks_umid);
+  qh_free_ptr(m->ks_cmid);
+  qh_free_ptr(m->ks_sval);
+  qh_free_ptr(m->ks_sval32);
   qh_free_ptr(m->ks_meta);
   qh_free_ptr(m->ks_val);
   qh_free_ptr(m->ks_val32);
@@ -399,6 +402,11 @@ __global__ void k_narrow_vals(const double2 *a, float2 *b, int64_t n) {
     b[i] = make_float2((float)a[i].x, (float)a[i].y);
 }
 
+__global__ void k_narrow_reals(const double *a, float *b, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    b[i] = (float)a[i];
+}
+
 extern "C" int qh_mat_make_fp32(qh_mat *m) {
   QH_REQUIRE(m, QH_EINVAL, "null matrix");
   if (!m->val32) {
@@ -408,9 +416,12 @@ extern "C" int qh_mat_make_fp32(qh_mat *m) {
   if (m->nnz)
     hipLaunchKernelGGL(k_narrow_vals, dim3(qh_blocks(m->nnz, 256)), dim3(256), 0, qh_stream(), m->val, m->val32,
                        m->nnz);
-  if (m->ks_val32 && m->ks_nval)  // refresh the sliced-layout copy too
+  if (m->ks_val32 && m->ks_nval)  // refresh the sliced-layout copies too
     hipLaunchKernelGGL(k_narrow_vals, dim3(qh_blocks(m->ks_nval, 256)), dim3(256), 0, qh_stream(), m->ks_val,
                        m->ks_val32, m->ks_nval);
+  if (m->ks_sval32 && m->ks_nsval)
+    hipLaunchKernelGGL(k_narrow_reals, dim3(qh_blocks(m->ks_nsval, 256)), dim3(256), 0, qh_stream(), m->ks_sval,
+                       m->ks_sval32, m->ks_nsval);
   QH_HIP(hipGetLastError());
   return QH_OK;
 }

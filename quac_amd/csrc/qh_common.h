@@ -41,8 +41,13 @@ int qh_num_cus();
 // value array (values of active lanes are stored consecutively in lane order).
 // When every active lane of the slot holds the bit-identical value, the value
 // is kept in the metadata (vre, vim) and base is QH_KS_UNIFORM: no per-lane
-// value is stored or loaded for that slot.  A slice lists its uniform slots
-// first, then the others, each group in increasing (ad, rank) order.
+// value is stored or loaded for that slot.  When the values differ but are all
+// real, or all imaginary, the slot is a scalar slot: one real number per lane
+// is stored in the scalar value array (index base + lane rank) and the value of
+// the lane is that number times (vre, vim) = (1, 0) or (0, 1).  The remaining
+// slots store complex values.  A slice lists its uniform slots first, then its
+// scalar slots, then its complex slots, each group in increasing (ad, rank)
+// order.
 struct qh_kmeta {
   uint64_t mask, sign;
   uint32_t ad, base;
@@ -51,8 +56,8 @@ struct qh_kmeta {
 #define QH_KS_UNIFORM 0xFFFFFFFFu
 // The FP32 path's copy of the slot metadata, 32 B per slot (one scalar load):
 // the byte offset ad8 = 8 |d| of a complex64 gather, and in `base` the
-// compacted-value index of a non-uniform slot or, for a uniform slot, the
-// negated offset -ad8; the uniform value in FP32.
+// value index of a scalar or complex slot or, for a uniform slot, the negated
+// offset -ad8; (vre, vim) in FP32.
 struct qh_kmeta32 {
   uint64_t mask, sign;
   uint32_t ad8, base;
@@ -76,12 +81,16 @@ struct qh_mat {
   int32_t nslices = 0;
   int32_t *ks_moff = nullptr;  // nslices+1 slot offsets
   int32_t *ks_umid = nullptr;  // nslices: end of the slice's uniform slots (they come first)
+  int32_t *ks_cmid = nullptr;  // nslices: end of the slice's scalar slots (the complex ones follow)
   qh_kmeta *ks_meta = nullptr;
-  double2 *ks_val = nullptr;   // nnz, slice/slot/lane order
+  double2 *ks_val = nullptr;   // values of the complex slots, slice/slot/lane order
   float2 *ks_val32 = nullptr;
+  double *ks_sval = nullptr;   // values of the scalar slots, slice/slot/lane order
+  float *ks_sval32 = nullptr;
   qh_kmeta32 *ks_meta32 = nullptr;  // FP32 metadata (ks_slots + QH_KS_PAD entries)
   int64_t ks_slots = 0;
-  int64_t ks_nval = 0;         // stored (non-uniform) values
+  int64_t ks_nval = 0;         // stored complex values
+  int64_t ks_nsval = 0;        // stored scalar values
 };
 // variant codes selecting the sliced-ELL kernel: 128 / 129 / 130 process 2 / 4 / 8
 // slots per pass; 131 / 132 process 4 / 8 slots per pass and read the uniform

@@ -221,16 +221,31 @@ __device__ inline double2 ks_load<double>(__amdgpu_buffer_rsrc_t rs, uint32_t of
   return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0));
 }
 
+template <typename R>
+__device__ inline R ks_load_real(__amdgpu_buffer_rsrc_t rs, uint32_t off);
+template <>
+__device__ inline float ks_load_real<float>(__amdgpu_buffer_rsrc_t rs, uint32_t off) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, off, 0, 0));
+}
+template <>
+__device__ inline double ks_load_real<double>(__amdgpu_buffer_rsrc_t rs, uint32_t off) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, off, 0, 0));
+}
+
+// slot groups of a slice
+enum { KS_UNI = 0, KS_SCA = 1, KS_CPX = 2 };
+
 // One pass over U slots of a slice starting at mp; cnt valid slots when TAIL
 // (the metadata array carries QH_KS_PAD empty slots past its end, so the reads
 // of a tail pass never leave it; slots past cnt get an empty lane mask).
-// VALS: the slots carry per-lane values in the compacted stream, else the
-// slot's uniform value is in its metadata.
+// KIND: uniform slots take their value from the metadata, scalar slots load one
+// real per lane (vr over the scalar values) and multiply it by (vre, vim),
+// complex slots load one complex value per lane (vr over the complex values).
 // VV: the uniform value of a slot is read with a vector load (broadcast to all
 // lanes) instead of a scalar load, which keeps the metadata held in SGPRs at six
 // dwords per slot and lets a pass carry 8 slots; mr covers the metadata array
 // and k is the slot index of mp[0].
-template <typename R, bool VALS, bool TAIL, int U, bool VV>
+template <typename R, int KIND, bool TAIL, int U, bool VV>
 __device__ __forceinline__ void ks_pass(const qh_kmeta *__restrict__ mp, int32_t cnt, uint32_t r8,
                                         __amdgpu_buffer_rsrc_t xr, __amdgpu_buffer_rsrc_t vr,
                                         __amdgpu_buffer_rsrc_t mr, int32_t k, double &are, double &aim) {
@@ -243,8 +258,9 @@ __device__ __forceinline__ void ks_pass(const qh_kmeta *__restrict__ mp, int32_t
     mask[j] = (!TAIL || j < cnt) ? m[j].mask : 0ull;
   }
   T xv[U], vv[U];
+  R sv[U];
   double2 uv[U];
-  if (!VALS && VV) {
+  if (KIND == KS_UNI && VV) {
 #pragma unroll
     for (int j = 0; j < U; j++)
       uv[j] = __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(
@@ -259,20 +275,26 @@ __device__ __forceinline__ void ks_pass(const qh_kmeta *__restrict__ mp, int32_t
     uint32_t off = neg ? r8 - ad8 : r8 + ad8;
     off = on ? off : QH_KS_OOB;
     xv[j] = ks_load<R>(xr, off);
-    if (VALS) {
+    if (KIND != KS_UNI) {
       // rank of the lane among the slot's active lanes
       const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask[j] >> 32),
                                                       __builtin_amdgcn_mbcnt_lo((uint32_t)mask[j], 0u));
-      vv[j] = ks_load<R>(vr, (m[j].base + rank) * (uint32_t)sizeof(T));
+      if (KIND == KS_SCA)
+        sv[j] = ks_load_real<R>(vr, (m[j].base + rank) * (uint32_t)sizeof(R));
+      else
+        vv[j] = ks_load<R>(vr, (m[j].base + rank) * (uint32_t)sizeof(T));
     }
   }
 #pragma unroll
   for (int j = 0; j < U; j++) {
     const double xre = (double)xv[j].x, xim = (double)xv[j].y;
     double ar, ai;
-    if (VALS) {
+    if (KIND == KS_CPX) {
       ar = (double)vv[j].x;
       ai = (double)vv[j].y;
+    } else if (KIND == KS_SCA) {
+      ar = (double)sv[j] * m[j].vre;
+      ai = (double)sv[j] * m[j].vim;
     } else if (VV) {
       ar = uv[j].x;
       ai = uv[j].y;
@@ -288,7 +310,7 @@ __device__ __forceinline__ void ks_pass(const qh_kmeta *__restrict__ mp, int32_t
 // FP32 pass over U slots of the 32-byte metadata: one scalar load per slot, the
 // offsets pre-scaled, products summed in FP32 within the pass and the pass sum
 // added to the FP64 row sum.
-template <bool VALS, bool TAIL, int U>
+template <int KIND, bool TAIL, int U>
 __device__ __forceinline__ void ks_pass32(const qh_kmeta32 *__restrict__ mp, int32_t cnt, uint32_t r8,
                                           __amdgpu_buffer_rsrc_t xr, __amdgpu_buffer_rsrc_t vr, double &are,
                                           double &aim) {
@@ -300,30 +322,37 @@ __device__ __forceinline__ void ks_pass32(const qh_kmeta32 *__restrict__ mp, int
     mask[j] = (!TAIL || j < cnt) ? m[j].mask : 0ull;
   }
   float2 xv[U], vv[U];
+  float sv[U];
 #pragma unroll
   for (int j = 0; j < U; j++) {
     const bool on = __builtin_amdgcn_inverse_ballot_w64(mask[j]);
     const bool neg = __builtin_amdgcn_inverse_ballot_w64(m[j].sign);
     uint32_t off;
-    if (VALS)
+    if (KIND != KS_UNI)
       off = neg ? r8 - m[j].ad8 : r8 + m[j].ad8;
     else
       off = r8 + (neg ? m[j].base : m[j].ad8);  // base holds -ad8 for uniform slots
     off = on ? off : QH_KS_OOB;
     xv[j] = ks_load<float>(xr, off);
-    if (VALS) {
+    if (KIND != KS_UNI) {
       const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask[j] >> 32),
                                                       __builtin_amdgcn_mbcnt_lo((uint32_t)mask[j], 0u));
-      vv[j] = ks_load<float>(vr, (m[j].base + rank) * 8u);
+      if (KIND == KS_SCA)
+        sv[j] = ks_load_real<float>(vr, (m[j].base + rank) * 4u);
+      else
+        vv[j] = ks_load<float>(vr, (m[j].base + rank) * 8u);
     }
   }
   float pr = 0.f, pi = 0.f;
 #pragma unroll
   for (int j = 0; j < U; j++) {
     float ar, ai;
-    if (VALS) {
+    if (KIND == KS_CPX) {
       ar = vv[j].x;
       ai = vv[j].y;
+    } else if (KIND == KS_SCA) {
+      ar = sv[j] * m[j].vre;
+      ai = sv[j] * m[j].vim;
     } else {
       ar = m[j].vre;
       ai = m[j].vim;
@@ -335,19 +364,19 @@ __device__ __forceinline__ void ks_pass32(const qh_kmeta32 *__restrict__ mp, int
   aim += (double)pi;
 }
 
-template <bool VALS, int U>
+template <int KIND, int U>
 __device__ __forceinline__ void ks_group32(const qh_kmeta32 *__restrict__ meta, int32_t k, int32_t kend,
                                            uint32_t r8, __amdgpu_buffer_rsrc_t xr, __amdgpu_buffer_rsrc_t vr,
                                            double &are, double &aim) {
   static_assert(U - 1 <= QH_KS_PAD, "tail pass reads past the padding");
   const qh_kmeta32 *mp = meta + k;
   const int32_t n = kend - k;
-  for (int32_t p = 0; p < n / U; p++, mp += U) ks_pass32<VALS, false, U>(mp, U, r8, xr, vr, are, aim);
-  if (n % U) ks_pass32<VALS, true, U>(mp, n % U, r8, xr, vr, are, aim);
+  for (int32_t p = 0; p < n / U; p++, mp += U) ks_pass32<KIND, false, U>(mp, U, r8, xr, vr, are, aim);
+  if (n % U) ks_pass32<KIND, true, U>(mp, n % U, r8, xr, vr, are, aim);
 }
 
-// all slots [k, kend) of one group (uniform or not) of a slice
-template <typename R, bool VALS, int U, bool VV>
+// all slots [k, kend) of one group of a slice
+template <typename R, int KIND, int U, bool VV>
 __device__ __forceinline__ void ks_group(const qh_kmeta *__restrict__ meta, int32_t k, int32_t kend, uint32_t r8,
                                          __amdgpu_buffer_rsrc_t xr, __amdgpu_buffer_rsrc_t vr,
                                          __amdgpu_buffer_rsrc_t mr, double &are, double &aim) {
@@ -355,8 +384,8 @@ __device__ __forceinline__ void ks_group(const qh_kmeta *__restrict__ meta, int3
   const qh_kmeta *mp = meta + k;
   const int32_t n = kend - k;
   for (int32_t p = 0; p < n / U; p++, mp += U, k += U)
-    ks_pass<R, VALS, false, U, VV>(mp, U, r8, xr, vr, mr, k, are, aim);
-  if (n % U) ks_pass<R, VALS, true, U, VV>(mp, n % U, r8, xr, vr, mr, k, are, aim);
+    ks_pass<R, KIND, false, U, VV>(mp, U, r8, xr, vr, mr, k, are, aim);
+  if (n % U) ks_pass<R, KIND, true, U, VV>(mp, n % U, r8, xr, vr, mr, k, are, aim);
 }
 
 // Stage kernel on the sliced layout.  The stage input Y_s = x + c_s K_{s-1} is
@@ -372,8 +401,9 @@ __device__ __forceinline__ void ks_group(const qh_kmeta *__restrict__ meta, int3
 template <typename R, int MODE, int U, bool VV>
 __global__ void __launch_bounds__(256) k_ks_stage(
     int32_t nrows, int32_t nslices, const int32_t *__restrict__ moff, const int32_t *__restrict__ umid,
-    const qh_kmeta *__restrict__ meta, const qh_kmeta32 *__restrict__ meta32, int32_t nmeta, int32_t nval,
-    const typename C2<R>::T *__restrict__ val,
+    const int32_t *__restrict__ cmid, const qh_kmeta *__restrict__ meta, const qh_kmeta32 *__restrict__ meta32,
+    int32_t nmeta, int32_t nval, const typename C2<R>::T *__restrict__ val, int32_t nsval,
+    const R *__restrict__ sval,
     const typename C2<R>::T *__restrict__ xin, typename C2<R>::T *__restrict__ kout,
     typename C2<R>::T *__restrict__ ynext, R cn, const typename C2<R>::T *__restrict__ x,
     const typename C2<R>::T *__restrict__ k1, const typename C2<R>::T *__restrict__ k2,
@@ -385,18 +415,21 @@ __global__ void __launch_bounds__(256) k_ks_stage(
   if (s >= nslices) return;  // whole wavefront
   const int lane = threadIdx.x & 63;
   const int32_t r = s * 64 + lane;
-  const int32_t k0 = moff[s], km = umid[s], kend = moff[s + 1];
+  const int32_t k0 = moff[s], km = umid[s], kc = cmid[s], kend = moff[s + 1];
   const __amdgpu_buffer_rsrc_t xr = ks_rsrc(xin, (int64_t)nrows * (int64_t)sizeof(T));
   const __amdgpu_buffer_rsrc_t vr = ks_rsrc(val, (int64_t)nval * (int64_t)sizeof(T));
+  const __amdgpu_buffer_rsrc_t sr = ks_rsrc(sval, (int64_t)nsval * (int64_t)sizeof(R));
   const __amdgpu_buffer_rsrc_t mr = ks_rsrc(meta, (int64_t)nmeta * (int64_t)sizeof(qh_kmeta));
   const uint32_t r8 = (uint32_t)r * (uint32_t)sizeof(T);
   double are = 0.0, aim = 0.0;
   if (sizeof(R) == 4 && !VV) {
-    ks_group32<false, U>(meta32, k0, km, r8, xr, vr, are, aim);
-    ks_group32<true, U>(meta32, km, kend, r8, xr, vr, are, aim);
+    ks_group32<KS_UNI, U>(meta32, k0, km, r8, xr, vr, are, aim);
+    ks_group32<KS_SCA, U>(meta32, km, kc, r8, xr, sr, are, aim);
+    ks_group32<KS_CPX, U>(meta32, kc, kend, r8, xr, vr, are, aim);
   } else {
-    ks_group<R, false, U, VV>(meta, k0, km, r8, xr, vr, mr, are, aim);
-    ks_group<R, true, U, VV>(meta, km, kend, r8, xr, vr, mr, are, aim);
+    ks_group<R, KS_UNI, U, VV>(meta, k0, km, r8, xr, vr, mr, are, aim);
+    ks_group<R, KS_SCA, U, VV>(meta, km, kc, r8, xr, sr, mr, are, aim);
+    ks_group<R, KS_CPX, U, VV>(meta, kc, kend, r8, xr, vr, mr, are, aim);
   }
   if (r >= nrows) return;
   T y;
@@ -460,12 +493,16 @@ __device__ inline uint64_t wave_min_u64(uint64_t v) {
   return v;
 }
 
+// Per-slice counts (FILL false): cnt[0*ns + s] slots, [1] complex values,
+// [2] uniform slots, [3] scalar slots, [4] scalar values.  FILL writes the
+// metadata (uniform, scalar, complex groups at moff, moff + uniform,
+// moff + uniform + scalar) and the values (complex at voff, scalar at svoff).
 template <bool FILL>
 __global__ void __launch_bounds__(64) k_ks_build(int32_t nrows, const int32_t *__restrict__ rp,
                                                  const int32_t *__restrict__ col, const double2 *__restrict__ val,
-                                                 int32_t *width, int32_t *nvals, int32_t *nuni,
-                                                 const int32_t *moff, const int32_t *voff, qh_kmeta *meta,
-                                                 double2 *kval, int *overflow) {
+                                                 int32_t *cnt, int32_t ns, const int32_t *moff,
+                                                 const int32_t *voff, const int32_t *svoff, qh_kmeta *meta,
+                                                 double2 *kval, double *sval, int *overflow) {
   __shared__ uint64_t keys[64][KS_MAXROW];
   const int s = blockIdx.x, lane = threadIdx.x;
   const int32_t r = s * 64 + lane;
@@ -501,10 +538,12 @@ __global__ void __launch_bounds__(64) k_ks_build(int32_t nrows, const int32_t *_
     prev_ad = ad;
   }
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  int p = 0, slot = 0, uslot = 0, nslot = 0;
-  int32_t base = FILL ? voff[s] : 0;  // compacted non-uniform values of this slice start here
+  int p = 0, slot = 0, uslot = 0, sslot = 0, cslot = 0;
+  int32_t base = FILL ? voff[s] : 0;    // complex values of this slice start here
+  int32_t sbase = FILL ? svoff[s] : 0;  // scalar values of this slice start here
   const int32_t mo = FILL ? moff[s] : 0;
-  const int32_t mn = FILL ? moff[s] + nuni[s] : 0;  // the non-uniform slots follow the uniform ones
+  const int32_t ms = FILL ? mo + cnt[2 * ns + s] : 0;  // scalar slots follow the uniform ones
+  const int32_t mc = FILL ? ms + cnt[3 * ns + s] : 0;  // complex slots follow the scalar ones
   for (;;) {
     const uint64_t cur = p < len ? (kk[p] >> 6) : ~0ull;
     const uint64_t m = wave_min_u64(cur);
@@ -519,33 +558,44 @@ __global__ void __launch_bounds__(64) k_ks_build(int32_t nrows, const int32_t *_
     const bool same = !act || (__double_as_longlong(v.x) == __double_as_longlong(fre) &&
                                __double_as_longlong(v.y) == __double_as_longlong(fim));
     const bool uni = __ballot(same) == ~0ull;
+    // scalar slot: all values real (or all imaginary)
+    const bool isre = !uni && __ballot(act && v.y != 0.0) == 0ull;
+    const bool isim = !uni && !isre && __ballot(act && v.x != 0.0) == 0ull;
+    const bool sca = isre || isim;
     if (FILL) {
       const bool neg = act && (col[beg + e] < r);
       const uint64_t sign = __ballot(neg);
+      const uint32_t rank = (uint32_t)__popcll(mask & lt);
       if (lane == 0) {
         qh_kmeta md;
         md.mask = mask;
         md.sign = sign;
         md.ad = (uint32_t)(m >> 1);
-        md.base = uni ? QH_KS_UNIFORM : (uint32_t)base;
-        md.vre = uni ? fre : 0.0;
-        md.vim = uni ? fim : 0.0;
-        meta[uni ? mo + uslot : mn + nslot] = md;
+        md.base = uni ? QH_KS_UNIFORM : (uint32_t)(sca ? sbase : base);
+        md.vre = uni ? fre : sca ? (isre ? 1.0 : 0.0) : 0.0;
+        md.vim = uni ? fim : sca ? (isre ? 0.0 : 1.0) : 0.0;
+        meta[uni ? mo + uslot : sca ? ms + sslot : mc + cslot] = md;
       }
-      if (act && !uni) kval[base + __popcll(mask & lt)] = v;
+      if (act && sca) sval[sbase + rank] = isre ? v.x : v.y;
+      if (act && !uni && !sca) kval[base + rank] = v;
     }
-    if (!uni) base += __popcll(mask);
+    if (sca) sbase += __popcll(mask);
+    else if (!uni) base += __popcll(mask);
     if (act) p++;
     slot++;
     if (uni)
       uslot++;
+    else if (sca)
+      sslot++;
     else
-      nslot++;
+      cslot++;
   }
   if (!FILL && lane == 0) {
-    width[s] = slot;
-    nvals[s] = base;
-    nuni[s] = uslot;
+    cnt[s] = slot;
+    cnt[ns + s] = base;
+    cnt[2 * ns + s] = uslot;
+    cnt[3 * ns + s] = sslot;
+    cnt[4 * ns + s] = sbase;
   }
 }
 
@@ -553,90 +603,109 @@ __global__ void k_narrow_ks(const double2 *a, float2 *b, int64_t n) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     b[i] = make_float2((float)a[i].x, (float)a[i].y);
 }
+__global__ void k_narrow_ks_real(const double *a, float *b, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    b[i] = (float)a[i];
+}
 
 int ks_build(qh_mat *A) {
   if (A->ks_state) return QH_OK;
   hipStream_t s = qh_stream();
   const int32_t ns = (int32_t)((A->rows + 63) / 64);
-  // the stage kernels address vectors and the value stream through buffer
-  // resources: byte offsets must stay below 2^31 (the value stream is checked
-  // once its size is known)
+  // the stage kernels address vectors and the value streams through buffer
+  // resources: byte offsets must stay below 2^31 (the value streams are checked
+  // once their sizes are known)
   if (A->rows * (int64_t)sizeof(double2) >= ((int64_t)1 << 31)) {
     A->ks_state = -1;
     return QH_OK;
   }
-  int32_t *width = nullptr, *nvals = nullptr, *nuni = nullptr, *voff = nullptr;
+  int32_t *cnt = nullptr, *voff = nullptr, *svoff = nullptr;
   int *ovf = nullptr;
-  int rc = qh_alloc_t(&width, (size_t)ns + 1);
-  if (!rc) rc = qh_alloc_t(&nvals, (size_t)ns + 1);
-  if (!rc) rc = qh_alloc_t(&nuni, (size_t)ns + 1);
+  int rc = qh_alloc_t(&cnt, (size_t)5 * ns + 1);
   if (!rc) rc = qh_alloc_t(&voff, (size_t)ns + 1);
+  if (!rc) rc = qh_alloc_t(&svoff, (size_t)ns + 1);
   if (!rc) rc = qh_alloc_t(&ovf, 1);
   if (!rc) rc = qh_alloc_t(&A->ks_moff, (size_t)ns + 1);
   if (!rc) rc = qh_alloc_t(&A->ks_umid, (size_t)ns);
+  if (!rc) rc = qh_alloc_t(&A->ks_cmid, (size_t)ns);
   auto cleanup = [&]() {
-    qh_free_ptr(width);
-    qh_free_ptr(nvals);
-    qh_free_ptr(nuni);
+    qh_free_ptr(cnt);
     qh_free_ptr(voff);
+    qh_free_ptr(svoff);
     qh_free_ptr(ovf);
+  };
+  auto drop = [&]() {
+    qh_free_ptr(A->ks_moff);
+    qh_free_ptr(A->ks_umid);
+    qh_free_ptr(A->ks_cmid);
+    A->ks_moff = A->ks_umid = A->ks_cmid = nullptr;
   };
   if (rc) {
     cleanup();
+    drop();
     return rc;
   }
   QH_HIP(hipMemsetAsync(ovf, 0, sizeof(int), s));
   hipLaunchKernelGGL((k_ks_build<false>), dim3(ns), dim3(64), 0, s, (int32_t)A->rows, A->rowptr, A->col, A->val,
-                     width, nvals, nuni, nullptr, nullptr, nullptr, nullptr, ovf);
+                     cnt, ns, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, ovf);
   QH_HIP(hipGetLastError());
   int hovf = 0;
-  std::vector<int32_t> hw(ns), hn(ns), hu(ns);
+  std::vector<int32_t> hc((size_t)5 * ns);
   QH_HIP(hipMemcpyAsync(&hovf, ovf, sizeof(int), hipMemcpyDeviceToHost, s));
-  QH_HIP(hipMemcpyAsync(hw.data(), width, sizeof(int32_t) * ns, hipMemcpyDeviceToHost, s));
-  QH_HIP(hipMemcpyAsync(hn.data(), nvals, sizeof(int32_t) * ns, hipMemcpyDeviceToHost, s));
-  QH_HIP(hipMemcpyAsync(hu.data(), nuni, sizeof(int32_t) * ns, hipMemcpyDeviceToHost, s));
+  QH_HIP(hipMemcpyAsync(hc.data(), cnt, sizeof(int32_t) * 5 * ns, hipMemcpyDeviceToHost, s));
   QH_HIP(hipStreamSynchronize(s));
-  int64_t nstored = 0;
-  for (int32_t i = 0; i < ns; i++) nstored += hn[i];
-  if (nstored * (int64_t)sizeof(double2) >= ((int64_t)1 << 31)) hovf = 1;
+  const int32_t *hw = hc.data(), *hn = hw + ns, *hu = hn + ns, *hs = hu + ns, *hsv = hs + ns;
+  int64_t nstored = 0, nsstored = 0;
+  for (int32_t i = 0; i < ns; i++) {
+    nstored += hn[i];
+    nsstored += hsv[i];
+  }
+  if (nstored * (int64_t)sizeof(double2) >= ((int64_t)1 << 31) ||
+      nsstored * (int64_t)sizeof(double) >= ((int64_t)1 << 31))
+    hovf = 1;
   if (hovf) {  // a row longer than the slice builder handles, or too many stored values: keep the CSR kernels
     A->ks_state = -1;
     cleanup();
-    qh_free_ptr(A->ks_moff);
-    qh_free_ptr(A->ks_umid);
-    A->ks_moff = nullptr;
-    A->ks_umid = nullptr;
+    drop();
     return QH_OK;
   }
-  std::vector<int32_t> ho(ns + 1), hv(ns + 1), hm(ns);
-  int64_t tot = 0, totv = 0;
+  std::vector<int32_t> ho(ns + 1), hv(ns + 1), hsvo(ns + 1), hm(ns), hcm(ns);
+  int64_t tot = 0, totv = 0, tots = 0;
   for (int32_t i = 0; i < ns; i++) {
     ho[i] = (int32_t)tot;
     hv[i] = (int32_t)totv;
+    hsvo[i] = (int32_t)tots;
     hm[i] = (int32_t)(tot + hu[i]);
+    hcm[i] = (int32_t)(tot + hu[i] + hs[i]);
     tot += hw[i];
     totv += hn[i];
+    tots += hsv[i];
   }
   ho[ns] = (int32_t)tot;
   hv[ns] = (int32_t)totv;
+  hsvo[ns] = (int32_t)tots;
   QH_HIP(hipMemcpyAsync(A->ks_moff, ho.data(), sizeof(int32_t) * (ns + 1), hipMemcpyHostToDevice, s));
   QH_HIP(hipMemcpyAsync(A->ks_umid, hm.data(), sizeof(int32_t) * ns, hipMemcpyHostToDevice, s));
+  QH_HIP(hipMemcpyAsync(A->ks_cmid, hcm.data(), sizeof(int32_t) * ns, hipMemcpyHostToDevice, s));
   QH_HIP(hipMemcpyAsync(voff, hv.data(), sizeof(int32_t) * (ns + 1), hipMemcpyHostToDevice, s));
+  QH_HIP(hipMemcpyAsync(svoff, hsvo.data(), sizeof(int32_t) * (ns + 1), hipMemcpyHostToDevice, s));
   rc = qh_alloc_t(&A->ks_meta, (size_t)tot + QH_KS_PAD);
   if (!rc) QH_HIP(hipMemsetAsync(A->ks_meta + tot, 0, sizeof(qh_kmeta) * QH_KS_PAD, s));
   if (!rc) rc = qh_alloc_t(&A->ks_val, (size_t)(totv > 0 ? totv : 1));
+  if (!rc) rc = qh_alloc_t(&A->ks_sval, (size_t)(tots > 0 ? tots : 1));
   if (rc) {
     cleanup();
     return rc;
   }
   hipLaunchKernelGGL((k_ks_build<true>), dim3(ns), dim3(64), 0, s, (int32_t)A->rows, A->rowptr, A->col, A->val,
-                     width, nvals, nuni, A->ks_moff, voff, A->ks_meta, A->ks_val, ovf);
+                     cnt, ns, A->ks_moff, voff, svoff, A->ks_meta, A->ks_val, A->ks_sval, ovf);
   QH_HIP(hipGetLastError());
   QH_HIP(hipStreamSynchronize(s));
   cleanup();
   A->nslices = ns;
   A->ks_slots = tot;
   A->ks_nval = totv;
+  A->ks_nsval = tots;
   A->ks_state = 1;
   return QH_OK;
 }
@@ -656,7 +725,7 @@ __global__ void k_make_meta32(const qh_kmeta *m, int64_t n, qh_kmeta32 *o) {
 }
 
 int ks_make_fp32(qh_mat *A) {
-  if (A->ks_state != 1 || (A->ks_val32 && A->ks_meta32)) return QH_OK;
+  if (A->ks_state != 1 || (A->ks_val32 && A->ks_sval32 && A->ks_meta32)) return QH_OK;
   int rc = QH_OK;
   if (!A->ks_val32) {
     rc = qh_alloc_t(&A->ks_val32, (size_t)(A->ks_nval > 0 ? A->ks_nval : 1));
@@ -664,6 +733,13 @@ int ks_make_fp32(qh_mat *A) {
     if (A->ks_nval)
       hipLaunchKernelGGL(k_narrow_ks, dim3(qh_blocks(A->ks_nval, 256)), dim3(256), 0, qh_stream(), A->ks_val,
                          A->ks_val32, A->ks_nval);
+  }
+  if (!A->ks_sval32) {
+    rc = qh_alloc_t(&A->ks_sval32, (size_t)(A->ks_nsval > 0 ? A->ks_nsval : 1));
+    if (rc) return rc;
+    if (A->ks_nsval)
+      hipLaunchKernelGGL(k_narrow_ks_real, dim3(qh_blocks(A->ks_nsval, 256)), dim3(256), 0, qh_stream(),
+                         A->ks_sval, A->ks_sval32, A->ks_nsval);
   }
   if (!A->ks_meta32) {
     const int64_t nm = A->ks_slots + QH_KS_PAD;
@@ -682,13 +758,14 @@ int launch_ks(int G, hipStream_t s, const qh_mat *A, const void *xin, void *kout
               const void *k1, const void *k2, const void *k3, void *xout, R h, float2 *xlo) {
   typedef typename C2<R>::T T;
   const T *kv = sizeof(R) == 8 ? (const T *)A->ks_val : (const T *)A->ks_val32;
+  const R *sv = sizeof(R) == 8 ? (const R *)A->ks_sval : (const R *)A->ks_sval32;
   const dim3 grid(qh_blocks(A->nslices, 4)), blk(256);
   const int32_t n = (int32_t)A->rows, ns = A->nslices;
-#define QH_LK(UU, VV)                                                                                       \
-  hipLaunchKernelGGL((k_ks_stage<R, MODE, UU, VV>), grid, blk, 0, s, n, ns, A->ks_moff, A->ks_umid, A->ks_meta, \
-                     A->ks_meta32, (int32_t)(A->ks_slots + QH_KS_PAD), (int32_t)A->ks_nval, kv,             \
-                     (const T *)xin, (T *)kout, (T *)ynext, cn, (const T *)x, (const T *)k1, (const T *)k2, \
-                     (const T *)k3, (T *)xout, h, xlo)
+#define QH_LK(UU, VV)                                                                                         \
+  hipLaunchKernelGGL((k_ks_stage<R, MODE, UU, VV>), grid, blk, 0, s, n, ns, A->ks_moff, A->ks_umid, A->ks_cmid, \
+                     A->ks_meta, A->ks_meta32, (int32_t)(A->ks_slots + QH_KS_PAD), (int32_t)A->ks_nval, kv,    \
+                     (int32_t)A->ks_nsval, sv, (const T *)xin, (T *)kout, (T *)ynext, cn, (const T *)x,       \
+                     (const T *)k1, (const T *)k2, (const T *)k3, (T *)xout, h, xlo)
   switch (G) {
     case 128: QH_LK(2, false); break;
     case 129: QH_LK(4, false); break;
@@ -1290,7 +1367,8 @@ extern "C" int qh_mat_set_lanes(qh_mat *m, int l64, int l32) {
   return QH_OK;
 }
 
-extern "C" int qh_mat_layout_info(qh_mat *m, int64_t *slices, int64_t *slots, int64_t *stored, int64_t *mbytes) {
+extern "C" int qh_mat_layout_info(qh_mat *m, int64_t *slices, int64_t *slots, int64_t *stored, int64_t *sreals,
+                                  int64_t *mbytes) {
   QH_REQUIRE(m, QH_EINVAL, "null matrix");
   int rc = ks_build(m);
   if (rc) return rc;
@@ -1298,7 +1376,8 @@ extern "C" int qh_mat_layout_info(qh_mat *m, int64_t *slices, int64_t *slots, in
   if (slices) *slices = ok ? m->nslices : 0;
   if (slots) *slots = ok ? m->ks_slots : 0;
   if (stored) *stored = ok ? m->ks_nval : 0;
-  if (mbytes) *mbytes = ok ? (int64_t)(m->ks_slots * sizeof(qh_kmeta) + 4 * (m->nslices + 1)) : 0;
+  if (sreals) *sreals = ok ? m->ks_nsval : 0;
+  if (mbytes) *mbytes = ok ? (int64_t)(m->ks_slots * sizeof(qh_kmeta) + 12 * m->nslices + 4) : 0;
   return QH_OK;
 }
 

@@ -108,9 +108,40 @@ def test_rk4_device_direct_matches_oracle_and_lanes():
     qs.destroy()
 
 
+def _slot_classes(rp, col, val):
+    """Slots of the sliced layout restated in numpy: an entry of row r at column
+    r + d sits in slot (r // 64, |d|, rank), rank 1 only for the d > 0 entry of a
+    row that also holds r - |d|.  A slot is uniform when all its values are
+    bit-identical, scalar when they are all real or all imaginary, else complex.
+    Returns (slots, complex values stored, real values stored)."""
+    n = len(rp) - 1
+    row = np.repeat(np.arange(n, dtype=np.int64), np.diff(rp))
+    col = col.astype(np.int64)
+    d = col - row
+    ad = np.abs(d)
+    keys = np.sort(row * n + col)
+    partner = row * n + (row - ad)
+    pos = np.minimum(np.searchsorted(keys, partner), len(keys) - 1)
+    rank = ((d > 0) & (keys[pos] == partner)).astype(np.int64)
+    sid = (row // 64) * (2 * n + 2) + 2 * ad + rank
+    order = np.argsort(sid, kind="stable")
+    sid, v = sid[order], val[order]
+    starts = np.flatnonzero(np.r_[True, sid[1:] != sid[:-1]])
+    size = np.diff(np.r_[starts, len(sid)])
+    re_bits, im_bits = v.real.view(np.int64), v.imag.view(np.int64)
+    uni = ((np.maximum.reduceat(re_bits, starts) == np.minimum.reduceat(re_bits, starts)) &
+           (np.maximum.reduceat(im_bits, starts) == np.minimum.reduceat(im_bits, starts)))
+    real = np.maximum.reduceat(np.abs(v.imag), starts) == 0
+    imag = np.maximum.reduceat(np.abs(v.real), starts) == 0
+    sca = ~uni & (real | imag)
+    cpx = ~uni & ~sca
+    return len(starts), int(size[cpx].sum()), int(size[sca].sum())
+
+
 @pytest.mark.parametrize("name", ["chain10", "tc_batch"])
 def test_sliced_ell_spmv_matches_csr(name):
-    """the sliced layout (slot union, compacted values, sign masks) against the CSR kernel"""
+    """the sliced layout (slot union, uniform / real-or-imaginary / complex slot
+    groups, compacted values, sign masks) against the CSR product, every variant"""
     rng = np.random.default_rng(5)
     if name == "chain10":
         qs, _, _ = build_system(recipes.spin_chain_recipe(10, seed=1))
@@ -135,11 +166,16 @@ def test_sliced_ell_spmv_matches_csr(name):
         yv = qa.DeviceVec(n)
         qa._qh(qa.lib().qh_spmv(A.h, xv.h, yv.h))
         outs.append(yv.get())
-    assert np.abs(outs[0] - outs[1]).max() <= 1e-12 * np.abs(outs[0]).max()
     rp, col, val = A.export()
     ref = np.zeros(n, np.complex128)
     np.add.at(ref, np.repeat(np.arange(n), np.diff(rp)), val * x[col])
-    assert np.abs(outs[1] - ref).max() <= 1e-12 * np.abs(ref).max()
+    for g, o in zip((8, 128, 129, 130, 131, 132), outs):
+        assert np.abs(o - ref).max() <= 1e-12 * np.abs(ref).max(), g
+    lay = A.layout_info()
+    slots, ncpx, nreal = _slot_classes(rp, col, val)
+    assert (lay["slots"], lay["stored_vals"], lay["stored_reals"]) == (slots, ncpx, nreal)
+    if name == "tc_batch":
+        assert nreal > ncpx > 0     # the sweep's superoperators store mostly real-or-imaginary slots
     for k in keep:
         k.destroy()
 
