@@ -41,9 +41,11 @@ def rk4_step_bytes(nnz, rows, S):
 
 def kernel_name(variant, real):
     """name of the stage kernel a lanes-per-row variant code selects"""
-    if 128 <= variant <= 132:
-        u, vv = {128: (2, 0), 129: (4, 0), 130: (8, 0), 131: (4, 1), 132: (8, 1)}[variant]
-        return f"k_ks_stage<{real}, MODE, {u}, {'true' if vv else 'false'}> (sliced layout)"
+    if 128 <= variant <= 134:
+        u, vv, pp = {128: (2, 0, 0), 129: (4, 0, 0), 130: (8, 0, 0), 131: (4, 1, 0), 132: (8, 1, 0),
+                     133: (2, 0, 1), 134: (4, 0, 1)}[variant]
+        tf = ("false", "true")
+        return f"k_ks_stage<{real}, MODE, {u}, {tf[vv]}, {tf[pp]}> (sliced layout)"
     return f"k_rk_stage<{real}, {variant}> (CSR, {variant} lanes per row)"
 
 

@@ -66,8 +66,9 @@ int qh_mat_make_fp32(qh_mat *m);
 /* block-diagonal concatenation of n matrices (batched parameter sweeps) */
 int qh_mat_blockdiag(int n, qh_mat *const *mats, qh_mat **out);
 /* select SpMV variant: lanes per row G in {1,2,4,8,16,32,64} of the CSR kernel,
- * 128..132 = slot-aligned sliced ELL (one wavefront per 64 rows; 2/4/8 slots per
- * pass, 131/132 with vector-loaded uniform values); 0 = autotune */
+ * 128..134 = slot-aligned sliced ELL (one wavefront per 64 rows; 2/4/8 slots per
+ * pass, 131/132 with vector-loaded uniform values, 133/134 with 2/4 slots per pass
+ * and the next pass's loads in flight); 0 = autotune */
 int qh_mat_set_lanes(qh_mat *m, int lanes64, int lanes32);
 int qh_mat_get_lanes(const qh_mat *m, int *lanes64, int *lanes32);
 /* sliced layout statistics (built on demand): slices, slots, stored complex

@@ -94,9 +94,10 @@ struct qh_mat {
 };
 // variant codes selecting the sliced-ELL kernel: 128 / 129 / 130 process 2 / 4 / 8
 // slots per pass; 131 / 132 process 4 / 8 slots per pass and read the uniform
-// slot values with vector loads
+// slot values with vector loads; 133 / 134 process 2 / 4 slots per pass with the
+// loads of the next pass in flight during the products of the current one
 #define QH_KSELL 128
-#define QH_IS_KSELL(g) ((g) >= 128 && (g) <= 132)
+#define QH_IS_KSELL(g) ((g) >= 128 && (g) <= 134)
 
 struct qh_vec {
   int64_t n = 0;

@@ -235,9 +235,12 @@ __device__ inline double ks_load_real<double>(__amdgpu_buffer_rsrc_t rs, uint32_
 // slot groups of a slice
 enum { KS_UNI = 0, KS_SCA = 1, KS_CPX = 2 };
 
-// One pass over U slots of a slice starting at mp; cnt valid slots when TAIL
-// (the metadata array carries QH_KS_PAD empty slots past its end, so the reads
-// of a tail pass never leave it; slots past cnt get an empty lane mask).
+// One pass over U slots of a slice starting at mp, split into the issue of its
+// loads (ks_issue*) and the products of the loaded values (ks_fma*), so that a
+// group can keep the loads of the next pass in flight while it multiplies the
+// current one.  cnt valid slots when TAIL (the metadata array carries
+// QH_KS_PAD empty slots past its end, so the reads of a tail pass never leave
+// it; slots past cnt get an empty lane mask).
 // KIND: uniform slots take their value from the metadata, scalar slots load one
 // real per lane (vr over the scalar values) and multiply it by (vre, vim),
 // complex slots load one complex value per lane (vr over the complex values).
@@ -245,10 +248,17 @@ enum { KS_UNI = 0, KS_SCA = 1, KS_CPX = 2 };
 // lanes) instead of a scalar load, which keeps the metadata held in SGPRs at six
 // dwords per slot and lets a pass carry 8 slots; mr covers the metadata array
 // and k is the slot index of mp[0].
+template <typename R, int KIND, int U, bool VV>
+struct KsBuf {
+  typename C2<R>::T xv[U], vv[U];
+  R sv[U];
+  double ur[U], ui[U];  // uniform value, or the (1, 0) / (0, 1) multiplier of a scalar slot
+};
+
 template <typename R, int KIND, bool TAIL, int U, bool VV>
-__device__ __forceinline__ void ks_pass(const qh_kmeta *__restrict__ mp, int32_t cnt, uint32_t r8,
-                                        __amdgpu_buffer_rsrc_t xr, __amdgpu_buffer_rsrc_t vr,
-                                        __amdgpu_buffer_rsrc_t mr, int32_t k, double &are, double &aim) {
+__device__ __forceinline__ void ks_issue(const qh_kmeta *__restrict__ mp, int32_t cnt, uint32_t r8,
+                                         __amdgpu_buffer_rsrc_t xr, __amdgpu_buffer_rsrc_t vr,
+                                         __amdgpu_buffer_rsrc_t mr, int32_t k, KsBuf<R, KIND, U, VV> &b) {
   typedef typename C2<R>::T T;
   qh_kmeta m[U];
   uint64_t mask[U];
@@ -257,15 +267,21 @@ __device__ __forceinline__ void ks_pass(const qh_kmeta *__restrict__ mp, int32_t
     m[j] = mp[j];
     mask[j] = (!TAIL || j < cnt) ? m[j].mask : 0ull;
   }
-  T xv[U], vv[U];
-  R sv[U];
-  double2 uv[U];
   if (KIND == KS_UNI && VV) {
 #pragma unroll
-    for (int j = 0; j < U; j++)
-      uv[j] = __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(
-                                              mr, (uint32_t)(k + j) * (uint32_t)sizeof(qh_kmeta) +
-                                                      (uint32_t)offsetof(qh_kmeta, vre), 0, 0));
+    for (int j = 0; j < U; j++) {
+      const double2 u = __builtin_bit_cast(
+          double2, __builtin_amdgcn_raw_buffer_load_b128(
+                       mr, (uint32_t)(k + j) * (uint32_t)sizeof(qh_kmeta) + (uint32_t)offsetof(qh_kmeta, vre), 0, 0));
+      b.ur[j] = u.x;
+      b.ui[j] = u.y;
+    }
+  } else if (KIND != KS_CPX) {
+#pragma unroll
+    for (int j = 0; j < U; j++) {
+      b.ur[j] = m[j].vre;
+      b.ui[j] = m[j].vim;
+    }
   }
 #pragma unroll
   for (int j = 0; j < U; j++) {
@@ -274,33 +290,34 @@ __device__ __forceinline__ void ks_pass(const qh_kmeta *__restrict__ mp, int32_t
     const uint32_t ad8 = m[j].ad * (uint32_t)sizeof(T);
     uint32_t off = neg ? r8 - ad8 : r8 + ad8;
     off = on ? off : QH_KS_OOB;
-    xv[j] = ks_load<R>(xr, off);
+    b.xv[j] = ks_load<R>(xr, off);
     if (KIND != KS_UNI) {
       // rank of the lane among the slot's active lanes
       const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask[j] >> 32),
                                                       __builtin_amdgcn_mbcnt_lo((uint32_t)mask[j], 0u));
       if (KIND == KS_SCA)
-        sv[j] = ks_load_real<R>(vr, (m[j].base + rank) * (uint32_t)sizeof(R));
+        b.sv[j] = ks_load_real<R>(vr, (m[j].base + rank) * (uint32_t)sizeof(R));
       else
-        vv[j] = ks_load<R>(vr, (m[j].base + rank) * (uint32_t)sizeof(T));
+        b.vv[j] = ks_load<R>(vr, (m[j].base + rank) * (uint32_t)sizeof(T));
     }
   }
+}
+
+template <typename R, int KIND, int U, bool VV>
+__device__ __forceinline__ void ks_fma(const KsBuf<R, KIND, U, VV> &b, double &are, double &aim) {
 #pragma unroll
   for (int j = 0; j < U; j++) {
-    const double xre = (double)xv[j].x, xim = (double)xv[j].y;
+    const double xre = (double)b.xv[j].x, xim = (double)b.xv[j].y;
     double ar, ai;
     if (KIND == KS_CPX) {
-      ar = (double)vv[j].x;
-      ai = (double)vv[j].y;
+      ar = (double)b.vv[j].x;
+      ai = (double)b.vv[j].y;
     } else if (KIND == KS_SCA) {
-      ar = (double)sv[j] * m[j].vre;
-      ai = (double)sv[j] * m[j].vim;
-    } else if (VV) {
-      ar = uv[j].x;
-      ai = uv[j].y;
+      ar = (double)b.sv[j] * b.ur[j];
+      ai = (double)b.sv[j] * b.ui[j];
     } else {
-      ar = m[j].vre;
-      ai = m[j].vim;
+      ar = b.ur[j];
+      ai = b.ui[j];
     }
     are = fma(ar, xre, fma(-ai, xim, are));
     aim = fma(ar, xim, fma(ai, xre, aim));
@@ -310,19 +327,26 @@ __device__ __forceinline__ void ks_pass(const qh_kmeta *__restrict__ mp, int32_t
 // FP32 pass over U slots of the 32-byte metadata: one scalar load per slot, the
 // offsets pre-scaled, products summed in FP32 within the pass and the pass sum
 // added to the FP64 row sum.
+template <int KIND, int U>
+struct KsBuf32 {
+  float2 xv[U], vv[U];
+  float sv[U];
+  float ur[U], ui[U];
+};
+
 template <int KIND, bool TAIL, int U>
-__device__ __forceinline__ void ks_pass32(const qh_kmeta32 *__restrict__ mp, int32_t cnt, uint32_t r8,
-                                          __amdgpu_buffer_rsrc_t xr, __amdgpu_buffer_rsrc_t vr, double &are,
-                                          double &aim) {
+__device__ __forceinline__ void ks_issue32(const qh_kmeta32 *__restrict__ mp, int32_t cnt, uint32_t r8,
+                                           __amdgpu_buffer_rsrc_t xr, __amdgpu_buffer_rsrc_t vr,
+                                           KsBuf32<KIND, U> &b) {
   qh_kmeta32 m[U];
   uint64_t mask[U];
 #pragma unroll
   for (int j = 0; j < U; j++) {
     m[j] = mp[j];
     mask[j] = (!TAIL || j < cnt) ? m[j].mask : 0ull;
+    b.ur[j] = m[j].vre;
+    b.ui[j] = m[j].vim;
   }
-  float2 xv[U], vv[U];
-  float sv[U];
 #pragma unroll
   for (int j = 0; j < U; j++) {
     const bool on = __builtin_amdgcn_inverse_ballot_w64(mask[j]);
@@ -333,59 +357,150 @@ __device__ __forceinline__ void ks_pass32(const qh_kmeta32 *__restrict__ mp, int
     else
       off = r8 + (neg ? m[j].base : m[j].ad8);  // base holds -ad8 for uniform slots
     off = on ? off : QH_KS_OOB;
-    xv[j] = ks_load<float>(xr, off);
+    b.xv[j] = ks_load<float>(xr, off);
     if (KIND != KS_UNI) {
       const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask[j] >> 32),
                                                       __builtin_amdgcn_mbcnt_lo((uint32_t)mask[j], 0u));
       if (KIND == KS_SCA)
-        sv[j] = ks_load_real<float>(vr, (m[j].base + rank) * 4u);
+        b.sv[j] = ks_load_real<float>(vr, (m[j].base + rank) * 4u);
       else
-        vv[j] = ks_load<float>(vr, (m[j].base + rank) * 8u);
+        b.vv[j] = ks_load<float>(vr, (m[j].base + rank) * 8u);
     }
   }
+}
+
+template <int KIND, int U>
+__device__ __forceinline__ void ks_fma32(const KsBuf32<KIND, U> &b, double &are, double &aim) {
   float pr = 0.f, pi = 0.f;
 #pragma unroll
   for (int j = 0; j < U; j++) {
     float ar, ai;
     if (KIND == KS_CPX) {
-      ar = vv[j].x;
-      ai = vv[j].y;
+      ar = b.vv[j].x;
+      ai = b.vv[j].y;
     } else if (KIND == KS_SCA) {
-      ar = sv[j] * m[j].vre;
-      ai = sv[j] * m[j].vim;
+      ar = b.sv[j] * b.ur[j];
+      ai = b.sv[j] * b.ui[j];
     } else {
-      ar = m[j].vre;
-      ai = m[j].vim;
+      ar = b.ur[j];
+      ai = b.ui[j];
     }
-    pr = fmaf(ar, xv[j].x, fmaf(-ai, xv[j].y, pr));
-    pi = fmaf(ar, xv[j].y, fmaf(ai, xv[j].x, pi));
+    pr = fmaf(ar, b.xv[j].x, fmaf(-ai, b.xv[j].y, pr));
+    pi = fmaf(ar, b.xv[j].y, fmaf(ai, b.xv[j].x, pi));
   }
   are += (double)pr;
   aim += (double)pi;
 }
 
+// Issue / multiply policies over the two metadata layouts, so that one group
+// driver serves both.
 template <int KIND, int U>
+struct KsOps32 {
+  typedef KsBuf32<KIND, U> Buf;
+  const qh_kmeta32 *mp;
+  uint32_t r8;
+  __amdgpu_buffer_rsrc_t xr, vr;
+  template <bool TAIL>
+  __device__ __forceinline__ void issue(int32_t cnt, Buf &b) {
+    ks_issue32<KIND, TAIL, U>(mp, cnt, r8, xr, vr, b);
+    mp += U;
+  }
+  __device__ __forceinline__ void fma(const Buf &b, double &are, double &aim) { ks_fma32<KIND, U>(b, are, aim); }
+};
+
+template <typename R, int KIND, int U, bool VV>
+struct KsOps {
+  typedef KsBuf<R, KIND, U, VV> Buf;
+  const qh_kmeta *mp;
+  uint32_t r8;
+  __amdgpu_buffer_rsrc_t xr, vr, mr;
+  int32_t k;
+  template <bool TAIL>
+  __device__ __forceinline__ void issue(int32_t cnt, Buf &b) {
+    ks_issue<R, KIND, TAIL, U, VV>(mp, cnt, r8, xr, vr, mr, k, b);
+    mp += U;
+    k += U;
+  }
+  __device__ __forceinline__ void fma(const Buf &b, double &are, double &aim) { ks_fma<R, KIND, U, VV>(b, are, aim); }
+};
+
+// all n slots of one group of a slice, U per pass.  PIPE: the loads of pass p+1
+// are issued before the products of pass p (two passes in flight; the passes
+// alternate between two buffers so no loaded register is copied while its load
+// is outstanding).  The products are summed in pass order either way.
+template <int U, bool PIPE, typename Ops>
+__device__ __forceinline__ void ks_run(Ops ops, int32_t n, double &are, double &aim) {
+  static_assert(U - 1 <= QH_KS_PAD, "tail pass reads past the padding");
+  typename Ops::Buf a, b;
+  const int32_t np = n / U, rem = n % U;
+  if (!PIPE) {
+    for (int32_t p = 0; p < np; p++) {
+      ops.template issue<false>(U, a);
+      ops.fma(a, are, aim);
+    }
+    if (rem) {
+      ops.template issue<true>(rem, a);
+      ops.fma(a, are, aim);
+    }
+    return;
+  }
+  if (np == 0) {
+    if (rem) {
+      ops.template issue<true>(rem, a);
+      ops.fma(a, are, aim);
+    }
+    return;
+  }
+  // the scheduling barriers keep the compiler from moving the products of a
+  // pass above the issue of the next pass's loads
+  ops.template issue<false>(U, a);
+  int32_t p = 1;
+  for (; p + 1 < np; p += 2) {
+    ops.template issue<false>(U, b);
+    __builtin_amdgcn_sched_barrier(0);
+    ops.fma(a, are, aim);
+    ops.template issue<false>(U, a);
+    __builtin_amdgcn_sched_barrier(0);
+    ops.fma(b, are, aim);
+  }
+  if (p < np) {  // a in flight, one full pass left
+    ops.template issue<false>(U, b);
+    __builtin_amdgcn_sched_barrier(0);
+    ops.fma(a, are, aim);
+    if (rem) {
+      ops.template issue<true>(rem, a);
+      __builtin_amdgcn_sched_barrier(0);
+      ops.fma(b, are, aim);
+      ops.fma(a, are, aim);
+    } else {
+      ops.fma(b, are, aim);
+    }
+  } else {  // a in flight, no full pass left
+    if (rem) {
+      ops.template issue<true>(rem, b);
+      __builtin_amdgcn_sched_barrier(0);
+      ops.fma(a, are, aim);
+      ops.fma(b, are, aim);
+    } else {
+      ops.fma(a, are, aim);
+    }
+  }
+}
+
+template <int KIND, int U, bool PIPE>
 __device__ __forceinline__ void ks_group32(const qh_kmeta32 *__restrict__ meta, int32_t k, int32_t kend,
                                            uint32_t r8, __amdgpu_buffer_rsrc_t xr, __amdgpu_buffer_rsrc_t vr,
                                            double &are, double &aim) {
-  static_assert(U - 1 <= QH_KS_PAD, "tail pass reads past the padding");
-  const qh_kmeta32 *mp = meta + k;
-  const int32_t n = kend - k;
-  for (int32_t p = 0; p < n / U; p++, mp += U) ks_pass32<KIND, false, U>(mp, U, r8, xr, vr, are, aim);
-  if (n % U) ks_pass32<KIND, true, U>(mp, n % U, r8, xr, vr, are, aim);
+  KsOps32<KIND, U> ops{meta + k, r8, xr, vr};
+  ks_run<U, PIPE>(ops, kend - k, are, aim);
 }
 
-// all slots [k, kend) of one group of a slice
-template <typename R, int KIND, int U, bool VV>
+template <typename R, int KIND, int U, bool VV, bool PIPE>
 __device__ __forceinline__ void ks_group(const qh_kmeta *__restrict__ meta, int32_t k, int32_t kend, uint32_t r8,
                                          __amdgpu_buffer_rsrc_t xr, __amdgpu_buffer_rsrc_t vr,
                                          __amdgpu_buffer_rsrc_t mr, double &are, double &aim) {
-  static_assert(U - 1 <= QH_KS_PAD, "tail pass reads past the padding");
-  const qh_kmeta *mp = meta + k;
-  const int32_t n = kend - k;
-  for (int32_t p = 0; p < n / U; p++, mp += U, k += U)
-    ks_pass<R, KIND, false, U, VV>(mp, U, r8, xr, vr, mr, k, are, aim);
-  if (n % U) ks_pass<R, KIND, true, U, VV>(mp, n % U, r8, xr, vr, mr, k, are, aim);
+  KsOps<R, KIND, U, VV> ops{meta + k, r8, xr, vr, mr, k};
+  ks_run<U, PIPE>(ops, kend - k, are, aim);
 }
 
 // Stage kernel on the sliced layout.  The stage input Y_s = x + c_s K_{s-1} is
@@ -398,7 +513,7 @@ __device__ __forceinline__ void ks_group(const qh_kmeta *__restrict__ meta, int3
 // the FMAs); U trades memory-level parallelism against the SGPRs the metadata
 // occupies, and is chosen by the autotuner.  The FP32 state is x = hi + lo
 // (complex64 pair); its updates are formed in FP64 and split again.
-template <typename R, int MODE, int U, bool VV>
+template <typename R, int MODE, int U, bool VV, bool PIPE>
 __global__ void __launch_bounds__(256) k_ks_stage(
     int32_t nrows, int32_t nslices, const int32_t *__restrict__ moff, const int32_t *__restrict__ umid,
     const int32_t *__restrict__ cmid, const qh_kmeta *__restrict__ meta, const qh_kmeta32 *__restrict__ meta32,
@@ -423,13 +538,13 @@ __global__ void __launch_bounds__(256) k_ks_stage(
   const uint32_t r8 = (uint32_t)r * (uint32_t)sizeof(T);
   double are = 0.0, aim = 0.0;
   if (sizeof(R) == 4 && !VV) {
-    ks_group32<KS_UNI, U>(meta32, k0, km, r8, xr, vr, are, aim);
-    ks_group32<KS_SCA, U>(meta32, km, kc, r8, xr, sr, are, aim);
-    ks_group32<KS_CPX, U>(meta32, kc, kend, r8, xr, vr, are, aim);
+    ks_group32<KS_UNI, U, PIPE>(meta32, k0, km, r8, xr, vr, are, aim);
+    ks_group32<KS_SCA, U, PIPE>(meta32, km, kc, r8, xr, sr, are, aim);
+    ks_group32<KS_CPX, U, PIPE>(meta32, kc, kend, r8, xr, vr, are, aim);
   } else {
-    ks_group<R, KS_UNI, U, VV>(meta, k0, km, r8, xr, vr, mr, are, aim);
-    ks_group<R, KS_SCA, U, VV>(meta, km, kc, r8, xr, sr, mr, are, aim);
-    ks_group<R, KS_CPX, U, VV>(meta, kc, kend, r8, xr, vr, mr, are, aim);
+    ks_group<R, KS_UNI, U, VV, PIPE>(meta, k0, km, r8, xr, vr, mr, are, aim);
+    ks_group<R, KS_SCA, U, VV, PIPE>(meta, km, kc, r8, xr, sr, mr, are, aim);
+    ks_group<R, KS_CPX, U, VV, PIPE>(meta, kc, kend, r8, xr, vr, mr, are, aim);
   }
   if (r >= nrows) return;
   T y;
@@ -761,17 +876,20 @@ int launch_ks(int G, hipStream_t s, const qh_mat *A, const void *xin, void *kout
   const R *sv = sizeof(R) == 8 ? (const R *)A->ks_sval : (const R *)A->ks_sval32;
   const dim3 grid(qh_blocks(A->nslices, 4)), blk(256);
   const int32_t n = (int32_t)A->rows, ns = A->nslices;
-#define QH_LK(UU, VV)                                                                                         \
-  hipLaunchKernelGGL((k_ks_stage<R, MODE, UU, VV>), grid, blk, 0, s, n, ns, A->ks_moff, A->ks_umid, A->ks_cmid, \
+#define QH_LK(UU, VV, PP)                                                                                     \
+  hipLaunchKernelGGL((k_ks_stage<R, MODE, UU, VV, PP>), grid, blk, 0, s, n, ns, A->ks_moff, A->ks_umid,          \
+                     A->ks_cmid,                                                                              \
                      A->ks_meta, A->ks_meta32, (int32_t)(A->ks_slots + QH_KS_PAD), (int32_t)A->ks_nval, kv,    \
                      (int32_t)A->ks_nsval, sv, (const T *)xin, (T *)kout, (T *)ynext, cn, (const T *)x,       \
                      (const T *)k1, (const T *)k2, (const T *)k3, (T *)xout, h, xlo)
   switch (G) {
-    case 128: QH_LK(2, false); break;
-    case 129: QH_LK(4, false); break;
-    case 130: QH_LK(8, false); break;
-    case 131: QH_LK(4, true); break;
-    case 132: QH_LK(8, true); break;
+    case 128: QH_LK(2, false, false); break;
+    case 129: QH_LK(4, false, false); break;
+    case 130: QH_LK(8, false, false); break;
+    case 131: QH_LK(4, true, false); break;
+    case 132: QH_LK(8, true, false); break;
+    case 133: QH_LK(2, false, true); break;
+    case 134: QH_LK(4, false, true); break;
     default:
       qh_set_error("unsupported sliced-layout variant %d", G);
       return QH_EINVAL;
@@ -841,7 +959,7 @@ int autotune(qh_mat *A, int *out_g) {
     if (!brc && A->ks_state == 1 && sizeof(R) == 4) brc = ks_make_fp32(A);
     if (brc) return brc;
     if (A->ks_state == 1)
-      for (int u = 128; u <= 132; u++) cands.push_back(u);
+      for (int u = 128; u <= 134; u++) cands.push_back(u);
   }
   for (int g : {2, 4, 8, 16, 32, 64})
     if (g <= 4 * avg + 4) cands.push_back(g);
@@ -1359,7 +1477,7 @@ extern "C" int qh_mat_set_lanes(qh_mat *m, int l64, int l32) {
   auto ok = [](int g) {
     return g == 0 || g == 1 || g == 2 || g == 4 || g == 8 || g == 16 || g == 32 || g == 64 || QH_IS_KSELL(g);
   };
-  QH_REQUIRE(ok(l64) && ok(l32), QH_EINVAL, "lanes per row must be a power of two <= 64, or 128..132 (sliced layout)");
+  QH_REQUIRE(ok(l64) && ok(l32), QH_EINVAL, "lanes per row must be a power of two <= 64, or 128..134 (sliced layout)");
   m->lanes64 = l64;
   m->lanes32 = l32;
   m->lanes_forced = (l64 != 0 || l32 != 0);
