@@ -393,97 +393,128 @@ __device__ __forceinline__ void ks_fma32(const KsBuf32<KIND, U> &b, double &are,
 }
 
 // Issue / multiply policies over the two metadata layouts, so that one group
-// driver serves both.
-template <int KIND, int U>
+// driver serves both; W is the number of slots of a pass.
+template <int KIND>
 struct KsOps32 {
-  typedef KsBuf32<KIND, U> Buf;
+  template <int W>
+  using Buf = KsBuf32<KIND, W>;
   const qh_kmeta32 *mp;
   uint32_t r8;
   __amdgpu_buffer_rsrc_t xr, vr;
-  template <bool TAIL>
-  __device__ __forceinline__ void issue(int32_t cnt, Buf &b) {
-    ks_issue32<KIND, TAIL, U>(mp, cnt, r8, xr, vr, b);
-    mp += U;
+  template <int W, bool TAIL = false>
+  __device__ __forceinline__ void issue(Buf<W> &b, int32_t cnt = W) {
+    ks_issue32<KIND, TAIL, W>(mp, cnt, r8, xr, vr, b);
+    mp += W;
   }
-  __device__ __forceinline__ void fma(const Buf &b, double &are, double &aim) { ks_fma32<KIND, U>(b, are, aim); }
+  template <int W>
+  __device__ __forceinline__ void fma(const Buf<W> &b, double &are, double &aim) {
+    ks_fma32<KIND, W>(b, are, aim);
+  }
 };
 
-template <typename R, int KIND, int U, bool VV>
+template <typename R, int KIND, bool VV>
 struct KsOps {
-  typedef KsBuf<R, KIND, U, VV> Buf;
+  template <int W>
+  using Buf = KsBuf<R, KIND, W, VV>;
   const qh_kmeta *mp;
   uint32_t r8;
   __amdgpu_buffer_rsrc_t xr, vr, mr;
   int32_t k;
-  template <bool TAIL>
-  __device__ __forceinline__ void issue(int32_t cnt, Buf &b) {
-    ks_issue<R, KIND, TAIL, U, VV>(mp, cnt, r8, xr, vr, mr, k, b);
-    mp += U;
-    k += U;
+  template <int W, bool TAIL = false>
+  __device__ __forceinline__ void issue(Buf<W> &b, int32_t cnt = W) {
+    ks_issue<R, KIND, TAIL, W, VV>(mp, cnt, r8, xr, vr, mr, k, b);
+    mp += W;
+    k += W;
   }
-  __device__ __forceinline__ void fma(const Buf &b, double &are, double &aim) { ks_fma<R, KIND, U, VV>(b, are, aim); }
+  template <int W>
+  __device__ __forceinline__ void fma(const Buf<W> &b, double &are, double &aim) {
+    ks_fma<R, KIND, W, VV>(b, are, aim);
+  }
 };
+
+// The last rem (< U) slots of a group: for U <= 4 a pass of exactly rem slots
+// (no load is issued for a slot that is not there), else a masked pass of U.
+// before() runs between the issue and the products (the pipelined driver
+// multiplies its pending pass there).
+template <int U, typename Ops, typename F>
+__device__ __forceinline__ void ks_tail(Ops &ops, int32_t rem, F &&before, double &are, double &aim) {
+  if constexpr (U <= 4) {
+    if (rem == 1) {
+      typename Ops::template Buf<1> t;
+      ops.template issue<1>(t);
+      __builtin_amdgcn_sched_barrier(0);
+      before();
+      ops.fma(t, are, aim);
+    }
+    if constexpr (U > 2) {
+      if (rem == 2) {
+        typename Ops::template Buf<2> t;
+        ops.template issue<2>(t);
+        __builtin_amdgcn_sched_barrier(0);
+        before();
+        ops.fma(t, are, aim);
+      }
+    }
+    if constexpr (U > 3) {
+      if (rem == 3) {
+        typename Ops::template Buf<3> t;
+        ops.template issue<3>(t);
+        __builtin_amdgcn_sched_barrier(0);
+        before();
+        ops.fma(t, are, aim);
+      }
+    }
+  } else {
+    typename Ops::template Buf<U> t;
+    ops.template issue<U, true>(t, rem);
+    __builtin_amdgcn_sched_barrier(0);
+    before();
+    ops.fma(t, are, aim);
+  }
+}
 
 // all n slots of one group of a slice, U per pass.  PIPE: the loads of pass p+1
 // are issued before the products of pass p (two passes in flight; the passes
 // alternate between two buffers so no loaded register is copied while its load
-// is outstanding).  The products are summed in pass order either way.
+// is outstanding, and the scheduling barriers keep the compiler from moving
+// the products of a pass above the issue of the next one).  The products are
+// summed in pass order either way.
 template <int U, bool PIPE, typename Ops>
 __device__ __forceinline__ void ks_run(Ops ops, int32_t n, double &are, double &aim) {
   static_assert(U - 1 <= QH_KS_PAD, "tail pass reads past the padding");
-  typename Ops::Buf a, b;
+  typename Ops::template Buf<U> a, b;
   const int32_t np = n / U, rem = n % U;
-  if (!PIPE) {
+  if (!PIPE || np == 0) {
     for (int32_t p = 0; p < np; p++) {
-      ops.template issue<false>(U, a);
+      ops.template issue<U>(a);
       ops.fma(a, are, aim);
     }
-    if (rem) {
-      ops.template issue<true>(rem, a);
-      ops.fma(a, are, aim);
-    }
+    if (rem) ks_tail<U>(ops, rem, [] {}, are, aim);
     return;
   }
-  if (np == 0) {
-    if (rem) {
-      ops.template issue<true>(rem, a);
-      ops.fma(a, are, aim);
-    }
-    return;
-  }
-  // the scheduling barriers keep the compiler from moving the products of a
-  // pass above the issue of the next pass's loads
-  ops.template issue<false>(U, a);
+  ops.template issue<U>(a);
   int32_t p = 1;
   for (; p + 1 < np; p += 2) {
-    ops.template issue<false>(U, b);
+    ops.template issue<U>(b);
     __builtin_amdgcn_sched_barrier(0);
     ops.fma(a, are, aim);
-    ops.template issue<false>(U, a);
+    ops.template issue<U>(a);
     __builtin_amdgcn_sched_barrier(0);
     ops.fma(b, are, aim);
   }
   if (p < np) {  // a in flight, one full pass left
-    ops.template issue<false>(U, b);
+    ops.template issue<U>(b);
     __builtin_amdgcn_sched_barrier(0);
     ops.fma(a, are, aim);
-    if (rem) {
-      ops.template issue<true>(rem, a);
-      __builtin_amdgcn_sched_barrier(0);
+    if (rem)
+      ks_tail<U>(ops, rem, [&] { ops.fma(b, are, aim); }, are, aim);
+    else
       ops.fma(b, are, aim);
-      ops.fma(a, are, aim);
-    } else {
-      ops.fma(b, are, aim);
-    }
   } else {  // a in flight, no full pass left
-    if (rem) {
-      ops.template issue<true>(rem, b);
-      __builtin_amdgcn_sched_barrier(0);
+    if (rem)
+      ks_tail<U>(ops, rem, [&] { ops.fma(a, are, aim); }, are, aim);
+    else
       ops.fma(a, are, aim);
-      ops.fma(b, are, aim);
-    } else {
-      ops.fma(a, are, aim);
-    }
   }
 }
 
@@ -491,7 +522,7 @@ template <int KIND, int U, bool PIPE>
 __device__ __forceinline__ void ks_group32(const qh_kmeta32 *__restrict__ meta, int32_t k, int32_t kend,
                                            uint32_t r8, __amdgpu_buffer_rsrc_t xr, __amdgpu_buffer_rsrc_t vr,
                                            double &are, double &aim) {
-  KsOps32<KIND, U> ops{meta + k, r8, xr, vr};
+  KsOps32<KIND> ops{meta + k, r8, xr, vr};
   ks_run<U, PIPE>(ops, kend - k, are, aim);
 }
 
@@ -499,7 +530,7 @@ template <typename R, int KIND, int U, bool VV, bool PIPE>
 __device__ __forceinline__ void ks_group(const qh_kmeta *__restrict__ meta, int32_t k, int32_t kend, uint32_t r8,
                                          __amdgpu_buffer_rsrc_t xr, __amdgpu_buffer_rsrc_t vr,
                                          __amdgpu_buffer_rsrc_t mr, double &are, double &aim) {
-  KsOps<R, KIND, U, VV> ops{meta + k, r8, xr, vr, mr, k};
+  KsOps<R, KIND, VV> ops{meta + k, r8, xr, vr, mr, k};
   ks_run<U, PIPE>(ops, kend - k, are, aim);
 }
 
