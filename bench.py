@@ -46,6 +46,8 @@ def kernel_name(variant, real):
                      133: (2, 0, 1), 134: (4, 0, 1)}[variant]
         tf = ("false", "true")
         return f"k_ks_stage<{real}, MODE, {u}, {tf[vv]}, {tf[pp]}> (sliced layout)"
+    if variant in (140, 141):
+        return f"k_ks2_stage<MODE, {2 if variant == 140 else 4}> (row-pair sliced layout)"
     return f"k_rk_stage<{real}, {variant}> (CSR, {variant} lanes per row)"
 
 

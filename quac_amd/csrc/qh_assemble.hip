@@ -352,6 +352,12 @@ extern "C" int qh_mat_destroy(qh_mat *m) {
   qh_free_ptr(m->ks_moff);
   qh_free_ptr(m->ks_umid);
   qh_free_ptr(m->ks_cmid);
+  qh_free_ptr(m->ks2_moff);
+  qh_free_ptr(m->ks2_umid);
+  qh_free_ptr(m->ks2_cmid);
+  qh_free_ptr(m->ks2_meta);
+  qh_free_ptr(m->ks2_val);
+  qh_free_ptr(m->ks2_sval);
   qh_free_ptr(m->ks_sval);
   qh_free_ptr(m->ks_sval32);
   qh_free_ptr(m->ks_meta);

@@ -63,6 +63,20 @@ struct qh_kmeta32 {
   uint32_t ad8, base;
   float vre, vim;
 };
+// Slot metadata of the FP32 row-pair sliced layout (one slice = 128 consecutive
+// rows = one wavefront, lane l owns rows 2l and 2l+1 of the slice).  A slot is one
+// 16-byte load per lane of the complex64 pair x[b], x[b+1] with b = 2l +- ad
+// (sign as above); row 2l takes half h0 of the pair and row 2l+1 half h1, and the
+// slot's lanes share (ad, h0, h1) and which of the two rows have an entry there.
+// ad8 holds 8 ad | h0 | h1 << 1.  Uniform slots keep the value of each of the two
+// rows in (v0re, v0im), (v1re, v1im) (zero for a row without the entry); scalar
+// slots store two reals per active lane (value = real * (v0re, v0im)); complex
+// slots store two complex64 per active lane.
+struct qh_kmeta2 {
+  uint64_t mask, sign;
+  uint32_t ad8, base;
+  float v0re, v0im, v1re, v1im;
+};
 // empty slots after the last one, so that a pass of up to 8 slots may read
 // its metadata without a bounds check
 #define QH_KS_PAD 16
@@ -91,6 +105,16 @@ struct qh_mat {
   int64_t ks_slots = 0;
   int64_t ks_nval = 0;         // stored complex values
   int64_t ks_nsval = 0;        // stored scalar values
+  // FP32 row-pair sliced layout (built on demand from the CSR)
+  int ks2_state = 0;           // 0 not built, 1 built, -1 not applicable
+  int32_t ks2_nslices = 0;
+  int32_t *ks2_moff = nullptr;  // nslices+1 slot offsets
+  int32_t *ks2_umid = nullptr;  // end of each slice's uniform slots
+  int32_t *ks2_cmid = nullptr;  // end of each slice's scalar slots
+  qh_kmeta2 *ks2_meta = nullptr;
+  float4 *ks2_val = nullptr;    // complex slots: (row 2l, row 2l+1) complex64 pairs
+  float2 *ks2_sval = nullptr;   // scalar slots: (row 2l, row 2l+1) real pairs
+  int64_t ks2_slots = 0, ks2_nval = 0, ks2_nsval = 0;
 };
 // variant codes selecting the sliced-ELL kernel: 128 / 129 / 130 process 2 / 4 / 8
 // slots per pass; 131 / 132 process 4 / 8 slots per pass and read the uniform
@@ -98,6 +122,8 @@ struct qh_mat {
 // loads of the next pass in flight during the products of the current one
 #define QH_KSELL 128
 #define QH_IS_KSELL(g) ((g) >= 128 && (g) <= 134)
+// 140 / 141: the FP32 row-pair sliced layout, 2 / 4 slots per pass (pipelined)
+#define QH_IS_KS2(g) ((g) == 140 || (g) == 141)
 
 struct qh_vec {
   int64_t n = 0;
@@ -106,6 +132,12 @@ struct qh_vec {
   float2 *lo = nullptr;        // FP32 compensation word (low part), zero-initialised
   int view = 0;                // 1: d points into another vector's storage (not freed here)
 };
+
+// FP32 row-pair sliced layout (qh_ks2.hip): build once per matrix, launch one
+// stage (mode 0 SpMV, 1 stages 1-3, 2 stage 4) with variant G = 140 / 141
+int ks2_build(qh_mat *A);
+int ks2_launch(int G, int mode, hipStream_t s, const qh_mat *A, const void *xin, void *kout, void *ynext, float cn,
+               const void *x, const void *k1, const void *k2, const void *k3, void *xout, float h, float2 *xlo);
 
 // device allocation helpers (return QH_* codes)
 int qh_alloc(void **p, size_t bytes);

@@ -1,0 +1,477 @@
+// qh_ks2.hip -- the FP32 stage kernel on the row-pair sliced layout.
+//
+// A wavefront owns a slice of 128 consecutive rows, lane l the two rows 2l and
+// 2l+1.  Every x gather is one 16-byte load per lane of a complex64 pair
+// x[b], x[b+1] that serves both rows of the lane (row 2l takes half h0, row
+// 2l+1 half h1), and every value and epilogue access is 8 or 16 bytes per lane.
+// The texture data path of a CU returns one 4-lane quad per clock, so 16-byte
+// lane accesses move twice the bytes per clock of the 8-byte accesses of the
+// one-row-per-lane layout (qh_spmv.hip), which that layout is bound by.
+//
+// Pairing: for each entry of row 2l at column c the builder looks for an entry
+// of row 2l+1 at c+1 (straight, b = c), c-1 (crossed, b = c-1) or c (same);
+// unmatched entries load alone (one row of the lane inactive).  A slot collects
+// the lanes whose loads share (|b - 2l|, h0, h1, which rows are active, rank);
+// the rest of the machinery (sign masks, uniform / scalar / complex slot groups,
+// compacted values, exact tails, two passes in flight) is that of the
+// one-row-per-lane layout.  Products are summed in FP32 within a pass and the
+// pass sums in FP64; the state is hi + lo as in the other FP32 kernels.
+#include <vector>
+
+#include "qh_common.h"
+#include "qh_ks.h"
+
+namespace {
+
+const int KS2_MAXROW = 32;  // longest row the pair builder takes (the CSR kernels serve longer rows)
+
+// ------------------------------------------------------------------ build
+// Per-slice counts (FILL false): cnt[0*ns + s] slots, [1] complex pairs, [2]
+// uniform slots, [3] scalar slots, [4] scalar pairs.  FILL writes the metadata
+// (uniform, scalar and complex groups at moff, moff + uniform, + scalar) and the
+// values (complex pairs at voff, scalar pairs at svoff).
+template <bool FILL>
+__global__ void __launch_bounds__(64) k_ks2_build(int32_t n, const int32_t *__restrict__ rp,
+                                                  const int32_t *__restrict__ col, const double2 *__restrict__ val,
+                                                  int32_t *cnt, int32_t ns, const int32_t *moff, const int32_t *voff,
+                                                  const int32_t *svoff, qh_kmeta2 *meta, float4 *kval, float2 *sval,
+                                                  int *overflow) {
+  // per lane up to 2 * KS2_MAXROW pair loads: key << 12 | e0 << 6 | e1 (63 = no entry)
+  __shared__ uint64_t keys[64][2 * KS2_MAXROW];
+  const int s = blockIdx.x, lane = threadIdx.x;
+  const int64_t r0 = (int64_t)s * 128 + 2 * lane, r1 = r0 + 1;
+  int len0 = 0, len1 = 0, beg0 = 0, beg1 = 0;
+  if (r0 < n) {
+    beg0 = rp[r0];
+    len0 = rp[r0 + 1] - beg0;
+  }
+  if (r1 < n) {
+    beg1 = rp[r1];
+    len1 = rp[r1 + 1] - beg1;
+  }
+  if (len0 > KS2_MAXROW || len1 > KS2_MAXROW) {
+    overflow[0] = 1;
+    len0 = len1 = 0;
+  }
+  uint64_t *kk = keys[lane];
+  int np = 0;
+  uint32_t used1 = 0;
+  auto push = [&](int64_t b, int h0, int h1, int e0, int e1) {
+    const int64_t o = b - r0;
+    const uint64_t ad = (uint64_t)(o < 0 ? -o : o);
+    const uint64_t pat = (uint64_t)((e0 != 63) | ((e1 != 63) << 1) | (h0 << 2) | (h1 << 3));
+    // negative offsets first so that the rank below separates b = r0 - ad from r0 + ad
+    const uint64_t key = (((ad << 4) | pat) << 1) | (o > 0 ? 1u : 0u);
+    const uint64_t v = (key << 12) | ((uint64_t)e0 << 6) | (uint64_t)e1;
+    int p = np;
+    while (p > 0 && kk[p - 1] > v) {
+      kk[p] = kk[p - 1];
+      p--;
+    }
+    kk[p] = v;
+    np++;
+  };
+  for (int e0 = 0; e0 < len0; e0++) {
+    const int64_t c = col[beg0 + e0];
+    int f = -1, how = 0;
+    for (int pass = 0; pass < 3 && f < 0; pass++) {
+      const int64_t want = pass == 0 ? c + 1 : pass == 1 ? c - 1 : c;
+      for (int e1 = 0; e1 < len1; e1++)
+        if (!((used1 >> e1) & 1u) && col[beg1 + e1] == want) {
+          f = e1;
+          how = pass;
+          break;
+        }
+    }
+    if (f >= 0) used1 |= 1u << f;
+    if (f >= 0 && how == 0) {
+      push(c, 0, 1, e0, f);  // straight: x[c], x[c+1]
+    } else if (f >= 0 && how == 1) {
+      push(c - 1, 1, 0, e0, f);  // crossed: row 2l+1 reads x[c-1]
+    } else {
+      // same column for both rows, or row 2l alone: keep the 16-byte load inside the vector
+      const bool last = c + 1 >= n;
+      push(last ? c - 1 : c, last ? 1 : 0, (f >= 0 && last) ? 1 : 0, e0, f >= 0 ? f : 63);
+    }
+  }
+  for (int e1 = 0; e1 < len1; e1++) {
+    if ((used1 >> e1) & 1u) continue;
+    const int64_t c = col[beg1 + e1];
+    const bool last = c + 1 >= n;
+    push(last ? c - 1 : c, 0, last ? 1 : 0, 63, e1);
+  }
+  // rank: the second load of a lane with the same (ad, pattern) gets rank 1
+  uint64_t prev = ~0ull;
+  for (int e = 0; e < np; e++) {
+    const uint64_t adp = kk[e] >> 13;
+    const uint64_t rank = (adp == prev) ? 1u : 0u;
+    kk[e] = (((adp << 1) | rank) << 12) | (kk[e] & 4095u);
+    prev = adp;
+  }
+  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  int p = 0, slot = 0, uslot = 0, sslot = 0, cslot = 0;
+  int32_t base = FILL ? voff[s] : 0;
+  int32_t sbase = FILL ? svoff[s] : 0;
+  const int32_t mo = FILL ? moff[s] : 0;
+  const int32_t ms = FILL ? mo + cnt[2 * ns + s] : 0;
+  const int32_t mc = FILL ? ms + cnt[3 * ns + s] : 0;
+  for (;;) {
+    const uint64_t cur = p < np ? (kk[p] >> 12) : ~0ull;
+    const uint64_t m = wave_min_u64(cur);
+    if (m == ~0ull) break;
+    const bool act = (cur == m);
+    const uint64_t mask = __ballot(act);
+    const int e0 = act ? (int)((kk[p] >> 6) & 63u) : 63, e1 = act ? (int)(kk[p] & 63u) : 63;
+    const double2 z = make_double2(0.0, 0.0);
+    const double2 v0 = (act && e0 != 63) ? val[beg0 + e0] : z;
+    const double2 v1 = (act && e1 != 63) ? val[beg1 + e1] : z;
+    const int first = __ffsll((unsigned long long)mask) - 1;
+    const double f0r = __shfl(v0.x, first, 64), f0i = __shfl(v0.y, first, 64);
+    const double f1r = __shfl(v1.x, first, 64), f1i = __shfl(v1.y, first, 64);
+    auto bits = [](double a) { return __double_as_longlong(a); };
+    const bool same = !act || (bits(v0.x) == bits(f0r) && bits(v0.y) == bits(f0i) && bits(v1.x) == bits(f1r) &&
+                               bits(v1.y) == bits(f1i));
+    const bool uni = __ballot(same) == ~0ull;
+    const bool isre = !uni && __ballot(act && (v0.y != 0.0 || v1.y != 0.0)) == 0ull;
+    const bool isim = !uni && !isre && __ballot(act && (v0.x != 0.0 || v1.x != 0.0)) == 0ull;
+    const bool sca = isre || isim;
+    if (FILL) {
+      // slot key m = (ad << 4 | pattern) << 1 | rank
+      const uint32_t pat = (uint32_t)((m >> 1) & 15u);
+      const uint32_t h0 = (pat >> 2) & 1u, h1 = (pat >> 3) & 1u;
+      const uint64_t ad = m >> 5;
+      // the pair starts at b = (column of row 2l) - h0, or (column of row 2l+1) - h1
+      const int64_t bcol = !act ? r0 : e0 != 63 ? (int64_t)col[beg0 + e0] - h0 : (int64_t)col[beg1 + e1] - h1;
+      const uint64_t sign = __ballot(act && bcol < r0);
+      const uint32_t rank = (uint32_t)__popcll(mask & lt);
+      if (lane == 0) {
+        qh_kmeta2 md;
+        md.mask = mask;
+        md.sign = sign;
+        md.ad8 = (uint32_t)(ad * 8u) | h0 | (h1 << 1);
+        md.base = uni ? 0u : (uint32_t)(sca ? sbase : base);
+        if (uni) {
+          md.v0re = (float)f0r;
+          md.v0im = (float)f0i;
+          md.v1re = (float)f1r;
+          md.v1im = (float)f1i;
+        } else {
+          md.v0re = md.v1re = sca ? (isre ? 1.f : 0.f) : 0.f;
+          md.v0im = md.v1im = sca ? (isre ? 0.f : 1.f) : 0.f;
+        }
+        meta[uni ? mo + uslot : sca ? ms + sslot : mc + cslot] = md;
+      }
+      if (act && sca)
+        sval[sbase + rank] = isre ? make_float2((float)v0.x, (float)v1.x) : make_float2((float)v0.y, (float)v1.y);
+      if (act && !uni && !sca)
+        kval[base + rank] = make_float4((float)v0.x, (float)v0.y, (float)v1.x, (float)v1.y);
+    }
+    if (sca)
+      sbase += __popcll(mask);
+    else if (!uni)
+      base += __popcll(mask);
+    if (act) p++;
+    slot++;
+    if (uni)
+      uslot++;
+    else if (sca)
+      sslot++;
+    else
+      cslot++;
+  }
+  if (!FILL && lane == 0) {
+    cnt[s] = slot;
+    cnt[ns + s] = base;
+    cnt[2 * ns + s] = uslot;
+    cnt[3 * ns + s] = sslot;
+    cnt[4 * ns + s] = sbase;
+  }
+}
+
+// ------------------------------------------------------------------ stage
+struct KsAcc2 {
+  double r0, i0, r1, i1;  // row sums of rows 2l and 2l+1
+};
+
+template <int KIND, int W>
+struct Ks2Buf {
+  float4 xv[W], vv[W];
+  float2 sv[W];
+  float u0r[W], u0i[W], u1r[W], u1i[W];
+  uint32_t h[W];
+};
+
+template <int KIND>
+struct Ks2Ops {
+  typedef KsAcc2 Acc;
+  template <int W>
+  using Buf = Ks2Buf<KIND, W>;
+  const qh_kmeta2 *mp;
+  uint32_t r8;  // byte offset of x[2l] within the stage input
+  __amdgpu_buffer_rsrc_t xr, vr;
+  template <int W>
+  __device__ __forceinline__ void issue(Buf<W> &b) {
+    qh_kmeta2 m[W];
+#pragma unroll
+    for (int j = 0; j < W; j++) {
+      m[j] = mp[j];
+      b.h[j] = m[j].ad8 & 3u;
+      b.u0r[j] = m[j].v0re;
+      b.u0i[j] = m[j].v0im;
+      b.u1r[j] = m[j].v1re;
+      b.u1i[j] = m[j].v1im;
+    }
+#pragma unroll
+    for (int j = 0; j < W; j++) {
+      const bool on = __builtin_amdgcn_inverse_ballot_w64(m[j].mask);
+      const bool neg = __builtin_amdgcn_inverse_ballot_w64(m[j].sign);
+      const uint32_t ad8 = m[j].ad8 & ~7u;
+      uint32_t off = neg ? r8 - ad8 : r8 + ad8;
+      off = on ? off : QH_KS_OOB;
+      b.xv[j] = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(xr, off, 0, 0));
+      if (KIND != KS_UNI) {
+        const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(m[j].mask >> 32),
+                                                        __builtin_amdgcn_mbcnt_lo((uint32_t)m[j].mask, 0u));
+        if (KIND == KS_SCA)
+          b.sv[j] = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(vr, (m[j].base + rank) * 8u, 0, 0));
+        else
+          b.vv[j] =
+              __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(vr, (m[j].base + rank) * 16u, 0, 0));
+      }
+    }
+    mp += W;
+  }
+  template <int W>
+  __device__ __forceinline__ void fma(const Buf<W> &b, Acc &acc) {
+    float p0r = 0.f, p0i = 0.f, p1r = 0.f, p1i = 0.f;
+#pragma unroll
+    for (int j = 0; j < W; j++) {
+      const float4 xv = b.xv[j];
+      const bool h0 = b.h[j] & 1u, h1 = b.h[j] & 2u;
+      const float x0r = h0 ? xv.z : xv.x, x0i = h0 ? xv.w : xv.y;
+      const float x1r = h1 ? xv.z : xv.x, x1i = h1 ? xv.w : xv.y;
+      float a0r, a0i, a1r, a1i;
+      if (KIND == KS_CPX) {
+        a0r = b.vv[j].x;
+        a0i = b.vv[j].y;
+        a1r = b.vv[j].z;
+        a1i = b.vv[j].w;
+      } else if (KIND == KS_SCA) {
+        a0r = b.sv[j].x * b.u0r[j];
+        a0i = b.sv[j].x * b.u0i[j];
+        a1r = b.sv[j].y * b.u0r[j];
+        a1i = b.sv[j].y * b.u0i[j];
+      } else {
+        a0r = b.u0r[j];
+        a0i = b.u0i[j];
+        a1r = b.u1r[j];
+        a1i = b.u1i[j];
+      }
+      p0r = fmaf(a0r, x0r, fmaf(-a0i, x0i, p0r));
+      p0i = fmaf(a0r, x0i, fmaf(a0i, x0r, p0i));
+      p1r = fmaf(a1r, x1r, fmaf(-a1i, x1i, p1r));
+      p1i = fmaf(a1r, x1i, fmaf(a1i, x1r, p1i));
+    }
+    acc.r0 += (double)p0r;
+    acc.i0 += (double)p0i;
+    acc.r1 += (double)p1r;
+    acc.i1 += (double)p1i;
+  }
+};
+
+// the two complex64 of rows (r, r+1) at byte offset o of a vector; the second is
+// zero (and not loaded / stored) when r is the last row
+__device__ inline float4 ld_pair(__amdgpu_buffer_rsrc_t rs, uint32_t o, bool two) {
+  if (two) return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rs, o, 0, 0));
+  const float2 a = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(rs, o, 0, 0));
+  return make_float4(a.x, a.y, 0.f, 0.f);
+}
+__device__ inline void st_pair(__amdgpu_buffer_rsrc_t rs, uint32_t o, bool two, float4 v) {
+  if (two)
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(__attribute__((ext_vector_type(4))) unsigned, v), rs, o,
+                                           0, 0);
+  else
+    __builtin_amdgcn_raw_buffer_store_b64(
+        __builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned, make_float2(v.x, v.y)), rs, o, 0, 0);
+}
+
+//   MODE_SPMV : kout = A xin
+//   MODE_STAGE: kout = A xin;  ynext = x + cn * kout          (stages 1-3)
+//   MODE_FINAL: xout = x + h/6 (K1 + 2 K2 + 2 K3 + A xin)    (stage 4)
+// with the FP32 state x = hi + lo updated in FP64 and split again.
+template <int MODE, int U>
+__global__ void __launch_bounds__(256) k_ks2_stage(int32_t n, int32_t nslices, const int32_t *__restrict__ moff,
+                                                   const int32_t *__restrict__ umid, const int32_t *__restrict__ cmid,
+                                                   const qh_kmeta2 *__restrict__ meta, int32_t nval,
+                                                   const float4 *__restrict__ val, int32_t nsval,
+                                                   const float2 *__restrict__ sval, const float2 *xin, float2 *kout,
+                                                   float2 *ynext, float cn, const float2 *x, const float2 *k1,
+                                                   const float2 *k2, const float2 *k3, float2 *xout, float h,
+                                                   float2 *xlo) {
+  const unsigned b = qh_xcd_block(blockIdx.x, gridDim.x);
+  const int s = (int)(b * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6));
+  if (s >= nslices) return;  // whole wavefront
+  const int lane = threadIdx.x & 63;
+  const int32_t r = s * 128 + 2 * lane;
+  const int32_t k0 = moff[s], km = umid[s], kc = cmid[s], kend = moff[s + 1];
+  const int64_t vb = (int64_t)n * 8;
+  const __amdgpu_buffer_rsrc_t xr = ks_rsrc(xin, vb);
+  const __amdgpu_buffer_rsrc_t vr = ks_rsrc(val, (int64_t)nval * 16);
+  const __amdgpu_buffer_rsrc_t sr = ks_rsrc(sval, (int64_t)nsval * 8);
+  const uint32_t r8 = (uint32_t)r * 8u;
+  KsAcc2 acc{0.0, 0.0, 0.0, 0.0};
+  ks_run<U, true>(Ks2Ops<KS_UNI>{meta + k0, r8, xr, vr}, km - k0, acc);
+  ks_run<U, true>(Ks2Ops<KS_SCA>{meta + km, r8, xr, sr}, kc - km, acc);
+  ks_run<U, true>(Ks2Ops<KS_CPX>{meta + kc, r8, xr, vr}, kend - kc, acc);
+  if (r >= n) return;
+  const bool two = r + 1 < n;
+  const float4 y = make_float4((float)acc.r0, (float)acc.i0, (float)acc.r1, (float)acc.i1);
+  if (MODE == MODE_SPMV) {
+    st_pair(ks_rsrc(kout, vb), r8, two, y);
+    return;
+  }
+  const float4 xo = ld_pair(ks_rsrc(x, vb), r8, two);
+  const __amdgpu_buffer_rsrc_t lr = ks_rsrc(xlo, vb);
+  const float4 lo = ld_pair(lr, r8, two);
+  if (MODE == MODE_STAGE) {
+    st_pair(ks_rsrc(kout, vb), r8, two, y);
+    const double c = (double)cn;
+    const float4 yn = make_float4((float)((double)xo.x + (double)lo.x + c * acc.r0),
+                                  (float)((double)xo.y + (double)lo.y + c * acc.i0),
+                                  (float)((double)xo.z + (double)lo.z + c * acc.r1),
+                                  (float)((double)xo.w + (double)lo.w + c * acc.i1));
+    st_pair(ks_rsrc(ynext, vb), r8, two, yn);
+    return;
+  }
+  const float4 a = ld_pair(ks_rsrc(k1, vb), r8, two), bq = ld_pair(ks_rsrc(k2, vb), r8, two),
+               cq = ld_pair(ks_rsrc(k3, vb), r8, two);
+  const double s6 = (double)h * (1.0 / 6.0), s3 = (double)h * (1.0 / 3.0);
+  const double n0r = ((double)xo.x + (double)lo.x) + (s6 * (double)a.x + s3 * (double)bq.x + s3 * (double)cq.x + s6 * acc.r0);
+  const double n0i = ((double)xo.y + (double)lo.y) + (s6 * (double)a.y + s3 * (double)bq.y + s3 * (double)cq.y + s6 * acc.i0);
+  const double n1r = ((double)xo.z + (double)lo.z) + (s6 * (double)a.z + s3 * (double)bq.z + s3 * (double)cq.z + s6 * acc.r1);
+  const double n1i = ((double)xo.w + (double)lo.w) + (s6 * (double)a.w + s3 * (double)bq.w + s3 * (double)cq.w + s6 * acc.i1);
+  const float4 o = make_float4((float)n0r, (float)n0i, (float)n1r, (float)n1i);
+  st_pair(ks_rsrc(xout, vb), r8, two, o);
+  st_pair(lr, r8, two,
+          make_float4((float)(n0r - (double)o.x), (float)(n0i - (double)o.y), (float)(n1r - (double)o.z),
+                      (float)(n1i - (double)o.w)));
+}
+
+}  // namespace
+
+// Build the row-pair layout of A (FP32 values) once; ks2_state -1 when it does
+// not apply (fewer than 2 rows, a row longer than KS2_MAXROW, or byte offsets
+// beyond the 2^31 reach of the buffer loads).
+int ks2_build(qh_mat *A) {
+  if (A->ks2_state) return QH_OK;
+  hipStream_t s = qh_stream();
+  if (A->rows < 2 || A->rows * (int64_t)sizeof(float2) >= ((int64_t)1 << 31)) {
+    A->ks2_state = -1;
+    return QH_OK;
+  }
+  const int32_t ns = (int32_t)((A->rows + 127) / 128);
+  int32_t *cnt = nullptr, *voff = nullptr, *svoff = nullptr;
+  int *ovf = nullptr;
+  int rc = qh_alloc_t(&cnt, (size_t)5 * ns);
+  if (!rc) rc = qh_alloc_t(&voff, (size_t)ns + 1);
+  if (!rc) rc = qh_alloc_t(&svoff, (size_t)ns + 1);
+  if (!rc) rc = qh_alloc_t(&ovf, 1);
+  auto cleanup = [&]() {
+    qh_free_ptr(cnt);
+    qh_free_ptr(voff);
+    qh_free_ptr(svoff);
+    qh_free_ptr(ovf);
+  };
+  if (rc) {
+    cleanup();
+    return rc;
+  }
+  QH_HIP(hipMemsetAsync(ovf, 0, sizeof(int), s));
+  hipLaunchKernelGGL((k_ks2_build<false>), dim3(ns), dim3(64), 0, s, (int32_t)A->rows, A->rowptr, A->col, A->val,
+                     cnt, ns, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, ovf);
+  QH_HIP(hipGetLastError());
+  int hovf = 0;
+  std::vector<int32_t> hc((size_t)5 * ns);
+  QH_HIP(hipMemcpyAsync(&hovf, ovf, sizeof(int), hipMemcpyDeviceToHost, s));
+  QH_HIP(hipMemcpyAsync(hc.data(), cnt, sizeof(int32_t) * 5 * ns, hipMemcpyDeviceToHost, s));
+  QH_HIP(hipStreamSynchronize(s));
+  const int32_t *hw = hc.data(), *hn = hw + ns, *hu = hn + ns, *hs = hu + ns, *hsv = hs + ns;
+  std::vector<int32_t> ho(ns + 1), hv(ns + 1), hso(ns + 1), hm(ns), hcm(ns);
+  int64_t tot = 0, totv = 0, tots = 0;
+  for (int32_t i = 0; i < ns; i++) {
+    ho[i] = (int32_t)tot;
+    hv[i] = (int32_t)totv;
+    hso[i] = (int32_t)tots;
+    hm[i] = (int32_t)(tot + hu[i]);
+    hcm[i] = (int32_t)(tot + hu[i] + hs[i]);
+    tot += hw[i];
+    totv += hn[i];
+    tots += hsv[i];
+  }
+  if (hovf || totv * 16 >= ((int64_t)1 << 31) || tots * 8 >= ((int64_t)1 << 31) || tot >= ((int64_t)1 << 31)) {
+    A->ks2_state = -1;
+    cleanup();
+    return QH_OK;
+  }
+  ho[ns] = (int32_t)tot;
+  hv[ns] = (int32_t)totv;
+  hso[ns] = (int32_t)tots;
+  rc = qh_alloc_t(&A->ks2_moff, (size_t)ns + 1);
+  if (!rc) rc = qh_alloc_t(&A->ks2_umid, (size_t)ns);
+  if (!rc) rc = qh_alloc_t(&A->ks2_cmid, (size_t)ns);
+  if (!rc) rc = qh_alloc_t(&A->ks2_meta, (size_t)(tot > 0 ? tot : 1));
+  if (!rc) rc = qh_alloc_t(&A->ks2_val, (size_t)(totv > 0 ? totv : 1));
+  if (!rc) rc = qh_alloc_t(&A->ks2_sval, (size_t)(tots > 0 ? tots : 1));
+  if (rc) {
+    cleanup();
+    return rc;
+  }
+  QH_HIP(hipMemcpyAsync(A->ks2_moff, ho.data(), sizeof(int32_t) * (ns + 1), hipMemcpyHostToDevice, s));
+  QH_HIP(hipMemcpyAsync(A->ks2_umid, hm.data(), sizeof(int32_t) * ns, hipMemcpyHostToDevice, s));
+  QH_HIP(hipMemcpyAsync(A->ks2_cmid, hcm.data(), sizeof(int32_t) * ns, hipMemcpyHostToDevice, s));
+  QH_HIP(hipMemcpyAsync(voff, hv.data(), sizeof(int32_t) * (ns + 1), hipMemcpyHostToDevice, s));
+  QH_HIP(hipMemcpyAsync(svoff, hso.data(), sizeof(int32_t) * (ns + 1), hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL((k_ks2_build<true>), dim3(ns), dim3(64), 0, s, (int32_t)A->rows, A->rowptr, A->col, A->val,
+                     cnt, ns, A->ks2_moff, voff, svoff, A->ks2_meta, A->ks2_val, A->ks2_sval, ovf);
+  QH_HIP(hipGetLastError());
+  QH_HIP(hipStreamSynchronize(s));
+  cleanup();
+  A->ks2_nslices = ns;
+  A->ks2_slots = tot;
+  A->ks2_nval = totv;
+  A->ks2_nsval = tots;
+  A->ks2_state = 1;
+  return QH_OK;
+}
+
+// One stage launch on the row-pair layout (G = 140 / 141: 2 / 4 slots per pass);
+// mode as in the one-row-per-lane kernels.
+int ks2_launch(int G, int mode, hipStream_t s, const qh_mat *A, const void *xin, void *kout, void *ynext, float cn,
+               const void *x, const void *k1, const void *k2, const void *k3, void *xout, float h, float2 *xlo) {
+  if (A->ks2_state != 1) {
+    qh_set_error("row-pair sliced layout not built for this matrix");
+    return QH_EINVAL;
+  }
+  const dim3 grid(qh_blocks(A->ks2_nslices, 4)), blk(256);
+  const int32_t n = (int32_t)A->rows, ns = A->ks2_nslices;
+#define QH_LK2(MM, UU)                                                                                          \
+  hipLaunchKernelGGL((k_ks2_stage<MM, UU>), grid, blk, 0, s, n, ns, A->ks2_moff, A->ks2_umid, A->ks2_cmid,      \
+                     A->ks2_meta, (int32_t)A->ks2_nval, A->ks2_val, (int32_t)A->ks2_nsval, A->ks2_sval,          \
+                     (const float2 *)xin, (float2 *)kout, (float2 *)ynext, cn, (const float2 *)x,                \
+                     (const float2 *)k1, (const float2 *)k2, (const float2 *)k3, (float2 *)xout, h, xlo)
+  const int u = G == 140 ? 2 : G == 141 ? 4 : 0;
+  if (u == 0 || mode < MODE_SPMV || mode > MODE_FINAL) {
+    qh_set_error("unsupported row-pair variant %d / mode %d", G, mode);
+    return QH_EINVAL;
+  }
+  if (mode == MODE_SPMV) {
+    if (u == 2) QH_LK2(MODE_SPMV, 2); else QH_LK2(MODE_SPMV, 4);
+  } else if (mode == MODE_STAGE) {
+    if (u == 2) QH_LK2(MODE_STAGE, 2); else QH_LK2(MODE_STAGE, 4);
+  } else {
+    if (u == 2) QH_LK2(MODE_FINAL, 2); else QH_LK2(MODE_FINAL, 4);
+  }
+#undef QH_LK2
+  QH_HIP(hipGetLastError());
+  return QH_OK;
+}

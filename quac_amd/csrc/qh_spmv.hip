@@ -26,6 +26,7 @@
 #include <vector>
 
 #include "qh_common.h"
+#include "qh_ks.h"
 
 namespace {
 
@@ -104,8 +105,6 @@ struct Acc<float> {
   }
   __device__ inline float2 value() const { return make_float2(re + cre, im + cim); }
 };
-
-enum { MODE_SPMV = 0, MODE_STAGE = 1, MODE_FINAL = 2 };
 
 template <typename R, int G, int MODE>
 __global__ void __launch_bounds__(256) k_rk_stage(
@@ -201,15 +200,6 @@ __global__ void __launch_bounds__(256) k_rk_stage(
 // together.  Products and sums are formed in FP64 for both storage precisions
 // (FP64 FMA issues at the FP32 rate on CDNA4).
 
-// Raw buffer resource over n elements (gfx9 descriptor word 3: 32-bit data
-// format).  A buffer load whose byte offset is at or beyond n * sizeof(T)
-// returns zero instead of touching memory: inactive lanes load with offset
-// QH_KS_OOB and so contribute exact zeros without any per-lane select.
-#define QH_KS_OOB 0x80000000u
-__device__ inline __amdgpu_buffer_rsrc_t ks_rsrc(const void *p, int64_t bytes) {
-  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes, 0x00020000);
-}
-
 template <typename R>
 __device__ inline typename C2<R>::T ks_load(__amdgpu_buffer_rsrc_t rs, uint32_t off);
 template <>
@@ -231,9 +221,6 @@ template <>
 __device__ inline double ks_load_real<double>(__amdgpu_buffer_rsrc_t rs, uint32_t off) {
   return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, off, 0, 0));
 }
-
-// slot groups of a slice
-enum { KS_UNI = 0, KS_SCA = 1, KS_CPX = 2 };
 
 // One pass over U slots of a slice starting at mp, split into the issue of its
 // loads (ks_issue*) and the products of the loaded values (ks_fma*), so that a
@@ -396,6 +383,7 @@ __device__ __forceinline__ void ks_fma32(const KsBuf32<KIND, U> &b, double &are,
 // driver serves both; W is the number of slots of a pass.
 template <int KIND>
 struct KsOps32 {
+  typedef KsAcc Acc;
   template <int W>
   using Buf = KsBuf32<KIND, W>;
   const qh_kmeta32 *mp;
@@ -407,13 +395,14 @@ struct KsOps32 {
     mp += W;
   }
   template <int W>
-  __device__ __forceinline__ void fma(const Buf<W> &b, double &are, double &aim) {
-    ks_fma32<KIND, W>(b, are, aim);
+  __device__ __forceinline__ void fma(const Buf<W> &b, Acc &a) {
+    ks_fma32<KIND, W>(b, a.re, a.im);
   }
 };
 
 template <typename R, int KIND, bool VV>
 struct KsOps {
+  typedef KsAcc Acc;
   template <int W>
   using Buf = KsBuf<R, KIND, W, VV>;
   const qh_kmeta *mp;
@@ -427,111 +416,25 @@ struct KsOps {
     k += W;
   }
   template <int W>
-  __device__ __forceinline__ void fma(const Buf<W> &b, double &are, double &aim) {
-    ks_fma<R, KIND, W, VV>(b, are, aim);
+  __device__ __forceinline__ void fma(const Buf<W> &b, Acc &a) {
+    ks_fma<R, KIND, W, VV>(b, a.re, a.im);
   }
 };
-
-// The last rem (< U) slots of a group: for U <= 4 a pass of exactly rem slots
-// (no load is issued for a slot that is not there), else a masked pass of U.
-// before() runs between the issue and the products (the pipelined driver
-// multiplies its pending pass there).
-template <int U, typename Ops, typename F>
-__device__ __forceinline__ void ks_tail(Ops &ops, int32_t rem, F &&before, double &are, double &aim) {
-  if constexpr (U <= 4) {
-    if (rem == 1) {
-      typename Ops::template Buf<1> t;
-      ops.template issue<1>(t);
-      __builtin_amdgcn_sched_barrier(0);
-      before();
-      ops.fma(t, are, aim);
-    }
-    if constexpr (U > 2) {
-      if (rem == 2) {
-        typename Ops::template Buf<2> t;
-        ops.template issue<2>(t);
-        __builtin_amdgcn_sched_barrier(0);
-        before();
-        ops.fma(t, are, aim);
-      }
-    }
-    if constexpr (U > 3) {
-      if (rem == 3) {
-        typename Ops::template Buf<3> t;
-        ops.template issue<3>(t);
-        __builtin_amdgcn_sched_barrier(0);
-        before();
-        ops.fma(t, are, aim);
-      }
-    }
-  } else {
-    typename Ops::template Buf<U> t;
-    ops.template issue<U, true>(t, rem);
-    __builtin_amdgcn_sched_barrier(0);
-    before();
-    ops.fma(t, are, aim);
-  }
-}
-
-// all n slots of one group of a slice, U per pass.  PIPE: the loads of pass p+1
-// are issued before the products of pass p (two passes in flight; the passes
-// alternate between two buffers so no loaded register is copied while its load
-// is outstanding, and the scheduling barriers keep the compiler from moving
-// the products of a pass above the issue of the next one).  The products are
-// summed in pass order either way.
-template <int U, bool PIPE, typename Ops>
-__device__ __forceinline__ void ks_run(Ops ops, int32_t n, double &are, double &aim) {
-  static_assert(U - 1 <= QH_KS_PAD, "tail pass reads past the padding");
-  typename Ops::template Buf<U> a, b;
-  const int32_t np = n / U, rem = n % U;
-  if (!PIPE || np == 0) {
-    for (int32_t p = 0; p < np; p++) {
-      ops.template issue<U>(a);
-      ops.fma(a, are, aim);
-    }
-    if (rem) ks_tail<U>(ops, rem, [] {}, are, aim);
-    return;
-  }
-  ops.template issue<U>(a);
-  int32_t p = 1;
-  for (; p + 1 < np; p += 2) {
-    ops.template issue<U>(b);
-    __builtin_amdgcn_sched_barrier(0);
-    ops.fma(a, are, aim);
-    ops.template issue<U>(a);
-    __builtin_amdgcn_sched_barrier(0);
-    ops.fma(b, are, aim);
-  }
-  if (p < np) {  // a in flight, one full pass left
-    ops.template issue<U>(b);
-    __builtin_amdgcn_sched_barrier(0);
-    ops.fma(a, are, aim);
-    if (rem)
-      ks_tail<U>(ops, rem, [&] { ops.fma(b, are, aim); }, are, aim);
-    else
-      ops.fma(b, are, aim);
-  } else {  // a in flight, no full pass left
-    if (rem)
-      ks_tail<U>(ops, rem, [&] { ops.fma(a, are, aim); }, are, aim);
-    else
-      ops.fma(a, are, aim);
-  }
-}
 
 template <int KIND, int U, bool PIPE>
 __device__ __forceinline__ void ks_group32(const qh_kmeta32 *__restrict__ meta, int32_t k, int32_t kend,
                                            uint32_t r8, __amdgpu_buffer_rsrc_t xr, __amdgpu_buffer_rsrc_t vr,
-                                           double &are, double &aim) {
+                                           KsAcc &acc) {
   KsOps32<KIND> ops{meta + k, r8, xr, vr};
-  ks_run<U, PIPE>(ops, kend - k, are, aim);
+  ks_run<U, PIPE>(ops, kend - k, acc);
 }
 
 template <typename R, int KIND, int U, bool VV, bool PIPE>
 __device__ __forceinline__ void ks_group(const qh_kmeta *__restrict__ meta, int32_t k, int32_t kend, uint32_t r8,
                                          __amdgpu_buffer_rsrc_t xr, __amdgpu_buffer_rsrc_t vr,
-                                         __amdgpu_buffer_rsrc_t mr, double &are, double &aim) {
+                                         __amdgpu_buffer_rsrc_t mr, KsAcc &acc) {
   KsOps<R, KIND, VV> ops{meta + k, r8, xr, vr, mr, k};
-  ks_run<U, PIPE>(ops, kend - k, are, aim);
+  ks_run<U, PIPE>(ops, kend - k, acc);
 }
 
 // Stage kernel on the sliced layout.  The stage input Y_s = x + c_s K_{s-1} is
@@ -567,16 +470,17 @@ __global__ void __launch_bounds__(256) k_ks_stage(
   const __amdgpu_buffer_rsrc_t sr = ks_rsrc(sval, (int64_t)nsval * (int64_t)sizeof(R));
   const __amdgpu_buffer_rsrc_t mr = ks_rsrc(meta, (int64_t)nmeta * (int64_t)sizeof(qh_kmeta));
   const uint32_t r8 = (uint32_t)r * (uint32_t)sizeof(T);
-  double are = 0.0, aim = 0.0;
+  KsAcc acc{0.0, 0.0};
   if (sizeof(R) == 4 && !VV) {
-    ks_group32<KS_UNI, U, PIPE>(meta32, k0, km, r8, xr, vr, are, aim);
-    ks_group32<KS_SCA, U, PIPE>(meta32, km, kc, r8, xr, sr, are, aim);
-    ks_group32<KS_CPX, U, PIPE>(meta32, kc, kend, r8, xr, vr, are, aim);
+    ks_group32<KS_UNI, U, PIPE>(meta32, k0, km, r8, xr, vr, acc);
+    ks_group32<KS_SCA, U, PIPE>(meta32, km, kc, r8, xr, sr, acc);
+    ks_group32<KS_CPX, U, PIPE>(meta32, kc, kend, r8, xr, vr, acc);
   } else {
-    ks_group<R, KS_UNI, U, VV, PIPE>(meta, k0, km, r8, xr, vr, mr, are, aim);
-    ks_group<R, KS_SCA, U, VV, PIPE>(meta, km, kc, r8, xr, sr, mr, are, aim);
-    ks_group<R, KS_CPX, U, VV, PIPE>(meta, kc, kend, r8, xr, vr, mr, are, aim);
+    ks_group<R, KS_UNI, U, VV, PIPE>(meta, k0, km, r8, xr, vr, mr, acc);
+    ks_group<R, KS_SCA, U, VV, PIPE>(meta, km, kc, r8, xr, sr, mr, acc);
+    ks_group<R, KS_CPX, U, VV, PIPE>(meta, kc, kend, r8, xr, vr, mr, acc);
   }
+  const double are = acc.re, aim = acc.im;
   if (r >= nrows) return;
   T y;
   y.x = (R)are;
@@ -629,15 +533,6 @@ __global__ void __launch_bounds__(256) k_ks_stage(
 // separates the two entries row-ad and row+ad of one row; the slice's slots are
 // the union of the lanes' keys in increasing order (a wave-wide min merge).
 const int KS_MAXROW = 64;
-
-__device__ inline uint64_t wave_min_u64(uint64_t v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    const uint64_t o = __shfl_xor(v, off, 64);
-    v = o < v ? o : v;
-  }
-  return v;
-}
 
 // Per-slice counts (FILL false): cnt[0*ns + s] slots, [1] complex values,
 // [2] uniform slots, [3] scalar slots, [4] scalar values.  FILL writes the
@@ -929,19 +824,33 @@ int launch_ks(int G, hipStream_t s, const qh_mat *A, const void *xin, void *kout
   return QH_OK;
 }
 
+// a stage on either sliced layout (the row-pair one is FP32 only)
+template <typename R, int MODE>
+int launch_sliced(int G, hipStream_t s, const qh_mat *A, const void *xin, void *kout, void *ynext, R cn,
+                  const void *x, const void *k1, const void *k2, const void *k3, void *xout, R h, float2 *xlo) {
+  if (QH_IS_KS2(G)) {
+    if (sizeof(R) != 4) {
+      qh_set_error("the row-pair sliced layout (variant %d) is FP32 only", G);
+      return QH_EINVAL;
+    }
+    return ks2_launch(G, MODE, s, A, xin, kout, ynext, (float)cn, x, k1, k2, k3, xout, (float)h, xlo);
+  }
+  return launch_ks<R, MODE>(G, s, A, xin, kout, ynext, cn, x, k1, k2, k3, xout, h, xlo);
+}
+
 template <typename R, int MODE>
 int launch_stage_g(int G, hipStream_t s, const qh_mat *A, const typename C2<R>::T *val, const void *x,
                    const void *kin, R ck, void *kout, const void *k1, const void *k2, const void *k3, void *xout,
                    R h, float2 *xlo) {
   typedef typename C2<R>::T T;
   const int32_t n = (int32_t)A->rows;
-  if (QH_IS_KSELL(G)) {  // plain y = A x on the sliced layout (stages go through launch_ks)
+  if (QH_IS_KSELL(G) || QH_IS_KS2(G)) {  // plain y = A x on a sliced layout (stages go through launch_sliced)
     if (MODE != MODE_SPMV) {
-      qh_set_error("sliced-layout stages go through launch_ks");
+      qh_set_error("sliced-layout stages go through launch_sliced");
       return QH_EINVAL;
     }
-    return launch_ks<R, MODE_SPMV>(G, s, A, x, kout, nullptr, (R)0, nullptr, nullptr, nullptr, nullptr, nullptr,
-                                   (R)0, nullptr);
+    return launch_sliced<R, MODE_SPMV>(G, s, A, x, kout, nullptr, (R)0, nullptr, nullptr, nullptr, nullptr, nullptr,
+                                       (R)0, nullptr);
   }
   const unsigned grid = qh_blocks((int64_t)n * G, 256);
 #define QH_L(GG)                                                                                                 \
@@ -965,21 +874,42 @@ int launch_stage_g(int G, hipStream_t s, const qh_mat *A, const typename C2<R>::
 }
 
 // ------------------------------------------------------------ autotuner
-// Time one SpMV for each lanes-per-row choice on the real matrix and keep the
-// fastest (rows-per-wavefront = 64/G).
+// Time each variant (CSR lanes per row, sliced layouts) on the real matrix with
+// one plain SpMV launch plus one RK stage launch (the stage epilogue's vector
+// traffic and registers are part of what the stepping runs) and keep the fastest.
 template <typename R>
 int autotune(qh_mat *A, int *out_g) {
   typedef typename C2<R>::T T;
   const typename C2<R>::T *val = sizeof(R) == 8 ? (const T *)A->val : (const T *)A->val32;
-  void *x = nullptr, *y = nullptr;
-  int rc = qh_alloc(&x, (size_t)A->rows * sizeof(T));
-  if (!rc) rc = qh_alloc(&y, (size_t)A->rows * sizeof(T));
-  if (rc) {
+  void *x = nullptr, *y = nullptr, *y2 = nullptr;
+  float2 *lo = nullptr;
+  const size_t vb = (size_t)A->rows * sizeof(T);
+  int rc = qh_alloc(&x, vb);
+  if (!rc) rc = qh_alloc(&y, vb);
+  if (!rc) rc = qh_alloc(&y2, vb);
+  if (!rc && sizeof(R) == 4) rc = qh_alloc_t(&lo, (size_t)A->rows);
+  auto release = [&]() {
     qh_free_ptr(x);
+    qh_free_ptr(y);
+    qh_free_ptr(y2);
+    qh_free_ptr(lo);
+  };
+  if (rc) {
+    release();
     return rc;
   }
   hipStream_t s = qh_stream();
-  QH_HIP(hipMemsetAsync(x, 0, (size_t)A->rows * sizeof(T), s));
+  QH_HIP(hipMemsetAsync(x, 0, vb, s));
+  QH_HIP(hipMemsetAsync(y2, 0, vb, s));
+  if (lo) QH_HIP(hipMemsetAsync(lo, 0, (size_t)A->rows * sizeof(float2), s));
+  auto run = [&](int g) {
+    launch_stage_g<R, MODE_SPMV>(g, s, A, val, x, nullptr, (R)0, y, nullptr, nullptr, nullptr, nullptr, (R)0,
+                                 nullptr);
+    if (QH_IS_KSELL(g) || QH_IS_KS2(g))
+      launch_sliced<R, MODE_STAGE>(g, s, A, x, y, y2, (R)0.5, x, nullptr, nullptr, nullptr, nullptr, (R)0, lo);
+    else
+      launch_stage_g<R, MODE_STAGE>(g, s, A, val, x, y2, (R)0.5, y, nullptr, nullptr, nullptr, nullptr, (R)0, lo);
+  };
   hipEvent_t e0, e1;
   QH_HIP(hipEventCreate(&e0));
   QH_HIP(hipEventCreate(&e1));
@@ -991,6 +921,14 @@ int autotune(qh_mat *A, int *out_g) {
     if (brc) return brc;
     if (A->ks_state == 1)
       for (int u = 128; u <= 134; u++) cands.push_back(u);
+    if (sizeof(R) == 4) {
+      brc = ks2_build(A);
+      if (brc) return brc;
+      if (A->ks2_state == 1) {
+        cands.push_back(140);
+        cands.push_back(141);
+      }
+    }
   }
   for (int g : {2, 4, 8, 16, 32, 64})
     if (g <= 4 * avg + 4) cands.push_back(g);
@@ -998,20 +936,16 @@ int autotune(qh_mat *A, int *out_g) {
   float best = 1e30f;
   int bestg = cands[0];
   for (int g : cands) {
-    for (int w = 0; w < 2; w++)
-      launch_stage_g<R, MODE_SPMV>(g, s, A, val, x, nullptr, (R)0, y, nullptr, nullptr, nullptr, nullptr, (R)0,
-                                   nullptr);
+    for (int w = 0; w < 2; w++) run(g);
     QH_HIP(hipEventRecord(e0, s));
     const int reps = 5;
-    for (int w = 0; w < reps; w++)
-      launch_stage_g<R, MODE_SPMV>(g, s, A, val, x, nullptr, (R)0, y, nullptr, nullptr, nullptr, nullptr, (R)0,
-                                   nullptr);
+    for (int w = 0; w < reps; w++) run(g);
     QH_HIP(hipEventRecord(e1, s));
     QH_HIP(hipEventSynchronize(e1));
     float ms = 0.f;
     QH_HIP(hipEventElapsedTime(&ms, e0, e1));
     if (getenv("QH_AUTOTUNE_VERBOSE"))
-      fprintf(stderr, "qh autotune: %s rows %lld nnz %lld variant %d: %.2f us per SpMV\n",
+      fprintf(stderr, "qh autotune: %s rows %lld nnz %lld variant %d: %.2f us per SpMV + stage launch\n",
               sizeof(R) == 8 ? "fp64" : "fp32", (long long)A->rows, (long long)A->nnz, g, 1e3 * ms / reps);
     if (ms < best) {
       best = ms;
@@ -1020,8 +954,7 @@ int autotune(qh_mat *A, int *out_g) {
   }
   QH_HIP(hipEventDestroy(e0));
   QH_HIP(hipEventDestroy(e1));
-  qh_free_ptr(x);
-  qh_free_ptr(y);
+  release();
   *out_g = bestg;
   return QH_OK;
 }
@@ -1039,6 +972,8 @@ int lanes_for(qh_mat *A, int prec, int *g) {
     if (!rc && QH_IS_KSELL(A->lanes32)) rc = ks_build(A);
     if (!rc && QH_IS_KSELL(A->lanes32)) rc = ks_make_fp32(A);
     if (!rc && QH_IS_KSELL(A->lanes32) && A->ks_state != 1) A->lanes32 = 8;
+    if (!rc && QH_IS_KS2(A->lanes32)) rc = ks2_build(A);
+    if (!rc && QH_IS_KS2(A->lanes32) && A->ks2_state != 1) A->lanes32 = 8;
     *g = A->lanes32;
   }
   return rc;
@@ -1155,7 +1090,7 @@ static int rk4_step(hipStream_t s, const qh_mat *A, int G, const void *xa, void 
   const T *val = sizeof(R) == 8 ? (const T *)A->val : (const T *)A->val32;
   void **k = g_st.k;
   int rc;
-  if (QH_IS_KSELL(G)) {
+  if (QH_IS_KSELL(G) || QH_IS_KS2(G)) {
     // stage inputs materialised in the epilogues: x -> Ya -> Yb -> Ya
     void *Ya = g_st.y[0], *Yb = g_st.y[1];
     const void *ins[4] = {xa, Ya, Yb, Ya};
@@ -1163,12 +1098,12 @@ static int rk4_step(hipStream_t s, const qh_mat *A, int G, const void *xa, void 
     const R cns[3] = {h / 2, h / 2, h};
     for (int st = 0; st < 3; st++) {
       if (ev) QH_HIP(hipEventRecord(ev[st], s));
-      rc = launch_ks<R, MODE_STAGE>(G, s, A, ins[st], k[st], outs[st], cns[st], xa, nullptr, nullptr, nullptr,
-                                    nullptr, h, xlo);
+      rc = launch_sliced<R, MODE_STAGE>(G, s, A, ins[st], k[st], outs[st], cns[st], xa, nullptr, nullptr, nullptr,
+                                        nullptr, h, xlo);
       if (rc) return rc;
     }
     if (ev) QH_HIP(hipEventRecord(ev[3], s));
-    rc = launch_ks<R, MODE_FINAL>(G, s, A, ins[3], nullptr, nullptr, (R)0, xa, k[0], k[1], k[2], xb, h, xlo);
+    rc = launch_sliced<R, MODE_FINAL>(G, s, A, ins[3], nullptr, nullptr, (R)0, xa, k[0], k[1], k[2], xb, h, xlo);
     if (ev) QH_HIP(hipEventRecord(ev[4], s));
     return rc;
   }
@@ -1508,7 +1443,9 @@ extern "C" int qh_mat_set_lanes(qh_mat *m, int l64, int l32) {
   auto ok = [](int g) {
     return g == 0 || g == 1 || g == 2 || g == 4 || g == 8 || g == 16 || g == 32 || g == 64 || QH_IS_KSELL(g);
   };
-  QH_REQUIRE(ok(l64) && ok(l32), QH_EINVAL, "lanes per row must be a power of two <= 64, or 128..134 (sliced layout)");
+  QH_REQUIRE(ok(l64) && (ok(l32) || QH_IS_KS2(l32)), QH_EINVAL,
+             "lanes per row must be a power of two <= 64, 128..134 (sliced layout), or for FP32 140/141 (row-pair "
+             "sliced layout)");
   m->lanes64 = l64;
   m->lanes32 = l32;
   m->lanes_forced = (l64 != 0 || l32 != 0);

@@ -94,8 +94,9 @@ def test_rk4_device_direct_matches_oracle_and_lanes():
     x0 = np.zeros(N * N, np.complex128)
     x0[loc] = 1
     ref = _oracle_run(r, x0, 0.01, 37)
-    for g in (1, 2, 4, 8, 16, 32, 64, 128, 129, 130, 131, 132, 133, 134):  # 128..134 = sliced layout
-        A.set_lanes(g, g)
+    # 128..134 = sliced layout; 140/141 = FP32 row-pair sliced layout (FP64 keeps 8 lanes per row there)
+    for g in (1, 2, 4, 8, 16, 32, 64, 128, 129, 130, 131, 132, 133, 134, 140, 141):
+        A.set_lanes(g if g < 140 else 8, g)
         v = qa.DeviceVec(N * N, qa.FP64)
         v.set(x0)
         qa.rk4_device(A, v, 0.01, 37)
@@ -103,6 +104,79 @@ def test_rk4_device_direct_matches_oracle_and_lanes():
         w = qa.DeviceVec(N * N, qa.FP32)
         w.set(x0)
         qa.rk4_device(A, w, 0.01, 37)
+        assert np.abs(w.get() - ref).max() < 1e-6, g
+    A.set_lanes(0, 0)
+    qs.destroy()
+
+
+def _odd_recipe():
+    """3-level x 5-level system: 225 rows, so the last lane of the last
+    128-row slice owns one row only"""
+    return {"subs": [("op", 3), ("op", 5)],
+            "terms": [("ham", 1.0, [(0, "n")]), ("ham", 0.7, [(1, "n")]),
+                      ("ham", 0.3, [(0, "raise"), (1, "lower")]), ("ham", 0.3, [(0, "lower"), (1, "raise")]),
+                      ("lin", 0.1, [(1, "lower")]), ("lin", 0.05, [(0, "lower")])], "mcwf": False}
+
+
+@pytest.mark.parametrize("name", ["chain10", "tc_batch", "odd"])
+def test_row_pair_layout_fp32_spmv(name):
+    """FP32 SpMV on the row-pair sliced layout (140 / 141) against the FP64 CSR
+    product: every row within 1e-6 of sum_k |a_k| |x_k| (complex64 values and
+    inputs, FP32 pass sums)"""
+    rng = np.random.default_rng(7)
+    keep = []
+    if name == "chain10":
+        qs, _, _ = build_system(recipes.spin_chain_recipe(10, seed=1))
+        A, keep = qs.matrix(), [qs]
+    elif name == "odd":
+        qs, _, _ = build_system(_odd_recipe())
+        A, keep = qs.matrix(), [qs]
+    else:
+        mats, tots = [], []
+        for p in recipes.tc_instances(40, 3):
+            q2, _, _ = build_system(recipes.tc_recipe(*p))
+            keep.append(q2)
+            mats.append(q2.matrix())
+            tots.append(recipes.tc_init(p[0], p[1])[0])
+        b = qa.Batch(mats, tots, prec=qa.FP32)
+        A = b.matrix()
+        keep.append(b)
+    n = A.info()[0]
+    x = (rng.normal(size=n) + 1j * rng.normal(size=n)).astype(np.complex64).astype(np.complex128)
+    xv = qa.DeviceVec(n, qa.FP32)
+    xv.set(x)
+    rp, col, val = A.export()
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    ref = np.zeros(n, np.complex128)
+    np.add.at(ref, rows, val * x[col])
+    mag = np.zeros(n)
+    np.add.at(mag, rows, np.abs(val) * np.abs(x[col]))
+    for g in (140, 141):
+        A.set_lanes(8, g)
+        yv = qa.DeviceVec(n, qa.FP32)
+        qa._qh(qa.lib().qh_spmv(A.h, xv.h, yv.h))
+        err = np.abs(yv.get() - ref)
+        assert np.all(err <= 1e-6 * mag + 1e-30), (g, float((err / np.maximum(mag, 1e-300)).max()))
+    A.set_lanes(0, 0)
+    for k in keep:
+        k.destroy()
+
+
+def test_row_pair_layout_rk4_odd_rows():
+    """FP32 RK4 on the row-pair layout for a matrix with an odd number of rows
+    (the last lane's second row absent): populations within 1e-6 of the oracle"""
+    r = _odd_recipe()
+    qs, _, _ = build_system(r)
+    A = qs.matrix()
+    N = 15
+    x0 = np.zeros(N * N, np.complex128)
+    x0[7 * N + 7] = 1.0
+    ref = _oracle_run(r, x0, 0.01, 200)
+    for g in (140, 141):
+        A.set_lanes(8, g)
+        w = qa.DeviceVec(N * N, qa.FP32)
+        w.set(x0)
+        qa.rk4_device(A, w, 0.01, 200)
         assert np.abs(w.get() - ref).max() < 1e-6, g
     A.set_lanes(0, 0)
     qs.destroy()
