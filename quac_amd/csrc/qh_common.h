@@ -109,6 +109,7 @@ struct qh_mat {
   int ks2_state = 0;           // 0 not built, 1 built, -1 not applicable
   int32_t ks2_nslices = 0;
   int32_t *ks2_moff = nullptr;  // nslices+1 slot offsets
+  int32_t *ks2_smid = nullptr;  // end of each slice's uniform slots with in-order halves (they come first)
   int32_t *ks2_umid = nullptr;  // end of each slice's uniform slots
   int32_t *ks2_cmid = nullptr;  // end of each slice's scalar slots
   qh_kmeta2 *ks2_meta = nullptr;

@@ -74,7 +74,9 @@ __device__ __forceinline__ void ks_tail(Ops &ops, int32_t rem, F &&before, typen
 // alternate between two buffers so no loaded register is copied while its load
 // is outstanding, and the scheduling barriers keep the compiler from moving
 // the products of a pass above the issue of the next one).  The products are
-// summed in pass order either way.
+// summed in pass order either way; ops.flush(acc) marks where the ops may move
+// partial sums into their final accumulators (after every second pass and at
+// the end).
 template <int U, bool PIPE, typename Ops>
 __device__ __forceinline__ void ks_run(Ops ops, int32_t n, typename Ops::Acc &acc) {
   static_assert(U - 1 <= QH_KS_PAD, "tail pass reads past the padding");
@@ -84,8 +86,10 @@ __device__ __forceinline__ void ks_run(Ops ops, int32_t n, typename Ops::Acc &ac
     for (int32_t p = 0; p < np; p++) {
       ops.template issue<U>(a);
       ops.fma(a, acc);
+      ops.flush(acc);
     }
     if (rem) ks_tail<U>(ops, rem, [] {}, acc);
+    ops.flush(acc);
     return;
   }
   ops.template issue<U>(a);
@@ -97,6 +101,7 @@ __device__ __forceinline__ void ks_run(Ops ops, int32_t n, typename Ops::Acc &ac
     ops.template issue<U>(a);
     __builtin_amdgcn_sched_barrier(0);
     ops.fma(b, acc);
+    ops.flush(acc);
   }
   if (p < np) {  // a in flight, one full pass left
     ops.template issue<U>(b);
@@ -112,6 +117,7 @@ __device__ __forceinline__ void ks_run(Ops ops, int32_t n, typename Ops::Acc &ac
     else
       ops.fma(a, acc);
   }
+  ops.flush(acc);
 }
 
 __device__ inline uint64_t wave_min_u64(uint64_t v) {

@@ -27,8 +27,10 @@ const int KS2_MAXROW = 32;  // longest row the pair builder takes (the CSR kerne
 
 // ------------------------------------------------------------------ build
 // Per-slice counts (FILL false): cnt[0*ns + s] slots, [1] complex pairs, [2]
-// uniform slots, [3] scalar slots, [4] scalar pairs.  FILL writes the metadata
-// (uniform, scalar and complex groups at moff, moff + uniform, + scalar) and the
+// uniform slots, [3] scalar slots, [4] scalar pairs, [5] uniform slots whose
+// rows take the halves in order (row 2l half 0, row 2l+1 half 1: no select in
+// the kernel).  FILL writes the metadata (uniform in-order slots at moff, the
+// other uniform slots after them, then the scalar and complex groups) and the
 // values (complex pairs at voff, scalar pairs at svoff).
 template <bool FILL>
 __global__ void __launch_bounds__(64) k_ks2_build(int32_t n, const int32_t *__restrict__ rp,
@@ -109,12 +111,13 @@ __global__ void __launch_bounds__(64) k_ks2_build(int32_t n, const int32_t *__re
     prev = adp;
   }
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  int p = 0, slot = 0, uslot = 0, sslot = 0, cslot = 0;
+  int p = 0, slot = 0, uslot = 0, sslot = 0, cslot = 0, ustr = 0;
   int32_t base = FILL ? voff[s] : 0;
   int32_t sbase = FILL ? svoff[s] : 0;
   const int32_t mo = FILL ? moff[s] : 0;
   const int32_t ms = FILL ? mo + cnt[2 * ns + s] : 0;
   const int32_t mc = FILL ? ms + cnt[3 * ns + s] : 0;
+  const int32_t mu2 = FILL ? mo + cnt[5 * ns + s] : 0;  // uniform slots that need the half selects
   for (;;) {
     const uint64_t cur = p < np ? (kk[p] >> 12) : ~0ull;
     const uint64_t m = wave_min_u64(cur);
@@ -135,6 +138,9 @@ __global__ void __launch_bounds__(64) k_ks2_build(int32_t n, const int32_t *__re
     const bool isre = !uni && __ballot(act && (v0.y != 0.0 || v1.y != 0.0)) == 0ull;
     const bool isim = !uni && !isre && __ballot(act && (v0.x != 0.0 || v1.x != 0.0)) == 0ull;
     const bool sca = isre || isim;
+    // in-order halves: an active row 2l takes half 0, an active row 2l+1 half 1
+    const uint32_t pat_ = (uint32_t)((m >> 1) & 15u);
+    const bool inorder = (!(pat_ & 1u) || !(pat_ & 4u)) && (!(pat_ & 2u) || (pat_ & 8u));
     if (FILL) {
       // slot key m = (ad << 4 | pattern) << 1 | rank
       const uint32_t pat = (uint32_t)((m >> 1) & 15u);
@@ -159,7 +165,7 @@ __global__ void __launch_bounds__(64) k_ks2_build(int32_t n, const int32_t *__re
           md.v0re = md.v1re = sca ? (isre ? 1.f : 0.f) : 0.f;
           md.v0im = md.v1im = sca ? (isre ? 0.f : 1.f) : 0.f;
         }
-        meta[uni ? mo + uslot : sca ? ms + sslot : mc + cslot] = md;
+        meta[uni ? (inorder ? mo + ustr : mu2 + (uslot - ustr)) : sca ? ms + sslot : mc + cslot] = md;
       }
       if (act && sca)
         sval[sbase + rank] = isre ? make_float2((float)v0.x, (float)v1.x) : make_float2((float)v0.y, (float)v1.y);
@@ -172,12 +178,14 @@ __global__ void __launch_bounds__(64) k_ks2_build(int32_t n, const int32_t *__re
       base += __popcll(mask);
     if (act) p++;
     slot++;
-    if (uni)
+    if (uni) {
       uslot++;
-    else if (sca)
+      if (inorder) ustr++;
+    } else if (sca) {
       sslot++;
-    else
+    } else {
       cslot++;
+    }
   }
   if (!FILL && lane == 0) {
     cnt[s] = slot;
@@ -185,12 +193,16 @@ __global__ void __launch_bounds__(64) k_ks2_build(int32_t n, const int32_t *__re
     cnt[2 * ns + s] = uslot;
     cnt[3 * ns + s] = sslot;
     cnt[4 * ns + s] = sbase;
+    cnt[5 * ns + s] = ustr;
   }
 }
 
 // ------------------------------------------------------------------ stage
+// row sums of rows 2l and 2l+1: FP32 partial sums of the last passes, flushed
+// into the FP64 sums every second pass
 struct KsAcc2 {
-  double r0, i0, r1, i1;  // row sums of rows 2l and 2l+1
+  double r0, i0, r1, i1;
+  float p0r, p0i, p1r, p1i;
 };
 
 template <int KIND, int W>
@@ -201,7 +213,9 @@ struct Ks2Buf {
   uint32_t h[W];
 };
 
-template <int KIND>
+// INORDER: every slot of the group has row 2l on half 0 and row 2l+1 on half 1
+// (known at compile time, no selects)
+template <int KIND, bool INORDER = false>
 struct Ks2Ops {
   typedef KsAcc2 Acc;
   template <int W>
@@ -226,7 +240,9 @@ struct Ks2Ops {
       const bool on = __builtin_amdgcn_inverse_ballot_w64(m[j].mask);
       const bool neg = __builtin_amdgcn_inverse_ballot_w64(m[j].sign);
       const uint32_t ad8 = m[j].ad8 & ~7u;
-      uint32_t off = neg ? r8 - ad8 : r8 + ad8;
+      // r8 + ad8 + (neg ? -2 ad8 : 0): one select against zero and a three-way add
+      const uint32_t back = neg ? (0u - 2u * ad8) : 0u;
+      uint32_t off = r8 + ad8 + back;
       off = on ? off : QH_KS_OOB;
       b.xv[j] = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(xr, off, 0, 0));
       if (KIND != KS_UNI) {
@@ -243,13 +259,23 @@ struct Ks2Ops {
   }
   template <int W>
   __device__ __forceinline__ void fma(const Buf<W> &b, Acc &acc) {
-    float p0r = 0.f, p0i = 0.f, p1r = 0.f, p1i = 0.f;
+    float p0r = acc.p0r, p0i = acc.p0i, p1r = acc.p1r, p1i = acc.p1i;
 #pragma unroll
     for (int j = 0; j < W; j++) {
       const float4 xv = b.xv[j];
-      const bool h0 = b.h[j] & 1u, h1 = b.h[j] & 2u;
-      const float x0r = h0 ? xv.z : xv.x, x0i = h0 ? xv.w : xv.y;
-      const float x1r = h1 ? xv.z : xv.x, x1i = h1 ? xv.w : xv.y;
+      float x0r, x0i, x1r, x1i;
+      if (INORDER) {
+        x0r = xv.x;
+        x0i = xv.y;
+        x1r = xv.z;
+        x1i = xv.w;
+      } else {
+        const bool h0 = b.h[j] & 1u, h1 = b.h[j] & 2u;
+        x0r = h0 ? xv.z : xv.x;
+        x0i = h0 ? xv.w : xv.y;
+        x1r = h1 ? xv.z : xv.x;
+        x1i = h1 ? xv.w : xv.y;
+      }
       float a0r, a0i, a1r, a1i;
       if (KIND == KS_CPX) {
         a0r = b.vv[j].x;
@@ -272,10 +298,17 @@ struct Ks2Ops {
       p1r = fmaf(a1r, x1r, fmaf(-a1i, x1i, p1r));
       p1i = fmaf(a1r, x1i, fmaf(a1i, x1r, p1i));
     }
-    acc.r0 += (double)p0r;
-    acc.i0 += (double)p0i;
-    acc.r1 += (double)p1r;
-    acc.i1 += (double)p1i;
+    acc.p0r = p0r;
+    acc.p0i = p0i;
+    acc.p1r = p1r;
+    acc.p1i = p1i;
+  }
+  __device__ __forceinline__ void flush(Acc &acc) {
+    acc.r0 += (double)acc.p0r;
+    acc.i0 += (double)acc.p0i;
+    acc.r1 += (double)acc.p1r;
+    acc.i1 += (double)acc.p1i;
+    acc.p0r = acc.p0i = acc.p1r = acc.p1i = 0.f;
   }
 };
 
@@ -301,7 +334,8 @@ __device__ inline void st_pair(__amdgpu_buffer_rsrc_t rs, uint32_t o, bool two, 
 // with the FP32 state x = hi + lo updated in FP64 and split again.
 template <int MODE, int U>
 __global__ void __launch_bounds__(256) k_ks2_stage(int32_t n, int32_t nslices, const int32_t *__restrict__ moff,
-                                                   const int32_t *__restrict__ umid, const int32_t *__restrict__ cmid,
+                                                   const int32_t *__restrict__ smid, const int32_t *__restrict__ umid,
+                                                   const int32_t *__restrict__ cmid,
                                                    const qh_kmeta2 *__restrict__ meta, int32_t nval,
                                                    const float4 *__restrict__ val, int32_t nsval,
                                                    const float2 *__restrict__ sval, const float2 *xin, float2 *kout,
@@ -313,14 +347,15 @@ __global__ void __launch_bounds__(256) k_ks2_stage(int32_t n, int32_t nslices, c
   if (s >= nslices) return;  // whole wavefront
   const int lane = threadIdx.x & 63;
   const int32_t r = s * 128 + 2 * lane;
-  const int32_t k0 = moff[s], km = umid[s], kc = cmid[s], kend = moff[s + 1];
+  const int32_t k0 = moff[s], ku = smid[s], km = umid[s], kc = cmid[s], kend = moff[s + 1];
   const int64_t vb = (int64_t)n * 8;
   const __amdgpu_buffer_rsrc_t xr = ks_rsrc(xin, vb);
   const __amdgpu_buffer_rsrc_t vr = ks_rsrc(val, (int64_t)nval * 16);
   const __amdgpu_buffer_rsrc_t sr = ks_rsrc(sval, (int64_t)nsval * 8);
   const uint32_t r8 = (uint32_t)r * 8u;
-  KsAcc2 acc{0.0, 0.0, 0.0, 0.0};
-  ks_run<U, true>(Ks2Ops<KS_UNI>{meta + k0, r8, xr, vr}, km - k0, acc);
+  KsAcc2 acc{0.0, 0.0, 0.0, 0.0, 0.f, 0.f, 0.f, 0.f};
+  ks_run<U, true>(Ks2Ops<KS_UNI, true>{meta + k0, r8, xr, vr}, ku - k0, acc);
+  ks_run<U, true>(Ks2Ops<KS_UNI>{meta + ku, r8, xr, vr}, km - ku, acc);
   ks_run<U, true>(Ks2Ops<KS_SCA>{meta + km, r8, xr, sr}, kc - km, acc);
   ks_run<U, true>(Ks2Ops<KS_CPX>{meta + kc, r8, xr, vr}, kend - kc, acc);
   if (r >= n) return;
@@ -372,7 +407,7 @@ int ks2_build(qh_mat *A) {
   const int32_t ns = (int32_t)((A->rows + 127) / 128);
   int32_t *cnt = nullptr, *voff = nullptr, *svoff = nullptr;
   int *ovf = nullptr;
-  int rc = qh_alloc_t(&cnt, (size_t)5 * ns);
+  int rc = qh_alloc_t(&cnt, (size_t)6 * ns);
   if (!rc) rc = qh_alloc_t(&voff, (size_t)ns + 1);
   if (!rc) rc = qh_alloc_t(&svoff, (size_t)ns + 1);
   if (!rc) rc = qh_alloc_t(&ovf, 1);
@@ -391,17 +426,18 @@ int ks2_build(qh_mat *A) {
                      cnt, ns, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, ovf);
   QH_HIP(hipGetLastError());
   int hovf = 0;
-  std::vector<int32_t> hc((size_t)5 * ns);
+  std::vector<int32_t> hc((size_t)6 * ns);
   QH_HIP(hipMemcpyAsync(&hovf, ovf, sizeof(int), hipMemcpyDeviceToHost, s));
-  QH_HIP(hipMemcpyAsync(hc.data(), cnt, sizeof(int32_t) * 5 * ns, hipMemcpyDeviceToHost, s));
+  QH_HIP(hipMemcpyAsync(hc.data(), cnt, sizeof(int32_t) * 6 * ns, hipMemcpyDeviceToHost, s));
   QH_HIP(hipStreamSynchronize(s));
-  const int32_t *hw = hc.data(), *hn = hw + ns, *hu = hn + ns, *hs = hu + ns, *hsv = hs + ns;
-  std::vector<int32_t> ho(ns + 1), hv(ns + 1), hso(ns + 1), hm(ns), hcm(ns);
+  const int32_t *hw = hc.data(), *hn = hw + ns, *hu = hn + ns, *hs = hu + ns, *hsv = hs + ns, *hst = hsv + ns;
+  std::vector<int32_t> ho(ns + 1), hv(ns + 1), hso(ns + 1), hm(ns), hcm(ns), hsm(ns);
   int64_t tot = 0, totv = 0, tots = 0;
   for (int32_t i = 0; i < ns; i++) {
     ho[i] = (int32_t)tot;
     hv[i] = (int32_t)totv;
     hso[i] = (int32_t)tots;
+    hsm[i] = (int32_t)(tot + hst[i]);
     hm[i] = (int32_t)(tot + hu[i]);
     hcm[i] = (int32_t)(tot + hu[i] + hs[i]);
     tot += hw[i];
@@ -417,6 +453,7 @@ int ks2_build(qh_mat *A) {
   hv[ns] = (int32_t)totv;
   hso[ns] = (int32_t)tots;
   rc = qh_alloc_t(&A->ks2_moff, (size_t)ns + 1);
+  if (!rc) rc = qh_alloc_t(&A->ks2_smid, (size_t)ns);
   if (!rc) rc = qh_alloc_t(&A->ks2_umid, (size_t)ns);
   if (!rc) rc = qh_alloc_t(&A->ks2_cmid, (size_t)ns);
   if (!rc) rc = qh_alloc_t(&A->ks2_meta, (size_t)(tot > 0 ? tot : 1));
@@ -427,6 +464,7 @@ int ks2_build(qh_mat *A) {
     return rc;
   }
   QH_HIP(hipMemcpyAsync(A->ks2_moff, ho.data(), sizeof(int32_t) * (ns + 1), hipMemcpyHostToDevice, s));
+  QH_HIP(hipMemcpyAsync(A->ks2_smid, hsm.data(), sizeof(int32_t) * ns, hipMemcpyHostToDevice, s));
   QH_HIP(hipMemcpyAsync(A->ks2_umid, hm.data(), sizeof(int32_t) * ns, hipMemcpyHostToDevice, s));
   QH_HIP(hipMemcpyAsync(A->ks2_cmid, hcm.data(), sizeof(int32_t) * ns, hipMemcpyHostToDevice, s));
   QH_HIP(hipMemcpyAsync(voff, hv.data(), sizeof(int32_t) * (ns + 1), hipMemcpyHostToDevice, s));
@@ -455,7 +493,8 @@ int ks2_launch(int G, int mode, hipStream_t s, const qh_mat *A, const void *xin,
   const dim3 grid(qh_blocks(A->ks2_nslices, 4)), blk(256);
   const int32_t n = (int32_t)A->rows, ns = A->ks2_nslices;
 #define QH_LK2(MM, UU)                                                                                          \
-  hipLaunchKernelGGL((k_ks2_stage<MM, UU>), grid, blk, 0, s, n, ns, A->ks2_moff, A->ks2_umid, A->ks2_cmid,      \
+  hipLaunchKernelGGL((k_ks2_stage<MM, UU>), grid, blk, 0, s, n, ns, A->ks2_moff, A->ks2_smid, A->ks2_umid,      \
+                     A->ks2_cmid,                                                                               \
                      A->ks2_meta, (int32_t)A->ks2_nval, A->ks2_val, (int32_t)A->ks2_nsval, A->ks2_sval,          \
                      (const float2 *)xin, (float2 *)kout, (float2 *)ynext, cn, (const float2 *)x,                \
                      (const float2 *)k1, (const float2 *)k2, (const float2 *)k3, (float2 *)xout, h, xlo)

@@ -398,6 +398,7 @@ struct KsOps32 {
   __device__ __forceinline__ void fma(const Buf<W> &b, Acc &a) {
     ks_fma32<KIND, W>(b, a.re, a.im);
   }
+  __device__ __forceinline__ void flush(Acc &) {}
 };
 
 template <typename R, int KIND, bool VV>
@@ -419,6 +420,7 @@ struct KsOps {
   __device__ __forceinline__ void fma(const Buf<W> &b, Acc &a) {
     ks_fma<R, KIND, W, VV>(b, a.re, a.im);
   }
+  __device__ __forceinline__ void flush(Acc &) {}
 };
 
 template <int KIND, int U, bool PIPE>
