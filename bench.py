@@ -89,6 +89,10 @@ def main():
     ap.add_argument("--test", type=int, default=200)
     ap.add_argument("--rank", type=int, default=128)
     ap.add_argument("--epochs", type=int, default=2000)
+    ap.add_argument("--fp32-variant", type=int, default=0,
+                    help="force the FP32 stage-kernel variant (qh_mat_set_lanes code; 0 = autotune). Counter "
+                         "collection perturbs the autotuner's timings, so profiled runs name the variant the "
+                         "plain run selects")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on exactly one GPU: this build targets a single MI355X "
@@ -117,6 +121,8 @@ def main():
     qa.synchronize()
     assemble_s = time.perf_counter() - t0
     A = qs.matrix()
+    if args.fp32_variant:
+        A.set_lanes(0, args.fp32_variant)
     rows_, nnz = A.info()
     assert rows_ == rows
     lay = subsystem_layout(recipe)
@@ -170,8 +176,15 @@ def main():
     # bytes the sliced-layout step actually has to move: 4 x (stored values + slot
     # metadata) + 17 vector passes (4 gathers, 3 K and 3 Y writes, 3 x reads for
     # the Y's, K1..K3 + x reads and the x' write in the last epilogue)
-    meta32 = layout["slots"] * 32 + 12 * layout["slices"]
-    ks_step32 = 4 * (layout["stored_vals"] * 8 + layout["stored_reals"] * 4 + meta32) + 17 * rows * 8
+    if lanes[1] in (140, 141):  # row-pair layout: 40-byte slot metadata, values stored per lane pair
+        layout = dict(A.pair_layout_info(), kind="row-pair (128 rows per wavefront)")
+        meta32 = layout["meta_bytes"]
+        vals32 = layout["stored_pairs"] * 16 + layout["stored_real_pairs"] * 8
+    else:
+        layout = dict(layout, kind="one row per lane (64 rows per wavefront)")
+        meta32 = layout["slots"] * 32 + 12 * layout["slices"]
+        vals32 = layout["stored_vals"] * 8 + layout["stored_reals"] * 4
+    ks_step32 = 4 * (vals32 + meta32) + 17 * rows * 8
 
     # mean stage-kernel duration over the timed FP32 region: its device time
     # (event pair) over its 4K stage launches (includes the renormalisation
@@ -186,7 +199,7 @@ def main():
     spmv_gbs64 = spmv_bytes(nnz, rows, 16) / (spmv_ms64 * 1e-3) / 1e9
 
     kname = kernel_name(lanes[1], "float")
-    traffic, traffic_src = read_traffic(os.path.join(ROOT, "profiles", "k_ks_stage_chain10_traffic.txt"))
+    traffic, traffic_src = read_traffic(os.path.join(ROOT, "profiles", "k_ks2_stage_chain10_traffic.txt"))
 
     steps_s32 = K / el32
     steps_s64 = K / el64

@@ -78,6 +78,11 @@ int qh_mat_get_lanes(const qh_mat *m, int *lanes64, int *lanes32);
  * and the metadata bytes */
 int qh_mat_layout_info(qh_mat *m, int64_t *slices, int64_t *slots, int64_t *stored_vals, int64_t *stored_reals,
                        int64_t *meta_bytes);
+/* FP32 row-pair sliced layout statistics (built on demand; zeros when it does
+ * not apply): 128-row slices, slots, stored complex pairs, stored real pairs,
+ * metadata bytes */
+int qh_mat_pair_layout_info(qh_mat *m, int64_t *slices, int64_t *slots, int64_t *stored_pairs,
+                            int64_t *stored_real_pairs, int64_t *meta_bytes);
 
 /* ------------------------------------------------------------------ vectors */
 typedef struct qh_vec qh_vec;

@@ -949,7 +949,9 @@ int autotune(qh_mat *A, int *out_g) {
     if (getenv("QH_AUTOTUNE_VERBOSE"))
       fprintf(stderr, "qh autotune: %s rows %lld nnz %lld variant %d: %.2f us per SpMV + stage launch\n",
               sizeof(R) == 8 ? "fp64" : "fp32", (long long)A->rows, (long long)A->nnz, g, 1e3 * ms / reps);
-    if (ms < best) {
+    // a later candidate has to beat the current choice by 2 %, so that variants
+    // within timing noise of each other do not flip from run to run
+    if (ms < 0.98f * best) {
       best = ms;
       bestg = g;
     }
@@ -1466,6 +1468,20 @@ extern "C" int qh_mat_layout_info(qh_mat *m, int64_t *slices, int64_t *slots, in
   if (stored) *stored = ok ? m->ks_nval : 0;
   if (sreals) *sreals = ok ? m->ks_nsval : 0;
   if (mbytes) *mbytes = ok ? (int64_t)(m->ks_slots * sizeof(qh_kmeta) + 12 * m->nslices + 4) : 0;
+  return QH_OK;
+}
+
+extern "C" int qh_mat_pair_layout_info(qh_mat *m, int64_t *slices, int64_t *slots, int64_t *stored_pairs,
+                                       int64_t *stored_real_pairs, int64_t *mbytes) {
+  QH_REQUIRE(m, QH_EINVAL, "null matrix");
+  int rc = ks2_build(m);
+  if (rc) return rc;
+  const bool ok = m->ks2_state == 1;
+  if (slices) *slices = ok ? m->ks2_nslices : 0;
+  if (slots) *slots = ok ? m->ks2_slots : 0;
+  if (stored_pairs) *stored_pairs = ok ? m->ks2_nval : 0;
+  if (stored_real_pairs) *stored_real_pairs = ok ? m->ks2_nsval : 0;
+  if (mbytes) *mbytes = ok ? (int64_t)(m->ks2_slots * sizeof(qh_kmeta2) + 16 * m->ks2_nslices + 4) : 0;
   return QH_OK;
 }
 
