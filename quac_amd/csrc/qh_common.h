@@ -143,6 +143,10 @@ int ks2_launch(int G, int mode, hipStream_t s, const qh_mat *A, const void *xin,
 // device allocation helpers (return QH_* codes)
 int qh_alloc(void **p, size_t bytes);
 void qh_free_ptr(void *p);
+// persistent scratch of the synchronous reductions: a device buffer of at least
+// dbytes and a pinned host buffer of at least hbytes (either pointer may be
+// null); valid until the next qh_scratch call, which may reallocate them
+int qh_scratch(size_t dbytes, void **d, size_t hbytes, void **h);
 
 template <typename T>
 static inline int qh_alloc_t(T **p, size_t count) {

@@ -157,8 +157,10 @@ __global__ void __launch_bounds__(256) k_pops(const void *x, const float2 *lo, i
 }
 
 int finish2(double2 *dpart, int nb, double *re, double *im) {
-  std::vector<double2> h(nb);
-  QH_HIP(hipMemcpyAsync(h.data(), dpart, sizeof(double2) * nb, hipMemcpyDeviceToHost, qh_stream()));
+  double2 *h = nullptr;
+  int rc = qh_scratch(0, nullptr, sizeof(double2) * nb, (void **)&h);
+  if (rc) return rc;
+  QH_HIP(hipMemcpyAsync(h, dpart, sizeof(double2) * nb, hipMemcpyDeviceToHost, qh_stream()));
   QH_HIP(hipStreamSynchronize(qh_stream()));
   double a = 0.0, b = 0.0;
   for (int i = 0; i < nb; i++) {
@@ -177,13 +179,23 @@ int blocks_for(int64_t n) {
   return (int)b;
 }
 
-int upload_chain(int nops, const qh_op *ops, int nlev, const qh_level *lev, qh_op **dops, qh_level **dlev) {
+size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// The op chain and the NB partial sums of one reduction in the scratch buffer:
+// [ops | levels | partials], each part 256-byte aligned.
+int upload_chain(int nops, const qh_op *ops, int nlev, const qh_level *lev, qh_op **dops, qh_level **dlev,
+                 size_t part_bytes, void **part) {
   for (int k = 0; k < nops; k++)
     QH_REQUIRE(ops[k].levels > 0 && ops[k].stride > 0 && ops[k].tab >= 0 && ops[k].tab + ops[k].levels <= nlev,
                QH_EINVAL, "observable op %d out of range", k);
-  int rc = qh_alloc_t(dops, nops > 0 ? nops : 1);
-  if (!rc) rc = qh_alloc_t(dlev, nlev > 0 ? nlev : 1);
+  const size_t bo = align256(sizeof(qh_op) * (nops > 0 ? nops : 1)),
+               bl = align256(sizeof(qh_level) * (nlev > 0 ? nlev : 1));
+  char *d = nullptr;
+  int rc = qh_scratch(bo + bl + part_bytes, (void **)&d, 0, nullptr);
   if (rc) return rc;
+  *dops = (qh_op *)d;
+  *dlev = (qh_level *)(d + bo);
+  *part = d + bo + bl;
   if (nops) QH_HIP(hipMemcpyAsync(*dops, ops, sizeof(qh_op) * nops, hipMemcpyHostToDevice, qh_stream()));
   if (nlev) QH_HIP(hipMemcpyAsync(*dlev, lev, sizeof(qh_level) * nlev, hipMemcpyHostToDevice, qh_stream()));
   return QH_OK;
@@ -196,8 +208,7 @@ int expect_impl(int dm, const qh_vec *x, int64_t tot, int nops, const qh_op *ops
   qh_op *dops = nullptr;
   qh_level *dlev = nullptr;
   double2 *part = nullptr;
-  int rc = upload_chain(nops, ops, nlev, lev, &dops, &dlev);
-  if (!rc) rc = qh_alloc_t(&part, NB);
+  int rc = upload_chain(nops, ops, nlev, lev, &dops, &dlev, sizeof(double2) * NB, (void **)&part);
   if (!rc) {
     const int nb = blocks_for(tot);
     if (dm)
@@ -208,9 +219,6 @@ int expect_impl(int dm, const qh_vec *x, int64_t tot, int nops, const qh_op *ops
                          nops, dlev, part);
     rc = finish2(part, nb, re, im);
   }
-  qh_free_ptr(dops);
-  qh_free_ptr(dlev);
-  qh_free_ptr(part);
   return rc;
 }
 
@@ -229,13 +237,11 @@ extern "C" int qh_expect_wf(const qh_vec *x, int64_t tot, int nops, const qh_op 
 extern "C" int qh_trace(const qh_vec *x, int64_t tot, double *re, double *im) {
   QH_REQUIRE(x && tot * tot == x->n, QH_EINVAL, "qh_trace: not a density matrix of dimension %lld", (long long)tot);
   double2 *part = nullptr;
-  int rc = qh_alloc_t(&part, NB);
+  int rc = qh_scratch(sizeof(double2) * NB, (void **)&part, 0, nullptr);
   if (rc) return rc;
   const int nb = blocks_for(tot);
   hipLaunchKernelGGL(k_trace, dim3(nb), dim3(256), 0, qh_stream(), (const void *)x->d, x->lo, tot, part);
-  rc = finish2(part, nb, re, im);
-  qh_free_ptr(part);
-  return rc;
+  return finish2(part, nb, re, im);
 }
 
 extern "C" int qh_populations(const qh_vec *x, int nsub, const int64_t *levels, const int64_t *strides, int dm,
@@ -251,15 +257,13 @@ extern "C" int qh_populations(const qh_vec *x, int nsub, const int64_t *levels, 
     tot *= levels[s];
   }
   QH_REQUIRE(dm ? tot * tot == x->n : tot == x->n, QH_EINVAL, "qh_populations: vector size mismatch");
-  double *part = nullptr;
-  int rc = qh_alloc_t(&part, (size_t)NB * MAXSUB);
-  if (rc) return rc;
+  double *part = nullptr, *h = nullptr;
   const int nb = blocks_for(tot);
+  int rc = qh_scratch(sizeof(double) * NB * MAXSUB, (void **)&part, sizeof(double) * (size_t)nb * MAXSUB, (void **)&h);
+  if (rc) return rc;
   hipLaunchKernelGGL(k_pops, dim3(nb), dim3(256), 0, qh_stream(), (const void *)x->d, x->lo, tot, dm, d, part);
-  std::vector<double> h((size_t)nb * MAXSUB);
-  QH_HIP(hipMemcpyAsync(h.data(), part, sizeof(double) * h.size(), hipMemcpyDeviceToHost, qh_stream()));
+  QH_HIP(hipMemcpyAsync(h, part, sizeof(double) * (size_t)nb * MAXSUB, hipMemcpyDeviceToHost, qh_stream()));
   QH_HIP(hipStreamSynchronize(qh_stream()));
-  qh_free_ptr(part);
   for (int s = 0; s < nsub; s++) {
     double a = 0.0;
     for (int b = 0; b < nb; b++) a += h[(size_t)b * MAXSUB + s];
@@ -362,13 +366,13 @@ extern "C" int qh_ens_expect(const qh_vec *ens, int64_t M, int64_t tot, int nops
   qh_op *dops = nullptr;
   qh_level *dlev = nullptr;
   double2 *out = nullptr;
-  rc = upload_chain(nops, ops, nlev, levels, &dops, &dlev);
-  if (!rc) rc = qh_alloc_t(&out, (size_t)M);
+  rc = upload_chain(nops, ops, nlev, levels, &dops, &dlev, sizeof(double2) * (size_t)M, (void **)&out);
+  double2 *h = nullptr;
+  if (!rc) rc = qh_scratch(0, nullptr, sizeof(double2) * (size_t)M, (void **)&h);
   if (!rc) {
     hipLaunchKernelGGL(k_expect_ens, dim3((unsigned)M), dim3(256), 0, qh_stream(), (const double2 *)ens->d, tot,
                        dops, nops, dlev, out);
-    std::vector<double2> h((size_t)M);
-    if (hipMemcpyAsync(h.data(), out, sizeof(double2) * M, hipMemcpyDeviceToHost, qh_stream()) != hipSuccess ||
+    if (hipMemcpyAsync(h, out, sizeof(double2) * M, hipMemcpyDeviceToHost, qh_stream()) != hipSuccess ||
         hipStreamSynchronize(qh_stream()) != hipSuccess) {
       qh_set_error("qh_ens_expect: device copy failed");
       rc = QH_EHIP;
@@ -382,8 +386,5 @@ extern "C" int qh_ens_expect(const qh_vec *ens, int64_t M, int64_t tot, int nops
       *im = si / (double)M;
     }
   }
-  qh_free_ptr(dops);
-  qh_free_ptr(dlev);
-  qh_free_ptr(out);
   return rc;
 }

@@ -3,6 +3,8 @@
 #include <cstring>
 #include <mutex>
 
+#include <algorithm>
+
 #include "qh_common.h"
 
 static thread_local char g_err[1024] = "";
@@ -52,9 +54,52 @@ int qh_num_cus() { return g_cus; }
 // Event pair on the library stream bracketing a caller's timed region.
 static hipEvent_t g_timer[2] = {nullptr, nullptr};
 
+// Scratch of the synchronous reductions (observables, error norms): one device
+// and one pinned host buffer per process, grown on demand and reused, so these
+// calls neither allocate nor free on the device in the common case.  A caller
+// enqueues its work on the library stream and synchronizes that stream before
+// it returns, so two calls never use the buffers at the same time.
+static void *g_sdev = nullptr, *g_shost = nullptr;
+static size_t g_sdev_n = 0, g_shost_n = 0;
+
+int qh_scratch(size_t dbytes, void **d, size_t hbytes, void **h) {
+  int rc = qh_ensure_init();
+  if (rc) return rc;
+  if (d && dbytes > g_sdev_n) {
+    QH_HIP(hipStreamSynchronize(g_stream));
+    if (g_sdev) (void)hipFree(g_sdev);
+    g_sdev = nullptr;
+    g_sdev_n = 0;
+    const size_t n = std::max(dbytes, (size_t)1 << 16);
+    rc = qh_alloc(&g_sdev, n);
+    if (rc) return rc;
+    g_sdev_n = n;
+  }
+  if (h && hbytes > g_shost_n) {
+    QH_HIP(hipStreamSynchronize(g_stream));
+    if (g_shost) (void)hipHostFree(g_shost);
+    g_shost = nullptr;
+    g_shost_n = 0;
+    const size_t n = std::max(hbytes, (size_t)1 << 16);
+    hipError_t e = hipHostMalloc(&g_shost, n, hipHostMallocDefault);
+    if (e != hipSuccess) {
+      qh_set_error("hipHostMalloc(%zu) failed: %s", n, hipGetErrorString(e));
+      return QH_ENOMEM;
+    }
+    g_shost_n = n;
+  }
+  if (d) *d = g_sdev;
+  if (h) *h = g_shost;
+  return QH_OK;
+}
+
 extern "C" int qh_finalize(void) {
   if (!g_init) return QH_OK;
   QH_HIP(hipStreamSynchronize(g_stream));
+  if (g_sdev) (void)hipFree(g_sdev);
+  if (g_shost) (void)hipHostFree(g_shost);
+  g_sdev = g_shost = nullptr;
+  g_sdev_n = g_shost_n = 0;
   for (int i = 0; i < 2; ++i)
     if (g_timer[i]) {
       QH_HIP(hipEventDestroy(g_timer[i]));

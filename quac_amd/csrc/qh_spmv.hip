@@ -1658,16 +1658,14 @@ extern "C" int qh_rk3bs_err(const qh_vec *x, const qh_vec *xn, const qh_vec *k1,
                             const qh_vec *k3, const qh_vec *k4, double h, double atol, double rtol, double *err) {
   QH_REQUIRE(x && xn && k1 && k2 && k3 && k4 && err && x->prec == QH_FP64, QH_EINVAL, "qh_rk3bs_err: bad args");
   const int nb = 512;
-  double *partial = nullptr;
-  int rc = qh_alloc_t(&partial, nb);
+  double *partial = nullptr, *hp = nullptr;
+  int rc = qh_scratch(sizeof(double) * nb, (void **)&partial, sizeof(double) * nb, (void **)&hp);
   if (rc) return rc;
   hipLaunchKernelGGL(k_rk3bs_err, dim3(nb), dim3(256), 0, qh_stream(), (const double2 *)x->d,
                      (const double2 *)xn->d, (const double2 *)k1->d, (const double2 *)k2->d,
                      (const double2 *)k3->d, (const double2 *)k4->d, h, atol, rtol, x->n, partial);
-  double hp[nb];
-  QH_HIP(hipMemcpyAsync(hp, partial, sizeof(hp), hipMemcpyDeviceToHost, qh_stream()));
+  QH_HIP(hipMemcpyAsync(hp, partial, sizeof(double) * nb, hipMemcpyDeviceToHost, qh_stream()));
   QH_HIP(hipStreamSynchronize(qh_stream()));
-  qh_free_ptr(partial);
   double sum = 0.0;
   for (int i = 0; i < nb; i++) sum += hp[i];
   *err = sqrt(sum / (double)x->n);
