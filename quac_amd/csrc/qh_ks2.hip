@@ -366,18 +366,20 @@ __global__ void __launch_bounds__(256) k_ks2_stage(int32_t n, int32_t nslices, c
     return;
   }
   const float4 xo = ld_pair(ks_rsrc(x, vb), r8, two);
-  const __amdgpu_buffer_rsrc_t lr = ks_rsrc(xlo, vb);
-  const float4 lo = ld_pair(lr, r8, two);
   if (MODE == MODE_STAGE) {
+    // the next stage input is stored in complex64: the low word of the state
+    // (below half an ulp of the high word) would change it by at most one
+    // rounding, so only the high word is read here (the update in stage 4
+    // uses both)
     st_pair(ks_rsrc(kout, vb), r8, two, y);
     const double c = (double)cn;
-    const float4 yn = make_float4((float)((double)xo.x + (double)lo.x + c * acc.r0),
-                                  (float)((double)xo.y + (double)lo.y + c * acc.i0),
-                                  (float)((double)xo.z + (double)lo.z + c * acc.r1),
-                                  (float)((double)xo.w + (double)lo.w + c * acc.i1));
+    const float4 yn = make_float4((float)((double)xo.x + c * acc.r0), (float)((double)xo.y + c * acc.i0),
+                                  (float)((double)xo.z + c * acc.r1), (float)((double)xo.w + c * acc.i1));
     st_pair(ks_rsrc(ynext, vb), r8, two, yn);
     return;
   }
+  const __amdgpu_buffer_rsrc_t lr = ks_rsrc(xlo, vb);
+  const float4 lo = ld_pair(lr, r8, two);
   const float4 a = ld_pair(ks_rsrc(k1, vb), r8, two), bq = ld_pair(ks_rsrc(k2, vb), r8, two),
                cq = ld_pair(ks_rsrc(k3, vb), r8, two);
   const double s6 = (double)h * (1.0 / 6.0), s3 = (double)h * (1.0 / 3.0);

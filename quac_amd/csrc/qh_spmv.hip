@@ -499,9 +499,10 @@ __global__ void __launch_bounds__(256) k_ks_stage(
       yn.x = xo.x + cn * y.x;
       yn.y = xo.y + cn * y.y;
     } else {
-      const float2 lo = xlo[r];
-      yn.x = (R)((double)xo.x + (double)lo.x + (double)cn * are);
-      yn.y = (R)((double)xo.y + (double)lo.y + (double)cn * aim);
+      // the complex64 stage input does not need the low word of the state (it
+      // would change it by at most one rounding); stage 4 uses both words
+      yn.x = (R)((double)xo.x + (double)cn * are);
+      yn.y = (R)((double)xo.y + (double)cn * aim);
     }
     ynext[r] = yn;
     return;
