@@ -67,11 +67,11 @@ struct qh_kmeta32 {
 // rows = one wavefront, lane l owns rows 2l and 2l+1 of the slice).  A slot is one
 // 16-byte load per lane of the complex64 pair x[b], x[b+1] with b = 2l +- ad
 // (sign as above); row 2l takes half h0 of the pair and row 2l+1 half h1, and the
-// slot's lanes share (ad, h0, h1) and which of the two rows have an entry there.
-// ad8 holds 8 ad | h0 | h1 << 1.  Uniform slots keep the value of each of the two
-// rows in (v0re, v0im), (v1re, v1im) (zero for a row without the entry); scalar
-// slots store two reals per active lane (value = real * (v0re, v0im)); complex
-// slots store two complex64 per active lane.
+// slot's lanes share (ad, h0, h1); a row of a lane without the entry has value
+// zero.  ad8 holds 8 ad | h0 | h1 << 1.  Uniform slots (both rows' values the same
+// across the active lanes) keep them in (v0re, v0im), (v1re, v1im); scalar slots
+// store two reals per active lane (value = real * (v0re, v0im)); complex slots
+// store two complex64 per active lane.
 struct qh_kmeta2 {
   uint64_t mask, sign;
   uint32_t ad8, base;

@@ -10,8 +10,9 @@
 //
 // Pairing: for each entry of row 2l at column c the builder looks for an entry
 // of row 2l+1 at c+1 (straight, b = c), c-1 (crossed, b = c-1) or c (same);
-// unmatched entries load alone (one row of the lane inactive).  A slot collects
-// the lanes whose loads share (|b - 2l|, h0, h1, which rows are active, rank);
+// unmatched entries load alone (one row of the lane inactive, zero value), with
+// the in-order halves where the vector allows it.  A slot collects the lanes
+// whose loads share (|b - 2l|, h0, h1, rank);
 // the rest of the machinery (sign masks, uniform / scalar / complex slot groups,
 // compacted values, exact tails, two passes in flight) is that of the
 // one-row-per-lane layout.  Products are summed in FP32 within a pass and the
@@ -61,7 +62,9 @@ __global__ void __launch_bounds__(64) k_ks2_build(int32_t n, const int32_t *__re
   auto push = [&](int64_t b, int h0, int h1, int e0, int e1) {
     const int64_t o = b - r0;
     const uint64_t ad = (uint64_t)(o < 0 ? -o : o);
-    const uint64_t pat = (uint64_t)((e0 != 63) | ((e1 != 63) << 1) | (h0 << 2) | (h1 << 3));
+    // which rows are active is not part of the key: a row without the entry gets
+    // a zero value, so single-row loads share the slots of the pairs beside them
+    const uint64_t pat = (uint64_t)((h0 << 2) | (h1 << 3));
     // negative offsets first so that the rank below separates b = r0 - ad from r0 + ad
     const uint64_t key = (((ad << 4) | pat) << 1) | (o > 0 ? 1u : 0u);
     const uint64_t v = (key << 12) | ((uint64_t)e0 << 6) | (uint64_t)e1;
@@ -90,17 +93,22 @@ __global__ void __launch_bounds__(64) k_ks2_build(int32_t n, const int32_t *__re
       push(c, 0, 1, e0, f);  // straight: x[c], x[c+1]
     } else if (f >= 0 && how == 1) {
       push(c - 1, 1, 0, e0, f);  // crossed: row 2l+1 reads x[c-1]
-    } else {
-      // same column for both rows, or row 2l alone: keep the 16-byte load inside the vector
+    } else if (f >= 0) {
+      // same column for both rows: keep the 16-byte load inside the vector
       const bool last = c + 1 >= n;
-      push(last ? c - 1 : c, last ? 1 : 0, (f >= 0 && last) ? 1 : 0, e0, f >= 0 ? f : 63);
+      push(last ? c - 1 : c, last ? 1 : 0, last ? 1 : 0, e0, f);
+    } else {
+      // row 2l alone: in-order halves (row 2l on half 0) unless at the end of the vector
+      const bool last = c + 1 >= n;
+      push(last ? c - 1 : c, last ? 1 : 0, last ? 0 : 1, e0, 63);
     }
   }
   for (int e1 = 0; e1 < len1; e1++) {
     if ((used1 >> e1) & 1u) continue;
+    // row 2l+1 alone: in-order halves (row 2l+1 on half 1) unless at column 0
     const int64_t c = col[beg1 + e1];
-    const bool last = c + 1 >= n;
-    push(last ? c - 1 : c, 0, last ? 1 : 0, 63, e1);
+    const bool first = c == 0;
+    push(first ? 0 : c - 1, first ? 1 : 0, first ? 0 : 1, 63, e1);
   }
   // rank: the second load of a lane with the same (ad, pattern) gets rank 1
   uint64_t prev = ~0ull;
@@ -140,7 +148,7 @@ __global__ void __launch_bounds__(64) k_ks2_build(int32_t n, const int32_t *__re
     const bool sca = isre || isim;
     // in-order halves: an active row 2l takes half 0, an active row 2l+1 half 1
     const uint32_t pat_ = (uint32_t)((m >> 1) & 15u);
-    const bool inorder = (!(pat_ & 1u) || !(pat_ & 4u)) && (!(pat_ & 2u) || (pat_ & 8u));
+    const bool inorder = !(pat_ & 4u) && (pat_ & 8u);  // h0 = 0, h1 = 1
     if (FILL) {
       // slot key m = (ad << 4 | pattern) << 1 | rank
       const uint32_t pat = (uint32_t)((m >> 1) & 15u);
