@@ -162,6 +162,98 @@ def test_row_pair_layout_fp32_spmv(name):
         k.destroy()
 
 
+def _pair_slot_classes(rp, col, val):
+    """The row-pair layout restated in Python: per lane (rows 2l, 2l+1) each entry
+    of row 2l at column c pairs with an unused entry of row 2l+1 at c+1 (load
+    x[c], x[c+1], halves (0, 1)), else c-1 (load from c-1, halves (1, 0)), else c
+    (halves (0, 0), or (1, 1) from c-1 at the last column); an unmatched entry
+    loads alone with the in-order halves (row 2l from c, row 2l+1 from c-1) unless
+    that leaves the vector.  Slots of a 128-row slice are the distinct
+    (|b - 2l|, h0, h1, rank) over its lanes; a slot is uniform when every active
+    lane holds bit-identical (value row 2l, value row 2l+1) (zero for an absent
+    row), scalar when all those are real or all imaginary, else complex.
+    Returns (slots, complex pairs stored, real pairs stored)."""
+    n = len(rp) - 1
+    slots = ncpx = nreal = 0
+    z = np.complex128(0)
+    for s0 in range(0, n, 128):
+        per_slot = {}
+        for r0 in range(s0, min(n, s0 + 128), 2):
+            r1 = r0 + 1
+            e0 = list(range(rp[r0], rp[r0 + 1]))
+            e1 = list(range(rp[r1], rp[r1 + 1])) if r1 < n else []
+            used = set()
+            loads = []
+            for a in e0:
+                c = int(col[a])
+                f, how = None, None
+                for how_, want in ((0, c + 1), (1, c - 1), (2, c)):
+                    for b in e1:
+                        if b not in used and int(col[b]) == want:
+                            f, how = b, how_
+                            break
+                    if f is not None:
+                        break
+                if f is not None:
+                    used.add(f)
+                last = c + 1 >= n
+                if how == 0:
+                    loads.append((c, 0, 1, a, f))
+                elif how == 1:
+                    loads.append((c - 1, 1, 0, a, f))
+                elif how == 2:
+                    loads.append((c - 1, 1, 1, a, f) if last else (c, 0, 0, a, f))
+                else:
+                    loads.append((c - 1, 1, 0, a, None) if last else (c, 0, 1, a, None))
+            for b in e1:
+                if b in used:
+                    continue
+                c = int(col[b])
+                loads.append((0, 1, 0, None, b) if c == 0 else (c - 1, 0, 1, None, b))
+            seen = {}
+            for (bb, h0, h1, a, b) in sorted(loads, key=lambda t: (abs(t[0] - r0), t[1], t[2], t[0] - r0 > 0)):
+                k = (abs(bb - r0), h0, h1)
+                rank = seen.get(k, 0)
+                seen[k] = rank + 1
+                v = (val[a] if a is not None else z, val[b] if b is not None else z)
+                per_slot.setdefault(k + (rank,), []).append(v)
+        for vs in per_slot.values():
+            slots += 1
+            bits = {(v0.real.tobytes(), v0.imag.tobytes(), v1.real.tobytes(), v1.imag.tobytes()) for v0, v1 in vs}
+            if len(bits) == 1:
+                continue
+            if all(v0.imag == 0 and v1.imag == 0 for v0, v1 in vs) or all(v0.real == 0 and v1.real == 0 for v0, v1 in vs):
+                nreal += len(vs)
+            else:
+                ncpx += len(vs)
+    return slots, ncpx, nreal
+
+
+@pytest.mark.parametrize("name", ["odd", "tc3"])
+def test_row_pair_layout_counts(name):
+    """slot and stored-value counts of the device-built row-pair layout against the
+    Python restatement of its pairing and classification"""
+    keep = []
+    if name == "odd":
+        qs, _, _ = build_system(_odd_recipe())
+        A, keep = qs.matrix(), [qs]
+    else:
+        mats, tots = [], []
+        for p in recipes.tc_instances(3, 3):
+            q2, _, _ = build_system(recipes.tc_recipe(*p))
+            keep.append(q2)
+            mats.append(q2.matrix())
+            tots.append(recipes.tc_init(p[0], p[1])[0])
+        b = qa.Batch(mats, tots, prec=qa.FP32)
+        A = b.matrix()
+        keep.append(b)
+    rp, col, val = A.export()
+    lay = A.pair_layout_info()
+    assert (lay["slots"], lay["stored_pairs"], lay["stored_real_pairs"]) == _pair_slot_classes(rp, col, val)
+    for k in keep:
+        k.destroy()
+
+
 def test_row_pair_layout_rk4_odd_rows():
     """FP32 RK4 on the row-pair layout for a matrix with an odd number of rows
     (the last lane's second row absent): populations within 1e-6 of the oracle"""
