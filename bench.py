@@ -46,8 +46,9 @@ def kernel_name(variant, real):
                      133: (2, 0, 1), 134: (4, 0, 1)}[variant]
         tf = ("false", "true")
         return f"k_ks_stage<{real}, MODE, {u}, {tf[vv]}, {tf[pp]}> (sliced layout)"
-    if variant in (140, 141):
-        return f"k_ks2_stage<MODE, {2 if variant == 140 else 4}> (row-pair sliced layout)"
+    if variant in (140, 141, 142):
+        u, w = {140: (2, "false"), 141: (4, "false"), 142: (2, "true")}[variant]
+        return f"k_ks2_stage<MODE, {u}, {w}> (row-pair sliced layout)"
     return f"k_rk_stage<{real}, {variant}> (CSR, {variant} lanes per row)"
 
 
@@ -176,7 +177,7 @@ def main():
     # bytes the sliced-layout step actually has to move: 4 x (stored values + slot
     # metadata) + 17 vector passes (4 gathers, 3 K and 3 Y writes, 3 x reads for
     # the Y's, K1..K3 + x reads and the x' write in the last epilogue)
-    if lanes[1] in (140, 141):  # row-pair layout: 40-byte slot metadata, values stored per lane pair
+    if lanes[1] in (140, 141, 142):  # row-pair layout: 40-byte slot metadata, values stored per lane pair
         layout = dict(A.pair_layout_info(), kind="row-pair (128 rows per wavefront)")
         meta32 = layout["meta_bytes"]
         vals32 = layout["stored_pairs"] * 16 + layout["stored_real_pairs"] * 8

@@ -353,6 +353,7 @@ extern "C" int qh_mat_destroy(qh_mat *m) {
   qh_free_ptr(m->ks_umid);
   qh_free_ptr(m->ks_cmid);
   qh_free_ptr(m->ks2_moff);
+  qh_free_ptr(m->ks2_wmid);
   qh_free_ptr(m->ks2_smid);
   qh_free_ptr(m->ks2_umid);
   qh_free_ptr(m->ks2_cmid);

@@ -109,7 +109,8 @@ struct qh_mat {
   int ks2_state = 0;           // 0 not built, 1 built, -1 not applicable
   int32_t ks2_nslices = 0;
   int32_t *ks2_moff = nullptr;  // nslices+1 slot offsets
-  int32_t *ks2_smid = nullptr;  // end of each slice's uniform slots with in-order halves (they come first)
+  int32_t *ks2_wmid = nullptr;  // end of each slice's in-order uniform slots inside its 1024-row window (first)
+  int32_t *ks2_smid = nullptr;  // end of each slice's uniform slots with in-order halves
   int32_t *ks2_umid = nullptr;  // end of each slice's uniform slots
   int32_t *ks2_cmid = nullptr;  // end of each slice's scalar slots
   qh_kmeta2 *ks2_meta = nullptr;
@@ -123,8 +124,9 @@ struct qh_mat {
 // loads of the next pass in flight during the products of the current one
 #define QH_KSELL 128
 #define QH_IS_KSELL(g) ((g) >= 128 && (g) <= 134)
-// 140 / 141: the FP32 row-pair sliced layout, 2 / 4 slots per pass (pipelined)
-#define QH_IS_KS2(g) ((g) == 140 || (g) == 141)
+// 140 / 141: the FP32 row-pair sliced layout, 2 / 4 slots per pass (pipelined);
+// 142: 2 slots per pass, the slots inside each 1024-row window read from LDS
+#define QH_IS_KS2(g) ((g) >= 140 && (g) <= 142)
 
 struct qh_vec {
   int64_t n = 0;

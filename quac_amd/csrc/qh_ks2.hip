@@ -25,14 +25,20 @@
 namespace {
 
 const int KS2_MAXROW = 32;  // longest row the pair builder takes (the CSR kernels serve longer rows)
+// rows of the LDS window of the windowed kernel: 8 slices, one workgroup of 8
+// wavefronts (for a density matrix of dimension 1024 exactly one column of rho,
+// which holds every left-multiplication entry of its rows)
+const int KS2_WIN = 1024;
 
 // ------------------------------------------------------------------ build
 // Per-slice counts (FILL false): cnt[0*ns + s] slots, [1] complex pairs, [2]
 // uniform slots, [3] scalar slots, [4] scalar pairs, [5] uniform slots whose
 // rows take the halves in order (row 2l half 0, row 2l+1 half 1: no select in
-// the kernel).  FILL writes the metadata (uniform in-order slots at moff, the
-// other uniform slots after them, then the scalar and complex groups) and the
-// values (complex pairs at voff, scalar pairs at svoff).
+// the kernel), [6] those of them whose loads all stay inside the slice's
+// KS2_WIN-row window.  FILL writes the metadata (uniform in-order window slots
+// at moff, the other uniform in-order slots, the other uniform slots, then the
+// scalar and complex groups) and the values (complex pairs at voff, scalar pairs
+// at svoff).
 template <bool FILL>
 __global__ void __launch_bounds__(64) k_ks2_build(int32_t n, const int32_t *__restrict__ rp,
                                                   const int32_t *__restrict__ col, const double2 *__restrict__ val,
@@ -119,13 +125,15 @@ __global__ void __launch_bounds__(64) k_ks2_build(int32_t n, const int32_t *__re
     prev = adp;
   }
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  int p = 0, slot = 0, uslot = 0, sslot = 0, cslot = 0, ustr = 0;
+  int p = 0, slot = 0, uslot = 0, sslot = 0, cslot = 0, ustr = 0, uwin = 0;
+  const int64_t w0 = ((int64_t)s * 128 / KS2_WIN) * KS2_WIN;  // first row of the slice's window
   int32_t base = FILL ? voff[s] : 0;
   int32_t sbase = FILL ? svoff[s] : 0;
   const int32_t mo = FILL ? moff[s] : 0;
   const int32_t ms = FILL ? mo + cnt[2 * ns + s] : 0;
   const int32_t mc = FILL ? ms + cnt[3 * ns + s] : 0;
   const int32_t mu2 = FILL ? mo + cnt[5 * ns + s] : 0;  // uniform slots that need the half selects
+  const int32_t mw = FILL ? mo + cnt[6 * ns + s] : 0;   // in-order uniform slots outside the window
   for (;;) {
     const uint64_t cur = p < np ? (kk[p] >> 12) : ~0ull;
     const uint64_t m = wave_min_u64(cur);
@@ -149,13 +157,17 @@ __global__ void __launch_bounds__(64) k_ks2_build(int32_t n, const int32_t *__re
     // in-order halves: an active row 2l takes half 0, an active row 2l+1 half 1
     const uint32_t pat_ = (uint32_t)((m >> 1) & 15u);
     const bool inorder = !(pat_ & 4u) && (pat_ & 8u);  // h0 = 0, h1 = 1
+    // the pair starts at b = (column of row 2l) - h0, or (column of row 2l+1) - h1
+    const int64_t bcol = !act ? r0
+                         : e0 != 63 ? (int64_t)col[beg0 + e0] - ((pat_ >> 2) & 1u)
+                                    : (int64_t)col[beg1 + e1] - ((pat_ >> 3) & 1u);
+    const bool inwin = __ballot(act && (bcol < w0 || bcol + 1 >= w0 + KS2_WIN)) == 0ull;
+    const bool win = uni && inorder && inwin;
     if (FILL) {
       // slot key m = (ad << 4 | pattern) << 1 | rank
       const uint32_t pat = (uint32_t)((m >> 1) & 15u);
       const uint32_t h0 = (pat >> 2) & 1u, h1 = (pat >> 3) & 1u;
       const uint64_t ad = m >> 5;
-      // the pair starts at b = (column of row 2l) - h0, or (column of row 2l+1) - h1
-      const int64_t bcol = !act ? r0 : e0 != 63 ? (int64_t)col[beg0 + e0] - h0 : (int64_t)col[beg1 + e1] - h1;
       const uint64_t sign = __ballot(act && bcol < r0);
       const uint32_t rank = (uint32_t)__popcll(mask & lt);
       if (lane == 0) {
@@ -173,7 +185,8 @@ __global__ void __launch_bounds__(64) k_ks2_build(int32_t n, const int32_t *__re
           md.v0re = md.v1re = sca ? (isre ? 1.f : 0.f) : 0.f;
           md.v0im = md.v1im = sca ? (isre ? 0.f : 1.f) : 0.f;
         }
-        meta[uni ? (inorder ? mo + ustr : mu2 + (uslot - ustr)) : sca ? ms + sslot : mc + cslot] = md;
+        meta[uni ? (win ? mo + uwin : inorder ? mw + (ustr - uwin) : mu2 + (uslot - ustr))
+                 : sca ? ms + sslot : mc + cslot] = md;
       }
       if (act && sca)
         sval[sbase + rank] = isre ? make_float2((float)v0.x, (float)v1.x) : make_float2((float)v0.y, (float)v1.y);
@@ -189,6 +202,7 @@ __global__ void __launch_bounds__(64) k_ks2_build(int32_t n, const int32_t *__re
     if (uni) {
       uslot++;
       if (inorder) ustr++;
+      if (win) uwin++;
     } else if (sca) {
       sslot++;
     } else {
@@ -202,6 +216,7 @@ __global__ void __launch_bounds__(64) k_ks2_build(int32_t n, const int32_t *__re
     cnt[3 * ns + s] = sslot;
     cnt[4 * ns + s] = sbase;
     cnt[5 * ns + s] = ustr;
+    cnt[6 * ns + s] = uwin;
   }
 }
 
@@ -320,6 +335,49 @@ struct Ks2Ops {
   }
 };
 
+// In-order uniform slots whose loads all stay inside the workgroup's window of
+// KS2_WIN rows: the pair is read from the LDS copy of the window (xs, KS2_WIN + 2
+// complex64, the last two zero for inactive lanes) instead of global memory.
+struct Ks2OpsWin {
+  typedef KsAcc2 Acc;
+  template <int W>
+  using Buf = Ks2Buf<KS_UNI, W>;
+  const qh_kmeta2 *mp;
+  uint32_t w8;  // byte offset of row 2l within the window
+  const float2 *xs;
+  template <int W>
+  __device__ __forceinline__ void issue(Buf<W> &b) {
+    qh_kmeta2 m[W];
+#pragma unroll
+    for (int j = 0; j < W; j++) {
+      m[j] = mp[j];
+      b.h[j] = 0u;
+      b.u0r[j] = m[j].v0re;
+      b.u0i[j] = m[j].v0im;
+      b.u1r[j] = m[j].v1re;
+      b.u1i[j] = m[j].v1im;
+    }
+#pragma unroll
+    for (int j = 0; j < W; j++) {
+      const bool on = __builtin_amdgcn_inverse_ballot_w64(m[j].mask);
+      const bool neg = __builtin_amdgcn_inverse_ballot_w64(m[j].sign);
+      const uint32_t ad8 = m[j].ad8 & ~7u;
+      const uint32_t back = neg ? (0u - 2u * ad8) : 0u;
+      uint32_t off = w8 + ad8 + back;
+      off = on ? off : (uint32_t)(KS2_WIN * 8);
+      const float2 *q = (const float2 *)((const char *)xs + off);
+      const float2 lo = q[0], hi = q[1];
+      b.xv[j] = make_float4(lo.x, lo.y, hi.x, hi.y);
+    }
+    mp += W;
+  }
+  template <int W>
+  __device__ __forceinline__ void fma(const Buf<W> &b, Acc &acc) {
+    Ks2Ops<KS_UNI, true>{}.fma(b, acc);
+  }
+  __device__ __forceinline__ void flush(Acc &acc) { Ks2Ops<KS_UNI, true>{}.flush(acc); }
+};
+
 // the two complex64 of rows (r, r+1) at byte offset o of a vector; the second is
 // zero (and not loaded / stored) when r is the last row
 __device__ inline float4 ld_pair(__amdgpu_buffer_rsrc_t rs, uint32_t o, bool two) {
@@ -340,10 +398,12 @@ __device__ inline void st_pair(__amdgpu_buffer_rsrc_t rs, uint32_t o, bool two, 
 //   MODE_STAGE: kout = A xin;  ynext = x + cn * kout          (stages 1-3)
 //   MODE_FINAL: xout = x + h/6 (K1 + 2 K2 + 2 K3 + A xin)    (stage 4)
 // with the FP32 state x = hi + lo updated in FP64 and split again.
-template <int MODE, int U>
-__global__ void __launch_bounds__(256) k_ks2_stage(int32_t n, int32_t nslices, const int32_t *__restrict__ moff,
-                                                   const int32_t *__restrict__ smid, const int32_t *__restrict__ umid,
-                                                   const int32_t *__restrict__ cmid,
+// WIN: workgroups of KS2_WIN / 128 wavefronts; each first copies its window of
+// the stage input into LDS and reads the window slots from there.
+template <int MODE, int U, bool WIN>
+__global__ void __launch_bounds__(WIN ? 8 * 64 : 256) k_ks2_stage(
+    int32_t n, int32_t nslices, const int32_t *__restrict__ moff, const int32_t *__restrict__ wmid,
+    const int32_t *__restrict__ smid, const int32_t *__restrict__ umid, const int32_t *__restrict__ cmid,
                                                    const qh_kmeta2 *__restrict__ meta, int32_t nval,
                                                    const float4 *__restrict__ val, int32_t nsval,
                                                    const float2 *__restrict__ sval, const float2 *xin, float2 *kout,
@@ -351,18 +411,37 @@ __global__ void __launch_bounds__(256) k_ks2_stage(int32_t n, int32_t nslices, c
                                                    const float2 *k2, const float2 *k3, float2 *xout, float h,
                                                    float2 *xlo) {
   const unsigned b = qh_xcd_block(blockIdx.x, gridDim.x);
-  const int s = (int)(b * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6));
-  if (s >= nslices) return;  // whole wavefront
-  const int lane = threadIdx.x & 63;
-  const int32_t r = s * 128 + 2 * lane;
-  const int32_t k0 = moff[s], ku = smid[s], km = umid[s], kc = cmid[s], kend = moff[s + 1];
+  const int s = (int)(b * (WIN ? KS2_WIN / 128 : 4) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6));
   const int64_t vb = (int64_t)n * 8;
   const __amdgpu_buffer_rsrc_t xr = ks_rsrc(xin, vb);
+  __shared__ float2 xs[WIN ? KS2_WIN + 2 : 2];
+  if (WIN) {
+    // the window rows [b KS2_WIN, (b+1) KS2_WIN), two per thread (zeros past n)
+    const int64_t w = (int64_t)b * KS2_WIN + 2 * threadIdx.x;
+    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (w + 1 < n)
+      v = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(xr, (uint32_t)w * 8u, 0, 0));
+    else if (w < n)
+      v = ld_pair(xr, (uint32_t)w * 8u, false);
+    xs[2 * threadIdx.x] = make_float2(v.x, v.y);
+    xs[2 * threadIdx.x + 1] = make_float2(v.z, v.w);
+    if (threadIdx.x == 0) xs[KS2_WIN] = xs[KS2_WIN + 1] = make_float2(0.f, 0.f);
+    __syncthreads();
+  }
+  if (s >= nslices) return;  // whole wavefront (after the window copy)
+  const int lane = threadIdx.x & 63;
+  const int32_t r = s * 128 + 2 * lane;
+  const int32_t k0 = moff[s], kw = wmid[s], ku = smid[s], km = umid[s], kc = cmid[s], kend = moff[s + 1];
   const __amdgpu_buffer_rsrc_t vr = ks_rsrc(val, (int64_t)nval * 16);
   const __amdgpu_buffer_rsrc_t sr = ks_rsrc(sval, (int64_t)nsval * 8);
   const uint32_t r8 = (uint32_t)r * 8u;
   KsAcc2 acc{0.0, 0.0, 0.0, 0.0, 0.f, 0.f, 0.f, 0.f};
-  ks_run<U, true>(Ks2Ops<KS_UNI, true>{meta + k0, r8, xr, vr}, ku - k0, acc);
+  if (WIN) {
+    ks_run<U, true>(Ks2OpsWin{meta + k0, (uint32_t)(r - (int32_t)b * KS2_WIN) * 8u, xs}, kw - k0, acc);
+    ks_run<U, true>(Ks2Ops<KS_UNI, true>{meta + kw, r8, xr, vr}, ku - kw, acc);
+  } else {
+    ks_run<U, true>(Ks2Ops<KS_UNI, true>{meta + k0, r8, xr, vr}, ku - k0, acc);
+  }
   ks_run<U, true>(Ks2Ops<KS_UNI>{meta + ku, r8, xr, vr}, km - ku, acc);
   ks_run<U, true>(Ks2Ops<KS_SCA>{meta + km, r8, xr, sr}, kc - km, acc);
   ks_run<U, true>(Ks2Ops<KS_CPX>{meta + kc, r8, xr, vr}, kend - kc, acc);
@@ -417,7 +496,7 @@ int ks2_build(qh_mat *A) {
   const int32_t ns = (int32_t)((A->rows + 127) / 128);
   int32_t *cnt = nullptr, *voff = nullptr, *svoff = nullptr;
   int *ovf = nullptr;
-  int rc = qh_alloc_t(&cnt, (size_t)6 * ns);
+  int rc = qh_alloc_t(&cnt, (size_t)7 * ns);
   if (!rc) rc = qh_alloc_t(&voff, (size_t)ns + 1);
   if (!rc) rc = qh_alloc_t(&svoff, (size_t)ns + 1);
   if (!rc) rc = qh_alloc_t(&ovf, 1);
@@ -436,17 +515,19 @@ int ks2_build(qh_mat *A) {
                      cnt, ns, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, ovf);
   QH_HIP(hipGetLastError());
   int hovf = 0;
-  std::vector<int32_t> hc((size_t)6 * ns);
+  std::vector<int32_t> hc((size_t)7 * ns);
   QH_HIP(hipMemcpyAsync(&hovf, ovf, sizeof(int), hipMemcpyDeviceToHost, s));
-  QH_HIP(hipMemcpyAsync(hc.data(), cnt, sizeof(int32_t) * 6 * ns, hipMemcpyDeviceToHost, s));
+  QH_HIP(hipMemcpyAsync(hc.data(), cnt, sizeof(int32_t) * 7 * ns, hipMemcpyDeviceToHost, s));
   QH_HIP(hipStreamSynchronize(s));
-  const int32_t *hw = hc.data(), *hn = hw + ns, *hu = hn + ns, *hs = hu + ns, *hsv = hs + ns, *hst = hsv + ns;
-  std::vector<int32_t> ho(ns + 1), hv(ns + 1), hso(ns + 1), hm(ns), hcm(ns), hsm(ns);
+  const int32_t *hw = hc.data(), *hn = hw + ns, *hu = hn + ns, *hs = hu + ns, *hsv = hs + ns, *hst = hsv + ns,
+                *hwn = hst + ns;
+  std::vector<int32_t> ho(ns + 1), hv(ns + 1), hso(ns + 1), hm(ns), hcm(ns), hsm(ns), hwm(ns);
   int64_t tot = 0, totv = 0, tots = 0;
   for (int32_t i = 0; i < ns; i++) {
     ho[i] = (int32_t)tot;
     hv[i] = (int32_t)totv;
     hso[i] = (int32_t)tots;
+    hwm[i] = (int32_t)(tot + hwn[i]);
     hsm[i] = (int32_t)(tot + hst[i]);
     hm[i] = (int32_t)(tot + hu[i]);
     hcm[i] = (int32_t)(tot + hu[i] + hs[i]);
@@ -463,6 +544,7 @@ int ks2_build(qh_mat *A) {
   hv[ns] = (int32_t)totv;
   hso[ns] = (int32_t)tots;
   rc = qh_alloc_t(&A->ks2_moff, (size_t)ns + 1);
+  if (!rc) rc = qh_alloc_t(&A->ks2_wmid, (size_t)ns);
   if (!rc) rc = qh_alloc_t(&A->ks2_smid, (size_t)ns);
   if (!rc) rc = qh_alloc_t(&A->ks2_umid, (size_t)ns);
   if (!rc) rc = qh_alloc_t(&A->ks2_cmid, (size_t)ns);
@@ -474,6 +556,7 @@ int ks2_build(qh_mat *A) {
     return rc;
   }
   QH_HIP(hipMemcpyAsync(A->ks2_moff, ho.data(), sizeof(int32_t) * (ns + 1), hipMemcpyHostToDevice, s));
+  QH_HIP(hipMemcpyAsync(A->ks2_wmid, hwm.data(), sizeof(int32_t) * ns, hipMemcpyHostToDevice, s));
   QH_HIP(hipMemcpyAsync(A->ks2_smid, hsm.data(), sizeof(int32_t) * ns, hipMemcpyHostToDevice, s));
   QH_HIP(hipMemcpyAsync(A->ks2_umid, hm.data(), sizeof(int32_t) * ns, hipMemcpyHostToDevice, s));
   QH_HIP(hipMemcpyAsync(A->ks2_cmid, hcm.data(), sizeof(int32_t) * ns, hipMemcpyHostToDevice, s));
@@ -492,7 +575,8 @@ int ks2_build(qh_mat *A) {
   return QH_OK;
 }
 
-// One stage launch on the row-pair layout (G = 140 / 141: 2 / 4 slots per pass);
+// One stage launch on the row-pair layout (G = 140 / 141: 2 / 4 slots per pass;
+// 142: 2 slots per pass, window slots read from LDS);
 // mode as in the one-row-per-lane kernels.
 int ks2_launch(int G, int mode, hipStream_t s, const qh_mat *A, const void *xin, void *kout, void *ynext, float cn,
                const void *x, const void *k1, const void *k2, const void *k3, void *xout, float h, float2 *xlo) {
@@ -500,25 +584,25 @@ int ks2_launch(int G, int mode, hipStream_t s, const qh_mat *A, const void *xin,
     qh_set_error("row-pair sliced layout not built for this matrix");
     return QH_EINVAL;
   }
-  const dim3 grid(qh_blocks(A->ks2_nslices, 4)), blk(256);
   const int32_t n = (int32_t)A->rows, ns = A->ks2_nslices;
-#define QH_LK2(MM, UU)                                                                                          \
-  hipLaunchKernelGGL((k_ks2_stage<MM, UU>), grid, blk, 0, s, n, ns, A->ks2_moff, A->ks2_smid, A->ks2_umid,      \
-                     A->ks2_cmid,                                                                               \
+  const bool win = G == 142;
+  const dim3 grid(qh_blocks(ns, win ? KS2_WIN / 128 : 4)), blk(win ? KS2_WIN / 2 : 256);
+#define QH_LK2(MM, UU, WW)                                                                                       \
+  hipLaunchKernelGGL((k_ks2_stage<MM, UU, WW>), grid, blk, 0, s, n, ns, A->ks2_moff, A->ks2_wmid, A->ks2_smid,  \
+                     A->ks2_umid, A->ks2_cmid,                                                                  \
                      A->ks2_meta, (int32_t)A->ks2_nval, A->ks2_val, (int32_t)A->ks2_nsval, A->ks2_sval,          \
                      (const float2 *)xin, (float2 *)kout, (float2 *)ynext, cn, (const float2 *)x,                \
                      (const float2 *)k1, (const float2 *)k2, (const float2 *)k3, (float2 *)xout, h, xlo)
-  const int u = G == 140 ? 2 : G == 141 ? 4 : 0;
-  if (u == 0 || mode < MODE_SPMV || mode > MODE_FINAL) {
+  if (!QH_IS_KS2(G) || mode < MODE_SPMV || mode > MODE_FINAL) {
     qh_set_error("unsupported row-pair variant %d / mode %d", G, mode);
     return QH_EINVAL;
   }
   if (mode == MODE_SPMV) {
-    if (u == 2) QH_LK2(MODE_SPMV, 2); else QH_LK2(MODE_SPMV, 4);
+    if (G == 140) QH_LK2(MODE_SPMV, 2, false); else if (G == 141) QH_LK2(MODE_SPMV, 4, false); else QH_LK2(MODE_SPMV, 2, true);
   } else if (mode == MODE_STAGE) {
-    if (u == 2) QH_LK2(MODE_STAGE, 2); else QH_LK2(MODE_STAGE, 4);
+    if (G == 140) QH_LK2(MODE_STAGE, 2, false); else if (G == 141) QH_LK2(MODE_STAGE, 4, false); else QH_LK2(MODE_STAGE, 2, true);
   } else {
-    if (u == 2) QH_LK2(MODE_FINAL, 2); else QH_LK2(MODE_FINAL, 4);
+    if (G == 140) QH_LK2(MODE_FINAL, 2, false); else if (G == 141) QH_LK2(MODE_FINAL, 4, false); else QH_LK2(MODE_FINAL, 2, true);
   }
 #undef QH_LK2
   QH_HIP(hipGetLastError());
