@@ -70,7 +70,7 @@ int qh_mat_blockdiag(int n, qh_mat *const *mats, qh_mat **out);
  * pass, 131/132 with vector-loaded uniform values, 133/134 with 2/4 slots per pass
  * and the next pass's loads in flight); FP32 only: 140/141/142 = row-pair sliced
  * layout (one wavefront per 128 rows, two rows per lane, 16-byte loads; 2/4/2
- * slots per pass, 142 reading the slots that stay inside each 1024-row window
+ * slots per pass, 142 reading the slots that stay inside each 2048-row window
  * from an LDS copy); 0 = autotune */
 int qh_mat_set_lanes(qh_mat *m, int lanes64, int lanes32);
 int qh_mat_get_lanes(const qh_mat *m, int *lanes64, int *lanes32);

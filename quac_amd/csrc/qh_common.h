@@ -109,7 +109,7 @@ struct qh_mat {
   int ks2_state = 0;           // 0 not built, 1 built, -1 not applicable
   int32_t ks2_nslices = 0;
   int32_t *ks2_moff = nullptr;  // nslices+1 slot offsets
-  int32_t *ks2_wmid = nullptr;  // end of each slice's in-order uniform slots inside its 1024-row window (first)
+  int32_t *ks2_wmid = nullptr;  // end of each slice's in-order uniform slots inside its 2048-row window (first)
   int32_t *ks2_smid = nullptr;  // end of each slice's uniform slots with in-order halves
   int32_t *ks2_umid = nullptr;  // end of each slice's uniform slots
   int32_t *ks2_cmid = nullptr;  // end of each slice's scalar slots
@@ -125,7 +125,7 @@ struct qh_mat {
 #define QH_KSELL 128
 #define QH_IS_KSELL(g) ((g) >= 128 && (g) <= 134)
 // 140 / 141: the FP32 row-pair sliced layout, 2 / 4 slots per pass (pipelined);
-// 142: 2 slots per pass, the slots inside each 1024-row window read from LDS
+// 142: 2 slots per pass, the slots inside each 2048-row window read from LDS
 #define QH_IS_KS2(g) ((g) >= 140 && (g) <= 142)
 
 struct qh_vec {

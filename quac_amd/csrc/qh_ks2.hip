@@ -25,10 +25,11 @@
 namespace {
 
 const int KS2_MAXROW = 32;  // longest row the pair builder takes (the CSR kernels serve longer rows)
-// rows of the LDS window of the windowed kernel: 8 slices, one workgroup of 8
-// wavefronts (for a density matrix of dimension 1024 exactly one column of rho,
-// which holds every left-multiplication entry of its rows)
-const int KS2_WIN = 1024;
+// rows of the LDS window of the windowed kernel: 16 slices, one workgroup of 16
+// wavefronts (for a density matrix of dimension 1024, two columns of rho: every
+// left-multiplication entry of their rows, and the right-multiplication entries
+// that flip the lowest column bit)
+const int KS2_WIN = 2048;
 
 // ------------------------------------------------------------------ build
 // Per-slice counts (FILL false): cnt[0*ns + s] slots, [1] complex pairs, [2]
@@ -401,7 +402,7 @@ __device__ inline void st_pair(__amdgpu_buffer_rsrc_t rs, uint32_t o, bool two, 
 // WIN: workgroups of KS2_WIN / 128 wavefronts; each first copies its window of
 // the stage input into LDS and reads the window slots from there.
 template <int MODE, int U, bool WIN>
-__global__ void __launch_bounds__(WIN ? 8 * 64 : 256) k_ks2_stage(
+__global__ void __launch_bounds__(WIN ? KS2_WIN / 2 : 256) k_ks2_stage(
     int32_t n, int32_t nslices, const int32_t *__restrict__ moff, const int32_t *__restrict__ wmid,
     const int32_t *__restrict__ smid, const int32_t *__restrict__ umid, const int32_t *__restrict__ cmid,
                                                    const qh_kmeta2 *__restrict__ meta, int32_t nval,
