@@ -129,6 +129,14 @@ int qh_lincomb(qh_vec *y, int n, const double *c_re_im, qh_vec *const *v);      
 /* weighted RMS error of RK3BS: sqrt(mean(|e|^2/(atol+rtol max(|x|,|xn|))^2)) */
 int qh_rk3bs_err(const qh_vec *x, const qh_vec *xn, const qh_vec *k1, const qh_vec *k2,
                  const qh_vec *k3, const qh_vec *k4, double h, double atol, double rtol, double *err);
+/* One Bogacki-Shampine 3(2) trial step of dx/dt = A x (TSRK3BS tableau, FSAL:
+ * k1 = A x is taken as given when have_k1): writes k2..k4, the candidate xn and
+ * the weighted RMS error estimate (atol + rtol max(|x|, |xn|)) into *err; y is
+ * workspace.  The step of the default time_step_sys integrator
+ * (src/qsystem.c:1003-1005, PETSc TSRK3BS); on the sliced layout its stages are
+ * three fused launches. */
+int qh_rk3bs_step(qh_mat *A, const qh_vec *x, qh_vec *xn, qh_vec *k1, qh_vec *k2, qh_vec *k3, qh_vec *k4,
+                  qh_vec *y, double h, double atol, double rtol, int have_k1, double *err);
 
 /* ------------------------------------------------------------- observables */
 /* Tr(ops rho) (qvec_utilities.c:1197-1281), FP64 accumulation on the device */

@@ -174,6 +174,7 @@ def _setup(L):
     sig("qh_vec_copy", i32, vp, vp)
     sig("qh_vec_size", i32, vp, P(i64), P(i32))
     sig("qh_spmv", i32, vp, vp, vp)
+    sig("qh_rk3bs_step", i32, vp, vp, vp, vp, vp, vp, vp, vp, dbl, dbl, dbl, i32, P(dbl))
     sig("qh_rk4", i32, vp, vp, dbl, i64, i64, i32)
     sig("qh_rk4_timed", i32, vp, vp, dbl, i64, i64, i32)
     sig("qh_last_stage_ms", i32, P(dbl), P(i64))

@@ -547,6 +547,8 @@ static int rk3bs_try(qsystem s, Vec x, PetscReal t, PetscReal h, work_t *w, int 
                      double *err) {
   int rc = 0;
   int td = s->num_time_dep > 0;
+  if (!td) /* time-independent RHS: the fused device step */
+    return qh_rk3bs_step(s->mat_A, x, w->xn, w->k[0], w->k[1], w->k[2], w->k[3], w->y, h, atol, rtol, have_k1, err);
   if (!have_k1) rc = td ? td_rhs(s, t, x, w->k[0]) : qh_spmv(s->mat_A, x, w->k[0]);
   if (!rc) rc = qh_axpy_to(w->y, x, 0.5 * h, 0.0, w->k[0]);
   if (!rc) rc = td ? td_rhs(s, t + 0.5 * h, w->y, w->k[1]) : qh_spmv(s->mat_A, w->y, w->k[1]);

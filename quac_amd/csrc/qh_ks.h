@@ -9,7 +9,10 @@
 
 namespace {
 
-enum { MODE_SPMV = 0, MODE_STAGE = 1, MODE_FINAL = 2 };
+// stage epilogues: SpMV only; RK4 stages 1-3; RK4 stage 4; Bogacki-Shampine
+// stage 3 (k3 and the 3rd-order candidate); Bogacki-Shampine error stage (k4 and
+// the weighted error norm)
+enum { MODE_SPMV = 0, MODE_STAGE = 1, MODE_FINAL = 2, MODE_BS3 = 3, MODE_BS3E = 4 };
 
 // Raw buffer resource over n elements (gfx9 descriptor word 3: 32-bit data
 // format).  A buffer load whose byte offset is at or beyond n * sizeof(T)

@@ -459,7 +459,7 @@ __global__ void __launch_bounds__(256) k_ks_stage(
     typename C2<R>::T *__restrict__ ynext, R cn, const typename C2<R>::T *__restrict__ x,
     const typename C2<R>::T *__restrict__ k1, const typename C2<R>::T *__restrict__ k2,
     const typename C2<R>::T *__restrict__ k3, typename C2<R>::T *__restrict__ xout, R h,
-    float2 *__restrict__ xlo) {
+    float2 *__restrict__ xlo, double atol, double rtol, double *__restrict__ eacc) {
   typedef typename C2<R>::T T;
   const unsigned b = qh_xcd_block(blockIdx.x, gridDim.x);
   const int s = (int)(b * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6));
@@ -483,12 +483,49 @@ __global__ void __launch_bounds__(256) k_ks_stage(
     ks_group<R, KS_CPX, U, VV, PIPE>(meta, kc, kend, r8, xr, vr, mr, acc);
   }
   const double are = acc.re, aim = acc.im;
+  if (MODE == MODE_BS3E) {
+    // Bogacki-Shampine error stage: k4 = A xn and the weighted squared error of
+    // h (b1 - b1^) k1 + ... + h (b4 - b4^) k4 with the weights atol + rtol max(|x|, |xn|)
+    // (as qh_rk3bs_err), summed over the wavefront into eacc[slice]
+    double v = 0.0;
+    if (r < nrows) {
+      const double c1 = 2.0 / 9.0 - 7.0 / 24.0, c2 = 1.0 / 3.0 - 1.0 / 4.0, c3 = 4.0 / 9.0 - 1.0 / 3.0,
+                   c4 = -1.0 / 8.0;
+      const T a = k1[r], bq = k2[r], cq = k3[r], xo = x[r], xn = xin[r];
+      const double hh = (double)h;
+      const double er = hh * (c1 * a.x + c2 * bq.x + c3 * cq.x + c4 * are);
+      const double ei = hh * (c1 * a.y + c2 * bq.y + c3 * cq.y + c4 * aim);
+      const double ax = sqrt((double)xo.x * xo.x + (double)xo.y * xo.y);
+      const double an = sqrt((double)xn.x * xn.x + (double)xn.y * xn.y);
+      const double sc = atol + rtol * (ax > an ? ax : an);
+      v = (er * er + ei * ei) / (sc * sc);
+      T y;
+      y.x = (R)are;
+      y.y = (R)aim;
+      kout[r] = y;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if (lane == 0) eacc[s] = v;
+    return;
+  }
   if (r >= nrows) return;
   T y;
   y.x = (R)are;
   y.y = (R)aim;
   if (MODE == MODE_SPMV) {
     kout[r] = y;
+    return;
+  }
+  if (MODE == MODE_BS3) {
+    // Bogacki-Shampine stage 3: k3 = A Y3 and x_n = x + h (2/9 k1 + 1/3 k2 + 4/9 k3)
+    kout[r] = y;
+    const T xo = x[r], a = k1[r], bq = k2[r];
+    const double hh = (double)h;
+    T o;
+    o.x = (R)((double)xo.x + hh * (2.0 / 9.0) * a.x + hh / 3.0 * bq.x + hh * (4.0 / 9.0) * are);
+    o.y = (R)((double)xo.y + hh * (2.0 / 9.0) * a.y + hh / 3.0 * bq.y + hh * (4.0 / 9.0) * aim);
+    xout[r] = o;
     return;
   }
   const T xo = x[r];
@@ -799,7 +836,8 @@ int ks_make_fp32(qh_mat *A) {
 // launch the sliced-layout kernel with the slot unroll encoded in the variant code G
 template <typename R, int MODE>
 int launch_ks(int G, hipStream_t s, const qh_mat *A, const void *xin, void *kout, void *ynext, R cn, const void *x,
-              const void *k1, const void *k2, const void *k3, void *xout, R h, float2 *xlo) {
+              const void *k1, const void *k2, const void *k3, void *xout, R h, float2 *xlo, double atol = 0.0,
+              double rtol = 0.0, double *eacc = nullptr) {
   typedef typename C2<R>::T T;
   const T *kv = sizeof(R) == 8 ? (const T *)A->ks_val : (const T *)A->ks_val32;
   const R *sv = sizeof(R) == 8 ? (const R *)A->ks_sval : (const R *)A->ks_sval32;
@@ -810,7 +848,7 @@ int launch_ks(int G, hipStream_t s, const qh_mat *A, const void *xin, void *kout
                      A->ks_cmid,                                                                              \
                      A->ks_meta, A->ks_meta32, (int32_t)(A->ks_slots + QH_KS_PAD), (int32_t)A->ks_nval, kv,    \
                      (int32_t)A->ks_nsval, sv, (const T *)xin, (T *)kout, (T *)ynext, cn, (const T *)x,       \
-                     (const T *)k1, (const T *)k2, (const T *)k3, (T *)xout, h, xlo)
+                     (const T *)k1, (const T *)k2, (const T *)k3, (T *)xout, h, xlo, atol, rtol, eacc)
   switch (G) {
     case 128: QH_LK(2, false, false); break;
     case 129: QH_LK(4, false, false); break;
@@ -1606,6 +1644,81 @@ __global__ void __launch_bounds__(256) k_rk3bs_err(const double2 *x, const doubl
 }
 unsigned gridn(int64_t n) { return (unsigned)std::min<int64_t>(qh_blocks(n, 256), 8192); }
 }  // namespace
+
+namespace {
+// sum of n partial sums into out[0] (one block)
+__global__ void __launch_bounds__(1024) k_sum_partials(const double *p, int32_t n, double *out) {
+  __shared__ double sh[1024];
+  double a = 0.0;
+  for (int32_t i = threadIdx.x; i < n; i += 1024) a += p[i];
+  sh[threadIdx.x] = a;
+  __syncthreads();
+  for (int o = 512; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = sh[0];
+}
+}  // namespace
+
+// One Bogacki-Shampine 3(2) trial step (PETSc TSRK3BS tableau, FSAL): k1 = A x
+// unless have_k1, k2 = A(x + h/2 k1), k3 = A(x + 3h/4 k2), xn = x + h(2/9 k1 +
+// 1/3 k2 + 4/9 k3), k4 = A xn, and the weighted RMS error estimate.  On the
+// sliced layout the stages are three fused launches (the stage input Y3, the
+// candidate xn and the error sums are formed in the epilogues; y holds Y2, k4
+// holds Y3 until the last stage); on the CSR kernels the same sequence runs as
+// separate launches.
+extern "C" int qh_rk3bs_step(qh_mat *A, const qh_vec *x, qh_vec *xn, qh_vec *k1, qh_vec *k2, qh_vec *k3, qh_vec *k4,
+                             qh_vec *y, double h, double atol, double rtol, int have_k1, double *err) {
+  QH_REQUIRE(A && x && xn && k1 && k2 && k3 && k4 && y && err, QH_EINVAL, "qh_rk3bs_step: null argument");
+  const qh_vec *vs[7] = {x, xn, k1, k2, k3, k4, y};
+  for (const qh_vec *v : vs)
+    QH_REQUIRE(v->prec == QH_FP64 && v->n == A->rows, QH_EINVAL,
+               "qh_rk3bs_step: FP64 vectors of the matrix dimension required");
+  int G = 0;
+  int rc = lanes_for(A, QH_FP64, &G);
+  if (rc) return rc;
+  if (!QH_IS_KSELL(G)) {
+    if (!have_k1) rc = qh_spmv(A, x, k1);
+    if (!rc) rc = qh_axpy_to(y, x, 0.5 * h, 0.0, k1);
+    if (!rc) rc = qh_spmv(A, y, k2);
+    if (!rc) rc = qh_axpy_to(y, x, 0.75 * h, 0.0, k2);
+    if (!rc) rc = qh_spmv(A, y, k3);
+    if (!rc) {
+      const double c[8] = {1.0, 0.0, h * 2.0 / 9.0, 0.0, h / 3.0, 0.0, h * 4.0 / 9.0, 0.0};
+      qh_vec *v[4] = {const_cast<qh_vec *>(x), k1, k2, k3};
+      rc = qh_lincomb(xn, 4, c, v);
+    }
+    if (!rc) rc = qh_spmv(A, xn, k4);
+    if (!rc) rc = qh_rk3bs_err(x, xn, k1, k2, k3, k4, h, atol, rtol, err);
+    return rc;
+  }
+  hipStream_t s = qh_stream();
+  // per-slice error partial sums, then their total at dacc[nslices]
+  double *dacc = nullptr, *hacc = nullptr;
+  rc = qh_scratch(sizeof(double) * ((size_t)A->nslices + 1), (void **)&dacc, sizeof(double), (void **)&hacc);
+  if (rc) return rc;
+  if (!have_k1)
+    rc = launch_ks<double, MODE_SPMV>(G, s, A, x->d, k1->d, nullptr, 0.0, nullptr, nullptr, nullptr, nullptr,
+                                      nullptr, 0.0, nullptr);
+  if (!rc) rc = qh_axpy_to(y, x, 0.5 * h, 0.0, k1);
+  if (!rc)
+    rc = launch_ks<double, MODE_STAGE>(G, s, A, y->d, k2->d, k4->d, 0.75 * h, x->d, nullptr, nullptr, nullptr,
+                                       nullptr, h, nullptr);
+  if (!rc)
+    rc = launch_ks<double, MODE_BS3>(G, s, A, k4->d, k3->d, nullptr, 0.0, x->d, k1->d, k2->d, nullptr, xn->d, h,
+                                     nullptr);
+  if (rc) return rc;
+  rc = launch_ks<double, MODE_BS3E>(G, s, A, xn->d, k4->d, nullptr, 0.0, x->d, k1->d, k2->d, k3->d, xn->d, h,
+                                    nullptr, atol, rtol, dacc);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, s, dacc, (int32_t)A->nslices, dacc + A->nslices);
+  QH_HIP(hipGetLastError());
+  QH_HIP(hipMemcpyAsync(hacc, dacc + A->nslices, sizeof(double), hipMemcpyDeviceToHost, s));
+  QH_HIP(hipStreamSynchronize(s));
+  *err = sqrt(hacc[0] / (double)A->rows);
+  return QH_OK;
+}
 
 extern "C" int qh_axpy_to(qh_vec *y, const qh_vec *x, double cr, double ci, const qh_vec *k) {
   QH_REQUIRE(y && x && k && y->n == x->n && k->n == x->n && x->prec == QH_FP64 && y->prec == QH_FP64 &&

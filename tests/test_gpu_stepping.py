@@ -4,6 +4,7 @@ Tolerances (written here, not hidden): the FP64 drop-in path must match the
 oracle's FP64 RK4 to 1e-10 in every population (and 1e-10 max-abs on the
 whole density matrix for the small cases); the FP32 bulk path must keep every
 population within 1e-6 of the FP64 path."""
+import ctypes
 import math
 
 import numpy as np
@@ -342,6 +343,52 @@ def test_sliced_ell_spmv_matches_csr(name):
     assert (lay["slots"], lay["stored_vals"], lay["stored_reals"]) == (slots, ncpx, nreal)
     if name == "tc_batch":
         assert nreal > ncpx > 0     # the sweep's superoperators store mostly real-or-imaginary slots
+    for k in keep:
+        k.destroy()
+
+
+@pytest.mark.parametrize("name", ["chain6", "tc_batch"])
+def test_rk3bs_step_fused_matches_separate_launches(name):
+    """qh_rk3bs_step on the sliced layout (three fused stage launches, error sum
+    in the last epilogue) against the same step on the CSR kernel (separate
+    SpMV / axpy / lincomb / error launches): k2..k4 and the candidate to 1e-12
+    relative, the error estimate to 1e-10 relative; with and without a given k1"""
+    rng = np.random.default_rng(11)
+    keep = []
+    if name == "chain6":
+        qs, _, _ = build_system(recipes.spin_chain_recipe(6, seed=2))
+        A, keep = qs.matrix(), [qs]
+    else:
+        mats, tots = [], []
+        for p in recipes.tc_instances(12, 4):
+            q2, _, _ = build_system(recipes.tc_recipe(*p))
+            keep.append(q2)
+            mats.append(q2.matrix())
+            tots.append(recipes.tc_init(p[0], p[1])[0])
+        b = qa.Batch(mats, tots, prec=qa.FP64)
+        A = b.matrix()
+        keep.append(b)
+    n = A.info()[0]
+    x = rng.normal(size=n) + 1j * rng.normal(size=n)
+    h, atol, rtol = 0.05, 5e-7, 5e-7
+    results = {}
+    for g in (8, 134):
+        A.set_lanes(g, 0)
+        xv = qa.DeviceVec(n)
+        xv.set(x)
+        v = {k: qa.DeviceVec(n) for k in ("xn", "k1", "k2", "k3", "k4", "y")}
+        out = []
+        for have_k1 in (0, 1):
+            err = ctypes.c_double()
+            qa._qh(qa.lib().qh_rk3bs_step(A.h, xv.h, v["xn"].h, v["k1"].h, v["k2"].h, v["k3"].h, v["k4"].h,
+                                          v["y"].h, h, atol, rtol, have_k1, ctypes.byref(err)))
+            out.append((err.value, {k: v[k].get() for k in ("xn", "k1", "k2", "k3", "k4")}))
+        results[g] = out
+    for (e8, v8), (e134, v134) in zip(results[8], results[134]):
+        for k in v8:
+            assert np.abs(v8[k] - v134[k]).max() <= 1e-12 * np.abs(v8[k]).max(), (name, k)
+        assert abs(e8 - e134) <= 1e-10 * e8, (e8, e134)
+    A.set_lanes(0, 0)
     for k in keep:
         k.destroy()
 
