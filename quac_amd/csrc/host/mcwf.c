@@ -106,41 +106,125 @@ static int spawn(qsystem s, qvec x) {
   return 0;
 }
 
+/* choose the jump operator from the squared norms n2[j] = ||C_j psi||^2
+ * (qsystem.c:1166-1218): p_j = Re(a_j n2_j), operator k with probability
+ * p_k / sum p, scale sqrt(a_k) / sqrt(p_k).  Returns 0 when no operator can
+ * act (sum p = 0; no draw is made); p is scratch of mcwf_njump entries. */
+static int choose_jump(qsystem s, const double *n2, double *p, int32_t *kout, double complex *sc) {
+  double tot = 0.0;
+  for (PetscInt j = 0; j < s->mcwf_njump; j++) {
+    p[j] = creal(s->mcwf_jump_a[j] * n2[j]);
+    tot += p[j];
+  }
+  if (!(tot > 0.0)) return 0;
+  const double u = quac_rng_uniform(s);
+  double acc = 0.0;
+  PetscInt k = s->mcwf_njump - 1;
+  for (PetscInt j = 0; j < s->mcwf_njump; j++) {
+    acc += p[j] / tot;
+    if (acc >= u) {
+      k = j;
+      break;
+    }
+  }
+  *kout = (int32_t)k;
+  *sc = csqrt(s->mcwf_jump_a[k]) / sqrt(p[k]);
+  return 1;
+}
+
 /* the jump of trajectory i (qsystem.c:1159-1235) */
 static int jump(qsystem s, Vec block, int64_t i, int64_t N, Vec work) {
   Vec v = NULL;
   if (qh_vec_view(block, i * N, N, &v)) return quac_fail(9, "%s", qh_last_error());
-  double *p = (double *)malloc(sizeof(double) * (size_t)s->mcwf_njump);
-  double tot = 0.0;
+  double *n2 = (double *)malloc(sizeof(double) * 2 * (size_t)s->mcwf_njump);
   int rc = 0;
   for (PetscInt j = 0; j < s->mcwf_njump && !rc; j++) {
-    double n2 = 0.0;
+    n2[j] = 0.0;
     rc = qh_spmv(s->mcwf_jump[j], v, work);
-    if (!rc) rc = qh_ens_norms(work, 1, &n2);
-    p[j] = creal(s->mcwf_jump_a[j] * n2);
-    tot += p[j];
+    if (!rc) rc = qh_ens_norms(work, 1, &n2[j]);
   }
-  if (!rc && tot > 0.0) {
-    const double u = quac_rng_uniform(s);
-    double acc = 0.0;
-    PetscInt k = s->mcwf_njump - 1;
-    for (PetscInt j = 0; j < s->mcwf_njump; j++) {
-      acc += p[j] / tot;
-      if (acc >= u) {
-        k = j;
-        break;
-      }
-    }
-    double complex sc = csqrt(s->mcwf_jump_a[k]) / sqrt(p[k]);
+  int32_t k = 0;
+  double complex sc = 0.0;
+  if (!rc && choose_jump(s, n2, n2 + s->mcwf_njump, &k, &sc)) {
     rc = qh_spmv(s->mcwf_jump[k], v, work);
     if (!rc) rc = qh_vec_scale(work, creal(sc), cimag(sc));
     if (!rc) rc = qh_vec_copy(work, v);
   }
   s->rand_number[i] = quac_rng_uniform(s);
-  free(p);
+  free(n2);
   qh_vec_destroy(v);
   if (rc) return quac_fail(9, "MCWF jump failed: %s", qh_last_error());
   return 0;
+}
+
+/* the device-resident loop applies (at most 64 amplitudes, rows of the
+ * generator with at most 16 entries, at most 16 jump operators) and is not
+ * switched off by QUAC_MCWF_STEPWISE */
+static int fast_path(qsystem s, int64_t N) {
+  (void)N;
+  if (getenv("QUAC_MCWF_STEPWISE")) return 0;
+  int fits = 0;
+  if (qh_mcwf_fits(s->mat_A, (int)s->mcwf_njump, &fits)) return 0;
+  return fits;
+}
+
+/* Trajectories of at most 64 amplitudes without a monitor: each trajectory runs
+ * on the device from one jump to the next (qh_mcwf_advance) and the jumps due
+ * are applied together (qh_mcwf_jump_norms, choose_jump, qh_mcwf_jump_apply),
+ * until every trajectory has taken all J steps.  The steps, the jump test and
+ * the jump are those of the step-by-step loop below; the random numbers of the
+ * jumps are drawn round by round in trajectory order, so a given seed yields
+ * other (equally distributed) trajectories than the step-by-step loop. */
+static int run_to_end(qsystem s, Vec block, int64_t M, int64_t N, int32_t J, double dt, double hlast) {
+  const int nj = (int)s->mcwf_njump;
+  int32_t *jst = (int32_t *)calloc((size_t)M, sizeof(int32_t));
+  int32_t *jmp = (int32_t *)malloc(sizeof(int32_t) * (size_t)M);
+  int64_t *idx = (int64_t *)malloc(sizeof(int64_t) * (size_t)M);
+  int32_t *kk = (int32_t *)malloc(sizeof(int32_t) * (size_t)M);
+  double *sre = (double *)malloc(sizeof(double) * (size_t)M);
+  double *sim = (double *)malloc(sizeof(double) * (size_t)M);
+  double *n2 = (double *)malloc(sizeof(double) * (size_t)M * (size_t)nj);
+  double *p = (double *)malloc(sizeof(double) * (size_t)nj);
+  int rc = (jst && jmp && idx && kk && sre && sim && n2 && p) ? 0 : quac_fail(9, "out of host memory");
+  while (!rc) {
+    if (qh_mcwf_advance(s->mat_A, block, M, s->rand_number, jst, J, dt, hlast, jmp)) {
+      rc = quac_fail(9, "MCWF steps failed: %s", qh_last_error());
+      break;
+    }
+    int32_t n = 0;
+    for (int64_t i = 0; i < M; i++)
+      if (jmp[i]) idx[n++] = i;
+    if (n == 0) break;
+    if (qh_mcwf_jump_norms(nj, s->mcwf_jump, block, N, idx, n, n2)) {
+      rc = quac_fail(9, "MCWF jump failed: %s", qh_last_error());
+      break;
+    }
+    /* draws in the order of jump(): the operator, then the next threshold;
+     * the trajectories that jump are compacted to the front of idx */
+    int32_t m = 0;
+    for (int32_t q = 0; q < n; q++) {
+      const int64_t i = idx[q];
+      double complex sc = 0.0;
+      if (choose_jump(s, n2 + (size_t)q * nj, p, &kk[m], &sc)) {
+        idx[m] = i;
+        sre[m] = creal(sc);
+        sim[m] = cimag(sc);
+        m++;
+      }
+      s->rand_number[i] = quac_rng_uniform(s);
+    }
+    if (m && qh_mcwf_jump_apply(nj, s->mcwf_jump, block, N, idx, m, kk, sre, sim))
+      rc = quac_fail(9, "MCWF jump failed: %s", qh_last_error());
+  }
+  free(jst);
+  free(jmp);
+  free(idx);
+  free(kk);
+  free(sre);
+  free(sim);
+  free(n2);
+  free(p);
+  return rc;
 }
 
 int quac_mcwf_time_step(qsystem s, qvec x, PetscReal t0, PetscReal tmax, PetscReal dt, PetscInt steps_max,
@@ -157,6 +241,16 @@ int quac_mcwf_time_step(qsystem s, qvec x, PetscReal t0, PetscReal tmax, PetscRe
   const PetscReal eps = 1e-12 * (fabs(tmax) + fabs(dt) + 1.0);
   int rc = 0;
   if (monitor) rc = monitor(s, 0, t, x);
+  if (!monitor && fast_path(s, N)) {
+    /* the step sizes of the loop below: dt, the last one cut at tmax */
+    PetscReal hlast = dt;
+    while (t < tmax - eps && step < steps_max && step < INT32_MAX) {
+      hlast = (t + dt > tmax - eps) ? tmax - t : dt;
+      t += hlast;
+      step++;
+    }
+    if (step > 0) rc = run_to_end(s, block, M, N, (int32_t)step, dt, hlast);
+  }
   while (!rc && t < tmax - eps && step < steps_max) {
     PetscReal h = dt;
     if (t + h > tmax - eps) h = tmax - t;

@@ -142,6 +142,9 @@ int ks2_build(qh_mat *A);
 int ks2_launch(int G, int mode, hipStream_t s, const qh_mat *A, const void *xin, void *kout, void *ynext, float cn,
                const void *x, const void *k1, const void *k2, const void *k3, void *xout, float h, float2 *xlo);
 
+// longest row of A into A->max_row (computed once per matrix)
+int qh_mat_max_row(qh_mat *A);
+
 // device allocation helpers (return QH_* codes)
 int qh_alloc(void **p, size_t bytes);
 void qh_free_ptr(void *p);

@@ -9,6 +9,7 @@
 // squared norms (the jump test of _sys_MCWF_EventFunction, qsystem.c:1138-1153)
 // come from one block per trajectory.  The jump itself (choice of the operator
 // and renormalisation, qsystem.c:1159-1235) is decided on the host.
+#include <cstring>
 #include <vector>
 
 #include "qh_common.h"
@@ -101,7 +102,271 @@ int ens_work(int64_t n) {
   return QH_OK;
 }
 
+// ------------------------------------------------- trajectories of up to 64 amplitudes
+// One wavefront per trajectory: lane r holds amplitude r and row r of the
+// generator (up to ENS_MAXNZ entries in registers); the products exchange the
+// amplitudes through the wavefront's LDS row.  A trajectory runs classic RK4
+// steps (size dt, the last of the run hlast) from its step index jst[i] up to J
+// and stops right after a step at which its squared norm fell below thr[i]
+// (the jump test of _sys_MCWF_EventFunction, qsystem.c:1138-1153): jump[i] = 1.
+const int ENS_W = 64;
+const int ENS_MAXNZ = 16;
+
+__device__ inline double2 ens_row_product(const int32_t *c, const double2 *a, int nz, const double2 *s) {
+  double re = 0.0, im = 0.0;
+#pragma unroll
+  for (int e = 0; e < ENS_MAXNZ; e++)
+    if (e < nz) {
+      const double2 q = s[c[e]];
+      re = fma(a[e].x, q.x, fma(-a[e].y, q.y, re));
+      im = fma(a[e].x, q.y, fma(a[e].y, q.x, im));
+    }
+  return make_double2(re, im);
+}
+
+__global__ void __launch_bounds__(256) k_mcwf_run(int32_t N, int64_t M, const int32_t *__restrict__ rp,
+                                                  const int32_t *__restrict__ col, const double2 *__restrict__ val,
+                                                  double2 *__restrict__ ens, const double *__restrict__ thr,
+                                                  int32_t *__restrict__ jst, int32_t J, double dt, double hlast,
+                                                  int32_t *__restrict__ jump) {
+  __shared__ double2 sh[4][ENS_W];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t i = (int64_t)blockIdx.x * 4 + w;
+  if (i >= M) return;  // whole wavefront
+  int32_t j = jst[i];
+  if (j >= J) {
+    if (lane == 0) jump[i] = 0;
+    return;
+  }
+  const bool on = lane < N;
+  int32_t c[ENS_MAXNZ];
+  double2 a[ENS_MAXNZ];
+  int nz = 0;
+  if (on) {
+    const int32_t beg = rp[lane];
+    nz = rp[lane + 1] - beg;
+#pragma unroll
+    for (int e = 0; e < ENS_MAXNZ; e++)
+      if (e < nz) {
+        c[e] = col[beg + e];
+        a[e] = val[beg + e];
+      }
+  }
+  double2 *s = sh[w];
+  double2 psi = on ? ens[i * N + lane] : make_double2(0.0, 0.0);
+  const double t = thr[i];
+  // A v for this lane's row; the LDS row is written and read by this wavefront
+  // only, in program order
+  auto apply = [&](double2 v) {
+    s[lane] = v;
+    __builtin_amdgcn_wave_barrier();
+    const double2 r = ens_row_product(c, a, nz, s);
+    __builtin_amdgcn_wave_barrier();
+    return r;
+  };
+  int32_t jumped = 0;
+  while (j < J) {
+    const double h = (j == J - 1) ? hlast : dt;
+    const double2 k1 = apply(psi);
+    const double2 k2 = apply(make_double2(psi.x + 0.5 * h * k1.x, psi.y + 0.5 * h * k1.y));
+    const double2 k3 = apply(make_double2(psi.x + 0.5 * h * k2.x, psi.y + 0.5 * h * k2.y));
+    const double2 k4 = apply(make_double2(psi.x + h * k3.x, psi.y + h * k3.y));
+    const double s6 = h * (1.0 / 6.0), s3 = h * (1.0 / 3.0);
+    psi = make_double2(psi.x + s6 * k1.x + s3 * k2.x + s3 * k3.x + s6 * k4.x,
+                       psi.y + s6 * k1.y + s3 * k2.y + s3 * k3.y + s6 * k4.y);
+    j++;
+    double n2 = on ? psi.x * psi.x + psi.y * psi.y : 0.0;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) n2 += __shfl_xor(n2, o, 64);
+    if (n2 < t) {
+      jumped = 1;
+      break;
+    }
+  }
+  if (on) ens[i * N + lane] = psi;
+  if (lane == 0) {
+    jst[i] = j;
+    jump[i] = jumped;
+  }
+}
+
+// the jump operators of an ensemble (CSR, N x N each)
+const int ENS_MAXJ = 16;
+struct JumpOps {
+  const int32_t *rp[ENS_MAXJ];
+  const int32_t *col[ENS_MAXJ];
+  const double2 *val[ENS_MAXJ];
+};
+
+// p[q * nj + k] = || C_k psi_{idx[q]} ||^2, one wavefront per (q, k)
+__global__ void __launch_bounds__(256) k_mcwf_jump_norms(int32_t N, const double2 *__restrict__ ens,
+                                                        const int64_t *__restrict__ idx, int32_t n, int32_t nj,
+                                                        JumpOps ops, double *__restrict__ p) {
+  const int64_t g = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (g >= (int64_t)n * nj) return;  // whole wavefront
+  const int64_t q = g / nj;
+  const int k = (int)(g % nj);
+  const double2 *psi = ens + idx[q] * N;
+  double v = 0.0;
+  if (lane < N) {
+    double re = 0.0, im = 0.0;
+    for (int32_t e = ops.rp[k][lane]; e < ops.rp[k][lane + 1]; e++) {
+      const double2 a = ops.val[k][e], x = psi[ops.col[k][e]];
+      re = fma(a.x, x.x, fma(-a.y, x.y, re));
+      im = fma(a.x, x.y, fma(a.y, x.x, im));
+    }
+    v = re * re + im * im;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  if (lane == 0) p[g] = v;
+}
+
+// psi_{idx[q]} <- sc[q] C_{kk[q]} psi_{idx[q]}, one wavefront per q
+__global__ void __launch_bounds__(256) k_mcwf_jump_apply(int32_t N, double2 *__restrict__ ens,
+                                                        const int64_t *__restrict__ idx, int32_t n,
+                                                        const int32_t *__restrict__ kk,
+                                                        const double2 *__restrict__ sc, JumpOps ops) {
+  __shared__ double2 sh[4][ENS_W];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t q = (int64_t)blockIdx.x * 4 + w;
+  if (q >= n) return;  // whole wavefront
+  double2 *psi = ens + idx[q] * N;
+  double2 *s = sh[w];
+  if (lane < N) s[lane] = psi[lane];
+  __builtin_amdgcn_wave_barrier();
+  if (lane < N) {
+    const int k = kk[q];
+    double re = 0.0, im = 0.0;
+    for (int32_t e = ops.rp[k][lane]; e < ops.rp[k][lane + 1]; e++) {
+      const double2 a = ops.val[k][e], x = s[ops.col[k][e]];
+      re = fma(a.x, x.x, fma(-a.y, x.y, re));
+      im = fma(a.x, x.y, fma(a.y, x.x, im));
+    }
+    const double2 f = sc[q];
+    psi[lane] = make_double2(f.x * re - f.y * im, f.x * im + f.y * re);
+  }
+}
+
+int jump_ops(int nj, qh_mat *const *C, int64_t N, JumpOps *ops) {
+  QH_REQUIRE(nj >= 1 && nj <= ENS_MAXJ && C, QH_EINVAL, "MCWF jumps: 1..16 jump operators required");
+  for (int k = 0; k < nj; k++) {
+    QH_REQUIRE(C[k] && C[k]->rows == N, QH_EINVAL, "MCWF jumps: operator %d is not N x N", k);
+    ops->rp[k] = C[k]->rowptr;
+    ops->col[k] = C[k]->col;
+    ops->val[k] = C[k]->val;
+  }
+  return QH_OK;
+}
+
 }  // namespace
+
+extern "C" int qh_mcwf_advance(const qh_mat *A, qh_vec *ens, int64_t M, const double *thr, int32_t *jstep,
+                               int32_t J, double dt, double hlast, int32_t *jump) {
+  int rc = qh_ensure_init();
+  if (rc) return rc;
+  QH_REQUIRE(A && ens && thr && jstep && jump && ens->prec == QH_FP64 && M >= 1 && M <= (1 << 30) &&
+                 ens->n == M * A->rows && A->rows >= 1 && A->rows <= ENS_W && J >= 0,
+             QH_EINVAL, "qh_mcwf_advance: FP64 ensemble of M trajectories of at most 64 amplitudes required");
+  const int32_t N = (int32_t)A->rows;
+  rc = qh_mat_max_row(const_cast<qh_mat *>(A));  // cached on the matrix
+  if (rc) return rc;
+  QH_REQUIRE(A->max_row <= ENS_MAXNZ, QH_EINVAL, "qh_mcwf_advance: a row with more than %d entries", ENS_MAXNZ);
+  hipStream_t s = qh_stream();
+  // device copies: thresholds, step indices, jump flags
+  char *d = nullptr, *h = nullptr;
+  const size_t bt = sizeof(double) * M, bj = sizeof(int32_t) * M;
+  rc = qh_scratch(bt + 2 * bj + 256, (void **)&d, bt + 2 * bj, (void **)&h);
+  if (rc) return rc;
+  double *dthr = (double *)d;
+  int32_t *djs = (int32_t *)(d + bt), *djump = (int32_t *)(d + bt + bj);
+  memcpy(h, thr, bt);
+  memcpy(h + bt, jstep, bj);
+  QH_HIP(hipMemcpyAsync(d, h, bt + bj, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_mcwf_run, dim3((unsigned)((M + 3) / 4)), dim3(256), 0, s, N, M, A->rowptr, A->col, A->val,
+                     (double2 *)ens->d, dthr, djs, J, dt, hlast, djump);
+  QH_HIP(hipGetLastError());
+  QH_HIP(hipMemcpyAsync(h + bt, djs, 2 * bj, hipMemcpyDeviceToHost, s));
+  QH_HIP(hipStreamSynchronize(s));
+  memcpy(jstep, h + bt, bj);
+  memcpy(jump, h + bt + bj, bj);
+  return QH_OK;
+}
+
+extern "C" int qh_mcwf_fits(const qh_mat *A, int nj, int *fits) {
+  int rc = qh_ensure_init();
+  if (rc) return rc;
+  QH_REQUIRE(A && fits, QH_EINVAL, "qh_mcwf_fits: null argument");
+  *fits = 0;
+  if (A->rows < 1 || A->rows > ENS_W || nj < 1 || nj > ENS_MAXJ) return QH_OK;
+  rc = qh_mat_max_row(const_cast<qh_mat *>(A));
+  if (rc) return rc;
+  *fits = A->max_row <= ENS_MAXNZ;
+  return QH_OK;
+}
+
+extern "C" int qh_mcwf_jump_norms(int nj, qh_mat *const *C, const qh_vec *ens, int64_t N, const int64_t *idx,
+                                  int32_t n, double *p) {
+  int rc = qh_ensure_init();
+  if (rc) return rc;
+  QH_REQUIRE(ens && idx && p && ens->prec == QH_FP64 && N >= 1 && N <= ENS_W && ens->n % N == 0 && n >= 0,
+             QH_EINVAL, "qh_mcwf_jump_norms: FP64 ensemble of at most 64 amplitudes per trajectory required");
+  JumpOps ops;
+  rc = jump_ops(nj, C, N, &ops);
+  if (rc || n == 0) return rc;
+  for (int32_t q = 0; q < n; q++)
+    QH_REQUIRE(idx[q] >= 0 && idx[q] < ens->n / N, QH_EINVAL, "qh_mcwf_jump_norms: trajectory out of range");
+  hipStream_t s = qh_stream();
+  char *d = nullptr, *h = nullptr;
+  const size_t bi = sizeof(int64_t) * n, bp = sizeof(double) * n * nj;
+  rc = qh_scratch(bi + bp + 256, (void **)&d, bi + bp, (void **)&h);
+  if (rc) return rc;
+  memcpy(h, idx, bi);
+  QH_HIP(hipMemcpyAsync(d, h, bi, hipMemcpyHostToDevice, s));
+  const int64_t waves = (int64_t)n * nj;
+  double *dp = (double *)(d + ((bi + 255) & ~(size_t)255));
+  hipLaunchKernelGGL(k_mcwf_jump_norms, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s, (int32_t)N,
+                     (const double2 *)ens->d, (const int64_t *)d, n, (int32_t)nj, ops, dp);
+  QH_HIP(hipGetLastError());
+  QH_HIP(hipMemcpyAsync(h, dp, bp, hipMemcpyDeviceToHost, s));
+  QH_HIP(hipStreamSynchronize(s));
+  memcpy(p, h, bp);
+  return QH_OK;
+}
+
+extern "C" int qh_mcwf_jump_apply(int nj, qh_mat *const *C, qh_vec *ens, int64_t N, const int64_t *idx, int32_t n,
+                                  const int32_t *k, const double *sre, const double *sim) {
+  int rc = qh_ensure_init();
+  if (rc) return rc;
+  QH_REQUIRE(ens && idx && k && sre && sim && ens->prec == QH_FP64 && N >= 1 && N <= ENS_W && ens->n % N == 0 &&
+                 n >= 0,
+             QH_EINVAL, "qh_mcwf_jump_apply: FP64 ensemble of at most 64 amplitudes per trajectory required");
+  JumpOps ops;
+  rc = jump_ops(nj, C, N, &ops);
+  if (rc || n == 0) return rc;
+  for (int32_t q = 0; q < n; q++) {
+    QH_REQUIRE(idx[q] >= 0 && idx[q] < ens->n / N, QH_EINVAL, "qh_mcwf_jump_apply: trajectory out of range");
+    QH_REQUIRE(k[q] >= 0 && k[q] < nj, QH_EINVAL, "qh_mcwf_jump_apply: operator index out of range");
+  }
+  hipStream_t s = qh_stream();
+  char *d = nullptr, *h = nullptr;
+  const size_t bi = (sizeof(int64_t) * n + 255) & ~(size_t)255, bk = (sizeof(int32_t) * n + 255) & ~(size_t)255,
+               bs = sizeof(double2) * n;
+  rc = qh_scratch(bi + bk + bs, (void **)&d, bi + bk + bs, (void **)&h);
+  if (rc) return rc;
+  memcpy(h, idx, sizeof(int64_t) * n);
+  memcpy(h + bi, k, sizeof(int32_t) * n);
+  double2 *hs = (double2 *)(h + bi + bk);
+  for (int32_t q = 0; q < n; q++) hs[q] = make_double2(sre[q], sim[q]);
+  QH_HIP(hipMemcpyAsync(d, h, bi + bk + bs, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_mcwf_jump_apply, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s, (int32_t)N,
+                     (double2 *)ens->d, (const int64_t *)d, n, (const int32_t *)(d + bi),
+                     (const double2 *)(d + bi + bk), ops);
+  QH_HIP(hipGetLastError());
+  QH_HIP(hipStreamSynchronize(s));
+  return QH_OK;
+}
 
 extern "C" int qh_ens_broadcast(const qh_vec *init, qh_vec *ens, int64_t M) {
   int rc = qh_ensure_init();

@@ -1325,7 +1325,7 @@ __global__ void k_max_row(const int32_t *__restrict__ rp, int64_t n, int32_t *ou
   if (threadIdx.x == 0) out[0] = red[0];
 }
 
-static int ensure_max_row(qh_mat *A) {
+int qh_mat_max_row(qh_mat *A) {
   if (A->max_row > 0) return QH_OK;
   int32_t *d = nullptr;
   int rc = qh_alloc_t(&d, 1);
@@ -1341,7 +1341,7 @@ static int ensure_max_row(qh_mat *A) {
 }
 
 static int rk4_small(qh_mat *A, qh_vec *x, double dt, int64_t nsteps, int64_t tot, int renorm_every) {
-  int rc0 = ensure_max_row(A);
+  int rc0 = qh_mat_max_row(A);
   if (rc0) return rc0;
   const int32_t n = (int32_t)A->rows;
   int bt = ((n + 63) / 64) * 64;

@@ -4,6 +4,7 @@ Parity is statistical (the reference draws from SPRNG, which is not available):
 ensemble averages must agree with the density-matrix solution within 4 standard
 errors of the mean plus the first-order jump-timing bias (gamma*dt/2 relative;
 both bounds are written in each test)."""
+import ctypes
 import math
 
 import numpy as np
@@ -41,10 +42,14 @@ def test_t1_decay_ensemble():
     qs.destroy()
 
 
-def test_driven_jc_ensemble_vs_density_matrix():
+@pytest.mark.parametrize("path", ["device_loop", "stepwise"])
+def test_driven_jc_ensemble_vs_density_matrix(path, monkeypatch):
     """emitter (2) x cavity (4), exchange coupling and both losses; averages of
-    M trajectories against the FP64 density-matrix RK4 of the same system"""
+    M trajectories against the FP64 density-matrix RK4 of the same system, on the
+    device-resident trajectory loop and on the step-by-step loop"""
     _fixed_rk4()
+    if path == "stepwise":
+        monkeypatch.setenv("QUAC_MCWF_STEPWISE", "1")
     M, dt, T = 4000, 0.01, 3.0
     g, wq, wc, kap, gam = 0.4, 1.0, 1.0, 0.3, 0.2
     qs = qa.System()
@@ -99,3 +104,148 @@ def test_monitor_sees_the_ensemble_and_norms_decay():
     e = x.expectation(o.n)
     assert 0.0 <= e.real <= 2.0 and abs(e.imag) < 1e-12
     qs.destroy()
+
+
+def _jc_generator(gam=1.0, kap=0.8, levels=4):
+    """-i Heff of an emitter (2) x cavity (levels) under MCWF (N = 2 levels)"""
+    qs = qa.System()
+    q = qs.create_op(2)
+    c = qs.create_op(levels)
+    qs.add_ham_term(1.0, q.n)
+    qs.add_ham_term(0.9, c.n)
+    qs.add_ham_term(0.4, q.dag, c)
+    qs.add_ham_term(0.4, q, c.dag)
+    qs.add_lin_term(gam, q)
+    qs.add_lin_term(kap, c)
+    qs.use_mcwf_solver(4, 1)
+    qs.construct_matrix()
+    return qs, qs.matrix()
+
+
+def _dense(A):
+    rp, col, val = A.export()
+    n = len(rp) - 1
+    d = np.zeros((n, n), np.complex128)
+    for r in range(n):
+        d[r, col[rp[r]:rp[r + 1]]] += val[rp[r]:rp[r + 1]]
+    return d
+
+
+def _P(a, t):
+    return a.ctypes.data_as(ctypes.POINTER(t))
+
+
+def test_mcwf_advance_matches_stepwise_ensemble_rk4():
+    """qh_mcwf_advance (each trajectory runs from its step index to its first
+    norm crossing, one wavefront per trajectory) against the ensemble RK4 kernel
+    taken one step at a time with the crossing found on the host: the stop step
+    of every trajectory exactly, the amplitudes to 1e-12.  Covers thresholds that
+    are never crossed, a trajectory that has already taken all steps, a late
+    start and a shorter last step."""
+    qs, A = _jc_generator()
+    L = qa.lib()
+    N = A.info()[0]
+    fits = ctypes.c_int()
+    qa._qh(L.qh_mcwf_fits(A.h, 2, ctypes.byref(fits)))
+    assert fits.value == 1
+    M, J, dt, hlast = 37, 60, 0.02, 0.013
+    rng = np.random.default_rng(5)
+    psi = rng.normal(size=(M, N)) + 1j * rng.normal(size=(M, N))
+    psi /= np.linalg.norm(psi, axis=1, keepdims=True)
+    thr = rng.uniform(0.4, 0.99, M)
+    thr[:3] = 0.0
+    j0 = np.zeros(M, np.int32)
+    j0[3] = J
+    j0[4] = 17
+    j0[5] = J - 1
+    ens = qa.DeviceVec(M * N)
+    ens.set(psi.ravel())
+    jst = j0.copy()
+    jump = np.full(M, 7, np.int32)
+    qa._qh(L.qh_mcwf_advance(A.h, ens.h, M, _P(thr, ctypes.c_double), _P(jst, ctypes.c_int32), J, dt, hlast,
+                             _P(jump, ctypes.c_int32)))
+    got = ens.get().reshape(M, N)
+    # reference: one ensemble RK4 step at a time; trajectories that have not
+    # started are held at their initial state, stopped ones are frozen
+    ref = qa.DeviceVec(M * N)
+    cur = psi.copy()
+    stop, frozen = j0.copy(), {}
+    done = j0 >= J
+    for j in range(J):
+        h = hlast if j == J - 1 else dt
+        ref.set(cur.ravel())
+        qa._qh(L.qh_ens_rk4(A.h, ref.h, M, h, 1))
+        nxt = ref.get().reshape(M, N)
+        for i in range(M):
+            if done[i] or j < j0[i]:
+                continue
+            cur[i] = nxt[i]
+            if np.vdot(cur[i], cur[i]).real < thr[i]:
+                done[i], stop[i], frozen[i] = True, j + 1, cur[i].copy()
+        if done.all():
+            break
+    for i in range(M):
+        want = frozen.get(i, cur[i] if j0[i] < J else psi[i])
+        exp_stop = stop[i] if i in frozen else J
+        assert jst[i] == exp_stop, (i, jst[i], exp_stop)
+        assert jump[i] == (1 if i in frozen else 0), (i, jump[i])
+        assert np.abs(got[i] - want).max() < 1e-12, (i, np.abs(got[i] - want).max())
+    assert len(frozen) >= 3 and M - len(frozen) >= 5   # crossings and runs to the end (4 by construction)
+    # a second call resumes the trajectories that stopped and finishes the rest
+    qa._qh(L.qh_mcwf_advance(A.h, ens.h, M, _P(np.zeros(M), ctypes.c_double), _P(jst, ctypes.c_int32), J, dt,
+                             hlast, _P(jump, ctypes.c_int32)))
+    assert (jst == J).all() and (jump == 0).all()
+    qs.destroy()
+
+
+def test_mcwf_jump_norms_and_apply_match_dense_products():
+    """qh_mcwf_jump_norms: ||C_k psi_i||^2 for every listed trajectory and
+    operator; qh_mcwf_jump_apply: psi_i <- s_i C_{k_i} psi_i for the listed
+    trajectories only, the others untouched (dense numpy products, 1e-12)"""
+    q1, A1 = _jc_generator()
+    q2, A2 = _jc_generator(gam=0.3, kap=2.0)
+    L = qa.lib()
+    C = [_dense(A1), _dense(A2)]
+    N = C[0].shape[0]
+    M = 23
+    rng = np.random.default_rng(8)
+    psi = rng.normal(size=(M, N)) + 1j * rng.normal(size=(M, N))
+    ens = qa.DeviceVec(M * N)
+    ens.set(psi.ravel())
+    ops = (ctypes.c_void_p * 2)(A1.h, A2.h)
+    idx = np.array([0, 5, 5, 22, 9, 13], np.int64)
+    p = np.empty(len(idx) * 2)
+    qa._qh(L.qh_mcwf_jump_norms(2, ops, ens.h, N, _P(idx, ctypes.c_int64), len(idx), _P(p, ctypes.c_double)))
+    want = np.array([[np.linalg.norm(C[k] @ psi[i]) ** 2 for k in range(2)] for i in idx]).ravel()
+    assert np.allclose(p, want, rtol=1e-12, atol=0)
+    idx = np.array([3, 17, 0, 22], np.int64)
+    kk = np.array([1, 0, 1, 0], np.int32)
+    sre, sim = rng.normal(size=4), rng.normal(size=4)
+    qa._qh(L.qh_mcwf_jump_apply(2, ops, ens.h, N, _P(idx, ctypes.c_int64), 4, _P(kk, ctypes.c_int32),
+                                _P(sre, ctypes.c_double), _P(sim, ctypes.c_double)))
+    got = ens.get().reshape(M, N)
+    want = psi.copy()
+    for q, i in enumerate(idx):
+        want[i] = (sre[q] + 1j * sim[q]) * (C[kk[q]] @ psi[i])
+    assert np.abs(got - want).max() < 1e-12 * np.abs(want).max()
+    q1.destroy()
+    q2.destroy()
+
+
+def test_device_loop_is_deterministic_for_a_seed():
+    """the same seed gives the same ensemble average on the device loop"""
+    _fixed_rk4()
+    vals = []
+    for _ in range(2):
+        qs = qa.System()
+        o = qs.create_op(3)
+        qs.add_ham_term(0.3, o.n)
+        qs.add_lin_term(0.7, o)
+        qs.use_mcwf_solver(500, 21)
+        qs.construct_matrix()
+        x = qs.create_qvec()
+        x.add(1.0, 2)
+        qs.time_step(x, 0.0, 2.0, 0.01, 100000)
+        vals.append(x.expectation(o.n))
+        qs.destroy()
+    assert vals[0] == vals[1]
