@@ -103,19 +103,24 @@ int ens_work(int64_t n) {
 }
 
 // ------------------------------------------------- trajectories of up to 64 amplitudes
-// One wavefront per trajectory: lane r holds amplitude r and row r of the
-// generator (up to ENS_MAXNZ entries in registers); the products exchange the
-// amplitudes through the wavefront's LDS row.  A trajectory runs classic RK4
-// steps (size dt, the last of the run hlast) from its step index jst[i] up to J
-// and stops right after a step at which its squared norm fell below thr[i]
-// (the jump test of _sys_MCWF_EventFunction, qsystem.c:1138-1153): jump[i] = 1.
+// A wavefront integrates 64 / S trajectories side by side, one segment of S
+// lanes (S a power of two >= N) per trajectory: lane r of a segment holds
+// amplitude r and row r of the generator (up to NZ entries in registers), and
+// the products exchange the amplitudes through the segment's part of the
+// wavefront's LDS row.  A trajectory runs classic RK4 steps (size dt, the last
+// of the run hlast) from its step index jst[i] up to J and stops right after a
+// step at which its squared norm fell below thr[i] (the jump test of
+// _sys_MCWF_EventFunction, qsystem.c:1138-1153): jump[i] = 1.  The wavefront
+// steps until all its trajectories have stopped; a stopped segment keeps its
+// amplitudes.
 const int ENS_W = 64;
 const int ENS_MAXNZ = 16;
 
+template <int NZ>
 __device__ inline double2 ens_row_product(const int32_t *c, const double2 *a, int nz, const double2 *s) {
   double re = 0.0, im = 0.0;
 #pragma unroll
-  for (int e = 0; e < ENS_MAXNZ; e++)
+  for (int e = 0; e < NZ; e++)
     if (e < nz) {
       const double2 q = s[c[e]];
       re = fma(a[e].x, q.x, fma(-a[e].y, q.y, re));
@@ -124,67 +129,73 @@ __device__ inline double2 ens_row_product(const int32_t *c, const double2 *a, in
   return make_double2(re, im);
 }
 
+template <int S, int NZ>
 __global__ void __launch_bounds__(256) k_mcwf_run(int32_t N, int64_t M, const int32_t *__restrict__ rp,
                                                   const int32_t *__restrict__ col, const double2 *__restrict__ val,
                                                   double2 *__restrict__ ens, const double *__restrict__ thr,
                                                   int32_t *__restrict__ jst, int32_t J, double dt, double hlast,
                                                   int32_t *__restrict__ jump) {
+  constexpr int P = ENS_W / S;  // trajectories per wavefront
   __shared__ double2 sh[4][ENS_W];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int64_t i = (int64_t)blockIdx.x * 4 + w;
-  if (i >= M) return;  // whole wavefront
-  int32_t j = jst[i];
-  if (j >= J) {
-    if (lane == 0) jump[i] = 0;
-    return;
-  }
-  const bool on = lane < N;
-  int32_t c[ENS_MAXNZ];
-  double2 a[ENS_MAXNZ];
+  const int seg = lane / S, r = lane % S;
+  const int64_t i = ((int64_t)blockIdx.x * 4 + w) * P + seg;
+  if (((int64_t)blockIdx.x * 4 + w) * P >= M) return;  // whole wavefront
+  const bool have = i < M;
+  int32_t j = have ? jst[i] : J;
+  const bool on = have && r < N;
+  int32_t c[NZ];
+  double2 a[NZ];
   int nz = 0;
   if (on) {
-    const int32_t beg = rp[lane];
-    nz = rp[lane + 1] - beg;
+    const int32_t beg = rp[r];
+    nz = rp[r + 1] - beg;
 #pragma unroll
-    for (int e = 0; e < ENS_MAXNZ; e++)
+    for (int e = 0; e < NZ; e++)
       if (e < nz) {
         c[e] = col[beg + e];
         a[e] = val[beg + e];
       }
   }
-  double2 *s = sh[w];
-  double2 psi = on ? ens[i * N + lane] : make_double2(0.0, 0.0);
-  const double t = thr[i];
+  double2 *sg = sh[w] + seg * S;
+  double2 psi = on ? ens[i * N + r] : make_double2(0.0, 0.0);
+  const double t = have ? thr[i] : 0.0;
   // A v for this lane's row; the LDS row is written and read by this wavefront
   // only, in program order
   auto apply = [&](double2 v) {
-    s[lane] = v;
+    sg[r] = v;
     __builtin_amdgcn_wave_barrier();
-    const double2 r = ens_row_product(c, a, nz, s);
+    const double2 q = ens_row_product<NZ>(c, a, nz, sg);
     __builtin_amdgcn_wave_barrier();
-    return r;
+    return q;
   };
+  bool act = j < J;
   int32_t jumped = 0;
-  while (j < J) {
+  while (__ballot(act)) {
     const double h = (j == J - 1) ? hlast : dt;
     const double2 k1 = apply(psi);
     const double2 k2 = apply(make_double2(psi.x + 0.5 * h * k1.x, psi.y + 0.5 * h * k1.y));
     const double2 k3 = apply(make_double2(psi.x + 0.5 * h * k2.x, psi.y + 0.5 * h * k2.y));
     const double2 k4 = apply(make_double2(psi.x + h * k3.x, psi.y + h * k3.y));
     const double s6 = h * (1.0 / 6.0), s3 = h * (1.0 / 3.0);
-    psi = make_double2(psi.x + s6 * k1.x + s3 * k2.x + s3 * k3.x + s6 * k4.x,
-                       psi.y + s6 * k1.y + s3 * k2.y + s3 * k3.y + s6 * k4.y);
-    j++;
-    double n2 = on ? psi.x * psi.x + psi.y * psi.y : 0.0;
+    const double2 nx = make_double2(psi.x + s6 * k1.x + s3 * k2.x + s3 * k3.x + s6 * k4.x,
+                                    psi.y + s6 * k1.y + s3 * k2.y + s3 * k3.y + s6 * k4.y);
+    double n2 = on ? nx.x * nx.x + nx.y * nx.y : 0.0;
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) n2 += __shfl_xor(n2, o, 64);
-    if (n2 < t) {
-      jumped = 1;
-      break;
+    for (int o = S / 2; o > 0; o >>= 1) n2 += __shfl_xor(n2, o, 64);
+    if (act) {
+      psi = nx;
+      j++;
+      if (n2 < t) {
+        jumped = 1;
+        act = false;
+      } else if (j >= J) {
+        act = false;
+      }
     }
   }
-  if (on) ens[i * N + lane] = psi;
-  if (lane == 0) {
+  if (on) ens[i * N + r] = psi;
+  if (have && r == 0) {
     jst[i] = j;
     jump[i] = jumped;
   }
@@ -284,8 +295,26 @@ extern "C" int qh_mcwf_advance(const qh_mat *A, qh_vec *ens, int64_t M, const do
   memcpy(h, thr, bt);
   memcpy(h + bt, jstep, bj);
   QH_HIP(hipMemcpyAsync(d, h, bt + bj, hipMemcpyHostToDevice, s));
-  hipLaunchKernelGGL(k_mcwf_run, dim3((unsigned)((M + 3) / 4)), dim3(256), 0, s, N, M, A->rowptr, A->col, A->val,
-                     (double2 *)ens->d, dthr, djs, J, dt, hlast, djump);
+  const int S = N <= 2 ? 2 : N <= 4 ? 4 : N <= 8 ? 8 : N <= 16 ? 16 : N <= 32 ? 32 : 64;
+  const int64_t waves = (M + ENS_W / S - 1) / (ENS_W / S);
+  const dim3 grid((unsigned)((waves + 3) / 4));
+  const bool short_rows = A->max_row <= 4;
+#define QH_MCWF_RUN(SS)                                                                                          \
+  if (S == SS) {                                                                                                  \
+    if (short_rows)                                                                                               \
+      hipLaunchKernelGGL((k_mcwf_run<SS, 4>), grid, dim3(256), 0, s, N, M, A->rowptr, A->col, A->val,            \
+                         (double2 *)ens->d, dthr, djs, J, dt, hlast, djump);                                      \
+    else                                                                                                          \
+      hipLaunchKernelGGL((k_mcwf_run<SS, ENS_MAXNZ>), grid, dim3(256), 0, s, N, M, A->rowptr, A->col, A->val,    \
+                         (double2 *)ens->d, dthr, djs, J, dt, hlast, djump);                                      \
+  }
+  QH_MCWF_RUN(2)
+  QH_MCWF_RUN(4)
+  QH_MCWF_RUN(8)
+  QH_MCWF_RUN(16)
+  QH_MCWF_RUN(32)
+  QH_MCWF_RUN(64)
+#undef QH_MCWF_RUN
   QH_HIP(hipGetLastError());
   QH_HIP(hipMemcpyAsync(h + bt, djs, 2 * bj, hipMemcpyDeviceToHost, s));
   QH_HIP(hipStreamSynchronize(s));
