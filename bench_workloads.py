@@ -364,18 +364,30 @@ def mcwf(args):
     ref = _pops(xd.data(), subs)
     qd.destroy()
     # trajectories
-    qs = qa.System()
-    ops = []
-    for (kind, L) in recipe["subs"]:
-        ops.append(qs.create_op(L))
-    for (k, a, o) in recipe["terms"]:
-        plist = [{"lower": ops[s_], "raise": ops[s_].dag, "n": ops[s_].n}[nm] for (s_, nm) in o]
-        (qs.add_ham_term if k == "ham" else qs.add_lin_term)(a, *plist)
-    qs.use_mcwf_solver(M, 5)
-    qs.construct_matrix()
-    x = qs.create_qvec()
-    x.add(1.0, loc // N)      # |e,0> as a wavefunction: Hilbert index of the DM diagonal entry
+    def make(m, seed):
+        qs = qa.System()
+        ops = []
+        for (kind, L) in recipe["subs"]:
+            ops.append(qs.create_op(L))
+        for (k, a, o) in recipe["terms"]:
+            plist = [{"lower": ops[s_], "raise": ops[s_].dag, "n": ops[s_].n}[nm] for (s_, nm) in o]
+            (qs.add_ham_term if k == "ham" else qs.add_lin_term)(a, *plist)
+        qs.use_mcwf_solver(m, seed)
+        qs.construct_matrix()
+        x = qs.create_qvec()
+        x.add(1.0, loc // N)      # |e,0> as a wavefunction: Hilbert index of the DM diagonal entry
+        return qs, ops, x
+
+    # warm-up on a separate ensemble of the same size to t = 2 (jumps occur):
+    # the process's first launches of the stepping and jump kernels and its
+    # first allocation of an ensemble of this size (about 6.5 ms in the HIP
+    # runtime) stay out of the timed call
+    qw, _, xw = make(M, 9)
+    qw.time_step(xw, 0.0, 2.0, dt, 10 ** 7)
+    qw.destroy()
+    qs, ops, x = make(M, 5)
     qa.synchronize()
+    # timed: the whole time_step call (jump operators, thresholds, trajectories)
     t0 = time.perf_counter()
     qs.time_step(x, 0.0, T, dt, 10 ** 7)
     qa.synchronize()
@@ -384,7 +396,7 @@ def mcwf(args):
     qs.destroy()
     return {"metric": "MCWF trajectory RK4 steps/s (ensemble average vs density matrix)",
             "value": round(M * steps / el, 1), "unit": "trajectory RK4 steps/s", "n_gpus": 1, "steps": steps,
-            "warmup": 0, "ms_per_step": round(1e3 * el / steps, 4), "higher_is_better": True, "scaling": "weak",
+            "warmup": 200, "ms_per_step": round(1e3 * el / steps, 4), "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": "fp64", "data": "synthetic",
             "config": {"workload": f"Jaynes-Cummings 2x10 wavefunctions, {M} trajectories, RK4 dt={dt} to t={T}"},
             "populations_mcwf": got, "populations_dm": [float(v) for v in ref],

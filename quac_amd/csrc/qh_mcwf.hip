@@ -116,17 +116,44 @@ int ens_work(int64_t n) {
 const int ENS_W = 64;
 const int ENS_MAXNZ = 16;
 
+// row product over all NZ register entries (entries past the row's end have
+// coefficient zero and point at an amplitude of the segment), so that the LDS
+// reads of a stage are issued together
 template <int NZ>
-__device__ inline double2 ens_row_product(const int32_t *c, const double2 *a, int nz, const double2 *s) {
+__device__ inline double2 ens_row_product(const int32_t *c, const double2 *a, const double2 *s) {
+  double2 q[NZ];
+#pragma unroll
+  for (int e = 0; e < NZ; e++) q[e] = s[c[e]];
   double re = 0.0, im = 0.0;
 #pragma unroll
-  for (int e = 0; e < NZ; e++)
-    if (e < nz) {
-      const double2 q = s[c[e]];
-      re = fma(a[e].x, q.x, fma(-a[e].y, q.y, re));
-      im = fma(a[e].x, q.y, fma(a[e].y, q.x, im));
-    }
+  for (int e = 0; e < NZ; e++) {
+    re = fma(a[e].x, q[e].x, fma(-a[e].y, q[e].y, re));
+    im = fma(a[e].x, q[e].y, fma(a[e].y, q[e].x, im));
+  }
   return make_double2(re, im);
+}
+
+// v of the lane that DPP control CTRL selects within the lane's row of 16
+template <int CTRL>
+__device__ inline double dpp_f64(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
+// sum of v over the aligned segment of S lanes, the bit-identical value in every
+// lane of the segment (each exchange adds two partial sums that the partner
+// lane adds in the other order): DPP quad_perm [1,0,3,2], [2,3,0,1], the
+// half-row mirror and the row mirror inside a row of 16, then lane shuffles
+template <int S>
+__device__ inline double seg_sum(double v) {
+  if (S >= 2) v += dpp_f64<0xB1>(v);
+  if (S >= 4) v += dpp_f64<0x4E>(v);
+  if (S >= 8) v += dpp_f64<0x141>(v);
+  if (S >= 16) v += dpp_f64<0x140>(v);
+  if (S >= 32) v += __shfl_xor(v, 16, 64);
+  if (S >= 64) v += __shfl_xor(v, 32, 64);
+  return v;
 }
 
 template <int S, int NZ>
@@ -146,16 +173,16 @@ __global__ void __launch_bounds__(256) k_mcwf_run(int32_t N, int64_t M, const in
   const bool on = have && r < N;
   int32_t c[NZ];
   double2 a[NZ];
-  int nz = 0;
-  if (on) {
-    const int32_t beg = rp[r];
-    nz = rp[r + 1] - beg;
+  const int32_t beg = on ? rp[r] : 0;
+  const int nz = on ? rp[r + 1] - beg : 0;
 #pragma unroll
-    for (int e = 0; e < NZ; e++)
-      if (e < nz) {
-        c[e] = col[beg + e];
-        a[e] = val[beg + e];
-      }
+  for (int e = 0; e < NZ; e++) {
+    c[e] = r;
+    a[e] = make_double2(0.0, 0.0);
+    if (e < nz) {
+      c[e] = col[beg + e];
+      a[e] = val[beg + e];
+    }
   }
   double2 *sg = sh[w] + seg * S;
   double2 psi = on ? ens[i * N + r] : make_double2(0.0, 0.0);
@@ -165,7 +192,7 @@ __global__ void __launch_bounds__(256) k_mcwf_run(int32_t N, int64_t M, const in
   auto apply = [&](double2 v) {
     sg[r] = v;
     __builtin_amdgcn_wave_barrier();
-    const double2 q = ens_row_product<NZ>(c, a, nz, sg);
+    const double2 q = ens_row_product<NZ>(c, a, sg);
     __builtin_amdgcn_wave_barrier();
     return q;
   };
@@ -180,9 +207,7 @@ __global__ void __launch_bounds__(256) k_mcwf_run(int32_t N, int64_t M, const in
     const double s6 = h * (1.0 / 6.0), s3 = h * (1.0 / 3.0);
     const double2 nx = make_double2(psi.x + s6 * k1.x + s3 * k2.x + s3 * k3.x + s6 * k4.x,
                                     psi.y + s6 * k1.y + s3 * k2.y + s3 * k3.y + s6 * k4.y);
-    double n2 = on ? nx.x * nx.x + nx.y * nx.y : 0.0;
-#pragma unroll
-    for (int o = S / 2; o > 0; o >>= 1) n2 += __shfl_xor(n2, o, 64);
+    const double n2 = seg_sum<S>(on ? nx.x * nx.x + nx.y * nx.y : 0.0);
     if (act) {
       psi = nx;
       j++;

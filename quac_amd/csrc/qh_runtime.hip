@@ -66,21 +66,23 @@ int qh_scratch(size_t dbytes, void **d, size_t hbytes, void **h) {
   int rc = qh_ensure_init();
   if (rc) return rc;
   if (d && dbytes > g_sdev_n) {
+    const size_t had = g_sdev_n;  // grown at least twofold, from 1 MiB
     QH_HIP(hipStreamSynchronize(g_stream));
     if (g_sdev) (void)hipFree(g_sdev);
     g_sdev = nullptr;
     g_sdev_n = 0;
-    const size_t n = std::max(dbytes, (size_t)1 << 16);
+    const size_t n = std::max(dbytes, std::max(2 * had, (size_t)1 << 20));
     rc = qh_alloc(&g_sdev, n);
     if (rc) return rc;
     g_sdev_n = n;
   }
   if (h && hbytes > g_shost_n) {
+    const size_t had = g_shost_n;
     QH_HIP(hipStreamSynchronize(g_stream));
     if (g_shost) (void)hipHostFree(g_shost);
     g_shost = nullptr;
     g_shost_n = 0;
-    const size_t n = std::max(hbytes, (size_t)1 << 16);
+    const size_t n = std::max(hbytes, std::max(2 * had, (size_t)1 << 20));
     hipError_t e = hipHostMalloc(&g_shost, n, hipHostMallocDefault);
     if (e != hipSuccess) {
       qh_set_error("hipHostMalloc(%zu) failed: %s", n, hipGetErrorString(e));
