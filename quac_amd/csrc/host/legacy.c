@@ -1,0 +1,422 @@
+/* legacy.c -- QuaC's original global-operator API (src/operators.c,
+ * src/dm_utilities.c, src/solver.c) over one process-wide qsystem.
+ *
+ * The reference builds its global superoperator full_A (and the Schrodinger
+ * ham_A) term by term with the same row rules as the qsystem path: add_to_ham
+ * adds -i a (I (x) H) + i a (H^T (x) I) (operators.c:290-337), add_lin adds the
+ * Lindblad dissipator of one operator (operators.c:679-719), the mult2/mult3
+ * forms the term of the operator product, a VEC pair being |p1><p2|
+ * (operators.c:405-488, 601-663, 743-821).  Here every such call records the
+ * same term in the process-wide qsystem, and construct_matrix assembles it on
+ * the device before the first state is made or evolved, and again when terms
+ * were added since.  The legacy time_step runs TSRK3BS with PETSc's default
+ * tolerances (solver.c:236-514 sets none), so unless the options say otherwise
+ * the adaptive step uses atol = rtol = 1e-4 and no step-size bounds; the final
+ * step is matched to time_max (the reference steps over it). */
+#include <complex.h>
+#include <math.h>
+#include <string.h>
+
+#include "quac_internal.h"
+#include "quac_legacy.h"
+
+int nid = 0;
+int np = 1;
+PetscInt total_levels = 1;
+int num_subsystems = 0;
+operator subsystem_list[MAX_SUB];
+
+static qsystem g_sys = NULL;
+static int g_dirty = 0;       /* terms added since the last assembly */
+static int g_stiff = 0;       /* add_to_ham_stiff* used (refused by time_step) */
+static PetscErrorCode (*g_mon)(TS, PetscInt, PetscReal, Vec, void *) = NULL;
+static void *g_monctx = NULL;
+
+/* states made by create_full_dm / create_dm: the Vec handle and its qvec */
+#define LEGACY_MAXSTATE 256
+static struct {
+  Vec v;
+  qvec q;
+} g_state[LEGACY_MAXSTATE];
+
+/* time-dependent coefficient functions of add_to_ham_time_dep (double f(double)) */
+#define LEGACY_MAXTD 100
+static struct {
+  double (*f)(double);
+} g_td[LEGACY_MAXTD];
+static int g_ntd = 0;
+
+qsystem quac_legacy_system(void) { return g_sys; }
+
+/* operators.c:996-1046: the first operator creates the system */
+static int ensure_system(void) {
+  if (g_sys) return 0;
+  initialize_system(&g_sys);
+  if (!g_sys) {
+    quac_fail(9, "cannot create the system");
+    return -1;
+  }
+  total_levels = 1;
+  num_subsystems = 0;
+  return 0;
+}
+
+static int ensure_terms_allowed(void) {
+  if (g_sys) return 0;
+  quac_fail(0, "You need to create operators before you add terms!");
+  return -1;
+}
+
+/* assemble (again) when terms were added since the last assembly */
+static int ensure_matrix(void) {
+  if (!g_sys) {
+    quac_fail(0, "You need to create operators and add terms first!");
+    return -1;
+  }
+  if (g_sys->mat_allocated == 1 && !g_dirty) return 0;
+  construct_matrix(g_sys);
+  if (g_sys->mat_allocated != 1) return -1; /* construct_matrix recorded the error */
+  g_dirty = 0;
+  return 0;
+}
+
+static int remember(Vec v, qvec q) {
+  for (int i = 0; i < LEGACY_MAXSTATE; i++)
+    if (!g_state[i].v) {
+      g_state[i].v = v;
+      g_state[i].q = q;
+      return 0;
+    }
+  quac_fail(9, "more than %d legacy states alive", LEGACY_MAXSTATE);
+  return -1;
+}
+
+static qvec lookup(Vec v) {
+  for (int i = 0; i < LEGACY_MAXSTATE; i++)
+    if (g_state[i].v == v && v) return g_state[i].q;
+  return NULL;
+}
+
+/* ---------------------------------------------------------------- operators */
+void create_op(int number_of_levels, operator *new_op) {
+  if (ensure_system()) return;
+  if (num_subsystems == MAX_SUB) QUAC_FAIL(0, "More than %d subsystems", MAX_SUB);
+  create_op_sys(g_sys, number_of_levels, new_op);
+  if (!*new_op) return;
+  subsystem_list[num_subsystems++] = *new_op;
+  total_levels = g_sys->total_levels;
+}
+
+void create_vec(int number_of_levels, vec_op *new_vec) {
+  if (ensure_system()) return;
+  if (num_subsystems == MAX_SUB) QUAC_FAIL(0, "More than %d subsystems", MAX_SUB);
+  create_vec_op_sys(g_sys, number_of_levels, new_vec);
+  if (!*new_vec) return;
+  subsystem_list[num_subsystems++] = (*new_vec)[0];
+  total_levels = g_sys->total_levels;
+}
+
+void no_lindblad_terms(void) {} /* operators.c:278-280: a hint; the solver follows the terms */
+
+void print_dense_ham(void) {} /* operators.c:39-48: the dense text dump is not produced */
+
+void set_initial_pop(operator op, double initial_pop) { op->initial_pop = initial_pop; }
+
+/* operators.c:1048-1096: VEC operators multiply only with VEC operators of the
+ * same subsystem */
+static int check_pair(operator op1, operator op2) {
+  const char *why = NULL;
+  if ((op1->my_op_type == VEC) != (op2->my_op_type == VEC))
+    why = "Multiplying a VEC_OP and a regular OP does not make sense!";
+  else if (op1->my_op_type == VEC && op1->n_before != op2->n_before)
+    why = "Multiplying two VEC_OPs from different subspaces does not make sense!";
+  if (!why) return 0;
+  quac_fail(0, "%s", why);
+  return -1;
+}
+
+void add_to_ham(PetscScalar a, operator op) {
+  if (ensure_terms_allowed()) return;
+  if (cabs(a) == 0) return; /* operators.c:297: zero terms are skipped */
+  add_ham_term(g_sys, a, 1, op);
+  g_dirty = 1;
+}
+
+void add_to_ham_stiff(PetscScalar a, operator op) {
+  g_stiff = 1;
+  add_to_ham(a, op);
+}
+
+void add_to_ham_mult2(PetscScalar a, operator op1, operator op2) {
+  if (ensure_terms_allowed() || check_pair(op1, op2)) return;
+  add_ham_term(g_sys, a, 2, op1, op2);
+  g_dirty = 1;
+}
+
+void add_to_ham_stiff_mult2(PetscScalar a, operator op1, operator op2) {
+  g_stiff = 1;
+  add_to_ham_mult2(a, op1, op2);
+}
+
+void add_to_ham_mult3(PetscScalar a, operator op1, operator op2, operator op3) {
+  if (ensure_terms_allowed()) return;
+  add_ham_term(g_sys, a, 3, op1, op2, op3);
+  g_dirty = 1;
+}
+
+void add_lin(PetscScalar a, operator op) {
+  if (ensure_terms_allowed()) return;
+  add_lin_term(g_sys, a, 1, op);
+  g_dirty = 1;
+}
+
+void add_lin_mult2(PetscScalar a, operator op1, operator op2) {
+  if (ensure_terms_allowed() || check_pair(op1, op2)) return;
+  add_lin_term(g_sys, a, 2, op1, op2);
+  g_dirty = 1;
+}
+
+void add_lin_mat(PetscScalar a, Mat m) {
+  (void)a;
+  (void)m;
+  QUAC_FAIL(9, "add_lin_mat: Lindblad terms from an arbitrary matrix are not supported (use operator terms)");
+}
+
+static void td_value(PetscReal t, void *ctx, PetscReal *re, PetscReal *im) {
+  *re = g_td[(intptr_t)ctx].f(t);
+  *im = 0.0;
+}
+
+/* operators.c:253-276: H(t) += f(t) op1 op2 ... */
+void add_to_ham_time_dep(double (*time_dep_func)(double), int num_ops, ...) {
+  if (ensure_terms_allowed()) return;
+  if (g_ntd == LEGACY_MAXTD) QUAC_FAIL(9, "more than %d time-dependent terms", LEGACY_MAXTD);
+  if (num_ops < 1 || num_ops > 64) QUAC_FAIL(9, "add_to_ham_time_dep: 1..64 operators");
+  operator ops[64];
+  va_list ap;
+  va_start(ap, num_ops);
+  for (int i = 0; i < num_ops; i++) ops[i] = va_arg(ap, operator);
+  va_end(ap);
+  g_td[g_ntd].f = time_dep_func;
+  add_ham_term_time_dep_list(g_sys, 1.0, (void *)(intptr_t)g_ntd, td_value, num_ops, ops);
+  g_ntd++;
+  g_dirty = 1;
+}
+
+/* ------------------------------------------------------------ density matrices */
+/* dm_utilities.c:635-655: a zero state of the system's dimension (a density
+ * matrix when there are Lindblad terms, else a wavefunction) */
+void create_full_dm(Vec *new_dm) {
+  *new_dm = NULL;
+  if (!g_sys) QUAC_FAIL(0, "You need to create operators before you create a density matrix!");
+  qvec q = NULL;
+  create_qvec_sys(g_sys, &q);
+  if (!q) return;
+  if (remember(q->data, q)) {
+    destroy_qvec(&q);
+    return;
+  }
+  *new_dm = q->data;
+}
+
+/* dm_utilities.c:616-623: a zero density matrix of size x size */
+void create_dm(Vec *new_dm, PetscInt size) {
+  *new_dm = NULL;
+  qvec q = NULL;
+  create_arb_qvec(&q, size * size, DENSITY_MATRIX);
+  if (!q) return;
+  if (remember(q->data, q)) {
+    destroy_qvec(&q);
+    return;
+  }
+  *new_dm = q->data;
+}
+
+void destroy_dm(Vec dm) {
+  for (int i = 0; i < LEGACY_MAXSTATE; i++)
+    if (g_state[i].v == dm && dm) {
+      destroy_qvec(&g_state[i].q);
+      g_state[i].v = NULL;
+      g_state[i].q = NULL;
+      return;
+    }
+}
+
+#define LOOKUP_OR_FAIL(q, v)                                                       \
+  qvec q = lookup(v);                                                              \
+  if (!q) QUAC_FAIL(9, "not a state made by create_full_dm / create_dm")
+
+/* population of level l of subsystem s in the initial product state */
+static double sub_pop(operator s, PetscInt l) {
+  if (s->my_op_type != VEC) return l == (PetscInt)s->initial_pop ? 1.0 : 0.0;
+  double tot = 0.0;
+  for (PetscInt j = 0; j < s->my_levels; j++) tot += s->vec_op_list[j]->initial_pop;
+  if (tot == 0.0) return l == 0 ? 1.0 : 0.0; /* dm_utilities.c:767-773 */
+  double p = 0.0;
+  for (PetscInt j = 0; j < s->my_levels; j++)
+    if (s->vec_op_list[j]->position == l) p += s->vec_op_list[j]->initial_pop;
+  return p / tot;
+}
+
+/* dm_utilities.c:665-804: the product state of the subsystems' initial
+ * populations (ladder: the level initial_pop; VEC: the populations of its
+ * levels, normalised).  Without VEC subsystems one entry is set; otherwise
+ * the whole density matrix is written (diagonal, off-diagonals zero). */
+void set_dm_from_initial_pop(Vec x) {
+  LOOKUP_OR_FAIL(q, x);
+  const int dm = q->my_type == DENSITY_MATRIX;
+  int simple = 1;
+  for (int i = 0; i < num_subsystems; i++)
+    if (subsystem_list[i]->my_op_type == VEC) simple = 0;
+  if (simple) {
+    PetscInt row = 0;
+    for (int i = 0; i < num_subsystems; i++) {
+      const PetscInt n_after = total_levels / (subsystem_list[i]->my_levels * subsystem_list[i]->n_before);
+      row += (PetscInt)subsystem_list[i]->initial_pop * n_after;
+    }
+    const PetscInt loc = dm ? total_levels * row + row : row;
+    double re = 0.0, im = 0.0;
+    QUAC_HIPCHK(qh_vec_get_entry(x, loc, &re, &im));
+    QUAC_HIPCHK(qh_vec_add_entry(x, loc, 1.0 - re, -im)); /* INSERT_VALUES */
+    return;
+  }
+  if (!dm) QUAC_FAIL(9, "set_dm_from_initial_pop: VEC populations need a density matrix (Lindblad terms)");
+  const PetscInt N = total_levels;
+  double *h = (double *)calloc((size_t)(2 * N * N), sizeof(double));
+  if (!h) QUAC_FAIL(9, "out of host memory");
+  for (PetscInt r = 0; r < N; r++) {
+    double p = 1.0;
+    for (int i = 0; i < num_subsystems && p != 0.0; i++) {
+      operator s = subsystem_list[i];
+      const PetscInt n_after = N / (s->my_levels * s->n_before);
+      p *= sub_pop(s, (r / n_after) % s->my_levels);
+    }
+    h[2 * (N * r + r)] = p;
+  }
+  const int rc = qh_vec_set(x, h);
+  free(h);
+  if (rc) QUAC_FAIL(9, "set_dm_from_initial_pop: %s", qh_last_error());
+}
+
+/* dm_utilities.c:965-981 (location sqrt(size) col + row); *val is written */
+void get_dm_element(Vec dm, PetscInt row, PetscInt col, PetscScalar *val) {
+  LOOKUP_OR_FAIL(q, dm);
+  get_dm_element_qvec(q, row, col, val);
+}
+
+/* dm_utilities.c:1049-1061 */
+void add_value_to_dm(Vec dm, PetscInt row, PetscInt col, PetscScalar val) {
+  LOOKUP_OR_FAIL(q, dm);
+  const PetscInt n = q->total_levels;
+  QUAC_HIPCHK(qh_vec_add_entry(dm, n * col + row, creal(val), cimag(val)));
+}
+
+void assemble_dm(Vec dm) { (void)dm; } /* values are in device memory as soon as they are added */
+
+/* dm_utilities.c:1127-1143: one population per ladder subsystem, one per level
+ * of a VEC subsystem */
+int get_num_populations(void) {
+  int n = 0;
+  for (int i = 0; i < num_subsystems; i++) n += subsystem_list[i]->my_op_type == VEC ? (int)subsystem_list[i]->my_levels : 1;
+  return n;
+}
+
+/* dm_utilities.c:1157-1325: <n> of each ladder subsystem and the population of
+ * each level of each VEC subsystem, from the diagonal (|psi_i|^2 for a
+ * wavefunction); (*populations) is the caller's array of get_num_populations() */
+void get_populations(Vec x, double **populations) {
+  LOOKUP_OR_FAIL(q, x);
+  PetscInt nl = 0;
+  PetscReal *probs = NULL, *vars = NULL;
+  get_bitstring_probs(q, &nl, &probs, &vars);
+  if (!probs) return;
+  int *first = (int *)malloc(sizeof(int) * (size_t)(num_subsystems + 1));
+  int num_pop = 0;
+  for (int i = 0; i < num_subsystems; i++) {
+    first[i] = num_pop;
+    num_pop += subsystem_list[i]->my_op_type == VEC ? (int)subsystem_list[i]->my_levels : 1;
+  }
+  for (int i = 0; i < num_pop; i++) (*populations)[i] = 0.0;
+  for (PetscInt r = 0; r < nl; r++)
+    for (int j = 0; j < num_subsystems; j++) {
+      operator s = subsystem_list[j];
+      const PetscInt n_after = total_levels / (s->my_levels * s->n_before);
+      const PetscInt cur = (r / n_after) % s->my_levels;
+      if (s->my_op_type == VEC) (*populations)[first[j] + cur] += probs[r];
+      else (*populations)[first[j]] += (double)cur * probs[r];
+    }
+  free(first);
+  free(probs);
+  free(vars);
+}
+
+/* dm_utilities.c:1352-1469: Tr(O rho) of the operator product (<psi|O|psi> for
+ * a wavefunction) */
+void get_expectation_value(Vec rho, PetscScalar *trace_val, int number_of_ops, ...) {
+  LOOKUP_OR_FAIL(q, rho);
+  if (number_of_ops < 1 || number_of_ops > 64) QUAC_FAIL(9, "get_expectation_value: 1..64 operators");
+  operator ops[64];
+  va_list ap;
+  va_start(ap, number_of_ops);
+  for (int i = 0; i < number_of_ops; i++) ops[i] = va_arg(ap, operator);
+  va_end(ap);
+  get_expectation_value_qvec_list(q, trace_val, number_of_ops, ops);
+}
+
+/* dm_utilities.c:1766-1786 */
+void trace_dm(PetscScalar *trace_val, Vec dm) {
+  LOOKUP_OR_FAIL(q, dm);
+  get_trace_qvec(q, trace_val);
+}
+
+/* ------------------------------------------------------------------- solvers */
+void set_ts_monitor(PetscErrorCode (*monitor)(TS, PetscInt, PetscReal, Vec, void *)) {
+  g_mon = monitor;
+  g_monctx = NULL;
+}
+
+void set_ts_monitor_ctx(PetscErrorCode (*monitor)(TS, PetscInt, PetscReal, Vec, void *), void *ctx) {
+  g_mon = monitor;
+  g_monctx = ctx;
+}
+
+/* set name to value for the duration of a call unless the caller set it */
+typedef struct {
+  const char *name;
+  int was_set;
+} opt_default;
+
+static void push_default(opt_default *d, const char *name, const char *value) {
+  d->name = name;
+  d->was_set = quac_get_option(name) != NULL;
+  if (!d->was_set) quac_set_option(name, value);
+}
+
+static void pop_default(const opt_default *d) {
+  if (!d->was_set) quac_unset_option(d->name);
+}
+
+/* solver.c:236-514 */
+void time_step(Vec x, PetscReal init_time, PetscReal time_max, PetscReal dt, PetscInt steps_max) {
+  if (g_stiff)
+    QUAC_FAIL(0, "%s", g_sys && g_sys->dm_equations ? "Lindblad-stiff solver untested." : "Stiff solver not implemented!");
+  if (ensure_matrix()) return;
+  LOOKUP_OR_FAIL(q, x);
+  set_ts_monitor_sys(g_sys, g_mon, g_monctx);
+  opt_default d[4];
+  push_default(&d[0], "ts_atol", "1e-4");
+  push_default(&d[1], "ts_rtol", "1e-4");
+  push_default(&d[2], "ts_adapt_dt_min", "1e-20");
+  push_default(&d[3], "ts_adapt_dt_max", "1e20");
+  time_step_sys(g_sys, q, init_time, time_max, dt, steps_max);
+  for (int i = 3; i >= 0; i--) pop_default(&d[i]);
+}
+
+/* solver.c:33-221: the stabilised Liouvillian solve of the qsystem path */
+void steady_state(Vec x) {
+  if (!g_sys || !g_sys->dm_equations)
+    QUAC_FAIL(0, "Warning! Steady state not supported for Schrodinger.");
+  if (ensure_matrix()) return;
+  LOOKUP_OR_FAIL(q, x);
+  steady_state_sys(g_sys, q);
+}

@@ -57,6 +57,19 @@ void quac_set_option(const char *name, const char *value) {
   g_nopt++;
 }
 
+void quac_unset_option(const char *name) {
+  if (name && name[0] == '-') name++;
+  for (int i = 0; i < g_nopt; i++)
+    if (!strcmp(g_opt_name[i], name)) {
+      g_nopt--;
+      if (i != g_nopt) {
+        memcpy(g_opt_name[i], g_opt_name[g_nopt], sizeof(g_opt_name[0]));
+        memcpy(g_opt_val[i], g_opt_val[g_nopt], sizeof(g_opt_val[0]));
+      }
+      return;
+    }
+}
+
 const char *quac_get_option(const char *name) {
   if (name && name[0] == '-') name++;
   for (int i = 0; i < g_nopt; i++)

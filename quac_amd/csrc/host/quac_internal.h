@@ -44,7 +44,8 @@ void quac_mcwf_free(qsystem s);
 int quac_mcwf_time_step(qsystem s, qvec x, PetscReal t0, PetscReal tmax, PetscReal dt, PetscInt steps_max,
                         int (*monitor)(qsystem, PetscInt, PetscReal, qvec));
 
-/* options */
+/* options (quac_unset_option: forget a name set by quac_set_option) */
+void quac_unset_option(const char *name);
 int quac_opt_int(const char *name, int dflt);
 double quac_opt_double(const char *name, double dflt);
 const char *quac_opt_str(const char *name, const char *dflt);

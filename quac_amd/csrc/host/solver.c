@@ -1,7 +1,8 @@
 /* solver.c -- steady state of a qsystem (solver.c:33-220 semantics).
  *
  * The reference offers steady_state(Vec) only for its legacy global-operator
- * API; here it works on a qsystem: the trace row is added to row 0 of the
+ * API (provided over a process-wide qsystem in legacy.c); here it works on a
+ * qsystem: the trace row is added to row 0 of the
  * Liouvillian, b = e0, GMRES(restart) with rtol 1e-11 by default
  * (solver.c:13-15), on the device in FP64.  The initial guess is whatever the
  * qvec holds (zero for a fresh qvec, matching the reference's effective x=0).

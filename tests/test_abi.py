@@ -1,6 +1,7 @@
 """CPU checks of the drop-in boundary: libquac.so builds, loads, exports every
-function that include/quac.h and include/quac_hip.h declare, and the product
-path fails loudly (no fallback) when no HIP device is present."""
+function that include/quac.h, include/quac_hip.h and include/quac_legacy.h
+declare (and the legacy API's globals), and the product path fails loudly (no
+fallback) when no HIP device is present."""
 import os
 import re
 import subprocess
@@ -40,13 +41,30 @@ def exported():
     return {l.split()[-1] for l in out.splitlines() if " T " in l}
 
 
-@pytest.mark.parametrize("header", ["quac.h", "quac_hip.h"])
+@pytest.mark.parametrize("header", ["quac.h", "quac_hip.h", "quac_legacy.h"])
 def test_every_declared_symbol_is_exported(header):
     assert os.path.exists(LIB), "build libquac.so first (python -m quac_amd.build)"
     names = declared(header)
     assert len(names) > 20
     missing = [n for n in names if n not in exported()]
     assert not missing, missing
+
+
+def test_legacy_globals_are_exported():
+    """operators.h:60-66 / quac.c:11-12: nid, np, total_levels, num_subsystems,
+    subsystem_list are data symbols of the library"""
+    out = subprocess.run(["nm", "-D", "--defined-only", LIB], stdout=subprocess.PIPE, text=True, check=True).stdout
+    data = {l.split()[-1] for l in out.splitlines() if l.split()[1] in ("B", "D")}
+    for n in ("nid", "np", "total_levels", "num_subsystems", "subsystem_list"):
+        assert n in data, n
+
+
+def test_legacy_programs_compile_against_the_headers(tmp_path):
+    """the legacy C test programs build against include/ and link with libquac.so"""
+    for prog in ("legacy_jc", "legacy_vec", "legacy_steady"):
+        subprocess.check_call(["gcc", "-O2", "-std=c11", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"),
+                               os.path.join(ROOT, "tests", "c", prog + ".c"), "-o", str(tmp_path / prog),
+                               "-L", os.path.dirname(LIB), "-lquac", "-Wl,-rpath," + os.path.dirname(LIB), "-lm"])
 
 
 def test_loads_and_fails_loudly_without_device():

@@ -1,6 +1,7 @@
 """The C drop-in path: a reference-style C program (variadic API, monitor with
 the Vec-aliasing idiom) compiled with gcc against include/quac.h and linked
 with libquac.so; its populations must match the oracle's FP64 RK4 to 1e-10."""
+import math
 import os
 import subprocess
 
@@ -39,3 +40,69 @@ def test_jc_c_program_matches_oracle(tmp_path):
         assert abs(float(row[2]) - pe) < 1e-10 and abs(float(row[3]) - pc) < 1e-10, (row, pe, pc)
     tr = [l for l in out.stdout.splitlines() if l.startswith("trace")][0].split()[1]
     assert abs(float(tr) - 1.0) < 1e-12
+
+
+def _build(tmp_path, prog):
+    exe = tmp_path / prog
+    libdir = os.path.join(ROOT, "quac_amd")
+    subprocess.check_call(["gcc", "-O2", "-std=c11", "-I", os.path.join(ROOT, "include"),
+                           os.path.join(ROOT, "tests", "c", prog + ".c"), "-o", str(exe),
+                           "-L", libdir, "-lquac", "-Wl,-rpath," + libdir, "-lm"])
+    return exe
+
+
+def _run(exe, *opts):
+    out = subprocess.run([str(exe), *opts], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout
+    return out.stdout
+
+
+def test_legacy_api_jc_program_matches_oracle(tmp_path):
+    """the legacy global-operator API (create_op / add_to_ham / add_to_ham_mult2 /
+    add_lin / set_initial_pop / create_full_dm / set_dm_from_initial_pop /
+    set_ts_monitor / time_step / get_populations) on the JC config: populations
+    every 100 RK4 steps within 1e-10 of the oracle's FP64 RK4"""
+    text = _run(_build(tmp_path, "legacy_jc"), "-ts_rk_type", "4", "-ts_adapt_type", "none")
+    rows = [l.split() for l in text.splitlines() if l and l[0].isdigit()]
+    assert [int(r[0]) for r in rows] == list(range(0, 1001, 100))
+    r = recipes.jc_recipe()
+    tot, terms = recipes.oracle_terms(r)
+    m = oracle.construct(tot, terms, dm=True)
+    N, loc = recipes.jc_init()
+    x = np.zeros(N * N, np.complex128)
+    x[loc] = 1
+    for k, row in enumerate(rows):
+        if k:
+            x = m.rk4(x, 0.01, 100)
+        pe = oracle.expect_dm(tot, x, [(0, 2, 1, -1)]).real
+        pc = oracle.expect_dm(tot, x, [(0, 10, 2, -1)]).real
+        assert abs(float(row[2]) - pe) < 1e-10 and abs(float(row[3]) - pc) < 1e-10, (row, pe, pc)
+    ex = float([l for l in text.splitlines() if l.startswith("expect")][0].split()[1])
+    assert abs(ex - float(rows[-1][3])) < 1e-12
+    tr = [l for l in text.splitlines() if l.startswith("trace")][0].split()
+    assert abs(float(tr[1]) - 1.0) < 1e-12 and tr[2:] == ["2", "20"]
+
+
+@pytest.mark.parametrize("opts,tol", [(("-ts_rk_type", "4", "-ts_adapt_type", "none"), 1e-9), ((), 2e-3)])
+def test_legacy_api_vec_decay_closed_form(tmp_path, opts, tol):
+    """a VEC three-level atom: |1> decays to |0> at rate 0.5 through the VEC-pair
+    Lindblad operator |0><1| (add_lin_mult2), initial populations 3/4 in |1>, 1/4
+    in |2> from set_initial_pop; at t = 2: p1 = 3/4 e^-1, p0 = 3/4 (1 - e^-1),
+    p2 = 1/4, cavity <n> = 0.  Fixed-step RK4 to 1e-9; the legacy default
+    (adaptive RK3BS at PETSc's default tolerances 1e-4) to 2e-3"""
+    text = _run(_build(tmp_path, "legacy_vec"), *opts)
+    f = [l for l in text.splitlines() if l.startswith("pops")][0].split()
+    assert f[1] == "4"
+    p0, p1, p2, nc = map(float, f[2:])
+    e = math.exp(-1.0)
+    assert abs(p1 - 0.75 * e) < tol and abs(p0 - 0.75 * (1 - e)) < tol and abs(p2 - 0.25) < tol and abs(nc) < tol, f
+
+
+def test_legacy_api_steady_state_closed_form(tmp_path):
+    """legacy steady_state(Vec) of a driven, detuned, damped qubit: rho_ee within
+    1e-6 of (omega^2/4) / (delta^2 + gamma^2/4 + omega^2/2), unit trace"""
+    text = _run(_build(tmp_path, "legacy_steady"))
+    f = [l for l in text.splitlines() if l.startswith("steady")][0].split()
+    omega, delta, gamma = 0.4, 0.3, 1.0
+    ree = (omega ** 2 / 4) / (delta ** 2 + gamma ** 2 / 4 + omega ** 2 / 2)
+    assert abs(float(f[1]) - ree) < 1e-6 and abs(float(f[2]) - 1.0) < 1e-9, (f, ree)
