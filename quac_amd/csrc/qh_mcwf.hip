@@ -105,7 +105,7 @@ int ens_work(int64_t n) {
 // ------------------------------------------------- trajectories of up to 64 amplitudes
 // A wavefront integrates 64 / S trajectories side by side, one segment of S
 // lanes (S a power of two >= N) per trajectory: lane r of a segment holds
-// amplitude r and row r of the generator (up to NZ entries in registers), and
+// amplitude r and row r of the generator (NZ entries in registers), and
 // the products exchange the amplitudes through the segment's part of the
 // wavefront's LDS row.  A trajectory runs classic RK4 steps (size dt, the last
 // of the run hlast) from its step index jst[i] up to J and stops right after a
@@ -323,15 +323,22 @@ extern "C" int qh_mcwf_advance(const qh_mat *A, qh_vec *ens, int64_t M, const do
   const int S = N <= 2 ? 2 : N <= 4 ? 4 : N <= 8 ? 8 : N <= 16 ? 16 : N <= 32 ? 32 : 64;
   const int64_t waves = (M + ENS_W / S - 1) / (ENS_W / S);
   const dim3 grid((unsigned)((waves + 3) / 4));
-  const bool short_rows = A->max_row <= 4;
+  // generator entries per lane: the row length rounded up to 2, 3, 4 or 16
+  // (each padded entry costs four FP64 FMAs per stage)
+  const int nzc = A->max_row <= 2 ? 2 : A->max_row <= 3 ? 3 : A->max_row <= 4 ? 4 : ENS_MAXNZ;
+#define QH_MCWF_RUN_NZ(SS, NZ)                                                                                   \
+  hipLaunchKernelGGL((k_mcwf_run<SS, NZ>), grid, dim3(256), 0, s, N, M, A->rowptr, A->col, A->val,               \
+                     (double2 *)ens->d, dthr, djs, J, dt, hlast, djump)
 #define QH_MCWF_RUN(SS)                                                                                          \
   if (S == SS) {                                                                                                  \
-    if (short_rows)                                                                                               \
-      hipLaunchKernelGGL((k_mcwf_run<SS, 4>), grid, dim3(256), 0, s, N, M, A->rowptr, A->col, A->val,            \
-                         (double2 *)ens->d, dthr, djs, J, dt, hlast, djump);                                      \
+    if (nzc == 2)                                                                                                 \
+      QH_MCWF_RUN_NZ(SS, 2);                                                                                      \
+    else if (nzc == 3)                                                                                            \
+      QH_MCWF_RUN_NZ(SS, 3);                                                                                      \
+    else if (nzc == 4)                                                                                            \
+      QH_MCWF_RUN_NZ(SS, 4);                                                                                      \
     else                                                                                                          \
-      hipLaunchKernelGGL((k_mcwf_run<SS, ENS_MAXNZ>), grid, dim3(256), 0, s, N, M, A->rowptr, A->col, A->val,    \
-                         (double2 *)ens->d, dthr, djs, J, dt, hlast, djump);                                      \
+      QH_MCWF_RUN_NZ(SS, ENS_MAXNZ);                                                                              \
   }
   QH_MCWF_RUN(2)
   QH_MCWF_RUN(4)
@@ -340,6 +347,7 @@ extern "C" int qh_mcwf_advance(const qh_mat *A, qh_vec *ens, int64_t M, const do
   QH_MCWF_RUN(32)
   QH_MCWF_RUN(64)
 #undef QH_MCWF_RUN
+#undef QH_MCWF_RUN_NZ
   QH_HIP(hipGetLastError());
   QH_HIP(hipMemcpyAsync(h + bt, djs, 2 * bj, hipMemcpyDeviceToHost, s));
   QH_HIP(hipStreamSynchronize(s));

@@ -122,6 +122,20 @@ def _jc_generator(gam=1.0, kap=0.8, levels=4):
     return qs, qs.matrix()
 
 
+def _qubit_ring_generator(nq):
+    """-i Heff of nq qubits, sigma_x on each and sigma_z sigma_z neighbours, decay
+    of each: every row holds the diagonal and nq sigma_x entries"""
+    qs = qa.System()
+    q = [qs.create_op(2) for _ in range(nq)]
+    for k in range(nq):
+        qs.add_ham_term(0.3 + 0.1 * k, q[k].sig_x)
+        qs.add_ham_term(0.2, q[k].sig_z, q[(k + 1) % nq].sig_z)
+        qs.add_lin_term(0.6, q[k])
+    qs.use_mcwf_solver(4, 1)
+    qs.construct_matrix()
+    return qs, qs.matrix()
+
+
 def _dense(A):
     rp, col, val = A.export()
     n = len(rp) - 1
@@ -135,16 +149,21 @@ def _P(a, t):
     return a.ctypes.data_as(ctypes.POINTER(t))
 
 
-def test_mcwf_advance_matches_stepwise_ensemble_rk4():
+@pytest.mark.parametrize("system,row", [("jc2x4", 2), ("qubits2", 3), ("qubits3", 4), ("qubits4", 5)])
+def test_mcwf_advance_matches_stepwise_ensemble_rk4(system, row):
     """qh_mcwf_advance (each trajectory runs from its step index to its first
-    norm crossing, one wavefront per trajectory) against the ensemble RK4 kernel
-    taken one step at a time with the crossing found on the host: the stop step
-    of every trajectory exactly, the amplitudes to 1e-12.  Covers thresholds that
-    are never crossed, a trajectory that has already taken all steps, a late
-    start and a shorter last step."""
-    qs, A = _jc_generator()
+    norm crossing, several trajectories per wavefront) against the ensemble RK4
+    kernel taken one step at a time with the crossing found on the host: the
+    stop step of every trajectory exactly, the amplitudes to 1e-12.  Covers
+    thresholds that are never crossed, a trajectory that has already taken all
+    steps, a late start and a shorter last step; generator rows of 3, 4 and 5
+    entries (3-, 4- and 16-entry register layouts; 8 and 16 lanes per
+    trajectory)."""
+    qs, A = _jc_generator() if system == "jc2x4" else _qubit_ring_generator(int(system[-1]))
     L = qa.lib()
     N = A.info()[0]
+    rp, _, _ = A.export()
+    assert int(np.diff(rp).max()) == row
     fits = ctypes.c_int()
     qa._qh(L.qh_mcwf_fits(A.h, 2, ctypes.byref(fits)))
     assert fits.value == 1
@@ -190,7 +209,7 @@ def test_mcwf_advance_matches_stepwise_ensemble_rk4():
         assert jst[i] == exp_stop, (i, jst[i], exp_stop)
         assert jump[i] == (1 if i in frozen else 0), (i, jump[i])
         assert np.abs(got[i] - want).max() < 1e-12, (i, np.abs(got[i] - want).max())
-    assert len(frozen) >= 3 and M - len(frozen) >= 5   # crossings and runs to the end (4 by construction)
+    assert len(frozen) >= 3 and M - len(frozen) >= 4   # crossings and runs to the end (4 by construction)
     # a second call resumes the trajectories that stopped and finishes the rest
     qa._qh(L.qh_mcwf_advance(A.h, ens.h, M, _P(np.zeros(M), ctypes.c_double), _P(jst, ctypes.c_int32), J, dt,
                              hlast, _P(jump, ctypes.c_int32)))
