@@ -202,8 +202,18 @@ void get_wf_element_qvec(qvec, PetscInt, PetscScalar *);
 void get_bitstring_probs(qvec, PetscInt *, PetscReal **, PetscReal **); /* :1682-1717 */
 void read_qvec_dm_binary(qvec *, const char[]);         /* :132-166 */
 void read_qvec_wf_binary(qvec *, const char[]);         /* :97-126 */
-void copy_qvec(qvec, qvec);
+void copy_qvec(qvec, qvec);                              /* :31-50 (WF -> DM as |psi><psi|) */
 void print_qvec(qvec);
+void create_arb_qvec_dims(qvec *, PetscInt, PetscInt *, qvec_type);  /* :523-570 */
+void copy_qvec_wf_to_dm(qvec, qvec);                     /* :57-90 */
+void check_qvec_equal(qvec, qvec, PetscBool *);          /* :26-30 */
+void ptrace_over_list_qvec(qvec, PetscInt, PetscInt *, qvec *);      /* :1591-1656 */
+void get_fidelity_qvec(qvec, qvec, PetscReal *, PetscReal *);        /* :2159-2245 */
+void get_superfidelity_qvec(qvec, qvec, PetscReal *);    /* :2102-2141 */
+void get_hilbert_schmidt_dist_qvec(qvec, qvec, PetscReal *);         /* :1657-1680 */
+void print_dm_qvec(qvec);                                /* :291-305 */
+void print_wf_qvec(qvec);                                /* :397-412 */
+void get_qvec_loc_fock_op(qvec, PetscInt *, PetscInt, ...);          /* :971-993 */
 void print_qvec_file(qvec, const char[]);               /* :203-211 */
 void print_dm_qvec_file(qvec, const char[]);            /* :213-241 */
 void print_wf_qvec_file(qvec, const char[]);            /* :246-268 */

@@ -63,6 +63,10 @@ void time_step(Vec, PetscReal, PetscReal, PetscReal, PetscInt);        /* :236-5
 void set_ts_monitor(PetscErrorCode (*)(TS, PetscInt, PetscReal, Vec, void *));            /* :526-529 */
 void set_ts_monitor_ctx(PetscErrorCode (*)(TS, PetscInt, PetscReal, Vec, void *), void *); /* :540-543 */
 
+/* quac.c:119-141: free an operator (with its dag and n) / a VEC operator array */
+void destroy_op(operator *);
+void destroy_vec(vec_op *);
+
 /* the process-wide system behind the calls above (NULL before the first create_op) */
 qsystem quac_legacy_system(void);
 

@@ -176,6 +176,15 @@ def _setup(L):
     sig("qh_spmv", i32, vp, vp, vp)
     sig("qh_rk3bs_step", i32, vp, vp, vp, vp, vp, vp, vp, vp, dbl, dbl, dbl, i32, P(dbl))
     sig("qh_rk4", i32, vp, vp, dbl, i64, i64, i32)
+    sig("create_arb_qvec_dims", None, P(vp), i64, P(i64), i32)
+    sig("destroy_qvec", None, P(vp))
+    sig("copy_qvec", None, vp, vp)
+    sig("copy_qvec_wf_to_dm", None, vp, vp)
+    sig("check_qvec_equal", None, vp, vp, P(i32))
+    sig("ptrace_over_list_qvec", None, vp, i64, P(i64), P(vp))
+    sig("get_fidelity_qvec", None, vp, vp, P(dbl), P(dbl))
+    sig("get_superfidelity_qvec", None, vp, vp, P(dbl))
+    sig("get_hilbert_schmidt_dist_qvec", None, vp, vp, P(dbl))
     sig("qh_ens_rk4", i32, vp, vp, i64, dbl, i64)
     sig("qh_ens_norms", i32, vp, i64, P(dbl))
     sig("qh_mcwf_fits", i32, vp, i32, P(i32))

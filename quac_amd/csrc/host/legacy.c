@@ -116,6 +116,9 @@ void create_vec(int number_of_levels, vec_op *new_vec) {
   total_levels = g_sys->total_levels;
 }
 
+void destroy_op(operator *op) { destroy_op_sys(op); }
+void destroy_vec(vec_op *op) { destroy_vec_op_sys(op); }
+
 void no_lindblad_terms(void) {} /* operators.c:278-280: a hint; the solver follows the terms */
 
 void print_dense_ham(void) {} /* operators.c:39-48: the dense text dump is not produced */

@@ -104,6 +104,24 @@ void create_arb_qvec(qvec *out, PetscInt n, qvec_type type) {
   *out = q;
 }
 
+/* qvec_utilities.c:523-570: a zero state with the given subsystem dimensions
+ * (a density matrix lists every dimension twice: ndims = 2 x subsystems) */
+void create_arb_qvec_dims(qvec *out, PetscInt ndims, PetscInt *dims, qvec_type type) {
+  if (quac_ensure_device()) return;
+  if (ndims < 1 || (type == DENSITY_MATRIX && ndims % 2)) QUAC_FAIL(9, "create_arb_qvec_dims: bad ndims %lld", (long long)ndims);
+  if (type != WAVEFUNCTION && type != DENSITY_MATRIX) QUAC_FAIL(9, "ERROR! Type not understood in create_arb_qvec_dims.");
+  PetscInt nstates = 1;
+  for (PetscInt i = 0; i < ndims; i++) nstates *= dims[i];
+  const PetscInt tl = type == DENSITY_MATRIX ? (PetscInt)llround(sqrt((double)nstates)) : nstates;
+  qvec q = new_qvec(type, nstates, tl);
+  if (!q) return;
+  q->n_ops = type == DENSITY_MATRIX ? ndims / 2 : ndims;
+  q->ndims_hspace = ndims;
+  q->hspace_dims = (PetscInt *)malloc(sizeof(PetscInt) * (size_t)ndims);
+  for (PetscInt i = 0; i < ndims; i++) q->hspace_dims[i] = dims[i];
+  *out = q;
+}
+
 /* qvec_utilities.c:1076-1090 */
 void destroy_qvec(qvec *state) {
   if (!state || !*state) return;
@@ -304,8 +322,16 @@ static void read_binary(qvec *out, const char *fname, qvec_type type) {
 void read_qvec_dm_binary(qvec *out, const char fname[]) { read_binary(out, fname, DENSITY_MATRIX); }
 void read_qvec_wf_binary(qvec *out, const char fname[]) { read_binary(out, fname, WAVEFUNCTION); }
 
+/* qvec_utilities.c:31-50: same type (and size), or a wavefunction into a
+ * density matrix (|psi><psi|) */
 void copy_qvec(qvec src, qvec dst) {
-  if (src->n != dst->n) QUAC_FAIL(9, "copy_qvec: sizes differ");
+  if (src->my_type == WAVEFUNCTION && dst->my_type == DENSITY_MATRIX) {
+    copy_qvec_wf_to_dm(src, dst);
+    return;
+  }
+  if (src->my_type == DENSITY_MATRIX && dst->my_type == WAVEFUNCTION)
+    QUAC_FAIL(1, "Coping a DM to a WF does not make sense!");
+  if (src->n != dst->n) QUAC_FAIL(1, "qvecs have different sizes!");
   QUAC_HIPCHK(qh_vec_copy(src->data, dst->data));
 }
 

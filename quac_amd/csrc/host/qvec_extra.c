@@ -1,0 +1,427 @@
+/* qvec_extra.c -- state comparisons and reductions of qvec_utilities.c that lie
+ * beside the stepping path: |psi><psi|, partial traces, fidelity, superfidelity,
+ * Hilbert-Schmidt distance, equality, text printers and the (op, fock) location
+ * helper.  They copy the state(s) to the host and work there (like the
+ * reference, which gathers the state to rank 0 for most of them).
+ *
+ * Density-matrix element (row, col) is entry N col + row of the vector; in a
+ * composite index subsystem 0 is the most significant digit. */
+#include <complex.h>
+#include <math.h>
+#include <string.h>
+
+#include "quac_internal.h"
+
+/* the state as host complex128 (caller frees) */
+static double complex *host_state(qvec q) {
+  double complex *h = (double complex *)malloc(sizeof(double complex) * (size_t)q->n);
+  if (!h) {
+    quac_fail(9, "out of host memory");
+    return NULL;
+  }
+  if (qh_vec_get(q->data, (double *)h)) {
+    free(h);
+    quac_fail(9, "state copy failed: %s", qh_last_error());
+    return NULL;
+  }
+  return h;
+}
+
+/* qvec_utilities.c:57-90: destination (N^2) = |psi><psi| of source (N) */
+void copy_qvec_wf_to_dm(qvec source, qvec destination) {
+  if (source->n * source->n != destination->n) QUAC_FAIL(1, "wf and dm do not have consistent sizes!");
+  double complex *w = host_state(source);
+  if (!w) return;
+  const PetscInt n = source->n;
+  double complex *d = (double complex *)malloc(sizeof(double complex) * (size_t)(n * n));
+  if (!d) {
+    free(w);
+    QUAC_FAIL(9, "out of host memory");
+  }
+  for (PetscInt c = 0; c < n; c++)
+    for (PetscInt r = 0; r < n; r++) d[n * c + r] = w[r] * conj(w[c]);
+  const int rc = qh_vec_set(destination->data, (const double *)d);
+  free(w);
+  free(d);
+  if (rc) QUAC_FAIL(9, "copy_qvec_wf_to_dm: %s", qh_last_error());
+}
+
+/* qvec_utilities.c:26-30 (VecEqual: every entry identical) */
+void check_qvec_equal(qvec state1, qvec state2, PetscBool *flag) {
+  *flag = PETSC_FALSE;
+  if (state1->n != state2->n || state1->my_type != state2->my_type) return;
+  double complex *a = host_state(state1), *b = a ? host_state(state2) : NULL;
+  if (a && b) *flag = memcmp(a, b, sizeof(double complex) * (size_t)state1->n) == 0 ? PETSC_TRUE : PETSC_FALSE;
+  free(a);
+  free(b);
+}
+
+/* qvec_utilities.c:1591-1656 with 1441-1590: the reduced density matrix over
+ * the subsystems not listed, kept in their original order (the first kept
+ * subsystem the most significant): rho_red(r, c) = sum_t rho((r, t), (c, t)),
+ * rho = |psi><psi| for a wavefunction.  The reference's density-matrix branch
+ * writes element (r, c) at (c, r) and makes the first kept subsystem the least
+ * significant; its tests use real symmetric states kept to one subsystem,
+ * where the two agree. */
+void ptrace_over_list_qvec(qvec full_state, PetscInt n_ops, PetscInt *op_loc_list, qvec *new_dm) {
+  const int dm = full_state->my_type == DENSITY_MATRIX;
+  if (!dm && full_state->my_type != WAVEFUNCTION)
+    QUAC_FAIL(9, "ptrace_over_list_qvec: a wavefunction or a density matrix is required");
+  const PetscInt nsub = full_state->n_ops;
+  if (n_ops > nsub) QUAC_FAIL(9, "Trying to ptrace more ops than are in the state!");
+  int *traced = (int *)calloc((size_t)nsub, sizeof(int));
+  for (PetscInt i = 0; i < n_ops; i++) {
+    if (op_loc_list[i] < 0 || op_loc_list[i] > nsub - 1) {
+      free(traced);
+      QUAC_FAIL(9, "op_loc_list has an index greater than nops in state!");
+    }
+    traced[op_loc_list[i]] = 1;
+  }
+  PetscInt nkeep = 0, Nk = 1, Nt = 1;
+  for (PetscInt i = 0; i < nsub; i++) {
+    if (traced[i]) Nt *= full_state->hspace_dims[i];
+    else {
+      Nk *= full_state->hspace_dims[i];
+      nkeep++;
+    }
+  }
+  if (nkeep == 0) {
+    free(traced);
+    QUAC_FAIL(9, "ptrace_over_list_qvec: every subsystem is traced over");
+  }
+  PetscInt *dims_new = (PetscInt *)malloc(sizeof(PetscInt) * (size_t)(2 * nkeep));
+  for (PetscInt i = 0, k = 0; i < nsub; i++)
+    if (!traced[i]) {
+      dims_new[k] = dims_new[k + nkeep] = full_state->hspace_dims[i];
+      k++;
+    }
+  /* full index of the (kept digits kk, traced digits tt) pair */
+  const PetscInt N = Nk * Nt;
+  PetscInt *full_of = (PetscInt *)malloc(sizeof(PetscInt) * (size_t)N);
+  for (PetscInt kk = 0; kk < Nk; kk++)
+    for (PetscInt tt = 0; tt < Nt; tt++) {
+      PetscInt rk = kk, rt = tt, idx = 0, mult = 1;
+      for (PetscInt i = nsub - 1; i >= 0; i--) { /* least significant subsystem first */
+        const PetscInt L = full_state->hspace_dims[i];
+        PetscInt digit;
+        if (traced[i]) {
+          digit = rt % L;
+          rt /= L;
+        } else {
+          digit = rk % L;
+          rk /= L;
+        }
+        idx += digit * mult;
+        mult *= L;
+      }
+      full_of[kk * Nt + tt] = idx;
+    }
+  double complex *x = host_state(full_state);
+  double complex *red = (double complex *)calloc((size_t)(Nk * Nk), sizeof(double complex));
+  if (x && red) {
+    for (PetscInt c = 0; c < Nk; c++)
+      for (PetscInt r = 0; r < Nk; r++) {
+        double complex acc = 0.0;
+        for (PetscInt t = 0; t < Nt; t++) {
+          const PetscInt fr = full_of[r * Nt + t], fc = full_of[c * Nt + t];
+          acc += dm ? x[N * fc + fr] : x[fr] * conj(x[fc]);
+        }
+        red[Nk * c + r] = acc;
+      }
+    create_arb_qvec_dims(new_dm, 2 * nkeep, dims_new, DENSITY_MATRIX);
+    if (*new_dm && qh_vec_set((*new_dm)->data, (const double *)red))
+      quac_fail(9, "ptrace_over_list_qvec: %s", qh_last_error());
+  }
+  free(x);
+  free(red);
+  free(full_of);
+  free(dims_new);
+  free(traced);
+}
+
+/* eigenvalues (w) and, if V, eigenvectors (columns of V, row-major n x n) of a
+ * real symmetric row-major matrix A (destroyed): cyclic Jacobi rotations
+ * (Golub & Van Loan, Matrix Computations, sec. 8.5) until the off-diagonal
+ * mass is below 1e-30 of the total */
+static void sym_eig(int n, double *A, double *w, double *V) {
+  if (V)
+    for (int i = 0; i < n; i++)
+      for (int j = 0; j < n; j++) V[i * n + j] = i == j;
+  for (int sweep = 0; sweep < 100; sweep++) {
+    double off = 0.0, tot = 0.0;
+    for (int i = 0; i < n; i++)
+      for (int j = 0; j < n; j++) {
+        tot += A[i * n + j] * A[i * n + j];
+        if (i != j) off += A[i * n + j] * A[i * n + j];
+      }
+    if (off <= 1e-30 * tot || off == 0.0) break;
+    for (int p = 0; p < n - 1; p++)
+      for (int q = p + 1; q < n; q++) {
+        const double apq = A[p * n + q];
+        if (apq == 0.0) continue;
+        const double theta = (A[q * n + q] - A[p * n + p]) / (2.0 * apq);
+        const double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+        const double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+        for (int k = 0; k < n; k++) { /* columns p, q */
+          const double akp = A[k * n + p], akq = A[k * n + q];
+          A[k * n + p] = c * akp - s * akq;
+          A[k * n + q] = s * akp + c * akq;
+        }
+        for (int k = 0; k < n; k++) { /* rows p, q */
+          const double apk = A[p * n + k], aqk = A[q * n + k];
+          A[p * n + k] = c * apk - s * aqk;
+          A[q * n + k] = s * apk + c * aqk;
+        }
+        if (V)
+          for (int k = 0; k < n; k++) {
+            const double vkp = V[k * n + p], vkq = V[k * n + q];
+            V[k * n + p] = c * vkp - s * vkq;
+            V[k * n + q] = s * vkp + c * vkq;
+          }
+      }
+  }
+  for (int i = 0; i < n; i++) w[i] = A[i * n + i];
+}
+
+/* real 2N x 2N embedding [[Re M, -Im M], [Im M, Re M]] (row-major) of the
+ * N x N matrix stored column-major in x; the embedding commutes with products,
+ * adjoints and functions of Hermitian matrices, and a Hermitian M's embedding
+ * has each eigenvalue of M twice */
+static void embed(PetscInt N, const double complex *x, double *E) {
+  const PetscInt M = 2 * N;
+  for (PetscInt r = 0; r < N; r++)
+    for (PetscInt c = 0; c < N; c++) {
+      const double complex v = x[N * c + r];
+      E[r * M + c] = creal(v);
+      E[r * M + c + N] = -cimag(v);
+      E[(r + N) * M + c] = cimag(v);
+      E[(r + N) * M + c + N] = creal(v);
+    }
+}
+
+static void matmul(PetscInt n, const double *A, const double *B, double *C) {
+  for (PetscInt i = 0; i < n; i++) {
+    for (PetscInt j = 0; j < n; j++) C[i * n + j] = 0.0;
+    for (PetscInt k = 0; k < n; k++) {
+      const double a = A[i * n + k];
+      if (a == 0.0) continue;
+      for (PetscInt j = 0; j < n; j++) C[i * n + j] += a * B[k * n + j];
+    }
+  }
+}
+
+/* qvec_utilities.c:2339-2430: (Tr sqrt(sqrt(rho) sigma sqrt(rho)))^2, negative
+ * eigenvalues dropped */
+static int fidelity_dm_dm(PetscInt N, const double complex *rho, const double complex *sig, double *fid) {
+  const PetscInt M = 2 * N;
+  double *R = (double *)malloc(sizeof(double) * (size_t)(M * M)), *V = (double *)malloc(sizeof(double) * (size_t)(M * M));
+  double *S = (double *)malloc(sizeof(double) * (size_t)(M * M)), *T = (double *)malloc(sizeof(double) * (size_t)(M * M));
+  double *w = (double *)malloc(sizeof(double) * (size_t)M);
+  if (!R || !V || !S || !T || !w) {
+    free(R), free(V), free(S), free(T), free(w);
+    return quac_fail(9, "out of host memory"), -1;
+  }
+  embed(N, rho, R);
+  sym_eig((int)M, R, w, V);
+  /* R = sqrt(rho) embedded = V diag(sqrt(max(w, 0))) V^T */
+  for (PetscInt i = 0; i < M; i++)
+    for (PetscInt k = 0; k < M; k++) T[i * M + k] = V[i * M + k] * sqrt(w[k] > 0 ? w[k] : 0.0);
+  for (PetscInt i = 0; i < M; i++)
+    for (PetscInt j = 0; j < M; j++) {
+      double acc = 0.0;
+      for (PetscInt k = 0; k < M; k++) acc += T[i * M + k] * V[j * M + k];
+      R[i * M + j] = acc;
+    }
+  embed(N, sig, S);
+  matmul(M, R, S, T);
+  matmul(M, T, R, S);
+  /* symmetrise against rounding before the eigenvalues */
+  for (PetscInt i = 0; i < M; i++)
+    for (PetscInt j = i + 1; j < M; j++) S[i * M + j] = S[j * M + i] = 0.5 * (S[i * M + j] + S[j * M + i]);
+  sym_eig((int)M, S, w, NULL);
+  double f = 0.0;
+  for (PetscInt i = 0; i < M; i++)
+    if (w[i] > 0) f += sqrt(w[i]);
+  f *= 0.5; /* every eigenvalue appears twice in the embedding */
+  *fid = f * f;
+  free(R), free(V), free(S), free(T), free(w);
+  return 0;
+}
+
+/* <psi|rho|psi> (qvec_utilities.c:2271-2337) */
+static double fidelity_dm_wf(PetscInt N, const double complex *rho, const double complex *psi) {
+  double complex acc = 0.0;
+  for (PetscInt c = 0; c < N; c++) {
+    double complex col = 0.0;
+    for (PetscInt r = 0; r < N; r++) col += conj(psi[r]) * rho[N * c + r];
+    acc += col * psi[c];
+  }
+  return creal(acc);
+}
+
+/* |<b|a>|^2 (qvec_utilities.c:2247-2252) */
+static double fidelity_wf_wf(PetscInt n, const double complex *a, const double complex *b) {
+  double complex acc = 0.0;
+  for (PetscInt i = 0; i < n; i++) acc += a[i] * conj(b[i]);
+  return creal(acc) * creal(acc) + cimag(acc) * cimag(acc);
+}
+
+/* qvec_utilities.c:2159-2245: fidelity of two states; for a spawned MCWF
+ * ensemble against a wavefunction the mean over the trajectories of
+ * |<psi|traj_i>|^2 / <traj_i|traj_i>, and in *fid_var the sample variance of
+ * the unnormalised |<psi|traj_i>|^2 about that mean (the reference's form) */
+void get_fidelity_qvec(qvec q1, qvec q2, PetscReal *fidelity, PetscReal *fid_var) {
+  *fid_var = 0.0;
+  *fidelity = 0.0;
+  const qvec_type t1 = q1->my_type, t2 = q2->my_type;
+  if ((t1 == WF_ENSEMBLE && t2 == WAVEFUNCTION) || (t2 == WF_ENSEMBLE && t1 == WAVEFUNCTION)) {
+    qvec ens = t1 == WF_ENSEMBLE ? q1 : q2, wf = t1 == WF_ENSEMBLE ? q2 : q1;
+    double complex *b = host_state(wf);
+    if (!b) return;
+    if (!ens->ens_spawned) {
+      double complex *a = host_state(ens);
+      if (a) *fidelity = fidelity_wf_wf(wf->n, a, b);
+      free(a);
+      free(b);
+      return;
+    }
+    const PetscInt M = ens->n_ensemble, N = ens->n;
+    double complex *a = (double complex *)malloc(sizeof(double complex) * (size_t)(M * N));
+    double *f = (double *)malloc(sizeof(double) * (size_t)M);
+    if (a && f && !qh_vec_get(ens->ens_datas[0], (double *)a)) {
+      double mean = 0.0;
+      for (PetscInt i = 0; i < M; i++) {
+        double nrm = 0.0;
+        for (PetscInt k = 0; k < N; k++) nrm += creal(a[i * N + k] * conj(a[i * N + k]));
+        f[i] = fidelity_wf_wf(N, a + i * N, b);
+        mean += f[i] / nrm;
+      }
+      mean /= (double)M;
+      double var = 0.0;
+      for (PetscInt i = 0; i < M; i++) var += (f[i] - mean) * (f[i] - mean);
+      *fidelity = mean;
+      *fid_var = M > 1 ? var / (double)(M - 1) : 0.0;
+    } else {
+      quac_fail(9, "get_fidelity_qvec: ensemble copy failed");
+    }
+    free(a);
+    free(f);
+    free(b);
+    return;
+  }
+  double complex *a = host_state(q1), *b = a ? host_state(q2) : NULL;
+  if (!a || !b) {
+    free(a);
+    return;
+  }
+  if (t1 == WAVEFUNCTION && t2 == WAVEFUNCTION) {
+    if (q1->n != q2->n) quac_fail(0, "The wavefunctions are not the same size! Fidelity cannot be calculated.");
+    else *fidelity = fidelity_wf_wf(q1->n, a, b);
+  } else if (t1 == DENSITY_MATRIX && t2 == DENSITY_MATRIX) {
+    if (q1->n != q2->n) quac_fail(0, "The input density matrices are not the same size! Fidelity cannot be calculated.");
+    else fidelity_dm_dm(q1->total_levels, a, b, fidelity);
+  } else if ((t1 == DENSITY_MATRIX && t2 == WAVEFUNCTION) || (t1 == WAVEFUNCTION && t2 == DENSITY_MATRIX)) {
+    qvec d = t1 == DENSITY_MATRIX ? q1 : q2, w = t1 == DENSITY_MATRIX ? q2 : q1;
+    if (d->n != w->n * w->n)
+      quac_fail(0, "The input density matrix and wavefunction are not compatible! Fidelity cannot be calculated.");
+    else *fidelity = fidelity_dm_wf(w->n, t1 == DENSITY_MATRIX ? a : b, t1 == DENSITY_MATRIX ? b : a);
+  } else {
+    quac_fail(9, "Types not understood in get_fidelity_qvec!");
+  }
+  free(a);
+  free(b);
+}
+
+/* PETSc VecDot(x, y) = sum x_i conj(y_i) */
+static double complex vdot(PetscInt n, const double complex *x, const double complex *y) {
+  double complex acc = 0.0;
+  for (PetscInt i = 0; i < n; i++) acc += x[i] * conj(y[i]);
+  return acc;
+}
+
+/* qvec_utilities.c:2102-2141: sqrt(Re(Tr(rho1 rho2) + sqrt(1 - Tr rho1^2) sqrt(1 - Tr rho2^2)));
+ * a wavefunction argument is first made |psi><psi| */
+void get_superfidelity_qvec(qvec q1, qvec q2, PetscReal *superfidelity) {
+  *superfidelity = 0.0;
+  if (q1->my_type == WAVEFUNCTION && q2->my_type == WAVEFUNCTION)
+    QUAC_FAIL(9, "Calculating superfidelity between two wavefunctions is not recommended. Use get_superfidelity_qvec!");
+  if ((q1->my_type != DENSITY_MATRIX && q1->my_type != WAVEFUNCTION) ||
+      (q2->my_type != DENSITY_MATRIX && q2->my_type != WAVEFUNCTION))
+    QUAC_FAIL(9, "Types not understand in get_superfidelity_qvec!");
+  qvec d1 = q1, d2 = q2, tmp = NULL;
+  if (q1->my_type == WAVEFUNCTION || q2->my_type == WAVEFUNCTION) {
+    qvec w = q1->my_type == WAVEFUNCTION ? q1 : q2, d = q1->my_type == WAVEFUNCTION ? q2 : q1;
+    create_arb_qvec(&tmp, d->n, DENSITY_MATRIX);
+    if (!tmp) return;
+    copy_qvec_wf_to_dm(w, tmp);
+    d1 = d;
+    d2 = tmp;
+  }
+  if (d1->n != d2->n) {
+    destroy_qvec(&tmp);
+    QUAC_FAIL(9, "get_superfidelity_qvec: sizes differ");
+  }
+  double complex *a = host_state(d1), *b = a ? host_state(d2) : NULL;
+  if (a && b) {
+    const double complex v1 = vdot(d1->n, a, b), v2 = vdot(d1->n, a, a), v3 = vdot(d1->n, b, b);
+    *superfidelity = sqrt(creal(v1 + csqrt(1.0 - v2) * csqrt(1.0 - v3)));
+  }
+  free(a);
+  free(b);
+  destroy_qvec(&tmp);
+}
+
+/* qvec_utilities.c:1657-1680: ||rho1 - rho2||_F^2 */
+void get_hilbert_schmidt_dist_qvec(qvec q1, qvec q2, PetscReal *hs_dist) {
+  *hs_dist = 0.0;
+  if (q1->my_type != DENSITY_MATRIX || q2->my_type != DENSITY_MATRIX) QUAC_FAIL(9, "HS is only for DM-DM comparisons");
+  if (q1->n != q2->n) QUAC_FAIL(9, "DMs are not the same size in hs_dist");
+  double complex *a = host_state(q1), *b = a ? host_state(q2) : NULL;
+  if (a && b) {
+    double acc = 0.0;
+    for (PetscInt i = 0; i < q1->n; i++) {
+      const double complex d = a[i] - b[i];
+      acc += creal(d) * creal(d) + cimag(d) * cimag(d);
+    }
+    *hs_dist = acc;
+  }
+  free(a);
+  free(b);
+}
+
+/* qvec_utilities.c:291-305 */
+void print_dm_qvec(qvec dm) {
+  double complex *x = host_state(dm);
+  if (!x) return;
+  const PetscInt N = dm->total_levels;
+  for (PetscInt i = 0; i < N; i++) {
+    for (PetscInt j = 0; j < N; j++) printf("%4.3e + %4.3ei ", creal(x[N * j + i]), cimag(x[N * j + i]));
+    printf("\n");
+  }
+  printf("\n");
+  free(x);
+}
+
+/* qvec_utilities.c:397-412 */
+void print_wf_qvec(qvec state) {
+  double complex *x = host_state(state);
+  if (!x) return;
+  for (PetscInt i = 0; i < state->n; i++) printf("%4.3e + %4.3ei\n", creal(x[i]), cimag(x[i]));
+  printf("\n");
+  free(x);
+}
+
+/* qvec_utilities.c:971-993: (operator, fock state) pairs */
+void get_qvec_loc_fock_op(qvec state, PetscInt *loc, PetscInt num_ops, ...) {
+  if (num_ops < 1 || num_ops > 64) QUAC_FAIL(9, "get_qvec_loc_fock_op: 1..64 operators");
+  operator ops[64];
+  PetscInt fock[64];
+  va_list ap;
+  va_start(ap, num_ops);
+  for (PetscInt i = 0; i < num_ops; i++) {
+    ops[i] = va_arg(ap, operator);
+    fock[i] = va_arg(ap, PetscInt);
+  }
+  va_end(ap);
+  get_qvec_loc_fock_op_list(state, loc, num_ops, ops, fock);
+}
