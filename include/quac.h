@@ -214,6 +214,28 @@ void get_hilbert_schmidt_dist_qvec(qvec, qvec, PetscReal *);         /* :1657-16
 void print_dm_qvec(qvec);                                /* :291-305 */
 void print_wf_qvec(qvec);                                /* :397-412 */
 void get_qvec_loc_fock_op(qvec, PetscInt *, PetscInt, ...);          /* :971-993 */
+void get_dm_element_qvec_local(qvec, PetscInt, PetscInt, PetscScalar *);        /* :351-359 */
+void get_wf_element_qvec_local(qvec, PetscInt, PetscScalar *);                  /* :416-431 */
+void get_wf_element_ens_i_qvec_local(qvec, PetscInt, PetscInt, PetscScalar *);  /* :434-449 */
+void get_qvec_local_idxs(qvec, PetscInt, PetscInt *);                           /* :1427-1438 */
+void check_qvec_consistent(qvec, qvec);                                         /* :10-24 */
+void change_qvec_dims(qvec, PetscInt, PetscInt *);                              /* :582-626 */
+/* cross-entropy benchmark scores of bitstring distributions (:1720-1991) */
+void get_linear_xeb_fidelity(qvec, qvec, PetscReal *, PetscReal *);
+void get_linear_xeb_unbiased_fidelity(qvec, qvec, PetscReal *, PetscReal *);
+void get_log_xeb_fidelity(qvec, qvec, PetscReal *, PetscReal *);
+void get_hog_score(qvec, qvec, PetscReal *, PetscReal *);
+void get_hog_score_fidelity(qvec, qvec, PetscReal *, PetscReal *);
+void get_linear_xeb_fidelity_probs(PetscReal *, PetscReal *, PetscInt, PetscReal *, PetscReal *, PetscInt, PetscInt,
+                                   PetscReal *, PetscReal *);
+void get_linear_xeb_unbiased_fidelity_probs(PetscReal *, PetscReal *, PetscInt, PetscReal *, PetscReal *, PetscInt,
+                                            PetscInt, PetscReal *, PetscReal *);
+void get_log_xeb_fidelity_probs(PetscReal *, PetscReal *, PetscInt, PetscReal *, PetscReal *, PetscInt, PetscInt,
+                                PetscReal *, PetscReal *);
+void get_hog_score_probs(PetscReal *, PetscReal *, PetscInt, PetscReal *, PetscReal *, PetscInt, PetscInt,
+                         PetscReal *, PetscReal *);
+void get_hog_score_fidelity_probs(PetscReal *, PetscReal *, PetscInt, PetscReal *, PetscReal *, PetscInt, PetscInt,
+                                  PetscReal *, PetscReal *);
 void print_qvec_file(qvec, const char[]);               /* :203-211 */
 void print_dm_qvec_file(qvec, const char[]);            /* :213-241 */
 void print_wf_qvec_file(qvec, const char[]);            /* :246-268 */

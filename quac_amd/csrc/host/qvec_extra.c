@@ -425,3 +425,175 @@ void get_qvec_loc_fock_op(qvec state, PetscInt *loc, PetscInt num_ops, ...) {
   va_end(ap);
   get_qvec_loc_fock_op_list(state, loc, num_ops, ops, fock);
 }
+
+/* ------------------------------------------- cross-entropy benchmark scores */
+/* Scores of an "experimental" bitstring distribution p_exp (with per-bitstring
+ * variances v_exp) against the ideal one p_ref over a Hilbert space of size D
+ * (qvec_utilities.c:1720-1991): linear XEB D sum p_exp p_ref - 1, its unbiased
+ * form (D sum p_exp p_ref - 1) / (D sum p_ref^2 - 1), log XEB
+ * log D + sum p_exp log p_ref + gamma_Euler (bitstrings with p_ref = 0 skipped
+ * in the mean and counted with log(3e-33) in the variance), and the heavy-output
+ * score sum over p_ref >= ln 2 / D of p_exp with its fidelity (2 h - 1) / ln 2. */
+static int same_size(PetscInt a, PetscInt b, const char *fn) {
+  if (a == b) return 0;
+  quac_fail(9, "Ref and Exp are not the same size in %s!", fn);
+  return -1;
+}
+
+void get_linear_xeb_fidelity_probs(PetscReal *probs_ref, PetscReal *vars_ref, PetscInt nloc_ref, PetscReal *probs_exp,
+                                   PetscReal *vars_exp, PetscInt nloc_exp, PetscInt h_dim, PetscReal *fid,
+                                   PetscReal *var) {
+  (void)vars_ref;
+  if (same_size(nloc_ref, nloc_exp, "get_linear_xeb_fidelity_probs")) return;
+  double s = 0.0, v = 0.0;
+  for (PetscInt i = 0; i < nloc_ref; i++) {
+    s += probs_exp[i] * probs_ref[i];
+    v += vars_exp[i] * probs_ref[i] * probs_ref[i];
+  }
+  *fid = (double)h_dim * s - 1.0;
+  *var = (double)h_dim * (double)h_dim * v;
+}
+
+void get_linear_xeb_unbiased_fidelity_probs(PetscReal *probs_ref, PetscReal *vars_ref, PetscInt nloc_ref,
+                                            PetscReal *probs_exp, PetscReal *vars_exp, PetscInt nloc_exp,
+                                            PetscInt h_dim, PetscReal *fid, PetscReal *var) {
+  (void)vars_ref;
+  if (same_size(nloc_ref, nloc_exp, "get_linear_xeb_unbiased_fidelity_probs")) return;
+  double num = 0.0, den = 0.0, v = 0.0;
+  for (PetscInt i = 0; i < nloc_ref; i++) {
+    num += probs_exp[i] * probs_ref[i];
+    den += probs_ref[i] * probs_ref[i];
+    v += vars_exp[i] * probs_ref[i] * probs_ref[i];
+  }
+  const double d = (double)h_dim * den - 1.0;
+  *fid = ((double)h_dim * num - 1.0) / d;
+  *var = (double)h_dim * (double)h_dim * v / (d * d);
+}
+
+void get_log_xeb_fidelity_probs(PetscReal *probs_ref, PetscReal *vars_ref, PetscInt nloc_ref, PetscReal *probs_exp,
+                                PetscReal *vars_exp, PetscInt nloc_exp, PetscInt h_dim, PetscReal *fid,
+                                PetscReal *var) {
+  (void)vars_ref;
+  if (same_size(nloc_ref, nloc_exp, "get_log_xeb_fidelity_probs")) return;
+  const double euler_gamma = 0.57721566490153286;
+  const double log_floor = log(3e-33);
+  double s = 0.0, v = 0.0;
+  for (PetscInt i = 0; i < nloc_ref; i++) {
+    if (probs_ref[i] == 0.0) {
+      v += vars_exp[i] * log_floor * log_floor;
+    } else {
+      const double l = log(probs_ref[i]);
+      s += probs_exp[i] * l;
+      v += vars_exp[i] * l * l;
+    }
+  }
+  *fid = log((double)h_dim) + s + euler_gamma;
+  *var = v;
+}
+
+void get_hog_score_probs(PetscReal *probs_ref, PetscReal *vars_ref, PetscInt nloc_ref, PetscReal *probs_exp,
+                         PetscReal *vars_exp, PetscInt nloc_exp, PetscInt h_dim, PetscReal *hog_score,
+                         PetscReal *hog_var) {
+  (void)vars_ref;
+  if (same_size(nloc_ref, nloc_exp, "get_hog_score_probs")) return;
+  const double heavy = log(2.0) / (double)h_dim;
+  double s = 0.0, v = 0.0;
+  for (PetscInt i = 0; i < nloc_ref; i++)
+    if (probs_ref[i] >= heavy) {
+      s += probs_exp[i];
+      v += vars_exp[i];
+    }
+  *hog_score = s;
+  *hog_var = v;
+}
+
+void get_hog_score_fidelity_probs(PetscReal *probs_ref, PetscReal *vars_ref, PetscInt nloc_ref, PetscReal *probs_exp,
+                                  PetscReal *vars_exp, PetscInt nloc_exp, PetscInt h_dim, PetscReal *fid,
+                                  PetscReal *var) {
+  double h = 0.0, hv = 0.0;
+  get_hog_score_probs(probs_ref, vars_ref, nloc_ref, probs_exp, vars_exp, nloc_exp, h_dim, &h, &hv);
+  *fid = (2.0 * h - 1.0) / log(2.0);
+  *var = (2.0 / log(2.0)) * (2.0 / log(2.0)) * hv;
+}
+
+typedef void (*probs_score)(PetscReal *, PetscReal *, PetscInt, PetscReal *, PetscReal *, PetscInt, PetscInt,
+                            PetscReal *, PetscReal *);
+
+/* the score of two states' bitstring distributions (diagonals) */
+static void score_states(qvec ref, qvec exp, probs_score f, PetscReal *a, PetscReal *b, const char *fn) {
+  if (ref->total_levels != exp->total_levels) QUAC_FAIL(9, "Ref and Exp are not the same size in %s!", fn);
+  PetscInt nr = 0, ne = 0;
+  PetscReal *pr = NULL, *vr = NULL, *pe = NULL, *ve = NULL;
+  get_bitstring_probs(ref, &nr, &pr, &vr);
+  if (pr) get_bitstring_probs(exp, &ne, &pe, &ve);
+  if (pr && pe) f(pr, vr, nr, pe, ve, ne, ref->total_levels, a, b);
+  free(pr), free(vr), free(pe), free(ve);
+}
+
+void get_linear_xeb_fidelity(qvec ref, qvec exp, PetscReal *fid, PetscReal *var) {
+  score_states(ref, exp, get_linear_xeb_fidelity_probs, fid, var, "get_linear_xeb_fidelity");
+}
+void get_linear_xeb_unbiased_fidelity(qvec ref, qvec exp, PetscReal *fid, PetscReal *var) {
+  score_states(ref, exp, get_linear_xeb_unbiased_fidelity_probs, fid, var, "get_linear_xeb_unbiased_fidelity");
+}
+void get_log_xeb_fidelity(qvec ref, qvec exp, PetscReal *fid, PetscReal *var) {
+  score_states(ref, exp, get_log_xeb_fidelity_probs, fid, var, "get_log_xeb_fidelity");
+}
+void get_hog_score(qvec ref, qvec exp, PetscReal *hog_score, PetscReal *hog_var) {
+  score_states(ref, exp, get_hog_score_probs, hog_score, hog_var, "get_hog_score");
+}
+void get_hog_score_fidelity(qvec ref, qvec exp, PetscReal *fid, PetscReal *var) {
+  score_states(ref, exp, get_hog_score_fidelity_probs, fid, var, "get_hog_score_fidelity");
+}
+
+/* --------------------------------------------------- single-process accessors */
+/* one process holds the whole state, so the _local forms are the global ones
+ * (qvec_utilities.c:351-359, 416-449) */
+void get_dm_element_qvec_local(qvec dm, PetscInt row, PetscInt col, PetscScalar *val) {
+  get_dm_element_qvec(dm, row, col, val);
+}
+void get_wf_element_qvec_local(qvec wf, PetscInt i, PetscScalar *val) { get_wf_element_qvec(wf, i, val); }
+
+/* amplitude i of trajectory ens_i of a spawned ensemble (the initial state before) */
+void get_wf_element_ens_i_qvec_local(qvec wf, PetscInt ens_i, PetscInt i, PetscScalar *val) {
+  *val = 0.0;
+  if (wf->my_type != WF_ENSEMBLE) QUAC_FAIL(9, "get_wf_element_ens_i_qvec_local: a WF_ENSEMBLE qvec is required");
+  if (i < 0 || i >= wf->n) QUAC_FAIL(9, "get_wf_element_ens_i_qvec_local: index out of range");
+  if (!wf->ens_spawned) {
+    get_wf_element_qvec(wf, i, val);
+    return;
+  }
+  if (ens_i < 0 || ens_i >= wf->n_ensemble) QUAC_FAIL(9, "get_wf_element_ens_i_qvec_local: trajectory out of range");
+  double re = 0.0, im = 0.0;
+  QUAC_HIPCHK(qh_vec_get_entry(wf->ens_datas[0], ens_i * wf->n + i, &re, &im));
+  *val = re + im * I;
+}
+
+/* the digits of a global index, the last dimension the least significant
+ * (qvec_utilities.c:1427-1438) */
+void get_qvec_local_idxs(qvec state, PetscInt global_idx, PetscInt *local_idxs) {
+  PetscInt r = global_idx;
+  for (PetscInt k = state->ndims_hspace - 1; k >= 0; k--) {
+    local_idxs[k] = r % state->hspace_dims[k];
+    r /= state->hspace_dims[k];
+  }
+}
+
+/* qvec_utilities.c:10-24: same type and size */
+void check_qvec_consistent(qvec q1, qvec q2) {
+  if (q1->my_type != q2->my_type) QUAC_FAIL(9, "qvecs are not the same type!");
+  if (q1->n != q2->n) QUAC_FAIL(9, "qvecs are not the same size!");
+}
+
+/* qvec_utilities.c:582-626: new subsystem dimensions of the same total size */
+void change_qvec_dims(qvec q, PetscInt ndims, PetscInt *dims) {
+  PetscInt tot = 1;
+  for (PetscInt i = 0; i < ndims; i++) tot *= dims[i];
+  if (tot != q->n) QUAC_FAIL(9, "change_qvec_dims: the dimensions do not multiply to the state size");
+  PetscInt *nd = (PetscInt *)malloc(sizeof(PetscInt) * (size_t)ndims);
+  memcpy(nd, dims, sizeof(PetscInt) * (size_t)ndims);
+  free(q->hspace_dims);
+  q->hspace_dims = nd;
+  q->ndims_hspace = ndims;
+  q->n_ops = q->my_type == DENSITY_MATRIX ? ndims / 2 : ndims;
+}

@@ -204,3 +204,42 @@ def test_copy_wf_to_dm_and_equality():
     qa.lib().check_qvec_equal(d1.h, d2.h, ctypes.byref(flag))
     assert flag.value == 0
     _free(wf, d1, d2)
+
+
+def test_xeb_scores_of_states_and_index_helpers():
+    """get_linear_xeb_fidelity / get_hog_score on a wavefunction (ideal) and a
+    density matrix (experiment) use their bitstring probabilities; the local
+    index digits (qvec_utilities.c:1427-1438) and change_qvec_dims"""
+    rng = np.random.default_rng(6)
+    n = 8
+    psi = rng.normal(size=n) + 1j * rng.normal(size=n)
+    psi /= np.linalg.norm(psi)
+    rho = _rand_dm(rng, n)
+    ref, exp = _arb([2, 2, 2], WF), _arb([2, 2, 2, 2, 2, 2], DM)
+    ref.set(psi)
+    _set_dm(exp, rho)
+    L = qa.lib()
+    P = ctypes.POINTER(ctypes.c_double)
+    for name in ("get_linear_xeb_fidelity", "get_hog_score"):
+        getattr(L, name).argtypes = [ctypes.c_void_p, ctypes.c_void_p, P, P]
+        getattr(L, name).restype = None
+    a, b = ctypes.c_double(), ctypes.c_double()
+    p_ref, p_exp = np.abs(psi) ** 2, np.diag(rho).real
+    L.get_linear_xeb_fidelity(ref.h, exp.h, ctypes.byref(a), ctypes.byref(b))
+    assert abs(a.value - (n * np.sum(p_exp * p_ref) - 1)) < 1e-12 and b.value == 0.0
+    L.get_hog_score(ref.h, exp.h, ctypes.byref(a), ctypes.byref(b))
+    assert abs(a.value - p_exp[p_ref >= np.log(2) / n].sum()) < 1e-12
+    # digits of a global index: the last dimension the least significant
+    L.get_qvec_local_idxs.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)]
+    L.get_qvec_local_idxs.restype = None
+    d = (ctypes.c_int64 * 6)()
+    L.get_qvec_local_idxs(exp.h, 45, d)
+    assert list(d) == [int(c) for c in np.binary_repr(45, 6)]
+    # regroup the three qubits as (4, 2) and trace the 4-level part
+    L.change_qvec_dims.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)]
+    L.change_qvec_dims.restype = None
+    L.change_qvec_dims(exp.h, 4, (ctypes.c_int64 * 4)(4, 2, 4, 2))
+    qa._check()
+    red = _rho(_ptrace(exp, [0]))
+    assert np.abs(red - np.einsum("aiAi->aA", rho.reshape(4, 2, 4, 2).transpose(1, 0, 3, 2))).max() < 1e-14
+    _free(ref, exp)
