@@ -252,6 +252,8 @@ void schedule_circuit_layers(qsystem, circuit *);               /* quantum_circu
 void apply_circuit_to_sys(qsystem, circuit *, PetscReal);       /* quantum_circuits.c:407-431 */
 void apply_circuit_to_qvec(qsystem, circuit, qvec);             /* quantum_circuits.c:143-159 */
 void apply_gate2(qvec, gate_type, ...);                         /* quantum_circuits.c:776-847 */
+void apply_single_qb_measurement_error_probs(PetscReal *, PetscInt, PetscReal, PetscReal, PetscInt); /* :620-644 */
+void apply_single_qb_measurement_error_state(qvec, PetscReal, PetscReal, PetscInt);                 /* :646-688 */
 
 /* ------------------------------------------------------------ src/solver.h */
 /* steady state of a qsystem (solver.c:33-220 semantics; the reference has it

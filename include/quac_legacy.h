@@ -63,6 +63,12 @@ void time_step(Vec, PetscReal, PetscReal, PetscReal, PetscInt);        /* :236-5
 void set_ts_monitor(PetscErrorCode (*)(TS, PetscInt, PetscReal, Vec, void *));            /* :526-529 */
 void set_ts_monitor_ctx(PetscErrorCode (*)(TS, PetscInt, PetscReal, Vec, void *), void *); /* :540-543 */
 
+/* quantum_gates.h:68-72: circuits of the legacy API (create_circuit and
+ * destroy_circuit are in quac.h); gate arguments as add_gate_to_circuit_sys */
+void add_gate_to_circuit(circuit *, PetscReal, gate_type, ...);       /* quantum_gates.c:651-714 */
+void add_circuit_to_circuit(circuit *, circuit, PetscReal);           /* quantum_gates.c:716-749 */
+void start_circuit_at_time(circuit *, PetscReal);                     /* quantum_gates.c:755-760 */
+
 /* quac.c:119-141: free an operator (with its dag and n) / a VEC operator array */
 void destroy_op(operator *);
 void destroy_vec(vec_op *);

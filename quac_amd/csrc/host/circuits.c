@@ -87,6 +87,32 @@ void add_gate_to_circuit_sys(circuit *circ, PetscReal time, gate_type type, ...)
   circ->num_gates++;
 }
 
+/* quantum_gates.c:651-714: the same gate record for the legacy global API (its
+ * circuits are started with start_circuit_at_time) */
+void add_gate_to_circuit(circuit *circ, PetscReal time, gate_type type, ...) {
+  if (gate_nqubits(type) == 0) QUAC_FAIL(0, "Gate not recognized!");
+  if (circ->num_gates == circ->gate_list_size) QUAC_FAIL(1, "Gate list not large enough!");
+  va_list ap;
+  va_start(ap, type);
+  fill_gate(&circ->gate_list[circ->num_gates], time, type, ap);
+  va_end(ap);
+  circ->num_gates++;
+}
+
+/* quantum_gates.c:716-749: append every gate of circ_to_add to circ at `time` */
+void add_circuit_to_circuit(circuit *circ, circuit circ_to_add, PetscReal time) {
+  if (circ->num_gates + circ_to_add.num_gates > circ->gate_list_size)
+    QUAC_FAIL(1, "Gate list not large enough to add this circuit!");
+  for (PetscInt i = 0; i < circ_to_add.num_gates; i++) {
+    struct quantum_gate_struct *g = &circ->gate_list[circ->num_gates];
+    *g = circ_to_add.gate_list[i];
+    g->time = time;
+    g->qubit_numbers = (PetscInt *)malloc(sizeof(PetscInt) * (size_t)(g->num_qubits > 0 ? g->num_qubits : 1));
+    for (PetscInt j = 0; j < g->num_qubits; j++) g->qubit_numbers[j] = circ_to_add.gate_list[i].qubit_numbers[j];
+    circ->num_gates++;
+  }
+}
+
 /* quantum_circuits.c:316-404: greedy ASAP layering */
 void schedule_circuit_layers(qsystem sys, circuit *circ) {
   PetscInt ns = sys->num_subsystems;

@@ -372,6 +372,13 @@ void trace_dm(PetscScalar *trace_val, Vec dm) {
   get_trace_qvec(q, trace_val);
 }
 
+/* quantum_gates.c:755-760: run the circuit during the next time_step, its gate
+ * times counted from `time` */
+void start_circuit_at_time(circuit *circ, PetscReal time) {
+  if (!g_sys) QUAC_FAIL(0, "You need to create operators before you start a circuit!");
+  apply_circuit_to_sys(g_sys, circ, time);
+}
+
 /* ------------------------------------------------------------------- solvers */
 void set_ts_monitor(PetscErrorCode (*monitor)(TS, PetscInt, PetscReal, Vec, void *)) {
   g_mon = monitor;

@@ -597,3 +597,41 @@ void change_qvec_dims(qvec q, PetscInt ndims, PetscInt *dims) {
   q->ndims_hspace = ndims;
   q->n_ops = q->my_type == DENSITY_MATRIX ? ndims / 2 : ndims;
 }
+
+/* ------------------------------------------------- classical readout errors */
+/* quantum_circuits.c:620-644: mix the bitstring probabilities of qubit
+ * `qubit_num` (stride 2^qubit_num): p01 = P(read 0 | state 1), p10 = P(read 1 |
+ * state 0), i.e. p0' = (1 - p01) p0 + p10 p1, p1' = p01 p0 + (1 - p10) p1 */
+void apply_single_qb_measurement_error_probs(PetscReal *probs, PetscInt n, PetscReal p01, PetscReal p10,
+                                             PetscInt qubit_num) {
+  const PetscInt n_bef = (PetscInt)1 << qubit_num, n_inc = 2 * n_bef;
+  for (PetscInt i = 0; i < n / 2; i++) {
+    const PetscInt i0 = (i / n_bef) * n_inc + i % n_bef, i1 = i0 + n_bef;
+    const double t0 = probs[i0], t1 = probs[i1];
+    probs[i0] = (1 - p01) * t0 + p10 * t1;
+    probs[i1] = p01 * t0 + (1 - p10) * t1;
+  }
+}
+
+/* quantum_circuits.c:646-688: the same mixing applied to the entries of a
+ * density matrix's vector, pairs at the stride of the dimensions before
+ * `qubit_num` in the state's dimension list */
+void apply_single_qb_measurement_error_state(qvec state, PetscReal p01, PetscReal p10, PetscInt qubit_num) {
+  if (state->my_type != DENSITY_MATRIX)
+    QUAC_FAIL(9, "Must be a DENSITY_MATRIX in apply_single_qb_measurement_error_state");
+  if (qubit_num < 0 || qubit_num >= state->ndims_hspace) QUAC_FAIL(9, "qubit out of range");
+  PetscInt n_bef = 1;
+  for (PetscInt i = 0; i < qubit_num; i++) n_bef *= state->hspace_dims[i];
+  const PetscInt n_inc = 2 * n_bef;
+  double complex *x = host_state(state);
+  if (!x) return;
+  for (PetscInt i = 0; i < state->n / 2; i++) {
+    const PetscInt i0 = (i / n_bef) * n_inc + i % n_bef, i1 = i0 + n_bef;
+    const double complex t0 = x[i0], t1 = x[i1];
+    x[i0] = (1 - p01) * t0 + p10 * t1;
+    x[i1] = p01 * t0 + (1 - p10) * t1;
+  }
+  const int rc = qh_vec_set(state->data, (const double *)x);
+  free(x);
+  if (rc) QUAC_FAIL(9, "apply_single_qb_measurement_error_state: %s", qh_last_error());
+}

@@ -106,3 +106,13 @@ def test_legacy_api_steady_state_closed_form(tmp_path):
     omega, delta, gamma = 0.4, 0.3, 1.0
     ree = (omega ** 2 / 4) / (delta ** 2 + gamma ** 2 / 4 + omega ** 2 / 2)
     assert abs(float(f[1]) - ree) < 1e-6 and abs(float(f[2]) - 1.0) < 1e-9, (f, ree)
+
+
+def test_legacy_api_circuit_flip_then_decay(tmp_path):
+    """legacy circuits: sigma_x at t = 1 (add_gate_to_circuit, copied by
+    add_circuit_to_circuit, started by start_circuit_at_time) on a qubit with
+    decay 0.7, fixed-step RK4: <n>(2) = exp(-0.7) within 1e-8"""
+    text = _run(_build(tmp_path, "legacy_circuit"), "-ts_rk_type", "4", "-ts_adapt_type", "none")
+    n = float([l for l in text.splitlines() if l.startswith("n ")][0].split()[1])
+    g = int([l for l in text.splitlines() if l.startswith("gates")][0].split()[1])
+    assert g == 1 and abs(n - math.exp(-0.7)) < 1e-8, (n, math.exp(-0.7))

@@ -243,3 +243,28 @@ def test_xeb_scores_of_states_and_index_helpers():
     red = _rho(_ptrace(exp, [0]))
     assert np.abs(red - np.einsum("aiAi->aA", rho.reshape(4, 2, 4, 2).transpose(1, 0, 3, 2))).max() < 1e-14
     _free(ref, exp)
+
+
+def test_readout_error_on_a_density_matrix_vector():
+    """apply_single_qb_measurement_error_state (quantum_circuits.c:646-688): the
+    readout mixing applied to the entry pairs of the density matrix's vector at
+    the stride of the dimensions listed before qubit_num"""
+    rng = np.random.default_rng(12)
+    rho = _rand_dm(rng, 4)
+    q = _arb([2, 2, 2, 2], DM)
+    _set_dm(q, rho)
+    x = rho.T.reshape(-1)
+    L = qa.lib()
+    L.apply_single_qb_measurement_error_state.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
+                                                          ctypes.c_int64]
+    L.apply_single_qb_measurement_error_state.restype = None
+    L.apply_single_qb_measurement_error_state(q.h, 0.05, 0.1, 1)
+    qa._check()
+    want = x.copy()
+    for i in range(8):
+        i0 = (i // 2) * 4 + i % 2
+        i1 = i0 + 2
+        want[i0] = 0.95 * x[i0] + 0.1 * x[i1]
+        want[i1] = 0.05 * x[i0] + 0.9 * x[i1]
+    assert np.abs(q.get() - want).max() < 1e-15
+    _free(q)

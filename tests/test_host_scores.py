@@ -76,3 +76,23 @@ def test_hog(dists):
     assert abs(h - mix[heavy].sum()) < 1e-12 and abs(hv - v[heavy].sum()) < 1e-15
     f, fv = _call("get_hog_score_fidelity_probs", p_ref, mix, v)
     assert abs(f - (2 * h - 1) / math.log(2)) < 1e-12 and abs(fv - (2 / math.log(2)) ** 2 * hv) < 1e-15
+
+
+def test_readout_error_mixing_vs_numpy():
+    """apply_single_qb_measurement_error_probs (quantum_circuits.c:620-644):
+    qubit q at stride 2^q; [p0', p1'] = [[1 - p01, p10], [p01, 1 - p10]] [p0, p1]
+    on that qubit, numpy on the reshaped distribution; total probability kept"""
+    L = qa.lib()
+    f = L.apply_single_qb_measurement_error_probs
+    f.restype = None
+    f.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.c_int64, ctypes.c_double, ctypes.c_double, ctypes.c_int64]
+    rng = np.random.default_rng(1)
+    p = rng.uniform(size=16)
+    p /= p.sum()
+    m = np.array([[1 - 0.03, 0.08], [0.03, 1 - 0.08]])
+    for q in range(4):
+        got = p.copy()
+        f(got.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), 16, 0.03, 0.08, q)
+        t = np.moveaxis(p.reshape(2, 2, 2, 2), 3 - q, 0)       # qubit q = bit q of the index
+        want = np.moveaxis(np.tensordot(m, t, axes=1), 0, 3 - q).reshape(-1)
+        assert np.abs(got - want).max() < 1e-15 and abs(got.sum() - 1) < 1e-14, q
