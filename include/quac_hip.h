@@ -230,7 +230,7 @@ int qh_ens_expect(const qh_vec *ens, int64_t M, int64_t tot, int nops, const qh_
                   const qh_level *levels, double *re, double *im);
 /* fits = 1 when the qh_mcwf_* calls below accept A and nj jump operators */
 int qh_mcwf_fits(const qh_mat *A, int nj, int *fits);
-/* Trajectories of at most 64 amplitudes (rows of A with at most 16 entries):
+/* Trajectories of at most 1024 amplitudes (rows of A with at most 16 entries):
  * every trajectory i with jstep[i] < J takes classic RK4 steps (size dt; step
  * J-1 has size hlast) and stops after the first step at which its squared norm
  * is below thr[i] (the jump test, qsystem.c:1138-1153), or after step J-1.
@@ -239,7 +239,7 @@ int qh_mcwf_fits(const qh_mat *A, int nj, int *fits);
 int qh_mcwf_advance(const qh_mat *A, qh_vec *ens, int64_t M, const double *thr, int32_t *jstep, int32_t J,
                     double dt, double hlast, int32_t *jump);
 /* p[q*nj + k] = ||C_k psi_{idx[q]}||^2 for q < n, k < nj (C: nj <= 16 N x N
- * operators, N <= 64; the jump probabilities of qsystem.c:1159-1200) */
+ * operators, N <= 1024; the jump probabilities of qsystem.c:1159-1200) */
 int qh_mcwf_jump_norms(int nj, qh_mat *const *C, const qh_vec *ens, int64_t N, const int64_t *idx, int32_t n,
                        double *p);
 /* psi_{idx[q]} <- (sre[q] + i sim[q]) C_{k[q]} psi_{idx[q]} for q < n, the idx[q]

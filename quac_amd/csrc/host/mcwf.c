@@ -157,7 +157,7 @@ static int jump(qsystem s, Vec block, int64_t i, int64_t N, Vec work) {
   return 0;
 }
 
-/* the device-resident loop applies (at most 64 amplitudes, rows of the
+/* the device-resident loop applies (at most 1024 amplitudes, rows of the
  * generator with at most 16 entries, at most 16 jump operators) and is not
  * switched off by QUAC_MCWF_STEPWISE */
 static int fast_path(qsystem s, int64_t N) {
@@ -168,7 +168,7 @@ static int fast_path(qsystem s, int64_t N) {
   return fits;
 }
 
-/* Trajectories of at most 64 amplitudes without a monitor: each trajectory runs
+/* Trajectories of at most 1024 amplitudes without a monitor: each trajectory runs
  * on the device from one jump to the next (qh_mcwf_advance) and the jumps due
  * are applied together (qh_mcwf_jump_norms, choose_jump, qh_mcwf_jump_apply),
  * until every trajectory has taken all J steps.  The steps, the jump test and

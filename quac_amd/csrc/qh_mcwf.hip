@@ -102,7 +102,7 @@ int ens_work(int64_t n) {
   return QH_OK;
 }
 
-// ------------------------------------------------- trajectories of up to 64 amplitudes
+// ----------------------------------------------- trajectories of up to 1024 amplitudes
 // A wavefront integrates 64 / S trajectories side by side, one segment of S
 // lanes (S a power of two >= N) per trajectory: lane r of a segment holds
 // amplitude r and row r of the generator (NZ entries in registers), and
@@ -234,7 +234,8 @@ struct JumpOps {
   const double2 *val[ENS_MAXJ];
 };
 
-// p[q * nj + k] = || C_k psi_{idx[q]} ||^2, one wavefront per (q, k)
+// p[q * nj + k] = || C_k psi_{idx[q]} ||^2, one wavefront per (q, k), rows
+// strided over its lanes
 __global__ void __launch_bounds__(256) k_mcwf_jump_norms(int32_t N, const double2 *__restrict__ ens,
                                                         const int64_t *__restrict__ idx, int32_t n, int32_t nj,
                                                         JumpOps ops, double *__restrict__ p) {
@@ -245,43 +246,117 @@ __global__ void __launch_bounds__(256) k_mcwf_jump_norms(int32_t N, const double
   const int k = (int)(g % nj);
   const double2 *psi = ens + idx[q] * N;
   double v = 0.0;
-  if (lane < N) {
+  for (int32_t r = lane; r < N; r += 64) {
     double re = 0.0, im = 0.0;
-    for (int32_t e = ops.rp[k][lane]; e < ops.rp[k][lane + 1]; e++) {
+    for (int32_t e = ops.rp[k][r]; e < ops.rp[k][r + 1]; e++) {
       const double2 a = ops.val[k][e], x = psi[ops.col[k][e]];
       re = fma(a.x, x.x, fma(-a.y, x.y, re));
       im = fma(a.x, x.y, fma(a.y, x.x, im));
     }
-    v = re * re + im * im;
+    v += re * re + im * im;
   }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   if (lane == 0) p[g] = v;
 }
 
-// psi_{idx[q]} <- sc[q] C_{kk[q]} psi_{idx[q]}, one wavefront per q
+// psi_{idx[q]} <- sc[q] C_{kk[q]} psi_{idx[q]}, one workgroup per q: psi is
+// staged in LDS, then every row is rewritten
+const int ENS_WMAX = 1024;  // amplitudes per trajectory
 __global__ void __launch_bounds__(256) k_mcwf_jump_apply(int32_t N, double2 *__restrict__ ens,
                                                         const int64_t *__restrict__ idx, int32_t n,
                                                         const int32_t *__restrict__ kk,
                                                         const double2 *__restrict__ sc, JumpOps ops) {
-  __shared__ double2 sh[4][ENS_W];
-  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int64_t q = (int64_t)blockIdx.x * 4 + w;
-  if (q >= n) return;  // whole wavefront
+  __shared__ double2 s[ENS_WMAX];
+  const int64_t q = blockIdx.x;
+  if (q >= n) return;  // whole workgroup
   double2 *psi = ens + idx[q] * N;
-  double2 *s = sh[w];
-  if (lane < N) s[lane] = psi[lane];
-  __builtin_amdgcn_wave_barrier();
-  if (lane < N) {
-    const int k = kk[q];
+  for (int32_t r = threadIdx.x; r < N; r += 256) s[r] = psi[r];
+  __syncthreads();
+  const int k = kk[q];
+  const double2 f = sc[q];
+  for (int32_t r = threadIdx.x; r < N; r += 256) {
     double re = 0.0, im = 0.0;
-    for (int32_t e = ops.rp[k][lane]; e < ops.rp[k][lane + 1]; e++) {
+    for (int32_t e = ops.rp[k][r]; e < ops.rp[k][r + 1]; e++) {
       const double2 a = ops.val[k][e], x = s[ops.col[k][e]];
       re = fma(a.x, x.x, fma(-a.y, x.y, re));
       im = fma(a.x, x.y, fma(a.y, x.x, im));
     }
-    const double2 f = sc[q];
-    psi[lane] = make_double2(f.x * re - f.y * im, f.x * im + f.y * re);
+    psi[r] = make_double2(f.x * re - f.y * im, f.x * im + f.y * re);
+  }
+}
+
+// Trajectories of 65..1024 amplitudes: one workgroup per trajectory, thread r
+// holding amplitude r and row r of the generator; the products exchange the
+// amplitudes through LDS between barriers, and the squared norm is a block sum
+// that every thread forms from the per-wavefront partials in the same order, so
+// the loop test is the same in every thread.  Steps, stop rule and outputs as
+// k_mcwf_run.
+template <int NZ, int LB>
+__global__ void __launch_bounds__(LB) k_mcwf_run_wg(int32_t N, const int32_t *__restrict__ rp,
+                                                     const int32_t *__restrict__ col,
+                                                     const double2 *__restrict__ val, double2 *__restrict__ ens,
+                                                     const double *__restrict__ thr, int32_t *__restrict__ jst,
+                                                     int32_t J, double dt, double hlast,
+                                                     int32_t *__restrict__ jump) {
+  __shared__ double2 sh[ENS_WMAX];
+  __shared__ double red[ENS_WMAX / 64];
+  const int64_t i = blockIdx.x;
+  const int r = threadIdx.x, w = r >> 6, nw = blockDim.x >> 6;
+  int32_t j = jst[i];
+  if (j >= J) {  // the same in every thread of the workgroup
+    if (r == 0) jump[i] = 0;
+    return;
+  }
+  const bool on = r < N;
+  int32_t c[NZ];
+  double2 a[NZ];
+  const int32_t beg = on ? rp[r] : 0;
+  const int nz = on ? rp[r + 1] - beg : 0;
+#pragma unroll
+  for (int e = 0; e < NZ; e++) {
+    c[e] = 0;
+    a[e] = make_double2(0.0, 0.0);
+    if (e < nz) {
+      c[e] = col[beg + e];
+      a[e] = val[beg + e];
+    }
+  }
+  double2 psi = on ? ens[i * N + r] : make_double2(0.0, 0.0);
+  const double t = thr[i];
+  auto apply = [&](double2 v) {
+    __syncthreads();  // the previous product's reads are done
+    if (on) sh[r] = v;
+    __syncthreads();
+    return ens_row_product<NZ>(c, a, sh);
+  };
+  int32_t jumped = 0;
+  while (true) {
+    const double h = (j == J - 1) ? hlast : dt;
+    const double2 k1 = apply(psi);
+    const double2 k2 = apply(make_double2(psi.x + 0.5 * h * k1.x, psi.y + 0.5 * h * k1.y));
+    const double2 k3 = apply(make_double2(psi.x + 0.5 * h * k2.x, psi.y + 0.5 * h * k2.y));
+    const double2 k4 = apply(make_double2(psi.x + h * k3.x, psi.y + h * k3.y));
+    const double s6 = h * (1.0 / 6.0), s3 = h * (1.0 / 3.0);
+    psi = make_double2(psi.x + s6 * k1.x + s3 * k2.x + s3 * k3.x + s6 * k4.x,
+                       psi.y + s6 * k1.y + s3 * k2.y + s3 * k3.y + s6 * k4.y);
+    j++;
+    const double part = seg_sum<64>(on ? psi.x * psi.x + psi.y * psi.y : 0.0);
+    if ((r & 63) == 0) red[w] = part;
+    __syncthreads();
+    double n2 = 0.0;
+    for (int k = 0; k < nw; k++) n2 += red[k];
+    __syncthreads();  // red is rewritten after the next step
+    if (n2 < t) {
+      jumped = 1;
+      break;
+    }
+    if (j >= J) break;
+  }
+  if (on) ens[i * N + r] = psi;
+  if (r == 0) {
+    jst[i] = j;
+    jump[i] = jumped;
   }
 }
 
@@ -303,8 +378,8 @@ extern "C" int qh_mcwf_advance(const qh_mat *A, qh_vec *ens, int64_t M, const do
   int rc = qh_ensure_init();
   if (rc) return rc;
   QH_REQUIRE(A && ens && thr && jstep && jump && ens->prec == QH_FP64 && M >= 1 && M <= (1 << 30) &&
-                 ens->n == M * A->rows && A->rows >= 1 && A->rows <= ENS_W && J >= 0,
-             QH_EINVAL, "qh_mcwf_advance: FP64 ensemble of M trajectories of at most 64 amplitudes required");
+                 ens->n == M * A->rows && A->rows >= 1 && A->rows <= ENS_WMAX && J >= 0,
+             QH_EINVAL, "qh_mcwf_advance: FP64 ensemble of M trajectories of at most 1024 amplitudes required");
   const int32_t N = (int32_t)A->rows;
   rc = qh_mat_max_row(const_cast<qh_mat *>(A));  // cached on the matrix
   if (rc) return rc;
@@ -326,6 +401,25 @@ extern "C" int qh_mcwf_advance(const qh_mat *A, qh_vec *ens, int64_t M, const do
   // generator entries per lane: the row length rounded up to 2, 3, 4 or 16
   // (each padded entry costs four FP64 FMAs per stage)
   const int nzc = A->max_row <= 2 ? 2 : A->max_row <= 3 ? 3 : A->max_row <= 4 ? 4 : ENS_MAXNZ;
+  if (N > ENS_W) {  // one workgroup of whole wavefronts per trajectory
+    const dim3 wg((unsigned)((N + 63) / 64 * 64)), tg((unsigned)M);
+    // a 512-thread bound leaves 256 VGPRs per lane (the 16-entry rows spill
+    // under the 1024-thread bound's 128)
+#define QH_MCWF_WG(NZ, LB)                                                                                     \
+  hipLaunchKernelGGL((k_mcwf_run_wg<NZ, LB>), tg, wg, 0, s, N, A->rowptr, A->col, A->val, (double2 *)ens->d,   \
+                     dthr, djs, J, dt, hlast, djump)
+    const int nzw = A->max_row <= 4 ? 4 : A->max_row <= 8 ? 8 : ENS_MAXNZ;
+    if (N <= 512) {
+      if (nzw == 4) QH_MCWF_WG(4, 512);
+      else if (nzw == 8) QH_MCWF_WG(8, 512);
+      else QH_MCWF_WG(ENS_MAXNZ, 512);
+    } else {
+      if (nzw == 4) QH_MCWF_WG(4, 1024);
+      else if (nzw == 8) QH_MCWF_WG(8, 1024);
+      else QH_MCWF_WG(ENS_MAXNZ, 1024);
+    }
+#undef QH_MCWF_WG
+  } else {
 #define QH_MCWF_RUN_NZ(SS, NZ)                                                                                   \
   hipLaunchKernelGGL((k_mcwf_run<SS, NZ>), grid, dim3(256), 0, s, N, M, A->rowptr, A->col, A->val,               \
                      (double2 *)ens->d, dthr, djs, J, dt, hlast, djump)
@@ -348,6 +442,7 @@ extern "C" int qh_mcwf_advance(const qh_mat *A, qh_vec *ens, int64_t M, const do
   QH_MCWF_RUN(64)
 #undef QH_MCWF_RUN
 #undef QH_MCWF_RUN_NZ
+  }
   QH_HIP(hipGetLastError());
   QH_HIP(hipMemcpyAsync(h + bt, djs, 2 * bj, hipMemcpyDeviceToHost, s));
   QH_HIP(hipStreamSynchronize(s));
@@ -361,7 +456,7 @@ extern "C" int qh_mcwf_fits(const qh_mat *A, int nj, int *fits) {
   if (rc) return rc;
   QH_REQUIRE(A && fits, QH_EINVAL, "qh_mcwf_fits: null argument");
   *fits = 0;
-  if (A->rows < 1 || A->rows > ENS_W || nj < 1 || nj > ENS_MAXJ) return QH_OK;
+  if (A->rows < 1 || A->rows > ENS_WMAX || nj < 1 || nj > ENS_MAXJ) return QH_OK;
   rc = qh_mat_max_row(const_cast<qh_mat *>(A));
   if (rc) return rc;
   *fits = A->max_row <= ENS_MAXNZ;
@@ -372,8 +467,8 @@ extern "C" int qh_mcwf_jump_norms(int nj, qh_mat *const *C, const qh_vec *ens, i
                                   int32_t n, double *p) {
   int rc = qh_ensure_init();
   if (rc) return rc;
-  QH_REQUIRE(ens && idx && p && ens->prec == QH_FP64 && N >= 1 && N <= ENS_W && ens->n % N == 0 && n >= 0,
-             QH_EINVAL, "qh_mcwf_jump_norms: FP64 ensemble of at most 64 amplitudes per trajectory required");
+  QH_REQUIRE(ens && idx && p && ens->prec == QH_FP64 && N >= 1 && N <= ENS_WMAX && ens->n % N == 0 && n >= 0,
+             QH_EINVAL, "qh_mcwf_jump_norms: FP64 ensemble of at most 1024 amplitudes per trajectory required");
   JumpOps ops;
   rc = jump_ops(nj, C, N, &ops);
   if (rc || n == 0) return rc;
@@ -401,9 +496,9 @@ extern "C" int qh_mcwf_jump_apply(int nj, qh_mat *const *C, qh_vec *ens, int64_t
                                   const int32_t *k, const double *sre, const double *sim) {
   int rc = qh_ensure_init();
   if (rc) return rc;
-  QH_REQUIRE(ens && idx && k && sre && sim && ens->prec == QH_FP64 && N >= 1 && N <= ENS_W && ens->n % N == 0 &&
+  QH_REQUIRE(ens && idx && k && sre && sim && ens->prec == QH_FP64 && N >= 1 && N <= ENS_WMAX && ens->n % N == 0 &&
                  n >= 0,
-             QH_EINVAL, "qh_mcwf_jump_apply: FP64 ensemble of at most 64 amplitudes per trajectory required");
+             QH_EINVAL, "qh_mcwf_jump_apply: FP64 ensemble of at most 1024 amplitudes per trajectory required");
   JumpOps ops;
   rc = jump_ops(nj, C, N, &ops);
   if (rc || n == 0) return rc;
@@ -422,7 +517,7 @@ extern "C" int qh_mcwf_jump_apply(int nj, qh_mat *const *C, qh_vec *ens, int64_t
   double2 *hs = (double2 *)(h + bi + bk);
   for (int32_t q = 0; q < n; q++) hs[q] = make_double2(sre[q], sim[q]);
   QH_HIP(hipMemcpyAsync(d, h, bi + bk + bs, hipMemcpyHostToDevice, s));
-  hipLaunchKernelGGL(k_mcwf_jump_apply, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s, (int32_t)N,
+  hipLaunchKernelGGL(k_mcwf_jump_apply, dim3((unsigned)n), dim3(256), 0, s, (int32_t)N,
                      (double2 *)ens->d, (const int64_t *)d, n, (const int32_t *)(d + bi),
                      (const double2 *)(d + bi + bk), ops);
   QH_HIP(hipGetLastError());

@@ -23,12 +23,15 @@ def _fixed_rk4():
     qa.set_option("ts_adapt_type", "none")
 
 
-def test_t1_decay_ensemble():
-    """time_step_tests.c:24-101 under MCWF: <n>(t) = exp(-gamma t)"""
+@pytest.mark.parametrize("levels", [2, 70])
+def test_t1_decay_ensemble(levels):
+    """time_step_tests.c:24-101 under MCWF: <n>(t) = exp(-gamma t) from one
+    excitation (a qubit: several trajectories per wavefront; a 70-level mode:
+    one workgroup per trajectory)"""
     _fixed_rk4()
     M, gamma, dt, T = 8000, 1.0, 0.01, 1.0
     qs = qa.System()
-    o = qs.create_op(2)
+    o = qs.create_op(levels)
     qs.add_lin_term(gamma, o)
     qs.use_mcwf_solver(M, 7)
     qs.construct_matrix()
@@ -149,17 +152,22 @@ def _P(a, t):
     return a.ctypes.data_as(ctypes.POINTER(t))
 
 
-@pytest.mark.parametrize("system,row", [("jc2x4", 2), ("qubits2", 3), ("qubits3", 4), ("qubits4", 5)])
+@pytest.mark.parametrize("system,row", [("jc2x4", 2), ("qubits2", 3), ("qubits3", 4), ("qubits4", 5),
+                                        ("jc2x40", 2), ("qubits7", 8), ("qubits10", 11)])
 def test_mcwf_advance_matches_stepwise_ensemble_rk4(system, row):
     """qh_mcwf_advance (each trajectory runs from its step index to its first
     norm crossing, several trajectories per wavefront) against the ensemble RK4
     kernel taken one step at a time with the crossing found on the host: the
     stop step of every trajectory exactly, the amplitudes to 1e-12.  Covers
     thresholds that are never crossed, a trajectory that has already taken all
-    steps, a late start and a shorter last step; generator rows of 3, 4 and 5
-    entries (3-, 4- and 16-entry register layouts; 8 and 16 lanes per
-    trajectory)."""
-    qs, A = _jc_generator() if system == "jc2x4" else _qubit_ring_generator(int(system[-1]))
+    steps, a late start and a shorter last step; generator rows of 2 to 5
+    entries (2-, 3-, 4- and 16-entry register layouts; 4 to 16 lanes per
+    trajectory) and, one workgroup per trajectory, 80, 128 and 1024 amplitudes
+    with rows of 2, 8 and 11 entries."""
+    if system.startswith("jc"):
+        qs, A = _jc_generator(levels=int(system[4:]))
+    else:
+        qs, A = _qubit_ring_generator(int(system[6:]))
     L = qa.lib()
     N = A.info()[0]
     rp, _, _ = A.export()
@@ -217,12 +225,14 @@ def test_mcwf_advance_matches_stepwise_ensemble_rk4(system, row):
     qs.destroy()
 
 
-def test_mcwf_jump_norms_and_apply_match_dense_products():
+@pytest.mark.parametrize("levels", [4, 100])
+def test_mcwf_jump_norms_and_apply_match_dense_products(levels):
     """qh_mcwf_jump_norms: ||C_k psi_i||^2 for every listed trajectory and
     operator; qh_mcwf_jump_apply: psi_i <- s_i C_{k_i} psi_i for the listed
-    trajectories only, the others untouched (dense numpy products, 1e-12)"""
-    q1, A1 = _jc_generator()
-    q2, A2 = _jc_generator(gam=0.3, kap=2.0)
+    trajectories only, the others untouched (dense numpy products, 1e-12);
+    8 and 200 amplitudes"""
+    q1, A1 = _jc_generator(levels=levels)
+    q2, A2 = _jc_generator(gam=0.3, kap=2.0, levels=levels)
     L = qa.lib()
     C = [_dense(A1), _dense(A2)]
     N = C[0].shape[0]
