@@ -253,6 +253,8 @@ void apply_circuit_to_sys(qsystem, circuit *, PetscReal);       /* quantum_circu
 void apply_circuit_to_qvec(qsystem, circuit, qvec);             /* quantum_circuits.c:143-159 */
 void apply_gate2(qvec, gate_type, ...);                         /* quantum_circuits.c:776-847 */
 void apply_single_qb_measurement_error_probs(PetscReal *, PetscInt, PetscReal, PetscReal, PetscInt); /* :620-644 */
+void apply_projective_measurement_qubit(qvec, PetscScalar *, PetscInt, ...);                       /* :484-502 */
+void apply_projective_measurement_qubit_list(qvec, PetscScalar *, PetscInt, operator *);           /* :504-582 */
 void apply_single_qb_measurement_error_state(qvec, PetscReal, PetscReal, PetscInt);                 /* :646-688 */
 
 /* ------------------------------------------------------------ src/solver.h */

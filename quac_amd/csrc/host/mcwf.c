@@ -25,7 +25,7 @@
 
 static uint64_t rotl64(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
 
-void quac_rng_seed(qsystem s, uint64_t seed) {
+void quac_rng_seed_state(unsigned long long *q, uint64_t seed) {
   if (seed == 0) seed = (uint64_t)time(NULL) ^ 0x5DEECE66DULL;
   uint64_t z = seed;
   for (int i = 0; i < 4; i++) {
@@ -33,12 +33,15 @@ void quac_rng_seed(qsystem s, uint64_t seed) {
     uint64_t t = z;
     t = (t ^ (t >> 30)) * 0xBF58476D1CE4E5B9ULL;
     t = (t ^ (t >> 27)) * 0x94D049BB133111EBULL;
-    s->mcwf_rng[i] = t ^ (t >> 31);
+    q[i] = t ^ (t >> 31);
   }
 }
 
-double quac_rng_uniform(qsystem s) {
-  unsigned long long *q = s->mcwf_rng;
+void quac_rng_seed(qsystem s, uint64_t seed) { quac_rng_seed_state(s->mcwf_rng, seed); }
+
+double quac_rng_uniform(qsystem s) { return quac_rng_uniform_state(s->mcwf_rng); }
+
+double quac_rng_uniform_state(unsigned long long *q) {
   const uint64_t r = rotl64(q[1] * 5, 7) * 9;
   const uint64_t t = q[1] << 17;
   q[2] ^= q[0];

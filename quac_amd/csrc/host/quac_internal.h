@@ -39,6 +39,9 @@ int quac_ensure_device(void);
 
 /* MCWF trajectories (mcwf.c) */
 void quac_rng_seed(qsystem s, uint64_t seed);
+/* the same generator on a caller's 4-word state (seed 0: from the clock) */
+void quac_rng_seed_state(unsigned long long *q, uint64_t seed);
+double quac_rng_uniform_state(unsigned long long *q);
 double quac_rng_uniform(qsystem s);
 void quac_mcwf_free(qsystem s);
 int quac_mcwf_time_step(qsystem s, qvec x, PetscReal t0, PetscReal tmax, PetscReal dt, PetscInt steps_max,

@@ -635,3 +635,92 @@ void apply_single_qb_measurement_error_state(qvec state, PetscReal p01, PetscRea
   free(x);
   if (rc) QUAC_FAIL(9, "apply_single_qb_measurement_error_state: %s", qh_last_error());
 }
+
+/* ---------------------------------------------------- projective measurement */
+/* the measurement's uniform numbers: xoshiro256** seeded once per process from
+ * the quac_seed option (0 or unset: from the clock) */
+static unsigned long long g_meas_rng[4];
+static int g_meas_seeded = 0;
+
+/* x <- P x on the qubit of op (P: 2 x 2, row-major), for a wavefunction, or
+ * P rho P^dag for a density matrix; x is the host copy of the state */
+static void project_qubit(qvec q, operator op, const double complex P[2][2], double complex *x) {
+  const PetscInt N = q->total_levels, stride = N / (2 * op->n_before);
+  if (q->my_type == WAVEFUNCTION) {
+    for (PetscInt i = 0; i < N; i++) {
+      if ((i / stride) % 2) continue;
+      const double complex a0 = x[i], a1 = x[i + stride];
+      x[i] = P[0][0] * a0 + P[0][1] * a1;
+      x[i + stride] = P[1][0] * a0 + P[1][1] * a1;
+    }
+    return;
+  }
+  /* rows (the fast index of the vector), then columns */
+  for (PetscInt c = 0; c < N; c++)
+    for (PetscInt r = 0; r < N; r++) {
+      if ((r / stride) % 2) continue;
+      double complex *a0 = &x[N * c + r], *a1 = &x[N * c + r + stride];
+      const double complex v0 = *a0, v1 = *a1;
+      *a0 = P[0][0] * v0 + P[0][1] * v1;
+      *a1 = P[1][0] * v0 + P[1][1] * v1;
+    }
+  for (PetscInt c = 0; c < N; c++) {
+    if ((c / stride) % 2) continue;
+    for (PetscInt r = 0; r < N; r++) {
+      double complex *b0 = &x[N * c + r], *b1 = &x[N * (c + stride) + r];
+      const double complex v0 = *b0, v1 = *b1;
+      *b0 = v0 * conj(P[0][0]) + v1 * conj(P[0][1]);
+      *b1 = v0 * conj(P[1][0]) + v1 * conj(P[1][1]);
+    }
+  }
+}
+
+/* quantum_circuits.c:484-582: measure each listed Pauli operator (sigma_x / y /
+ * z of a qubit) in turn: outcome k (eigenvalue evals[k], eigenvector evecs[k])
+ * with probability (1 + evals[k] <op>) / 2, the state projected on it and
+ * renormalised; *meas_val is the product of the outcomes' eigenvalues.  (The
+ * reference draws eigenvector 0 with probability (1 + <op>) / 2, which is the
+ * same for sigma_z, whose evals[0] = +1.) */
+void apply_projective_measurement_qubit_list(qvec q, PetscScalar *meas_val, PetscInt num_ops, operator *ops) {
+  *meas_val = 1.0;
+  if (q->my_type != DENSITY_MATRIX && q->my_type != WAVEFUNCTION)
+    QUAC_FAIL(9, "apply_projective_measurement_qubit: a wavefunction or a density matrix is required");
+  if (!g_meas_seeded) {
+    quac_rng_seed_state(g_meas_rng, (uint64_t)quac_opt_int("quac_seed", 0));
+    g_meas_seeded = 1;
+  }
+  for (PetscInt i = 0; i < num_ops; i++) {
+    operator op = ops[i];
+    if (op->my_levels != 2 || (op->my_op_type != SIGMA_X && op->my_op_type != SIGMA_Y && op->my_op_type != SIGMA_Z))
+      QUAC_FAIL(9, "apply_projective_measurement_qubit: Pauli operators of qubits only");
+    PetscScalar ev = 0.0;
+    get_expectation_value_qvec_list(q, &ev, 1, &op);
+    double p0 = 0.5 * (1.0 + creal(op->evals[0]) * creal(ev));
+    p0 = p0 < 0.0 ? 0.0 : p0 > 1.0 ? 1.0 : p0;
+    const int k = quac_rng_uniform_state(g_meas_rng) < p0 ? 0 : 1;
+    const double prob = k == 0 ? p0 : 1.0 - p0;
+    if (!(prob > 0.0)) QUAC_FAIL(9, "apply_projective_measurement_qubit: outcome of zero probability");
+    double complex P[2][2];
+    for (int a = 0; a < 2; a++)
+      for (int b = 0; b < 2; b++) P[a][b] = op->evecs[k][a] * conj(op->evecs[k][b]);
+    double complex *x = host_state(q);
+    if (!x) return;
+    project_qubit(q, op, P, x);
+    const double sc = q->my_type == DENSITY_MATRIX ? 1.0 / prob : 1.0 / sqrt(prob);
+    for (PetscInt j = 0; j < q->n; j++) x[j] *= sc;
+    const int rc = qh_vec_set(q->data, (const double *)x);
+    free(x);
+    if (rc) QUAC_FAIL(9, "apply_projective_measurement_qubit: %s", qh_last_error());
+    *meas_val *= op->evals[k];
+  }
+}
+
+void apply_projective_measurement_qubit(qvec q, PetscScalar *meas_val, PetscInt num_ops, ...) {
+  if (num_ops < 1 || num_ops > 64) QUAC_FAIL(9, "apply_projective_measurement_qubit: 1..64 operators");
+  operator ops[64];
+  va_list ap;
+  va_start(ap, num_ops);
+  for (PetscInt i = 0; i < num_ops; i++) ops[i] = va_arg(ap, operator);
+  va_end(ap);
+  apply_projective_measurement_qubit_list(q, meas_val, num_ops, ops);
+}

@@ -5,6 +5,7 @@ Pinned by the reference's own known-answer tests (tests/qvec_utilities_tests.c,
 values restated here with their line numbers) and by the textbook definitions
 computed independently with numpy / scipy on random complex states."""
 import ctypes
+import math
 
 import numpy as np
 import pytest
@@ -268,3 +269,53 @@ def test_readout_error_on_a_density_matrix_vector():
         want[i1] = 0.05 * x[i0] + 0.9 * x[i1]
     assert np.abs(q.get() - want).max() < 1e-15
     _free(q)
+
+
+@pytest.mark.parametrize("kind", ["wf", "dm"])
+def test_projective_measurement_of_a_bell_pair(kind):
+    """apply_projective_measurement_qubit_list (quantum_circuits.c:484-582) on
+    (|00> + |11>)/sqrt(2): sigma_z of qubit 0 gives +-1 with probability 1/2 and
+    collapses the pair, so sigma_z of qubit 1 then agrees (product +1) and the
+    state is |00><00| or |11><11|; a following sigma_x of qubit 0 gives +-1 with
+    probability 1/2 and leaves the trace / norm 1 (generator seeded by quac_seed
+    on its first use in the process)"""
+    qa.set_option("quac_seed", "17")
+    qs = qa.System()
+    q0, q1 = qs.create_op(2), qs.create_op(2)
+    qs.add_ham_term(1.0, q0.n)
+    if kind == "dm":
+        qs.add_lin_term(0.1, q0)
+    qs.construct_matrix()
+    L = qa.lib()
+    f = L.apply_projective_measurement_qubit_list
+    f.restype = None
+    f.argtypes = [ctypes.c_void_p, ctypes.POINTER(qa.Cx), ctypes.c_int64, ctypes.c_void_p]
+
+    def measure(x, ops):
+        mv = qa.Cx()
+        arr = qa._oparray(ops)
+        f(x.h, ctypes.byref(mv), len(ops), ctypes.cast(arr, ctypes.c_void_p))
+        qa._check()
+        return mv.value()
+
+    outcomes = []
+    for trial in range(200):
+        x = qs.create_qvec()
+        if kind == "dm":
+            for i in (0, 3):
+                for j in (0, 3):
+                    x.add(0.5, i, j)
+        else:
+            x.add(2 ** -0.5, 0)
+            x.add(2 ** -0.5, 3)
+        assert abs(measure(x, [q0.sig_z, q1.sig_z]) - 1.0) < 1e-12
+        v = x.get()
+        probs = (np.abs(v) ** 2 if kind == "wf" else _rho(x).diagonal().real)
+        assert abs(probs.sum() - 1) < 1e-12 and (abs(probs[0] - 1) < 1e-12 or abs(probs[3] - 1) < 1e-12)
+        outcomes.append(probs[0] > 0.5)
+        assert abs(abs(measure(x, [q0.sig_x])) - 1.0) < 1e-12
+        tot = np.sum(np.abs(x.get()) ** 2) if kind == "wf" else _rho(x).trace().real
+        assert abs(tot - 1) < 1e-12
+    # binomial(200, 1/2): 4 standard deviations
+    assert abs(sum(outcomes) - 100) < 4 * math.sqrt(50)
+    qs.destroy()
