@@ -1000,16 +1000,43 @@ int autotune(qh_mat *A, int *out_g) {
   return QH_OK;
 }
 
+// The autotuner's choice is remembered per (precision, rows, nnz) for the rest
+// of the process, so the matrices of a parameter sweep (one structure, other
+// values) are timed once; the variant decides only the speed (every variant
+// computes the same product).
+struct TuneMemo {
+  int prec;
+  int64_t rows, nnz;
+  int g;
+};
+static TuneMemo g_tune[64];
+static int g_ntune = 0;
+
+template <typename R>
+int autotune_memo(qh_mat *A, int *out_g) {
+  const int prec = sizeof(R) == 8 ? QH_FP64 : QH_FP32;
+  for (int i = 0; i < g_ntune && i < 64; i++)
+    if (g_tune[i].prec == prec && g_tune[i].rows == A->rows && g_tune[i].nnz == A->nnz) {
+      *out_g = g_tune[i].g;
+      return QH_OK;
+    }
+  int rc = autotune<R>(A, out_g);
+  if (rc) return rc;
+  g_tune[g_ntune % 64] = TuneMemo{prec, A->rows, A->nnz, *out_g};
+  g_ntune++;
+  return QH_OK;
+}
+
 int lanes_for(qh_mat *A, int prec, int *g) {
   int rc = QH_OK;
   if (prec == QH_FP64) {
-    if (!A->lanes64) rc = autotune<double>(A, &A->lanes64);
+    if (!A->lanes64) rc = autotune_memo<double>(A, &A->lanes64);
     if (!rc && QH_IS_KSELL(A->lanes64)) rc = ks_build(A);
     if (!rc && QH_IS_KSELL(A->lanes64) && A->ks_state != 1) A->lanes64 = 8;
     *g = A->lanes64;
   } else {
     if (!A->val32) rc = qh_mat_make_fp32(A);
-    if (!rc && !A->lanes32) rc = autotune<float>(A, &A->lanes32);
+    if (!rc && !A->lanes32) rc = autotune_memo<float>(A, &A->lanes32);
     if (!rc && QH_IS_KSELL(A->lanes32)) rc = ks_build(A);
     if (!rc && QH_IS_KSELL(A->lanes32)) rc = ks_make_fp32(A);
     if (!rc && QH_IS_KSELL(A->lanes32) && A->ks_state != 1) A->lanes32 = 8;

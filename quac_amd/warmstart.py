@@ -58,6 +58,7 @@ class WarmStart:
         t0 = time.perf_counter()
         feats = np.asarray(feats, np.float64)
         n = feats.shape[0]
+        batch = max(1, min(batch, n))          # a training set smaller than one batch is one batch
         self.fmean = feats.mean(0)
         self.fstd = feats.std(0) + 1e-12
         X = np.asarray(sols, np.complex64)

@@ -63,3 +63,21 @@ def test_warm_start_pipeline_small():
     print(f"Kerr 3x{L}: cold {np.mean(cold):.1f} warm {np.mean(warm):.1f} iterations; "
           f"warm-start error {np.mean(e_net):.3f} (trivial guess {np.mean(e_triv):.3f})")
     assert np.mean(e_net) < 0.95 * np.mean(e_triv)
+
+
+def test_training_set_smaller_than_a_batch():
+    """fit with fewer parameter sets than the batch size trains on all of them
+    as one batch (and predicts Hermitian, unit-trace warm starts)"""
+    train = recipes.kerr_sweep_params(4, 12)
+    feats, sols = [], []
+    for p in train:
+        sol, it, res = _solve(p, tol=1e-10)
+        feats.append(recipes.kerr_features(p))
+        sols.append(sol)
+    ws = WarmStart(N, rank=8, hidden=(32, 32), seed=2)
+    stats = ws.fit(np.array(feats), np.array(sols), epochs=30, batch=250, lr=3e-3)
+    assert stats["rank"] <= 8 and len(stats["losses"]) > 0
+    x0 = ws.predict([recipes.kerr_features(recipes.kerr_sweep_params(5, 1)[0])])[0]
+    rho = x0.reshape(N, N, order="F")
+    assert abs(np.trace(rho) - 1) < 1e-10 and np.abs(rho - rho.conj().T).max() < 1e-12
+    ws.destroy()
