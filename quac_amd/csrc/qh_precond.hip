@@ -21,7 +21,8 @@
 // (zheevd returned non-finite vectors for N >= 216 with this rocSOLVER, and
 // zheevj takes 50 ms at N = 512).  Every result is checked (finite, W^dag W = I
 // to 1e-8) before it is used.  Each application costs four N^3 complex GEMMs
-// (rocBLAS) and one elementwise pass.
+// (rocBLAS) and one elementwise pass, or, with a real eigenbasis, four real
+// GEMMs of 2N x N by N x N, two transposes and the elementwise pass.
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
 
@@ -36,6 +37,11 @@ struct qh_pc {
   double2 *W = nullptr;      // eigenvectors, N x N column-major
   double2 *D = nullptr;      // denominators Dm, N x N column-major
   double2 *T = nullptr;      // workspace N x N
+  // real eigenbasis (a Hamiltonian with real matrix elements): W as real N x N,
+  // Dm transposed, a second workspace; the apply then runs real GEMMs
+  double *Wr = nullptr;
+  double2 *Dt = nullptr;
+  double2 *T2 = nullptr;
   rocblas_handle blas = nullptr;
 };
 
@@ -153,6 +159,23 @@ __global__ void k_to_complex(const double *R, int64_t total, double2 *W) {
     W[t] = make_double2(R[t], 0.0);
 }
 
+// out = in^T for N x N complex matrices (32 x 32 tiles through LDS, both sides
+// coalesced); block (32, 8)
+__global__ void __launch_bounds__(256) k_ctranspose(const double2 *__restrict__ in, double2 *__restrict__ out,
+                                                    int64_t n) {
+  __shared__ double2 tile[32][33];
+  const int64_t bx = (int64_t)blockIdx.x * 32, by = (int64_t)blockIdx.y * 32;
+  for (int k = threadIdx.y; k < 32; k += 8) {
+    const int64_t r = bx + threadIdx.x, c = by + k;
+    if (r < n && c < n) tile[k][threadIdx.x] = in[r + n * c];
+  }
+  __syncthreads();
+  for (int k = threadIdx.y; k < 32; k += 8) {
+    const int64_t r = by + threadIdx.x, c = bx + k;
+    if (r < n && c < n) out[r + n * c] = tile[threadIdx.x][k];
+  }
+}
+
 const rocblas_double_complex ONE(1.0, 0.0), ZERO(0.0, 0.0);
 
 // eigensolvers in the order they are tried
@@ -165,6 +188,9 @@ extern "C" int qh_pc_destroy(qh_pc *pc) {
   qh_free_ptr(pc->W);
   qh_free_ptr(pc->D);
   qh_free_ptr(pc->T);
+  qh_free_ptr(pc->Wr);
+  qh_free_ptr(pc->Dt);
+  qh_free_ptr(pc->T2);
   if (pc->blas) rocblas_destroy_handle(pc->blas);
   delete pc;
   return QH_OK;
@@ -288,6 +314,10 @@ extern "C" int qh_pc_create_hbasis(const qh_mat *heff, qh_pc **out) {
       QH_HIP(hipMemcpyAsync(&hflag, flag, sizeof(int), hipMemcpyDeviceToHost, s));
       QH_HIP(hipStreamSynchronize(s));
       ok = (hinfo0 == 0 && hflag == 0);
+      if (ok && real) {  // keep the real eigenvectors for the real-GEMM apply
+        pc->Wr = R;
+        R = nullptr;
+      }
     }
     qh_free_ptr(flag);
     qh_free_ptr(R);
@@ -323,6 +353,11 @@ extern "C" int qh_pc_create_hbasis(const qh_mat *heff, qh_pc **out) {
   for (int64_t i = 0; i < n; i++) scale = std::max(scale, std::max(std::fabs(hE[i]), std::fabs(hg[i])));
   const double floor_ = (scale > 0.0 ? scale : 1.0) * 1e-10;
   hipLaunchKernelGGL(k_denoms, dim3(grid1(nn)), dim3(256), 0, s, E, g, n, floor_, pc->D);
+  if (pc->Wr) {
+    if (qh_alloc_t(&pc->Dt, nn) || qh_alloc_t(&pc->T2, nn)) return fail(QH_ENOMEM);
+    const dim3 tg((unsigned)((n + 31) / 32), (unsigned)((n + 31) / 32));
+    hipLaunchKernelGGL(k_ctranspose, tg, dim3(32, 8), 0, s, pc->D, pc->Dt, n);
+  }
   if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s) != hipSuccess) {
     qh_set_error("qh_pc_create_hbasis: denominator kernel failed");
     return fail(QH_EHIP);
@@ -336,6 +371,29 @@ extern "C" int qh_pc_create_hbasis(const qh_mat *heff, qh_pc **out) {
 int qh_pc_apply_dev(const qh_pc *pc, const double2 *v, double2 *z) {
   const rocblas_int n = (rocblas_int)pc->n;
   hipStream_t s = qh_stream();
+  if (pc->Wr) {
+    // W real: a complex N x N matrix column-major is a real 2N x N matrix
+    // (rows re, im interleaved), and X W = its real product with W.  With
+    // Y = W^T V W and Z = Y ./ Dm:  U = V W;  A = (U^T W) ./ Dm^T = Z^T;
+    // B = A W^T = (W Z)^T;  z = B^T W^T = W Z W^T.
+    const double one = 1.0, zero = 0.0;
+    const rocblas_int m2 = 2 * n;
+    const dim3 tg((unsigned)((n + 31) / 32), (unsigned)((n + 31) / 32));
+    double *Z = (double *)z, *T = (double *)pc->T, *T2 = (double *)pc->T2;
+    QH_RB(rocblas_dgemm(pc->blas, rocblas_operation_none, rocblas_operation_none, m2, n, n, &one,
+                        (const double *)v, m2, pc->Wr, n, &zero, Z, m2));
+    hipLaunchKernelGGL(k_ctranspose, tg, dim3(32, 8), 0, s, z, pc->T, pc->n);
+    QH_RB(rocblas_dgemm(pc->blas, rocblas_operation_none, rocblas_operation_none, m2, n, n, &one, T, m2, pc->Wr, n,
+                        &zero, Z, m2));
+    hipLaunchKernelGGL(k_cdiv_inplace, dim3(grid1(pc->n * pc->n)), dim3(256), 0, s, z, pc->Dt, pc->n * pc->n);
+    QH_RB(rocblas_dgemm(pc->blas, rocblas_operation_none, rocblas_operation_transpose, m2, n, n, &one, Z, m2,
+                        pc->Wr, n, &zero, T, m2));
+    hipLaunchKernelGGL(k_ctranspose, tg, dim3(32, 8), 0, s, pc->T, pc->T2, pc->n);
+    QH_RB(rocblas_dgemm(pc->blas, rocblas_operation_none, rocblas_operation_transpose, m2, n, n, &one, T2, m2,
+                        pc->Wr, n, &zero, Z, m2));
+    QH_HIP(hipGetLastError());
+    return QH_OK;
+  }
   const rocblas_double_complex *W = (const rocblas_double_complex *)pc->W;
   rocblas_double_complex *T = (rocblas_double_complex *)pc->T;
   rocblas_double_complex *Z = (rocblas_double_complex *)z;
