@@ -210,6 +210,10 @@ int qh_pc_destroy(qh_pc *pc);
 #define QH_PC_HBASIS 2
 int qh_gmres_pc(const qh_mat *A, int64_t tot, qh_vec *x, int restart, double tol, int64_t maxit,
                 int pc_kind, const qh_pc *pc, int64_t *iters, double *res);
+/* eigenvalues w (ascending) and, if V != NULL, eigenvectors (column-major) of a
+ * real symmetric n x n host matrix, on the device (rocSOLVER); used by the
+ * density-matrix fidelity of get_fidelity_qvec (qvec_utilities.c:2339-2430) */
+int qh_sym_eig(int64_t n, const double *A, double *w, double *V);
 
 /* ---------------------------------------------------------- MCWF ensembles */
 /*

@@ -142,8 +142,28 @@ void ptrace_over_list_qvec(qvec full_state, PetscInt n_ops, PetscInt *op_loc_lis
 /* eigenvalues (w) and, if V, eigenvectors (columns of V, row-major n x n) of a
  * real symmetric row-major matrix A (destroyed): cyclic Jacobi rotations
  * (Golub & Van Loan, Matrix Computations, sec. 8.5) until the off-diagonal
- * mass is below 1e-30 of the total */
-static void sym_eig(int n, double *A, double *w, double *V) {
+ * mass is below 1e-30 of the total; above n = 128 the device eigensolver
+ * (qh_sym_eig, O(n^3) once instead of per sweep) */
+static void sym_eig_jacobi(int n, double *A, double *w, double *V);
+static int sym_eig(int n, double *A, double *w, double *V) {
+  if (n <= 128) {
+    sym_eig_jacobi(n, A, w, V);
+    return 0;
+  }
+  double *C = V ? (double *)malloc(sizeof(double) * (size_t)n * (size_t)n) : NULL;
+  if (V && !C) return quac_fail(9, "out of host memory"), -1;
+  if (qh_sym_eig(n, A, w, C)) {
+    free(C);
+    return quac_fail(9, "fidelity eigensolver: %s", qh_last_error()), -1;
+  }
+  if (V) /* column-major C (column k = eigenvector k) to row-major V */
+    for (int i = 0; i < n; i++)
+      for (int k = 0; k < n; k++) V[(size_t)i * n + k] = C[(size_t)k * n + i];
+  free(C);
+  return 0;
+}
+
+static void sym_eig_jacobi(int n, double *A, double *w, double *V) {
   if (V)
     for (int i = 0; i < n; i++)
       for (int j = 0; j < n; j++) V[i * n + j] = i == j;
@@ -222,7 +242,10 @@ static int fidelity_dm_dm(PetscInt N, const double complex *rho, const double co
     return quac_fail(9, "out of host memory"), -1;
   }
   embed(N, rho, R);
-  sym_eig((int)M, R, w, V);
+  if (sym_eig((int)M, R, w, V)) {
+    free(R), free(V), free(S), free(T), free(w);
+    return -1;
+  }
   /* R = sqrt(rho) embedded = V diag(sqrt(max(w, 0))) V^T */
   for (PetscInt i = 0; i < M; i++)
     for (PetscInt k = 0; k < M; k++) T[i * M + k] = V[i * M + k] * sqrt(w[k] > 0 ? w[k] : 0.0);
@@ -238,7 +261,10 @@ static int fidelity_dm_dm(PetscInt N, const double complex *rho, const double co
   /* symmetrise against rounding before the eigenvalues */
   for (PetscInt i = 0; i < M; i++)
     for (PetscInt j = i + 1; j < M; j++) S[i * M + j] = S[j * M + i] = 0.5 * (S[i * M + j] + S[j * M + i]);
-  sym_eig((int)M, S, w, NULL);
+  if (sym_eig((int)M, S, w, NULL)) {
+    free(R), free(V), free(S), free(T), free(w);
+    return -1;
+  }
   double f = 0.0;
   for (PetscInt i = 0; i < M; i++)
     if (w[i] > 0) f += sqrt(w[i]);

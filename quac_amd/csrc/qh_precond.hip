@@ -367,6 +367,64 @@ extern "C" int qh_pc_create_hbasis(const qh_mat *heff, qh_pc **out) {
   return QH_OK;
 }
 
+// Eigenvalues w (ascending) and, if V, orthonormal eigenvectors of the real
+// symmetric n x n host matrix A (either storage order; V column-major, column
+// k the eigenvector of w[k]) on the device: rocSOLVER dsyevd, then dsyevdj if
+// the result is not finite.
+extern "C" int qh_sym_eig(int64_t n, const double *A, double *w, double *V) {
+  int rc = qh_ensure_init();
+  if (rc) return rc;
+  QH_REQUIRE(A && w && n >= 1 && n <= 32768, QH_EINVAL, "qh_sym_eig: 1 <= n <= 32768 required");
+  hipStream_t s = qh_stream();
+  rocblas_handle h = nullptr;
+  if (rocblas_create_handle(&h) != rocblas_status_success || rocblas_set_stream(h, s) != rocblas_status_success) {
+    if (h) rocblas_destroy_handle(h);
+    qh_set_error("qh_sym_eig: rocBLAS handle");
+    return QH_EHIP;
+  }
+  double *dA = nullptr, *dw = nullptr, *de = nullptr;
+  rocblas_int *info = nullptr;
+  rc = qh_alloc_t(&dA, (size_t)(n * n));
+  if (!rc) rc = qh_alloc_t(&dw, (size_t)n);
+  if (!rc) rc = qh_alloc_t(&de, (size_t)n);
+  if (!rc) rc = qh_alloc_t(&info, 1);
+  std::vector<double> hv;
+  int ok = 0;
+  for (int method = 0; method < 2 && !rc && !ok; method++) {
+    if (hipMemcpyAsync(dA, A, sizeof(double) * n * n, hipMemcpyHostToDevice, s) != hipSuccess) {
+      rc = QH_EHIP;
+      break;
+    }
+    const rocblas_evect ev = V ? rocblas_evect_original : rocblas_evect_none;
+    rocblas_status st = method == 0
+                            ? rocsolver_dsyevd(h, ev, rocblas_fill_lower, (rocblas_int)n, dA, (rocblas_int)n, dw, de, info)
+                            : rocsolver_dsyevdj(h, ev, rocblas_fill_lower, (rocblas_int)n, dA, (rocblas_int)n, dw, info);
+    if (st != rocblas_status_success) continue;
+    int hinfo = 1;
+    if (hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipMemcpyAsync(w, dw, sizeof(double) * n, hipMemcpyDeviceToHost, s) != hipSuccess ||
+        (V && hipMemcpyAsync(V, dA, sizeof(double) * n * n, hipMemcpyDeviceToHost, s) != hipSuccess) ||
+        hipStreamSynchronize(s) != hipSuccess) {
+      rc = QH_EHIP;
+      break;
+    }
+    ok = hinfo == 0;
+    for (int64_t i = 0; ok && i < n; i++) ok = std::isfinite(w[i]);
+    for (int64_t i = 0; ok && V && i < n * n; i++) ok = std::isfinite(V[i]);
+  }
+  qh_free_ptr(dA);
+  qh_free_ptr(dw);
+  qh_free_ptr(de);
+  qh_free_ptr(info);
+  rocblas_destroy_handle(h);
+  if (rc) {
+    qh_set_error("qh_sym_eig: device copy failed");
+    return rc;
+  }
+  QH_REQUIRE(ok, QH_EHIP, "qh_sym_eig: no eigensolver produced a finite result");
+  return QH_OK;
+}
+
 // z = M^-1 v for vec layouts of N x N matrices (column-major, loc = N*col + row)
 int qh_pc_apply_dev(const qh_pc *pc, const double2 *v, double2 *z) {
   const rocblas_int n = (rocblas_int)pc->n;
