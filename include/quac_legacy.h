@@ -43,6 +43,7 @@ void add_lin_mult2(PetscScalar, operator, operator);                   /* :743-8
 void add_lin_mat(PetscScalar, Mat);                                    /* :837-901  */
 void print_dense_ham(void);                                            /* :39-48    */
 void set_initial_pop(operator, double);                                /* :1386-1398 */
+void combine_ops_to_mat(Mat *, PetscInt, int, ...);                    /* :923-994  */
 
 /* dm_utilities.c */
 void create_dm(Vec *, PetscInt);                                       /* :616-623  */

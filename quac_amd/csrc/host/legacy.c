@@ -119,6 +119,26 @@ void create_vec(int number_of_levels, vec_op *new_vec) {
 void destroy_op(operator *op) { destroy_op_sys(op); }
 void destroy_vec(vec_op *op) { destroy_vec_op_sys(op); }
 
+/* operators.c:923-994: the total_levels x total_levels matrix of the product
+ * op1 op2 ... (subsystems not named act as the identity; no operators: the
+ * identity), as a device CSR the caller destroys with qh_mat_destroy */
+void combine_ops_to_mat(Mat *matrix_out, PetscInt tot_levels, int number_of_ops, ...) {
+  *matrix_out = NULL;
+  if (!g_sys || num_subsystems == 0) QUAC_FAIL(0, "You need to create operators before combine_ops_to_mat!");
+  if (tot_levels != total_levels) QUAC_FAIL(9, "combine_ops_to_mat: tot_levels must be total_levels");
+  if (number_of_ops < 0 || number_of_ops > 64) QUAC_FAIL(9, "combine_ops_to_mat: 0..64 operators");
+  operator ops[64];
+  va_list ap;
+  va_start(ap, number_of_ops);
+  for (int i = 0; i < number_of_ops; i++) ops[i] = va_arg(ap, operator);
+  va_end(ap);
+  if (number_of_ops == 0) {
+    ops[0] = subsystem_list[0]->my_op_type == VEC ? subsystem_list[0]->vec_op_list[0]->eye : subsystem_list[0]->eye;
+    number_of_ops = 1;
+  }
+  construct_op_matrix_wf_list(g_sys, 1.0, matrix_out, number_of_ops, ops);
+}
+
 void no_lindblad_terms(void) {} /* operators.c:278-280: a hint; the solver follows the terms */
 
 void print_dense_ham(void) {} /* operators.c:39-48: the dense text dump is not produced */

@@ -10,6 +10,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 
+#include "quac_hip.h"
 #include "quac_legacy.h"
 
 int main(int argc, char **args) {
@@ -31,6 +32,23 @@ int main(int argc, char **args) {
   time_step(rho, 0.0, 2.0, 0.01, 100000);
   get_populations(rho, &pops);
   printf("pops %d %.15e %.15e %.15e %.15e\n", get_num_populations(), pops[0], pops[1], pops[2], pops[3]);
+  /* combine_ops_to_mat (operators.c:923-994): |2><2| (x) I and the identity */
+  for (int k = 0; k < 2; k++) {
+    Mat m = NULL;
+    if (k == 0) combine_ops_to_mat(&m, total_levels, 2, atom[2], atom[2]);
+    else combine_ops_to_mat(&m, total_levels, 0);
+    int64_t rows = 0, nnz = 0;
+    qh_mat_info(m, &rows, &nnz);
+    int64_t rp[16], col[64];
+    double val[128];
+    qh_mat_export(m, rp, col, val);
+    double tr = 0.0;
+    for (int64_t r = 0; r < rows; r++)
+      for (int64_t e = rp[r]; e < rp[r + 1]; e++)
+        if (col[e] == r) tr += val[2 * e];
+    printf("comb%d %lld %.15f\n", k, (long long)rows, tr);
+    qh_mat_destroy(m);
+  }
   destroy_dm(rho);
   free(pops);
   QuaC_finalize();

@@ -96,6 +96,10 @@ def test_legacy_api_vec_decay_closed_form(tmp_path, opts, tol):
     p0, p1, p2, nc = map(float, f[2:])
     e = math.exp(-1.0)
     assert abs(p1 - 0.75 * e) < tol and abs(p0 - 0.75 * (1 - e)) < tol and abs(p2 - 0.25) < tol and abs(nc) < tol, f
+    # combine_ops_to_mat: |2><2| (x) I over 9 levels has trace 3; no operators: the identity
+    c0 = [l.split() for l in text.splitlines() if l.startswith("comb0")][0]
+    c1 = [l.split() for l in text.splitlines() if l.startswith("comb1")][0]
+    assert c0[1] == "9" and abs(float(c0[2]) - 3.0) < 1e-15 and c1[1] == "9" and abs(float(c1[2]) - 9.0) < 1e-15
 
 
 def test_legacy_api_steady_state_closed_form(tmp_path):
