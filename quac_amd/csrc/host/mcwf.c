@@ -231,7 +231,7 @@ static int run_to_end(qsystem s, Vec block, int64_t M, int64_t N, int32_t J, dou
 }
 
 int quac_mcwf_time_step(qsystem s, qvec x, PetscReal t0, PetscReal tmax, PetscReal dt, PetscInt steps_max,
-                        int (*monitor)(qsystem, PetscInt, PetscReal, qvec)) {
+                        int (*monitor)(qsystem, PetscInt, PetscReal, qvec), PetscInt *nsteps) {
   if (x->my_type != WF_ENSEMBLE) return quac_fail(0, "time_step_sys with use_mcwf_solver needs a qvec from create_qvec_sys");
   if (build_jumps(s) || spawn(s, x)) return 9;
   const int64_t M = x->n_ensemble, N = x->n;
@@ -270,6 +270,7 @@ int quac_mcwf_time_step(qsystem s, qvec x, PetscReal t0, PetscReal tmax, PetscRe
   free(nrm);
   qh_vec_destroy(work);
   s->current_time = t;
+  if (nsteps) *nsteps = step;
   return rc;
 }
 

@@ -575,8 +575,12 @@ void time_step_sys(qsystem qsys, qvec x, PetscReal init_time, PetscReal time_max
   if (dt <= 0) QUAC_FAIL(0, "time_step_sys: dt must be positive");
   if (qsys->mcwf_solver) {
     /* trajectories of the ensemble (qsystem.c:1024-1079) */
-    if (quac_mcwf_time_step(qsys, x, init_time, time_max, dt, steps_max, qsys->ts_monitor ? mcwf_monitor : NULL))
+    PetscInt nsteps = 0;
+    if (quac_mcwf_time_step(qsys, x, init_time, time_max, dt, steps_max, qsys->ts_monitor ? mcwf_monitor : NULL,
+                            &nsteps))
       QUAC_FAIL(9, "time_step_sys (MCWF) failed");
+    g_last_steps = nsteps;
+    g_last_t = qsys->current_time;
     return;
   }
 

@@ -45,7 +45,7 @@ double quac_rng_uniform_state(unsigned long long *q);
 double quac_rng_uniform(qsystem s);
 void quac_mcwf_free(qsystem s);
 int quac_mcwf_time_step(qsystem s, qvec x, PetscReal t0, PetscReal tmax, PetscReal dt, PetscInt steps_max,
-                        int (*monitor)(qsystem, PetscInt, PetscReal, qvec));
+                        int (*monitor)(qsystem, PetscInt, PetscReal, qvec), PetscInt *nsteps);
 
 /* options (quac_unset_option: forget a name set by quac_set_option) */
 void quac_unset_option(const char *name);

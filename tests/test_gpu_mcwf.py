@@ -278,3 +278,25 @@ def test_device_loop_is_deterministic_for_a_seed():
         vals.append(x.expectation(o.n))
         qs.destroy()
     assert vals[0] == vals[1]
+
+
+def test_device_loop_respects_steps_max_and_reports_stats():
+    """time_step_sys under MCWF stops after steps_max steps (the step-by-step
+    rule, also on the device loop) and reports the steps taken and the final
+    time through quac_ts_stats; a second call continues from there"""
+    _fixed_rk4()
+    qs = qa.System()
+    o = qs.create_op(3)
+    qs.add_ham_term(0.4, o.n)
+    qs.add_lin_term(0.5, o)
+    qs.use_mcwf_solver(200, 4)
+    qs.construct_matrix()
+    x = qs.create_qvec()
+    x.add(1.0, 2)
+    steps, t = qs.time_step(x, 0.0, 1.0, 0.01, 37)      # returns quac_ts_stats
+    assert steps == 37 and abs(t - 0.37) < 1e-12
+    steps, t = qs.time_step(x, t, 1.0, 0.01, 1000)
+    assert steps == 63 and abs(t - 1.0) < 1e-12
+    e = x.expectation(o.n).real
+    assert 0.0 <= e <= 2.0
+    qs.destroy()
