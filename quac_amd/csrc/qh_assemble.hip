@@ -42,51 +42,56 @@ __device__ inline cxd cconj(cxd a) { return cxd_make(a.re, -a.im); }
 enum { M_IG = 0, M_GI = 1, M_GG = 2 };
 
 // _get_val_j_ops (kron.c:161-207) with the per-op rule of kron.c:294-538:
-// j = i + (next(sub) - sub) * n_after with sub = i / n_after % levels.
+// j = i + (next(sub) - sub) * n_after with sub = i / n_after % levels.  Every
+// index and stride is below the row count (< 2^32), so the digit extraction
+// uses 32-bit unsigned division (64-bit division is a long emulated sequence on
+// the GPU).
+__device__ inline uint32_t digit(uint32_t x, uint32_t stride, uint32_t levels) { return (x / stride) % levels; }
+
 __device__ bool chain_fwd(const Ctx &c, const qh_op *ops, int n, uint32_t i, int mode, uint32_t &j, cxd &v) {
-  int64_t cur = i;
+  uint32_t cur = i;
   cxd val = cxd_make(1.0, 0.0);
   for (int k = 0; k < n; k++) {
     const qh_op op = ops[k];
-    const int64_t L = op.levels, s = op.stride;
+    const uint32_t L = (uint32_t)op.levels, s = (uint32_t)op.stride;
     if (mode == M_GG) {
-      int64_t i1 = cur / c.tot, i2 = cur - i1 * c.tot;
-      int64_t s1 = (i1 / s) % L, s2 = (i2 / s) % L;
+      uint32_t i1 = cur / c.tot, i2 = cur - i1 * c.tot;
+      const uint32_t s1 = digit(i1, s, L), s2 = digit(i2, s, L);
       const qh_level e1 = c.tab[op.tab + s1], e2 = c.tab[op.tab + s2];
       if (e1.next < 0 || e2.next < 0) return false;
-      i1 += (e1.next - s1) * s;
-      i2 += (e2.next - s2) * s;
+      i1 = (uint32_t)((int64_t)i1 + ((int64_t)e1.next - s1) * s);
+      i2 = (uint32_t)((int64_t)i2 + ((int64_t)e2.next - s2) * s);
       cur = i1 * c.tot + i2;
       cxd t = cmul(cxd_make(e1.re, -e1.im), cxd_make(e2.re, e2.im));
       val = cmul(t, val);
     } else {
-      const int64_t st = (mode == M_GI) ? s * c.tot : s;
-      const int64_t sub = (cur / st) % L;
+      const uint32_t st = (mode == M_GI) ? s * c.tot : s;
+      const uint32_t sub = digit(cur, st, L);
       const qh_level e = c.tab[op.tab + sub];
       if (e.next < 0) return false;
-      cur += (e.next - sub) * st;
+      cur = (uint32_t)((int64_t)cur + ((int64_t)e.next - sub) * st);
       cxd ev = cxd_make(e.re, mode == M_GI ? -e.im : e.im);
       val = cmul(ev, val);
     }
   }
-  j = (uint32_t)cur;
+  j = cur;
   v = val;
   return true;
 }
 
 // inverse of the forward chain (IG or GI): the unique i with chain(i) = r
 __device__ bool chain_inv(const Ctx &c, const qh_op *ops, int n, uint32_t r, int mode, uint32_t &i) {
-  int64_t cur = r;
+  uint32_t cur = r;
   for (int k = n - 1; k >= 0; k--) {
     const qh_op op = ops[k];
-    const int64_t L = op.levels;
-    const int64_t st = (mode == M_GI) ? op.stride * (int64_t)c.tot : op.stride;
-    const int64_t sub = (cur / st) % L;
+    const uint32_t L = (uint32_t)op.levels;
+    const uint32_t st = (mode == M_GI) ? (uint32_t)op.stride * c.tot : (uint32_t)op.stride;
+    const uint32_t sub = digit(cur, st, L);
     const int32_t p = c.tab[op.tab + sub].prev;
     if (p < 0) return false;
-    cur += (p - sub) * st;
+    cur = (uint32_t)((int64_t)cur + ((int64_t)p - sub) * st);
   }
-  i = (uint32_t)cur;
+  i = cur;
   return true;
 }
 
