@@ -251,6 +251,7 @@ void add_gate_to_circuit_sys(circuit *, PetscReal, gate_type, ...); /* quantum_c
 void schedule_circuit_layers(qsystem, circuit *);               /* quantum_circuits.c:316-404 */
 void apply_circuit_to_sys(qsystem, circuit *, PetscReal);       /* quantum_circuits.c:407-431 */
 void apply_circuit_to_qvec(qsystem, circuit, qvec);             /* quantum_circuits.c:143-159 */
+void apply_circuit_to_qvec2(circuit, qvec);                      /* quantum_circuits.c:123-140 */
 void apply_gate2(qvec, gate_type, ...);                         /* quantum_circuits.c:776-847 */
 void apply_single_qb_measurement_error_probs(PetscReal *, PetscInt, PetscReal, PetscReal, PetscInt); /* :620-644 */
 void apply_projective_measurement_qubit(qvec, PetscScalar *, PetscInt, ...);                       /* :484-502 */

@@ -277,10 +277,33 @@ int qc_apply_gate(struct quantum_gate_struct *g, qvec x, int superop, qsystem sy
   return 0;
 }
 
+/* the gate on every trajectory of a spawned MCWF ensemble (quantum_circuits.c:
+ * 127-132, 146-151), otherwise on the state itself */
+static int apply_on_members(struct quantum_gate_struct *g, qvec state, int superop, qsystem sys) {
+  if (state->my_type != WF_ENSEMBLE || !state->ens_spawned) return qc_apply_gate(g, state, superop, sys);
+  struct qvec member = *state;
+  member.my_type = WAVEFUNCTION;
+  for (PetscInt i = 0; i < state->n_ensemble; i++) {
+    Vec v = NULL;
+    if (qh_vec_view(state->ens_datas[0], i * state->n, state->n, &v)) return quac_fail(9, "%s", qh_last_error());
+    member.data = v;
+    const int rc = qc_apply_gate(g, &member, superop, sys);
+    qh_vec_destroy(v);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
 /* quantum_circuits.c:143-159 */
 void apply_circuit_to_qvec(qsystem qsys, circuit circ, qvec state) {
   for (PetscInt i = 0; i < circ.num_gates; i++)
-    if (qc_apply_gate(&circ.gate_list[i], state, 1, qsys)) return;
+    if (apply_on_members(&circ.gate_list[i], state, 1, qsys)) return;
+}
+
+/* quantum_circuits.c:123-140: every gate with the in-place convention of apply_gate2 */
+void apply_circuit_to_qvec2(circuit circ, qvec state) {
+  for (PetscInt i = 0; i < circ.num_gates; i++)
+    if (apply_on_members(&circ.gate_list[i], state, 0, NULL)) return;
 }
 
 /* quantum_circuits.c:776-847 */

@@ -1,6 +1,7 @@
 """Gates (both qubit conventions), observables, PETSc-binary state input, and the
 steady-state GMRES on the device, against the oracle and closed forms (1e-12
 max-abs for gates/observables in FP64)."""
+import ctypes
 import math
 import os
 
@@ -161,4 +162,60 @@ def test_steady_state_kerr_small_vs_oracle():
     assert np.abs(m.spmv(got)).max() < 1e-8
     rho = dense.unvec(got, tot)
     assert abs(np.trace(rho) - 1) < 1e-10 and np.abs(rho - rho.conj().T).max() < 1e-8
+    qs.destroy()
+
+
+def _apply_circuit2(circ, x):
+    f = qa.lib().apply_circuit_to_qvec2
+    f.restype = None
+    f.argtypes = [qa.CircuitStruct, ctypes.c_void_p]
+    f(circ.c, x.h)
+    qa._check()
+
+
+def test_apply_circuit_to_qvec2_inplace_convention_vs_oracle():
+    """apply_circuit_to_qvec2 (quantum_circuits.c:123-140): every gate of the
+    circuit with the in-place convention of apply_gate2, against the oracle"""
+    nq = 3
+    r = {"subs": [("op", 2)] * nq, "terms": [("lin", 0.0, [(0, "n")])], "mcwf": False}
+    qs, _, _ = build_system(r, construct=False)
+    x = qs.create_qvec()
+    rho = _rand_rho(8)
+    x.set(dense.vec(rho))
+    circ = qa.Circuit(10)
+    gates = [("H", 1, (0,), 0), ("CNOT", -1, (0, 1), 0), ("RZ", 8, (2,), 0.4), ("RX", 6, (1,), 2.0)]
+    for g in gates:
+        circ.add_gate(1.0, g[0], list(g[2]), theta=g[3])
+    _apply_circuit2(circ, x)
+    ref = dense.vec(rho)
+    for g in gates:
+        ref = oracle.gate_inplace([2] * nq, True, g[1], g[2], ref, theta=g[3])
+    assert np.abs(x.get() - ref).max() < 1e-12
+    circ.destroy()
+    qs.destroy()
+
+
+def test_circuits_act_on_every_trajectory_of_a_spawned_ensemble():
+    """a spawned MCWF ensemble (quantum_circuits.c:127-132, 146-151): sigma_x on
+    qubit 0 flips subsystem 0 of every trajectory with the superoperator
+    convention (apply_circuit_to_qvec) and the last subsystem with the in-place
+    one (apply_circuit_to_qvec2)"""
+    qa.set_option("ts_rk_type", "4")
+    qa.set_option("ts_adapt_type", "none")
+    qs = qa.System()
+    q0, q1 = qs.create_op(2), qs.create_op(2)
+    qs.add_ham_term(0.0, q0.n)
+    qs.add_lin_term(0.2, q0)
+    qs.use_mcwf_solver(40, 3)
+    qs.construct_matrix()
+    x = qs.create_qvec()
+    x.add(1.0, 0)
+    qs.time_step(x, 0.0, 0.0, 0.01, 10)        # spawns the trajectories, takes no step
+    circ = qa.Circuit(2)
+    circ.add_gate(1.0, "SIGMAX", [0])
+    qs.apply_circuit_to_qvec(circ, x)
+    assert abs(x.expectation(q0.n).real - 1.0) < 1e-12 and abs(x.expectation(q1.n).real) < 1e-12
+    _apply_circuit2(circ, x)
+    assert abs(x.expectation(q0.n).real - 1.0) < 1e-12 and abs(x.expectation(q1.n).real - 1.0) < 1e-12
+    circ.destroy()
     qs.destroy()
