@@ -57,6 +57,17 @@ int get_num_populations(void);                                         /* :1127-
 void get_populations(Vec, double **);                                  /* :1157-1325 */
 void get_expectation_value(Vec, PetscScalar *, int, ...);              /* :1352-1469 */
 void trace_dm(PetscScalar *, Vec);                                     /* :1766-1786 */
+void partial_trace_over(Vec, Vec, int, ...);                           /* :150-260  */
+void partial_trace_keep(Vec, Vec, int, ...);                           /* :468-603  */
+void get_fidelity(Vec, Vec, double *);                                 /* :1618-1722 */
+void get_bipartite_concurrence(Vec, double *);                         /* :1487-1602 */
+void print_dm(Vec, int);                                               /* :13-25    */
+void print_psi(Vec, int);                                              /* :120-132  */
+void print_dm_sparse(Vec, int);                                        /* :32-43    */
+void print_dm_sparse_to_file(Vec, int, char[]);                        /* :51-66    */
+void print_mat_sparse_to_file(Mat, char[]);                            /* :71-92    */
+void print_mat_sparse(Mat);                                            /* :98-113   */
+void get_dm_element_local(Vec, PetscInt, PetscInt, PetscScalar *);     /* :999-1016 */
 
 /* solver.c */
 void steady_state(Vec);                                                /* :33-221   */

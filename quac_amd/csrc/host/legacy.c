@@ -327,6 +327,9 @@ void get_dm_element(Vec dm, PetscInt row, PetscInt col, PetscScalar *val) {
   get_dm_element_qvec(q, row, col, val);
 }
 
+/* dm_utilities.c:999-1016: the same element (one process holds every entry) */
+void get_dm_element_local(Vec dm, PetscInt row, PetscInt col, PetscScalar *val) { get_dm_element(dm, row, col, val); }
+
 /* dm_utilities.c:1049-1061 */
 void add_value_to_dm(Vec dm, PetscInt row, PetscInt col, PetscScalar val) {
   LOOKUP_OR_FAIL(q, dm);
@@ -397,6 +400,183 @@ void trace_dm(PetscScalar *trace_val, Vec dm) {
 void start_circuit_at_time(circuit *circ, PetscReal time) {
   if (!g_sys) QUAC_FAIL(0, "You need to create operators before you start a circuit!");
   apply_circuit_to_sys(g_sys, circ, time);
+}
+
+/* index of the subsystem an operator (or any level of a VEC subsystem) acts on */
+static int subsystem_of(operator op) {
+  for (int i = 0; i < num_subsystems; i++)
+    if (subsystem_list[i]->n_before == op->n_before && subsystem_list[i]->my_levels == op->my_levels) return i;
+  return -1;
+}
+
+/* the reduced density matrix over `traced` (flags per subsystem) into pt */
+static void ptrace_into(Vec full_dm, Vec ptraced_dm, const int *traced) {
+  LOOKUP_OR_FAIL(q, full_dm);
+  LOOKUP_OR_FAIL(p, ptraced_dm);
+  PetscInt list[MAX_SUB], n = 0;
+  for (int i = 0; i < num_subsystems; i++)
+    if (traced[i]) list[n++] = i;
+  qvec red = NULL;
+  ptrace_over_list_qvec(q, n, list, &red);
+  if (!red) return;
+  if (red->n != p->n) {
+    destroy_qvec(&red);
+    QUAC_FAIL(0, "the partial trace has %lld entries but the target density matrix %lld", (long long)red->n,
+              (long long)p->n);
+  }
+  const int rc = qh_vec_copy(red->data, ptraced_dm);
+  destroy_qvec(&red);
+  if (rc) QUAC_FAIL(9, "partial trace: %s", qh_last_error());
+}
+
+/* dm_utilities.c:150-260: trace over the listed subsystems */
+void partial_trace_over(Vec full_dm, Vec ptraced_dm, int number_of_ops, ...) {
+  int traced[MAX_SUB] = {0};
+  va_list ap;
+  va_start(ap, number_of_ops);
+  for (int i = 0; i < number_of_ops; i++) {
+    const int k = subsystem_of(va_arg(ap, operator));
+    if (k >= 0) traced[k] = 1;
+  }
+  va_end(ap);
+  ptrace_into(full_dm, ptraced_dm, traced);
+}
+
+/* dm_utilities.c:468-603: keep the listed subsystems, trace over the others */
+void partial_trace_keep(Vec full_dm, Vec ptraced_dm, int number_of_ops, ...) {
+  int traced[MAX_SUB];
+  for (int i = 0; i < MAX_SUB; i++) traced[i] = 1;
+  va_list ap;
+  va_start(ap, number_of_ops);
+  for (int i = 0; i < number_of_ops; i++) {
+    const int k = subsystem_of(va_arg(ap, operator));
+    if (k >= 0) traced[k] = 0;
+  }
+  va_end(ap);
+  ptrace_into(full_dm, ptraced_dm, traced);
+}
+
+/* dm_utilities.c:1618-1722: Tr sqrt(sqrt(rho) sigma sqrt(rho)), positive
+ * eigenvalues only -- not squared, unlike get_fidelity_qvec, whose value is the
+ * square of this one */
+void get_fidelity(Vec dm, Vec dm_r, double *fidelity) {
+  LOOKUP_OR_FAIL(a, dm);
+  LOOKUP_OR_FAIL(b, dm_r);
+  if (a->n != b->n) QUAC_FAIL(0, "get_fidelity: the density matrices differ in size");
+  PetscReal f2 = -1.0, var = 0.0;
+  get_fidelity_qvec(a, b, &f2, &var);
+  if (f2 < 0.0) return;
+  *fidelity = sqrt(f2);
+}
+
+/* dm_utilities.c:1487-1602: concurrence of a two-qubit density matrix */
+void get_bipartite_concurrence(Vec dm, double *concurrence) {
+  LOOKUP_OR_FAIL(q, dm);
+  if (q->n != 16) QUAC_FAIL(0, "The input density matrix is not 4x4! Concurrence cannot be calculated.");
+  double complex rho[16];
+  QUAC_HIPCHK(qh_vec_get(dm, (double *)rho));
+  quac_concurrence_2q(rho, concurrence);
+}
+
+/* the density matrix as host complex128 after checking h_dim (caller frees) */
+static double complex *dm_block(Vec rho, int h_dim, const char *fn) {
+  qvec q = lookup(rho);
+  if (!q) return quac_fail(9, "%s: not a state made by create_full_dm / create_dm", fn), NULL;
+  if (q->my_type != DENSITY_MATRIX || h_dim < 0 || h_dim > q->total_levels)
+    return quac_fail(0, "%s: h_dim %d does not fit the density matrix", fn, h_dim), NULL;
+  double complex *h = (double complex *)malloc(sizeof(double complex) * (size_t)q->n);
+  if (!h) return quac_fail(9, "out of host memory"), NULL;
+  if (qh_vec_get(rho, (double *)h)) {
+    free(h);
+    return quac_fail(9, "%s: %s", fn, qh_last_error()), NULL;
+  }
+  return h;
+}
+
+/* element (i, j) of an N x N density matrix stored column-major */
+#define DM_AT(h, N, i, j) ((h)[(size_t)(N) * (size_t)(j) + (size_t)(i)])
+
+/* dm_utilities.c:13-25: the leading h_dim x h_dim block of the density matrix */
+void print_dm(Vec rho, int h_dim) {
+  double complex *h = dm_block(rho, h_dim, "print_dm");
+  if (!h) return;
+  const PetscInt N = lookup(rho)->total_levels;
+  for (int i = 0; i < h_dim; i++) {
+    for (int j = 0; j < h_dim; j++) printf("%4.3e + %4.3ei ", creal(DM_AT(h, N, i, j)), cimag(DM_AT(h, N, i, j)));
+    printf("\n");
+  }
+  printf("\n");
+  free(h);
+}
+
+/* dm_utilities.c:32-43 / 51-66: "i j re im" for the entries above 1e-10 in modulus */
+static void dm_sparse(FILE *fp, Vec rho, int h_dim, const char *fn) {
+  double complex *h = dm_block(rho, h_dim, fn);
+  if (!h) return;
+  const PetscInt N = lookup(rho)->total_levels;
+  for (int i = 0; i < h_dim; i++)
+    for (int j = 0; j < h_dim; j++)
+      if (cabs(DM_AT(h, N, i, j)) > 1e-10)
+        fprintf(fp, "%d %d %e %e\n", i, j, creal(DM_AT(h, N, i, j)), cimag(DM_AT(h, N, i, j)));
+  free(h);
+}
+
+void print_dm_sparse(Vec rho, int h_dim) { dm_sparse(stdout, rho, h_dim, "print_dm_sparse"); }
+
+void print_dm_sparse_to_file(Vec rho, int h_dim, char filename[]) {
+  FILE *fp = fopen(filename, "w");
+  if (!fp) QUAC_FAIL(1, "print_dm_sparse_to_file: cannot open %s", filename);
+  dm_sparse(fp, rho, h_dim, "print_dm_sparse_to_file");
+  fclose(fp);
+}
+
+/* dm_utilities.c:71-92 / 98-113: the stored entries of a matrix, row by row;
+ * the file form drops entries of modulus <= 1e-10 and prints 24 digits */
+static void mat_sparse(FILE *fp, Mat A, int to_file) {
+  int64_t rows = 0, nnz = 0;
+  if (qh_mat_info(A, &rows, &nnz)) QUAC_FAIL(9, "print_mat_sparse: %s", qh_last_error());
+  int64_t *rp = (int64_t *)malloc(sizeof(int64_t) * (size_t)(rows + 1));
+  int64_t *col = (int64_t *)malloc(sizeof(int64_t) * (size_t)(nnz > 0 ? nnz : 1));
+  double *val = (double *)malloc(sizeof(double) * 2 * (size_t)(nnz > 0 ? nnz : 1));
+  if (!rp || !col || !val || qh_mat_export(A, rp, col, val)) {
+    free(rp);
+    free(col);
+    free(val);
+    QUAC_FAIL(9, "print_mat_sparse: %s", rp && col && val ? qh_last_error() : "out of host memory");
+  }
+  for (int64_t r = 0; r < rows; r++)
+    for (int64_t e = rp[r]; e < rp[r + 1]; e++) {
+      const double re = val[2 * e], im = val[2 * e + 1];
+      if (!to_file) fprintf(fp, "%d %d %e %e\n", (int)r, (int)col[e], re, im);
+      else if (hypot(re, im) > 1e-10) fprintf(fp, "%d %d %12.24e %12.24e\n", (int)r, (int)col[e], re, im);
+    }
+  free(rp);
+  free(col);
+  free(val);
+}
+
+void print_mat_sparse(Mat A) { mat_sparse(stdout, A, 0); }
+
+void print_mat_sparse_to_file(Mat A, char filename[]) {
+  FILE *fp = fopen(filename, "w");
+  if (!fp) QUAC_FAIL(1, "print_mat_sparse_to_file: cannot open %s", filename);
+  mat_sparse(fp, A, 1);
+  fclose(fp);
+}
+
+/* dm_utilities.c:120-132: the first h_dim entries of the vector */
+void print_psi(Vec rho, int h_dim) {
+  LOOKUP_OR_FAIL(q, rho);
+  if (h_dim < 0 || h_dim > q->n) QUAC_FAIL(0, "print_psi: h_dim %d exceeds the vector", h_dim);
+  double complex *h = (double complex *)malloc(sizeof(double complex) * (size_t)q->n);
+  if (!h) QUAC_FAIL(9, "out of host memory");
+  if (qh_vec_get(rho, (double *)h)) {
+    free(h);
+    QUAC_FAIL(9, "print_psi: %s", qh_last_error());
+  }
+  for (int i = 0; i < h_dim; i++) printf("%f + %f i\n", creal(h[i]), cimag(h[i]));
+  printf("\n");
+  free(h);
 }
 
 /* ------------------------------------------------------------------- solvers */

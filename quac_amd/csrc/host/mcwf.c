@@ -90,7 +90,17 @@ void quac_mcwf_free(qsystem s) {
 /* copy the initial condition into every trajectory and draw the thresholds
  * (qsystem.c:977-989) */
 static int spawn(qsystem s, qvec x) {
-  if (x->ens_spawned) return 0;
+  if (x->ens_spawned) {
+    /* trajectories written one by one (add_to_wf_ens_loc): only the thresholds */
+    if (x->n_ensemble != s->num_local_trajs)
+      return quac_fail(9, "the ensemble holds %lld trajectories, the system %lld", (long long)x->n_ensemble,
+                       (long long)s->num_local_trajs);
+    if (!s->rand_number) {
+      s->rand_number = (PetscReal *)malloc(sizeof(PetscReal) * (size_t)x->n_ensemble);
+      for (int64_t i = 0; i < x->n_ensemble; i++) s->rand_number[i] = quac_rng_uniform(s);
+    }
+    return 0;
+  }
   const int64_t M = s->num_local_trajs, N = x->n;
   Vec block = NULL;
   if (qh_vec_create(M * N, QH_FP64, &block)) return quac_fail(9, "cannot allocate %lld trajectories: %s",

@@ -47,6 +47,9 @@ void quac_mcwf_free(qsystem s);
 int quac_mcwf_time_step(qsystem s, qvec x, PetscReal t0, PetscReal tmax, PetscReal dt, PetscInt steps_max,
                         int (*monitor)(qsystem, PetscInt, PetscReal, qvec), PetscInt *nsteps);
 
+/* Wootters' concurrence of a two-qubit density matrix (4 x 4, column-major) */
+int quac_concurrence_2q(const double _Complex *rho, double *conc);
+
 /* options (quac_unset_option: forget a name set by quac_set_option) */
 void quac_unset_option(const char *name);
 int quac_opt_int(const char *name, int dflt);

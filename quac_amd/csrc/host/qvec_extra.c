@@ -750,3 +750,110 @@ void apply_projective_measurement_qubit(qvec q, PetscScalar *meas_val, PetscInt 
   va_end(ap);
   apply_projective_measurement_qubit_list(q, meas_val, num_ops, ops);
 }
+
+/* ------------------------------------------------- two-qubit concurrence */
+/* Wootters' concurrence of a two-qubit density matrix (dm_utilities.c:1487-1602):
+ * l1 - l2 - l3 - l4, l_i the square roots, in decreasing order, of the
+ * eigenvalues of rho rho~ with rho~ = (sy (x) sy) rho* (sy (x) sy); eigenvalues
+ * below 1e-14 count as 0 and, as in dm_utilities.c:1568-1582, the result is not
+ * clipped at 0 (Wootters' C is max(0, that)).  The eigenvalues of rho rho~ are
+ * those of sqrt(rho) rho~ sqrt(rho), a Hermitian matrix handled on the real
+ * embedding as in the fidelity.  rho: 4 x 4 column-major. */
+int quac_concurrence_2q(const double complex *rho, double *conc) {
+  /* sy (x) sy = [[0,0,0,-1],[0,0,1,0],[0,1,0,0],[-1,0,0,0]] */
+  static const int perm[4] = {3, 2, 1, 0};
+  static const double sgn[4] = {-1, 1, 1, -1};
+  double complex tilde[16];
+  for (int c = 0; c < 4; c++)
+    for (int r = 0; r < 4; r++) tilde[4 * c + r] = sgn[r] * sgn[c] * conj(rho[4 * perm[c] + perm[r]]);
+  const int M = 8;
+  double R[64], V[64], S[64], T[64], w[8];
+  embed(4, rho, R);
+  sym_eig_jacobi(M, R, w, V);
+  for (int i = 0; i < M; i++)
+    for (int k = 0; k < M; k++) T[i * M + k] = V[i * M + k] * sqrt(w[k] > 0 ? w[k] : 0.0);
+  for (int i = 0; i < M; i++)
+    for (int j = 0; j < M; j++) {
+      double acc = 0.0;
+      for (int k = 0; k < M; k++) acc += T[i * M + k] * V[j * M + k];
+      R[i * M + j] = acc;
+    }
+  embed(4, tilde, S);
+  matmul(M, R, S, T);
+  matmul(M, T, R, S);
+  for (int i = 0; i < M; i++)
+    for (int j = i + 1; j < M; j++) S[i * M + j] = S[j * M + i] = 0.5 * (S[i * M + j] + S[j * M + i]);
+  sym_eig_jacobi(M, S, w, NULL);
+  /* each eigenvalue twice: sort descending and take every other */
+  for (int i = 0; i < M; i++)
+    for (int j = i + 1; j < M; j++)
+      if (w[j] > w[i]) {
+        const double t = w[i];
+        w[i] = w[j];
+        w[j] = t;
+      }
+  double l[4];
+  for (int i = 0; i < 4; i++) l[i] = w[2 * i] < 1e-14 ? 0.0 : sqrt(w[2 * i]);
+  *conc = l[0] - l[1] - l[2] - l[3];
+  return 0;
+}
+
+/* --------------------------------------------- matrices and ensemble members */
+/* qvec_utilities.c:169-195: state <- A state.  For a spawned ensemble every
+ * trajectory is multiplied by A.  (The reference's spawned branch multiplies
+ * state->data, the initial condition, into every trajectory; here each
+ * trajectory's own amplitudes are used, which is what the unspawned and
+ * non-ensemble branches do.) */
+void qvec_mat_mult(Mat circ_mat, qvec state) {
+  int64_t rows = 0, nnz = 0;
+  if (qh_mat_info(circ_mat, &rows, &nnz)) QUAC_FAIL(9, "qvec_mat_mult: %s", qh_last_error());
+  if (rows != state->n) QUAC_FAIL(9, "qvec_mat_mult: the matrix has %lld rows, the state %lld entries", (long long)rows,
+                                  (long long)state->n);
+  const int spawned = state->my_type == WF_ENSEMBLE && state->ens_spawned;
+  const PetscInt members = spawned ? state->n_ensemble : 1;
+  Vec tmp = NULL;
+  QUAC_HIPCHK(qh_vec_create(state->n, QH_FP64, &tmp));
+  int rc = 0;
+  for (PetscInt i = 0; i < members && !rc; i++) {
+    Vec v = state->data;
+    if (spawned && (rc = qh_vec_view(state->ens_datas[0], i * state->n, state->n, &v))) break;
+    rc = qh_spmv(circ_mat, v, tmp);
+    if (!rc) rc = qh_vec_copy(tmp, v);
+    if (spawned) qh_vec_destroy(v);
+  }
+  qh_vec_destroy(tmp);
+  if (rc) QUAC_FAIL(9, "qvec_mat_mult: %s", qh_last_error());
+}
+
+/* trajectories of a WF_ENSEMBLE held in one zeroed device block, marked spawned */
+static int ensure_members(qvec state) {
+  if (state->ens_spawned) return 0;
+  if (state->n_ensemble < 1) return quac_fail(9, "the ensemble has no trajectories"), -1;
+  Vec block = NULL;
+  if (qh_vec_create(state->n_ensemble * state->n, QH_FP64, &block))
+    return quac_fail(9, "cannot allocate the trajectories: %s", qh_last_error()), -1;
+  state->ens_datas = (Vec *)malloc(sizeof(Vec));
+  state->ens_datas[0] = block;
+  state->ens_spawned = PETSC_TRUE;
+  return 0;
+}
+
+/* qvec_utilities.c:899-911: add alpha to amplitude loc of trajectory i_ens.  The
+ * ensemble counts as spawned afterwards: its trajectories start at zero, not
+ * at the initial condition, and time stepping does not overwrite them. */
+void add_to_wf_ens_loc(qvec state, PetscInt i_ens, PetscScalar alpha, PetscInt loc) {
+  if (state->my_type != WF_ENSEMBLE) QUAC_FAIL(9, "add_to_wf_ens_loc: a WF_ENSEMBLE qvec is required");
+  if (loc >= state->n || loc < 0) QUAC_FAIL(0, "ERROR! Adding to a qvec location that is out of range!");
+  if (i_ens < 0 || i_ens >= state->n_ensemble) QUAC_FAIL(9, "add_to_wf_ens_loc: trajectory out of range");
+  if (ensure_members(state)) return;
+  QUAC_HIPCHK(qh_vec_add_entry(state->ens_datas[0], i_ens * state->n + loc, creal(alpha), cimag(alpha)));
+}
+
+/* qvec_utilities.c:456-466: the amplitudes of trajectory state->ens_i */
+void print_wf_ens_i_qvec(qvec state) {
+  for (PetscInt i = 0; i < state->n; i++) {
+    PetscScalar v = 0.0;
+    get_wf_element_ens_i_qvec_local(state, state->ens_i, i, &v);
+    printf("%e + %ei\n", creal(v), cimag(v));
+  }
+}

@@ -61,7 +61,7 @@ def test_legacy_globals_are_exported():
 
 def test_legacy_programs_compile_against_the_headers(tmp_path):
     """the legacy C test programs build against include/ and link with libquac.so"""
-    for prog in ("legacy_jc", "legacy_vec", "legacy_steady", "legacy_circuit"):
+    for prog in ("legacy_jc", "legacy_vec", "legacy_steady", "legacy_circuit", "legacy_dm"):
         subprocess.check_call(["gcc", "-O2", "-std=c11", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"),
                                os.path.join(ROOT, "tests", "c", prog + ".c"), "-o", str(tmp_path / prog),
                                "-L", os.path.dirname(LIB), "-lquac", "-Wl,-rpath," + os.path.dirname(LIB), "-lm"])

@@ -51,8 +51,9 @@ def _build(tmp_path, prog):
     return exe
 
 
-def _run(exe, *opts):
-    out = subprocess.run([str(exe), *opts], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=120)
+def _run(exe, *opts, cwd=None):
+    out = subprocess.run([str(exe), *opts], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=120,
+                         cwd=cwd)
     assert out.returncode == 0, out.stdout
     return out.stdout
 
@@ -120,3 +121,41 @@ def test_legacy_api_circuit_flip_then_decay(tmp_path):
     n = float([l for l in text.splitlines() if l.startswith("n ")][0].split()[1])
     g = int([l for l in text.splitlines() if l.startswith("gates")][0].split()[1])
     assert g == 1 and abs(n - math.exp(-0.7)) < 1e-8, (n, math.exp(-0.7))
+
+
+def test_legacy_dm_utilities_closed_forms(tmp_path):
+    """legacy partial traces, concurrence and fidelity on two qubits:
+    - rho_a (x) rho_b: partial_trace_over(b) = rho_a and partial_trace_keep(b) =
+      rho_b, to 1e-14 (a swapped keep/over or subsystem order fails this, the
+      factors differ);
+    - Werner p |Phi+><Phi+| + (1 - p) I / 4: concurrence (3p - 1) / 2, left
+      unclipped as in dm_utilities.c:1568-1582 (so -0.2 at p = 0.2), and
+      fidelity to |Phi+> sqrt(p + (1 - p) / 4) (Tr sqrt(sqrt(rho) sigma
+      sqrt(rho)), not squared: dm_utilities.c:1690-1699);
+    - cos t |01> + e^{i f} sin t |10>: concurrence sin 2t;
+    - the sparse printers' entries, on screen and in their files."""
+    text = _run(_build(tmp_path, "legacy_dm"), cwd=str(tmp_path))
+    body = text.split("sparse_begin\n")[1].split("sparse_end")[0]
+    text = text.replace(body, "")
+    lines = {l.split()[0] + (l.split()[1] if l.startswith("werner") else ""): l.split()
+             for l in text.splitlines() if l.strip()}
+    # sparse printers (dm_utilities.c:32-113): |Phi+><Phi+| has 1/2 at (0,0), (0,3),
+    # (3,0), (3,3); a a^dag (x) I = |0><0| (x) I has 1 at (0,0), (1,1)
+    want = [(i, j) for i in (0, 3) for j in (0, 3)]
+    for src in (body, (tmp_path / "dm_sparse.txt").read_text()):
+        rows = [l.split() for l in src.splitlines() if l.strip()]
+        assert [(int(r[0]), int(r[1])) for r in rows] == want, src
+        assert all(abs(float(r[2]) - 0.5) < 1e-6 and float(r[3]) == 0.0 for r in rows), src
+    rows = [l.split() for l in (tmp_path / "mat_sparse.txt").read_text().splitlines() if l.strip()]
+    assert [(int(r[0]), int(r[1]), float(r[2])) for r in rows] == [(0, 0, 1.0), (1, 1, 1.0)], rows
+    ra = np.array([[0.7, 0.2 + 0.1j], [0.2 - 0.1j, 0.3]])
+    rb = np.array([[0.1, -0.05j], [0.05j, 0.9]])
+    for tag, want in (("over_b", ra), ("keep_b", rb)):
+        v = [float(x) for x in lines[tag][1:]]
+        got = np.array([complex(v[0], v[1]), complex(v[2], v[3]), complex(v[4], v[5]), complex(v[6], v[7])])
+        assert np.abs(got - want.flatten(order="F")).max() < 1e-14, (tag, got, want)
+    for p in (0.8, 0.2):
+        f = lines["werner%.2f" % p]
+        assert abs(float(f[2]) - (3 * p - 1) / 2) < 1e-12, (p, f)
+        assert abs(float(f[3]) - math.sqrt(p + (1 - p) / 4)) < 1e-12, (p, f)
+    assert abs(float(lines["pure"][1]) - math.sin(0.6)) < 1e-7, lines["pure"]

@@ -2,7 +2,8 @@
 libquac.so, no device needed): linear XEB, unbiased linear XEB, log XEB and the
 heavy-output score (qvec_utilities.c:1720-1991), against their definitions
 computed with numpy, plus the values that fix them: an ideal sampler of a
-Porter-Thomas distribution scores linear XEB ~ 1, a uniform sampler 0."""
+Porter-Thomas distribution scores linear XEB ~ 1, a uniform sampler 0.  Also
+the two-qubit concurrence routine against Wootters' formula in numpy."""
 import ctypes
 import math
 
@@ -96,3 +97,36 @@ def test_readout_error_mixing_vs_numpy():
         t = np.moveaxis(p.reshape(2, 2, 2, 2), 3 - q, 0)       # qubit q = bit q of the index
         want = np.moveaxis(np.tensordot(m, t, axes=1), 0, 3 - q).reshape(-1)
         assert np.abs(got - want).max() < 1e-15 and abs(got.sum() - 1) < 1e-14, q
+
+
+def _wootters(rho):
+    """l1 - l2 - l3 - l4 from the eigenvalues of rho (sy sy) rho* (sy sy), each
+    computed by numpy's general eigensolver (no Hermitian structure used)"""
+    sy = np.array([[0, -1j], [1j, 0]])
+    yy = np.kron(sy, sy)
+    ev = np.linalg.eigvals(rho @ yy @ rho.conj() @ yy).real
+    ev = np.where(ev < 1e-14, 0.0, ev)
+    l = np.sort(np.sqrt(ev))[::-1]
+    return l[0] - l[1:].sum()
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_concurrence_matches_wootters_on_random_states(seed):
+    """quac_concurrence_2q (host routine behind get_bipartite_concurrence,
+    dm_utilities.c:1487-1602) against Wootters' formula evaluated with numpy on
+    random two-qubit density matrices: ranks 1 to 4, so entangled (positive) and
+    separable (negative, unclipped like the reference) cases both occur; and the
+    subsystem-swap invariance C(SWAP rho SWAP) = C(rho)"""
+    f = qa.lib().quac_concurrence_2q
+    f.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+    rng = np.random.default_rng(seed)
+    rank = 1 + seed % 4
+    g = rng.normal(size=(4, rank)) + 1j * rng.normal(size=(4, rank))
+    rho = g @ g.conj().T
+    rho /= np.trace(rho).real
+    swap = np.eye(4)[[0, 2, 1, 3]]
+    for m in (rho, swap @ rho @ swap):
+        buf = np.ascontiguousarray(m.flatten(order="F"))
+        out = ctypes.c_double(-9.0)
+        f(buf.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ctypes.byref(out))
+        assert abs(out.value - _wootters(rho)) < 1e-7, (seed, out.value, _wootters(rho))
