@@ -185,6 +185,9 @@ def _setup(L):
     sig("get_fidelity_qvec", None, vp, vp, P(dbl), P(dbl))
     sig("get_superfidelity_qvec", None, vp, vp, P(dbl))
     sig("get_hilbert_schmidt_dist_qvec", None, vp, vp, P(dbl))
+    sig("qvec_mat_mult", None, vp, vp)
+    sig("add_to_wf_ens_loc", None, vp, i64, Cx, i64)
+    sig("get_wf_element_ens_i_qvec_local", None, vp, i64, i64, P(Cx))
     sig("qh_ens_rk4", i32, vp, vp, i64, dbl, i64)
     sig("qh_ens_norms", i32, vp, i64, P(dbl))
     sig("qh_mcwf_fits", i32, vp, i32, P(i32))

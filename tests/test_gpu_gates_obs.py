@@ -219,3 +219,61 @@ def test_circuits_act_on_every_trajectory_of_a_spawned_ensemble():
     assert abs(x.expectation(q0.n).real - 1.0) < 1e-12 and abs(x.expectation(q1.n).real - 1.0) < 1e-12
     circ.destroy()
     qs.destroy()
+
+
+def _ens_amp(x, i, loc):
+    v = qa.Cx()
+    qa.lib().get_wf_element_ens_i_qvec_local(x.h, i, loc, ctypes.byref(v))
+    qa._check()
+    return complex(v.re, v.im)
+
+
+def test_qvec_mat_mult_and_per_trajectory_writes():
+    """qvec_mat_mult (qvec_utilities.c:169-195) with A = 0.5 a0 a1^dag on two
+    qubits, subsystem 0 the most significant digit: A|10> = 0.5|01>, so y[1] =
+    0.5 x[2] and every other entry 0 (a transposed A would move entry 1 to 2).
+    On a wavefunction, then on an ensemble whose trajectories were written one by
+    one with add_to_wf_ens_loc (qvec_utilities.c:899-911): trajectory i holds
+    (i + 1) at 2 and 0.3 at 0, the others start at zero, and after A each holds
+    0.5 (i + 1) at 1 only.  A time step of zero length afterwards keeps them (the
+    ensemble is not overwritten with the initial condition)."""
+    qs = qa.System()
+    q0, q1 = qs.create_op(2), qs.create_op(2)
+    qs.add_ham_term(0.0, q0.n)
+    qs.construct_matrix()
+    A = qs.op_matrix_wf(0.5, q0, q1.dag)
+    x = qs.create_qvec()
+    rng = np.random.default_rng(5)
+    v = rng.normal(size=4) + 1j * rng.normal(size=4)
+    x.set(v)
+    qa.lib().qvec_mat_mult(A.h, x.h)
+    qa._check()
+    want = np.zeros(4, complex)
+    want[1] = 0.5 * v[2]
+    assert np.abs(x.get() - want).max() < 1e-15, x.get()
+    qs.destroy()
+
+    qa.set_option("ts_rk_type", "4")
+    qa.set_option("ts_adapt_type", "none")
+    qs = qa.System()
+    q0, q1 = qs.create_op(2), qs.create_op(2)
+    qs.add_ham_term(0.0, q0.n)
+    qs.add_lin_term(0.2, q0)
+    qs.use_mcwf_solver(3, 3)
+    qs.construct_matrix()
+    A = qs.op_matrix_wf(0.5, q0, q1.dag)
+    x = qs.create_qvec()
+    for i in range(3):
+        qa.lib().add_to_wf_ens_loc(x.h, i, qa.Cx.of(i + 1.0), 2)
+        qa.lib().add_to_wf_ens_loc(x.h, i, qa.Cx.of(0.3), 0)
+    qa._check()
+    for i in range(3):
+        got = [_ens_amp(x, i, k) for k in range(4)]
+        assert np.abs(np.array(got) - [0.3, 0, i + 1.0, 0]).max() < 1e-15, (i, got)
+    qa.lib().qvec_mat_mult(A.h, x.h)
+    qa._check()
+    qs.time_step(x, 0.0, 0.0, 0.01, 10)
+    for i in range(3):
+        got = [_ens_amp(x, i, k) for k in range(4)]
+        assert np.abs(np.array(got) - [0, 0.5 * (i + 1), 0, 0]).max() < 1e-15, (i, got)
+    qs.destroy()
