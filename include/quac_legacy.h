@@ -68,6 +68,8 @@ void print_dm_sparse_to_file(Vec, int, char[]);                        /* :51-66
 void print_mat_sparse_to_file(Mat, char[]);                            /* :71-92    */
 void print_mat_sparse(Mat);                                            /* :98-113   */
 void get_dm_element_local(Vec, PetscInt, PetscInt, PetscScalar *);     /* :999-1016 */
+void measure_dm(Vec, operator);                                        /* :270-308  */
+void partial_trace_over_one(Vec, Vec, PetscInt, PetscInt, PetscInt, PetscInt); /* :1078-1117 */
 
 /* solver.c */
 void steady_state(Vec);                                                /* :33-221   */

@@ -456,6 +456,38 @@ void partial_trace_keep(Vec full_dm, Vec ptraced_dm, int number_of_ops, ...) {
   ptrace_into(full_dm, ptraced_dm, traced);
 }
 
+/* dm_utilities.c:1078-1117: trace over the middle factor of a space
+ * nbef x nop x naf (cur_levels = nbef nop naf), ADDING into ptraced_dm
+ * (nbef naf)^2.  Entry (r, c) of either matrix sits at dim r + c here, as in
+ * the reference; the partial trace commutes with the transpose, so this is the
+ * partial trace in either reading. */
+void partial_trace_over_one(Vec full_dm, Vec ptraced_dm, PetscInt nbef, PetscInt nop, PetscInt naf,
+                            PetscInt cur_levels) {
+  LOOKUP_OR_FAIL(q, full_dm);
+  LOOKUP_OR_FAIL(p, ptraced_dm);
+  const PetscInt ns = nbef * naf;
+  if (nbef < 1 || nop < 1 || naf < 1 || cur_levels != nbef * nop * naf || q->n != cur_levels * cur_levels ||
+      p->n != ns * ns)
+    QUAC_FAIL(0, "partial_trace_over_one: the sizes do not match the density matrices");
+  double complex *f = (double complex *)malloc(sizeof(double complex) * (size_t)q->n);
+  double complex *r = (double complex *)malloc(sizeof(double complex) * (size_t)p->n);
+  int rc = !f || !r;
+  if (!rc) rc = qh_vec_get(full_dm, (double *)f) || qh_vec_get(ptraced_dm, (double *)r);
+  if (!rc) {
+    for (PetscInt ib = 0; ib < nbef; ib++)
+      for (PetscInt jb = 0; jb < nbef; jb++)
+        for (PetscInt ia = 0; ia < naf; ia++)
+          for (PetscInt ja = 0; ja < naf; ja++)
+            for (PetscInt k = 0; k < nop; k++)
+              r[ns * (naf * ib + ia) + naf * jb + ja] +=
+                  f[cur_levels * (naf * nop * ib + naf * k + ia) + naf * nop * jb + naf * k + ja];
+    rc = qh_vec_set(ptraced_dm, (const double *)r);
+  }
+  free(f);
+  free(r);
+  if (rc) QUAC_FAIL(9, "partial_trace_over_one: %s", f && r ? qh_last_error() : "out of host memory");
+}
+
 /* dm_utilities.c:1618-1722: Tr sqrt(sqrt(rho) sigma sqrt(rho)), positive
  * eigenvalues only -- not squared, unlike get_fidelity_qvec, whose value is the
  * square of this one */
@@ -476,6 +508,52 @@ void get_bipartite_concurrence(Vec dm, double *concurrence) {
   double complex rho[16];
   QUAC_HIPCHK(qh_vec_get(dm, (double *)rho));
   quac_concurrence_2q(rho, concurrence);
+}
+
+/* dm_utilities.c:270-308: rho <- A rho A^dag / Tr(A rho A^dag) for the operator A
+ * (its full total_levels-dimensional matrix); rho is left unchanged when that
+ * trace is 0.  Host arithmetic over A's CSR: O(N^2 nnz/row). */
+void measure_dm(Vec dm, operator op) {
+  LOOKUP_OR_FAIL(q, dm);
+  if (q->my_type != DENSITY_MATRIX || q->total_levels != total_levels)
+    QUAC_FAIL(0, "measure_dm: a full density matrix of the system is required");
+  const PetscInt N = total_levels;
+  Mat m = NULL;
+  combine_ops_to_mat(&m, total_levels, 1, op);
+  if (!m) return;
+  int64_t rows = 0, nnz = 0;
+  qh_mat_info(m, &rows, &nnz);
+  int64_t *rp = (int64_t *)malloc(sizeof(int64_t) * (size_t)(rows + 1));
+  int64_t *col = (int64_t *)malloc(sizeof(int64_t) * (size_t)(nnz > 0 ? nnz : 1));
+  double complex *val = (double complex *)malloc(sizeof(double complex) * (size_t)(nnz > 0 ? nnz : 1));
+  double complex *rho = (double complex *)malloc(sizeof(double complex) * (size_t)(N * N));
+  double complex *T = (double complex *)calloc((size_t)(N * N), sizeof(double complex));
+  double complex *R = (double complex *)calloc((size_t)(N * N), sizeof(double complex));
+  int rc = !rp || !col || !val || !rho || !T || !R || rows != N;
+  if (!rc) rc = qh_mat_export(m, rp, col, (double *)val) || qh_vec_get(dm, (double *)rho);
+  qh_mat_destroy(m);
+  if (!rc) {
+    /* T = A rho, then R = T A^dag: R(i, j) = sum_k T(i, k) conj(A(j, k)); column-major */
+    for (PetscInt i = 0; i < N; i++)
+      for (int64_t e = rp[i]; e < rp[i + 1]; e++)
+        for (PetscInt c = 0; c < N; c++) T[N * c + i] += val[e] * rho[N * c + col[e]];
+    for (PetscInt j = 0; j < N; j++)
+      for (int64_t e = rp[j]; e < rp[j + 1]; e++)
+        for (PetscInt i = 0; i < N; i++) R[N * j + i] += T[N * col[e] + i] * conj(val[e]);
+    double complex tr = 0.0;
+    for (PetscInt i = 0; i < N; i++) tr += R[N * i + i];
+    if (tr != 0.0) {
+      for (PetscInt k = 0; k < N * N; k++) R[k] /= tr;
+      rc = qh_vec_set(dm, (const double *)R);
+    }
+  }
+  free(rp);
+  free(col);
+  free(val);
+  free(rho);
+  free(T);
+  free(R);
+  if (rc) QUAC_FAIL(9, "measure_dm: %s", rows != N ? "operator size mismatch" : qh_last_error());
 }
 
 /* the density matrix as host complex128 after checking h_dim (caller frees) */

@@ -2,11 +2,14 @@
  * Legacy density-matrix utilities on two qubits a (subsystem 0) and b:
  *  - a product state rho_a (x) rho_b written with add_value_to_dm; the partial
  *    traces (partial_trace_over b, partial_trace_keep b; dm_utilities.c:150-260,
- *    468-603) print the 2 x 2 factors;
+ *    468-603) and partial_trace_over_one (dm_utilities.c:1078-1117) print
+ *    the 2 x 2 factors;
  *  - Werner states p |Phi+><Phi+| + (1 - p) I / 4: concurrence
  *    (dm_utilities.c:1487-1602) and fidelity to |Phi+> (dm_utilities.c:1618-1722)
  *    for p = 0.8 and p = 0.2;
  *  - the pure state cos t |01> + e^{i f} sin t |10>: concurrence sin 2t;
+ *  - measure_dm (dm_utilities.c:270-308) with the projector n of a on the
+ *    Werner state p = 0.8;
  *  - the sparse printers (dm_utilities.c:32-113) on |Phi+><Phi+| and on the
  *    matrix a a^dag (x) I, written to dm_sparse.txt / mat_sparse.txt in the
  *    working directory.
@@ -68,6 +71,15 @@ int main(int argc, char **args) {
   partial_trace_keep(rho, red, 1, b);
   show("keep_b", red);
   destroy_dm(red);
+  /* partial_trace_over_one: a is the middle factor of 1 x 2 x 2, b of 2 x 2 x 1 */
+  create_dm(&red, 2);
+  partial_trace_over_one(rho, red, 1, 2, 2, 4);
+  show("one_a", red);
+  destroy_dm(red);
+  create_dm(&red, 2);
+  partial_trace_over_one(rho, red, 2, 2, 1, 4);
+  show("one_b", red);
+  destroy_dm(red);
   create_full_dm(&bell);
   werner(bell, 1.0);
   for (int k = 0; k < 2; k++) {
@@ -90,6 +102,20 @@ int main(int argc, char **args) {
     put(w, pm, 4);
     get_bipartite_concurrence(w, &conc);
     printf("pure %.15e\n", conc);
+    destroy_dm(w);
+  }
+  {
+    /* measure_dm with the projector n of a on a Werner state (p = 0.8) */
+    create_full_dm(&w);
+    werner(w, 0.8);
+    measure_dm(w, a->n);
+    printf("measure");
+    for (int k = 0; k < 16; k++) {
+      PetscScalar v;
+      get_dm_element(w, k % 4, k / 4, &v);
+      printf(" %.15e %.15e", creal(v), cimag(v));
+    }
+    printf("\n");
     destroy_dm(w);
   }
   /* sparse printers: the Bell state (4 entries of 1/2) and sigma_x of a as a

@@ -127,12 +127,14 @@ def test_legacy_dm_utilities_closed_forms(tmp_path):
     """legacy partial traces, concurrence and fidelity on two qubits:
     - rho_a (x) rho_b: partial_trace_over(b) = rho_a and partial_trace_keep(b) =
       rho_b, to 1e-14 (a swapped keep/over or subsystem order fails this, the
-      factors differ);
+      factors differ); partial_trace_over_one over the middle factor of 1x2x2
+      gives rho_b, of 2x2x1 rho_a;
     - Werner p |Phi+><Phi+| + (1 - p) I / 4: concurrence (3p - 1) / 2, left
       unclipped as in dm_utilities.c:1568-1582 (so -0.2 at p = 0.2), and
       fidelity to |Phi+> sqrt(p + (1 - p) / 4) (Tr sqrt(sqrt(rho) sigma
       sqrt(rho)), not squared: dm_utilities.c:1690-1699);
     - cos t |01> + e^{i f} sin t |10>: concurrence sin 2t;
+    - measure_dm with a projector (dm_utilities.c:270-308);
     - the sparse printers' entries, on screen and in their files."""
     text = _run(_build(tmp_path, "legacy_dm"), cwd=str(tmp_path))
     body = text.split("sparse_begin\n")[1].split("sparse_end")[0]
@@ -150,7 +152,7 @@ def test_legacy_dm_utilities_closed_forms(tmp_path):
     assert [(int(r[0]), int(r[1]), float(r[2])) for r in rows] == [(0, 0, 1.0), (1, 1, 1.0)], rows
     ra = np.array([[0.7, 0.2 + 0.1j], [0.2 - 0.1j, 0.3]])
     rb = np.array([[0.1, -0.05j], [0.05j, 0.9]])
-    for tag, want in (("over_b", ra), ("keep_b", rb)):
+    for tag, want in (("over_b", ra), ("keep_b", rb), ("one_a", rb), ("one_b", ra)):
         v = [float(x) for x in lines[tag][1:]]
         got = np.array([complex(v[0], v[1]), complex(v[2], v[3]), complex(v[4], v[5]), complex(v[6], v[7])])
         assert np.abs(got - want.flatten(order="F")).max() < 1e-14, (tag, got, want)
@@ -159,3 +161,11 @@ def test_legacy_dm_utilities_closed_forms(tmp_path):
         assert abs(float(f[2]) - (3 * p - 1) / 2) < 1e-12, (p, f)
         assert abs(float(f[3]) - math.sqrt(p + (1 - p) / 4)) < 1e-12, (p, f)
     assert abs(float(lines["pure"][1]) - math.sin(0.6)) < 1e-7, lines["pure"]
+    # measure_dm with n of a (subsystem 0, the most significant digit): only
+    # |10>, |11> survive, 2 ((1 - p)/4, (1 - p)/4 + p/2) = (0.1, 0.9) at p = 0.8,
+    # the coherence to |00> removed
+    v = [float(x) for x in lines["measure"][1:]]
+    got = np.array([complex(v[2 * k], v[2 * k + 1]) for k in range(16)]).reshape(4, 4, order="F")
+    want = np.zeros((4, 4))
+    want[2, 2], want[3, 3] = 0.1, 0.9
+    assert np.abs(got - want).max() < 1e-14, got
