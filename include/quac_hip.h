@@ -58,6 +58,10 @@ typedef struct qh_mat qh_mat;
 int qh_assemble(int64_t tot, int64_t dim, int nterms, const qh_term *terms, int nops, const qh_op *ops,
                 int nlev, const qh_level *levels, int flags, qh_mat **out);
 int qh_mat_destroy(qh_mat *m);
+/* a device matrix from a host CSR (square; rowptr[0] = 0, 0 < nnz < 2^31;
+ * strictly increasing columns in a row; complex128 values interleaved), checked
+ * on the host before the upload -- PETSc's MatCreateSeqAIJWithArrays */
+int qh_mat_from_csr(int64_t rows, const int64_t *rowptr, const int64_t *col, const double *val, qh_mat **out);
 int qh_mat_info(const qh_mat *m, int64_t *rows, int64_t *nnz);
 /* copy the CSR back (int64 like PetscInt; values complex128 interleaved) */
 int qh_mat_export(const qh_mat *m, int64_t *rowptr, int64_t *col, double *val);

@@ -294,6 +294,57 @@ static int apply_on_members(struct quantum_gate_struct *g, qvec state, int super
   return 0;
 }
 
+/* quantum_circuits.c:257-314: the circuit's matrix G_n ... G_1 on the Hilbert
+ * space (total_levels rows; gate times are ignored, as there).  Column j is the
+ * circuit applied to the basis state e_j, gate by gate, with the device gate
+ * path and qubit convention of apply_circuit_to_qvec on a wavefunction (the
+ * reference builds each factor with the same row generators, argument -1).  The
+ * matrix is dense on the host while it is built: total_levels <= 2048. */
+void combine_circuit_to_mat_sys(qsystem sys, Mat *matrix_out, circuit circ) {
+  *matrix_out = NULL;
+  if (!sys || sys->num_subsystems == 0) QUAC_FAIL(0, "combine_circuit_to_mat_sys: the system has no subsystems");
+  if (circ.num_gates < 1) QUAC_FAIL(0, "combine_circuit_to_mat_sys: the circuit has no gates");
+  const PetscInt N = sys->total_levels;
+  if (N > 2048) QUAC_FAIL(9, "combine_circuit_to_mat_sys: total_levels %lld is above 2048", (long long)N);
+  PetscInt *dims = (PetscInt *)malloc(sizeof(PetscInt) * (size_t)sys->num_subsystems);
+  double complex *U = (double complex *)malloc(sizeof(double complex) * (size_t)(N * N));
+  int64_t *rp = (int64_t *)malloc(sizeof(int64_t) * (size_t)(N + 1));
+  int64_t *col = (int64_t *)malloc(sizeof(int64_t) * (size_t)(N * N));
+  double complex *val = (double complex *)malloc(sizeof(double complex) * (size_t)(N * N));
+  qvec q = NULL;
+  int rc = !dims || !U || !rp || !col || !val;
+  if (!rc) {
+    for (PetscInt i = 0; i < sys->num_subsystems; i++) dims[i] = sys->subsystem_list[i]->my_levels;
+    create_arb_qvec_dims(&q, sys->num_subsystems, dims, WAVEFUNCTION);
+    rc = !q;
+  }
+  for (PetscInt j = 0; j < N && !rc; j++) {
+    rc = qh_vec_zero(q->data) || qh_vec_add_entry(q->data, j, 1.0, 0.0);
+    for (PetscInt g = 0; g < circ.num_gates && !rc; g++) rc = qc_apply_gate(&circ.gate_list[g], q, 1, sys);
+    if (!rc) rc = qh_vec_get(q->data, (double *)(U + (size_t)N * j));
+  }
+  if (!rc) {
+    int64_t nnz = 0;
+    for (PetscInt r = 0; r < N; r++) {
+      rp[r] = nnz;
+      for (PetscInt c = 0; c < N; c++)
+        if (U[(size_t)N * c + r] != 0.0) {
+          col[nnz] = c;
+          val[nnz++] = U[(size_t)N * c + r];
+        }
+    }
+    rp[N] = nnz;
+    rc = qh_mat_from_csr(N, rp, col, (const double *)val, matrix_out);
+  }
+  if (q) destroy_qvec(&q);
+  free(dims);
+  free(U);
+  free(rp);
+  free(col);
+  free(val);
+  if (rc) QUAC_FAIL(9, "combine_circuit_to_mat_sys: %s", qh_last_error());
+}
+
 /* quantum_circuits.c:143-159 */
 void apply_circuit_to_qvec(qsystem qsys, circuit circ, qvec state) {
   for (PetscInt i = 0; i < circ.num_gates; i++)

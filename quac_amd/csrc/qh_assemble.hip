@@ -20,6 +20,7 @@
 // per-row scratch segment) -> scan of exact lengths -> compact (+ -i scale,
 // + complex64 copy).
 #include <hipcub/hipcub.hpp>
+#include <vector>
 
 #include "qh_common.h"
 
@@ -481,6 +482,48 @@ extern "C" int qh_mat_blockdiag(int n, qh_mat *const *mats, qh_mat **out) {
   }
   QH_HIP(hipGetLastError());
   QH_HIP(hipStreamSynchronize(s));
+  *out = m;
+  return QH_OK;
+}
+
+/* A device matrix from a host CSR: square, rows in order, strictly increasing
+ * columns within a row, complex128 values interleaved.  Everything is checked
+ * on the host before the upload, so the SpMV kernels never see a column out of
+ * range. */
+extern "C" int qh_mat_from_csr(int64_t rows, const int64_t *rowptr, const int64_t *col, const double *val,
+                               qh_mat **out) {
+  QH_REQUIRE(rows > 0 && rowptr && col && val && out, QH_EINVAL, "qh_mat_from_csr: bad arguments");
+  const int64_t nnz = rowptr[rows];
+  QH_REQUIRE(rowptr[0] == 0 && nnz > 0 && rows < INT32_MAX && nnz < INT32_MAX, QH_ERANGE,
+             "qh_mat_from_csr: rowptr[0] must be 0 and 0 < nnz < 2^31");
+  std::vector<int32_t> rp32((size_t)rows + 1), c32((size_t)nnz);
+  for (int64_t r = 0; r < rows; r++) {
+    QH_REQUIRE(rowptr[r] <= rowptr[r + 1] && rowptr[r + 1] <= nnz, QH_EINVAL, "qh_mat_from_csr: rowptr not monotone");
+    rp32[r] = (int32_t)rowptr[r];
+    for (int64_t e = rowptr[r]; e < rowptr[r + 1]; e++) {
+      QH_REQUIRE(col[e] >= 0 && col[e] < rows && (e == rowptr[r] || col[e] > col[e - 1]), QH_EINVAL,
+                 "qh_mat_from_csr: row %lld has a column out of range or out of order", (long long)r);
+      c32[e] = (int32_t)col[e];
+    }
+  }
+  rp32[rows] = (int32_t)nnz;
+  qh_mat *m = new qh_mat();
+  m->rows = rows;
+  m->nnz = nnz;
+  int rc = qh_alloc_t(&m->rowptr, (size_t)rows + 1);
+  if (!rc) rc = qh_alloc_t(&m->col, (size_t)nnz);
+  if (!rc) rc = qh_alloc_t(&m->val, (size_t)nnz);
+  if (rc) {
+    qh_mat_destroy(m);
+    return rc;
+  }
+  hipError_t e = hipMemcpy(m->rowptr, rp32.data(), sizeof(int32_t) * ((size_t)rows + 1), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(m->col, c32.data(), sizeof(int32_t) * (size_t)nnz, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(m->val, val, 16 * (size_t)nnz, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    qh_mat_destroy(m);
+    QH_HIP(e);
+  }
   *out = m;
   return QH_OK;
 }

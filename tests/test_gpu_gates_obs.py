@@ -277,3 +277,41 @@ def test_qvec_mat_mult_and_per_trajectory_writes():
         got = [_ens_amp(x, i, k) for k in range(4)]
         assert np.abs(np.array(got) - [0, 0.5 * (i + 1), 0, 0]).max() < 1e-15, (i, got)
     qs.destroy()
+
+
+def test_combine_circuit_to_mat_sys_matches_the_oracle():
+    """combine_circuit_to_mat_sys (quantum_circuits.c:257-314): the 8 x 8 matrix of
+    H(0), CNOT(0 -> 1), RZ(0.4) on 2, RX(2.0) on 1, CNOT(2 -> 0) on three qubits
+    equals, column by column, the oracle's gates (qubit q = subsystem q) applied in
+    order to the basis states -- H and CNOT do not commute, so a reversed product
+    fails -- and is unitary.  The matrix arrives through qh_mat_from_csr, so its
+    CSR (sorted columns, 0 < nnz) is read back as well."""
+    nq = 3
+    r = {"subs": [("op", 2)] * nq, "terms": [("ham", 0.0, [(0, "n")])], "mcwf": False}
+    qs, _, _ = build_system(r, construct=False)
+    circ = qa.Circuit(10)
+    gates = [("H", 1, (0,), 0), ("CNOT", -1, (0, 1), 0), ("RZ", 8, (2,), 0.4), ("RX", 6, (1,), 2.0),
+             ("CNOT", -1, (2, 0), 0)]
+    for k, g in enumerate(gates):
+        circ.add_gate(1.0 + k, g[0], list(g[2]), theta=g[3])
+    h = ctypes.c_void_p()
+    qa.lib().combine_circuit_to_mat_sys(qs.h, ctypes.byref(h), circ.c)
+    qa._check()
+    m = qa.Matrix(h, owned=True)
+    rp, col, val = m.export()
+    U = np.zeros((8, 8), complex)
+    for row in range(8):
+        assert np.all(np.diff(col[rp[row]:rp[row + 1]]) > 0)
+        U[row, col[rp[row]:rp[row + 1]]] = val[rp[row]:rp[row + 1]]
+    want = np.zeros((8, 8), complex)
+    for j in range(8):
+        e = np.zeros(8, complex)
+        e[j] = 1.0
+        for g in gates:
+            e = oracle.gate_superop([2] * nq, [1, 2, 4], False, g[1], g[2], e, theta=g[3])
+        want[:, j] = e
+    assert np.abs(U - want).max() < 1e-14, np.abs(U - want).max()
+    assert np.abs(U @ U.conj().T - np.eye(8)).max() < 1e-14
+    del m
+    circ.destroy()
+    qs.destroy()
