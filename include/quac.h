@@ -218,6 +218,7 @@ void get_dm_element_qvec_local(qvec, PetscInt, PetscInt, PetscScalar *);        
 void get_wf_element_qvec_local(qvec, PetscInt, PetscScalar *);                  /* :416-431 */
 void get_wf_element_ens_i_qvec_local(qvec, PetscInt, PetscInt, PetscScalar *);  /* :434-449 */
 void combine_circuit_to_mat_sys(qsystem, Mat *, circuit);         /* quantum_circuits.c:257-314 */
+void apply_op_to_qvec_mat(qvec, PetscScalar **, operator);         /* qvec_utilities.c:1553-1589 */
 void qvec_mat_mult(Mat, qvec);                                      /* qvec_utilities.c:169-195 */
 void add_to_wf_ens_loc(qvec, PetscInt, PetscScalar, PetscInt);      /* qvec_utilities.c:899-911 */
 void print_wf_ens_i_qvec(qvec);                                     /* qvec_utilities.c:456-466 */

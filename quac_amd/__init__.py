@@ -186,6 +186,7 @@ def _setup(L):
     sig("get_superfidelity_qvec", None, vp, vp, P(dbl))
     sig("get_hilbert_schmidt_dist_qvec", None, vp, vp, P(dbl))
     sig("qvec_mat_mult", None, vp, vp)
+    sig("apply_op_to_qvec_mat", None, vp, P(P(Cx)), vp)
     sig("combine_circuit_to_mat_sys", None, vp, P(vp), CircuitStruct)
     sig("add_to_wf_ens_loc", None, vp, i64, Cx, i64)
     sig("get_wf_element_ens_i_qvec_local", None, vp, i64, i64, P(Cx))

@@ -857,3 +857,64 @@ void print_wf_ens_i_qvec(qvec state) {
     printf("%e + %ei\n", creal(v), cimag(v));
   }
 }
+
+/* qvec_utilities.c:1553-1589 (rows from kron.c:209-266): q <- M q for a
+ * wavefunction (every trajectory of a spawned ensemble), rho <- M rho M^dag for
+ * a density matrix (the superoperator conj(M) (x) M on the column-major vector),
+ * with M = I (x) mat (x) I on the operator's subsystem and mat[r][c] its row r,
+ * column c.  The reference keeps at most two entries per row of mat; any mat is
+ * taken here.  The matrix is built on the host and applied by qvec_mat_mult. */
+void apply_op_to_qvec_mat(qvec q, PetscScalar **mat, operator op) {
+  const PetscInt N = q->total_levels, L = op->my_levels;
+  const PetscInt na = N / (L * op->n_before);
+  if (L < 1 || na < 1 || L * op->n_before * na != N) QUAC_FAIL(9, "apply_op_to_qvec_mat: the operator does not fit the state");
+  const int dm = q->my_type == DENSITY_MATRIX;
+  const PetscInt D = dm ? N * N : N, per = dm ? L * L : L;
+  int64_t *rp = (int64_t *)malloc(sizeof(int64_t) * (size_t)(D + 1));
+  int64_t *col = (int64_t *)malloc(sizeof(int64_t) * (size_t)(D * per));
+  double complex *val = (double complex *)malloc(sizeof(double complex) * (size_t)(D * per));
+  int64_t *mc = (int64_t *)malloc(sizeof(int64_t) * (size_t)(N * L));   /* rows of M: columns */
+  double complex *mv = (double complex *)malloc(sizeof(double complex) * (size_t)(N * L));
+  PetscInt *mn = (PetscInt *)malloc(sizeof(PetscInt) * (size_t)N);
+  if (!rp || !col || !val || !mc || !mv || !mn) {
+    free(rp), free(col), free(val), free(mc), free(mv), free(mn);
+    QUAC_FAIL(9, "out of host memory");
+  }
+  for (PetscInt i = 0; i < N; i++) {
+    const PetscInt is = (i / na) % L;
+    mn[i] = 0;
+    for (PetscInt j = 0; j < L; j++)
+      if (mat[is][j] != 0.0) {
+        mc[L * i + mn[i]] = i + (j - is) * na;
+        mv[L * i + mn[i]++] = mat[is][j];
+      }
+  }
+  int64_t nnz = 0;
+  for (PetscInt i = 0; i < D; i++) {
+    rp[i] = nnz;
+    if (!dm) {
+      for (PetscInt k = 0; k < mn[i]; k++) col[nnz] = mc[L * i + k], val[nnz++] = mv[L * i + k];
+    } else {
+      const PetscInt i1 = i / N, i2 = i % N;
+      for (PetscInt k1 = 0; k1 < mn[i1]; k1++)
+        for (PetscInt k2 = 0; k2 < mn[i2]; k2++) {
+          col[nnz] = N * mc[L * i1 + k1] + mc[L * i2 + k2];
+          val[nnz++] = conj(mv[L * i1 + k1]) * mv[L * i2 + k2];
+        }
+    }
+  }
+  rp[D] = nnz;
+  Mat A = NULL;
+  int rc = 0;
+  if (nnz == 0) { /* mat = 0: the state becomes 0 */
+    rc = q->my_type == WF_ENSEMBLE && q->ens_spawned ? qh_vec_zero(q->ens_datas[0]) : qh_vec_zero(q->data);
+  } else {
+    rc = qh_mat_from_csr(D, rp, col, (const double *)val, &A);
+  }
+  free(rp), free(col), free(val), free(mc), free(mv), free(mn);
+  if (rc) QUAC_FAIL(9, "apply_op_to_qvec_mat: %s", qh_last_error());
+  if (A) {
+    qvec_mat_mult(A, q);
+    qh_mat_destroy(A);
+  }
+}

@@ -315,3 +315,32 @@ def test_combine_circuit_to_mat_sys_matches_the_oracle():
     del m
     circ.destroy()
     qs.destroy()
+
+
+@pytest.mark.parametrize("kind", ["wf", "dm"])
+def test_apply_op_to_qvec_mat_matches_krons(kind):
+    """apply_op_to_qvec_mat (qvec_utilities.c:1553-1589; rows from kron.c:209-266):
+    a dense complex 3 x 3 mat on the middle subsystem of qubit (x) qutrit (x)
+    qubit, subsystem 0 the most significant digit: psi <- (I2 (x) mat (x) I2) psi,
+    rho <- M rho M^dag, against numpy krons.  A dense mat on the middle subsystem
+    catches a wrong stride, a transposed mat or a missing conjugate."""
+    rng = np.random.default_rng(11)
+    term = ("lin", 0.1, [(0, "lower")]) if kind == "dm" else ("ham", 0.1, [(0, "n")])
+    qs, subs, _ = build_system({"subs": [("op", 2), ("op", 3), ("op", 2)], "terms": [term], "mcwf": False})
+    x = qs.create_qvec()
+    mat = rng.normal(size=(3, 3)) + 1j * rng.normal(size=(3, 3))
+    rows = [(qa.Cx * 3)(*[qa.Cx.of(v) for v in mat[r]]) for r in range(3)]
+    ptrs = (ctypes.POINTER(qa.Cx) * 3)(*[ctypes.cast(row, ctypes.POINTER(qa.Cx)) for row in rows])
+    M = np.kron(np.kron(np.eye(2), mat), np.eye(2))
+    if kind == "wf":
+        v = rng.normal(size=12) + 1j * rng.normal(size=12)
+        x.set(v)
+        want = M @ v
+    else:
+        rho = _rand_rho(12)
+        x.set(dense.vec(rho))
+        want = dense.vec(M @ rho @ M.conj().T)
+    qa.lib().apply_op_to_qvec_mat(x.h, ptrs, ctypes.cast(subs[1][1].ptr, ctypes.c_void_p))
+    qa._check()
+    assert np.abs(x.get() - want).max() < 1e-12 * max(1.0, np.abs(want).max())
+    qs.destroy()
