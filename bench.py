@@ -1,22 +1,36 @@
-"""Benchmark: RK4 steps/s of the fused FP32 bulk path (population error checked
-against the plain FP64 path in the same run) and SpMV % of HBM peak, on the
-10-qubit dissipative spin chain (rho 1024x1024; superoperator 1 048 576 rows,
-24 641 536 nnz), RK4 dt = 0.005.  One JSON line on stdout.
+"""Benchmark of the metric "batched Lindblad RK4 steps/s at 1e-6 pop err vs plain
+FP64; SpMV % HBM peak".  One JSON line on stdout.
 
-    python bench.py [--gpus 1] [--steps 2000] [--warmup 20] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--workload ...]
 
-Timed regions (each bracketed by a device synchronize on both sides):
-  * FP64 drop-in path: K steps, hipGraph-captured fused stage kernels
-  * FP32 bulk path:    K steps, same, with FP64 trace renormalisation every 100 steps
-  * per-launch stage-kernel durations: the same FP32 steps launched eagerly with a
-    HIP event pair around every stage kernel (qh_rk4_timed) -> roofline.achieved
-The CPU baseline is the C oracle's FP64 RK4 on the host cores over a bounded
-sample of steps of the same workload.
+Headline (default workload):
+  * value: the batched parameter sweep of 512 Tavis-Cummings instances of uneven
+    size (seed 2; 100 .. 230 400 superoperator rows), RK4 dt = 0.01, advanced as
+    one FP32 batch (block-diagonal sliced matrix, fused stage kernels, hipGraph
+    replay): instance RK4 steps per second over K timed steps, beside the plain
+    FP64 path run one instance at a time over the same K steps, with the maximum
+    population error of the batch against that FP64 path after the same W + K
+    steps.
+  * roofline: the 10-qubit dissipative spin chain (rho 1024 x 1024; superoperator
+    1 048 576 rows, 24 641 536 nnz), RK4 dt = 0.005, FP32 and FP64 over K steps:
+    the FP32 fused SpMV + RK4 stage kernel's launch duration against the bytes its
+    format has to move (roofline.achieved / frac), the HBM traffic counted by
+    rocprofv3 (roofline.traffic, from the committed counter summary), and the CSR
+    model's bytes under roofline.csr_model.  GPU FP64 and FP32 populations are
+    compared with the CPU oracle after the same W + K steps.
+  * hbm_copy: the achievable copy bandwidth of this GPU (device-to-device copy of
+    1 GiB), beside the 8 TB/s peak.
+  * cpu_baseline: the C oracle's FP64 RK4 (same CSR, row-ordered MatMult, same RK4)
+    on this host's cores, on a bounded sample of the sweep (every 16th instance),
+    median of 3 runs at 1 thread and at all cores of the GPU's CPU share.
+Before any timed region the autotuner runs and the step graph is captured and
+instantiated (qh_rk4_prepare), whatever W is.
 """
 import argparse
 import json
 import os
 import platform
+import statistics
 import sys
 import time
 
@@ -27,15 +41,17 @@ sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E peak (GB/s)
+METRIC = "batched Lindblad RK4 steps/s at 1e-6 pop err vs plain FP64; SpMV % HBM peak"
+CPU_SHARE = 16         # host threads of one GPU's share of the box
 
 
 def spmv_bytes(nnz, rows, S):
-    """algorithmic bytes of one SpMV: nnz*(S+4) + 4*(rows+1) + 2*rows*S"""
+    """CSR-model bytes of one SpMV: nnz*(S+4) + 4*(rows+1) + 2*rows*S"""
     return nnz * (S + 4) + 4 * (rows + 1) + 2 * rows * S
 
 
 def rk4_step_bytes(nnz, rows, S):
-    """algorithmic bytes of one fused RK4 step: 4 matrix streams + 14 vector passes"""
+    """CSR-model bytes of one fused RK4 step: 4 matrix streams + 14 vector passes"""
     return 4 * (nnz * (S + 4) + 4 * (rows + 1)) + 14 * rows * S
 
 
@@ -52,8 +68,25 @@ def kernel_name(variant, real):
     return f"k_rk_stage<{real}, {variant}> (CSR, {variant} lanes per row)"
 
 
+def format_step_bytes(A, rows):
+    """bytes the FP32 sliced layout of A has to move per RK4 step: 4 x (stored
+    values + slot metadata) + 17 vector passes of 8 B per row (4 gathered stage
+    inputs, 3 K and 3 Y writes, 3 x reads for the Y's, K1..K3 and x read and x'
+    written in the last epilogue); and the layout statistics"""
+    lanes = A.lanes()
+    if lanes[1] in (140, 141, 142):  # row-pair layout: 40-byte slot metadata, values per lane pair
+        layout = dict(A.pair_layout_info(), kind="row-pair (128 rows per wavefront)")
+        meta = layout["meta_bytes"]
+        vals = layout["stored_pairs"] * 16 + layout["stored_real_pairs"] * 8
+    else:
+        layout = dict(A.layout_info(), kind="one row per lane (64 rows per wavefront)")
+        meta = layout["slots"] * 32 + 12 * layout["slices"]
+        vals = layout["stored_vals"] * 8 + layout["stored_reals"] * 4
+    return 4 * (vals + meta) + 17 * rows * 8, layout
+
+
 def read_traffic(path):
-    """HBM bytes per stage launch from the committed rocprofv3 --pmc summary
+    """HBM bytes per stage launch from a committed rocprofv3 --pmc summary
     (FETCH_SIZE + WRITE_SIZE, each scaled by its copy-kernel factor); None when absent"""
     try:
         for line in open(path):
@@ -74,26 +107,270 @@ def cpu_model():
     return platform.processor()
 
 
+def host_threads():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return n, max(1, min(n, CPU_SHARE))
+
+
+def roofline_obj(kname, workload, fmt_bytes, launch_ms, ev_ms, traffic, traffic_src, csr_bytes):
+    ach = fmt_bytes / (launch_ms * 1e-3) / 1e9
+    out = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": kname, "workload": workload,
+           "bytes_per_launch": fmt_bytes, "bytes_basis": "bytes the sliced format moves per launch (stored values "
+                                                         "+ slot metadata + 17/4 vector passes of 8 B per row)",
+           "launch_ms": round(launch_ms, 5), "launch_ms_per_kernel_events": round(ev_ms, 5),
+           "csr_model": {"bytes_per_launch": csr_bytes,
+                         "gbs": round(csr_bytes / (launch_ms * 1e-3) / 1e9, 1),
+                         "over_peak": round(csr_bytes / (launch_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}}
+    if traffic:
+        tg = traffic / (launch_ms * 1e-3) / 1e9
+        out.update(traffic_source=traffic_src, traffic_over_format_bytes=round(traffic / fmt_bytes, 3),
+                   traffic_gbs=round(tg, 1), traffic_frac=round(tg / HBM_PEAK_GBS, 4))
+    return out
+
+
+# ---------------------------------------------------------------- sweep (value)
+def run_sweep(qa, recipes, build_system, subsystem_layout, K, W, n_inst, renorm):
+    from bench_workloads import _pops
+    dt = 0.01
+    insts = recipes.tc_instances(n_inst, 2)
+    t0 = time.perf_counter()
+    systems, mats, tots, subs, x0s = [], [], [], [], []
+    for p in insts:
+        r = recipes.tc_recipe(*p)
+        qs, _, _ = build_system(r)
+        N, loc = recipes.tc_init(p[0], p[1])
+        x0 = np.zeros(N * N, np.complex128)
+        x0[loc] = 1.0
+        systems.append(qs)
+        mats.append(qs.matrix())
+        tots.append(N)
+        subs.append(subsystem_layout(r))
+        x0s.append(x0)
+    qa.synchronize()
+    asm_s = time.perf_counter() - t0
+    rows = sum(t * t for t in tots)
+    nnz = sum(m.info()[1] for m in mats)
+
+    b = qa.Batch(mats, tots, prec=qa.FP32)
+    for i, x0 in enumerate(x0s):
+        b.set_state(i, x0)
+    bA, bx = b.matrix(), b.state()
+    qa.rk4_prepare(bA, bx, dt)                  # autotune + graph capture, untimed
+    b.rk4(dt, W, renorm_every=0)
+    qa.synchronize()
+    t = time.perf_counter()
+    with qa.StreamTimer() as tm32:
+        b.rk4(dt, K, renorm_every=renorm)
+    qa.synchronize()
+    el32 = time.perf_counter() - t
+    pops32 = b.populations(subs)
+    lanes = bA.lanes()
+    fmt_step, layout = format_step_bytes(bA, rows)
+    nprof = min(20, max(K, 1))
+    ev_ms = qa.rk4_device(bA, bx, dt, nprof, timed=True)   # per-launch events (state advanced past the check)
+    launch_ms = tm32.ms / (4 * K) if K > 0 else ev_ms
+
+    el64 = 0.0
+    worst = 0.0
+    for i, m in enumerate(mats):
+        v = qa.DeviceVec(tots[i] ** 2, qa.FP64)
+        v.set(x0s[i])
+        qa.rk4_prepare(m, v, dt)
+        qa.rk4_device(m, v, dt, W)
+        qa.synchronize()
+        t = time.perf_counter()
+        qa.rk4_device(m, v, dt, K)
+        qa.synchronize()
+        el64 += time.perf_counter() - t
+        worst = max(worst, float(np.abs(_pops(v, subs[i]) - pops32[i]).max()))
+    b.destroy()
+    for qs in systems:
+        qs.destroy()
+    v32 = n_inst * K / el32
+    v64 = n_inst * K / el64
+    rf = roofline_obj(kernel_name(lanes[1], "float") + " (mean of the 4 fused RK4 stage launches)",
+                      f"{n_inst} Tavis-Cummings instances, one block-diagonal batch", fmt_step / 4, launch_ms,
+                      ev_ms, None, None, rk4_step_bytes(nnz, rows, 8) / 4)
+    return {"v32": v32, "v64": v64, "el32": el32, "el64": el64, "worst": worst, "asm_s": asm_s, "rows": rows,
+            "nnz": nnz, "rows_min": min(t * t for t in tots), "rows_max": max(t * t for t in tots), "lanes": lanes,
+            "roofline": rf, "layout": layout}
+
+
+# --------------------------------------------------------- 10-qubit chain leg
+def run_chain(qa, recipes, build_system, subsystem_layout, K, W, renorm, nq=10, variant=0):
+    from bench_workloads import _pops, _trace
+    dt = 0.005
+    recipe = recipes.spin_chain_recipe(nq, seed=1)
+    N, loc = recipes.spin_chain_init(nq)
+    rows = N * N
+    t0 = time.perf_counter()
+    qs, _, _ = build_system(recipe)
+    qa.synchronize()
+    asm_s = time.perf_counter() - t0
+    A = qs.matrix()
+    if variant:
+        A.set_lanes(0, variant)
+    rows_, nnz = A.info()
+    assert rows_ == rows
+    subs = subsystem_layout(recipe)
+    x0 = np.zeros(rows, np.complex128)
+    x0[loc] = 1.0
+    res = {}
+    for prec in (qa.FP64, qa.FP32):
+        rn = renorm if prec == qa.FP32 else 0
+        v = qa.DeviceVec(rows, prec)
+        v.set(x0)
+        qa.rk4_prepare(A, v, dt, tot=N, renorm_every=rn)   # autotune + graph capture, untimed
+        qa.rk4_device(A, v, dt, W, tot=N, renorm_every=rn)
+        qa.synchronize()
+        t = time.perf_counter()
+        with qa.StreamTimer() as tm:
+            qa.rk4_device(A, v, dt, K, tot=N, renorm_every=rn)
+        qa.synchronize()
+        res[prec] = (time.perf_counter() - t, tm.ms, _pops(v, subs), _trace(v, N).real)
+    nprof = min(50, max(K, 1))
+    ev = {}
+    for prec in (qa.FP32, qa.FP64):
+        v = qa.DeviceVec(rows, prec)
+        v.set(x0)
+        ev[prec] = qa.rk4_device(A, v, dt, nprof, tot=N, timed=True)
+    spmv32 = qa.bench_spmv(A, qa.FP32, 30)
+    spmv64 = qa.bench_spmv(A, qa.FP64, 30)
+    lanes = A.lanes()
+    fmt_step, layout = format_step_bytes(A, rows)
+    qs.destroy()
+    el64, dev64, p64, tr64 = res[qa.FP64]
+    el32, dev32, p32, tr32 = res[qa.FP32]
+    launch32 = dev32 / (4 * K) if K > 0 else ev[qa.FP32]
+    launch64 = dev64 / (4 * K) if K > 0 else ev[qa.FP64]
+    traffic, src = read_traffic(os.path.join(ROOT, "profiles", "k_ks2_stage_chain10_traffic.txt"))
+    rf = roofline_obj(kernel_name(lanes[1], "float") + " (mean of the 4 fused RK4 stage launches)",
+                      f"{nq}-qubit dissipative spin chain, superoperator {rows} rows {nnz} nnz, FP32",
+                      fmt_step / 4, launch32, ev[qa.FP32], traffic, src, rk4_step_bytes(nnz, rows, 8) / 4)
+    sp32 = spmv_bytes(nnz, rows, 8) / (spmv32 * 1e-3) / 1e9
+    sp64 = spmv_bytes(nnz, rows, 16) / (spmv64 * 1e-3) / 1e9
+    return {
+        "recipe": recipe, "x0": x0, "N": N, "dt": dt, "pops64": p64, "pops32": p32, "roofline": rf,
+        "out": {"workload": f"{nq}-qubit dissipative spin chain, rho {N}x{N}, superoperator {rows} rows {nnz} nnz, "
+                            f"RK4 dt={dt}, h_i,J_i~U[0.5,1.5] seed 1, gamma1=1e-3, gamma2=5e-3",
+                "fp32_steps_per_s": round(K / el32, 2), "fp64_steps_per_s": round(K / el64, 2),
+                "fp32_over_fp64": round(el64 / el32, 3), "renorm_every_fp32": renorm,
+                "max_pop_err_fp32_vs_fp64": float(np.abs(p32 - p64).max()), "trace_fp32": tr32, "trace_fp64": tr64,
+                "fp64_stage": {"kernel": kernel_name(lanes[0], "double"), "launch_ms": round(launch64, 5),
+                               "launch_ms_per_kernel_events": round(ev[qa.FP64], 5),
+                               "csr_model_gbs": round(rk4_step_bytes(nnz, rows, 16) / 4 / (launch64 * 1e-3) / 1e9,
+                                                      1)},
+                "spmv": {"fp32_ms": round(spmv32, 5), "fp64_ms": round(spmv64, 5),
+                         "fp32_csr_model_gbs": round(sp32, 1), "fp64_csr_model_gbs": round(sp64, 1),
+                         "lanes_per_row": lanes},
+                "assembly_s": round(asm_s, 4), "sliced_layout": layout}}
+
+
+def measure_copy(qa):
+    """device-to-device copy of 2^26 complex128 (1 GiB read + 1 GiB written)"""
+    n = 1 << 26
+    src = qa.DeviceVec(n, qa.FP64)
+    dst = qa.DeviceVec(n, qa.FP64)
+    qa._qh(qa.lib().qh_vec_copy(src.h, dst.h))
+    qa.synchronize()
+    reps = 10
+    with qa.StreamTimer() as tm:
+        for _ in range(reps):
+            qa._qh(qa.lib().qh_vec_copy(src.h, dst.h))
+    ms = tm.ms / reps
+    gbs = 2 * 16 * n / (ms * 1e-3) / 1e9
+    return {"gbs": round(gbs, 1), "frac_of_peak": round(gbs / HBM_PEAK_GBS, 4), "bytes_per_copy": 2 * 16 * n,
+            "ms": round(ms, 4), "kind": "hipMemcpyAsync device to device, 1 GiB read + 1 GiB written"}
+
+
+# ------------------------------------------------------------------- CPU legs
+def cpu_sweep(recipes, n_inst, every, budget_s):
+    """oracle FP64 RK4 over every `every`-th instance of the sweep, one instance at
+    a time: instance-steps/s, median of 3 runs at 1 thread and at all threads"""
+    import oracle
+    nproc, allc = host_threads()
+    insts = recipes.tc_instances(n_inst, 2)[::every]
+    ms, xs = [], []
+    t = time.perf_counter()
+    for p in insts:
+        tot, terms = recipes.oracle_terms(recipes.tc_recipe(*p))
+        ms.append(oracle.construct(tot, terms, dm=True))
+        N, loc = recipes.tc_init(p[0], p[1])
+        x0 = np.zeros(N * N, np.complex128)
+        x0[loc] = 1.0
+        xs.append(x0)
+    asm = time.perf_counter() - t
+
+    def one(steps, th):
+        t = time.perf_counter()
+        for m, x in zip(ms, xs):
+            m.rk4(x, 0.01, steps, nthreads=th)
+        return time.perf_counter() - t
+
+    out = {}
+    for th in (1, allc):
+        probe = one(1, th)
+        steps = max(1, int(budget_s / 3 / max(probe, 1e-4)))
+        runs = [len(insts) * steps / one(steps, th) for _ in range(3)]
+        out[th] = (statistics.median(runs), steps, runs)
+    return out, len(insts), asm, nproc, allc
+
+
+def cpu_chain(recipes, chain, W, K):
+    """the oracle on the 10-qubit chain: W + K steps at all threads (populations
+    against the GPU's), and steps/s at 1 thread and all threads (median of 3)"""
+    import oracle
+    from qbuild import populations_np
+    nproc, allc = host_threads()
+    tot, terms = recipes.oracle_terms(chain["recipe"])
+    t = time.perf_counter()
+    m = oracle.construct(tot, terms, dm=True)
+    asm = time.perf_counter() - t
+    t = time.perf_counter()
+    x = m.rk4(chain["x0"], chain["dt"], W + K, nthreads=allc)
+    el = time.perf_counter() - t
+    pops = populations_np(x, chain["recipe"])
+    rate = {}
+    for th, steps in ((1, 2), (allc, 10)):
+        runs = []
+        for _ in range(3):
+            t = time.perf_counter()
+            m.rk4(chain["x0"], chain["dt"], steps, nthreads=th)
+            runs.append(steps / (time.perf_counter() - t))
+        rate[th] = statistics.median(runs)
+    return {"pop_err_fp64_vs_oracle": float(np.abs(chain["pops64"] - pops).max()),
+            "pop_err_fp32_vs_oracle": float(np.abs(chain["pops32"] - pops).max()),
+            "oracle_steps": W + K, "oracle_s": round(el, 3), "oracle_assembly_s": round(asm, 3),
+            "cpu_steps_per_s_1thread": round(rate[1], 3), "cpu_steps_per_s_allthreads": round(rate[allc], 3),
+            "threads": allc}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2000)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--nq", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--renorm", type=int, default=100)
-    ap.add_argument("--workload", default="chain10", choices=["chain10", "tc512", "circuit12", "jc", "copy", "kerr", "mcwf"])
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU time per thread count of the sweep sample")
+    ap.add_argument("--renorm", type=int, default=10, help="FP32 trace renormalisation period (steps)")
+    ap.add_argument("--workload", default="headline",
+                    choices=["headline", "tc512", "circuit12", "jc", "copy", "kerr", "mcwf"])
     ap.add_argument("--instances", type=int, default=512)
     ap.add_argument("--levels", type=int, default=8)
     ap.add_argument("--train", type=int, default=2000)
     ap.add_argument("--test", type=int, default=200)
     ap.add_argument("--rank", type=int, default=128)
     ap.add_argument("--epochs", type=int, default=2000)
+    ap.add_argument("--skip-sweep", action="store_true", help="only the 10-qubit chain leg (profiling runs)")
     ap.add_argument("--fp32-variant", type=int, default=0,
-                    help="force the FP32 stage-kernel variant (qh_mat_set_lanes code; 0 = autotune). Counter "
-                         "collection perturbs the autotuner's timings, so profiled runs name the variant the "
-                         "plain run selects")
+                    help="force the chain's FP32 stage-kernel variant (qh_mat_set_lanes code; 0 = autotune). "
+                         "Counter collection perturbs the autotuner's timings, so profiled runs name the variant "
+                         "the plain run selects")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on exactly one GPU: this build targets a single MI355X "
@@ -104,7 +381,7 @@ def main():
     import recipes
     from qbuild import build_system, subsystem_layout
 
-    if args.workload != "chain10":
+    if args.workload != "headline":
         import bench_workloads
         out = getattr(bench_workloads, args.workload)(args)
         print(json.dumps(out), flush=True)
@@ -112,164 +389,65 @@ def main():
 
     qa.set_option("ts_rk_type", "4")
     qa.set_option("ts_adapt_type", "none")
-    K, W, dt = args.steps, args.warmup, 0.005
-    recipe = recipes.spin_chain_recipe(args.nq, seed=1)
-    N, loc = recipes.spin_chain_init(args.nq)
-    rows = N * N
+    K, W = args.steps, args.warmup
+    log = (lambda m: print(m, file=sys.stderr, flush=True))
 
-    t0 = time.perf_counter()
-    qs, _, _ = build_system(recipe)
-    qa.synchronize()
-    assemble_s = time.perf_counter() - t0
-    A = qs.matrix()
-    if args.fp32_variant:
-        A.set_lanes(0, args.fp32_variant)
-    rows_, nnz = A.info()
-    assert rows_ == rows
-    lay = subsystem_layout(recipe)
-    levels, strides = [L for L, _ in lay], [s for _, s in lay]
-    x0 = np.zeros(rows, np.complex128)
-    x0[loc] = 1.0
-
-    dev_ms = {}
-
-    def run(prec, renorm):
-        v = qa.DeviceVec(rows, prec)
-        v.set(x0)
-        qa.rk4_device(A, v, dt, W, tot=N, renorm_every=renorm)   # warmup (autotune + graph capture)
-        qa.synchronize()
-        t = time.perf_counter()
-        with qa.StreamTimer() as tm:   # HIP events on the stream the stage kernels run on
-            qa.rk4_device(A, v, dt, K, tot=N, renorm_every=renorm)
-        qa.synchronize()
-        el = time.perf_counter() - t
-        dev_ms[prec] = tm.ms
-        dv = qa.DeviceVec(rows, qa.FP64)
-        qa._qh(qa.lib().qh_vec_copy(v.h, dv.h))
-        pops = np.zeros(len(levels))
-        import ctypes
-        lv = np.array(levels, np.int64)
-        st = np.array(strides, np.int64)
-        qa._qh(qa.lib().qh_populations(dv.h, len(levels), lv.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
-                                       st.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), 1,
-                                       pops.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
-        tr = ctypes.c_double()
-        ti = ctypes.c_double()
-        qa._qh(qa.lib().qh_trace(dv.h, N, ctypes.byref(tr), ctypes.byref(ti)))
-        return el, pops, tr.value
-
-    el64, pops64, tr64 = run(qa.FP64, 0)
-    el32, pops32, tr32 = run(qa.FP32, args.renorm)
-    pop_err = float(np.abs(pops32 - pops64).max())
-
-    # per-launch kernel durations (events around every stage kernel), FP32 and FP64
-    nprof = min(50, K)
-    v = qa.DeviceVec(rows, qa.FP32)
-    v.set(x0)
-    stage_ms32 = qa.rk4_device(A, v, dt, nprof, tot=N, timed=True)
-    v64 = qa.DeviceVec(rows, qa.FP64)
-    v64.set(x0)
-    stage_ms64 = qa.rk4_device(A, v64, dt, nprof, tot=N, timed=True)
-    spmv_ms32 = qa.bench_spmv(A, qa.FP32, 30)
-    spmv_ms64 = qa.bench_spmv(A, qa.FP64, 30)
-    lanes = A.lanes()
-    layout = A.layout_info()
-    # bytes the sliced-layout step actually has to move: 4 x (stored values + slot
-    # metadata) + 17 vector passes (4 gathers, 3 K and 3 Y writes, 3 x reads for
-    # the Y's, K1..K3 + x reads and the x' write in the last epilogue)
-    if lanes[1] in (140, 141, 142):  # row-pair layout: 40-byte slot metadata, values stored per lane pair
-        layout = dict(A.pair_layout_info(), kind="row-pair (128 rows per wavefront)")
-        meta32 = layout["meta_bytes"]
-        vals32 = layout["stored_pairs"] * 16 + layout["stored_real_pairs"] * 8
-    else:
-        layout = dict(layout, kind="one row per lane (64 rows per wavefront)")
-        meta32 = layout["slots"] * 32 + 12 * layout["slices"]
-        vals32 = layout["stored_vals"] * 8 + layout["stored_reals"] * 4
-    ks_step32 = 4 * (vals32 + meta32) + 17 * rows * 8
-
-    # mean stage-kernel duration over the timed FP32 region: its device time
-    # (event pair) over its 4K stage launches (includes the renormalisation
-    # kernels every renorm steps and the gaps between graph nodes)
-    launch_ms32 = dev_ms[qa.FP32] / (4 * K)
-    launch_ms64 = dev_ms[qa.FP64] / (4 * K)
-    b_step32 = rk4_step_bytes(nnz, rows, 8)
-    b_step64 = rk4_step_bytes(nnz, rows, 16)
-    ach_stage32 = (b_step32 / 4) / (launch_ms32 * 1e-3) / 1e9
-    ach_stage64 = (b_step64 / 4) / (launch_ms64 * 1e-3) / 1e9
-    spmv_gbs32 = spmv_bytes(nnz, rows, 8) / (spmv_ms32 * 1e-3) / 1e9
-    spmv_gbs64 = spmv_bytes(nnz, rows, 16) / (spmv_ms64 * 1e-3) / 1e9
-
-    kname = kernel_name(lanes[1], "float")
-    traffic, traffic_src = read_traffic(os.path.join(ROOT, "profiles", "k_ks2_stage_chain10_traffic.txt"))
-
-    steps_s32 = K / el32
-    steps_s64 = K / el64
+    sw = None
+    if not args.skip_sweep:
+        log(f"sweep: {args.instances} Tavis-Cummings instances, K={K} W={W}")
+        sw = run_sweep(qa, recipes, build_system, subsystem_layout, K, W, args.instances, 50)
+    log(f"10-qubit chain leg, K={K} W={W}")
+    ch = run_chain(qa, recipes, build_system, subsystem_layout, K, W, args.renorm, args.nq, args.fp32_variant)
+    log("copy bandwidth")
+    cp = measure_copy(qa)
 
     cpu = None
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import oracle  # CPU baseline leg only
-        cores = min(os.cpu_count() or 1, 16)
-        tot, terms = recipes.oracle_terms(recipe)
-        t = time.perf_counter()
-        m = oracle.construct(tot, terms, dm=True)
-        cpu_asm = time.perf_counter() - t
-        xs = x0.copy()
-        t = time.perf_counter()
-        xs = m.rk4(xs, dt, 1, nthreads=cores)
-        one = time.perf_counter() - t
-        n_cpu = max(1, int(args.cpu_seconds / max(one, 1e-3)))
-        t = time.perf_counter()
-        xs = m.rk4(xs, dt, n_cpu, nthreads=cores)
-        cpu_el = time.perf_counter() - t
-        cpu = {"value": round(n_cpu / cpu_el, 4), "unit": "RK4 steps/s (FP64)", "cores": cores, "kind": "port",
-               "sample": f"{n_cpu} RK4 steps of the same 10-qubit chain (1+{n_cpu} after a 1-step probe), "
-                         f"OpenMP over {cores} threads, {cpu_model()}; oracle assembly {cpu_asm:.2f} s"}
+        log("CPU oracle: chain populations and rates")
+        ch["out"]["oracle_check"] = cpu_chain(recipes, ch, W, K)
+        log("CPU oracle: sweep sample")
+        every = 16
+        rates, n_s, asm, nproc, allc = cpu_sweep(recipes, args.instances, every, args.cpu_seconds)
+        cpu = {"value": round(rates[allc][0], 3), "unit": "instance RK4 steps/s (FP64)", "cores": allc,
+               "kind": "port", "value_1thread": round(rates[1][0], 3), "nproc": nproc,
+               "runs_allthreads": [round(r, 3) for r in rates[allc][2]],
+               "runs_1thread": [round(r, 3) for r in rates[1][2]],
+               "sample": f"every {every}th instance of the {args.instances}-instance sweep ({n_s} instances), "
+                         f"{rates[allc][1]} RK4 steps each at {allc} threads and {rates[1][1]} at 1 thread, one "
+                         f"instance at a time; median of 3 runs; {cpu_model()}; oracle assembly {asm:.2f} s",
+               "chain10": {k: ch["out"]["oracle_check"][k] for k in ("cpu_steps_per_s_1thread",
+                                                                      "cpu_steps_per_s_allthreads", "threads")}}
 
+    if sw is not None:
+        value, unit = round(sw["v32"], 2), "instance RK4 steps/s"
+        ms_step = 1e3 * sw["el32"] / K
+        config = {"workload": f"{args.instances} Tavis-Cummings instances (seed 2), rows {sw['rows_min']}.."
+                              f"{sw['rows_max']}, total {sw['rows']} rows {sw['nnz']} nnz, RK4 dt=0.01, one FP32 "
+                              f"batch vs the plain FP64 path one instance at a time",
+                  "instances": args.instances, "parallelism": "single GPU", "renorm_every": 50,
+                  "batch_lanes": sw["lanes"]}
+    else:
+        value, unit = ch["out"]["fp32_steps_per_s"], "RK4 steps/s"
+        ms_step = 1e3 / value
+        config = {"workload": ch["out"]["workload"], "parallelism": "single GPU"}
     out = {
-        "metric": "batched Lindblad RK4 steps/s at 1e-6 pop err vs plain FP64; SpMV % HBM peak",
-        "value": round(steps_s32, 3),
-        "unit": "RK4 steps/s",
-        "n_gpus": 1,
-        "steps": K,
-        "warmup": W,
-        "ms_per_step": round(1e3 * el32 / K, 5),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "fp32 (complex64 matrix values and stage vectors, products and row sums in fp64, "
-                 "hi+lo complex64 state; fp64 trace renormalisation and observables)",
-        "data": "synthetic",
-        "config": {"workload": f"{args.nq}-qubit dissipative spin chain, rho {N}x{N}, superoperator {rows} rows "
-                               f"{nnz} nnz, RK4 dt={dt}, h_i,J_i~U[0.5,1.5] seed 1, gamma1=1e-3, gamma2=5e-3",
-                   "parallelism": "single GPU", "renorm_every": args.renorm},
-        "roofline": {"bound": "hbm", "achieved": round(ach_stage32, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(ach_stage32 / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": f"{kname} (mean of the 4 fused RK4 stage launches)",
-                     "bytes_per_launch": b_step32 / 4, "launch_ms": round(launch_ms32, 5),
-                     "launch_ms_per_kernel_events": round(stage_ms32, 5),
-                     "format_bytes_per_launch": ks_step32 / 4,
-                     "format_gbs": round((ks_step32 / 4) / (launch_ms32 * 1e-3) / 1e9, 1),
-                     "traffic_source": traffic_src,
-                     "traffic_gbs": round(traffic / (launch_ms32 * 1e-3) / 1e9, 1) if traffic else None,
-                     "traffic_frac": round(traffic / (launch_ms32 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
-                     if traffic else None},
+        "metric": METRIC, "value": value, "unit": unit, "n_gpus": 1, "steps": K, "warmup": W,
+        "ms_per_step": round(ms_step, 5), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32 (complex64 matrix values and stage vectors, products and row sums in fp64, hi+lo complex64 "
+                 "state; fp64 trace renormalisation and observables)",
+        "data": "synthetic", "config": config,
+        "roofline": ch["roofline"],
         "cpu_baseline": cpu,
-        "fp64_steps_per_s": round(steps_s64, 3),
-        "fp32_over_fp64": round(steps_s32 / steps_s64, 3),
-        "max_pop_err_fp32_vs_fp64": pop_err,
-        "trace_fp32": tr32, "trace_fp64": tr64,
-        "fp64_stage": {"achieved_gbs": round(ach_stage64, 1), "frac": round(ach_stage64 / HBM_PEAK_GBS, 4),
-                       "launch_ms": round(launch_ms64, 5), "launch_ms_per_kernel_events": round(stage_ms64, 5)},
-        "spmv": {"fp32_gbs": round(spmv_gbs32, 1), "fp32_frac": round(spmv_gbs32 / HBM_PEAK_GBS, 4),
-                 "fp64_gbs": round(spmv_gbs64, 1), "fp64_frac": round(spmv_gbs64 / HBM_PEAK_GBS, 4),
-                 "fp32_ms": round(spmv_ms32, 5), "fp64_ms": round(spmv_ms64, 5), "lanes_per_row": lanes},
-        "assembly_s": round(assemble_s, 4),
-        "sliced_layout": dict(layout, step_bytes_fp32=ks_step32,
-                              step_gbs_fp32=round(ks_step32 / (el32 / K) / 1e9, 1)),
     }
+    if sw is not None:
+        out.update({"plain_fp64_instance_steps_per_s": round(sw["v64"], 2),
+                    "batched_over_plain": round(sw["v32"] / sw["v64"], 3),
+                    "max_pop_err_fp32_batch_vs_fp64": sw["worst"], "sweep_assembly_s": round(sw["asm_s"], 3),
+                    "roofline_sweep": sw["roofline"], "sweep_layout": sw["layout"]})
+    out["chain10"] = ch["out"]
+    out["hbm_copy"] = cp
     print(json.dumps(out), flush=True)
-    qs.destroy()
 
 
 if __name__ == "__main__":

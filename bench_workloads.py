@@ -129,77 +129,62 @@ def tc512(args):
             "max_pop_err_fp32_batch_vs_fp64": worst, "assembly_s": round(asm_s, 3)}
 
 
-def _gate_u(name, theta):
-    s = 2 ** -0.5
-    if name == "H":
-        return np.array([[s, s], [s, -s]], np.complex128)
-    if name == "RX":
-        return np.array([[math.cos(theta / 2), -1j * math.sin(theta / 2)],
-                         [-1j * math.sin(theta / 2), math.cos(theta / 2)]])
-    if name == "RZ":
-        return np.diag([1.0, np.exp(1j * theta)])
-    if name == "CNOT":
-        return np.array([[1, 0, 0, 0], [0, 1, 0, 0], [0, 0, 0, 1], [0, 0, 1, 0]], np.complex128)
-    raise ValueError(name)
-
-
-def _layers(gates, nq):
-    """greedy ASAP layering (quantum_circuits.c:316-404 semantics)"""
-    cur = [0] * nq
-    layers = []
-    for g in gates:
-        lv = max(cur[q] for q in g[1])
-        while len(layers) <= lv:
-            layers.append([])
-        layers[lv].append(g)
-        for q in g[1]:
-            cur[q] = lv + 1
-    return layers
-
-
 def circuit12(args):
+    """the circuit through QuaC's circuit API: add_gate_to_circuit_sys (run time
+    0.1 per gate), schedule_circuit_layers, apply_circuit_to_sys, and one
+    time_step_sys call whose gate events fire between the fixed RK4 steps"""
     nq = 12
     N = 2 ** nq
     recipe = recipes.noisy_circuit_recipe(nq)
-    qs, _, _ = build_system(recipe)
-    A = qs.matrix()
     gates = recipes.random_circuit(nq, 300, 3)
-    layers = _layers(gates, nq)
     subs = subsystem_layout(recipe)
-    x0 = np.zeros(N * N, np.complex128)
-    x0[0] = 1.0
-    strides = [N // (2 ** (q + 1)) for q in range(nq)]  # qubit q = subsystem q (MSB first)
+    qa.set_option("ts_rk_type", "4")
+    qa.set_option("ts_adapt_type", "none")
     res = {}
+    t0 = time.perf_counter()
+    qs, _, _ = build_system(recipe)
+    qa.synchronize()
+    asm_s = time.perf_counter() - t0
+    nl = 0
     for prec in (qa.FP64, qa.FP32):
-        v = qa.DeviceVec(N * N, prec)
-        v.set(x0)
-        qa.rk4_device(A, v, 0.01, 1)  # autotune + graph capture outside the timed region
-        v.set(x0)
+        qa.set_option("quac_precision", "fp32" if prec == qa.FP32 else "fp64")
+        qa.set_option("quac_renorm", "10" if prec == qa.FP32 else "0")
+        # warm-up: autotune and graph capture of this precision, outside the timed region
+        w = qs.create_qvec()
+        w.add_loc(1.0, 0)
+        qs.time_step(w, 0.0, 0.1, 0.01, 100)
+        w.destroy()
+        circ = qa.Circuit(len(gates))
+        for (name, qb, th) in gates:
+            circ.add_gate(1.0, name, list(qb), theta=th, run_time=0.1)
+        qs.schedule(circ)
+        nl = circ.num_layers
+        qs.apply_circuit(circ, 0.0)
+        x = qs.create_qvec()
+        x.add_loc(1.0, 0)
         qa.synchronize()
         t = time.perf_counter()
-        for layer in layers:
-            for (name, qb, th) in layer:
-                u = _gate_u(name, th)
-                st = np.array([strides[q] for q in qb], np.int64)
-                uu = np.ascontiguousarray(u.reshape(-1)).view(np.float64)
-                qa._qh(qa.lib().qh_gate(v.h, N, 1, len(qb), st.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
-                                        uu.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
-            qa.rk4_device(A, v, 0.01, 10, tot=N, renorm_every=10 if prec == qa.FP32 else 0)
+        steps, _ = qs.time_step(x, 0.9, 1.0 + 0.1 * nl, 0.01, 10 ** 7)
         qa.synchronize()
         el = time.perf_counter() - t
-        res[prec] = (el, _pops(v, subs), _trace(v, N))
+        v = x.data()
+        res[prec] = (el, steps, _pops(v, subs), _trace(v, N))
+        x.destroy()
+        circ.destroy()
+    qa.set_option("quac_precision", "fp64")
+    qa.set_option("quac_renorm", "0")
     qs.destroy()
-    el64, p64, t64 = res[qa.FP64]
-    el32, p32, t32 = res[qa.FP32]
-    steps = 10 * len(layers)
+    el64, steps, p64, t64 = res[qa.FP64]
+    el32, _, p32, t32 = res[qa.FP32]
     return {"metric": "batched Lindblad RK4 steps/s at 1e-6 pop err vs plain FP64; SpMV % HBM peak",
             "value": round(steps / el32, 2), "unit": "RK4 steps/s (with gates)", "n_gpus": 1, "steps": steps,
-            "warmup": 1, "ms_per_step": round(1e3 * el32 / steps, 5), "higher_is_better": True,
+            "warmup": 10, "ms_per_step": round(1e3 * el32 / steps, 5), "higher_is_better": True,
             "scaling": "weak", "vs_baseline": None, "dtype": "fp32 bulk vs fp64", "data": "synthetic",
             "config": {"workload": f"12-qubit noisy circuit, rho 4096x4096, {len(gates)} gates (seed 3) in "
-                                   f"{len(layers)} layers, RK4 10 x 0.01 between layers, T1 1e-3, dephasing 2e-3"},
+                                   f"{nl} layers through schedule_circuit_layers / apply_circuit_to_sys / "
+                                   f"time_step_sys, RK4 10 x 0.01 between layers, T1 1e-3, dephasing 2e-3"},
             "fp64_steps_per_s": round(steps / el64, 2), "fp32_over_fp64": round(el64 / el32, 3),
-            "wall_s_fp32": round(el32, 4), "wall_s_fp64": round(el64, 4),
+            "wall_s_fp32": round(el32, 4), "wall_s_fp64": round(el64, 4), "assembly_s": round(asm_s, 4),
             "max_pop_err_fp32_vs_fp64": float(np.abs(p32 - p64).max()),
             "trace_fp32": abs(t32), "trace_fp64": abs(t64), "populations_fp64": [round(x, 10) for x in p64]}
 

@@ -121,12 +121,24 @@ int qh_rk4(qh_mat *A, qh_vec *x, double dt, int64_t nsteps, int64_t tot, int ren
 /* same steps launched eagerly with a HIP event pair around every stage kernel;
  * the mean stage-kernel duration is then returned by qh_last_stage_ms */
 int qh_rk4_timed(qh_mat *A, qh_vec *x, double dt, int64_t nsteps, int64_t tot, int renorm_every);
+/* the set-up of qh_rk4 with the same arguments, without stepping: SpMV variant
+ * chosen (autotuned on a first call), stage buffers allocated, step graph
+ * captured and instantiated (the TSSetUp of qsystem.c:937-1023) */
+int qh_rk4_prepare(qh_mat *A, qh_vec *x, double dt, int64_t tot, int renorm_every);
 /*
- * Time-dependent RHS (qsystem.c:1288-1330): y = A x + sum_k s_k(t) A_k x with
- * s_k given per stage by the host.  Used by the host RK4 loop for td terms.
+ * Time-dependent RHS (qsystem.c:1288-1330): y = A x + sum_k s_k A_k x in one
+ * kernel (A's CSR and a merged CSR of the A_k, built once and cached on A),
+ * s_k = s_re_im[2k] + i s_re_im[2k+1] given by the host, |s_k| <= 1e-10 skipped
+ * (qsystem.c:34,1320).  ntd <= 150.  FP64 or FP32 vectors (x, y one precision).
  */
 int qh_spmv_td(const qh_mat *A, int ntd, qh_mat *const *Ak, const double *s_re_im,
                const qh_vec *x, qh_vec *y);
+/* One classic RK4 step of size h of dx/dt = A x + sum_k s_k(t) A_k x, in place
+ * (FP64, or the FP32 hi+lo state): four fused stage launches; s4_re_im holds the
+ * coefficients of the stage times t, t+h/2, t+h/2, t+h (4 x ntd complex).  The
+ * RHS evaluations of time_step_sys with time-dependent terms (qsystem.c:946-950,
+ * 1288-1330); the stage buffers are kept with the cached merged structure. */
+int qh_rk4_td_step(qh_mat *A, int ntd, qh_mat *const *Ak, const double *s4_re_im, qh_vec *x, double h);
 /* generic stage helpers for host-driven integrators (RK3BS, td RK4) */
 int qh_axpy_to(qh_vec *y, const qh_vec *x, double c_re, double c_im, const qh_vec *k); /* y = x + c k */
 int qh_lincomb(qh_vec *y, int n, const double *c_re_im, qh_vec *const *v);             /* y = sum c_i v_i */

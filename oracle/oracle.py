@@ -21,6 +21,10 @@ IG, GG, GI = -1, 0, 1
 
 _lib = None
 
+# qo_tdfunc: f(t, k, &re, &im), the envelope of time-dependent term k
+TDFUNC = ctypes.CFUNCTYPE(None, ctypes.c_double, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                          ctypes.POINTER(ctypes.c_double))
+
 
 def build():
     subprocess.check_call(["make", "-s", "-C", HERE])
@@ -47,6 +51,10 @@ def lib():
         L.qo_rk4.argtypes = [ctypes.c_void_p, P(dbl), dbl, i64, ctypes.c_int]
         L.qo_rk3bs.argtypes = [ctypes.c_void_p, P(dbl), dbl, dbl, dbl, i64, dbl, dbl, dbl, dbl,
                                P(i64), P(dbl)]
+        L.qo_rk3bs_trial.argtypes = [ctypes.c_void_p, P(dbl), dbl, ctypes.c_int, P(dbl), P(dbl), P(dbl),
+                                     P(dbl), P(dbl), dbl, dbl, P(dbl)]
+        L.qo_rk4_td.argtypes = [ctypes.c_void_p, ctypes.c_int, P(ctypes.c_void_p), P(dbl), ctypes.c_int,
+                                TDFUNC, P(dbl), dbl, dbl, i64]
         L.qo_expect_dm.argtypes = [i64, P(dbl), ctypes.c_int, P(i64), P(dbl), P(dbl)]
         L.qo_expect_wf.argtypes = [i64, P(dbl), ctypes.c_int, P(i64), P(dbl), P(dbl)]
         L.qo_chain.argtypes = [i64, i64, ctypes.c_int, P(i64), ctypes.c_int, P(dbl), P(dbl), P(i64)]
@@ -114,6 +122,38 @@ class CSR:
         lib().qo_rk3bs(self.h, _p(x.view(np.float64), ctypes.c_double), t0, tmax, dt0, steps_max,
                        atol, rtol, hmin, hmax, ctypes.byref(st), ctypes.byref(te))
         return x, st.value, te.value
+
+    def rk3bs_trial(self, x, h, atol=5e-7, rtol=5e-7, k1=None):
+        """one Bogacki-Shampine trial step: (xn, err, [k1, k2, k3, k4]); k1 given = FSAL"""
+        x = np.ascontiguousarray(x, np.complex128)
+        ks = [np.array(k1, np.complex128, copy=True) if (i == 0 and k1 is not None) else np.empty_like(x)
+              for i in range(4)]
+        xn = np.empty_like(x)
+        err = ctypes.c_double()
+        d = ctypes.c_double
+        rc = lib().qo_rk3bs_trial(self.h, _p(x.view(np.float64), d), h, int(k1 is not None),
+                                  *[_p(k.view(np.float64), d) for k in ks], _p(xn.view(np.float64), d),
+                                  atol, rtol, ctypes.byref(err))
+        assert rc == 0
+        return xn, err.value, ks
+
+    def rk4_td(self, td, td_a, f, x, t0, dt, nsteps, wf_scale=False):
+        """classic RK4 of dx/dt = A x + sum_k a_k f_k(t) A_k x (x -i when wf_scale);
+        td: list of CSR (the A_k), td_a: complex a_k, f(t, k) -> complex"""
+        x = np.array(x, np.complex128, copy=True)
+
+        def cb(t, k, re, im):
+            v = complex(f(t, k))
+            re[0] = v.real
+            im[0] = v.imag
+
+        cfun = TDFUNC(cb)
+        hs = (ctypes.c_void_p * max(1, len(td)))(*[m.h for m in td])
+        a = np.array([[complex(c).real, complex(c).imag] for c in td_a] or [[0.0, 0.0]], np.float64).ravel()
+        rc = lib().qo_rk4_td(self.h, len(td), hs, _p(a, ctypes.c_double), int(wf_scale), cfun,
+                             _p(x.view(np.float64), ctypes.c_double), t0, dt, nsteps)
+        assert rc == 0
+        return x
 
     def gmres(self, tot, x0=None, restart=100, tol=1e-8, maxit=100000, jacobi=False):
         x = np.zeros(self.n, np.complex128) if x0 is None else np.array(x0, np.complex128, copy=True)

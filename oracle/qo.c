@@ -496,8 +496,45 @@ int qo_rk4_td(const qo_csr *m, int ntd, qo_csr *const *td, const double *td_a, i
 }
 
 /*
- * Bogacki-Shampine 3(2) (PETSc TSRK3BS tableau, FSAL) with a basic controller:
- * err = ||(x3 - x2)_i / (atol + rtol*max|x_i|)||_2 / sqrt(n) (weighted RMS),
+ * One Bogacki-Shampine 3(2) trial step (PETSc TSRK3BS tableau, qsystem.c:1004;
+ * Bogacki & Shampine 1989): k1 = A x (taken as given when have_k1: FSAL),
+ * k2 = A(x + h/2 k1), k3 = A(x + 3h/4 k2), xn = x + h(2/9 k1 + 1/3 k2 + 4/9 k3),
+ * k4 = A xn, and the embedded error e = h((2/9-7/24) k1 + (1/3-1/4) k2 +
+ * (4/9-1/3) k3 - 1/8 k4) measured as the weighted RMS norm
+ * sqrt(mean_i |e_i|^2 / (atol + rtol max(|x_i|, |xn_i|))^2).
+ */
+int qo_rk3bs_trial(const qo_csr *m, const double *x, double h, int have_k1, double *k1, double *k2,
+                   double *k3, double *k4, double *xn, double atol, double rtol, double *err)
+{
+  int64_t n2 = 2 * m->n;
+  double *y = malloc(sizeof(double) * n2);
+  if (!y) return QO_ERR_MEM;
+  if (!have_k1) qo_spmv(m, x, k1);
+  axpy_to(n2, y, x, 0.5 * h, k1);
+  qo_spmv(m, y, k2);
+  axpy_to(n2, y, x, 0.75 * h, k2);
+  qo_spmv(m, y, k3);
+  for (int64_t i = 0; i < n2; i++)
+    xn[i] = x[i] + h * ((2.0 / 9.0) * k1[i] + (1.0 / 3.0) * k2[i] + (4.0 / 9.0) * k3[i]);
+  qo_spmv(m, xn, k4);
+  double e2 = 0.0;
+  for (int64_t i = 0; i < n2; i += 2) {
+    double er = h * ((2.0 / 9.0 - 7.0 / 24.0) * k1[i] + (1.0 / 3.0 - 1.0 / 4.0) * k2[i] +
+                     (4.0 / 9.0 - 1.0 / 3.0) * k3[i] - (1.0 / 8.0) * k4[i]);
+    double ei = h * ((2.0 / 9.0 - 7.0 / 24.0) * k1[i + 1] + (1.0 / 3.0 - 1.0 / 4.0) * k2[i + 1] +
+                     (4.0 / 9.0 - 1.0 / 3.0) * k3[i + 1] - (1.0 / 8.0) * k4[i + 1]);
+    double ax = sqrt(x[i] * x[i] + x[i + 1] * x[i + 1]);
+    double an = sqrt(xn[i] * xn[i] + xn[i + 1] * xn[i + 1]);
+    double sc = atol + rtol * (ax > an ? ax : an);
+    e2 += (er * er + ei * ei) / (sc * sc);
+  }
+  *err = sqrt(e2 / (double)m->n);
+  free(y);
+  return QO_OK;
+}
+
+/*
+ * Adaptive Bogacki-Shampine 3(2) with a basic controller around qo_rk3bs_trial:
  * h_new = h * clamp(0.9 * err^(-1/3), 0.1, 10) limited to [hmin, hmax], and the
  * last step clipped to land on tmax (TS_EXACTFINALTIME_MATCHSTEP, qsystem.c:1001).
  */
@@ -508,32 +545,15 @@ int qo_rk3bs(const qo_csr *m, double *x, double t0, double tmax, double dt0,
   int64_t n2 = 2 * m->n;
   double *k1 = malloc(sizeof(double) * n2), *k2 = malloc(sizeof(double) * n2);
   double *k3 = malloc(sizeof(double) * n2), *k4 = malloc(sizeof(double) * n2);
-  double *y = malloc(sizeof(double) * n2), *xn = malloc(sizeof(double) * n2);
+  double *xn = malloc(sizeof(double) * n2);
   double t = t0, h = dt0;
   int64_t steps = 0;
   int have_k1 = 0;
   while (t < tmax - 1e-12 && steps < steps_max) {
     if (t + h > tmax) h = tmax - t;
-    if (!have_k1) qo_spmv(m, x, k1);
-    axpy_to(n2, y, x, 0.5 * h, k1);
-    qo_spmv(m, y, k2);
-    axpy_to(n2, y, x, 0.75 * h, k2);
-    qo_spmv(m, y, k3);
-    for (int64_t i = 0; i < n2; i++)
-      xn[i] = x[i] + h * ((2.0 / 9.0) * k1[i] + (1.0 / 3.0) * k2[i] + (4.0 / 9.0) * k3[i]);
-    qo_spmv(m, xn, k4);
-    double e2 = 0.0;
-    for (int64_t i = 0; i < n2; i += 2) {
-      double er = h * ((2.0 / 9.0 - 7.0 / 24.0) * k1[i] + (1.0 / 3.0 - 1.0 / 4.0) * k2[i] +
-                       (4.0 / 9.0 - 1.0 / 3.0) * k3[i] - (1.0 / 8.0) * k4[i]);
-      double ei = h * ((2.0 / 9.0 - 7.0 / 24.0) * k1[i + 1] + (1.0 / 3.0 - 1.0 / 4.0) * k2[i + 1] +
-                       (4.0 / 9.0 - 1.0 / 3.0) * k3[i + 1] - (1.0 / 8.0) * k4[i + 1]);
-      double ax = sqrt(x[i] * x[i] + x[i + 1] * x[i + 1]);
-      double an = sqrt(xn[i] * xn[i] + xn[i + 1] * xn[i + 1]);
-      double sc = atol + rtol * (ax > an ? ax : an);
-      e2 += (er * er + ei * ei) / (sc * sc);
-    }
-    double err = sqrt(e2 / (double)m->n);
+    double err = 0.0;
+    int rc = qo_rk3bs_trial(m, x, h, have_k1, k1, k2, k3, k4, xn, atol, rtol, &err);
+    if (rc) return rc;
     double fac = 0.9 * pow(err > 1e-300 ? err : 1e-300, -1.0 / 3.0);
     if (fac < 0.1) fac = 0.1;
     if (fac > 10.0) fac = 10.0;
@@ -551,7 +571,7 @@ int qo_rk3bs(const qo_csr *m, double *x, double t0, double tmax, double dt0,
   }
   if (steps_taken) *steps_taken = steps;
   if (t_end) *t_end = t;
-  free(k1); free(k2); free(k3); free(k4); free(y); free(xn);
+  free(k1); free(k2); free(k3); free(k4); free(xn);
   return QO_OK;
 }
 

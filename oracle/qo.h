@@ -23,10 +23,21 @@
  *                      values) + dense brute-force Lindbladian for VEC/OTHER ops
  *   qo_spmv/qo_rk4     degree-4 Taylor polynomial of exp(hA) on tiny dense A,
  *                      T1-decay closed forms (1/e), trace conservation
+ *   qo_rk3bs_trial     closed forms of one step on a linear ODE: the candidate is
+ *                      (1 + z + z^2/2 + z^3/6) x and the embedded error
+ *                      -(z^3 + z^4)/48 x (z = hA), on tiny dense A
  *   qo_rk3bs           T1-decay closed form (reference tolerance 1e-4)
+ *   qo_rk4_td          analytic Rabi drive with a decaying envelope
+ *                      (time_dep_tests.c:46-64,142-213), WF and DM forms; with
+ *                      f = 1 equal to qo_rk4 on the summed matrix
  *   qo_expect_dm       dense Tr(O rho) brute force; rand_dm.dat known answers
+ *   qo_expect_wf       dense <psi|O|psi> brute force; the |1,1> known answers of
+ *                      qvec_utilities_tests.c:232-278
  *   qo_gate_*          textbook U rho U^dagger on dense matrices; the reference's
- *                      hard-coded CNOT/H row values
+ *                      hard-coded rows of every gate in quantum_gates_test_sys.c
+ *                      (sigma x/y/z on one and two qubits, H, CNOT, CXZ and CZ in
+ *                      both orders); RX/RY/RZ/U1-3/EYE against exp(-i theta P/2)
+ *                      and the OpenQASM U3 definition
  *   qo_gmres           residual ||L'x - e0|| and the analytic driven-damped
  *                      two-level steady state
  */
@@ -90,6 +101,10 @@ int qo_rk4(const qo_csr *m, double *x, double dt, int64_t nsteps, int nthreads);
 typedef void (*qo_tdfunc)(double t, int k, double *re, double *im);
 int qo_rk4_td(const qo_csr *m, int ntd, qo_csr *const *td, const double *td_a, int wf_scale,
               qo_tdfunc f, double *x, double t0, double dt, int64_t nsteps);
+/* one Bogacki-Shampine 3(2) trial step: k1 (given when have_k1), k2, k3, k4,
+ * candidate xn and the weighted RMS norm of the embedded error */
+int qo_rk3bs_trial(const qo_csr *m, const double *x, double h, int have_k1, double *k1, double *k2,
+                   double *k3, double *k4, double *xn, double atol, double rtol, double *err);
 /* Bogacki-Shampine 3(2) adaptive, PETSc-TSAdaptBasic-style controller */
 int qo_rk3bs(const qo_csr *m, double *x, double t0, double tmax, double dt0,
              int64_t steps_max, double atol, double rtol, double hmin, double hmax,

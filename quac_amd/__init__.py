@@ -62,6 +62,16 @@ OperatorStruct._fields_ = [
     ("evals", Cx * 2), ("evecs", Cx * 4)]
 
 
+class QhOp(ctypes.Structure):
+    """qh_op of include/quac_hip.h: one operator of a monomial chain"""
+    _fields_ = [("stride", ctypes.c_int64), ("levels", ctypes.c_int32), ("tab", ctypes.c_int32)]
+
+
+class QhLevel(ctypes.Structure):
+    """qh_level of include/quac_hip.h: next/prev level and value of one table entry"""
+    _fields_ = [("next", ctypes.c_int32), ("prev", ctypes.c_int32), ("re", ctypes.c_double), ("im", ctypes.c_double)]
+
+
 class GateStruct(ctypes.Structure):
     _fields_ = [("time", ctypes.c_double), ("run_time", ctypes.c_double), ("my_gate_type", ctypes.c_int),
                 ("qubit_numbers", ctypes.POINTER(ctypes.c_int64)), ("num_qubits", ctypes.c_int64),
@@ -176,6 +186,7 @@ def _setup(L):
     sig("qh_spmv", i32, vp, vp, vp)
     sig("qh_rk3bs_step", i32, vp, vp, vp, vp, vp, vp, vp, vp, dbl, dbl, dbl, i32, P(dbl))
     sig("qh_rk4", i32, vp, vp, dbl, i64, i64, i32)
+    sig("qh_rk4_prepare", i32, vp, vp, dbl, i64, i32)
     sig("create_arb_qvec_dims", None, P(vp), i64, P(i64), i32)
     sig("destroy_qvec", None, P(vp))
     sig("copy_qvec", None, vp, vp)
@@ -217,6 +228,8 @@ def _setup(L):
     sig("qh_timer_begin", i32)
     sig("qh_timer_end", i32, P(dbl))
     sig("qh_trace", i32, vp, i64, P(dbl), P(dbl))
+    sig("qh_expect_dm", i32, vp, i64, i32, P(QhOp), i32, P(QhLevel), P(dbl), P(dbl))
+    sig("qh_expect_wf", i32, vp, i64, i32, P(QhOp), i32, P(QhLevel), P(dbl), P(dbl))
     sig("qh_populations", i32, vp, i32, P(i64), P(i64), i32, P(dbl))
     sig("qh_gate", i32, vp, i64, i32, i32, P(i64), P(dbl))
     sig("qh_gmres", i32, vp, i64, vp, i32, dbl, i64, i32, P(i64), P(dbl))
@@ -768,6 +781,12 @@ def rk4_device(mat, vec, dt, nsteps, tot=0, renorm_every=0, timed=False):
         _qh(lib().qh_last_stage_ms(ctypes.byref(ms), ctypes.byref(n)))
         return ms.value
     return None
+
+
+def rk4_prepare(mat, vec, dt, tot=0, renorm_every=0):
+    """qh_rk4_prepare: autotune, stage buffers and step-graph capture of a later
+    rk4_device call with the same arguments"""
+    _qh(lib().qh_rk4_prepare(mat.h, vec.h, dt, tot, renorm_every))
 
 
 def set_graph_steps(n):

@@ -481,12 +481,10 @@ static int mcwf_monitor(qsystem s, PetscInt step, PetscReal t, qvec x) {
   return s->ts_monitor(&ts, step, t, x->ens_spawned ? x->ens_datas[0] : x->data, s->ts_ctx);
 }
 
-/* time-dependent RHS y = A x + sum_k s_k(t) A_k x (qsystem.c:1300-1330) */
-static int td_rhs(qsystem s, PetscReal t, Vec x, Vec y) {
-  int ntd = (int)s->num_time_dep;
-  double sc[2 * 150];
-  Mat mats[150];
-  for (int k = 0; k < ntd; k++) {
+/* s_k(t) = a_k f_k(t) of every time-dependent term, times -i for a WF or MCWF
+ * generator (qsystem.c:1300-1330), as re/im pairs; mats gets the A_k */
+static void td_coefs(qsystem s, PetscReal t, double *sc, Mat *mats) {
+  for (PetscInt k = 0; k < s->num_time_dep; k++) {
     PetscScalar f = 1.0;
     if (s->time_dep[k].time_dep_func) {
       f = s->time_dep[k].time_dep_func(t, s->time_dep[k].ctx);
@@ -499,9 +497,36 @@ static int td_rhs(qsystem s, PetscReal t, Vec x, Vec y) {
     if (!s->dm_equations || s->mcwf_solver) v = -PETSC_i * v;
     sc[2 * k] = creal(v);
     sc[2 * k + 1] = cimag(v);
-    mats[k] = s->time_dep[k].mat_A;
+    if (mats) mats[k] = s->time_dep[k].mat_A;
   }
-  return qh_spmv_td(s->mat_A, ntd, mats, sc, x, y);
+}
+
+/* time-dependent RHS y = A x + sum_k s_k(t) A_k x (qsystem.c:1300-1330): one fused kernel */
+static int td_rhs(qsystem s, PetscReal t, Vec x, Vec y) {
+  double sc[2 * NUM_INIT_ALLOC];
+  Mat mats[NUM_INIT_ALLOC];
+  td_coefs(s, t, sc, mats);
+  return qh_spmv_td(s->mat_A, (int)s->num_time_dep, mats, sc, x, y);
+}
+
+/* rho <- rho / Tr(rho) (FP64 trace of the FP32 state) */
+static int renorm_trace(Vec x, PetscInt tot) {
+  double re = 0.0, im = 0.0;
+  int rc = qh_trace(x, tot, &re, &im);
+  if (!rc && re != 0.0) rc = qh_vec_scale(x, 1.0 / re, 0.0);
+  return rc;
+}
+
+/* one classic RK4 step of length h at time t with the time-dependent terms: the
+ * envelopes at the four stage times (each evaluated once per stage, as the
+ * reference's RHS callback is), then four fused stage launches */
+static int rk4_td_step(qsystem s, Vec x, PetscReal t, PetscReal h) {
+  const int n = (int)s->num_time_dep;
+  double sc[4 * 2 * NUM_INIT_ALLOC];
+  Mat mats[NUM_INIT_ALLOC];
+  const PetscReal ts[4] = {t, t + 0.5 * h, t + 0.5 * h, t + h};
+  for (int j = 0; j < 4; j++) td_coefs(s, ts[j], sc + 2 * (size_t)j * n, mats);
+  return qh_rk4_td_step(s->mat_A, n, mats, sc, x, h);
 }
 
 typedef struct {
@@ -519,24 +544,6 @@ static void work_free(work_t *w) {
   for (int i = 0; i < 4; i++) qh_vec_destroy(w->k[i]);
   qh_vec_destroy(w->y);
   qh_vec_destroy(w->xn);
-}
-
-/* one RK4 step of length h at time t with a host-driven RHS (time-dependent terms) */
-static int rk4_td_step(qsystem s, Vec x, PetscReal t, PetscReal h, work_t *w) {
-  int rc = td_rhs(s, t, x, w->k[0]);
-  if (!rc) rc = qh_axpy_to(w->y, x, 0.5 * h, 0.0, w->k[0]);
-  if (!rc) rc = td_rhs(s, t + 0.5 * h, w->y, w->k[1]);
-  if (!rc) rc = qh_axpy_to(w->y, x, 0.5 * h, 0.0, w->k[1]);
-  if (!rc) rc = td_rhs(s, t + 0.5 * h, w->y, w->k[2]);
-  if (!rc) rc = qh_axpy_to(w->y, x, h, 0.0, w->k[2]);
-  if (!rc) rc = td_rhs(s, t + h, w->y, w->k[3]);
-  if (!rc) {
-    double c[10] = {1.0, 0.0, h / 6.0, 0.0, h / 3.0, 0.0, h / 3.0, 0.0, h / 6.0, 0.0};
-    Vec v[5] = {x, w->k[0], w->k[1], w->k[2], w->k[3]};
-    rc = qh_lincomb(w->y, 5, c, v);
-    if (!rc) rc = qh_vec_copy(w->y, x);
-  }
-  return rc;
 }
 
 /*
@@ -601,8 +608,9 @@ void time_step_sys(qsystem qsys, qvec x, PetscReal init_time, PetscReal time_max
   const PetscReal eps = 1e-12 * (fabs(time_max) + fabs(dt) + 1.0);
   if (qsys->ts_monitor && call_monitor(qsys, &ts, x)) QUAC_FAIL(9, "monitor returned an error");
 
-  if (rk4 && fixed && !td) {
-    /* fused device RK4 (FP64 drop-in path, or the FP32 bulk path) */
+  if (rk4 && fixed) {
+    /* fused device RK4 (FP64 drop-in path, or the FP32 bulk path); with
+     * time-dependent terms one fused-RHS step at a time */
     Vec work = x->data;
     Vec w32 = NULL;
     if (fp32) {
@@ -631,7 +639,18 @@ void time_step_sys(qsystem qsys, qvec x, PetscReal init_time, PetscReal time_max
         if (t + h > time_max - eps) h = time_max - t;
         if (have_ev && t + h > t_ev - eps) h = t_ev - t;
       }
-      if (nfull == 1 && h != dt) {
+      if (td) {
+        const PetscReal hs = (nfull == 1 && h != dt) ? h : dt;
+        int bad = 0;
+        for (PetscInt j = 0; j < nfull && !bad; j++) {
+          bad = rk4_td_step(qsys, work, t + (double)j * hs, hs);
+          /* FP32: trace renormalisation every quac_renorm steps */
+          if (!bad && renorm_ok && (step + j + 1) % renorm == 0) bad = renorm_trace(work, x->total_levels);
+        }
+        if (bad) break;
+        t += (double)nfull * hs;
+        if (have_ev && fabs(t - t_ev) < 1e-9) t = t_ev;
+      } else if (nfull == 1 && h != dt) {
         if (qh_rk4(qsys->mat_A, work, h, 1, renorm_ok ? x->total_levels : 0, 0)) break;
         t += h;
       } else {
@@ -641,9 +660,12 @@ void time_step_sys(qsystem qsys, qvec x, PetscReal init_time, PetscReal time_max
       }
       step += nfull;
       if (events && have_ev && fabs(t - t_ev) <= 1e-9) {
-        if (fp32) QUAC_HIPCHK(qh_vec_copy(w32, x->data));
+        /* the FP32 path applies the gates to its hi+lo state directly (the gate
+         * kernel computes in FP64 and splits the result again) */
+        Vec keep = x->data;
+        x->data = work;
         qc_apply_events(qsys, t, x);
-        if (fp32) QUAC_HIPCHK(qh_vec_copy(x->data, w32));
+        x->data = keep;
       }
       if (qsys->ts_monitor) {
         if (fp32) QUAC_HIPCHK(qh_vec_copy(w32, x->data));
@@ -657,8 +679,8 @@ void time_step_sys(qsystem qsys, qvec x, PetscReal init_time, PetscReal time_max
       qh_vec_destroy(w32);
     }
   } else {
-    /* host-driven integrators on device vectors: RK4 with time-dependent terms,
-     * or adaptive RK3BS (atol = rtol = 5e-7, dt limits [0.01, 1]: qsystem.c:902,995) */
+    /* host-driven RK3BS on device vectors, adaptive (atol = rtol = 5e-7, dt
+     * limits [0.01, 1]: qsystem.c:902,995) or at a fixed step */
     work_t w;
     if (work_alloc(&w, x->n)) {
       work_free(&w);
@@ -675,12 +697,9 @@ void time_step_sys(qsystem qsys, qvec x, PetscReal init_time, PetscReal time_max
       if (t + hh > time_max) hh = time_max - t;
       if (have_ev && t + hh > t_ev) hh = t_ev - t;
       if (fixed) {
-        rc = rk4 ? rk4_td_step(qsys, x->data, t, hh, &w) : 0;
-        if (!rk4) {
-          double err;
-          rc = rk3bs_try(qsys, x->data, t, hh, &w, 0, atol, rtol, &err);
-          if (!rc) rc = qh_vec_copy(w.xn, x->data);
-        }
+        double err;
+        rc = rk3bs_try(qsys, x->data, t, hh, &w, 0, atol, rtol, &err);
+        if (!rc) rc = qh_vec_copy(w.xn, x->data);
         t += hh;
         step++;
       } else {

@@ -351,6 +351,7 @@ cleanup:
 
 extern "C" int qh_mat_destroy(qh_mat *m) {
   if (!m) return QH_OK;
+  qh_td_cache_free(m->td_cache);
   qh_free_ptr(m->rowptr);
   qh_free_ptr(m->col);
   qh_free_ptr(m->val);

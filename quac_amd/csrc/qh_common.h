@@ -81,6 +81,9 @@ struct qh_kmeta2 {
 // its metadata without a bounds check
 #define QH_KS_PAD 16
 
+struct qh_td_cache;  // merged time-dependent matrices and stage buffers (qh_td.hip)
+void qh_td_cache_free(qh_td_cache *c);
+
 struct qh_mat {
   int64_t rows = 0, nnz = 0;
   int32_t *rowptr = nullptr;   // rows+1
@@ -117,6 +120,7 @@ struct qh_mat {
   float4 *ks2_val = nullptr;    // complex slots: (row 2l, row 2l+1) complex64 pairs
   float2 *ks2_sval = nullptr;   // scalar slots: (row 2l, row 2l+1) real pairs
   int64_t ks2_slots = 0, ks2_nval = 0, ks2_nsval = 0;
+  qh_td_cache *td_cache = nullptr;  // built by the first time-dependent RHS on this matrix
 };
 // variant codes selecting the sliced-ELL kernel: 128 / 129 / 130 process 2 / 4 / 8
 // slots per pass; 131 / 132 process 4 / 8 slots per pass and read the uniform

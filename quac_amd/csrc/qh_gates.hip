@@ -19,6 +19,9 @@ struct U4 {
 
 // insert a zero digit of a 2-level subsystem at stride s
 __device__ inline int64_t ins0(int64_t t, int64_t s) { return (t / s) * (2 * s) + t % s; }
+// the same for a power-of-two stride s = 2^k: shifts and a mask instead of the
+// emulated 64-bit division
+__device__ inline int64_t ins0p(int64_t t, int k) { return ((t >> k) << (k + 1)) | (t & ((1LL << k) - 1)); }
 
 __device__ inline double2 ldv(const void *d, const float2 *lo, int64_t i) {
   if (!lo) return ((const double2 *)d)[i];
@@ -35,17 +38,24 @@ __device__ inline void stv(void *d, float2 *lo, int64_t i, double2 v) {
   lo[i] = make_float2((float)(v.x - (double)hr), (float)(v.y - (double)hi));
 }
 
-// K = 2 (1 qubit) or 4 (2 qubits); DM: X' = U X U^dagger on a KxK block; WF: psi' = U psi
-template <int K, bool DM>
+// K = 2 (1 qubit) or 4 (2 qubits); DM: X' = U X U^dagger on a KxK block; WF: psi' = U psi.
+// P2: every inserted stride is a power of two, passed as its exponent in lg.
+template <int K, bool DM, bool P2>
 __global__ void __launch_bounds__(256) k_gate(void *x, float2 *lo, int64_t nthreads, int nins, int64_t s0,
-                                              int64_t s1, int64_t s2, int64_t s3, int64_t bs0, int64_t bs1,
+                                              int64_t s1, int64_t s2, int64_t s3, int4 lg, int64_t bs0, int64_t bs1,
                                               int64_t cs0, int64_t cs1, U4 u) {
   const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (t >= nthreads) return;
   // insert the gate's zero digits, lowest stride first
   int64_t base = t;
-  const int64_t ss[4] = {s0, s1, s2, s3};
-  for (int q = 0; q < nins; q++) base = ins0(base, ss[q]);
+  if (P2) {
+    const int ks[4] = {lg.x, lg.y, lg.z, lg.w};
+#pragma unroll
+    for (int q = 0; q < (DM ? 2 * (K / 2) : K / 2); q++) base = ins0p(base, ks[q]);
+  } else {
+    const int64_t ss[4] = {s0, s1, s2, s3};
+    for (int q = 0; q < nins; q++) base = ins0(base, ss[q]);
+  }
   // offsets of the K basis states of the gate (first qubit = most significant)
   int64_t ro[K], co[K];
 #pragma unroll
@@ -141,18 +151,29 @@ extern "C" int qh_gate(qh_vec *x, int64_t tot, int dm, int nq, const int64_t *st
   const int64_t cs0 = dm ? strides[0] * tot : 0, cs1 = (dm && nq == 2) ? strides[1] * tot : 0;
   const dim3 grid(qh_blocks(nthreads, 256));
   hipStream_t s = qh_stream();
-  if (nq == 1 && dm)
-    hipLaunchKernelGGL((k_gate<2, true>), grid, dim3(256), 0, s, x->d, x->lo, nthreads, nins, ins[0], ins[1], ins[2],
-                       ins[3], bs0, bs1, cs0, cs1, m);
-  else if (nq == 1)
-    hipLaunchKernelGGL((k_gate<2, false>), grid, dim3(256), 0, s, x->d, x->lo, nthreads, nins, ins[0], ins[1],
-                       ins[2], ins[3], bs0, bs1, cs0, cs1, m);
-  else if (dm)
-    hipLaunchKernelGGL((k_gate<4, true>), grid, dim3(256), 0, s, x->d, x->lo, nthreads, nins, ins[0], ins[1], ins[2],
-                       ins[3], bs0, bs1, cs0, cs1, m);
-  else
-    hipLaunchKernelGGL((k_gate<4, false>), grid, dim3(256), 0, s, x->d, x->lo, nthreads, nins, ins[0], ins[1],
-                       ins[2], ins[3], bs0, bs1, cs0, cs1, m);
+  // exponents when every inserted stride is a power of two
+  bool p2 = true;
+  int lgv[4] = {0, 0, 0, 0};
+  for (int q = 0; q < nins; q++) {
+    if (ins[q] & (ins[q] - 1)) p2 = false;
+    int k = 0;
+    while (k < 62 && ((int64_t)1 << k) < ins[q]) k++;
+    lgv[q] = k;
+  }
+  const int4 lg = make_int4(lgv[0], lgv[1], lgv[2], lgv[3]);
+#define QH_GATE(KK, DMM, PP)                                                                                        \
+  hipLaunchKernelGGL((k_gate<KK, DMM, PP>), grid, dim3(256), 0, s, x->d, x->lo, nthreads, nins, ins[0], ins[1],     \
+                     ins[2], ins[3], lg, bs0, bs1, cs0, cs1, m)
+  if (nq == 1 && dm) {
+    if (p2) QH_GATE(2, true, true); else QH_GATE(2, true, false);
+  } else if (nq == 1) {
+    if (p2) QH_GATE(2, false, true); else QH_GATE(2, false, false);
+  } else if (dm) {
+    if (p2) QH_GATE(4, true, true); else QH_GATE(4, true, false);
+  } else {
+    if (p2) QH_GATE(4, false, true); else QH_GATE(4, false, false);
+  }
+#undef QH_GATE
   QH_HIP(hipGetLastError());
   return QH_OK;
 }

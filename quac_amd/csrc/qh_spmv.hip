@@ -1391,6 +1391,56 @@ static int rk4_small(qh_mat *A, qh_vec *x, double dt, int64_t nsteps, int64_t to
   return QH_OK;
 }
 
+// steps per captured graph: pairs of steps (the state ping-pongs back into the
+// caller's buffer), exactly renorm_every (rounded up to a pair) when the trace
+// is renormalised
+static int graph_pair_steps(int renorm_every) {
+  int pair_steps = g_graph_steps & ~1;
+  if (pair_steps < 2) pair_steps = 2;
+  if (renorm_every > 0) pair_steps = renorm_every + (renorm_every & 1);
+  return pair_steps;
+}
+
+// capture and instantiate the graph of pair_steps steps of x under A, unless the
+// one held was captured with the same parameters
+static int graph_prepare(const qh_mat *A, qh_vec *x, double dt, int64_t tot, int renorm_every, int G,
+                         int pair_steps) {
+  hipStream_t s = qh_stream();
+  const bool recapture = !g_st.exec || g_st.gx != x->d || g_st.glo != x->lo || g_st.gdt != dt ||
+                         g_st.gsteps != pair_steps || g_st.grenorm != renorm_every || g_st.gG != G ||
+                         g_st.gtot != tot;
+  if (!recapture) return QH_OK;
+  if (g_st.exec) (void)hipGraphExecDestroy(g_st.exec);
+  if (g_st.graph) (void)hipGraphDestroy(g_st.graph);
+  g_st.exec = nullptr;
+  g_st.graph = nullptr;
+  QH_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  int crc = QH_OK;
+  for (int st = 0; st < pair_steps && !crc; st++) {
+    void *xa = (st & 1) ? g_st.xalt : x->d, *xb = (st & 1) ? x->d : g_st.xalt;
+    crc = x->prec == QH_FP64 ? rk4_step<double>(s, A, G, xa, xb, x->lo, dt, nullptr)
+                             : rk4_step<float>(s, A, G, xa, xb, x->lo, (float)dt, nullptr);
+  }
+  if (!crc && renorm_every > 0) crc = renorm_launch(s, x, x->d, tot, 0);
+  hipGraph_t g = nullptr;
+  hipError_t ce = hipStreamEndCapture(s, &g);
+  if (crc) {
+    if (g) (void)hipGraphDestroy(g);
+    return crc;
+  }
+  QH_HIP(ce);
+  g_st.graph = g;
+  QH_HIP(hipGraphInstantiate(&g_st.exec, g, nullptr, nullptr, 0));
+  g_st.gx = x->d;
+  g_st.glo = x->lo;
+  g_st.gdt = dt;
+  g_st.gsteps = pair_steps;
+  g_st.grenorm = renorm_every;
+  g_st.gG = G;
+  g_st.gtot = tot;
+  return QH_OK;
+}
+
 static int rk4_impl(qh_mat *A, qh_vec *x, double dt, int64_t nsteps, int64_t tot, int renorm_every, int timed) {
   int rc = qh_ensure_init();
   if (rc) return rc;
@@ -1441,47 +1491,11 @@ static int rk4_impl(qh_mat *A, qh_vec *x, double dt, int64_t nsteps, int64_t tot
   }
 
   // graph path: capture 2*P steps (P pairs) once, replay
-  int pair_steps = g_graph_steps & ~1;
-  if (pair_steps < 2) pair_steps = 2;
-  if (renorm_every > 0) {
-    // renormalise exactly every renorm_every steps: capture that many (rounded to pairs)
-    pair_steps = renorm_every + (renorm_every & 1);
-  }
+  const int pair_steps = graph_pair_steps(renorm_every);
   const int64_t full = nsteps / pair_steps;
   if (full > 0) {
-    const bool recapture = !g_st.exec || g_st.gx != x->d || g_st.glo != x->lo || g_st.gdt != dt ||
-                           g_st.gsteps != pair_steps || g_st.grenorm != renorm_every || g_st.gG != G ||
-                           g_st.gtot != tot;
-    if (recapture) {
-      if (g_st.exec) (void)hipGraphExecDestroy(g_st.exec);
-      if (g_st.graph) (void)hipGraphDestroy(g_st.graph);
-      g_st.exec = nullptr;
-      g_st.graph = nullptr;
-      QH_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-      int crc = QH_OK;
-      for (int st = 0; st < pair_steps && !crc; st++) {
-        void *xa = (st & 1) ? g_st.xalt : x->d, *xb = (st & 1) ? x->d : g_st.xalt;
-        crc = x->prec == QH_FP64 ? rk4_step<double>(s, A, G, xa, xb, x->lo, dt, nullptr)
-                                 : rk4_step<float>(s, A, G, xa, xb, x->lo, (float)dt, nullptr);
-      }
-      if (!crc && renorm_every > 0) crc = renorm_launch(s, x, x->d, tot, 0);
-      hipGraph_t g = nullptr;
-      hipError_t ce = hipStreamEndCapture(s, &g);
-      if (crc) {
-        if (g) (void)hipGraphDestroy(g);
-        return crc;
-      }
-      QH_HIP(ce);
-      g_st.graph = g;
-      QH_HIP(hipGraphInstantiate(&g_st.exec, g, nullptr, nullptr, 0));
-      g_st.gx = x->d;
-      g_st.glo = x->lo;
-      g_st.gdt = dt;
-      g_st.gsteps = pair_steps;
-      g_st.grenorm = renorm_every;
-      g_st.gG = G;
-      g_st.gtot = tot;
-    }
+    rc = graph_prepare(A, x, dt, tot, renorm_every, G, pair_steps);
+    if (rc) return rc;
     for (int64_t r = 0; r < full; r++) QH_HIP(hipGraphLaunch(g_st.exec, s));
   }
   // remainder: eager launches
@@ -1504,6 +1518,25 @@ extern "C" int qh_rk4(qh_mat *A, qh_vec *x, double dt, int64_t nsteps, int64_t t
 
 extern "C" int qh_rk4_timed(qh_mat *A, qh_vec *x, double dt, int64_t nsteps, int64_t tot, int renorm_every) {
   return rk4_impl(A, x, dt, nsteps, tot, renorm_every, 1);
+}
+
+// the set-up of a qh_rk4 call without its steps: the SpMV variant chosen (the
+// autotuner runs here on a first call), the stage buffers allocated and the
+// step graph captured and instantiated, so that a later qh_rk4 call with the
+// same arguments only replays it
+extern "C" int qh_rk4_prepare(qh_mat *A, qh_vec *x, double dt, int64_t tot, int renorm_every) {
+  int rc = qh_ensure_init();
+  if (rc) return rc;
+  QH_REQUIRE(A && x && x->n == A->rows, QH_EINVAL, "qh_rk4_prepare: vector size != matrix rows");
+  QH_REQUIRE(renorm_every == 0 || (x->prec == QH_FP32 && tot > 0 && tot * tot == x->n), QH_EINVAL,
+             "qh_rk4_prepare: trace renormalisation needs an FP32 density matrix of dimension tot");
+  if (A->rows <= QH_SMALL_ROWS && !A->lanes_forced) return QH_OK;  // one-workgroup path: nothing to capture
+  int G = 0;
+  rc = lanes_for(A, x->prec, &G);
+  if (!rc) rc = stepper_prepare(A, x->prec, x->n);
+  if (!rc) rc = graph_prepare(A, x, dt, tot, renorm_every, G, graph_pair_steps(renorm_every));
+  if (!rc) QH_HIP(hipStreamSynchronize(qh_stream()));
+  return rc;
 }
 
 extern "C" int qh_mat_set_lanes(qh_mat *m, int l64, int l32) {
@@ -1642,12 +1675,6 @@ __global__ void k_lincomb(double2 *y, LinArgs a, int64_t n) {
     y[i] = make_double2(re, im);
   }
 }
-__global__ void k_add_scaled(double2 *y, const double2 *w, double sr, double si, int64_t n) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    double2 a = y[i], b = w[i];
-    y[i] = make_double2(a.x + sr * b.x - si * b.y, a.y + sr * b.y + si * b.x);
-  }
-}
 __global__ void __launch_bounds__(256) k_rk3bs_err(const double2 *x, const double2 *xn, const double2 *k1,
                                                    const double2 *k2, const double2 *k3, const double2 *k4,
                                                    double h, double atol, double rtol, int64_t n, double *partial) {
@@ -1770,27 +1797,6 @@ extern "C" int qh_lincomb(qh_vec *y, int n, const double *c, qh_vec *const *v) {
   hipLaunchKernelGGL(k_lincomb, dim3(gridn(y->n)), dim3(256), 0, qh_stream(), (double2 *)y->d, a, y->n);
   QH_HIP(hipGetLastError());
   return QH_OK;
-}
-
-extern "C" int qh_spmv_td(const qh_mat *A, int ntd, qh_mat *const *Ak, const double *s, const qh_vec *x,
-                          qh_vec *y) {
-  QH_REQUIRE(x && y && x->prec == QH_FP64 && y->prec == QH_FP64, QH_EINVAL, "qh_spmv_td: FP64 only");
-  int rc = qh_spmv(A, x, y);
-  if (rc || ntd == 0) return rc;
-  qh_vec *w = nullptr;
-  rc = qh_vec_create(x->n, QH_FP64, &w);
-  if (rc) return rc;
-  for (int k = 0; k < ntd && !rc; k++) {
-    // only terms above min_time_dep (qsystem.c:34,1320)
-    if (sqrt(s[2 * k] * s[2 * k] + s[2 * k + 1] * s[2 * k + 1]) <= 1e-10) continue;
-    rc = qh_spmv(Ak[k], x, w);
-    if (!rc)
-      hipLaunchKernelGGL(k_add_scaled, dim3(gridn(x->n)), dim3(256), 0, qh_stream(), (double2 *)y->d,
-                         (const double2 *)w->d, s[2 * k], s[2 * k + 1], x->n);
-  }
-  qh_vec_destroy(w);
-  QH_HIP(hipGetLastError());
-  return rc;
 }
 
 extern "C" int qh_rk3bs_err(const qh_vec *x, const qh_vec *xn, const qh_vec *k1, const qh_vec *k2,

@@ -28,7 +28,8 @@ def _rand_rho(N):
 
 GATECASES = [("H", 1, (0,), 0.0), ("H", 1, (3,), 0.0), ("RZ", 8, (1,), 0.7), ("RX", 6, (2,), 1.3),
              ("SIGMAX", 2, (3,), 0.0), ("SIGMAY", 3, (0,), 0.0), ("CNOT", -1, (0, 1), 0.0),
-             ("CNOT", -1, (3, 1), 0.0), ("CZ", -3, (2, 0), 0.0), ("CXZ", -2, (1, 3), 0.0)]
+             ("CNOT", -1, (3, 1), 0.0), ("CZ", -3, (2, 0), 0.0), ("CXZ", -2, (1, 3), 0.0),
+             ("SIGMAZ", 4, (2,), 0.0), ("EYE", 5, (1,), 0.0), ("RY", 7, (3,), 2.1), ("CXZ", -2, (3, 0), 0.0)]
 
 
 @pytest.mark.parametrize("name,code,qubits,theta", GATECASES)
@@ -41,6 +42,27 @@ def test_gate_superop_convention_vs_oracle(name, code, qubits, theta):
     x.set(dense.vec(rho))
     qs.apply_gate(x, name, list(qubits), theta=theta)
     ref = oracle.gate_superop([2] * nq, [1, 2, 4, 8], True, code, qubits, dense.vec(rho), theta=theta)
+    assert np.abs(x.get() - ref).max() < 1e-12
+    qs.destroy()
+
+
+@pytest.mark.parametrize("gate", ["U1", "U2", "U3"])
+def test_u_gates_vs_oracle(gate):
+    """U1/U2/U3 take (theta, phi, lambda) into the OpenQASM U3 matrix
+    (U3_gate_func_wf, quantum_circuits.c:2055-2110) on both paths"""
+    nq, th, ph, la = 3, 1.1, -0.6, 2.3
+    code = {"U1": 9, "U2": 10, "U3": 11}[gate]
+    r = {"subs": [("op", 2)] * nq, "terms": [("lin", 0.0, [(0, "n")])], "mcwf": False}
+    qs, _, _ = build_system(r, construct=False)
+    rho = _rand_rho(2 ** nq)
+    x = qs.create_qvec()
+    x.set(dense.vec(rho))
+    qs.apply_gate(x, gate, [1], theta=th, phi=ph, lam=la)
+    ref = oracle.gate_superop([2] * nq, [1, 2, 4], True, code, (1,), dense.vec(rho), theta=th, phi=ph, lam=la)
+    assert np.abs(x.get() - ref).max() < 1e-12
+    x.set(dense.vec(rho))
+    x.apply_gate2(gate, [2], theta=th, phi=ph, lam=la)
+    ref = oracle.gate_inplace([2] * nq, True, code, (2,), dense.vec(rho), theta=th, phi=ph, lam=la)
     assert np.abs(x.get() - ref).max() < 1e-12
     qs.destroy()
 
@@ -115,6 +137,34 @@ def test_expectation_trace_populations_vs_oracle():
     assert abs(x.trace() - np.trace(rho)) < 1e-12
     assert abs(x.element(3, 5) - rho[3, 5]) < 1e-15
     assert np.abs(x.bitstring_probs() - np.real(np.diag(rho))).max() < 1e-15
+    qs.destroy()
+
+
+def test_wf_expectation_vs_oracle_and_known_answers():
+    """<psi|O|psi> on the device (qvec_utilities.c:1353-1420 for a WF) against the
+    oracle's qo_expect_wf (pinned by dense brute force) and the |1,1> known
+    answers of qvec_utilities_tests.c:232-278"""
+    r = {"subs": [("op", 2), ("op", 3), ("vec", 3)], "terms": [("ham", 0.0, [(0, "n")])], "mcwf": False}
+    qs, subs, op = build_system(r, construct=False)
+    tot = 18
+    psi = RNG.normal(size=tot) + 1j * RNG.normal(size=tot)
+    x = qs.create_wf()
+    x.set(psi)
+    for refs in ([(0, "lower"), (1, "raise")], [(1, "n")], [(0, "sy")], [(0, "sx"), (0, "sz")],
+                 [(2, ("vec", 1)), (2, ("vec", 2)), (1, "lower")]):
+        got = x.expectation(*[op(q) for q in refs])
+        _, terms = recipes.oracle_terms({"subs": r["subs"], "terms": [("ham", 1.0, refs)]})
+        want = oracle.expect_wf(tot, psi, terms[0][2])
+        assert abs(got - want) < 1e-12 * max(1.0, abs(want)), refs
+    qs.destroy()
+    r2 = {"subs": [("op", 2), ("op", 2)], "terms": [("ham", 0.0, [(0, "n")])], "mcwf": False}
+    qs, subs, op = build_system(r2, construct=False)
+    x = qs.create_wf()
+    x.add_loc(1.0, 3)  # |1,1>
+    assert x.expectation(op((0, "n"))) == 1.0
+    assert x.expectation(op((1, "n"))) == 1.0
+    assert x.expectation(op((0, "raise")), op((1, "lower"))) == 0.0
+    assert x.expectation(op((1, "raise")), op((0, "lower"))) == 0.0
     qs.destroy()
 
 

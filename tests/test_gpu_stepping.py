@@ -393,6 +393,40 @@ def test_rk3bs_step_fused_matches_separate_launches(name):
         k.destroy()
 
 
+@pytest.mark.parametrize("lanes", [8, 134])
+def test_rk3bs_step_matches_oracle_trial(lanes):
+    """qh_rk3bs_step (fused sliced-layout launches, and the CSR kernels) against
+    the oracle's qo_rk3bs_trial, which is pinned by the closed forms of one
+    Bogacki-Shampine step: k1..k4 and the candidate to 1e-12 relative, the
+    weighted error norm to 1e-10 relative, with and without a given k1"""
+    rng = np.random.default_rng(21)
+    r = recipes.spin_chain_recipe(5, seed=3)
+    qs, _, _ = build_system(r)
+    A = qs.matrix()
+    tot, terms = recipes.oracle_terms(r)
+    m = oracle.construct(tot, terms, dm=True)
+    n = A.info()[0]
+    x = rng.normal(size=n) + 1j * rng.normal(size=n)
+    h, atol, rtol = 0.08, 1e-6, 1e-5
+    A.set_lanes(lanes, 0)
+    xv = qa.DeviceVec(n)
+    xv.set(x)
+    v = {k: qa.DeviceVec(n) for k in ("xn", "k1", "k2", "k3", "k4", "y")}
+    for have_k1 in (0, 1):
+        err = ctypes.c_double()
+        qa._qh(qa.lib().qh_rk3bs_step(A.h, xv.h, v["xn"].h, v["k1"].h, v["k2"].h, v["k3"].h, v["k4"].h,
+                                      v["y"].h, h, atol, rtol, have_k1, ctypes.byref(err)))
+        xn, e_or, ks = m.rk3bs_trial(x, h, atol=atol, rtol=rtol)
+        got = {"xn": v["xn"].get(), "k1": v["k1"].get(), "k2": v["k2"].get(), "k3": v["k3"].get(),
+               "k4": v["k4"].get()}
+        want = {"xn": xn, "k1": ks[0], "k2": ks[1], "k3": ks[2], "k4": ks[3]}
+        for k in got:
+            assert np.abs(got[k] - want[k]).max() <= 1e-12 * np.abs(want[k]).max(), (lanes, have_k1, k)
+        assert abs(err.value - e_or) <= 1e-10 * e_or, (err.value, e_or)
+    A.set_lanes(0, 0)
+    qs.destroy()
+
+
 def test_spmv_matches_oracle():
     r = recipes.kerr_recipe(recipes.kerr_params(4, 1)[0], levels=4)
     tot, terms = recipes.oracle_terms(r)

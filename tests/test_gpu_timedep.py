@@ -1,0 +1,177 @@
+"""Time-dependent terms through the fused RHS kernel (qsystem.c:1288-1330):
+y = A x + sum_k s_k(t) A_k x in one launch per RK stage (qh_spmv_td,
+qh_rk4_td_step), against the oracle's qo_rk4_td (pinned by the analytic Rabi
+drive and by qo_rk4 on the summed matrix) and the closed form of
+time_dep_tests.c:142-213: FP64 to 1e-10, the FP32 state to 1e-6 on populations."""
+import cmath
+import ctypes
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+import quac_amd as qa
+import recipes
+from qbuild import populations_np
+
+pytestmark = pytest.mark.gpu
+
+F = [lambda t: math.cos(1.3 * t), lambda t: cmath.exp(0.4j * t) * math.sin(0.7 * t + 0.2),
+     lambda t: math.exp(-0.2 * t)]
+
+
+def _recipe(dm):
+    terms = [("ham", 0.4, [(1, "n")]), ("td_ham", 0.3 + 0.2j, [(0, "raise"), (1, "lower")]),
+             ("td_ham", 0.3 - 0.2j, [(0, "lower"), (1, "raise")])]
+    if dm:
+        terms += [("td_lin", 0.05, [(1, "lower")]), ("lin", 0.25, [(0, "lower")])]
+    return {"subs": [("op", 2), ("op", 3)], "terms": terms, "dm": dm}
+
+
+def _product(recipe, prec):
+    qa.set_option("ts_rk_type", "4")
+    qa.set_option("ts_adapt_type", "none")
+    qa.set_option("quac_precision", prec)
+    qs = qa.System()
+    q = qs.create_op(2)
+    c = qs.create_op(3)
+    op = {(0, "raise"): q.dag, (0, "lower"): q, (1, "raise"): c.dag, (1, "lower"): c, (1, "n"): c.n}
+    k = 0
+    for (kind, a, ops) in recipe["terms"]:
+        plist = [op[o] for o in ops]
+        if kind == "ham":
+            qs.add_ham_term(a, *plist)
+        elif kind == "lin":
+            qs.add_lin_term(a, *plist)
+        elif kind == "td_ham":
+            qs.add_ham_term_time_dep(a, F[k], *plist)
+            k += 1
+        else:
+            qs.add_lin_term_time_dep(a, F[k], *plist)
+            k += 1
+    qs.construct_matrix()
+    return qs, q, c
+
+
+def _oracle(recipe, x0, t0, dt, nsteps):
+    dm = recipe["dm"]
+    tot, terms = recipes.oracle_terms(recipe)
+    m = oracle.construct(tot, terms, dm=dm)
+    idx = [k for k, t in enumerate(terms) if t[0] < 0]
+    td = [oracle.construct(tot, terms, dm=dm, td_index=k) for k in idx]
+    a = [terms[k][1] for k in idx]
+    return m.rk4_td(td, a, lambda t, k: F[k](t), x0, t0, dt, nsteps, wf_scale=not dm), m, td, a
+
+
+@pytest.mark.parametrize("dm", [True, False])
+def test_td_rk4_fp64_vs_oracle(dm):
+    recipe = _recipe(dm)
+    qs, q, c = _product(recipe, "fp64")
+    try:
+        N = 6
+        x = qs.create_qvec() if dm else qs.create_wf()
+        rng = np.random.default_rng(3)
+        if dm:
+            a = rng.normal(size=(N, N)) + 1j * rng.normal(size=(N, N))
+            rho = a @ a.conj().T
+            x0 = (rho / np.trace(rho)).flatten(order="F")
+        else:
+            x0 = rng.normal(size=N) + 1j * rng.normal(size=N)
+            x0 /= np.linalg.norm(x0)
+        x.set(x0)
+        steps, tend = qs.time_step(x, 0.3, 2.3, 0.01, 10 ** 6)
+        assert steps == 200 and abs(tend - 2.3) < 1e-9
+        ref, _, _, _ = _oracle(recipe, x0, 0.3, 0.01, 200)
+        assert np.abs(x.get() - ref).max() < 1e-10
+    finally:
+        qs.destroy()
+
+
+def test_td_rk4_fp32_vs_oracle():
+    recipe = _recipe(True)
+    qa.set_option("quac_renorm", "20")
+    qs, q, c = _product(recipe, "fp32")
+    try:
+        x = qs.create_qvec()
+        x.add_loc(1.0, 3 * 6 + 3)  # |1,0><1,0|
+        qs.time_step(x, 0.0, 3.0, 0.01, 10 ** 6)
+        x0 = np.zeros(36, np.complex128)
+        x0[3 * 6 + 3] = 1.0
+        ref, _, _, _ = _oracle(recipe, x0, 0.0, 0.01, 300)
+        assert np.abs(populations_np(x.get(), recipe) - populations_np(ref, recipe)).max() < 1e-6
+        assert abs(x.trace() - 1.0) < 1e-6
+    finally:
+        qa.set_option("quac_precision", "fp64")
+        qa.set_option("quac_renorm", "0")
+        qs.destroy()
+
+
+def test_spmv_td_matches_dense_sum():
+    """qh_spmv_td against A + sum_k s_k A_k from the oracle's matrices; a
+    coefficient of magnitude <= 1e-10 is skipped (qsystem.c:34,1320)"""
+    recipe = _recipe(True)
+    qs, q, c = _product(recipe, "fp64")
+    try:
+        tot, terms = recipes.oracle_terms(recipe)
+        m = oracle.construct(tot, terms, dm=True)
+        idx = [k for k, t in enumerate(terms) if t[0] < 0]
+        tds = [oracle.construct(tot, terms, dm=True, td_index=k) for k in idx]
+
+        def dense(csr):
+            rp, col, val = csr.export()
+            D = np.zeros((csr.n, csr.n), np.complex128)
+            for r in range(csr.n):
+                D[r, col[rp[r]:rp[r + 1]]] += val[rp[r]:rp[r + 1]]
+            return D
+
+        A = dense(m)
+        n = A.shape[0]
+        rng = np.random.default_rng(8)
+        xh = rng.normal(size=n) + 1j * rng.normal(size=n)
+        s = np.array([0.7 - 0.3j, 1e-11, -1.2 + 0.5j])
+        want = A @ xh + sum(s[k] * (dense(tds[k]) @ xh) for k in (0, 2))
+        mats = (ctypes.c_void_p * 3)(*[qa.lib().quac_system_td_matrix(qs.h, k) for k in range(3)])
+        sc = np.ascontiguousarray(np.column_stack([s.real, s.imag]).ravel())
+        xv = qa.DeviceVec(n)
+        xv.set(xh)
+        yv = qa.DeviceVec(n)
+        L = qa.lib()
+        L.qh_spmv_td.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p),
+                                 ctypes.POINTER(ctypes.c_double), ctypes.c_void_p, ctypes.c_void_p]
+        L.qh_spmv_td.restype = ctypes.c_int
+        qa._qh(L.qh_spmv_td(qs.matrix().h, 3, mats, sc.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), xv.h, yv.h))
+        assert np.abs(yv.get() - want).max() < 1e-12 * np.abs(want).max()
+    finally:
+        qs.destroy()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_td_rabi_wavefunction_closed_form(prec):
+    """time_dep_tests.c:142-213: H(t) = 0.5 omega e^{-alpha t} (a + a^dag) on a
+    wavefunction from |0>, omega = 2 pi, alpha = 0.1, RK4 dt = 0.01 to t = 20:
+    <sz> = cos(omega (1 - e^{-alpha t})/alpha) (RK4's own error at dt = 0.01 is
+    below 1e-7; the reference allows 1e-4)"""
+    om, al = 2 * math.pi, 0.1
+    qa.set_option("ts_rk_type", "4")
+    qa.set_option("ts_adapt_type", "none")
+    qa.set_option("quac_precision", prec)
+    qs = qa.System()
+    try:
+        o = qs.create_op(2)
+        env = lambda t: math.exp(-al * t)
+        qs.add_ham_term_time_dep(0.5 * om, env, o)
+        qs.add_ham_term_time_dep(0.5 * om, env, o.dag)
+        qs.construct_matrix()
+        x = qs.create_wf()
+        x.add_loc(1.0, 0)
+        qs.time_step(x, 0.0, 20.0, 0.01, 10 ** 6)
+        psi = x.get()
+        th = om * (1 - math.exp(-al * 20.0)) / al
+        sz = abs(psi[0]) ** 2 - abs(psi[1]) ** 2
+        sy = 2 * (psi[0].conjugate() * psi[1]).imag
+        tol = 2e-7 if prec == "fp64" else 1e-5
+        assert abs(sz - math.cos(th)) < tol and abs(sy + math.sin(th)) < tol, (sz, sy)
+    finally:
+        qa.set_option("quac_precision", "fp64")
+        qs.destroy()
