@@ -252,12 +252,21 @@ int qh_ens_expect(const qh_vec *ens, int64_t M, int64_t tot, int nops, const qh_
 int qh_mcwf_fits(const qh_mat *A, int nj, int *fits);
 /* Trajectories of at most 1024 amplitudes (rows of A with at most 16 entries):
  * every trajectory i with jstep[i] < J takes classic RK4 steps (size dt; step
- * J-1 has size hlast) and stops after the first step at which its squared norm
- * is below thr[i] (the jump test, qsystem.c:1138-1153), or after step J-1.
- * On return jstep[i] is the number of steps trajectory i has taken and
- * jump[i] = 1 if it stopped for a jump (host arrays of M). */
-int qh_mcwf_advance(const qh_mat *A, qh_vec *ens, int64_t M, const double *thr, int32_t *jstep, int32_t J,
-                    double dt, double hlast, int32_t *jump);
+ * J-1 has size hlast; the first step of a call is shortened by tsub[i], the time
+ * already covered inside step jstep[i]) and stops at the first norm crossing:
+ * when its squared norm falls below thr[i] during a step (the jump test,
+ * qsystem.c:1138-1153), the crossing time inside the step is located (the norm
+ * of the RK4 step as a polynomial of the step size, the event location of
+ * qsystem.c:1017-1019) and the state is the RK4 step re-taken to that time.  On
+ * return jstep[i] / tsub[i] give where trajectory i stands (tsub[i] = 0 unless it
+ * stopped inside a step) and jump[i] = 1 if it stopped for a jump (host arrays of M). */
+int qh_mcwf_advance(const qh_mat *A, qh_vec *ens, int64_t M, const double *thr, int32_t *jstep, double *tsub,
+                    int32_t J, double dt, double hlast, int32_t *jump);
+/* The norm crossing of one trajectory inside an RK4 step of size h from psi0
+ * (any dimension): *tau = the first time in (0, h] at which the squared norm of
+ * the RK4 step falls to thr (h when it stays above), out = that step re-taken to
+ * *tau.  psi0 and out are distinct FP64 vectors (views of an ensemble allowed). */
+int qh_mcwf_cross(const qh_mat *A, const qh_vec *psi0, qh_vec *out, double h, double thr, double *tau);
 /* p[q*nj + k] = ||C_k psi_{idx[q]}||^2 for q < n, k < nj (C: nj <= 16 N x N
  * operators, N <= 1024; the jump probabilities of qsystem.c:1159-1200) */
 int qh_mcwf_jump_norms(int nj, qh_mat *const *C, const qh_vec *ens, int64_t N, const int64_t *idx, int32_t n,

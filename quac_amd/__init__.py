@@ -204,7 +204,7 @@ def _setup(L):
     sig("qh_ens_rk4", i32, vp, vp, i64, dbl, i64)
     sig("qh_ens_norms", i32, vp, i64, P(dbl))
     sig("qh_mcwf_fits", i32, vp, i32, P(i32))
-    sig("qh_mcwf_advance", i32, vp, vp, i64, P(dbl), P(i32), i32, dbl, dbl, P(i32))
+    sig("qh_mcwf_advance", i32, vp, vp, i64, P(dbl), P(i32), P(dbl), i32, dbl, dbl, P(i32))
     sig("qh_mcwf_jump_norms", i32, i32, P(vp), vp, i64, P(i64), i32, P(dbl))
     sig("qh_mcwf_jump_apply", i32, i32, P(vp), vp, i64, P(i64), i32, P(i32), P(dbl), P(dbl))
     sig("qh_rk4_timed", i32, vp, vp, dbl, i64, i64, i32)

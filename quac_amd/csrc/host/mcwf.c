@@ -1,15 +1,17 @@
 /* mcwf.c -- Monte Carlo wavefunction trajectories of a qsystem under
  * use_mcwf_solver (qsystem.c:358-403, 1024-1079, 1138-1235).
  *
- * All trajectories advance together on the device (qh_ens_rk4): one RK4 step of
- * d psi/dt = -i Heff psi for the whole ensemble, then the squared norms of all
- * trajectories are compared with their thresholds r_i.  A trajectory whose norm
- * fell below r_i jumps at the end of that step: operator k is chosen with
- * probability a_k ||C_k psi||^2 / sum_j a_j ||C_j psi||^2, psi <- sqrt(a_k) C_k psi /
- * sqrt(a_k ||C_k psi||^2), and a new threshold is drawn.  The reference locates
- * the crossing inside the step with PETSc's event handling; here the jump time
- * is resolved to the step (first order in dt), and every trajectory uses the
- * user's dt with classic RK4.
+ * All trajectories advance together on the device: RK4 steps of
+ * d psi/dt = -i Heff psi, and after every step the squared norm of each
+ * trajectory is compared with its threshold r_i.  When it fell below r_i during
+ * the step, the crossing time inside the step is located (the event location of
+ * qsystem.c:1017-1019): the RK4 step is a polynomial in its size on this linear
+ * system, so the squared norm is a degree-8 polynomial whose root is found to
+ * machine precision, and the state there is the step re-taken to that time.  The
+ * trajectory jumps there (operator k with probability a_k ||C_k psi||^2 /
+ * sum_j a_j ||C_j psi||^2, psi <- sqrt(a_k) C_k psi / sqrt(a_k ||C_k psi||^2), a new
+ * threshold drawn) and completes the rest of the step from the jumped state, so
+ * every trajectory stays on the user's time grid (classic RK4 at the user's dt).
  *
  * Random numbers: SPRNG (the reference's generator) is not available; uniforms
  * in [0, 1) come from xoshiro256** seeded through SplitMix64 with the seed given
@@ -182,15 +184,17 @@ static int fast_path(qsystem s, int64_t N) {
 }
 
 /* Trajectories of at most 1024 amplitudes without a monitor: each trajectory runs
- * on the device from one jump to the next (qh_mcwf_advance) and the jumps due
- * are applied together (qh_mcwf_jump_norms, choose_jump, qh_mcwf_jump_apply),
- * until every trajectory has taken all J steps.  The steps, the jump test and
+ * on the device from one jump to the next (qh_mcwf_advance, which stops it at
+ * the located crossing inside its step) and the jumps due are applied together
+ * (qh_mcwf_jump_norms, choose_jump, qh_mcwf_jump_apply), until every trajectory
+ * has taken all J steps.  The steps, the jump test and
  * the jump are those of the step-by-step loop below; the random numbers of the
  * jumps are drawn round by round in trajectory order, so a given seed yields
  * other (equally distributed) trajectories than the step-by-step loop. */
 static int run_to_end(qsystem s, Vec block, int64_t M, int64_t N, int32_t J, double dt, double hlast) {
   const int nj = (int)s->mcwf_njump;
   int32_t *jst = (int32_t *)calloc((size_t)M, sizeof(int32_t));
+  double *tsub = (double *)calloc((size_t)M, sizeof(double));
   int32_t *jmp = (int32_t *)malloc(sizeof(int32_t) * (size_t)M);
   int64_t *idx = (int64_t *)malloc(sizeof(int64_t) * (size_t)M);
   int32_t *kk = (int32_t *)malloc(sizeof(int32_t) * (size_t)M);
@@ -198,9 +202,9 @@ static int run_to_end(qsystem s, Vec block, int64_t M, int64_t N, int32_t J, dou
   double *sim = (double *)malloc(sizeof(double) * (size_t)M);
   double *n2 = (double *)malloc(sizeof(double) * (size_t)M * (size_t)nj);
   double *p = (double *)malloc(sizeof(double) * (size_t)nj);
-  int rc = (jst && jmp && idx && kk && sre && sim && n2 && p) ? 0 : quac_fail(9, "out of host memory");
+  int rc = (jst && tsub && jmp && idx && kk && sre && sim && n2 && p) ? 0 : quac_fail(9, "out of host memory");
   while (!rc) {
-    if (qh_mcwf_advance(s->mat_A, block, M, s->rand_number, jst, J, dt, hlast, jmp)) {
+    if (qh_mcwf_advance(s->mat_A, block, M, s->rand_number, jst, tsub, J, dt, hlast, jmp)) {
       rc = quac_fail(9, "MCWF steps failed: %s", qh_last_error());
       break;
     }
@@ -230,6 +234,7 @@ static int run_to_end(qsystem s, Vec block, int64_t M, int64_t N, int32_t J, dou
       rc = quac_fail(9, "MCWF jump failed: %s", qh_last_error());
   }
   free(jst);
+  free(tsub);
   free(jmp);
   free(idx);
   free(kk);
@@ -237,6 +242,39 @@ static int run_to_end(qsystem s, Vec block, int64_t M, int64_t N, int32_t J, dou
   free(sim);
   free(n2);
   free(p);
+  return rc;
+}
+
+/* trajectory i fell below its threshold during a step of size h that started
+ * from its state in pre: locate the crossing (qh_mcwf_cross), jump there, and
+ * take the rest of the step from the jumped state, again locating a crossing
+ * of the new threshold if one falls inside it */
+static int jump_in_step(qsystem s, Vec pre, Vec block, int64_t i, int64_t N, double h, Vec work) {
+  Vec p = NULL, c = NULL;
+  if (qh_vec_view(pre, i * N, N, &p) || qh_vec_view(block, i * N, N, &c)) {
+    qh_vec_destroy(p);
+    return quac_fail(9, "%s", qh_last_error());
+  }
+  double rem = h;
+  int rc = 0;
+  for (int pass = 0; pass < 1000 && !rc; pass++) {
+    double tau = rem;
+    if (qh_mcwf_cross(s->mat_A, p, c, rem, s->rand_number[i], &tau)) {
+      rc = quac_fail(9, "MCWF crossing failed: %s", qh_last_error());
+      break;
+    }
+    rc = jump(s, block, i, N, work);
+    rem -= tau;
+    if (rc || rem <= 1e-14 * h) break;
+    double n2 = 0.0;
+    if (qh_vec_copy(c, p) || qh_ens_rk4(s->mat_A, c, 1, rem, 1) || qh_ens_norms(c, 1, &n2)) {
+      rc = quac_fail(9, "MCWF step failed: %s", qh_last_error());
+      break;
+    }
+    if (n2 >= s->rand_number[i]) break;
+  }
+  qh_vec_destroy(p);
+  qh_vec_destroy(c);
   return rc;
 }
 
@@ -264,19 +302,25 @@ int quac_mcwf_time_step(qsystem s, qvec x, PetscReal t0, PetscReal tmax, PetscRe
     }
     if (step > 0) rc = run_to_end(s, block, M, N, (int32_t)step, dt, hlast);
   }
+  /* step by step: every trajectory takes the step; the states at the step's
+   * start are kept to locate the crossings of those that fell below threshold */
+  Vec pre = NULL;
+  if (!rc && t < tmax - eps && step < steps_max && qh_vec_create(M * N, QH_FP64, &pre))
+    rc = quac_fail(9, "%s", qh_last_error());
   while (!rc && t < tmax - eps && step < steps_max) {
     PetscReal h = dt;
     if (t + h > tmax - eps) h = tmax - t;
-    if (qh_ens_rk4(s->mat_A, block, M, h, 1) || qh_ens_norms(block, M, nrm)) {
+    if (qh_vec_copy(block, pre) || qh_ens_rk4(s->mat_A, block, M, h, 1) || qh_ens_norms(block, M, nrm)) {
       rc = quac_fail(9, "MCWF step failed: %s", qh_last_error());
       break;
     }
     for (int64_t i = 0; i < M && !rc; i++)
-      if (nrm[i] < s->rand_number[i]) rc = jump(s, block, i, N, work);
+      if (nrm[i] < s->rand_number[i]) rc = jump_in_step(s, pre, block, i, N, h, work);
     t += h;
     step++;
     if (!rc && monitor) rc = monitor(s, step, t, x);
   }
+  qh_vec_destroy(pre);
   free(nrm);
   qh_vec_destroy(work);
   s->current_time = t;

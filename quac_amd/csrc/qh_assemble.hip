@@ -28,6 +28,7 @@
 
 namespace {
 
+struct ChainG;
 struct Ctx {
   const qh_term *terms;
   int nterms;
@@ -35,6 +36,7 @@ struct Ctx {
   const qh_level *tab;
   uint32_t tot;
   int conj_a;
+  __device__ inline ChainG chain(int op0) const;
 };
 
 __device__ inline cxd cmul(cxd a, cxd b) { return cxd_make(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re); }
@@ -102,9 +104,20 @@ struct Contrib {
   cxd v;
 };
 
+// the chain rules of a context with plain division and tables in global memory
+struct ChainG {
+  const Ctx &c;
+  const qh_op *ops;
+  __device__ bool fwd(int n, uint32_t i, int mode, uint32_t &j, cxd &v) const { return chain_fwd(c, ops, n, i, mode, j, v); }
+  __device__ bool inv(int n, uint32_t r, int mode, uint32_t &i) const { return chain_inv(c, ops, n, r, mode, i); }
+};
+
+__device__ inline ChainG Ctx::chain(int op0) const { return ChainG{*this, ops + op0}; }
+
 // contributions of one term to target row r, in insertion order; returns count
-__device__ int term_contribs(const Ctx &c, const qh_term &t, uint32_t r, Contrib out[3]) {
-  const qh_op *ops = c.ops + t.op0;
+template <class CTX>
+__device__ int term_contribs(const CTX &c, const qh_term &t, uint32_t r, Contrib out[3]) {
+  const auto ch = c.chain(t.op0);
   const int n = t.nops;
   cxd a = t.zero_a ? cxd_make(0.0, 0.0) : cxd_make(t.a_re, t.a_im);
   const cxd I = cxd_make(0.0, 1.0);
@@ -113,32 +126,32 @@ __device__ int term_contribs(const Ctx &c, const qh_term &t, uint32_t r, Contrib
   cxd v;
   switch (t.kind) {
     case QH_HAM_ONLY:  // kron.c:638-657: A[i][j] += a*val
-      if (chain_fwd(c, ops, n, r, M_IG, j, v)) out[m++] = {(uint64_t)r * 4, j, cmul(a, v)};
+      if (ch.fwd(n, r, M_IG, j, v)) out[m++] = {(uint64_t)r * 4, j, cmul(a, v)};
       break;
     case QH_LIN_MCWF:  // kron.c:664-700: A[i][j] += -i*a/2*val (ops list already doubled)
-      if (chain_fwd(c, ops, n, r, M_IG, j, v)) {
+      if (ch.fwd(n, r, M_IG, j, v)) {
         cxd s = cmul(cxd_make(-0.0, -1.0), a);
         s = cxd_make(s.re / 2, s.im / 2);
         out[m++] = {(uint64_t)r * 4, j, cmul(s, v)};
       }
       break;
     case QH_HAM_DM:  // kron.c:702-729
-      if (chain_fwd(c, ops, n, r, M_IG, j, v)) {
+      if (ch.fwd(n, r, M_IG, j, v)) {
         cxd na = cxd_make(-a.re, -a.im);
         out[m++] = {(uint64_t)r * 4 + 0, j, cmul(cmul(na, I), v)};
       }
-      if (chain_inv(c, ops, n, r, M_GI, i) && chain_fwd(c, ops, n, i, M_GI, j, v) && j == r) {
+      if (ch.inv(n, r, M_GI, i) && ch.fwd(n, i, M_GI, j, v) && j == r) {
         cxd aa = c.conj_a ? cconj(a) : a;
         out[m++] = {(uint64_t)i * 4 + 1, i, cmul(cmul(aa, I), cconj(v))};
       }
       break;
     case QH_LIN_DM: {  // kron.c:995-1036
       cxd h = cxd_make(-0.5 * a.re, -0.5 * a.im);
-      if (chain_inv(c, ops, n, r, M_IG, i) && chain_fwd(c, ops, n, i, M_IG, j, v) && j == r)
+      if (ch.inv(n, r, M_IG, i) && ch.fwd(n, i, M_IG, j, v) && j == r)
         out[m++] = {(uint64_t)i * 4 + 0, r, cmul(cmul(h, cconj(v)), v)};
-      if (chain_inv(c, ops, n, r, M_GI, i) && chain_fwd(c, ops, n, i, M_GI, j, v) && j == r)
+      if (ch.inv(n, r, M_GI, i) && ch.fwd(n, i, M_GI, j, v) && j == r)
         out[m++] = {(uint64_t)i * 4 + 1, r, cmul(h, cconj(cmul(cconj(v), v)))};
-      if (chain_fwd(c, ops, n, r, M_GG, j, v)) out[m++] = {(uint64_t)r * 4 + 2, j, cmul(a, v)};
+      if (ch.fwd(n, r, M_GG, j, v)) out[m++] = {(uint64_t)r * 4 + 2, j, cmul(a, v)};
       break;
     }
     default:
@@ -217,6 +230,224 @@ __global__ void __launch_bounds__(256) k_compact(uint32_t rows, const int64_t *o
   }
 }
 
+// ------------------------------------------------------------ fast path
+// Division of a 32-bit index by a fixed divisor d with a precomputed multiplier
+// (Granlund & Montgomery 1994): with l = ceil(log2 d) and
+// m = floor(2^32 (2^l - d) / d) + 1, n / d = (mulhi(n, m) + n) >> l for every
+// n < 2^32 (the sum taken in 64 bits).
+struct FDiv {
+  uint32_t m, sh, d;
+};
+FDiv fdiv_make(uint32_t d) {
+  uint32_t sh = 0;
+  while (sh < 32 && ((uint64_t)1 << sh) < d) sh++;
+  const uint64_t m = (((uint64_t)1 << 32) * (((uint64_t)1 << sh) - d)) / d + 1;
+  return FDiv{(uint32_t)m, sh, d};
+}
+__device__ inline uint32_t fdiv(uint32_t n, const FDiv &f) {
+  return (uint32_t)(((uint64_t)__umulhi(n, f.m) + n) >> f.sh);
+}
+// (x / a) % L
+__device__ inline uint32_t fdigit(uint32_t x, const FDiv &a, const FDiv &L) {
+  const uint32_t q = fdiv(x, a);
+  return q - L.d * fdiv(q, L);
+}
+
+// an operator with its divisors: the stride, the stride times tot (the column
+// digit of the transposed part), and the level count
+struct OpF {
+  FDiv s, sg, L;
+  int32_t tab;
+  uint32_t pad[2];  // 48 bytes: the level table after the operators stays 16-byte aligned
+};
+static_assert(sizeof(OpF) == 48, "OpF size");
+
+struct ChainF;
+struct CtxF {
+  const qh_term *terms;
+  int nterms;
+  const OpF *ops;      // in LDS
+  const qh_level *tab; // in LDS
+  uint32_t tot;
+  FDiv ftot;
+  int conj_a;
+  __device__ inline ChainF chain(int op0) const;
+};
+
+// chain_fwd / chain_inv of the plain context, with the divisions replaced by
+// multiplier divisions and the tables read from LDS
+struct ChainF {
+  const CtxF &c;
+  const OpF *ops;
+  __device__ bool fwd(int n, uint32_t i, int mode, uint32_t &j, cxd &v) const {
+    uint32_t cur = i;
+    cxd val = cxd_make(1.0, 0.0);
+    for (int k = 0; k < n; k++) {
+      const OpF &op = ops[k];
+      const uint32_t s = op.s.d;
+      if (mode == M_GG) {
+        uint32_t i1 = fdiv(cur, c.ftot), i2 = cur - i1 * c.tot;
+        const uint32_t s1 = fdigit(i1, op.s, op.L), s2 = fdigit(i2, op.s, op.L);
+        const qh_level e1 = c.tab[op.tab + s1], e2 = c.tab[op.tab + s2];
+        if (e1.next < 0 || e2.next < 0) return false;
+        i1 = (uint32_t)((int64_t)i1 + ((int64_t)e1.next - s1) * s);
+        i2 = (uint32_t)((int64_t)i2 + ((int64_t)e2.next - s2) * s);
+        cur = i1 * c.tot + i2;
+        cxd t = cmul(cxd_make(e1.re, -e1.im), cxd_make(e2.re, e2.im));
+        val = cmul(t, val);
+      } else {
+        const FDiv &st = (mode == M_GI) ? op.sg : op.s;
+        const uint32_t sub = fdigit(cur, st, op.L);
+        const qh_level e = c.tab[op.tab + sub];
+        if (e.next < 0) return false;
+        cur = (uint32_t)((int64_t)cur + ((int64_t)e.next - sub) * st.d);
+        cxd ev = cxd_make(e.re, mode == M_GI ? -e.im : e.im);
+        val = cmul(ev, val);
+      }
+    }
+    j = cur;
+    v = val;
+    return true;
+  }
+  __device__ bool inv(int n, uint32_t r, int mode, uint32_t &i) const {
+    uint32_t cur = r;
+    for (int k = n - 1; k >= 0; k--) {
+      const OpF &op = ops[k];
+      const FDiv &st = (mode == M_GI) ? op.sg : op.s;
+      const uint32_t sub = fdigit(cur, st, op.L);
+      const int32_t p = c.tab[op.tab + sub].prev;
+      if (p < 0) return false;
+      cur = (uint32_t)((int64_t)cur + ((int64_t)p - sub) * st.d);
+    }
+    i = cur;
+    return true;
+  }
+};
+__device__ inline ChainF CtxF::chain(int op0) const { return ChainF{*this, ops + op0}; }
+
+// copy the operators and level tables into LDS; returns the context on them
+__device__ inline CtxF ctx_to_lds(const CtxF &g, const OpF *gops, int nops, const qh_level *gtab, int nlev) {
+  extern __shared__ __align__(16) char smem[];
+  OpF *sops = (OpF *)smem;
+  qh_level *stab = (qh_level *)(smem + sizeof(OpF) * nops);
+  for (int i = threadIdx.x; i < nops; i += blockDim.x) sops[i] = gops[i];
+  for (int i = threadIdx.x; i < nlev; i += blockDim.x) stab[i] = gtab[i];
+  __syncthreads();
+  CtxF c = g;
+  c.ops = sops;
+  c.tab = stab;
+  return c;
+}
+
+// Exact row lengths: the distinct columns of a row collected in KC registers
+// (fully unrolled compares, no indexed register access).  A row with more than
+// KC distinct columns sets *ovf (the caller then takes the general path);
+// *maxlen gets the longest row.
+template <int KC>
+__global__ void __launch_bounds__(256) k_count_exact(CtxF g, const OpF *gops, int nops, const qh_level *gtab,
+                                                     int nlev, uint32_t rows, int64_t *cnt, int *ovf, int *maxlen) {
+  const CtxF c = ctx_to_lds(g, gops, nops, gtab, nlev);
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  int n = 0;
+  bool over = false;
+  if (r < rows) {
+    uint32_t kc[KC];
+#pragma unroll
+    for (int q = 0; q < KC; q++) kc[q] = 0xFFFFFFFFu;
+    auto put = [&](uint32_t col) {
+      bool found = false;
+#pragma unroll
+      for (int q = 0; q < KC; q++) found |= (kc[q] == col);
+      if (found) return;
+      if (n == KC) {
+        over = true;
+        return;
+      }
+#pragma unroll
+      for (int q = 0; q < KC; q++)
+        if (q == n) kc[q] = col;
+      n++;
+    };
+    Contrib tmp[3];
+    for (int t = 0; t < c.nterms; t++) {
+      const int m = term_contribs(c, c.terms[t], r, tmp);
+      for (int x = 0; x < m; x++) put(tmp[x].col);
+    }
+    put(r);  // the explicit diagonal zero (qsystem.c:656-662)
+    cnt[r] = n;
+  }
+  if (over) atomicOr(ovf, 1);
+  // block maximum of the lengths, one atomic per block
+  __shared__ int bm[4];
+  int w = n;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) w = max(w, __shfl_xor(w, off, 64));
+  if ((threadIdx.x & 63) == 0) bm[threadIdx.x >> 6] = w;
+  __syncthreads();
+  if (threadIdx.x == 0) atomicMax(maxlen, max(max(bm[0], bm[1]), max(bm[2], bm[3])));
+}
+
+// The row with PETSc's insert/merge rule (a new column is set, an existing one
+// added to, in insertion order) in KC register slots, then written in column
+// order straight into the final CSR (-i scale and the complex64 copy applied).
+template <int KC>
+__global__ void __launch_bounds__(256) k_fill_exact(CtxF g, const OpF *gops, int nops, const qh_level *gtab,
+                                                    int nlev, uint32_t rows, const int64_t *start, int32_t *rowptr,
+                                                    int32_t *colout, double2 *valout, float2 *val32, int scale) {
+  const CtxF c = ctx_to_lds(g, gops, nops, gtab, nlev);
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= rows) return;
+  uint32_t kc[KC];
+  double vr[KC], vi[KC];
+#pragma unroll
+  for (int q = 0; q < KC; q++) {
+    kc[q] = 0xFFFFFFFFu;
+    vr[q] = vi[q] = 0.0;
+  }
+  int n = 0;
+  auto put = [&](uint32_t col, cxd v) {
+    bool found = false;
+#pragma unroll
+    for (int q = 0; q < KC; q++)
+      if (kc[q] == col) {
+        vr[q] = vr[q] + v.re;
+        vi[q] = vi[q] + v.im;
+        found = true;
+      }
+    if (found || n == KC) return;  // a longer row was excluded by the count pass
+#pragma unroll
+    for (int q = 0; q < KC; q++)
+      if (q == n) {
+        kc[q] = col;
+        vr[q] = v.re;
+        vi[q] = v.im;
+      }
+    n++;
+  };
+  Contrib tmp[3];
+  for (int t = 0; t < c.nterms; t++) {
+    const int m = term_contribs(c, c.terms[t], r, tmp);
+    for (int x = 0; x < m; x++) put(tmp[x].col, tmp[x].v);
+  }
+  put(r, cxd_make(0.0, 0.0));
+  const int64_t d = start[r];
+  rowptr[r] = (int32_t)d;
+  if (r == rows - 1) rowptr[rows] = (int32_t)(d + n);
+#pragma unroll
+  for (int q = 0; q < KC; q++) {
+    if (q < n) {
+      int rank = 0;
+#pragma unroll
+      for (int u = 0; u < KC; u++) rank += (kc[u] < kc[q]) ? 1 : 0;
+      cxd v = cxd_make(vr[q], vi[q]);
+      if (scale) v = cmul(cxd_make(-0.0, -1.0), v);  // MatScale(A,-i), qsystem.c:670-673
+      colout[d + rank] = (int32_t)kc[q];
+      valout[d + rank] = make_double2(v.re, v.im);
+      if (val32) val32[d + rank] = make_float2((float)v.re, (float)v.im);
+    }
+  }
+}
+
 int exclusive_scan(const int64_t *in, int64_t *out, int64_t n, int64_t *total) {
   void *tmp = nullptr;
   size_t tmpb = 0;
@@ -275,6 +506,8 @@ extern "C" int qh_assemble(int64_t tot, int64_t dim, int nterms, const qh_term *
   qh_mat *m = nullptr;
   int64_t total = 0, nnz = 0;
   Ctx c;
+  OpF *d_opf = nullptr;
+  int *d_flags = nullptr;
 
 #define QH_TRY(x)          \
   do {                     \
@@ -306,6 +539,69 @@ extern "C" int qh_assemble(int64_t tot, int64_t dim, int nterms, const qh_term *
 
   QH_TRY(qh_alloc_t(&cnt, rows));
   QH_TRY(qh_alloc_t(&off, rows));
+  {
+    // fast path: exact row lengths with the distinct columns in registers, rows
+    // written straight into the final CSR; taken when the operator tables fit
+    // in LDS and no row has more than 32 distinct columns
+    const size_t lds = sizeof(OpF) * (size_t)nops + sizeof(qh_level) * (size_t)nlev;
+    if (lds <= 60 * 1024) {
+      std::vector<OpF> hf(nops > 0 ? nops : 1);
+      for (int k = 0; k < nops; k++) {
+        hf[k].s = fdiv_make((uint32_t)ops[k].stride);
+        hf[k].sg = fdiv_make((uint32_t)(ops[k].stride * tot));
+        hf[k].L = fdiv_make((uint32_t)ops[k].levels);
+        hf[k].tab = ops[k].tab;
+        hf[k].pad[0] = hf[k].pad[1] = 0;
+      }
+      QH_TRY(qh_alloc_t(&d_opf, hf.size()));
+      QH_TRY(qh_alloc_t(&d_flags, 2));
+      QH_TRYHIP(hipMemcpyAsync(d_opf, hf.data(), sizeof(OpF) * hf.size(), hipMemcpyHostToDevice, s));
+      QH_TRYHIP(hipMemsetAsync(d_flags, 0, 2 * sizeof(int), s));
+      CtxF cf;
+      cf.terms = d_terms;
+      cf.nterms = nterms;
+      cf.ops = nullptr;
+      cf.tab = nullptr;
+      cf.tot = (uint32_t)tot;
+      cf.ftot = fdiv_make((uint32_t)tot);
+      cf.conj_a = c.conj_a;
+      hipLaunchKernelGGL(k_count_exact<32>, dim3(qh_blocks(rows, 256)), dim3(256), lds, s, cf, d_opf, nops, d_lev,
+                         nlev, rows, cnt, d_flags, d_flags + 1);
+      QH_TRYHIP(hipGetLastError());
+      int hf2[2] = {0, 0};
+      QH_TRYHIP(hipMemcpyAsync(hf2, d_flags, sizeof(hf2), hipMemcpyDeviceToHost, s));
+      QH_TRYHIP(hipStreamSynchronize(s));
+      if (!hf2[0]) {
+        QH_TRY(exclusive_scan(cnt, off, rows, &nnz));
+        if (nnz >= (int64_t)INT32_MAX) {
+          qh_set_error("qh_assemble: nnz %lld exceeds the 32-bit CSR range", (long long)nnz);
+          rc = QH_ERANGE;
+          goto cleanup;
+        }
+        m = new qh_mat();
+        m->rows = dim;
+        m->nnz = nnz;
+        QH_TRY(qh_alloc_t(&m->rowptr, (size_t)rows + 1));
+        QH_TRY(qh_alloc_t(&m->col, (size_t)nnz));
+        QH_TRY(qh_alloc_t(&m->val, (size_t)nnz));
+        if (flags & QH_ASM_FP32) QH_TRY(qh_alloc_t(&m->val32, (size_t)nnz));
+        const int sc = (flags & QH_ASM_SCALE_MINUS_I) ? 1 : 0;
+        const dim3 g(qh_blocks(rows, 256));
+#define QH_FILLX(KC)                                                                                              \
+  hipLaunchKernelGGL(k_fill_exact<KC>, g, dim3(256), lds, s, cf, d_opf, nops, d_lev, nlev, rows, off, m->rowptr, \
+                     m->col, m->val, m->val32, sc)
+        if (hf2[1] <= 8) QH_FILLX(8);
+        else if (hf2[1] <= 16) QH_FILLX(16);
+        else QH_FILLX(32);
+#undef QH_FILLX
+        QH_TRYHIP(hipGetLastError());
+        QH_TRYHIP(hipStreamSynchronize(s));
+        *out = m;
+        m = nullptr;
+        goto cleanup;
+      }
+    }
+  }
   hipLaunchKernelGGL(k_count, dim3(qh_blocks(rows, 256)), dim3(256), 0, s, c, rows, cnt);
   QH_TRYHIP(hipGetLastError());
   QH_TRY(exclusive_scan(cnt, off, rows, &total));
@@ -336,6 +632,8 @@ extern "C" int qh_assemble(int64_t tot, int64_t dim, int nterms, const qh_term *
   m = nullptr;
 cleanup:
   if (m) qh_mat_destroy(m);
+  qh_free_ptr(d_opf);
+  qh_free_ptr(d_flags);
   qh_free_ptr(d_terms);
   qh_free_ptr(d_ops);
   qh_free_ptr(d_lev);

@@ -156,12 +156,67 @@ __device__ inline double seg_sum(double v) {
   return v;
 }
 
+// Location of a norm crossing inside an RK4 step.  On the linear system
+// d psi/dt = A psi a classic RK4 step of size tau from psi0 is exactly
+// P(tau) = v0 + tau v1 + tau^2 v2 + tau^3 v3 + tau^4 v4 with v_k = A^k psi0 / k!,
+// so ||P(tau)||^2 = sum_m c_m tau^m with c_m = sum_{j+k=m} Re<v_j, v_k>, a
+// degree-8 polynomial.  The crossing ||P(tau)||^2 = thr (the event
+// norm - r = 0 of _sys_MCWF_EventFunction, qsystem.c:1138-1153, located by
+// PETSc's event handler to 1e-9 at qsystem.c:1017-1019) is found on (0, h] by
+// the Illinois variant of regula falsi, and the state there is P(tau): the RK4
+// step re-taken to the crossing.
+// lane partial of the nine c_m from the lane's parts of v0..v4
+__device__ inline void cross_partials(const double2 v[5], double c[9]) {
+#pragma unroll
+  for (int m = 0; m < 9; m++) c[m] = 0.0;
+#pragma unroll
+  for (int j = 0; j < 5; j++)
+#pragma unroll
+    for (int k = j; k < 5; k++) c[j + k] += (j == k ? 1.0 : 2.0) * (v[j].x * v[k].x + v[j].y * v[k].y);
+}
+__device__ inline double poly8(const double c[9], double t) {
+  double f = c[8];
+#pragma unroll
+  for (int m = 7; m >= 0; m--) f = f * t + c[m];
+  return f;
+}
+// the first tau in (0, h] with ||P(tau)||^2 <= thr (h when the polynomial stays
+// above thr by rounding although the stage arithmetic fell below it)
+__device__ inline double cross_tau(const double c[9], double thr, double h) {
+  double a = 0.0, fa = poly8(c, 0.0) - thr, b = h, fb = poly8(c, h) - thr;
+  if (fb > 0.0 || fa <= 0.0) return h;
+  int side = 0;
+  for (int it = 0; it < 200 && b - a > 1e-15 * h; it++) {
+    const double t = b - fb * (b - a) / (fb - fa);
+    const double ft = poly8(c, t) - thr;
+    if (ft > 0.0) {
+      a = t;
+      fa = ft;
+      if (side == 1) fb *= 0.5;
+      side = 1;
+    } else {
+      b = t;
+      fb = ft;
+      if (side == -1) fa *= 0.5;
+      side = -1;
+      if (ft == 0.0) break;
+    }
+  }
+  return b;
+}
+__device__ inline double2 horner4(const double2 v[5], double t) {
+  double2 r = v[4];
+#pragma unroll
+  for (int k = 3; k >= 0; k--) r = make_double2(r.x * t + v[k].x, r.y * t + v[k].y);
+  return r;
+}
+
 template <int S, int NZ>
 __global__ void __launch_bounds__(256) k_mcwf_run(int32_t N, int64_t M, const int32_t *__restrict__ rp,
                                                   const int32_t *__restrict__ col, const double2 *__restrict__ val,
                                                   double2 *__restrict__ ens, const double *__restrict__ thr,
-                                                  int32_t *__restrict__ jst, int32_t J, double dt, double hlast,
-                                                  int32_t *__restrict__ jump) {
+                                                  int32_t *__restrict__ jst, double *__restrict__ tsub, int32_t J,
+                                                  double dt, double hlast, int32_t *__restrict__ jump) {
   constexpr int P = ENS_W / S;  // trajectories per wavefront
   __shared__ double2 sh[4][ENS_W];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -170,6 +225,7 @@ __global__ void __launch_bounds__(256) k_mcwf_run(int32_t N, int64_t M, const in
   if (((int64_t)blockIdx.x * 4 + w) * P >= M) return;  // whole wavefront
   const bool have = i < M;
   int32_t j = have ? jst[i] : J;
+  double sub = have ? tsub[i] : 0.0;  // time already covered inside step j
   const bool on = have && r < N;
   int32_t c[NZ];
   double2 a[NZ];
@@ -199,7 +255,7 @@ __global__ void __launch_bounds__(256) k_mcwf_run(int32_t N, int64_t M, const in
   bool act = j < J;
   int32_t jumped = 0;
   while (__ballot(act)) {
-    const double h = (j == J - 1) ? hlast : dt;
+    const double h = ((j == J - 1) ? hlast : dt) - sub;
     const double2 k1 = apply(psi);
     const double2 k2 = apply(make_double2(psi.x + 0.5 * h * k1.x, psi.y + 0.5 * h * k1.y));
     const double2 k3 = apply(make_double2(psi.x + 0.5 * h * k2.x, psi.y + 0.5 * h * k2.y));
@@ -208,20 +264,51 @@ __global__ void __launch_bounds__(256) k_mcwf_run(int32_t N, int64_t M, const in
     const double2 nx = make_double2(psi.x + s6 * k1.x + s3 * k2.x + s3 * k3.x + s6 * k4.x,
                                     psi.y + s6 * k1.y + s3 * k2.y + s3 * k3.y + s6 * k4.y);
     const double n2 = seg_sum<S>(on ? nx.x * nx.x + nx.y * nx.y : 0.0);
+    const bool crossed = act && n2 < t;
+    double2 pc = nx;
+    double tau = h;
+    if (__ballot(crossed)) {  // wave-uniform: every segment takes the same products
+      double2 v[5];
+      v[0] = psi;
+      v[1] = k1;
+      const double2 a2 = apply(k1);
+      v[2] = make_double2(0.5 * a2.x, 0.5 * a2.y);
+      const double2 a3 = apply(v[2]);
+      v[3] = make_double2(a3.x * (1.0 / 3.0), a3.y * (1.0 / 3.0));
+      const double2 a4 = apply(v[3]);
+      v[4] = make_double2(0.25 * a4.x, 0.25 * a4.y);
+      if (!on) v[0] = v[1] = v[2] = v[3] = v[4] = make_double2(0.0, 0.0);
+      double cm[9];
+      cross_partials(v, cm);
+#pragma unroll
+      for (int m = 0; m < 9; m++) cm[m] = seg_sum<S>(cm[m]);
+      tau = crossed ? cross_tau(cm, t, h) : h;
+      pc = horner4(v, tau);
+    }
     if (act) {
-      psi = nx;
-      j++;
-      if (n2 < t) {
+      if (crossed) {
         jumped = 1;
         act = false;
-      } else if (j >= J) {
-        act = false;
+        if (tau < h * (1.0 - 1e-14)) {  // the jump falls inside the step: resume there
+          psi = pc;
+          sub += tau;
+        } else {
+          psi = nx;
+          j++;
+          sub = 0.0;
+        }
+      } else {
+        psi = nx;
+        j++;
+        sub = 0.0;
+        if (j >= J) act = false;
       }
     }
   }
   if (on) ens[i * N + r] = psi;
   if (have && r == 0) {
     jst[i] = j;
+    tsub[i] = sub;
     jump[i] = jumped;
   }
 }
@@ -297,13 +384,14 @@ __global__ void __launch_bounds__(LB) k_mcwf_run_wg(int32_t N, const int32_t *__
                                                      const int32_t *__restrict__ col,
                                                      const double2 *__restrict__ val, double2 *__restrict__ ens,
                                                      const double *__restrict__ thr, int32_t *__restrict__ jst,
-                                                     int32_t J, double dt, double hlast,
+                                                     double *__restrict__ tsub, int32_t J, double dt, double hlast,
                                                      int32_t *__restrict__ jump) {
   __shared__ double2 sh[ENS_WMAX];
-  __shared__ double red[ENS_WMAX / 64];
+  __shared__ double red[ENS_WMAX / 64][9];
   const int64_t i = blockIdx.x;
   const int r = threadIdx.x, w = r >> 6, nw = blockDim.x >> 6;
   int32_t j = jst[i];
+  double sub = tsub[i];
   if (j >= J) {  // the same in every thread of the workgroup
     if (r == 0) jump[i] = 0;
     return;
@@ -330,33 +418,138 @@ __global__ void __launch_bounds__(LB) k_mcwf_run_wg(int32_t N, const int32_t *__
     __syncthreads();
     return ens_row_product<NZ>(c, a, sh);
   };
+  // the block sum of nv values per thread, the same order (hence the same
+  // value) in every thread
+  auto block_sum = [&](double *v, int nv) {
+#pragma unroll
+    for (int m = 0; m < 9; m++)
+      if (m < nv) {
+        const double part = seg_sum<64>(v[m]);
+        if ((r & 63) == 0) red[w][m] = part;
+      }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < 9; m++)
+      if (m < nv) {
+        double acc = 0.0;
+        for (int k = 0; k < nw; k++) acc += red[k][m];
+        v[m] = acc;
+      }
+    __syncthreads();  // red is rewritten by the next sum
+  };
   int32_t jumped = 0;
   while (true) {
-    const double h = (j == J - 1) ? hlast : dt;
+    const double h = ((j == J - 1) ? hlast : dt) - sub;
     const double2 k1 = apply(psi);
     const double2 k2 = apply(make_double2(psi.x + 0.5 * h * k1.x, psi.y + 0.5 * h * k1.y));
     const double2 k3 = apply(make_double2(psi.x + 0.5 * h * k2.x, psi.y + 0.5 * h * k2.y));
     const double2 k4 = apply(make_double2(psi.x + h * k3.x, psi.y + h * k3.y));
     const double s6 = h * (1.0 / 6.0), s3 = h * (1.0 / 3.0);
-    psi = make_double2(psi.x + s6 * k1.x + s3 * k2.x + s3 * k3.x + s6 * k4.x,
-                       psi.y + s6 * k1.y + s3 * k2.y + s3 * k3.y + s6 * k4.y);
-    j++;
-    const double part = seg_sum<64>(on ? psi.x * psi.x + psi.y * psi.y : 0.0);
-    if ((r & 63) == 0) red[w] = part;
-    __syncthreads();
-    double n2 = 0.0;
-    for (int k = 0; k < nw; k++) n2 += red[k];
-    __syncthreads();  // red is rewritten after the next step
-    if (n2 < t) {
+    const double2 nx = make_double2(psi.x + s6 * k1.x + s3 * k2.x + s3 * k3.x + s6 * k4.x,
+                                    psi.y + s6 * k1.y + s3 * k2.y + s3 * k3.y + s6 * k4.y);
+    double n2[9];
+    n2[0] = on ? nx.x * nx.x + nx.y * nx.y : 0.0;
+    block_sum(n2, 1);
+    if (n2[0] < t) {
+      // the crossing inside the step (cross_tau), the same in every thread
+      double2 v[5];
+      v[0] = psi;
+      v[1] = k1;
+      const double2 a2 = apply(k1);
+      v[2] = make_double2(0.5 * a2.x, 0.5 * a2.y);
+      const double2 a3 = apply(v[2]);
+      v[3] = make_double2(a3.x * (1.0 / 3.0), a3.y * (1.0 / 3.0));
+      const double2 a4 = apply(v[3]);
+      v[4] = make_double2(0.25 * a4.x, 0.25 * a4.y);
+      if (!on) v[0] = v[1] = v[2] = v[3] = v[4] = make_double2(0.0, 0.0);
+      double cm[9];
+      cross_partials(v, cm);
+      block_sum(cm, 9);
+      const double tau = cross_tau(cm, t, h);
       jumped = 1;
+      if (tau < h * (1.0 - 1e-14)) {
+        psi = horner4(v, tau);
+        sub += tau;
+      } else {
+        psi = nx;
+        j++;
+        sub = 0.0;
+      }
       break;
     }
+    psi = nx;
+    j++;
+    sub = 0.0;
     if (j >= J) break;
   }
   if (on) ens[i * N + r] = psi;
   if (r == 0) {
     jst[i] = j;
+    tsub[i] = sub;
     jump[i] = jumped;
+  }
+}
+
+// The crossing of one trajectory of any size (the step-by-step loop): one
+// workgroup forms v1..v4 in the global scratch v (4 N entries), the nine c_m,
+// tau = cross_tau on thread 0, and out = P(tau).
+__global__ void __launch_bounds__(1024) k_mcwf_cross(int32_t N, const int32_t *__restrict__ rp,
+                                                    const int32_t *__restrict__ col,
+                                                    const double2 *__restrict__ val, const double2 *psi0,
+                                                    double2 *out, double2 *v, double h, double thr,
+                                                    double *tau_out) {
+  __shared__ double red[16][9];
+  __shared__ double tau_s;
+  const int tid = threadIdx.x, w = tid >> 6, nw = blockDim.x >> 6;
+  const double inv[4] = {1.0, 0.5, 1.0 / 3.0, 0.25};
+  for (int k = 0; k < 4; k++) {
+    const double2 *src = k == 0 ? psi0 : v + (int64_t)(k - 1) * N;
+    double2 *dst = v + (int64_t)k * N;
+    for (int32_t r = tid; r < N; r += blockDim.x) {
+      double re = 0.0, im = 0.0;
+      for (int32_t e = rp[r]; e < rp[r + 1]; e++) {
+        const double2 a = val[e], x = src[col[e]];
+        re = fma(a.x, x.x, fma(-a.y, x.y, re));
+        im = fma(a.x, x.y, fma(a.y, x.x, im));
+      }
+      dst[r] = make_double2(inv[k] * re, inv[k] * im);
+    }
+    __syncthreads();
+  }
+  double cm[9];
+#pragma unroll
+  for (int m = 0; m < 9; m++) cm[m] = 0.0;
+  for (int32_t r = tid; r < N; r += blockDim.x) {
+    double2 q[5];
+    q[0] = psi0[r];
+    for (int k = 0; k < 4; k++) q[k + 1] = v[(int64_t)k * N + r];
+    double p[9];
+    cross_partials(q, p);
+#pragma unroll
+    for (int m = 0; m < 9; m++) cm[m] += p[m];
+  }
+#pragma unroll
+  for (int m = 0; m < 9; m++) {
+    const double part = seg_sum<64>(cm[m]);
+    if ((tid & 63) == 0) red[w][m] = part;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double c[9];
+    for (int m = 0; m < 9; m++) {
+      c[m] = 0.0;
+      for (int k = 0; k < nw; k++) c[m] += red[k][m];
+    }
+    tau_s = cross_tau(c, thr, h);
+    tau_out[0] = tau_s;
+  }
+  __syncthreads();
+  const double tau = tau_s;
+  for (int32_t r = tid; r < N; r += blockDim.x) {
+    double2 q[5];
+    q[0] = psi0[r];
+    for (int k = 0; k < 4; k++) q[k + 1] = v[(int64_t)k * N + r];
+    out[r] = horner4(q, tau);
   }
 }
 
@@ -374,10 +567,10 @@ int jump_ops(int nj, qh_mat *const *C, int64_t N, JumpOps *ops) {
 }  // namespace
 
 extern "C" int qh_mcwf_advance(const qh_mat *A, qh_vec *ens, int64_t M, const double *thr, int32_t *jstep,
-                               int32_t J, double dt, double hlast, int32_t *jump) {
+                               double *tsub, int32_t J, double dt, double hlast, int32_t *jump) {
   int rc = qh_ensure_init();
   if (rc) return rc;
-  QH_REQUIRE(A && ens && thr && jstep && jump && ens->prec == QH_FP64 && M >= 1 && M <= (1 << 30) &&
+  QH_REQUIRE(A && ens && thr && jstep && tsub && jump && ens->prec == QH_FP64 && M >= 1 && M <= (1 << 30) &&
                  ens->n == M * A->rows && A->rows >= 1 && A->rows <= ENS_WMAX && J >= 0,
              QH_EINVAL, "qh_mcwf_advance: FP64 ensemble of M trajectories of at most 1024 amplitudes required");
   const int32_t N = (int32_t)A->rows;
@@ -385,16 +578,17 @@ extern "C" int qh_mcwf_advance(const qh_mat *A, qh_vec *ens, int64_t M, const do
   if (rc) return rc;
   QH_REQUIRE(A->max_row <= ENS_MAXNZ, QH_EINVAL, "qh_mcwf_advance: a row with more than %d entries", ENS_MAXNZ);
   hipStream_t s = qh_stream();
-  // device copies: thresholds, step indices, jump flags
+  // device copies: thresholds and covered step times, step indices, jump flags
   char *d = nullptr, *h = nullptr;
   const size_t bt = sizeof(double) * M, bj = sizeof(int32_t) * M;
-  rc = qh_scratch(bt + 2 * bj + 256, (void **)&d, bt + 2 * bj, (void **)&h);
+  rc = qh_scratch(2 * bt + 2 * bj + 256, (void **)&d, 2 * bt + 2 * bj, (void **)&h);
   if (rc) return rc;
-  double *dthr = (double *)d;
-  int32_t *djs = (int32_t *)(d + bt), *djump = (int32_t *)(d + bt + bj);
+  double *dthr = (double *)d, *dsub = (double *)(d + bt);
+  int32_t *djs = (int32_t *)(d + 2 * bt), *djump = (int32_t *)(d + 2 * bt + bj);
   memcpy(h, thr, bt);
-  memcpy(h + bt, jstep, bj);
-  QH_HIP(hipMemcpyAsync(d, h, bt + bj, hipMemcpyHostToDevice, s));
+  memcpy(h + bt, tsub, bt);
+  memcpy(h + 2 * bt, jstep, bj);
+  QH_HIP(hipMemcpyAsync(d, h, 2 * bt + bj, hipMemcpyHostToDevice, s));
   const int S = N <= 2 ? 2 : N <= 4 ? 4 : N <= 8 ? 8 : N <= 16 ? 16 : N <= 32 ? 32 : 64;
   const int64_t waves = (M + ENS_W / S - 1) / (ENS_W / S);
   const dim3 grid((unsigned)((waves + 3) / 4));
@@ -407,7 +601,7 @@ extern "C" int qh_mcwf_advance(const qh_mat *A, qh_vec *ens, int64_t M, const do
     // under the 1024-thread bound's 128)
 #define QH_MCWF_WG(NZ, LB)                                                                                     \
   hipLaunchKernelGGL((k_mcwf_run_wg<NZ, LB>), tg, wg, 0, s, N, A->rowptr, A->col, A->val, (double2 *)ens->d,   \
-                     dthr, djs, J, dt, hlast, djump)
+                     dthr, djs, dsub, J, dt, hlast, djump)
     const int nzw = A->max_row <= 4 ? 4 : A->max_row <= 8 ? 8 : ENS_MAXNZ;
     if (N <= 512) {
       if (nzw == 4) QH_MCWF_WG(4, 512);
@@ -422,7 +616,7 @@ extern "C" int qh_mcwf_advance(const qh_mat *A, qh_vec *ens, int64_t M, const do
   } else {
 #define QH_MCWF_RUN_NZ(SS, NZ)                                                                                   \
   hipLaunchKernelGGL((k_mcwf_run<SS, NZ>), grid, dim3(256), 0, s, N, M, A->rowptr, A->col, A->val,               \
-                     (double2 *)ens->d, dthr, djs, J, dt, hlast, djump)
+                     (double2 *)ens->d, dthr, djs, dsub, J, dt, hlast, djump)
 #define QH_MCWF_RUN(SS)                                                                                          \
   if (S == SS) {                                                                                                  \
     if (nzc == 2)                                                                                                 \
@@ -444,10 +638,11 @@ extern "C" int qh_mcwf_advance(const qh_mat *A, qh_vec *ens, int64_t M, const do
 #undef QH_MCWF_RUN_NZ
   }
   QH_HIP(hipGetLastError());
-  QH_HIP(hipMemcpyAsync(h + bt, djs, 2 * bj, hipMemcpyDeviceToHost, s));
+  QH_HIP(hipMemcpyAsync(h + bt, dsub, bt + 2 * bj, hipMemcpyDeviceToHost, s));
   QH_HIP(hipStreamSynchronize(s));
-  memcpy(jstep, h + bt, bj);
-  memcpy(jump, h + bt + bj, bj);
+  memcpy(tsub, h + bt, bt);
+  memcpy(jstep, h + 2 * bt, bj);
+  memcpy(jump, h + 2 * bt + bj, bj);
   return QH_OK;
 }
 
@@ -590,5 +785,28 @@ extern "C" int qh_ens_norms(const qh_vec *ens, int64_t M, double *norms2) {
   QH_HIP(hipGetLastError());
   QH_HIP(hipMemcpyAsync(norms2, g_ew.norms, sizeof(double) * M, hipMemcpyDeviceToHost, s));
   QH_HIP(hipStreamSynchronize(s));
+  return QH_OK;
+}
+
+extern "C" int qh_mcwf_cross(const qh_mat *A, const qh_vec *psi0, qh_vec *out, double h, double thr,
+                             double *tau) {
+  int rc = qh_ensure_init();
+  if (rc) return rc;
+  QH_REQUIRE(A && psi0 && out && tau && psi0->prec == QH_FP64 && out->prec == QH_FP64 && psi0->n == A->rows &&
+                 out->n == A->rows && psi0->d != out->d && h > 0.0 && A->rows < ((int64_t)1 << 31),
+             QH_EINVAL, "qh_mcwf_cross: FP64 state and output of the generator's dimension, h > 0");
+  const int64_t N = A->rows;
+  char *d = nullptr, *hp = nullptr;
+  rc = qh_scratch(sizeof(double2) * 4 * (size_t)N + 256, (void **)&d, sizeof(double), (void **)&hp);
+  if (rc) return rc;
+  double2 *v = (double2 *)(d + 256);
+  double *dtau = (double *)d;
+  hipStream_t s = qh_stream();
+  hipLaunchKernelGGL(k_mcwf_cross, dim3(1), dim3(1024), 0, s, (int32_t)N, A->rowptr, A->col, A->val,
+                     (const double2 *)psi0->d, (double2 *)out->d, v, h, thr, dtau);
+  QH_HIP(hipGetLastError());
+  QH_HIP(hipMemcpyAsync(hp, dtau, sizeof(double), hipMemcpyDeviceToHost, s));
+  QH_HIP(hipStreamSynchronize(s));
+  *tau = *(double *)hp;
   return QH_OK;
 }

@@ -119,3 +119,30 @@ def test_device_wf_and_mcwf_assembly():
             drp, dcol, dval = _device_csr(recipes.fixture_recipe(fam, 2, fl))
             assert np.array_equal(drp, rp) and np.array_equal(dcol, col)
             assert np.array_equal(dval.view(np.int64), val.view(np.int64))
+
+
+def test_device_assembly_long_rows_general_path():
+    """rows with more than 32 distinct columns (powers of the ladder operators of
+    a 40-level mode: up to 81 entries per row) take the general assembly path
+    (per-row insertion in memory); bit-exact against the oracle like the fast path"""
+    terms = [("ham", 0.1 * k + 0.05j, [(0, "raise")] * k) for k in range(1, 21)]
+    terms += [("ham", 0.1 * k - 0.05j, [(0, "lower")] * k) for k in range(1, 21)]
+    terms += [("lin", 0.3, [(0, "lower")])]
+    r = {"subs": [("op", 40)], "terms": terms, "mcwf": False}
+    tot, oterms = recipes.oracle_terms(r)
+    orp, ocol, oval = oracle.construct(tot, oterms, dm=True).export()
+    assert np.diff(orp).max() > 32
+    drp, dcol, dval = _device_csr(r)
+    assert np.array_equal(drp, orp) and np.array_equal(dcol, ocol)
+    assert np.array_equal(dval.view(np.int64), oval.view(np.int64))
+
+
+def test_device_assembly_noisy_circuit_12_structure():
+    """12 qubits with T1 decay and dephasing (superoperator 16 777 216 rows): the
+    diagonal plus the sigma-minus G* x G entry of every qubit whose row and column
+    digits are both 0, i.e. nnz = N^2 (1 + n/4) = 67 108 864, at most 13 per row"""
+    r = recipes.noisy_circuit_recipe(12)
+    qs, _, _ = build_system(r)
+    rows, nnz = qs.matrix().info()
+    assert rows == 4 ** 12 and nnz == 4 ** 12 * 4 == 67108864
+    qs.destroy()
