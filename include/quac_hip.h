@@ -160,6 +160,18 @@ int qh_expect_dm(const qh_vec *x, int64_t tot, int nops, const qh_op *ops, int n
                  const qh_level *levels, double *re, double *im);
 int qh_expect_wf(const qh_vec *x, int64_t tot, int nops, const qh_op *ops, int nlev,
                  const qh_level *levels, double *re, double *im);
+/*
+ * qh_expect_dm without allocation.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments and the
+ * bit-identical result of qh_expect_dm; it calls no hipMalloc/hipFree and keeps
+ * its op chain, level table and partial sums in a fixed 105.25 KiB device array
+ * per GPU (held until the process ends).  Limits: nops <= 64, nlev <= 4096
+ * (QH_ERANGE beyond).  Calls on one GPU share that array: calls that overlap
+ * (several host threads at once) race.
+ */
+int qh_expect_dm_noalloc(const qh_vec *x, int64_t tot, int nops, const qh_op *ops, int nlev,
+                         const qh_level *levels, double *re, double *im);
 /* trace of a DM (get_trace_qvec, qvec_utilities.c:378-392) */
 int qh_trace(const qh_vec *x, int64_t tot, double *re, double *im);
 /* <n_s> for every subsystem s at once: Re rho_ii weighted by level (populations) */

@@ -230,6 +230,7 @@ def _setup(L):
     sig("qh_trace", i32, vp, i64, P(dbl), P(dbl))
     sig("qh_expect_dm", i32, vp, i64, i32, P(QhOp), i32, P(QhLevel), P(dbl), P(dbl))
     sig("qh_expect_wf", i32, vp, i64, i32, P(QhOp), i32, P(QhLevel), P(dbl), P(dbl))
+    sig("qh_expect_dm_noalloc", i32, vp, i64, i32, P(QhOp), i32, P(QhLevel), P(dbl), P(dbl))
     sig("qh_populations", i32, vp, i32, P(i64), P(i64), i32, P(dbl))
     sig("qh_gate", i32, vp, i64, i32, i32, P(i64), P(dbl))
     sig("qh_gmres", i32, vp, i64, vp, i32, dbl, i64, i32, P(i64), P(dbl))

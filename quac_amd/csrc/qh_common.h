@@ -156,6 +156,9 @@ void qh_free_ptr(void *p);
 // dbytes and a pinned host buffer of at least hbytes (either pointer may be
 // null); valid until the next qh_scratch call, which may reallocate them
 int qh_scratch(size_t dbytes, void **d, size_t hbytes, void **h);
+// device address (on the current device) of the fixed scratch array of
+// qh_expect_dm_noalloc, resolved once per device (qh_obs.hip)
+int qh_noalloc_setup(char **out);
 
 template <typename T>
 static inline int qh_alloc_t(T **p, size_t count) {

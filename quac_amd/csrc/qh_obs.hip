@@ -2,6 +2,7 @@
 // the trace, and all subsystem populations in one pass over the diagonal.
 // qvec_utilities.c:322-392 (trace), :1197-1281 (DM expectation), :1682-2022
 // (bitstring probabilities = Re diag rho).
+#include <mutex>
 #include <vector>
 
 #include "qh_common.h"
@@ -387,4 +388,97 @@ extern "C" int qh_ens_expect(const qh_vec *ens, int64_t M, int64_t tot, int nops
     }
   }
   return rc;
+}
+
+// ------------------------------------------- expectation without allocation
+// qh_expect_dm_noalloc: the same reduction as qh_expect_dm for callers that may
+// not allocate device memory.  Its op chain, level table, partial sums and total
+// live in the file-scope array `scratch` below (each device holds its own copy
+// until the process ends).  Host code never passes `scratch` itself to a kernel
+// or copy: its address on the current device comes from hipGetSymbolAddress,
+// resolved once per device and kept in g_scr.  Nothing in the array has to start
+// at zero: each call writes the tables, the nb partial sums and the total it
+// reads.  Every call runs on the library stream and waits for it before it
+// returns, so calls from one host thread follow each other; calls that overlap
+// on one GPU (several host threads at once) race on the array.
+namespace {
+const int NA_MAX_OPS = 64;      // operators in the chain
+const int NA_MAX_LEV = 4096;    // entries of the level table
+const size_t NA_OPS_B = 1024;   // 64 x 16 B (qh_op), 256-aligned
+const size_t NA_LEV_B = 98304;  // 4096 x 24 B (qh_level), 256-aligned
+const size_t NA_PART_B = sizeof(double2) * NB;
+const size_t NA_BYTES = NA_OPS_B + NA_LEV_B + NA_PART_B + 256;
+static_assert(sizeof(qh_op) * NA_MAX_OPS <= NA_OPS_B, "op area");
+static_assert(sizeof(qh_level) * NA_MAX_LEV <= NA_LEV_B, "level area");
+const int NA_MAX_DEV = 64;
+
+__device__ __align__(256) unsigned char scratch[NA_BYTES];
+
+void *g_scr[NA_MAX_DEV];
+std::mutex g_scr_mu;
+
+// the partial sums in index order, as the host loop of finish2 adds them, so
+// that the total is bit-identical to qh_expect_dm's
+__global__ void k_sum_partials_seq(const double2 *part, int nb, double2 *out) {
+  double a = 0.0, b = 0.0;
+  for (int i = 0; i < nb; i++) {
+    a += part[i].x;
+    b += part[i].y;
+  }
+  out[0] = make_double2(a, b);
+}
+}  // namespace
+
+// device address of `scratch` on the current device (resolved on the first call
+// per device; qh_init makes that call for its device)
+int qh_noalloc_setup(char **out) {
+  int dev = 0;
+  QH_HIP(hipGetDevice(&dev));
+  QH_REQUIRE(dev >= 0 && dev < NA_MAX_DEV, QH_ENODEV, "qh_expect_dm_noalloc: device %d beyond %d", dev,
+             NA_MAX_DEV);
+  std::lock_guard<std::mutex> lock(g_scr_mu);
+  if (!g_scr[dev]) {
+    void *p = nullptr;
+    QH_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    QH_REQUIRE(p != nullptr, QH_EHIP, "hipGetSymbolAddress(scratch) returned no address on device %d", dev);
+    g_scr[dev] = p;
+  }
+  if (out) *out = (char *)g_scr[dev];
+  return QH_OK;
+}
+
+extern "C" int qh_expect_dm_noalloc(const qh_vec *x, int64_t tot, int nops, const qh_op *ops, int nlev,
+                                    const qh_level *levels, double *re, double *im) {
+  QH_REQUIRE(x && re && im && (nops == 0 || ops) && (nlev == 0 || levels), QH_EINVAL,
+             "qh_expect_dm_noalloc: null argument");
+  QH_REQUIRE(tot > 0 && tot * tot == x->n, QH_EINVAL, "qh_expect_dm_noalloc: vector size mismatch");
+  QH_REQUIRE(nops >= 0 && nops <= NA_MAX_OPS, QH_ERANGE,
+             "qh_expect_dm_noalloc: %d operators in the chain, at most %d", nops, NA_MAX_OPS);
+  QH_REQUIRE(nlev >= 0 && nlev <= NA_MAX_LEV, QH_ERANGE,
+             "qh_expect_dm_noalloc: %d level-table entries, at most %d", nlev, NA_MAX_LEV);
+  for (int k = 0; k < nops; k++)
+    QH_REQUIRE(ops[k].levels > 0 && ops[k].stride > 0 && ops[k].tab >= 0 && ops[k].tab + ops[k].levels <= nlev,
+               QH_EINVAL, "observable op %d out of range", k);
+  int rc = qh_ensure_init();
+  if (rc) return rc;
+  char *d = nullptr;
+  rc = qh_noalloc_setup(&d);
+  if (rc) return rc;
+  qh_op *dops = (qh_op *)d;
+  qh_level *dlev = (qh_level *)(d + NA_OPS_B);
+  double2 *part = (double2 *)(d + NA_OPS_B + NA_LEV_B);
+  double2 *total = (double2 *)(d + NA_OPS_B + NA_LEV_B + NA_PART_B);
+  hipStream_t s = qh_stream();
+  if (nops) QH_HIP(hipMemcpyAsync(dops, ops, sizeof(qh_op) * nops, hipMemcpyHostToDevice, s));
+  if (nlev) QH_HIP(hipMemcpyAsync(dlev, levels, sizeof(qh_level) * nlev, hipMemcpyHostToDevice, s));
+  const int nb = blocks_for(tot);
+  hipLaunchKernelGGL(k_expect_dm, dim3(nb), dim3(256), 0, s, (const void *)x->d, x->lo, tot, dops, nops, dlev, part);
+  hipLaunchKernelGGL(k_sum_partials_seq, dim3(1), dim3(1), 0, s, (const double2 *)part, nb, total);
+  QH_HIP(hipGetLastError());
+  double2 h;
+  QH_HIP(hipMemcpyAsync(&h, total, sizeof(double2), hipMemcpyDeviceToHost, s));
+  QH_HIP(hipStreamSynchronize(s));
+  *re = h.x;
+  *im = h.y;
+  return QH_OK;
 }

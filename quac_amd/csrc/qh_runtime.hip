@@ -44,7 +44,8 @@ extern "C" int qh_init(int device) {
   g_cus = prop.multiProcessorCount;
   g_device = device;
   g_init = 1;
-  return QH_OK;
+  // resolve the device address of the no-allocation entry point's scratch array
+  return qh_noalloc_setup(nullptr);
 }
 
 int qh_ensure_init() { return g_init ? QH_OK : qh_init(0); }
