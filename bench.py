@@ -70,9 +70,10 @@ def kernel_name(variant, real):
 
 def format_step_bytes(A, rows):
     """bytes the FP32 sliced layout of A has to move per RK4 step: 4 x (stored
-    values + slot metadata) + 17 vector passes of 8 B per row (4 gathered stage
-    inputs, 3 K and 3 Y writes, 3 x reads for the Y's, K1..K3 and x read and x'
-    written in the last epilogue); and the layout statistics"""
+    values + slot metadata) + 20 vector passes of 8 B per row (stages 1-3: the
+    gathered stage input, the state's high word for Y, the K and Y writes;
+    stage 4: the gathered input, the state's high and low words, K1..K3, and the
+    new high and low words written); and the layout statistics"""
     lanes = A.lanes()
     if lanes[1] in (140, 141, 142):  # row-pair layout: 40-byte slot metadata, values per lane pair
         layout = dict(A.pair_layout_info(), kind="row-pair (128 rows per wavefront)")
@@ -82,7 +83,7 @@ def format_step_bytes(A, rows):
         layout = dict(A.layout_info(), kind="one row per lane (64 rows per wavefront)")
         meta = layout["slots"] * 32 + 12 * layout["slices"]
         vals = layout["stored_vals"] * 8 + layout["stored_reals"] * 4
-    return 4 * (vals + meta) + 17 * rows * 8, layout
+    return 4 * (vals + meta) + 20 * rows * 8, layout
 
 
 def read_traffic(path):
@@ -120,7 +121,7 @@ def roofline_obj(kname, workload, fmt_bytes, launch_ms, ev_ms, traffic, traffic_
     out = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": kname, "workload": workload,
            "bytes_per_launch": fmt_bytes, "bytes_basis": "bytes the sliced format moves per launch (stored values "
-                                                         "+ slot metadata + 17/4 vector passes of 8 B per row)",
+                                                         "+ slot metadata + 20/4 vector passes of 8 B per row)",
            "launch_ms": round(launch_ms, 5), "launch_ms_per_kernel_events": round(ev_ms, 5),
            "csr_model": {"bytes_per_launch": csr_bytes,
                          "gbs": round(csr_bytes / (launch_ms * 1e-3) / 1e9, 1),
