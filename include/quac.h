@@ -276,6 +276,11 @@ enum { QUAC_SS_PC_NONE = 0, QUAC_SS_PC_JACOBI = 1, QUAC_SS_PC_HBASIS = 2 };
 int quac_assemble_heff(qsystem, Mat *out);
 void steady_state_sys_opts(qsystem, qvec, PetscInt restart, PetscReal tol, PetscInt maxit, PetscInt precond,
                            PetscInt *iters, PetscReal *residual);
+/* steady_state_sys_opts with the GMRES search space augmented by u, a predicted
+ * steady state (the learned warm start; no counterpart in the reference): each
+ * restart cycle minimises the residual over its Krylov space plus span(u). */
+void steady_state_sys_aug(qsystem, qvec x, qvec u, PetscInt restart, PetscReal tol, PetscInt maxit, PetscInt precond,
+                          PetscInt *iters, PetscReal *residual);
 
 /* ------------------------------------------------------ new in this library */
 enum { QUAC_ERRORS_EXIT = 0, QUAC_ERRORS_RETURN = 1 };

@@ -238,6 +238,13 @@ int qh_pc_destroy(qh_pc *pc);
 #define QH_PC_HBASIS 2
 int qh_gmres_pc(const qh_mat *A, int64_t tot, qh_vec *x, int restart, double tol, int64_t maxit,
                 int pc_kind, const qh_pc *pc, int64_t *iters, double *res);
+/* qh_gmres_pc with the search space of every cycle augmented by u, a predicted
+ * solution (augmented GMRES): the residual is minimised over x + M^-1 K_k + span(u),
+ * so a cycle needs at most the iterations of the plain one.  The steady-state
+ * solve seeded by the learned warm start (no counterpart in the reference, whose
+ * solve starts from zero, solver.c:143-198). */
+int qh_gmres_aug(const qh_mat *A, int64_t tot, qh_vec *x, const qh_vec *u, int restart, double tol, int64_t maxit,
+                 int pc_kind, const qh_pc *pc, int64_t *iters, double *res);
 /* eigenvalues w (ascending) and, if V != NULL, eigenvectors (column-major) of a
  * real symmetric n x n host matrix, on the device (rocSOLVER); used by the
  * density-matrix fidelity of get_fidelity_qvec (qvec_utilities.c:2339-2430) */

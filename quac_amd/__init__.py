@@ -156,6 +156,7 @@ def _setup(L):
     sig("apply_gate2_list", None, vp, i32, P(i64), dbl, dbl, dbl, vp)
     sig("apply_gate_sys_list", None, vp, vp, i32, P(i64), dbl, dbl, dbl, vp)
     sig("steady_state_sys_opts", None, vp, vp, i64, dbl, i64, i64, P(i64), P(dbl))
+    sig("steady_state_sys_aug", None, vp, vp, vp, i64, dbl, i64, i64, P(i64), P(dbl))
     sig("quac_system_matrix", vp, vp)
     sig("quac_system_td_matrix", vp, vp, i64)
     sig("quac_system_total_levels", i64, vp)
@@ -676,14 +677,20 @@ class System:
         lib().quac_ts_stats(ctypes.byref(s), ctypes.byref(t))
         return s.value, t.value
 
-    def steady_state(self, x, restart=100, tol=1e-11, maxit=10000, precond=None, jacobi=None):
+    def steady_state(self, x, restart=100, tol=1e-11, maxit=10000, precond=None, jacobi=None, aug=None):
         """GMRES steady state into x (x is the initial guess); precond: "hbasis"
-        (default), "jacobi" or "none"; jacobi=True/False is the older spelling"""
+        (default), "jacobi" or "none"; jacobi=True/False is the older spelling;
+        aug: a QVec holding a predicted steady state that augments every cycle's
+        search space (steady_state_sys_aug)"""
         if precond is None:
             precond = "hbasis" if jacobi is None else ("jacobi" if jacobi else "none")
         pc = {"none": 0, "jacobi": 1, "hbasis": 2}[precond]
         it, res = ctypes.c_int64(), ctypes.c_double()
-        lib().steady_state_sys_opts(self.h, x.h, restart, tol, maxit, pc, ctypes.byref(it), ctypes.byref(res))
+        if aug is None:
+            lib().steady_state_sys_opts(self.h, x.h, restart, tol, maxit, pc, ctypes.byref(it), ctypes.byref(res))
+        else:
+            lib().steady_state_sys_aug(self.h, x.h, aug.h, restart, tol, maxit, pc, ctypes.byref(it),
+                                       ctypes.byref(res))
         _check()
         return it.value, res.value
 

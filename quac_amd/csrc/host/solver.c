@@ -13,13 +13,16 @@
 
 #include "quac_internal.h"
 
-void steady_state_sys_opts(qsystem qsys, qvec x, PetscInt restart, PetscReal tol, PetscInt maxit, PetscInt precond,
-                           PetscInt *iters, PetscReal *residual) {
+static void solve(qsystem qsys, qvec x, qvec u, PetscInt restart, PetscReal tol, PetscInt maxit, PetscInt precond,
+                  PetscInt *iters, PetscReal *residual) {
   if (!qsys->mat_allocated) QUAC_FAIL(0, "Matrix does not seem to be constructed! You need to call construct_matrix!");
   if (!qsys->dm_equations || qsys->mcwf_solver)
     QUAC_FAIL(0, "Steady state is only supported for density-matrix (Lindblad) systems");
   if (x->my_type != DENSITY_MATRIX || x->n != qsys->dim) QUAC_FAIL(0, "steady_state_sys needs a density matrix");
   if (precond < QUAC_SS_PC_NONE || precond > QUAC_SS_PC_HBASIS) QUAC_FAIL(0, "unknown steady-state preconditioner");
+  if (u && (u->my_type != DENSITY_MATRIX || u->n != qsys->dim))
+    QUAC_FAIL(0, "steady_state_sys_aug: the predicted state must be a density matrix of the system");
+  Vec uv = u ? u->data : NULL;
   int64_t it = 0;
   double res = 0.0;
   int rc = 0;
@@ -31,17 +34,29 @@ void steady_state_sys_opts(qsystem qsys, qvec x, PetscInt restart, PetscReal tol
     if (quac_assemble_heff(qsys, &heff)) return;
     rc = qh_pc_create_hbasis(heff, &pc);
     qh_mat_destroy(heff);
-    if (!rc) rc = qh_gmres_pc(qsys->mat_A, qsys->total_levels, x->data, (int)restart, tol, maxit, QH_PC_HBASIS, pc,
-                              &it, &res);
+    if (!rc) rc = qh_gmres_aug(qsys->mat_A, qsys->total_levels, x->data, uv, (int)restart, tol, maxit, QH_PC_HBASIS,
+                               pc, &it, &res);
     qh_pc_destroy(pc);
   } else {
-    rc = qh_gmres_pc(qsys->mat_A, qsys->total_levels, x->data, (int)restart, tol, maxit, (int)precond, NULL, &it,
-                     &res);
+    rc = qh_gmres_aug(qsys->mat_A, qsys->total_levels, x->data, uv, (int)restart, tol, maxit, (int)precond, NULL, &it,
+                      &res);
   }
   if (rc) QUAC_FAIL(9, "steady state solve failed: %s", qh_last_error());
   if (iters) *iters = it;
   if (residual) *residual = res;
   if (quac_opt_int("quac_verbose", 0)) PetscPrintf(0, "Iterations %lld residual %e\n", (long long)it, res);
+}
+
+void steady_state_sys_opts(qsystem qsys, qvec x, PetscInt restart, PetscReal tol, PetscInt maxit, PetscInt precond,
+                           PetscInt *iters, PetscReal *residual) {
+  solve(qsys, x, NULL, restart, tol, maxit, precond, iters, residual);
+}
+
+/* the solve with its search space augmented by a predicted steady state u
+ * (qh_gmres_aug); x holds the initial guess as in steady_state_sys_opts */
+void steady_state_sys_aug(qsystem qsys, qvec x, qvec u, PetscInt restart, PetscReal tol, PetscInt maxit,
+                          PetscInt precond, PetscInt *iters, PetscReal *residual) {
+  solve(qsys, x, u, restart, tol, maxit, precond, iters, residual);
 }
 
 /* defaults: restart 100, rtol 1e-11, at most 10000 iterations (solver.c:13-15,

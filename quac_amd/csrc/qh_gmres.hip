@@ -187,8 +187,17 @@ int64_t qh_pc_dim(const qh_pc *pc);
 // Hamiltonian-eigenbasis operator).  Each cycle starts from the true residual
 // r = e0 - A'x of the caller's x, builds the Krylov basis of A' M^-1 from
 // r/||r||, and ends with x += M^-1 (V y).
-static int gmres_impl(const qh_mat *A, int64_t tot, qh_vec *x, int restart, double tol, int64_t maxit, int pck,
-                      const qh_pc *pc, int64_t *iters, double *res) {
+//
+// With an augmentation vector u (a predicted solution) every cycle minimises the
+// residual over x + M^-1 span(V_k) + span(u) instead (augmented GMRES, Saad 1997):
+// with c = A' u split as V_{k+1} g + c_perp (g built by orthogonalising c against
+// each new basis vector, L = ||c_perp||), and the Givens rotations of the
+// Hessenberg matrix applied to g as well, the residual of the best (y, alpha) is
+// |rho| L / sqrt(|gamma|^2 + L^2) <= |rho| (rho: the plain GMRES residual, gamma:
+// the rotated g's last entry), at alpha = conj(gamma) rho / (|gamma|^2 + L^2).  A
+// cycle therefore never needs more iterations than the plain one from the same x.
+static int gmres_impl(const qh_mat *A, int64_t tot, qh_vec *x, const qh_vec *aug, int restart, double tol,
+                      int64_t maxit, int pck, const qh_pc *pc, int64_t *iters, double *res) {
   int rc = qh_ensure_init();
   if (rc) return rc;
   QH_REQUIRE(A && x && x->prec == QH_FP64 && x->n == A->rows && tot * tot == A->rows, QH_EINVAL,
@@ -197,18 +206,22 @@ static int gmres_impl(const qh_mat *A, int64_t tot, qh_vec *x, int restart, doub
   QH_REQUIRE(pck >= 0 && pck <= 2, QH_EINVAL, "qh_gmres: preconditioner kind must be 0, 1 or 2");
   QH_REQUIRE(pck != 2 || (pc && qh_pc_dim(pc) == tot), QH_EINVAL,
              "qh_gmres: the Hamiltonian-eigenbasis preconditioner must match the Hilbert dimension");
+  QH_REQUIRE(!aug || (aug->prec == QH_FP64 && aug->n == A->rows), QH_EINVAL,
+             "qh_gmres: the augmentation vector must be an FP64 vector of the matrix size");
   const int64_t n = A->rows;
   const int R = restart;
   hipStream_t s = qh_stream();
   double2 *V = nullptr, *w = nullptr, *z = nullptr, *t = nullptr, *dinv = nullptr, *part = nullptr, *h = nullptr,
-          *yv = nullptr;
+          *yv = nullptr, *cu = nullptr, *crem = nullptr;
   double2 *xd = (double2 *)x->d;
   rc = qh_alloc_t(&V, (size_t)n * (R + 1));
   if (!rc) rc = qh_alloc_t(&w, n);
   if (!rc) rc = qh_alloc_t(&z, n);
   if (!rc) rc = qh_alloc_t(&t, n);
   if (!rc) rc = qh_alloc_t(&part, (size_t)DOT_BLOCKS * (R + 2));
-  if (!rc) rc = qh_alloc_t(&h, 2 * (R + 2));
+  if (!rc) rc = qh_alloc_t(&h, 2 * (R + 2) + 2);  // CGS2 dots, norms; g and L^2 of the augmentation
+  if (!rc && aug) rc = qh_alloc_t(&cu, n);
+  if (!rc && aug) rc = qh_alloc_t(&crem, n);
   if (!rc) rc = qh_alloc_t(&yv, R + 1);
   if (!rc && pck == 1) rc = qh_alloc_t(&dinv, n);
   if (rc) goto done;
@@ -250,10 +263,25 @@ static int gmres_impl(const qh_mat *A, int64_t tot, qh_vec *x, int restart, doub
       hipLaunchKernelGGL(k_reduce_rows, dim3(1), dim3(256), 0, s, part, DOT_BLOCKS, hout);
     };
 
+    // the augmentation's g_j = v_j^H c_rem and ||c_rem||^2 after removing v_j, into
+    // the two slots after the CGS2 ones
+    double2 *const hg = h + 2 * (R + 2);
+    auto aug_step = [&](const double2 *v) {
+      hipLaunchKernelGGL(k_multidot, dim3(DOT_BLOCKS, 1), dim3(256), 0, s, v, n, crem, n, part);
+      hipLaunchKernelGGL(k_reduce_rows, dim3(1), dim3(256), 0, s, part, DOT_BLOCKS, hg);
+      hipLaunchKernelGGL(k_multiaxpy, dim3(gr(n)), dim3(256), 0, s, v, n, 1, hg, crem, n);
+      norm2(crem, hg + 1);
+    };
+    if (aug) {
+      rc = apply_a((const double2 *)aug->d, cu);
+      if (rc) goto done;
+    }
+
     int64_t it = 0;
     double rn = 0.0;
-    std::vector<cd> H((size_t)(R + 1) * R), cs(R), sn(R), g(R + 1), y(R);
-    std::vector<double2> hh(2 * (R + 2));
+    std::vector<cd> H((size_t)(R + 1) * R), cs(R), sn(R), g(R + 1), y(R), Gp(R + 1);
+    std::vector<double2> hh(2 * (R + 2) + 2);
+    double L2 = 0.0;
     for (;;) {
       rc = apply_a(xd, w);
       if (rc) break;
@@ -267,7 +295,23 @@ static int gmres_impl(const qh_mat *A, int64_t tot, qh_vec *x, int restart, doub
       hipLaunchKernelGGL(k_normalize, dim3(gr(n)), dim3(256), 0, s, w, h, V, n);
       std::fill(H.begin(), H.end(), cd(0.0, 0.0));
       std::fill(g.begin(), g.end(), cd(0.0, 0.0));
+      std::fill(Gp.begin(), Gp.end(), cd(0.0, 0.0));
       g[0] = rn;
+      if (aug) {
+        QH_HIP(hipMemcpyAsync(crem, cu, 16 * n, hipMemcpyDeviceToDevice, s));
+        aug_step(V);
+        QH_HIP(hipMemcpyAsync(hh.data() + 2 * (R + 2), hg, sizeof(double2) * 2, hipMemcpyDeviceToHost, s));
+        QH_HIP(hipStreamSynchronize(s));
+        Gp[0] = cd(hh[2 * (R + 2)].x, hh[2 * (R + 2)].y);
+        L2 = hh[2 * (R + 2) + 1].x;
+      }
+      // the residual norm of the cycle's best combination so far
+      auto cycle_res = [&](int kk) {
+        const double rho = std::abs(g[kk]);
+        if (!aug) return rho;
+        const double den = std::norm(Gp[kk]) + L2;
+        return den > 0.0 ? rho * sqrt(L2 / den) : rho;
+      };
       int k = 0;
       for (; k < R && it < maxit; k++, it++) {
         rc = precond(V + n * k, z);
@@ -280,7 +324,8 @@ static int gmres_impl(const qh_mat *A, int64_t tot, qh_vec *x, int restart, doub
         hipLaunchKernelGGL(k_multiaxpy, dim3(gr(n)), dim3(256), 0, s, V, n, k + 1, h + (R + 2), w, n);
         norm2(w, h + (k + 1));
         hipLaunchKernelGGL(k_normalize, dim3(gr(n)), dim3(256), 0, s, w, h + (k + 1), V + n * (k + 1), n);
-        QH_HIP(hipMemcpyAsync(hh.data(), h, sizeof(double2) * 2 * (R + 2), hipMemcpyDeviceToHost, s));
+        if (aug) aug_step(V + n * (k + 1));
+        QH_HIP(hipMemcpyAsync(hh.data(), h, sizeof(double2) * (2 * (R + 2) + 2), hipMemcpyDeviceToHost, s));
         QH_HIP(hipStreamSynchronize(s));
         for (int jj = 0; jj <= k; jj++)
           H[(size_t)jj * R + k] = cd(hh[jj].x + hh[R + 2 + jj].x, hh[jj].y + hh[R + 2 + jj].y);
@@ -304,17 +349,37 @@ static int gmres_impl(const qh_mat *A, int64_t tot, qh_vec *x, int restart, doub
         g[k + 1] = -sn[k] * gk;
         H[(size_t)k * R + k] = std::conj(cs[k]) * a + std::conj(sn[k]) * b;
         H[(size_t)(k + 1) * R + k] = 0.0;
-        if (std::abs(g[k + 1]) <= tol) {
+        if (aug) {
+          // the new entry of g, then rotation k on its rows k, k+1
+          const cd gnew(hh[2 * (R + 2)].x, hh[2 * (R + 2)].y), gp = Gp[k];
+          Gp[k] = std::conj(cs[k]) * gp + std::conj(sn[k]) * gnew;
+          Gp[k + 1] = -sn[k] * gp + cs[k] * gnew;
+          L2 = hh[2 * (R + 2) + 1].x;
+        }
+        if (cycle_res(k + 1) <= tol) {
           k++;
           it++;
           break;
         }
       }
       if (rc) break;
+      // the augmentation's coefficient, and the right-hand side it leaves for y
+      cd alpha(0.0, 0.0);
+      if (aug) {
+        const double den = std::norm(Gp[k]) + L2;
+        if (den > 0.0) alpha = std::conj(Gp[k]) * g[k] / den;
+      }
       for (int i = k - 1; i >= 0; i--) {
-        cd sacc = g[i];
+        cd sacc = g[i] - Gp[i] * alpha;
         for (int jj = i + 1; jj < k; jj++) sacc -= H[(size_t)i * R + jj] * y[jj];
         y[i] = sacc / H[(size_t)i * R + i];
+      }
+      if (aug && alpha != cd(0.0, 0.0)) {
+        // x += alpha u
+        const double2 ah = make_double2(alpha.real(), alpha.imag());
+        QH_HIP(hipMemcpyAsync(yv, &ah, sizeof(double2), hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_update, dim3(gr(n)), dim3(256), 0, s, xd, (const double2 *)aug->d, n, 1, yv, n);
+        QH_HIP(hipStreamSynchronize(s));
       }
       if (k > 0) {
         // x += M^-1 (V y)
@@ -344,15 +409,22 @@ done:
   qh_free_ptr(part);
   qh_free_ptr(h);
   qh_free_ptr(yv);
+  qh_free_ptr(cu);
+  qh_free_ptr(crem);
   return rc;
 }
 
 extern "C" int qh_gmres(const qh_mat *A, int64_t tot, qh_vec *x, int restart, double tol, int64_t maxit, int jacobi,
                         int64_t *iters, double *res) {
-  return gmres_impl(A, tot, x, restart, tol, maxit, jacobi ? 1 : 0, nullptr, iters, res);
+  return gmres_impl(A, tot, x, nullptr, restart, tol, maxit, jacobi ? 1 : 0, nullptr, iters, res);
 }
 
 extern "C" int qh_gmres_pc(const qh_mat *A, int64_t tot, qh_vec *x, int restart, double tol, int64_t maxit,
                            int pc_kind, const qh_pc *pc, int64_t *iters, double *res) {
-  return gmres_impl(A, tot, x, restart, tol, maxit, pc_kind, pc, iters, res);
+  return gmres_impl(A, tot, x, nullptr, restart, tol, maxit, pc_kind, pc, iters, res);
+}
+
+extern "C" int qh_gmres_aug(const qh_mat *A, int64_t tot, qh_vec *x, const qh_vec *u, int restart, double tol,
+                            int64_t maxit, int pc_kind, const qh_pc *pc, int64_t *iters, double *res) {
+  return gmres_impl(A, tot, x, u, restart, tol, maxit, pc_kind, pc, iters, res);
 }

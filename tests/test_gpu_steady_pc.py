@@ -117,3 +117,28 @@ def test_warm_start_from_solution_returns_immediately():
     it2, res2 = qs.steady_state(x, tol=1e-10)
     assert it2 == 0 and res2 <= 1e-10
     qs.destroy()
+
+
+def test_augmented_gmres_with_exact_and_poor_predictions():
+    """qh_gmres_aug (steady_state_sys_aug): with the exact steady state as the
+    augmentation vector the first iteration already meets the tolerance; with a
+    poor prediction (a random Hermitian unit-trace state) the solve needs no more
+    iterations than the plain one, and both reach the same solution"""
+    r = recipes.kerr_recipe(recipes.kerr_sweep_params(9, 1)[0], levels=4)
+    qs, _, _ = build_system(r)
+    N = 64
+    x = qs.create_qvec()
+    it_c, res_c = qs.steady_state(x, restart=100, tol=1e-10, maxit=3000, precond="hbasis")
+    exact = x.get()
+    rng = np.random.default_rng(4)
+    a = rng.normal(size=(N, N)) + 1j * rng.normal(size=(N, N))
+    poor = (a @ a.conj().T)
+    poor = (poor / np.trace(poor)).reshape(-1, order="F")
+    for pred, bound in ((exact, 1), (poor, it_c)):
+        u = qs.create_qvec()
+        u.set(pred)
+        y = qs.create_qvec()
+        it_a, res_a = qs.steady_state(y, restart=100, tol=1e-10, maxit=3000, precond="hbasis", aug=u)
+        assert res_a <= 1e-10 and it_a <= bound, (it_a, bound)
+        assert np.abs(y.get() - exact).max() < 1e-8
+    qs.destroy()

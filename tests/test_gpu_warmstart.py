@@ -4,10 +4,11 @@ trained on the device, then held-out solves with and without the warm start.
 
 Checked: every seeded solve reaches the same residual as the plain solve (GMRES
 finishes it), the warm starts are Hermitian with unit trace, training lowers
-the loss, and the network's warm start is closer to the true steady state than
-the trivial guess (the mean of the training solutions).  The iteration counts
-of both runs are printed; the full-size comparison (8 levels, 2000 / 200 sets)
-is bench.py --workload kerr."""
+the loss, the network's warm start is closer to the true steady state than the
+trivial guess (the mean of the training solutions), and the solve whose search
+space is augmented by the network's prediction needs no more iterations than the
+plain solve on any held-out set and fewer on average.  The full-size
+comparison (8 levels, 2000 / 200 sets) is bench.py --workload kerr."""
 import numpy as np
 import pytest
 
@@ -22,12 +23,16 @@ L = 4
 N = L ** 3
 
 
-def _solve(p, x0=None, tol=1e-8):
+def _solve(p, x0=None, tol=1e-8, aug=None):
     qs, _, _ = build_system(recipes.kerr_recipe(p, levels=L))
     x = qs.create_qvec()
     if x0 is not None:
         x.set(x0)
-    it, res = qs.steady_state(x, restart=100, tol=tol, maxit=3000, precond="hbasis")
+    u = None
+    if aug is not None:
+        u = qs.create_qvec()
+        u.set(aug)
+    it, res = qs.steady_state(x, restart=100, tol=tol, maxit=3000, precond="hbasis", aug=u)
     sol = x.get()
     qs.destroy()
     return sol, it, res
@@ -46,22 +51,27 @@ def test_warm_start_pipeline_small():
     stats = ws.fit(np.array(feats), sols, epochs=400, batch=100, lr=3e-3)
     assert stats["losses"][-1][1] < stats["losses"][0][1] / 5
     trivial = hermitian_unit_trace(sols.mean(0), N)
-    cold, warm, e_net, e_triv = [], [], [], []
+    cold, warm, augm, e_net, e_triv = [], [], [], [], []
     for p in recipes.kerr_sweep_params(5, 24):
         sol, it_c, res_c = _solve(p)
         x0 = ws.predict([recipes.kerr_features(p)])[0]
         rho = x0.reshape(N, N, order="F")
         assert abs(np.trace(rho) - 1) < 1e-10 and np.abs(rho - rho.conj().T).max() < 1e-12
         _, it_w, res_w = _solve(p, x0=x0)
-        assert res_c <= 1e-8 and res_w <= 1e-8
+        sol_a, it_a, res_a = _solve(p, aug=x0)
+        assert res_c <= 1e-8 and res_w <= 1e-8 and res_a <= 1e-8
+        assert np.abs(sol_a - sol).max() < 1e-6
+        assert it_a <= it_c, (it_a, it_c)     # one restart cycle: never more than the plain solve
         cold.append(it_c)
         warm.append(it_w)
+        augm.append(it_a)
         n = np.linalg.norm(sol)
         e_net.append(np.linalg.norm(x0 - sol) / n)
         e_triv.append(np.linalg.norm(trivial - sol) / n)
     ws.destroy()
-    print(f"Kerr 3x{L}: cold {np.mean(cold):.1f} warm {np.mean(warm):.1f} iterations; "
-          f"warm-start error {np.mean(e_net):.3f} (trivial guess {np.mean(e_triv):.3f})")
+    print(f"Kerr 3x{L}: cold {np.mean(cold):.1f}, seeded {np.mean(warm):.1f}, augmented {np.mean(augm):.1f} "
+          f"iterations; warm-start error {np.mean(e_net):.3f} (trivial guess {np.mean(e_triv):.3f})")
+    assert np.mean(augm) < np.mean(cold)
     assert np.mean(e_net) < 0.95 * np.mean(e_triv)
 
 
