@@ -114,68 +114,82 @@ struct ChainG {
 
 __device__ inline ChainG Ctx::chain(int op0) const { return ChainG{*this, ops + op0}; }
 
-// contributions of one term to target row r, in insertion order; returns count
-template <class CTX>
-__device__ int term_contribs(const CTX &c, const qh_term &t, uint32_t r, Contrib out[3]) {
+// compare-exchange by key, field by field (a select of whole structs would go
+// through memory)
+__device__ inline void cswap(Contrib &a, Contrib &b) {
+  const bool sw = a.key > b.key;
+  const uint64_t k0 = sw ? b.key : a.key, k1 = sw ? a.key : b.key;
+  const uint32_t c0 = sw ? b.col : a.col, c1 = sw ? a.col : b.col;
+  const double r0 = sw ? b.v.re : a.v.re, r1 = sw ? a.v.re : b.v.re;
+  const double i0 = sw ? b.v.im : a.v.im, i1 = sw ? a.v.im : b.v.im;
+  a.key = k0;
+  a.col = c0;
+  a.v = cxd_make(r0, i0);
+  b.key = k1;
+  b.col = c1;
+  b.v = cxd_make(r1, i1);
+}
+
+// The contributions of one term to target row r, passed to emit(col, value) in
+// PETSc's insertion order (by iteration, then part): at most three, held in
+// three fixed slots and ordered by a three-element sorting network, so that
+// everything stays in registers.
+template <class CTX, class F>
+__device__ void term_emit(const CTX &c, const qh_term &t, uint32_t r, F &&emit) {
   const auto ch = c.chain(t.op0);
   const int n = t.nops;
   cxd a = t.zero_a ? cxd_make(0.0, 0.0) : cxd_make(t.a_re, t.a_im);
   const cxd I = cxd_make(0.0, 1.0);
-  int m = 0;
+  const uint64_t NONE = ~(uint64_t)0;
+  Contrib s0 = {NONE, 0u, cxd_make(0.0, 0.0)}, s1 = s0, s2 = s0;
   uint32_t j, i;
   cxd v;
   switch (t.kind) {
     case QH_HAM_ONLY:  // kron.c:638-657: A[i][j] += a*val
-      if (ch.fwd(n, r, M_IG, j, v)) out[m++] = {(uint64_t)r * 4, j, cmul(a, v)};
+      if (ch.fwd(n, r, M_IG, j, v)) s0 = {(uint64_t)r * 4, j, cmul(a, v)};
       break;
     case QH_LIN_MCWF:  // kron.c:664-700: A[i][j] += -i*a/2*val (ops list already doubled)
       if (ch.fwd(n, r, M_IG, j, v)) {
-        cxd s = cmul(cxd_make(-0.0, -1.0), a);
-        s = cxd_make(s.re / 2, s.im / 2);
-        out[m++] = {(uint64_t)r * 4, j, cmul(s, v)};
+        cxd sc = cmul(cxd_make(-0.0, -1.0), a);
+        sc = cxd_make(sc.re / 2, sc.im / 2);
+        s0 = {(uint64_t)r * 4, j, cmul(sc, v)};
       }
       break;
     case QH_HAM_DM:  // kron.c:702-729
       if (ch.fwd(n, r, M_IG, j, v)) {
         cxd na = cxd_make(-a.re, -a.im);
-        out[m++] = {(uint64_t)r * 4 + 0, j, cmul(cmul(na, I), v)};
+        s0 = {(uint64_t)r * 4 + 0, j, cmul(cmul(na, I), v)};
       }
       if (ch.inv(n, r, M_GI, i) && ch.fwd(n, i, M_GI, j, v) && j == r) {
         cxd aa = c.conj_a ? cconj(a) : a;
-        out[m++] = {(uint64_t)i * 4 + 1, i, cmul(cmul(aa, I), cconj(v))};
+        s1 = {(uint64_t)i * 4 + 1, i, cmul(cmul(aa, I), cconj(v))};
       }
       break;
     case QH_LIN_DM: {  // kron.c:995-1036
       cxd h = cxd_make(-0.5 * a.re, -0.5 * a.im);
       if (ch.inv(n, r, M_IG, i) && ch.fwd(n, i, M_IG, j, v) && j == r)
-        out[m++] = {(uint64_t)i * 4 + 0, r, cmul(cmul(h, cconj(v)), v)};
+        s0 = {(uint64_t)i * 4 + 0, r, cmul(cmul(h, cconj(v)), v)};
       if (ch.inv(n, r, M_GI, i) && ch.fwd(n, i, M_GI, j, v) && j == r)
-        out[m++] = {(uint64_t)i * 4 + 1, r, cmul(h, cconj(cmul(cconj(v), v)))};
-      if (ch.fwd(n, r, M_GG, j, v)) out[m++] = {(uint64_t)r * 4 + 2, j, cmul(a, v)};
+        s1 = {(uint64_t)i * 4 + 1, r, cmul(h, cconj(cmul(cconj(v), v)))};
+      if (ch.fwd(n, r, M_GG, j, v)) s2 = {(uint64_t)r * 4 + 2, j, cmul(a, v)};
       break;
     }
     default:
       break;
   }
-  // tiny insertion sort by key (at most 3 entries)
-  for (int x = 1; x < m; x++) {
-    Contrib y = out[x];
-    int z = x - 1;
-    while (z >= 0 && out[z].key > y.key) {
-      out[z + 1] = out[z];
-      z--;
-    }
-    out[z + 1] = y;
-  }
-  return m;
+  cswap(s0, s1);
+  cswap(s1, s2);
+  cswap(s0, s1);
+  if (s0.key != NONE) emit(s0.col, s0.v);
+  if (s1.key != NONE) emit(s1.col, s1.v);
+  if (s2.key != NONE) emit(s2.col, s2.v);
 }
 
 __global__ void __launch_bounds__(256) k_count(Ctx c, uint32_t rows, int64_t *cnt) {
   uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= rows) return;
   int64_t n = 1;  // the explicit diagonal zero (qsystem.c:656-662)
-  Contrib tmp[3];
-  for (int t = 0; t < c.nterms; t++) n += term_contribs(c, c.terms[t], r, tmp);
+  for (int t = 0; t < c.nterms; t++) term_emit(c, c.terms[t], r, [&](uint32_t, cxd) { n++; });
   cnt[r] = n;
 }
 
@@ -203,11 +217,8 @@ __global__ void __launch_bounds__(256) k_fill(Ctx c, uint32_t rows, const int64_
   uint32_t *col = tcol + off[r];
   cxd *val = tval + off[r];
   int len = 0;
-  Contrib tmp[3];
-  for (int t = 0; t < c.nterms; t++) {
-    int m = term_contribs(c, c.terms[t], r, tmp);
-    for (int x = 0; x < m; x++) seg_insert(col, val, len, tmp[x].col, tmp[x].v);
-  }
+  for (int t = 0; t < c.nterms; t++)
+    term_emit(c, c.terms[t], r, [&](uint32_t cc, cxd v) { seg_insert(col, val, len, cc, v); });
   seg_insert(col, val, len, r, cxd_make(0.0, 0.0));
   len_out[r] = len;
 }
@@ -368,11 +379,7 @@ __global__ void __launch_bounds__(256) k_count_exact(CtxF g, const OpF *gops, in
         if (q == n) kc[q] = col;
       n++;
     };
-    Contrib tmp[3];
-    for (int t = 0; t < c.nterms; t++) {
-      const int m = term_contribs(c, c.terms[t], r, tmp);
-      for (int x = 0; x < m; x++) put(tmp[x].col);
-    }
+    for (int t = 0; t < c.nterms; t++) term_emit(c, c.terms[t], r, [&](uint32_t cc, cxd) { put(cc); });
     put(r);  // the explicit diagonal zero (qsystem.c:656-662)
     cnt[r] = n;
   }
@@ -424,11 +431,7 @@ __global__ void __launch_bounds__(256) k_fill_exact(CtxF g, const OpF *gops, int
       }
     n++;
   };
-  Contrib tmp[3];
-  for (int t = 0; t < c.nterms; t++) {
-    const int m = term_contribs(c, c.terms[t], r, tmp);
-    for (int x = 0; x < m; x++) put(tmp[x].col, tmp[x].v);
-  }
+  for (int t = 0; t < c.nterms; t++) term_emit(c, c.terms[t], r, [&](uint32_t cc, cxd v) { put(cc, v); });
   put(r, cxd_make(0.0, 0.0));
   const int64_t d = start[r];
   rowptr[r] = (int32_t)d;
