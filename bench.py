@@ -367,6 +367,9 @@ def main():
     ap.add_argument("--test", type=int, default=200)
     ap.add_argument("--rank", type=int, default=128)
     ap.add_argument("--epochs", type=int, default=2000)
+    ap.add_argument("--local-train", type=int, default=500, help="kerr: training sets of the local sweep")
+    ap.add_argument("--local-test", type=int, default=50, help="kerr: held-out sets of the local sweep")
+    ap.add_argument("--local-rank", type=int, default=64, help="kerr: basis rank of the local sweep")
     ap.add_argument("--skip-sweep", action="store_true", help="only the 10-qubit chain leg (profiling runs)")
     ap.add_argument("--fp32-variant", type=int, default=0,
                     help="force the chain's FP32 stage-kernel variant (qh_mat_set_lanes code; 0 = autotune). "

@@ -23,13 +23,16 @@ __device__ inline int64_t ins0(int64_t t, int64_t s) { return (t / s) * (2 * s) 
 // emulated 64-bit division
 __device__ inline int64_t ins0p(int64_t t, int k) { return ((t >> k) << (k + 1)) | (t & ((1LL << k) - 1)); }
 
+// entry i of the state: complex128, or the FP32 hi + lo pair (F32)
+template <bool F32>
 __device__ inline double2 ldv(const void *d, const float2 *lo, int64_t i) {
-  if (!lo) return ((const double2 *)d)[i];
+  if (!F32) return ((const double2 *)d)[i];
   float2 h = ((const float2 *)d)[i], l = lo[i];
   return make_double2((double)h.x + (double)l.x, (double)h.y + (double)l.y);
 }
+template <bool F32>
 __device__ inline void stv(void *d, float2 *lo, int64_t i, double2 v) {
-  if (!lo) {
+  if (!F32) {
     ((double2 *)d)[i] = v;
     return;
   }
@@ -40,7 +43,8 @@ __device__ inline void stv(void *d, float2 *lo, int64_t i, double2 v) {
 
 // K = 2 (1 qubit) or 4 (2 qubits); DM: X' = U X U^dagger on a KxK block; WF: psi' = U psi.
 // P2: every inserted stride is a power of two, passed as its exponent in lg.
-template <int K, bool DM, bool P2>
+// F32: the state is the FP32 hi + lo pair (x the high words, lo the low words).
+template <int K, bool DM, bool P2, bool F32>
 __global__ void __launch_bounds__(256) k_gate(void *x, float2 *lo, int64_t nthreads, int nins, int64_t s0,
                                               int64_t s1, int64_t s2, int64_t s3, int4 lg, int64_t bs0, int64_t bs1,
                                               int64_t cs0, int64_t cs1, U4 u) {
@@ -71,7 +75,7 @@ __global__ void __launch_bounds__(256) k_gate(void *x, float2 *lo, int64_t nthre
   if (!DM) {
     double2 p[K];
 #pragma unroll
-    for (int a = 0; a < K; a++) p[a] = ldv(x, lo, base + ro[a]);
+    for (int a = 0; a < K; a++) p[a] = ldv<F32>(x, lo, base + ro[a]);
 #pragma unroll
     for (int a = 0; a < K; a++) {
       double r = 0.0, i = 0.0;
@@ -80,7 +84,7 @@ __global__ void __launch_bounds__(256) k_gate(void *x, float2 *lo, int64_t nthre
         r += u.re[a * K + c] * p[c].x - u.im[a * K + c] * p[c].y;
         i += u.re[a * K + c] * p[c].y + u.im[a * K + c] * p[c].x;
       }
-      stv(x, lo, base + ro[a], make_double2(r, i));
+      stv<F32>(x, lo, base + ro[a], make_double2(r, i));
     }
     return;
   }
@@ -89,11 +93,13 @@ __global__ void __launch_bounds__(256) k_gate(void *x, float2 *lo, int64_t nthre
 #pragma unroll
   for (int c = 0; c < K; c++)
 #pragma unroll
-    for (int d = 0; d < K; d++) X[c][d] = ldv(x, lo, base + ro[c] + co[d]);
-  // Y = U X
-  double2 Y[K][K];
+    for (int d = 0; d < K; d++) X[c][d] = ldv<F32>(x, lo, base + ro[c] + co[d]);
+  // row a of Y = U X, then row a of Z = Y U^dagger (Z[a][b] = sum_d Y[a][d]
+  // conj(U[b][d])) stored at once: one row of Y lives at a time, so the block
+  // and one row are the registers held
 #pragma unroll
-  for (int a = 0; a < K; a++)
+  for (int a = 0; a < K; a++) {
+    double2 Y[K];
 #pragma unroll
     for (int d = 0; d < K; d++) {
       double r = 0.0, i = 0.0;
@@ -102,22 +108,84 @@ __global__ void __launch_bounds__(256) k_gate(void *x, float2 *lo, int64_t nthre
         r += u.re[a * K + c] * X[c][d].x - u.im[a * K + c] * X[c][d].y;
         i += u.re[a * K + c] * X[c][d].y + u.im[a * K + c] * X[c][d].x;
       }
-      Y[a][d] = make_double2(r, i);
+      Y[d] = make_double2(r, i);
     }
-  // Z = Y U^dagger: Z[a][b] = sum_d Y[a][d] conj(U[b][d])
-#pragma unroll
-  for (int a = 0; a < K; a++)
 #pragma unroll
     for (int b = 0; b < K; b++) {
       double r = 0.0, i = 0.0;
 #pragma unroll
       for (int d = 0; d < K; d++) {
         const double ur = u.re[b * K + d], ui = -u.im[b * K + d];
-        r += Y[a][d].x * ur - Y[a][d].y * ui;
-        i += Y[a][d].x * ui + Y[a][d].y * ur;
+        r += Y[d].x * ur - Y[d].y * ui;
+        i += Y[d].x * ui + Y[d].y * ur;
       }
-      stv(x, lo, base + ro[a] + co[b], make_double2(r, i));
+      stv<F32>(x, lo, base + ro[a] + co[b], make_double2(r, i));
     }
+  }
+}
+
+// Two-qubit gate on a density matrix, four lanes per 4x4 block: lane q of a
+// quad loads column q of the block, forms column q of Y = U X, the quad
+// transposes Y through LDS, and lane q forms and stores row q of
+// Z = Y U^dagger.  Each lane holds 4 entries instead of 16, so the kernel runs
+// at full occupancy.
+template <bool P2, bool F32>
+__global__ void __launch_bounds__(256) k_gate2_dm(void *x, float2 *lo, int64_t nblk, int64_t s0, int64_t s1,
+                                                  int64_t s2, int64_t s3, int4 lg, int64_t bs0, int64_t bs1,
+                                                  int64_t cs0, int64_t cs1, U4 u) {
+  __shared__ double2 sh[4][256];
+  const int tid = threadIdx.x, q = tid & 3;
+  const int64_t t = (blockIdx.x * (int64_t)blockDim.x + tid) >> 2;
+  const bool on = t < nblk;
+  int64_t base = on ? t : 0;
+  if (P2) {
+    base = ins0p(base, lg.x);
+    base = ins0p(base, lg.y);
+    base = ins0p(base, lg.z);
+    base = ins0p(base, lg.w);
+  } else {
+    base = ins0(base, s0);
+    base = ins0(base, s1);
+    base = ins0(base, s2);
+    base = ins0(base, s3);
+  }
+  int64_t ro[4];
+#pragma unroll
+  for (int a = 0; a < 4; a++) ro[a] = ((a >> 1) & 1) * bs0 + (a & 1) * bs1;
+  const int64_t coq = ((q >> 1) & 1) * cs0 + (q & 1) * cs1;
+  double2 X[4];
+#pragma unroll
+  for (int c = 0; c < 4; c++) X[c] = on ? ldv<F32>(x, lo, base + ro[c] + coq) : make_double2(0.0, 0.0);
+  // column q of Y = U X
+#pragma unroll
+  for (int a = 0; a < 4; a++) {
+    double r = 0.0, i = 0.0;
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      r += u.re[a * 4 + c] * X[c].x - u.im[a * 4 + c] * X[c].y;
+      i += u.re[a * 4 + c] * X[c].y + u.im[a * 4 + c] * X[c].x;
+    }
+    sh[a][tid] = make_double2(r, i);
+  }
+  __syncthreads();
+  // row q of Y: Y[q][d] is column d's entry q, held by lane d of the quad
+  double2 Y[4];
+#pragma unroll
+  for (int d = 0; d < 4; d++) Y[d] = sh[q][tid - q + d];
+  if (!on) return;
+  const int64_t rq = base + ((q >> 1) & 1) * bs0 + (q & 1) * bs1;
+#pragma unroll
+  for (int b = 0; b < 4; b++) {
+    double r = 0.0, i = 0.0;
+#pragma unroll
+    for (int d = 0; d < 4; d++) {
+      const double ur = u.re[b * 4 + d], ui = -u.im[b * 4 + d];
+      r += Y[d].x * ur - Y[d].y * ui;
+      i += Y[d].x * ui + Y[d].y * ur;
+    }
+    const int64_t cb = ((b >> 1) & 1) * cs0 + (b & 1) * cs1;
+    stv<F32>(x, lo, rq + cb, make_double2(r, i));
+  }
 }
 
 }  // namespace
@@ -161,15 +229,37 @@ extern "C" int qh_gate(qh_vec *x, int64_t tot, int dm, int nq, const int64_t *st
     lgv[q] = k;
   }
   const int4 lg = make_int4(lgv[0], lgv[1], lgv[2], lgv[3]);
-#define QH_GATE(KK, DMM, PP)                                                                                        \
-  hipLaunchKernelGGL((k_gate<KK, DMM, PP>), grid, dim3(256), 0, s, x->d, x->lo, nthreads, nins, ins[0], ins[1],     \
-                     ins[2], ins[3], lg, bs0, bs1, cs0, cs1, m)
+#define QH_GATE(KK, DMM, PP)                                                                                      \
+  do {                                                                                                            \
+    if (x->lo)                                                                                                    \
+      hipLaunchKernelGGL((k_gate<KK, DMM, PP, true>), grid, dim3(256), 0, s, x->d, x->lo, nthreads, nins, ins[0], \
+                         ins[1], ins[2], ins[3], lg, bs0, bs1, cs0, cs1, m);                                      \
+    else                                                                                                          \
+      hipLaunchKernelGGL((k_gate<KK, DMM, PP, false>), grid, dim3(256), 0, s, x->d, x->lo, nthreads, nins,        \
+                         ins[0], ins[1], ins[2], ins[3], lg, bs0, bs1, cs0, cs1, m);                              \
+  } while (0)
   if (nq == 1 && dm) {
     if (p2) QH_GATE(2, true, true); else QH_GATE(2, true, false);
   } else if (nq == 1) {
     if (p2) QH_GATE(2, false, true); else QH_GATE(2, false, false);
   } else if (dm) {
-    if (p2) QH_GATE(4, true, true); else QH_GATE(4, true, false);
+    // four lanes per block (nthreads blocks, always 4 inserted strides)
+    const dim3 g4(qh_blocks(nthreads * 4, 256));
+    if (p2) {
+      if (x->lo)
+        hipLaunchKernelGGL((k_gate2_dm<true, true>), g4, dim3(256), 0, s, x->d, x->lo, nthreads, ins[0], ins[1],
+                           ins[2], ins[3], lg, bs0, bs1, cs0, cs1, m);
+      else
+        hipLaunchKernelGGL((k_gate2_dm<true, false>), g4, dim3(256), 0, s, x->d, x->lo, nthreads, ins[0], ins[1],
+                           ins[2], ins[3], lg, bs0, bs1, cs0, cs1, m);
+    } else {
+      if (x->lo)
+        hipLaunchKernelGGL((k_gate2_dm<false, true>), g4, dim3(256), 0, s, x->d, x->lo, nthreads, ins[0], ins[1],
+                           ins[2], ins[3], lg, bs0, bs1, cs0, cs1, m);
+      else
+        hipLaunchKernelGGL((k_gate2_dm<false, false>), g4, dim3(256), 0, s, x->d, x->lo, nthreads, ins[0], ins[1],
+                           ins[2], ins[3], lg, bs0, bs1, cs0, cs1, m);
+    }
   } else {
     if (p2) QH_GATE(4, false, true); else QH_GATE(4, false, false);
   }

@@ -285,6 +285,28 @@ def kerr_sweep_params(seed, count):
     return out
 
 
+KERR_CENTER = {"delta": [0.3, -0.2, 0.1], "K": [0.2, 0.3, 0.25], "F": [0.5, 0.6, 0.4], "J": 0.3,
+               "kappa": [0.1] * 3}
+
+
+def kerr_local_params(seed, count, rel=0.05, center=None):
+    """parameter sets of a local sweep: every parameter of the centre set (inside
+    the ranges of kerr_sweep_params) scaled by an independent factor in
+    U[1 - rel, 1 + rel], the shared cavity loss included"""
+    c = center or KERR_CENTER
+    rng = SplitMix(seed)
+    out = []
+    for _ in range(count):
+        f = lambda v: v * (1.0 + rng.uniform(-rel, rel))
+        d = [f(v) for v in c["delta"]]
+        K = [f(v) for v in c["K"]]
+        F = [f(v) for v in c["F"]]
+        J = f(c["J"])
+        kap = f(c["kappa"][0])
+        out.append({"delta": d, "K": K, "F": F, "J": J, "kappa": [kap] * 3})
+    return out
+
+
 def kerr_features(p):
     """the 10 network inputs: every Hamiltonian parameter in units of the loss
     (the steady state of L = -i[H, .] + kappa sum D[a_k] depends on H / kappa only)"""
