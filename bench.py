@@ -193,9 +193,10 @@ def run_sweep(qa, recipes, build_system, subsystem_layout, K, W, n_inst, renorm)
         qs.destroy()
     v32 = n_inst * K / el32
     v64 = n_inst * K / el64
+    traffic, src = read_traffic(os.path.join(ROOT, "profiles", "k_ks2_stage_tc512_traffic.txt"))
     rf = roofline_obj(kernel_name(lanes[1], "float") + " (mean of the 4 fused RK4 stage launches)",
                       f"{n_inst} Tavis-Cummings instances, one block-diagonal batch", fmt_step / 4, launch_ms,
-                      ev_ms, None, None, rk4_step_bytes(nnz, rows, 8) / 4)
+                      ev_ms, traffic, src, rk4_step_bytes(nnz, rows, 8) / 4)
     return {"v32": v32, "v64": v64, "el32": el32, "el64": el64, "worst": worst, "asm_s": asm_s, "rows": rows,
             "nnz": nnz, "rows_min": min(t * t for t in tots), "rows_max": max(t * t for t in tots), "lanes": lanes,
             "roofline": rf, "layout": layout}

@@ -153,9 +153,11 @@ __global__ void __launch_bounds__(256) k_gate2_dm(void *x, float2 *lo, int64_t n
 #pragma unroll
   for (int a = 0; a < 4; a++) ro[a] = ((a >> 1) & 1) * bs0 + (a & 1) * bs1;
   const int64_t coq = ((q >> 1) & 1) * cs0 + (q & 1) * cs1;
+  // a quad past the last block (all four lanes share t) loads block 0 and
+  // stores nothing: unconditional loads, all in flight together
   double2 X[4];
 #pragma unroll
-  for (int c = 0; c < 4; c++) X[c] = on ? ldv<F32>(x, lo, base + ro[c] + coq) : make_double2(0.0, 0.0);
+  for (int c = 0; c < 4; c++) X[c] = ldv<F32>(x, lo, base + ro[c] + coq);
   // column q of Y = U X
 #pragma unroll
   for (int a = 0; a < 4; a++) {
