@@ -86,6 +86,38 @@ def test_spin_chain6_fp64_and_fp32_vs_oracle():
     qa.set_option("quac_precision", "fp64")
 
 
+def test_spin_chain10_full_size_vs_oracle():
+    """The 10-qubit dissipative chain at full size (rho 1024 x 1024, 1 048 576
+    superoperator rows), 20 RK4 steps of 0.005 through time_step_sys in FP64 and
+    in FP32, against the oracle's FP64 RK4 on its own CSR: FP64 1e-10 on every
+    population (and 1e-12 on the whole state), FP32 1e-6."""
+    _fixed_rk4()
+    r = recipes.spin_chain_recipe(10, seed=1)
+    N, loc = recipes.spin_chain_init(10)
+    x0 = np.zeros(N * N, np.complex128)
+    x0[loc] = 1
+    tot, terms = recipes.oracle_terms(r)
+    ref = oracle.construct(tot, terms, dm=True).rk4(x0, 0.005, 20, nthreads=8)
+    pr = populations_np(ref, r)
+    lay = subsystem_layout(r)
+    try:
+        for prec, tol in (("fp64", 1e-10), ("fp32", 1e-6)):
+            qa.set_option("quac_precision", prec)
+            qs, subs, op = build_system(r)
+            x = qs.create_qvec()
+            x.add_loc(1.0, loc)
+            steps, tend = qs.time_step(x, 0.0, 0.1, 0.005, 100000)
+            assert steps == 20 and abs(tend - 0.1) < 1e-12
+            pg = x.populations([L for L, _ in lay], [s for _, s in lay])
+            assert np.abs(pg - pr).max() < tol, (prec, np.abs(pg - pr).max())
+            if prec == "fp64":
+                assert np.abs(x.get() - ref).max() < 1e-12
+            assert abs(x.trace() - 1.0) < (1e-12 if prec == "fp64" else 1e-6)
+            qs.destroy()
+    finally:
+        qa.set_option("quac_precision", "fp64")
+
+
 def test_rk4_device_direct_matches_oracle_and_lanes():
     """qh_rk4 on the assembled matrix for every lanes-per-row variant."""
     r = recipes.spin_chain_recipe(5, seed=3)
