@@ -104,7 +104,10 @@ int ens_work(int64_t n) {
 
 // ----------------------------------------------- trajectories of up to 1024 amplitudes
 // A wavefront integrates 64 / S trajectories side by side, one segment of S
-// lanes (S a power of two >= N) per trajectory: lane r of a segment holds
+// lanes per trajectory (S a power of two >= N, or S = 0: segments of exactly N
+// lanes, floor(64 / N) of them, where that fits more trajectories into a
+// wavefront than the power of two does; the lanes past the last segment idle):
+// lane r of a segment holds
 // amplitude r and row r of the generator (NZ entries in registers), and
 // the products exchange the amplitudes through the segment's part of the
 // wavefront's LDS row.  A trajectory runs classic RK4 steps (size dt, the last
@@ -217,13 +220,28 @@ __global__ void __launch_bounds__(256) k_mcwf_run(int32_t N, int64_t M, const in
                                                   double2 *__restrict__ ens, const double *__restrict__ thr,
                                                   int32_t *__restrict__ jst, double *__restrict__ tsub, int32_t J,
                                                   double dt, double hlast, int32_t *__restrict__ jump) {
-  constexpr int P = ENS_W / S;  // trajectories per wavefront
+  const int SL = S ? S : N;             // lanes per segment
+  const int P = S ? ENS_W / S : ENS_W / N;  // trajectories per wavefront
   __shared__ double2 sh[4][ENS_W];
+  __shared__ double red[4][ENS_W];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int seg = lane / S, r = lane % S;
+  const int seg = lane / SL, r = lane - seg * SL;
   const int64_t i = ((int64_t)blockIdx.x * 4 + w) * P + seg;
   if (((int64_t)blockIdx.x * 4 + w) * P >= M) return;  // whole wavefront
-  const bool have = i < M;
+  const bool have = seg < P && i < M;  // seg == P: a lane past the last segment
+  // sum over the lane's segment, the bit-identical value in all its lanes:
+  // DPP for power-of-two segments, else every lane adds the segment's values in
+  // lane order from the wavefront's LDS row (an idle lane sums segment 0)
+  auto ssum = [&](double v) -> double {
+    if constexpr (S != 0) return seg_sum<S>(v);
+    red[w][lane] = v;
+    __builtin_amdgcn_wave_barrier();
+    const double *q = red[w] + (seg < P ? seg : 0) * N;
+    double acc = 0.0;
+    for (int k = 0; k < N; k++) acc += q[k];
+    __builtin_amdgcn_wave_barrier();
+    return acc;
+  };
   int32_t j = have ? jst[i] : J;
   double sub = have ? tsub[i] : 0.0;  // time already covered inside step j
   const bool on = have && r < N;
@@ -240,7 +258,7 @@ __global__ void __launch_bounds__(256) k_mcwf_run(int32_t N, int64_t M, const in
       a[e] = val[beg + e];
     }
   }
-  double2 *sg = sh[w] + seg * S;
+  double2 *sg = sh[w] + seg * SL;  // idle lanes: their own lane's entry
   double2 psi = on ? ens[i * N + r] : make_double2(0.0, 0.0);
   const double t = have ? thr[i] : 0.0;
   // A v for this lane's row; the LDS row is written and read by this wavefront
@@ -263,7 +281,7 @@ __global__ void __launch_bounds__(256) k_mcwf_run(int32_t N, int64_t M, const in
     const double s6 = h * (1.0 / 6.0), s3 = h * (1.0 / 3.0);
     const double2 nx = make_double2(psi.x + s6 * k1.x + s3 * k2.x + s3 * k3.x + s6 * k4.x,
                                     psi.y + s6 * k1.y + s3 * k2.y + s3 * k3.y + s6 * k4.y);
-    const double n2 = seg_sum<S>(on ? nx.x * nx.x + nx.y * nx.y : 0.0);
+    const double n2 = ssum(on ? nx.x * nx.x + nx.y * nx.y : 0.0);
     const bool crossed = act && n2 < t;
     double2 pc = nx;
     double tau = h;
@@ -281,7 +299,7 @@ __global__ void __launch_bounds__(256) k_mcwf_run(int32_t N, int64_t M, const in
       double cm[9];
       cross_partials(v, cm);
 #pragma unroll
-      for (int m = 0; m < 9; m++) cm[m] = seg_sum<S>(cm[m]);
+      for (int m = 0; m < 9; m++) cm[m] = ssum(cm[m]);
       tau = crossed ? cross_tau(cm, t, h) : h;
       pc = horner4(v, tau);
     }
@@ -589,8 +607,10 @@ extern "C" int qh_mcwf_advance(const qh_mat *A, qh_vec *ens, int64_t M, const do
   memcpy(h + bt, tsub, bt);
   memcpy(h + 2 * bt, jstep, bj);
   QH_HIP(hipMemcpyAsync(d, h, 2 * bt + bj, hipMemcpyHostToDevice, s));
-  const int S = N <= 2 ? 2 : N <= 4 ? 4 : N <= 8 ? 8 : N <= 16 ? 16 : N <= 32 ? 32 : 64;
-  const int64_t waves = (M + ENS_W / S - 1) / (ENS_W / S);
+  int S = N <= 2 ? 2 : N <= 4 ? 4 : N <= 8 ? 8 : N <= 16 ? 16 : N <= 32 ? 32 : 64;
+  if (ENS_W / N > ENS_W / S) S = 0;  // segments of exactly N lanes fit more trajectories
+  const int per = S ? ENS_W / S : ENS_W / N;
+  const int64_t waves = (M + per - 1) / per;
   const dim3 grid((unsigned)((waves + 3) / 4));
   // generator entries per lane: the row length rounded up to 2, 3, 4 or 16
   // (each padded entry costs four FP64 FMAs per stage)
@@ -628,6 +648,7 @@ extern "C" int qh_mcwf_advance(const qh_mat *A, qh_vec *ens, int64_t M, const do
     else                                                                                                          \
       QH_MCWF_RUN_NZ(SS, ENS_MAXNZ);                                                                              \
   }
+  QH_MCWF_RUN(0)
   QH_MCWF_RUN(2)
   QH_MCWF_RUN(4)
   QH_MCWF_RUN(8)

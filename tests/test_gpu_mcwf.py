@@ -181,6 +181,7 @@ def _cross_np(Ad, psi0, h, thr):
 
 
 @pytest.mark.parametrize("system,row", [("jc2x4", 2), ("qubits2", 3), ("qubits3", 4), ("qubits4", 5),
+                                        ("jc2x3", 2), ("jc2x5", 2), ("jc2x10", 2),
                                         ("jc2x40", 2), ("qubits7", 8), ("qubits10", 11)])
 def test_mcwf_advance_matches_stepwise_ensemble_rk4(system, row):
     """qh_mcwf_advance (each trajectory runs from its step index to its first
@@ -190,9 +191,10 @@ def test_mcwf_advance_matches_stepwise_ensemble_rk4(system, row):
     trajectory stops, and the amplitudes there, to 1e-12.  Covers thresholds
     that are never crossed, a trajectory that has already taken all steps, a late
     start and a shorter last step; generator rows of 2 to 5 entries (2-, 3-, 4-
-    and 16-entry register layouts; 4 to 16 lanes per trajectory) and, one
-    workgroup per trajectory, 80, 128 and 1024 amplitudes with rows of 2, 8 and
-    11 entries."""
+    and 16-entry register layouts; 4 to 16 lanes per trajectory; segments of
+    exactly 6, 10 and 20 lanes, 10, 6 and 3 trajectories per wavefront) and,
+    one workgroup per trajectory, 80, 128 and 1024 amplitudes with rows of 2, 8
+    and 11 entries."""
     if system.startswith("jc"):
         qs, A = _jc_generator(levels=int(system[4:]))
     else:
