@@ -159,6 +159,28 @@ __device__ inline double seg_sum(double v) {
   return v;
 }
 
+// v of the lane DPP control CTRL selects, 0 where the source lane lies outside
+// the row or the lane's row is not in ROWS
+template <int CTRL, int ROWS>
+__device__ inline double dpp_f64_or0(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROWS, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROWS, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
+// inclusive prefix sum of v over the 64 lanes: Hillis-Steele inside each row of
+// 16 (row_shr 1, 2, 4, 8), then the row totals carried forward (row_bcast 15
+// into rows 1 and 3, row_bcast 31 into rows 2 and 3)
+__device__ inline double wave_prefix_f64(double v) {
+  v += dpp_f64_or0<0x111, 0xF>(v);
+  v += dpp_f64_or0<0x112, 0xF>(v);
+  v += dpp_f64_or0<0x114, 0xF>(v);
+  v += dpp_f64_or0<0x118, 0xF>(v);
+  v += dpp_f64_or0<0x142, 0xA>(v);
+  v += dpp_f64_or0<0x143, 0xC>(v);
+  return v;
+}
+
 // Location of a norm crossing inside an RK4 step.  On the linear system
 // d psi/dt = A psi a classic RK4 step of size tau from psi0 is exactly
 // P(tau) = v0 + tau v1 + tau^2 v2 + tau^3 v3 + tau^4 v4 with v_k = A^k psi0 / k!,
@@ -231,7 +253,21 @@ __global__ void __launch_bounds__(256) k_mcwf_run(int32_t N, int64_t M, const in
   const bool have = seg < P && i < M;  // seg == P: a lane past the last segment
   // sum over the lane's segment, the bit-identical value in all its lanes:
   // DPP for power-of-two segments, else every lane adds the segment's values in
-  // lane order from the wavefront's LDS row (an idle lane sums segment 0)
+  // lane order from the wavefront's LDS row (an idle lane sums segment 0); used
+  // for the nine polynomial coefficients of a crossing, whose terms have either
+  // sign
+  // the per-step squared norm on segments of N lanes: the difference of two
+  // entries of the wavefront's prefix sum, read by every lane of the segment
+  // from the same two lanes (the values are non-negative, so the difference
+  // keeps full relative accuracy at the scale of the segment's own sum)
+  auto nsum = [&](double v) -> double {
+    if constexpr (S != 0) return seg_sum<S>(v);
+    const double pf = wave_prefix_f64(v);
+    const int b = (seg < P ? seg : 0) * N;
+    const double pe = __shfl(pf, b + N - 1, 64);
+    const double ps = __shfl(pf, b > 0 ? b - 1 : 0, 64);
+    return b > 0 ? pe - ps : pe;
+  };
   auto ssum = [&](double v) -> double {
     if constexpr (S != 0) return seg_sum<S>(v);
     red[w][lane] = v;
@@ -281,7 +317,7 @@ __global__ void __launch_bounds__(256) k_mcwf_run(int32_t N, int64_t M, const in
     const double s6 = h * (1.0 / 6.0), s3 = h * (1.0 / 3.0);
     const double2 nx = make_double2(psi.x + s6 * k1.x + s3 * k2.x + s3 * k3.x + s6 * k4.x,
                                     psi.y + s6 * k1.y + s3 * k2.y + s3 * k3.y + s6 * k4.y);
-    const double n2 = ssum(on ? nx.x * nx.x + nx.y * nx.y : 0.0);
+    const double n2 = nsum(on ? nx.x * nx.x + nx.y * nx.y : 0.0);
     const bool crossed = act && n2 < t;
     double2 pc = nx;
     double tau = h;
