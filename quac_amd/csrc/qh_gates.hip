@@ -124,18 +124,20 @@ __global__ void __launch_bounds__(256) k_gate(void *x, float2 *lo, int64_t nthre
   }
 }
 
-// Two-qubit gate on a density matrix, four lanes per 4x4 block: lane q of a
-// quad loads column q of the block, forms column q of Y = U X, the quad
-// transposes Y through LDS, and lane q forms and stores row q of
-// Z = Y U^dagger.  Each lane holds 4 entries instead of 16, so the kernel runs
-// at full occupancy.
+// Two-qubit gate on a density matrix: a workgroup takes 64 consecutive 4x4
+// blocks, and wavefront q of its four handles column q of all of them.  Lane t
+// of wavefront q loads column q of block t and forms column q of Y = U X; the
+// workgroup transposes Y through LDS; then the lane forms and stores row q of
+// Z = Y U^dagger for block t.  Every load and store instruction of a wavefront
+// covers 64 consecutive blocks (one contiguous run where the gate's qubits are
+// not the lowest ones), and each lane holds 4 entries, not 16.
 template <bool P2, bool F32>
 __global__ void __launch_bounds__(256) k_gate2_dm(void *x, float2 *lo, int64_t nblk, int64_t s0, int64_t s1,
                                                   int64_t s2, int64_t s3, int4 lg, int64_t bs0, int64_t bs1,
                                                   int64_t cs0, int64_t cs1, U4 u) {
-  __shared__ double2 sh[4][256];
-  const int tid = threadIdx.x, q = tid & 3;
-  const int64_t t = (blockIdx.x * (int64_t)blockDim.x + tid) >> 2;
+  __shared__ double2 sh[4][256];  // sh[a][q * 64 + t] = Y[a][q] of block t
+  const int tid = threadIdx.x, q = tid >> 6, tl = tid & 63;
+  const int64_t t = (int64_t)blockIdx.x * 64 + tl;
   const bool on = t < nblk;
   int64_t base = on ? t : 0;
   if (P2) {
@@ -153,8 +155,8 @@ __global__ void __launch_bounds__(256) k_gate2_dm(void *x, float2 *lo, int64_t n
 #pragma unroll
   for (int a = 0; a < 4; a++) ro[a] = ((a >> 1) & 1) * bs0 + (a & 1) * bs1;
   const int64_t coq = ((q >> 1) & 1) * cs0 + (q & 1) * cs1;
-  // a quad past the last block (all four lanes share t) loads block 0 and
-  // stores nothing: unconditional loads, all in flight together
+  // a lane past the last block loads block 0 and stores nothing:
+  // unconditional loads, all in flight together
   double2 X[4];
 #pragma unroll
   for (int c = 0; c < 4; c++) X[c] = ldv<F32>(x, lo, base + ro[c] + coq);
@@ -167,13 +169,13 @@ __global__ void __launch_bounds__(256) k_gate2_dm(void *x, float2 *lo, int64_t n
       r += u.re[a * 4 + c] * X[c].x - u.im[a * 4 + c] * X[c].y;
       i += u.re[a * 4 + c] * X[c].y + u.im[a * 4 + c] * X[c].x;
     }
-    sh[a][tid] = make_double2(r, i);
+    sh[a][q * 64 + tl] = make_double2(r, i);
   }
   __syncthreads();
-  // row q of Y: Y[q][d] is column d's entry q, held by lane d of the quad
+  // row q of Y: Y[q][d] is entry q of column d, written by wavefront d
   double2 Y[4];
 #pragma unroll
-  for (int d = 0; d < 4; d++) Y[d] = sh[q][tid - q + d];
+  for (int d = 0; d < 4; d++) Y[d] = sh[q][d * 64 + tl];
   if (!on) return;
   const int64_t rq = base + ((q >> 1) & 1) * bs0 + (q & 1) * bs1;
 #pragma unroll
@@ -245,8 +247,9 @@ extern "C" int qh_gate(qh_vec *x, int64_t tot, int dm, int nq, const int64_t *st
   } else if (nq == 1) {
     if (p2) QH_GATE(2, false, true); else QH_GATE(2, false, false);
   } else if (dm) {
-    // four lanes per block (nthreads blocks, always 4 inserted strides)
-    const dim3 g4(qh_blocks(nthreads * 4, 256));
+    // 64 blocks per workgroup of four wavefronts (nthreads blocks, always 4
+    // inserted strides)
+    const dim3 g4(qh_blocks(nthreads, 64));
     if (p2) {
       if (x->lo)
         hipLaunchKernelGGL((k_gate2_dm<true, true>), g4, dim3(256), 0, s, x->d, x->lo, nthreads, ins[0], ins[1],
