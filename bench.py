@@ -78,7 +78,7 @@ def format_step_bytes(A, rows):
     if lanes[1] in (140, 141, 142):  # row-pair layout: 40-byte slot metadata, values per lane pair
         layout = dict(A.pair_layout_info(), kind="row-pair (128 rows per wavefront)")
         meta = layout["meta_bytes"]
-        vals = layout["stored_pairs"] * 16 + layout["stored_real_pairs"] * 8
+        vals = layout["stored_pairs"] * 16 + layout["stored_real_pairs"] * 8 + layout["stored_real_singles"] * 4
     else:
         layout = dict(A.layout_info(), kind="one row per lane (64 rows per wavefront)")
         meta = layout["slots"] * 32 + 12 * layout["slices"]

@@ -85,9 +85,9 @@ int qh_mat_layout_info(qh_mat *m, int64_t *slices, int64_t *slots, int64_t *stor
                        int64_t *meta_bytes);
 /* FP32 row-pair sliced layout statistics (built on demand; zeros when it does
  * not apply): 128-row slices, slots, stored complex pairs, stored real pairs,
- * metadata bytes */
+ * metadata bytes, stored single reals (slots whose two rows share each value) */
 int qh_mat_pair_layout_info(qh_mat *m, int64_t *slices, int64_t *slots, int64_t *stored_pairs,
-                            int64_t *stored_real_pairs, int64_t *meta_bytes);
+                            int64_t *stored_real_pairs, int64_t *meta_bytes, int64_t *stored_real_singles);
 
 /* ------------------------------------------------------------------ vectors */
 typedef struct qh_vec qh_vec;

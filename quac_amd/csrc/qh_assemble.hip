@@ -664,10 +664,12 @@ extern "C" int qh_mat_destroy(qh_mat *m) {
   qh_free_ptr(m->ks2_wmid);
   qh_free_ptr(m->ks2_smid);
   qh_free_ptr(m->ks2_umid);
+  qh_free_ptr(m->ks2_emid);
   qh_free_ptr(m->ks2_cmid);
   qh_free_ptr(m->ks2_meta);
   qh_free_ptr(m->ks2_val);
   qh_free_ptr(m->ks2_sval);
+  qh_free_ptr(m->ks2_sval1);
   qh_free_ptr(m->ks_sval);
   qh_free_ptr(m->ks_sval32);
   qh_free_ptr(m->ks_meta);

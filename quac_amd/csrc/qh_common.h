@@ -115,11 +115,13 @@ struct qh_mat {
   int32_t *ks2_wmid = nullptr;  // end of each slice's in-order uniform slots inside its 2048-row window (first)
   int32_t *ks2_smid = nullptr;  // end of each slice's uniform slots with in-order halves
   int32_t *ks2_umid = nullptr;  // end of each slice's uniform slots
-  int32_t *ks2_cmid = nullptr;  // end of each slice's scalar slots
+  int32_t *ks2_emid = nullptr;  // end of each slice's scalar slots with a value per row
+  int32_t *ks2_cmid = nullptr;  // end of each slice's scalar slots (equal-scalar slots included)
   qh_kmeta2 *ks2_meta = nullptr;
   float4 *ks2_val = nullptr;    // complex slots: (row 2l, row 2l+1) complex64 pairs
   float2 *ks2_sval = nullptr;   // scalar slots: (row 2l, row 2l+1) real pairs
-  int64_t ks2_slots = 0, ks2_nval = 0, ks2_nsval = 0;
+  float *ks2_sval1 = nullptr;   // equal-scalar slots: one real value for both rows of a lane
+  int64_t ks2_slots = 0, ks2_nval = 0, ks2_nsval = 0, ks2_nsval1 = 0;
   qh_td_cache *td_cache = nullptr;  // built by the first time-dependent RHS on this matrix
 };
 // variant codes selecting the sliced-ELL kernel: 128 / 129 / 130 process 2 / 4 / 8

@@ -23,8 +23,8 @@ __device__ inline __amdgpu_buffer_rsrc_t ks_rsrc(const void *p, int64_t bytes) {
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes, 0x00020000);
 }
 
-// slot groups of a slice
-enum { KS_UNI = 0, KS_SCA = 1, KS_CPX = 2 };
+// slot groups of a slice (KS_SC1: real values, one per lane for both of its rows)
+enum { KS_UNI = 0, KS_SCA = 1, KS_CPX = 2, KS_SC1 = 3 };
 
 // row sum of a one-row-per-lane pass (FP64)
 struct KsAcc {

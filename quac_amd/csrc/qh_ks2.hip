@@ -15,7 +15,8 @@
 // whose loads share (|b - 2l|, h0, h1, rank);
 // the rest of the machinery (sign masks, uniform / scalar / complex slot groups,
 // compacted values, exact tails, two passes in flight) is that of the
-// one-row-per-lane layout.  Products are summed in FP32 within a pass and the
+// one-row-per-lane layout, with one more group: scalar slots whose two rows
+// hold the same value in every lane store one real per lane (4 bytes, not 8).  Products are summed in FP32 within a pass and the
 // pass sums in FP64; the state is hi + lo as in the other FP32 kernels.
 #include <vector>
 
@@ -33,19 +34,21 @@ const int KS2_WIN = 2048;
 
 // ------------------------------------------------------------------ build
 // Per-slice counts (FILL false): cnt[0*ns + s] slots, [1] complex pairs, [2]
-// uniform slots, [3] scalar slots, [4] scalar pairs, [5] uniform slots whose
-// rows take the halves in order (row 2l half 0, row 2l+1 half 1: no select in
-// the kernel), [6] those of them whose loads all stay inside the slice's
-// KS2_WIN-row window.  FILL writes the metadata (uniform in-order window slots
-// at moff, the other uniform in-order slots, the other uniform slots, then the
-// scalar and complex groups) and the values (complex pairs at voff, scalar pairs
-// at svoff).
+// uniform slots, [3] scalar slots with a value per row, [4] their value pairs,
+// [5] uniform slots whose rows take the halves in order (row 2l half 0, row
+// 2l+1 half 1: no select in the kernel), [6] those of them whose loads all stay
+// inside the slice's KS2_WIN-row window, [7] scalar slots whose two rows hold
+// the same value in every lane, [8] their values (one per lane).  FILL writes
+// the metadata (uniform in-order window slots at moff, the other uniform
+// in-order slots, the other uniform slots, then the scalar, the equal-scalar and
+// the complex groups) and the values (complex pairs at voff, scalar pairs at
+// svoff, equal scalars at s1off).
 template <bool FILL>
 __global__ void __launch_bounds__(64) k_ks2_build(int32_t n, const int32_t *__restrict__ rp,
                                                   const int32_t *__restrict__ col, const double2 *__restrict__ val,
                                                   int32_t *cnt, int32_t ns, const int32_t *moff, const int32_t *voff,
-                                                  const int32_t *svoff, qh_kmeta2 *meta, float4 *kval, float2 *sval,
-                                                  int *overflow) {
+                                                  const int32_t *svoff, const int32_t *s1off, qh_kmeta2 *meta,
+                                                  float4 *kval, float2 *sval, float *sval1, int *overflow) {
   // per lane up to 2 * KS2_MAXROW pair loads: key << 12 | e0 << 6 | e1 (63 = no entry)
   __shared__ uint64_t keys[64][2 * KS2_MAXROW];
   const int s = blockIdx.x, lane = threadIdx.x;
@@ -126,13 +129,15 @@ __global__ void __launch_bounds__(64) k_ks2_build(int32_t n, const int32_t *__re
     prev = adp;
   }
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  int p = 0, slot = 0, uslot = 0, sslot = 0, cslot = 0, ustr = 0, uwin = 0;
+  int p = 0, slot = 0, uslot = 0, sslot = 0, s1slot = 0, cslot = 0, ustr = 0, uwin = 0;
   const int64_t w0 = ((int64_t)s * 128 / KS2_WIN) * KS2_WIN;  // first row of the slice's window
   int32_t base = FILL ? voff[s] : 0;
   int32_t sbase = FILL ? svoff[s] : 0;
+  int32_t s1base = FILL ? s1off[s] : 0;
   const int32_t mo = FILL ? moff[s] : 0;
   const int32_t ms = FILL ? mo + cnt[2 * ns + s] : 0;
-  const int32_t mc = FILL ? ms + cnt[3 * ns + s] : 0;
+  const int32_t ms1 = FILL ? ms + cnt[3 * ns + s] : 0;
+  const int32_t mc = FILL ? ms1 + cnt[7 * ns + s] : 0;
   const int32_t mu2 = FILL ? mo + cnt[5 * ns + s] : 0;  // uniform slots that need the half selects
   const int32_t mw = FILL ? mo + cnt[6 * ns + s] : 0;   // in-order uniform slots outside the window
   for (;;) {
@@ -155,6 +160,9 @@ __global__ void __launch_bounds__(64) k_ks2_build(int32_t n, const int32_t *__re
     const bool isre = !uni && __ballot(act && (v0.y != 0.0 || v1.y != 0.0)) == 0ull;
     const bool isim = !uni && !isre && __ballot(act && (v0.x != 0.0 || v1.x != 0.0)) == 0ull;
     const bool sca = isre || isim;
+    // equal scalars: both rows of every active lane hold the same FP32 value
+    auto part = [&](double2 v) { return (float)(isre ? v.x : v.y); };
+    const bool eq1 = sca && __ballot(act && part(v0) != part(v1)) == 0ull;
     // in-order halves: an active row 2l takes half 0, an active row 2l+1 half 1
     const uint32_t pat_ = (uint32_t)((m >> 1) & 15u);
     const bool inorder = !(pat_ & 4u) && (pat_ & 8u);  // h0 = 0, h1 = 1
@@ -176,7 +184,7 @@ __global__ void __launch_bounds__(64) k_ks2_build(int32_t n, const int32_t *__re
         md.mask = mask;
         md.sign = sign;
         md.ad8 = (uint32_t)(ad * 8u) | h0 | (h1 << 1);
-        md.base = uni ? 0u : (uint32_t)(sca ? sbase : base);
+        md.base = uni ? 0u : (uint32_t)(eq1 ? s1base : sca ? sbase : base);
         if (uni) {
           md.v0re = (float)f0r;
           md.v0im = (float)f0i;
@@ -187,14 +195,18 @@ __global__ void __launch_bounds__(64) k_ks2_build(int32_t n, const int32_t *__re
           md.v0im = md.v1im = sca ? (isre ? 0.f : 1.f) : 0.f;
         }
         meta[uni ? (win ? mo + uwin : inorder ? mw + (ustr - uwin) : mu2 + (uslot - ustr))
-                 : sca ? ms + sslot : mc + cslot] = md;
+                 : eq1 ? ms1 + s1slot : sca ? ms + sslot : mc + cslot] = md;
       }
-      if (act && sca)
-        sval[sbase + rank] = isre ? make_float2((float)v0.x, (float)v1.x) : make_float2((float)v0.y, (float)v1.y);
+      if (act && eq1)
+        sval1[s1base + rank] = part(v0);
+      else if (act && sca)
+        sval[sbase + rank] = make_float2(part(v0), part(v1));
       if (act && !uni && !sca)
         kval[base + rank] = make_float4((float)v0.x, (float)v0.y, (float)v1.x, (float)v1.y);
     }
-    if (sca)
+    if (eq1)
+      s1base += __popcll(mask);
+    else if (sca)
       sbase += __popcll(mask);
     else if (!uni)
       base += __popcll(mask);
@@ -204,6 +216,8 @@ __global__ void __launch_bounds__(64) k_ks2_build(int32_t n, const int32_t *__re
       uslot++;
       if (inorder) ustr++;
       if (win) uwin++;
+    } else if (eq1) {
+      s1slot++;
     } else if (sca) {
       sslot++;
     } else {
@@ -218,6 +232,8 @@ __global__ void __launch_bounds__(64) k_ks2_build(int32_t n, const int32_t *__re
     cnt[4 * ns + s] = sbase;
     cnt[5 * ns + s] = ustr;
     cnt[6 * ns + s] = uwin;
+    cnt[7 * ns + s] = s1slot;
+    cnt[8 * ns + s] = s1base;
   }
 }
 
@@ -233,6 +249,7 @@ template <int KIND, int W>
 struct Ks2Buf {
   float4 xv[W], vv[W];
   float2 sv[W];
+  float s1[W];
   float u0r[W], u0i[W], u1r[W], u1i[W];
   uint32_t h[W];
 };
@@ -272,7 +289,9 @@ struct Ks2Ops {
       if (KIND != KS_UNI) {
         const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(m[j].mask >> 32),
                                                         __builtin_amdgcn_mbcnt_lo((uint32_t)m[j].mask, 0u));
-        if (KIND == KS_SCA)
+        if (KIND == KS_SC1)
+          b.s1[j] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(vr, (m[j].base + rank) * 4u, 0, 0));
+        else if (KIND == KS_SCA)
           b.sv[j] = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(vr, (m[j].base + rank) * 8u, 0, 0));
         else
           b.vv[j] =
@@ -306,6 +325,9 @@ struct Ks2Ops {
         a0i = b.vv[j].y;
         a1r = b.vv[j].z;
         a1i = b.vv[j].w;
+      } else if (KIND == KS_SC1) {
+        a0r = a1r = b.s1[j] * b.u0r[j];
+        a0i = a1i = b.s1[j] * b.u0i[j];
       } else if (KIND == KS_SCA) {
         a0r = b.sv[j].x * b.u0r[j];
         a0i = b.sv[j].x * b.u0i[j];
@@ -404,10 +426,11 @@ __device__ inline void st_pair(__amdgpu_buffer_rsrc_t rs, uint32_t o, bool two, 
 template <int MODE, int U, bool WIN>
 __global__ void __launch_bounds__(WIN ? KS2_WIN / 2 : 256) k_ks2_stage(
     int32_t n, int32_t nslices, const int32_t *__restrict__ moff, const int32_t *__restrict__ wmid,
-    const int32_t *__restrict__ smid, const int32_t *__restrict__ umid, const int32_t *__restrict__ cmid,
-                                                   const qh_kmeta2 *__restrict__ meta, int32_t nval,
+    const int32_t *__restrict__ smid, const int32_t *__restrict__ umid, const int32_t *__restrict__ emid,
+    const int32_t *__restrict__ cmid, const qh_kmeta2 *__restrict__ meta, int32_t nval,
                                                    const float4 *__restrict__ val, int32_t nsval,
-                                                   const float2 *__restrict__ sval, const float2 *xin, float2 *kout,
+                                                   const float2 *__restrict__ sval, int32_t nsval1,
+                                                   const float *__restrict__ sval1, const float2 *xin, float2 *kout,
                                                    float2 *ynext, float cn, const float2 *x, const float2 *k1,
                                                    const float2 *k2, const float2 *k3, float2 *xout, float h,
                                                    float2 *xlo) {
@@ -432,9 +455,11 @@ __global__ void __launch_bounds__(WIN ? KS2_WIN / 2 : 256) k_ks2_stage(
   if (s >= nslices) return;  // whole wavefront (after the window copy)
   const int lane = threadIdx.x & 63;
   const int32_t r = s * 128 + 2 * lane;
-  const int32_t k0 = moff[s], kw = wmid[s], ku = smid[s], km = umid[s], kc = cmid[s], kend = moff[s + 1];
+  const int32_t k0 = moff[s], kw = wmid[s], ku = smid[s], km = umid[s], ke = emid[s], kc = cmid[s],
+                kend = moff[s + 1];
   const __amdgpu_buffer_rsrc_t vr = ks_rsrc(val, (int64_t)nval * 16);
   const __amdgpu_buffer_rsrc_t sr = ks_rsrc(sval, (int64_t)nsval * 8);
+  const __amdgpu_buffer_rsrc_t s1r = ks_rsrc(sval1, (int64_t)nsval1 * 4);
   const uint32_t r8 = (uint32_t)r * 8u;
   KsAcc2 acc{0.0, 0.0, 0.0, 0.0, 0.f, 0.f, 0.f, 0.f};
   if (WIN) {
@@ -444,7 +469,8 @@ __global__ void __launch_bounds__(WIN ? KS2_WIN / 2 : 256) k_ks2_stage(
     ks_run<U, true>(Ks2Ops<KS_UNI, true>{meta + k0, r8, xr, vr}, ku - k0, acc);
   }
   ks_run<U, true>(Ks2Ops<KS_UNI>{meta + ku, r8, xr, vr}, km - ku, acc);
-  ks_run<U, true>(Ks2Ops<KS_SCA>{meta + km, r8, xr, sr}, kc - km, acc);
+  ks_run<U, true>(Ks2Ops<KS_SCA>{meta + km, r8, xr, sr}, ke - km, acc);
+  ks_run<U, true>(Ks2Ops<KS_SC1>{meta + ke, r8, xr, s1r}, kc - ke, acc);
   ks_run<U, true>(Ks2Ops<KS_CPX>{meta + kc, r8, xr, vr}, kend - kc, acc);
   if (r >= n) return;
   const bool two = r + 1 < n;
@@ -495,16 +521,19 @@ int ks2_build(qh_mat *A) {
     return QH_OK;
   }
   const int32_t ns = (int32_t)((A->rows + 127) / 128);
-  int32_t *cnt = nullptr, *voff = nullptr, *svoff = nullptr;
+  const int NC = 9;  // per-slice counts of k_ks2_build
+  int32_t *cnt = nullptr, *voff = nullptr, *svoff = nullptr, *s1off = nullptr;
   int *ovf = nullptr;
-  int rc = qh_alloc_t(&cnt, (size_t)7 * ns);
+  int rc = qh_alloc_t(&cnt, (size_t)NC * ns);
   if (!rc) rc = qh_alloc_t(&voff, (size_t)ns + 1);
   if (!rc) rc = qh_alloc_t(&svoff, (size_t)ns + 1);
+  if (!rc) rc = qh_alloc_t(&s1off, (size_t)ns + 1);
   if (!rc) rc = qh_alloc_t(&ovf, 1);
   auto cleanup = [&]() {
     qh_free_ptr(cnt);
     qh_free_ptr(voff);
     qh_free_ptr(svoff);
+    qh_free_ptr(s1off);
     qh_free_ptr(ovf);
   };
   if (rc) {
@@ -513,30 +542,34 @@ int ks2_build(qh_mat *A) {
   }
   QH_HIP(hipMemsetAsync(ovf, 0, sizeof(int), s));
   hipLaunchKernelGGL((k_ks2_build<false>), dim3(ns), dim3(64), 0, s, (int32_t)A->rows, A->rowptr, A->col, A->val,
-                     cnt, ns, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, ovf);
+                     cnt, ns, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, ovf);
   QH_HIP(hipGetLastError());
   int hovf = 0;
-  std::vector<int32_t> hc((size_t)7 * ns);
+  std::vector<int32_t> hc((size_t)NC * ns);
   QH_HIP(hipMemcpyAsync(&hovf, ovf, sizeof(int), hipMemcpyDeviceToHost, s));
-  QH_HIP(hipMemcpyAsync(hc.data(), cnt, sizeof(int32_t) * 7 * ns, hipMemcpyDeviceToHost, s));
+  QH_HIP(hipMemcpyAsync(hc.data(), cnt, sizeof(int32_t) * NC * ns, hipMemcpyDeviceToHost, s));
   QH_HIP(hipStreamSynchronize(s));
   const int32_t *hw = hc.data(), *hn = hw + ns, *hu = hn + ns, *hs = hu + ns, *hsv = hs + ns, *hst = hsv + ns,
-                *hwn = hst + ns;
-  std::vector<int32_t> ho(ns + 1), hv(ns + 1), hso(ns + 1), hm(ns), hcm(ns), hsm(ns), hwm(ns);
-  int64_t tot = 0, totv = 0, tots = 0;
+                *hwn = hst + ns, *hs1 = hwn + ns, *hs1v = hs1 + ns;
+  std::vector<int32_t> ho(ns + 1), hv(ns + 1), hso(ns + 1), hs1o(ns + 1), hm(ns), hem(ns), hcm(ns), hsm(ns), hwm(ns);
+  int64_t tot = 0, totv = 0, tots = 0, tots1 = 0;
   for (int32_t i = 0; i < ns; i++) {
     ho[i] = (int32_t)tot;
     hv[i] = (int32_t)totv;
     hso[i] = (int32_t)tots;
+    hs1o[i] = (int32_t)tots1;
     hwm[i] = (int32_t)(tot + hwn[i]);
     hsm[i] = (int32_t)(tot + hst[i]);
     hm[i] = (int32_t)(tot + hu[i]);
-    hcm[i] = (int32_t)(tot + hu[i] + hs[i]);
+    hem[i] = (int32_t)(tot + hu[i] + hs[i]);
+    hcm[i] = (int32_t)(tot + hu[i] + hs[i] + hs1[i]);
     tot += hw[i];
     totv += hn[i];
     tots += hsv[i];
+    tots1 += hs1v[i];
   }
-  if (hovf || totv * 16 >= ((int64_t)1 << 31) || tots * 8 >= ((int64_t)1 << 31) || tot >= ((int64_t)1 << 31)) {
+  if (hovf || totv * 16 >= ((int64_t)1 << 31) || tots * 8 >= ((int64_t)1 << 31) ||
+      tots1 * 4 >= ((int64_t)1 << 31) || tot >= ((int64_t)1 << 31)) {
     A->ks2_state = -1;
     cleanup();
     return QH_OK;
@@ -544,14 +577,17 @@ int ks2_build(qh_mat *A) {
   ho[ns] = (int32_t)tot;
   hv[ns] = (int32_t)totv;
   hso[ns] = (int32_t)tots;
+  hs1o[ns] = (int32_t)tots1;
   rc = qh_alloc_t(&A->ks2_moff, (size_t)ns + 1);
   if (!rc) rc = qh_alloc_t(&A->ks2_wmid, (size_t)ns);
   if (!rc) rc = qh_alloc_t(&A->ks2_smid, (size_t)ns);
   if (!rc) rc = qh_alloc_t(&A->ks2_umid, (size_t)ns);
+  if (!rc) rc = qh_alloc_t(&A->ks2_emid, (size_t)ns);
   if (!rc) rc = qh_alloc_t(&A->ks2_cmid, (size_t)ns);
   if (!rc) rc = qh_alloc_t(&A->ks2_meta, (size_t)(tot > 0 ? tot : 1));
   if (!rc) rc = qh_alloc_t(&A->ks2_val, (size_t)(totv > 0 ? totv : 1));
   if (!rc) rc = qh_alloc_t(&A->ks2_sval, (size_t)(tots > 0 ? tots : 1));
+  if (!rc) rc = qh_alloc_t(&A->ks2_sval1, (size_t)(tots1 > 0 ? tots1 : 1));
   if (rc) {
     cleanup();
     return rc;
@@ -560,11 +596,14 @@ int ks2_build(qh_mat *A) {
   QH_HIP(hipMemcpyAsync(A->ks2_wmid, hwm.data(), sizeof(int32_t) * ns, hipMemcpyHostToDevice, s));
   QH_HIP(hipMemcpyAsync(A->ks2_smid, hsm.data(), sizeof(int32_t) * ns, hipMemcpyHostToDevice, s));
   QH_HIP(hipMemcpyAsync(A->ks2_umid, hm.data(), sizeof(int32_t) * ns, hipMemcpyHostToDevice, s));
+  QH_HIP(hipMemcpyAsync(A->ks2_emid, hem.data(), sizeof(int32_t) * ns, hipMemcpyHostToDevice, s));
   QH_HIP(hipMemcpyAsync(A->ks2_cmid, hcm.data(), sizeof(int32_t) * ns, hipMemcpyHostToDevice, s));
   QH_HIP(hipMemcpyAsync(voff, hv.data(), sizeof(int32_t) * (ns + 1), hipMemcpyHostToDevice, s));
   QH_HIP(hipMemcpyAsync(svoff, hso.data(), sizeof(int32_t) * (ns + 1), hipMemcpyHostToDevice, s));
+  QH_HIP(hipMemcpyAsync(s1off, hs1o.data(), sizeof(int32_t) * (ns + 1), hipMemcpyHostToDevice, s));
   hipLaunchKernelGGL((k_ks2_build<true>), dim3(ns), dim3(64), 0, s, (int32_t)A->rows, A->rowptr, A->col, A->val,
-                     cnt, ns, A->ks2_moff, voff, svoff, A->ks2_meta, A->ks2_val, A->ks2_sval, ovf);
+                     cnt, ns, A->ks2_moff, voff, svoff, s1off, A->ks2_meta, A->ks2_val, A->ks2_sval, A->ks2_sval1,
+                     ovf);
   QH_HIP(hipGetLastError());
   QH_HIP(hipStreamSynchronize(s));
   cleanup();
@@ -572,6 +611,7 @@ int ks2_build(qh_mat *A) {
   A->ks2_slots = tot;
   A->ks2_nval = totv;
   A->ks2_nsval = tots;
+  A->ks2_nsval1 = tots1;
   A->ks2_state = 1;
   return QH_OK;
 }
@@ -590,8 +630,8 @@ int ks2_launch(int G, int mode, hipStream_t s, const qh_mat *A, const void *xin,
   const dim3 grid(qh_blocks(ns, win ? KS2_WIN / 128 : 4)), blk(win ? KS2_WIN / 2 : 256);
 #define QH_LK2(MM, UU, WW)                                                                                       \
   hipLaunchKernelGGL((k_ks2_stage<MM, UU, WW>), grid, blk, 0, s, n, ns, A->ks2_moff, A->ks2_wmid, A->ks2_smid,  \
-                     A->ks2_umid, A->ks2_cmid,                                                                  \
-                     A->ks2_meta, (int32_t)A->ks2_nval, A->ks2_val, (int32_t)A->ks2_nsval, A->ks2_sval,          \
+                     A->ks2_umid, A->ks2_emid, A->ks2_cmid, A->ks2_meta, (int32_t)A->ks2_nval, A->ks2_val,       \
+                     (int32_t)A->ks2_nsval, A->ks2_sval, (int32_t)A->ks2_nsval1, A->ks2_sval1,                  \
                      (const float2 *)xin, (float2 *)kout, (float2 *)ynext, cn, (const float2 *)x,                \
                      (const float2 *)k1, (const float2 *)k2, (const float2 *)k3, (float2 *)xout, h, xlo)
   if (!QH_IS_KS2(G) || mode < MODE_SPMV || mode > MODE_FINAL) {

@@ -204,10 +204,12 @@ def _pair_slot_classes(rp, col, val):
     that leaves the vector.  Slots of a 128-row slice are the distinct
     (|b - 2l|, h0, h1, rank) over its lanes; a slot is uniform when every active
     lane holds bit-identical (value row 2l, value row 2l+1) (zero for an absent
-    row), scalar when all those are real or all imaginary, else complex.
-    Returns (slots, complex pairs stored, real pairs stored)."""
+    row), scalar when all those are real or all imaginary, else complex; a
+    scalar slot stores one real per lane when every lane's two rows hold the same
+    FP32 value (an absent row holds zero), else a real pair per lane.
+    Returns (slots, complex pairs stored, real pairs stored, single reals stored)."""
     n = len(rp) - 1
-    slots = ncpx = nreal = 0
+    slots = ncpx = nreal = nsingle = 0
     z = np.complex128(0)
     for s0 in range(0, n, 128):
         per_slot = {}
@@ -255,11 +257,16 @@ def _pair_slot_classes(rp, col, val):
             bits = {(v0.real.tobytes(), v0.imag.tobytes(), v1.real.tobytes(), v1.imag.tobytes()) for v0, v1 in vs}
             if len(bits) == 1:
                 continue
-            if all(v0.imag == 0 and v1.imag == 0 for v0, v1 in vs) or all(v0.real == 0 and v1.real == 0 for v0, v1 in vs):
-                nreal += len(vs)
+            isre = all(v0.imag == 0 and v1.imag == 0 for v0, v1 in vs)
+            if isre or all(v0.real == 0 and v1.real == 0 for v0, v1 in vs):
+                part = (lambda z: np.float32(z.real)) if isre else (lambda z: np.float32(z.imag))
+                if all(part(v0) == part(v1) for v0, v1 in vs):
+                    nsingle += len(vs)
+                else:
+                    nreal += len(vs)
             else:
                 ncpx += len(vs)
-    return slots, ncpx, nreal
+    return slots, ncpx, nreal, nsingle
 
 
 @pytest.mark.parametrize("name", ["odd", "tc3"])
@@ -282,7 +289,11 @@ def test_row_pair_layout_counts(name):
         keep.append(b)
     rp, col, val = A.export()
     lay = A.pair_layout_info()
-    assert (lay["slots"], lay["stored_pairs"], lay["stored_real_pairs"]) == _pair_slot_classes(rp, col, val)
+    got = (lay["slots"], lay["stored_pairs"], lay["stored_real_pairs"], lay["stored_real_singles"])
+    want = _pair_slot_classes(rp, col, val)
+    assert got == want, (got, want)
+    if name == "tc3":
+        assert want[3] > 0   # the Tavis-Cummings batch has equal-scalar slots
     for k in keep:
         k.destroy()
 
