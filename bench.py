@@ -62,9 +62,10 @@ def kernel_name(variant, real):
                      133: (2, 0, 1), 134: (4, 0, 1)}[variant]
         tf = ("false", "true")
         return f"k_ks_stage<{real}, MODE, {u}, {tf[vv]}, {tf[pp]}> (sliced layout)"
-    if variant in (140, 141, 142):
-        u, w = {140: (2, "false"), 141: (4, "false"), 142: (2, "true")}[variant]
-        return f"k_ks2_stage<MODE, {u}, {w}> (row-pair sliced layout)"
+    if variant in (140, 141, 142, 143, 144):
+        u, w, nt = {140: (2, "false", "false"), 141: (4, "false", "false"), 142: (2, "true", "false"),
+                    143: (2, "false", "true"), 144: (2, "true", "true")}[variant]
+        return f"k_ks2_stage<MODE, {u}, {w}, {nt}> (row-pair sliced layout)"
     return f"k_rk_stage<{real}, {variant}> (CSR, {variant} lanes per row)"
 
 
@@ -75,7 +76,7 @@ def format_step_bytes(A, rows):
     stage 4: the gathered input, the state's high and low words, K1..K3, and the
     new high and low words written); and the layout statistics"""
     lanes = A.lanes()
-    if lanes[1] in (140, 141, 142):  # row-pair layout: 40-byte slot metadata, values per lane pair
+    if lanes[1] in (140, 141, 142, 143, 144):  # row-pair layout: 40-byte slot metadata, values per lane pair
         layout = dict(A.pair_layout_info(), kind="row-pair (128 rows per wavefront)")
         meta = layout["meta_bytes"]
         vals = layout["stored_pairs"] * 16 + layout["stored_real_pairs"] * 8 + layout["stored_real_singles"] * 4

@@ -75,7 +75,8 @@ int qh_mat_blockdiag(int n, qh_mat *const *mats, qh_mat **out);
  * and the next pass's loads in flight); FP32 only: 140/141/142 = row-pair sliced
  * layout (one wavefront per 128 rows, two rows per lane, 16-byte loads; 2/4/2
  * slots per pass, 142 reading the slots that stay inside each 2048-row window
- * from an LDS copy); 0 = autotune */
+ * from an LDS copy; 143/144 = 140/142 with the streamed data under the
+ * non-temporal cache policy); 0 = autotune */
 int qh_mat_set_lanes(qh_mat *m, int lanes64, int lanes32);
 int qh_mat_get_lanes(const qh_mat *m, int *lanes64, int *lanes32);
 /* sliced layout statistics (built on demand): slices, slots, stored complex

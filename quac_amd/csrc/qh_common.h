@@ -131,8 +131,9 @@ struct qh_mat {
 #define QH_KSELL 128
 #define QH_IS_KSELL(g) ((g) >= 128 && (g) <= 134)
 // 140 / 141: the FP32 row-pair sliced layout, 2 / 4 slots per pass (pipelined);
-// 142: 2 slots per pass, the slots inside each 2048-row window read from LDS
-#define QH_IS_KS2(g) ((g) >= 140 && (g) <= 142)
+// 142: 2 slots per pass, the slots inside each 2048-row window read from LDS;
+// 143 / 144: 140 / 142 with the streamed data under the non-temporal cache policy
+#define QH_IS_KS2(g) ((g) >= 140 && (g) <= 144)
 
 struct qh_vec {
   int64_t n = 0;

@@ -256,7 +256,7 @@ struct Ks2Buf {
 
 // INORDER: every slot of the group has row 2l on half 0 and row 2l+1 on half 1
 // (known at compile time, no selects)
-template <int KIND, bool INORDER = false>
+template <int KIND, bool INORDER = false, int AUX = 0>
 struct Ks2Ops {
   typedef KsAcc2 Acc;
   template <int W>
@@ -290,12 +290,12 @@ struct Ks2Ops {
         const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(m[j].mask >> 32),
                                                         __builtin_amdgcn_mbcnt_lo((uint32_t)m[j].mask, 0u));
         if (KIND == KS_SC1)
-          b.s1[j] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(vr, (m[j].base + rank) * 4u, 0, 0));
+          b.s1[j] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(vr, (m[j].base + rank) * 4u, 0, AUX));
         else if (KIND == KS_SCA)
-          b.sv[j] = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(vr, (m[j].base + rank) * 8u, 0, 0));
+          b.sv[j] = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(vr, (m[j].base + rank) * 8u, 0, AUX));
         else
           b.vv[j] =
-              __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(vr, (m[j].base + rank) * 16u, 0, 0));
+              __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(vr, (m[j].base + rank) * 16u, 0, AUX));
       }
     }
     mp += W;
@@ -402,19 +402,24 @@ struct Ks2OpsWin {
 };
 
 // the two complex64 of rows (r, r+1) at byte offset o of a vector; the second is
-// zero (and not loaded / stored) when r is the last row
+// zero (and not loaded / stored) when r is the last row.  AUX: the cache policy
+// bits of the buffer instruction (0, or 2 = non-temporal: streamed data that is
+// not read again soon, so it does not displace the gathered stage input from
+// the caches)
+template <int AUX = 0>
 __device__ inline float4 ld_pair(__amdgpu_buffer_rsrc_t rs, uint32_t o, bool two) {
-  if (two) return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rs, o, 0, 0));
-  const float2 a = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(rs, o, 0, 0));
+  if (two) return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rs, o, 0, AUX));
+  const float2 a = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(rs, o, 0, AUX));
   return make_float4(a.x, a.y, 0.f, 0.f);
 }
+template <int AUX = 0>
 __device__ inline void st_pair(__amdgpu_buffer_rsrc_t rs, uint32_t o, bool two, float4 v) {
   if (two)
     __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(__attribute__((ext_vector_type(4))) unsigned, v), rs, o,
-                                           0, 0);
+                                           0, AUX);
   else
     __builtin_amdgcn_raw_buffer_store_b64(
-        __builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned, make_float2(v.x, v.y)), rs, o, 0, 0);
+        __builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned, make_float2(v.x, v.y)), rs, o, 0, AUX);
 }
 
 //   MODE_SPMV : kout = A xin
@@ -423,7 +428,10 @@ __device__ inline void st_pair(__amdgpu_buffer_rsrc_t rs, uint32_t o, bool two, 
 // with the FP32 state x = hi + lo updated in FP64 and split again.
 // WIN: workgroups of KS2_WIN / 128 wavefronts; each first copies its window of
 // the stage input into LDS and reads the window slots from there.
-template <int MODE, int U, bool WIN>
+// NT: the matrix values, the state words, K1..K3 and the K and low-word stores
+// are streamed with the non-temporal cache policy (the gathered stage input and
+// the next stage input Y keep the default policy)
+template <int MODE, int U, bool WIN, bool NT = false>
 __global__ void __launch_bounds__(WIN ? KS2_WIN / 2 : 256) k_ks2_stage(
     int32_t n, int32_t nslices, const int32_t *__restrict__ moff, const int32_t *__restrict__ wmid,
     const int32_t *__restrict__ smid, const int32_t *__restrict__ umid, const int32_t *__restrict__ emid,
@@ -468,10 +476,11 @@ __global__ void __launch_bounds__(WIN ? KS2_WIN / 2 : 256) k_ks2_stage(
   } else {
     ks_run<U, true>(Ks2Ops<KS_UNI, true>{meta + k0, r8, xr, vr}, ku - k0, acc);
   }
+  constexpr int AUX = NT ? 2 : 0;
   ks_run<U, true>(Ks2Ops<KS_UNI>{meta + ku, r8, xr, vr}, km - ku, acc);
-  ks_run<U, true>(Ks2Ops<KS_SCA>{meta + km, r8, xr, sr}, ke - km, acc);
-  ks_run<U, true>(Ks2Ops<KS_SC1>{meta + ke, r8, xr, s1r}, kc - ke, acc);
-  ks_run<U, true>(Ks2Ops<KS_CPX>{meta + kc, r8, xr, vr}, kend - kc, acc);
+  ks_run<U, true>(Ks2Ops<KS_SCA, false, AUX>{meta + km, r8, xr, sr}, ke - km, acc);
+  ks_run<U, true>(Ks2Ops<KS_SC1, false, AUX>{meta + ke, r8, xr, s1r}, kc - ke, acc);
+  ks_run<U, true>(Ks2Ops<KS_CPX, false, AUX>{meta + kc, r8, xr, vr}, kend - kc, acc);
   if (r >= n) return;
   const bool two = r + 1 < n;
   const float4 y = make_float4((float)acc.r0, (float)acc.i0, (float)acc.r1, (float)acc.i1);
@@ -479,13 +488,13 @@ __global__ void __launch_bounds__(WIN ? KS2_WIN / 2 : 256) k_ks2_stage(
     st_pair(ks_rsrc(kout, vb), r8, two, y);
     return;
   }
-  const float4 xo = ld_pair(ks_rsrc(x, vb), r8, two);
+  const float4 xo = ld_pair<AUX>(ks_rsrc(x, vb), r8, two);
   if (MODE == MODE_STAGE) {
     // the next stage input is stored in complex64: the low word of the state
     // (below half an ulp of the high word) would change it by at most one
     // rounding, so only the high word is read here (the update in stage 4
     // uses both)
-    st_pair(ks_rsrc(kout, vb), r8, two, y);
+    st_pair<AUX>(ks_rsrc(kout, vb), r8, two, y);
     const double c = (double)cn;
     const float4 yn = make_float4((float)((double)xo.x + c * acc.r0), (float)((double)xo.y + c * acc.i0),
                                   (float)((double)xo.z + c * acc.r1), (float)((double)xo.w + c * acc.i1));
@@ -493,9 +502,9 @@ __global__ void __launch_bounds__(WIN ? KS2_WIN / 2 : 256) k_ks2_stage(
     return;
   }
   const __amdgpu_buffer_rsrc_t lr = ks_rsrc(xlo, vb);
-  const float4 lo = ld_pair(lr, r8, two);
-  const float4 a = ld_pair(ks_rsrc(k1, vb), r8, two), bq = ld_pair(ks_rsrc(k2, vb), r8, two),
-               cq = ld_pair(ks_rsrc(k3, vb), r8, two);
+  const float4 lo = ld_pair<AUX>(lr, r8, two);
+  const float4 a = ld_pair<AUX>(ks_rsrc(k1, vb), r8, two), bq = ld_pair<AUX>(ks_rsrc(k2, vb), r8, two),
+               cq = ld_pair<AUX>(ks_rsrc(k3, vb), r8, two);
   const double s6 = (double)h * (1.0 / 6.0), s3 = (double)h * (1.0 / 3.0);
   const double n0r = ((double)xo.x + (double)lo.x) + (s6 * (double)a.x + s3 * (double)bq.x + s3 * (double)cq.x + s6 * acc.r0);
   const double n0i = ((double)xo.y + (double)lo.y) + (s6 * (double)a.y + s3 * (double)bq.y + s3 * (double)cq.y + s6 * acc.i0);
@@ -503,7 +512,7 @@ __global__ void __launch_bounds__(WIN ? KS2_WIN / 2 : 256) k_ks2_stage(
   const double n1i = ((double)xo.w + (double)lo.w) + (s6 * (double)a.w + s3 * (double)bq.w + s3 * (double)cq.w + s6 * acc.i1);
   const float4 o = make_float4((float)n0r, (float)n0i, (float)n1r, (float)n1i);
   st_pair(ks_rsrc(xout, vb), r8, two, o);
-  st_pair(lr, r8, two,
+  st_pair<AUX>(lr, r8, two,
           make_float4((float)(n0r - (double)o.x), (float)(n0i - (double)o.y), (float)(n1r - (double)o.z),
                       (float)(n1i - (double)o.w)));
 }
@@ -617,7 +626,8 @@ int ks2_build(qh_mat *A) {
 }
 
 // One stage launch on the row-pair layout (G = 140 / 141: 2 / 4 slots per pass;
-// 142: 2 slots per pass, window slots read from LDS);
+// 142: 2 slots per pass, window slots read from LDS; 143 / 144: 140 / 142 with
+// the non-temporal cache policy on the streamed data);
 // mode as in the one-row-per-lane kernels.
 int ks2_launch(int G, int mode, hipStream_t s, const qh_mat *A, const void *xin, void *kout, void *ynext, float cn,
                const void *x, const void *k1, const void *k2, const void *k3, void *xout, float h, float2 *xlo) {
@@ -626,10 +636,10 @@ int ks2_launch(int G, int mode, hipStream_t s, const qh_mat *A, const void *xin,
     return QH_EINVAL;
   }
   const int32_t n = (int32_t)A->rows, ns = A->ks2_nslices;
-  const bool win = G == 142;
+  const bool win = G == 142 || G == 144;
   const dim3 grid(qh_blocks(ns, win ? KS2_WIN / 128 : 4)), blk(win ? KS2_WIN / 2 : 256);
-#define QH_LK2(MM, UU, WW)                                                                                       \
-  hipLaunchKernelGGL((k_ks2_stage<MM, UU, WW>), grid, blk, 0, s, n, ns, A->ks2_moff, A->ks2_wmid, A->ks2_smid,  \
+#define QH_LK2(MM, UU, WW, NN)                                                                                   \
+  hipLaunchKernelGGL((k_ks2_stage<MM, UU, WW, NN>), grid, blk, 0, s, n, ns, A->ks2_moff, A->ks2_wmid, A->ks2_smid,  \
                      A->ks2_umid, A->ks2_emid, A->ks2_cmid, A->ks2_meta, (int32_t)A->ks2_nval, A->ks2_val,       \
                      (int32_t)A->ks2_nsval, A->ks2_sval, (int32_t)A->ks2_nsval1, A->ks2_sval1,                  \
                      (const float2 *)xin, (float2 *)kout, (float2 *)ynext, cn, (const float2 *)x,                \
@@ -638,13 +648,22 @@ int ks2_launch(int G, int mode, hipStream_t s, const qh_mat *A, const void *xin,
     qh_set_error("unsupported row-pair variant %d / mode %d", G, mode);
     return QH_EINVAL;
   }
-  if (mode == MODE_SPMV) {
-    if (G == 140) QH_LK2(MODE_SPMV, 2, false); else if (G == 141) QH_LK2(MODE_SPMV, 4, false); else QH_LK2(MODE_SPMV, 2, true);
-  } else if (mode == MODE_STAGE) {
-    if (G == 140) QH_LK2(MODE_STAGE, 2, false); else if (G == 141) QH_LK2(MODE_STAGE, 4, false); else QH_LK2(MODE_STAGE, 2, true);
-  } else {
-    if (G == 140) QH_LK2(MODE_FINAL, 2, false); else if (G == 141) QH_LK2(MODE_FINAL, 4, false); else QH_LK2(MODE_FINAL, 2, true);
+#define QH_LK2G(MM)                      \
+  switch (G) {                           \
+    case 140: QH_LK2(MM, 2, false, false); break; \
+    case 141: QH_LK2(MM, 4, false, false); break; \
+    case 142: QH_LK2(MM, 2, true, false); break;  \
+    case 143: QH_LK2(MM, 2, false, true); break;  \
+    default: QH_LK2(MM, 2, true, true); break;    \
   }
+  if (mode == MODE_SPMV) {
+    QH_LK2G(MODE_SPMV)
+  } else if (mode == MODE_STAGE) {
+    QH_LK2G(MODE_STAGE)
+  } else {
+    QH_LK2G(MODE_FINAL)
+  }
+#undef QH_LK2G
 #undef QH_LK2
   QH_HIP(hipGetLastError());
   return QH_OK;

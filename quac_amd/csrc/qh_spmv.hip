@@ -966,7 +966,7 @@ int autotune(qh_mat *A, int *out_g) {
       brc = ks2_build(A);
       if (brc) return brc;
       if (A->ks2_state == 1)
-        for (int u = 140; u <= 142; u++) cands.push_back(u);
+        for (int u = 140; u <= 144; u++) cands.push_back(u);
     }
   }
   for (int g : {2, 4, 8, 16, 32, 64})
@@ -1545,7 +1545,7 @@ extern "C" int qh_mat_set_lanes(qh_mat *m, int l64, int l32) {
     return g == 0 || g == 1 || g == 2 || g == 4 || g == 8 || g == 16 || g == 32 || g == 64 || QH_IS_KSELL(g);
   };
   QH_REQUIRE(ok(l64) && (ok(l32) || QH_IS_KS2(l32)), QH_EINVAL,
-             "lanes per row must be a power of two <= 64, 128..134 (sliced layout), or for FP32 140..142 (row-pair "
+             "lanes per row must be a power of two <= 64, 128..134 (sliced layout), or for FP32 140..144 (row-pair "
              "sliced layout)");
   m->lanes64 = l64;
   m->lanes32 = l32;

@@ -127,8 +127,8 @@ def test_rk4_device_direct_matches_oracle_and_lanes():
     x0 = np.zeros(N * N, np.complex128)
     x0[loc] = 1
     ref = _oracle_run(r, x0, 0.01, 37)
-    # 128..134 = sliced layout; 140..142 = FP32 row-pair sliced layout (FP64 keeps 8 lanes per row there)
-    for g in (1, 2, 4, 8, 16, 32, 64, 128, 129, 130, 131, 132, 133, 134, 140, 141, 142):
+    # 128..134 = sliced layout; 140..144 = FP32 row-pair sliced layout (FP64 keeps 8 lanes per row there)
+    for g in (1, 2, 4, 8, 16, 32, 64, 128, 129, 130, 131, 132, 133, 134, 140, 141, 142, 143, 144):
         A.set_lanes(g if g < 140 else 8, g)
         v = qa.DeviceVec(N * N, qa.FP64)
         v.set(x0)
@@ -153,7 +153,7 @@ def _odd_recipe():
 
 @pytest.mark.parametrize("name", ["chain10", "tc_batch", "odd"])
 def test_row_pair_layout_fp32_spmv(name):
-    """FP32 SpMV on the row-pair sliced layout (140 / 141 / 142) against the FP64 CSR
+    """FP32 SpMV on the row-pair sliced layout (140 .. 144) against the FP64 CSR
     product: every row within 1e-6 of sum_k |a_k| |x_k| (complex64 values and
     inputs, FP32 pass sums)"""
     rng = np.random.default_rng(7)
@@ -184,7 +184,7 @@ def test_row_pair_layout_fp32_spmv(name):
     np.add.at(ref, rows, val * x[col])
     mag = np.zeros(n)
     np.add.at(mag, rows, np.abs(val) * np.abs(x[col]))
-    for g in (140, 141, 142):
+    for g in (140, 141, 142, 143, 144):
         A.set_lanes(8, g)
         yv = qa.DeviceVec(n, qa.FP32)
         qa._qh(qa.lib().qh_spmv(A.h, xv.h, yv.h))
@@ -308,7 +308,7 @@ def test_row_pair_layout_rk4_odd_rows():
     x0 = np.zeros(N * N, np.complex128)
     x0[7 * N + 7] = 1.0
     ref = _oracle_run(r, x0, 0.01, 200)
-    for g in (140, 141, 142):
+    for g in (140, 141, 142, 143, 144):
         A.set_lanes(8, g)
         w = qa.DeviceVec(N * N, qa.FP32)
         w.set(x0)
