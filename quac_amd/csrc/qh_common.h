@@ -84,7 +84,12 @@ struct qh_kmeta2 {
 struct qh_td_cache;  // merged time-dependent matrices and stage buffers (qh_td.hip)
 void qh_td_cache_free(qh_td_cache *c);
 
+// a process-wide serial number for every qh_mat created (caches keyed on a
+// matrix compare it with the pointer, which a new matrix may reuse)
+uint64_t qh_next_mat_uid();
+
 struct qh_mat {
+  uint64_t uid = qh_next_mat_uid();
   int64_t rows = 0, nnz = 0;
   int32_t *rowptr = nullptr;   // rows+1
   int32_t *col = nullptr;      // nnz

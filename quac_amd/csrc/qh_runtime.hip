@@ -5,6 +5,8 @@
 
 #include <algorithm>
 
+#include <atomic>
+
 #include "qh_common.h"
 
 static thread_local char g_err[1024] = "";
@@ -49,6 +51,11 @@ extern "C" int qh_init(int device) {
 }
 
 int qh_ensure_init() { return g_init ? QH_OK : qh_init(0); }
+
+uint64_t qh_next_mat_uid() {
+  static std::atomic<uint64_t> next{1};
+  return next.fetch_add(1, std::memory_order_relaxed);
+}
 hipStream_t qh_stream() { return g_stream; }
 int qh_num_cus() { return g_cus; }
 

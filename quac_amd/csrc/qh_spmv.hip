@@ -1095,6 +1095,7 @@ __global__ void k_scale_by_dev(float2 *hi, float2 *lo, int64_t n, const double *
 // ------------------------------------------------------------ stepper state
 struct Stepper {
   const qh_mat *A = nullptr;
+  uint64_t Auid = 0;  // A's serial number: a new matrix may reuse a destroyed one's address
   int prec = 0;
   int64_t n = 0;
   void *k[3] = {nullptr, nullptr, nullptr};
@@ -1128,7 +1129,7 @@ static void stepper_release() {
 }
 
 static int stepper_prepare(const qh_mat *A, int prec, int64_t n) {
-  if (g_st.A == A && g_st.prec == prec && g_st.n == n && g_st.k[0]) return QH_OK;
+  if (g_st.A == A && g_st.Auid == A->uid && g_st.prec == prec && g_st.n == n && g_st.k[0]) return QH_OK;
   (void)hipStreamSynchronize(qh_stream());
   stepper_release();
   size_t es = prec == QH_FP64 ? 16 : 8;
@@ -1145,6 +1146,7 @@ static int stepper_prepare(const qh_mat *A, int prec, int64_t n) {
     return rc;
   }
   g_st.A = A;
+  g_st.Auid = A->uid;
   g_st.prec = prec;
   g_st.n = n;
   return QH_OK;

@@ -122,6 +122,7 @@ __global__ void k_narrow_td(const double2 *a, float2 *b, int64_t n) {
 struct qh_td_cache {
   int ntd = 0;
   std::vector<const qh_mat *> ak;  // the matrices it was built from
+  std::vector<uint64_t> akid;      // and their serial numbers
   int64_t rows = 0, tnnz = 0;
   int32_t max_row = 0;
   int32_t *trp = nullptr, *tcol = nullptr, *tk = nullptr;
@@ -163,7 +164,7 @@ void qh_td_cache_free(qh_td_cache *c) {
 static int td_cache_get(qh_mat *A, int ntd, qh_mat *const *Ak, qh_td_cache **out) {
   qh_td_cache *c = A->td_cache;
   bool ok = c && c->ntd == ntd;
-  for (int k = 0; ok && k < ntd; k++) ok = c->ak[k] == Ak[k];
+  for (int k = 0; ok && k < ntd; k++) ok = c->ak[k] == Ak[k] && Ak[k] && c->akid[k] == Ak[k]->uid;
   if (ok) {
     *out = c;
     return QH_OK;
@@ -208,6 +209,8 @@ static int td_cache_get(qh_mat *A, int ntd, qh_mat *const *Ak, qh_td_cache **out
   c = new qh_td_cache();
   c->ntd = ntd;
   c->ak.assign(Ak, Ak + ntd);
+  c->akid.resize(ntd);
+  for (int k = 0; k < ntd; k++) c->akid[k] = Ak[k]->uid;
   c->rows = n;
   c->tnnz = tot;
   c->max_row = mr;

@@ -118,6 +118,32 @@ def test_spin_chain10_full_size_vs_oracle():
         qa.set_option("quac_precision", "fp64")
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_rk4_after_a_matrix_is_destroyed_and_another_created(prec):
+    """Two chains of the same size with different couplings, the second system
+    built right after the first is destroyed (its matrix object may take the
+    freed address): the step graph and stage buffers kept between calls must
+    follow the new matrix, so each run matches the oracle for its own system."""
+    _fixed_rk4()
+    pr = qa.FP64 if prec == "fp64" else qa.FP32
+    tol = 1e-10 if prec == "fp64" else 1e-6
+    N, loc = recipes.spin_chain_init(5)
+    x0 = np.zeros(N * N, np.complex128)
+    x0[loc] = 1
+    for seed in (1, 2, 3):
+        r = recipes.spin_chain_recipe(5, seed=seed)
+        qs, _, _ = build_system(r)
+        A = qs.matrix()
+        v = qa.DeviceVec(N * N, pr)
+        v.set(x0)
+        qa.rk4_device(A, v, 0.01, 20, tot=N, renorm_every=10 if pr == qa.FP32 else 0)
+        ref = _oracle_run(r, x0, 0.01, 20)
+        err = np.abs(populations_np(v.get(), r) - populations_np(ref, r)).max()
+        assert err < tol, (seed, prec, err)
+        del v
+        qs.destroy()
+
+
 def test_rk4_device_direct_matches_oracle_and_lanes():
     """qh_rk4 on the assembled matrix for every lanes-per-row variant."""
     r = recipes.spin_chain_recipe(5, seed=3)
