@@ -57,11 +57,12 @@ def rk4_step_bytes(nnz, rows, S):
 
 def kernel_name(variant, real):
     """name of the stage kernel a lanes-per-row variant code selects"""
-    if 128 <= variant <= 134:
-        u, vv, pp = {128: (2, 0, 0), 129: (4, 0, 0), 130: (8, 0, 0), 131: (4, 1, 0), 132: (8, 1, 0),
-                     133: (2, 0, 1), 134: (4, 0, 1)}[variant]
+    if 128 <= variant <= 136:
+        u, vv, pp, nt = {128: (2, 0, 0, 0), 129: (4, 0, 0, 0), 130: (8, 0, 0, 0), 131: (4, 1, 0, 0),
+                         132: (8, 1, 0, 0), 133: (2, 0, 1, 0), 134: (4, 0, 1, 0), 135: (4, 0, 0, 1),
+                         136: (4, 1, 0, 1)}[variant]
         tf = ("false", "true")
-        return f"k_ks_stage<{real}, MODE, {u}, {tf[vv]}, {tf[pp]}> (sliced layout)"
+        return f"k_ks_stage<{real}, MODE, {u}, {tf[vv]}, {tf[pp]}, {tf[nt]}> (sliced layout)"
     if variant in (140, 141, 142, 143, 144):
         u, w, nt = {140: (2, "false", "false"), 141: (4, "false", "false"), 142: (2, "true", "false"),
                     143: (2, "false", "true"), 144: (2, "true", "true")}[variant]

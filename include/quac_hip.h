@@ -70,9 +70,10 @@ int qh_mat_make_fp32(qh_mat *m);
 /* block-diagonal concatenation of n matrices (batched parameter sweeps) */
 int qh_mat_blockdiag(int n, qh_mat *const *mats, qh_mat **out);
 /* select SpMV variant: lanes per row G in {1,2,4,8,16,32,64} of the CSR kernel,
- * 128..134 = slot-aligned sliced ELL (one wavefront per 64 rows; 2/4/8 slots per
+ * 128..136 = slot-aligned sliced ELL (one wavefront per 64 rows; 2/4/8 slots per
  * pass, 131/132 with vector-loaded uniform values, 133/134 with 2/4 slots per pass
- * and the next pass's loads in flight); FP32 only: 140/141/142 = row-pair sliced
+ * and the next pass's loads in flight, 135/136 = 129/131 with the streamed data
+ * under the non-temporal cache policy); FP32 only: 140/141/142 = row-pair sliced
  * layout (one wavefront per 128 rows, two rows per lane, 16-byte loads; 2/4/2
  * slots per pass, 142 reading the slots that stay inside each 2048-row window
  * from an LDS copy; 143/144 = 140/142 with the streamed data under the

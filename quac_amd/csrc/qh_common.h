@@ -134,7 +134,8 @@ struct qh_mat {
 // slot values with vector loads; 133 / 134 process 2 / 4 slots per pass with the
 // loads of the next pass in flight during the products of the current one
 #define QH_KSELL 128
-#define QH_IS_KSELL(g) ((g) >= 128 && (g) <= 134)
+// 135 / 136: 129 / 131 with the streamed data under the non-temporal cache policy
+#define QH_IS_KSELL(g) ((g) >= 128 && (g) <= 136)
 // 140 / 141: the FP32 row-pair sliced layout, 2 / 4 slots per pass (pipelined);
 // 142: 2 slots per pass, the slots inside each 2048-row window read from LDS;
 // 143 / 144: 140 / 142 with the streamed data under the non-temporal cache policy

@@ -200,26 +200,21 @@ __global__ void __launch_bounds__(256) k_rk_stage(
 // together.  Products and sums are formed in FP64 for both storage precisions
 // (FP64 FMA issues at the FP32 rate on CDNA4).
 
-template <typename R>
-__device__ inline typename C2<R>::T ks_load(__amdgpu_buffer_rsrc_t rs, uint32_t off);
-template <>
-__device__ inline float2 ks_load<float>(__amdgpu_buffer_rsrc_t rs, uint32_t off) {
-  return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(rs, off, 0, 0));
-}
-template <>
-__device__ inline double2 ks_load<double>(__amdgpu_buffer_rsrc_t rs, uint32_t off) {
-  return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0));
+// AUX: the cache policy bits of the buffer load (0, or 2 = non-temporal)
+template <typename R, int AUX = 0>
+__device__ inline typename C2<R>::T ks_load(__amdgpu_buffer_rsrc_t rs, uint32_t off) {
+  if constexpr (sizeof(R) == 4)
+    return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(rs, off, 0, AUX));
+  else
+    return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, AUX));
 }
 
-template <typename R>
-__device__ inline R ks_load_real(__amdgpu_buffer_rsrc_t rs, uint32_t off);
-template <>
-__device__ inline float ks_load_real<float>(__amdgpu_buffer_rsrc_t rs, uint32_t off) {
-  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, off, 0, 0));
-}
-template <>
-__device__ inline double ks_load_real<double>(__amdgpu_buffer_rsrc_t rs, uint32_t off) {
-  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, off, 0, 0));
+template <typename R, int AUX = 0>
+__device__ inline R ks_load_real(__amdgpu_buffer_rsrc_t rs, uint32_t off) {
+  if constexpr (sizeof(R) == 4)
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, off, 0, AUX));
+  else
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, off, 0, AUX));
 }
 
 // One pass over U slots of a slice starting at mp, split into the issue of its
@@ -242,7 +237,7 @@ struct KsBuf {
   double ur[U], ui[U];  // uniform value, or the (1, 0) / (0, 1) multiplier of a scalar slot
 };
 
-template <typename R, int KIND, bool TAIL, int U, bool VV>
+template <typename R, int KIND, bool TAIL, int U, bool VV, int AUX = 0>
 __device__ __forceinline__ void ks_issue(const qh_kmeta *__restrict__ mp, int32_t cnt, uint32_t r8,
                                          __amdgpu_buffer_rsrc_t xr, __amdgpu_buffer_rsrc_t vr,
                                          __amdgpu_buffer_rsrc_t mr, int32_t k, KsBuf<R, KIND, U, VV> &b) {
@@ -283,9 +278,9 @@ __device__ __forceinline__ void ks_issue(const qh_kmeta *__restrict__ mp, int32_
       const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask[j] >> 32),
                                                       __builtin_amdgcn_mbcnt_lo((uint32_t)mask[j], 0u));
       if (KIND == KS_SCA)
-        b.sv[j] = ks_load_real<R>(vr, (m[j].base + rank) * (uint32_t)sizeof(R));
+        b.sv[j] = ks_load_real<R, AUX>(vr, (m[j].base + rank) * (uint32_t)sizeof(R));
       else
-        b.vv[j] = ks_load<R>(vr, (m[j].base + rank) * (uint32_t)sizeof(T));
+        b.vv[j] = ks_load<R, AUX>(vr, (m[j].base + rank) * (uint32_t)sizeof(T));
     }
   }
 }
@@ -321,7 +316,7 @@ struct KsBuf32 {
   float ur[U], ui[U];
 };
 
-template <int KIND, bool TAIL, int U>
+template <int KIND, bool TAIL, int U, int AUX = 0>
 __device__ __forceinline__ void ks_issue32(const qh_kmeta32 *__restrict__ mp, int32_t cnt, uint32_t r8,
                                            __amdgpu_buffer_rsrc_t xr, __amdgpu_buffer_rsrc_t vr,
                                            KsBuf32<KIND, U> &b) {
@@ -349,9 +344,9 @@ __device__ __forceinline__ void ks_issue32(const qh_kmeta32 *__restrict__ mp, in
       const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask[j] >> 32),
                                                       __builtin_amdgcn_mbcnt_lo((uint32_t)mask[j], 0u));
       if (KIND == KS_SCA)
-        b.sv[j] = ks_load_real<float>(vr, (m[j].base + rank) * 4u);
+        b.sv[j] = ks_load_real<float, AUX>(vr, (m[j].base + rank) * 4u);
       else
-        b.vv[j] = ks_load<float>(vr, (m[j].base + rank) * 8u);
+        b.vv[j] = ks_load<float, AUX>(vr, (m[j].base + rank) * 8u);
     }
   }
 }
@@ -381,7 +376,7 @@ __device__ __forceinline__ void ks_fma32(const KsBuf32<KIND, U> &b, double &are,
 
 // Issue / multiply policies over the two metadata layouts, so that one group
 // driver serves both; W is the number of slots of a pass.
-template <int KIND>
+template <int KIND, int AUX = 0>
 struct KsOps32 {
   typedef KsAcc Acc;
   template <int W>
@@ -391,7 +386,7 @@ struct KsOps32 {
   __amdgpu_buffer_rsrc_t xr, vr;
   template <int W, bool TAIL = false>
   __device__ __forceinline__ void issue(Buf<W> &b, int32_t cnt = W) {
-    ks_issue32<KIND, TAIL, W>(mp, cnt, r8, xr, vr, b);
+    ks_issue32<KIND, TAIL, W, AUX>(mp, cnt, r8, xr, vr, b);
     mp += W;
   }
   template <int W>
@@ -401,7 +396,7 @@ struct KsOps32 {
   __device__ __forceinline__ void flush(Acc &) {}
 };
 
-template <typename R, int KIND, bool VV>
+template <typename R, int KIND, bool VV, int AUX = 0>
 struct KsOps {
   typedef KsAcc Acc;
   template <int W>
@@ -412,7 +407,7 @@ struct KsOps {
   int32_t k;
   template <int W, bool TAIL = false>
   __device__ __forceinline__ void issue(Buf<W> &b, int32_t cnt = W) {
-    ks_issue<R, KIND, TAIL, W, VV>(mp, cnt, r8, xr, vr, mr, k, b);
+    ks_issue<R, KIND, TAIL, W, VV, AUX>(mp, cnt, r8, xr, vr, mr, k, b);
     mp += W;
     k += W;
   }
@@ -423,19 +418,19 @@ struct KsOps {
   __device__ __forceinline__ void flush(Acc &) {}
 };
 
-template <int KIND, int U, bool PIPE>
+template <int KIND, int U, bool PIPE, int AUX = 0>
 __device__ __forceinline__ void ks_group32(const qh_kmeta32 *__restrict__ meta, int32_t k, int32_t kend,
                                            uint32_t r8, __amdgpu_buffer_rsrc_t xr, __amdgpu_buffer_rsrc_t vr,
                                            KsAcc &acc) {
-  KsOps32<KIND> ops{meta + k, r8, xr, vr};
+  KsOps32<KIND, AUX> ops{meta + k, r8, xr, vr};
   ks_run<U, PIPE>(ops, kend - k, acc);
 }
 
-template <typename R, int KIND, int U, bool VV, bool PIPE>
+template <typename R, int KIND, int U, bool VV, bool PIPE, int AUX = 0>
 __device__ __forceinline__ void ks_group(const qh_kmeta *__restrict__ meta, int32_t k, int32_t kend, uint32_t r8,
                                          __amdgpu_buffer_rsrc_t xr, __amdgpu_buffer_rsrc_t vr,
                                          __amdgpu_buffer_rsrc_t mr, KsAcc &acc) {
-  KsOps<R, KIND, VV> ops{meta + k, r8, xr, vr, mr, k};
+  KsOps<R, KIND, VV, AUX> ops{meta + k, r8, xr, vr, mr, k};
   ks_run<U, PIPE>(ops, kend - k, acc);
 }
 
@@ -449,7 +444,38 @@ __device__ __forceinline__ void ks_group(const qh_kmeta *__restrict__ meta, int3
 // the FMAs); U trades memory-level parallelism against the SGPRs the metadata
 // occupies, and is chosen by the autotuner.  The FP32 state is x = hi + lo
 // (complex64 pair); its updates are formed in FP64 and split again.
-template <typename R, int MODE, int U, bool VV, bool PIPE>
+// entry r of a complex vector (scalar type R), read or written with the
+// non-temporal hint when NT
+template <bool NT, typename R>
+__device__ inline typename C2<R>::T ld_nt(const typename C2<R>::T *p, int64_t r) {
+  if constexpr (NT) {
+    typedef typename C2<R>::V V;
+    const V v = __builtin_nontemporal_load(reinterpret_cast<const V *>(p + r));
+    typename C2<R>::T t;
+    t.x = v[0];
+    t.y = v[1];
+    return t;
+  } else {
+    return p[r];
+  }
+}
+template <bool NT, typename R>
+__device__ inline void st_nt(typename C2<R>::T *p, int64_t r, typename C2<R>::T t) {
+  if constexpr (NT) {
+    typedef typename C2<R>::V V;
+    V v;
+    v[0] = t.x;
+    v[1] = t.y;
+    __builtin_nontemporal_store(v, reinterpret_cast<V *>(p + r));
+  } else {
+    p[r] = t;
+  }
+}
+
+// NT: the matrix values, the state words, K1..K3 and the K and low-word stores
+// of the RK4 stages under the non-temporal cache policy (the gathered input and
+// the next stage input keep the default one)
+template <typename R, int MODE, int U, bool VV, bool PIPE, bool NT = false>
 __global__ void __launch_bounds__(256) k_ks_stage(
     int32_t nrows, int32_t nslices, const int32_t *__restrict__ moff, const int32_t *__restrict__ umid,
     const int32_t *__restrict__ cmid, const qh_kmeta *__restrict__ meta, const qh_kmeta32 *__restrict__ meta32,
@@ -473,14 +499,15 @@ __global__ void __launch_bounds__(256) k_ks_stage(
   const __amdgpu_buffer_rsrc_t mr = ks_rsrc(meta, (int64_t)nmeta * (int64_t)sizeof(qh_kmeta));
   const uint32_t r8 = (uint32_t)r * (uint32_t)sizeof(T);
   KsAcc acc{0.0, 0.0};
+  constexpr int AUX = NT ? 2 : 0;
   if (sizeof(R) == 4 && !VV) {
     ks_group32<KS_UNI, U, PIPE>(meta32, k0, km, r8, xr, vr, acc);
-    ks_group32<KS_SCA, U, PIPE>(meta32, km, kc, r8, xr, sr, acc);
-    ks_group32<KS_CPX, U, PIPE>(meta32, kc, kend, r8, xr, vr, acc);
+    ks_group32<KS_SCA, U, PIPE, AUX>(meta32, km, kc, r8, xr, sr, acc);
+    ks_group32<KS_CPX, U, PIPE, AUX>(meta32, kc, kend, r8, xr, vr, acc);
   } else {
     ks_group<R, KS_UNI, U, VV, PIPE>(meta, k0, km, r8, xr, vr, mr, acc);
-    ks_group<R, KS_SCA, U, VV, PIPE>(meta, km, kc, r8, xr, sr, mr, acc);
-    ks_group<R, KS_CPX, U, VV, PIPE>(meta, kc, kend, r8, xr, vr, mr, acc);
+    ks_group<R, KS_SCA, U, VV, PIPE, AUX>(meta, km, kc, r8, xr, sr, mr, acc);
+    ks_group<R, KS_CPX, U, VV, PIPE, AUX>(meta, kc, kend, r8, xr, vr, mr, acc);
   }
   const double are = acc.re, aim = acc.im;
   if (MODE == MODE_BS3E) {
@@ -528,9 +555,9 @@ __global__ void __launch_bounds__(256) k_ks_stage(
     xout[r] = o;
     return;
   }
-  const T xo = x[r];
+  const T xo = ld_nt<NT, R>(x, r);
   if (MODE == MODE_STAGE) {
-    kout[r] = y;
+    st_nt<NT, R>(kout, r, y);
     T yn;
     if (sizeof(R) == 8) {
       yn.x = xo.x + cn * y.x;
@@ -544,7 +571,7 @@ __global__ void __launch_bounds__(256) k_ks_stage(
     ynext[r] = yn;
     return;
   }
-  const T a = k1[r], bq = k2[r], cq = k3[r];
+  const T a = ld_nt<NT, R>(k1, r), bq = ld_nt<NT, R>(k2, r), cq = ld_nt<NT, R>(k3, r);
   if (sizeof(R) == 8) {
     const R s6 = h * (R)(1.0 / 6.0), s3 = h * (R)(1.0 / 3.0);
     const R dre = s6 * a.x + s3 * bq.x + s3 * cq.x + s6 * y.x;
@@ -558,13 +585,13 @@ __global__ void __launch_bounds__(256) k_ks_stage(
     const double s6 = (double)h * (1.0 / 6.0), s3 = (double)h * (1.0 / 3.0);
     const double dre = s6 * (double)a.x + s3 * (double)bq.x + s3 * (double)cq.x + s6 * are;
     const double dim = s6 * (double)a.y + s3 * (double)bq.y + s3 * (double)cq.y + s6 * aim;
-    const float2 lo = xlo[r];
+    const float2 lo = ld_nt<NT, float>(xlo, r);
     const double nr = ((double)xo.x + (double)lo.x) + dre, ni = ((double)xo.y + (double)lo.y) + dim;
     T o;
     o.x = (R)nr;
     o.y = (R)ni;
     xout[r] = o;
-    xlo[r] = make_float2((float)(nr - (double)o.x), (float)(ni - (double)o.y));
+    st_nt<NT, float>(xlo, r, make_float2((float)(nr - (double)o.x), (float)(ni - (double)o.y)));
   }
 }
 
@@ -843,20 +870,22 @@ int launch_ks(int G, hipStream_t s, const qh_mat *A, const void *xin, void *kout
   const R *sv = sizeof(R) == 8 ? (const R *)A->ks_sval : (const R *)A->ks_sval32;
   const dim3 grid(qh_blocks(A->nslices, 4)), blk(256);
   const int32_t n = (int32_t)A->rows, ns = A->nslices;
-#define QH_LK(UU, VV, PP)                                                                                     \
-  hipLaunchKernelGGL((k_ks_stage<R, MODE, UU, VV, PP>), grid, blk, 0, s, n, ns, A->ks_moff, A->ks_umid,          \
+#define QH_LK(UU, VV, PP, NN)                                                                                 \
+  hipLaunchKernelGGL((k_ks_stage<R, MODE, UU, VV, PP, NN>), grid, blk, 0, s, n, ns, A->ks_moff, A->ks_umid,          \
                      A->ks_cmid,                                                                              \
                      A->ks_meta, A->ks_meta32, (int32_t)(A->ks_slots + QH_KS_PAD), (int32_t)A->ks_nval, kv,    \
                      (int32_t)A->ks_nsval, sv, (const T *)xin, (T *)kout, (T *)ynext, cn, (const T *)x,       \
                      (const T *)k1, (const T *)k2, (const T *)k3, (T *)xout, h, xlo, atol, rtol, eacc)
   switch (G) {
-    case 128: QH_LK(2, false, false); break;
-    case 129: QH_LK(4, false, false); break;
-    case 130: QH_LK(8, false, false); break;
-    case 131: QH_LK(4, true, false); break;
-    case 132: QH_LK(8, true, false); break;
-    case 133: QH_LK(2, false, true); break;
-    case 134: QH_LK(4, false, true); break;
+    case 128: QH_LK(2, false, false, false); break;
+    case 129: QH_LK(4, false, false, false); break;
+    case 130: QH_LK(8, false, false, false); break;
+    case 131: QH_LK(4, true, false, false); break;
+    case 132: QH_LK(8, true, false, false); break;
+    case 133: QH_LK(2, false, true, false); break;
+    case 134: QH_LK(4, false, true, false); break;
+    case 135: QH_LK(4, false, false, true); break;
+    case 136: QH_LK(4, true, false, true); break;
     default:
       qh_set_error("unsupported sliced-layout variant %d", G);
       return QH_EINVAL;
@@ -961,7 +990,7 @@ int autotune(qh_mat *A, int *out_g) {
     if (!brc && A->ks_state == 1 && sizeof(R) == 4) brc = ks_make_fp32(A);
     if (brc) return brc;
     if (A->ks_state == 1)
-      for (int u = 128; u <= 134; u++) cands.push_back(u);
+      for (int u = 128; u <= 136; u++) cands.push_back(u);
     if (sizeof(R) == 4) {
       brc = ks2_build(A);
       if (brc) return brc;
@@ -1549,7 +1578,7 @@ extern "C" int qh_mat_set_lanes(qh_mat *m, int l64, int l32) {
     return g == 0 || g == 1 || g == 2 || g == 4 || g == 8 || g == 16 || g == 32 || g == 64 || QH_IS_KSELL(g);
   };
   QH_REQUIRE(ok(l64) && (ok(l32) || QH_IS_KS2(l32)), QH_EINVAL,
-             "lanes per row must be a power of two <= 64, 128..134 (sliced layout), or for FP32 140..144 (row-pair "
+             "lanes per row must be a power of two <= 64, 128..136 (sliced layout), or for FP32 140..144 (row-pair "
              "sliced layout)");
   m->lanes64 = l64;
   m->lanes32 = l32;

@@ -153,8 +153,8 @@ def test_rk4_device_direct_matches_oracle_and_lanes():
     x0 = np.zeros(N * N, np.complex128)
     x0[loc] = 1
     ref = _oracle_run(r, x0, 0.01, 37)
-    # 128..134 = sliced layout; 140..144 = FP32 row-pair sliced layout (FP64 keeps 8 lanes per row there)
-    for g in (1, 2, 4, 8, 16, 32, 64, 128, 129, 130, 131, 132, 133, 134, 140, 141, 142, 143, 144):
+    # 128..136 = sliced layout; 140..144 = FP32 row-pair sliced layout (FP64 keeps 8 lanes per row there)
+    for g in (1, 2, 4, 8, 16, 32, 64, 128, 129, 130, 131, 132, 133, 134, 135, 136, 140, 141, 142, 143, 144):
         A.set_lanes(g if g < 140 else 8, g)
         v = qa.DeviceVec(N * N, qa.FP64)
         v.set(x0)
@@ -397,7 +397,7 @@ def test_sliced_ell_spmv_matches_csr(name):
     xv = qa.DeviceVec(n)
     xv.set(x)
     outs = []
-    for g in (8, 128, 129, 130, 131, 132, 133, 134):
+    for g in (8, 128, 129, 130, 131, 132, 133, 134, 135, 136):
         A.set_lanes(g, g)
         yv = qa.DeviceVec(n)
         qa._qh(qa.lib().qh_spmv(A.h, xv.h, yv.h))
@@ -405,7 +405,7 @@ def test_sliced_ell_spmv_matches_csr(name):
     rp, col, val = A.export()
     ref = np.zeros(n, np.complex128)
     np.add.at(ref, np.repeat(np.arange(n), np.diff(rp)), val * x[col])
-    for g, o in zip((8, 128, 129, 130, 131, 132, 133, 134), outs):
+    for g, o in zip((8, 128, 129, 130, 131, 132, 133, 134, 135, 136), outs):
         assert np.abs(o - ref).max() <= 1e-12 * np.abs(ref).max(), g
     lay = A.layout_info()
     slots, ncpx, nreal = _slot_classes(rp, col, val)
