@@ -87,9 +87,12 @@ int qh_mat_layout_info(qh_mat *m, int64_t *slices, int64_t *slots, int64_t *stor
                        int64_t *meta_bytes);
 /* FP32 row-pair sliced layout statistics (built on demand; zeros when it does
  * not apply): 128-row slices, slots, stored complex pairs, stored real pairs,
- * metadata bytes, stored single reals (slots whose two rows share each value) */
+ * metadata bytes (slot records, per-slice offsets, separable-block tables),
+ * stored single reals (slots whose two rows share each value), and the slots of
+ * a separable diagonal (no stored values: g_i + h_j formed in the kernel) */
 int qh_mat_pair_layout_info(qh_mat *m, int64_t *slices, int64_t *slots, int64_t *stored_pairs,
-                            int64_t *stored_real_pairs, int64_t *meta_bytes, int64_t *stored_real_singles);
+                            int64_t *stored_real_pairs, int64_t *meta_bytes, int64_t *stored_real_singles,
+                            int64_t *sep_slots);
 
 /* ------------------------------------------------------------------ vectors */
 typedef struct qh_vec qh_vec;

@@ -177,7 +177,7 @@ def _setup(L):
     sig("qh_mat_set_lanes", i32, vp, i32, i32)
     sig("qh_mat_get_lanes", i32, vp, P(i32), P(i32))
     sig("qh_mat_layout_info", i32, vp, P(i64), P(i64), P(i64), P(i64), P(i64))
-    sig("qh_mat_pair_layout_info", i32, vp, P(i64), P(i64), P(i64), P(i64), P(i64), P(i64))
+    sig("qh_mat_pair_layout_info", i32, vp, P(i64), P(i64), P(i64), P(i64), P(i64), P(i64), P(i64))
     sig("qh_vec_create", i32, i64, i32, P(vp))
     sig("qh_vec_destroy", i32, vp)
     sig("qh_vec_set", i32, vp, P(dbl))
@@ -374,11 +374,11 @@ class Matrix:
 
     def pair_layout_info(self):
         """FP32 row-pair sliced layout: dict(slices, slots, stored_pairs, stored_real_pairs, meta_bytes,
-        stored_real_singles)"""
-        v = [ctypes.c_int64() for _ in range(6)]
+        stored_real_singles, sep_slots)"""
+        v = [ctypes.c_int64() for _ in range(7)]
         _qh(lib().qh_mat_pair_layout_info(self.h, *[ctypes.byref(x) for x in v]))
-        return dict(zip(("slices", "slots", "stored_pairs", "stored_real_pairs", "meta_bytes", "stored_real_singles"),
-                        [x.value for x in v]))
+        return dict(zip(("slices", "slots", "stored_pairs", "stored_real_pairs", "meta_bytes", "stored_real_singles",
+                         "sep_slots"), [x.value for x in v]))
 
     def __del__(self):
         if getattr(self, "owned", False) and self.h and _lib is not None:

@@ -670,6 +670,9 @@ extern "C" int qh_mat_destroy(qh_mat *m) {
   qh_free_ptr(m->ks2_val);
   qh_free_ptr(m->ks2_sval);
   qh_free_ptr(m->ks2_sval1);
+  qh_free_ptr(m->ks2_xmid);
+  qh_free_ptr(m->ks2_sep);
+  qh_free_ptr(m->ks2_gh);
   qh_free_ptr(m->ks_sval);
   qh_free_ptr(m->ks_sval32);
   qh_free_ptr(m->ks_meta);
@@ -786,6 +789,9 @@ extern "C" int qh_mat_blockdiag(int n, qh_mat *const *mats, qh_mat **out) {
   }
   QH_HIP(hipGetLastError());
   QH_HIP(hipStreamSynchronize(s));
+  m->blk_off.resize(n + 1);
+  m->blk_off[0] = 0;
+  for (int i = 0; i < n; i++) m->blk_off[i + 1] = m->blk_off[i] + mats[i]->rows;
   *out = m;
   return QH_OK;
 }

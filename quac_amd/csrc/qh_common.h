@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <vector>
 #include <cstdio>
 
 #include "../../include/quac_hip.h"
@@ -81,6 +82,15 @@ struct qh_kmeta2 {
 // its metadata without a bounds check
 #define QH_KS_PAD 16
 
+// A block of the row-pair layout whose diagonal separates: the diagonal entry of
+// row off + N j + i (density-matrix row i, column j of an N x N block, N even) is
+// g_i + h_j, with g at gh[gbase ..] and h at gh[hbase ..] (complex128); m, sh
+// divide a row index within the block by N (q = (mulhi(t, m) + t) >> sh).
+struct qh_sepblk {
+  int64_t off;
+  uint32_t n, m, sh, gbase, hbase, pad;
+};
+
 struct qh_td_cache;  // merged time-dependent matrices and stage buffers (qh_td.hip)
 void qh_td_cache_free(qh_td_cache *c);
 
@@ -126,7 +136,11 @@ struct qh_mat {
   float4 *ks2_val = nullptr;    // complex slots: (row 2l, row 2l+1) complex64 pairs
   float2 *ks2_sval = nullptr;   // scalar slots: (row 2l, row 2l+1) real pairs
   float *ks2_sval1 = nullptr;   // equal-scalar slots: one real value for both rows of a lane
-  int64_t ks2_slots = 0, ks2_nval = 0, ks2_nsval = 0, ks2_nsval1 = 0;
+  int32_t *ks2_xmid = nullptr;  // end of each slice's complex slots (separable-diagonal slots follow)
+  qh_sepblk *ks2_sep = nullptr; // blocks with a separable diagonal
+  double2 *ks2_gh = nullptr;    // their g and h vectors
+  int64_t ks2_slots = 0, ks2_nval = 0, ks2_nsval = 0, ks2_nsval1 = 0, ks2_nsep = 0, ks2_ngh = 0, ks2_nsepblk = 0;
+  std::vector<int64_t> blk_off;  // row offsets of the blocks of a block-diagonal matrix (and its row count)
   qh_td_cache *td_cache = nullptr;  // built by the first time-dependent RHS on this matrix
 };
 // variant codes selecting the sliced-ELL kernel: 128 / 129 / 130 process 2 / 4 / 8

@@ -18,6 +18,8 @@
 // one-row-per-lane layout, with one more group: scalar slots whose two rows
 // hold the same value in every lane store one real per lane (4 bytes, not 8).  Products are summed in FP32 within a pass and the
 // pass sums in FP64; the state is hi + lo as in the other FP32 kernels.
+#include <algorithm>
+#include <cmath>
 #include <vector>
 
 #include "qh_common.h"
@@ -38,17 +40,20 @@ const int KS2_WIN = 2048;
 // [5] uniform slots whose rows take the halves in order (row 2l half 0, row
 // 2l+1 half 1: no select in the kernel), [6] those of them whose loads all stay
 // inside the slice's KS2_WIN-row window, [7] scalar slots whose two rows hold
-// the same value in every lane, [8] their values (one per lane).  FILL writes
+// the same value in every lane, [8] their values (one per lane), [9] slots of a
+// separable diagonal (stored nothing: the kernel forms g_i + h_j).  FILL writes
 // the metadata (uniform in-order window slots at moff, the other uniform
-// in-order slots, the other uniform slots, then the scalar, the equal-scalar and
-// the complex groups) and the values (complex pairs at voff, scalar pairs at
-// svoff, equal scalars at s1off).
+// in-order slots, the other uniform slots, then the scalar, the equal-scalar,
+// the complex and the separable-diagonal groups) and the values (complex pairs
+// at voff, scalar pairs at svoff, equal scalars at s1off).  slice_blk[s]: the
+// separable block that holds the whole slice, or -1.
 template <bool FILL>
 __global__ void __launch_bounds__(64) k_ks2_build(int32_t n, const int32_t *__restrict__ rp,
                                                   const int32_t *__restrict__ col, const double2 *__restrict__ val,
                                                   int32_t *cnt, int32_t ns, const int32_t *moff, const int32_t *voff,
                                                   const int32_t *svoff, const int32_t *s1off, qh_kmeta2 *meta,
-                                                  float4 *kval, float2 *sval, float *sval1, int *overflow) {
+                                                  float4 *kval, float2 *sval, float *sval1,
+                                                  const int32_t *__restrict__ slice_blk, int *overflow) {
   // per lane up to 2 * KS2_MAXROW pair loads: key << 12 | e0 << 6 | e1 (63 = no entry)
   __shared__ uint64_t keys[64][2 * KS2_MAXROW];
   const int s = blockIdx.x, lane = threadIdx.x;
@@ -129,7 +134,8 @@ __global__ void __launch_bounds__(64) k_ks2_build(int32_t n, const int32_t *__re
     prev = adp;
   }
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  int p = 0, slot = 0, uslot = 0, sslot = 0, s1slot = 0, cslot = 0, ustr = 0, uwin = 0;
+  int p = 0, slot = 0, uslot = 0, sslot = 0, s1slot = 0, cslot = 0, xslot = 0, ustr = 0, uwin = 0;
+  const int32_t sblk = slice_blk ? slice_blk[s] : -1;
   const int64_t w0 = ((int64_t)s * 128 / KS2_WIN) * KS2_WIN;  // first row of the slice's window
   int32_t base = FILL ? voff[s] : 0;
   int32_t sbase = FILL ? svoff[s] : 0;
@@ -138,6 +144,7 @@ __global__ void __launch_bounds__(64) k_ks2_build(int32_t n, const int32_t *__re
   const int32_t ms = FILL ? mo + cnt[2 * ns + s] : 0;
   const int32_t ms1 = FILL ? ms + cnt[3 * ns + s] : 0;
   const int32_t mc = FILL ? ms1 + cnt[7 * ns + s] : 0;
+  const int32_t mx = FILL ? mo + cnt[s] - cnt[9 * ns + s] : 0;  // separable-diagonal slots
   const int32_t mu2 = FILL ? mo + cnt[5 * ns + s] : 0;  // uniform slots that need the half selects
   const int32_t mw = FILL ? mo + cnt[6 * ns + s] : 0;   // in-order uniform slots outside the window
   for (;;) {
@@ -159,7 +166,12 @@ __global__ void __launch_bounds__(64) k_ks2_build(int32_t n, const int32_t *__re
     const bool uni = __ballot(same) == ~0ull;
     const bool isre = !uni && __ballot(act && (v0.y != 0.0 || v1.y != 0.0)) == 0ull;
     const bool isim = !uni && !isre && __ballot(act && (v0.x != 0.0 || v1.x != 0.0)) == 0ull;
-    const bool sca = isre || isim;
+    // separable diagonal: the slot (|d| = 0, in-order halves, rank 0) holds the
+    // diagonal entries of both rows of every active lane, in a separable block
+    const bool dgl = act && e0 != 63 && e1 != 63 && col[beg0 + e0] == r0 && col[beg1 + e1] == r1;
+    const bool sep = sblk >= 0 && !uni && (m >> 5) == 0 && ((m >> 1) & 15u) == 8u && (m & 1u) == 0 &&
+                     __ballot(act && !dgl) == 0ull;
+    const bool sca = (isre || isim) && !sep;
     // equal scalars: both rows of every active lane hold the same FP32 value
     auto part = [&](double2 v) { return (float)(isre ? v.x : v.y); };
     const bool eq1 = sca && __ballot(act && part(v0) != part(v1)) == 0ull;
@@ -184,7 +196,7 @@ __global__ void __launch_bounds__(64) k_ks2_build(int32_t n, const int32_t *__re
         md.mask = mask;
         md.sign = sign;
         md.ad8 = (uint32_t)(ad * 8u) | h0 | (h1 << 1);
-        md.base = uni ? 0u : (uint32_t)(eq1 ? s1base : sca ? sbase : base);
+        md.base = uni ? 0u : sep ? (uint32_t)sblk : (uint32_t)(eq1 ? s1base : sca ? sbase : base);
         if (uni) {
           md.v0re = (float)f0r;
           md.v0im = (float)f0i;
@@ -195,16 +207,18 @@ __global__ void __launch_bounds__(64) k_ks2_build(int32_t n, const int32_t *__re
           md.v0im = md.v1im = sca ? (isre ? 0.f : 1.f) : 0.f;
         }
         meta[uni ? (win ? mo + uwin : inorder ? mw + (ustr - uwin) : mu2 + (uslot - ustr))
-                 : eq1 ? ms1 + s1slot : sca ? ms + sslot : mc + cslot] = md;
+                 : sep ? mx + xslot : eq1 ? ms1 + s1slot : sca ? ms + sslot : mc + cslot] = md;
       }
       if (act && eq1)
         sval1[s1base + rank] = part(v0);
       else if (act && sca)
         sval[sbase + rank] = make_float2(part(v0), part(v1));
-      if (act && !uni && !sca)
+      if (act && !uni && !sca && !sep)
         kval[base + rank] = make_float4((float)v0.x, (float)v0.y, (float)v1.x, (float)v1.y);
     }
-    if (eq1)
+    if (sep) {
+      // nothing stored
+    } else if (eq1)
       s1base += __popcll(mask);
     else if (sca)
       sbase += __popcll(mask);
@@ -216,6 +230,8 @@ __global__ void __launch_bounds__(64) k_ks2_build(int32_t n, const int32_t *__re
       uslot++;
       if (inorder) ustr++;
       if (win) uwin++;
+    } else if (sep) {
+      xslot++;
     } else if (eq1) {
       s1slot++;
     } else if (sca) {
@@ -234,6 +250,7 @@ __global__ void __launch_bounds__(64) k_ks2_build(int32_t n, const int32_t *__re
     cnt[6 * ns + s] = uwin;
     cnt[7 * ns + s] = s1slot;
     cnt[8 * ns + s] = s1base;
+    cnt[9 * ns + s] = xslot;
   }
 }
 
@@ -401,6 +418,61 @@ struct Ks2OpsWin {
   __device__ __forceinline__ void flush(Acc &acc) { Ks2Ops<KS_UNI, true>{}.flush(acc); }
 };
 
+// Separable-diagonal slots (at most one per slice): the lane's own pair x[2l],
+// x[2l+1] times the diagonal entries g_i + h_j and g_(i+1) + h_j, formed in FP64
+// from the block's g and h (t = 2l - off, j = t / N, i = t - j N; i is even, so
+// i + 1 stays in the column) and rounded to FP32 as the stored diagonal would be.
+struct Ks2OpsSep {
+  typedef KsAcc2 Acc;
+  template <int W>
+  struct Buf {
+    float4 xv[W];
+    double2 g0[W], g1[W], h[W];
+  };
+  const qh_kmeta2 *mp;
+  uint32_t r8;  // byte offset of x[2l] within the stage input
+  int64_t r;    // row 2l
+  __amdgpu_buffer_rsrc_t xr, ghr;
+  const qh_sepblk *bp;
+  template <int W>
+  __device__ __forceinline__ void issue(Buf<W> &b) {
+#pragma unroll
+    for (int j = 0; j < W; j++) {
+      const qh_kmeta2 m = mp[j];
+      const bool on = __builtin_amdgcn_inverse_ballot_w64(m.mask);
+      b.xv[j] = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(xr, on ? r8 : QH_KS_OOB, 0, 0));
+      const qh_sepblk q = bp[m.base];
+      const uint32_t t = (uint32_t)(r - q.off);
+      const uint32_t jj = (uint32_t)(((uint64_t)__umulhi(t, q.m) + t) >> q.sh);
+      const uint32_t ii = t - jj * q.n;
+      const uint32_t go = on ? (q.gbase + ii) * 16u : QH_KS_OOB, ho = on ? (q.hbase + jj) * 16u : QH_KS_OOB;
+      b.g0[j] = __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(ghr, go, 0, 0));
+      b.g1[j] = __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(ghr, on ? go + 16u : go, 0, 0));
+      b.h[j] = __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(ghr, ho, 0, 0));
+    }
+    mp += W;
+  }
+  template <int W>
+  __device__ __forceinline__ void fma(const Buf<W> &b, Acc &acc) {
+    float p0r = acc.p0r, p0i = acc.p0i, p1r = acc.p1r, p1i = acc.p1i;
+#pragma unroll
+    for (int j = 0; j < W; j++) {
+      const float a0r = (float)(b.g0[j].x + b.h[j].x), a0i = (float)(b.g0[j].y + b.h[j].y);
+      const float a1r = (float)(b.g1[j].x + b.h[j].x), a1i = (float)(b.g1[j].y + b.h[j].y);
+      const float4 xv = b.xv[j];
+      p0r = fmaf(a0r, xv.x, fmaf(-a0i, xv.y, p0r));
+      p0i = fmaf(a0r, xv.y, fmaf(a0i, xv.x, p0i));
+      p1r = fmaf(a1r, xv.z, fmaf(-a1i, xv.w, p1r));
+      p1i = fmaf(a1r, xv.w, fmaf(a1i, xv.z, p1i));
+    }
+    acc.p0r = p0r;
+    acc.p0i = p0i;
+    acc.p1r = p1r;
+    acc.p1i = p1i;
+  }
+  __device__ __forceinline__ void flush(Acc &acc) { Ks2Ops<KS_UNI, true>{}.flush(acc); }
+};
+
 // the two complex64 of rows (r, r+1) at byte offset o of a vector; the second is
 // zero (and not loaded / stored) when r is the last row.  AUX: the cache policy
 // bits of the buffer instruction (0, or 2 = non-temporal: streamed data that is
@@ -435,7 +507,8 @@ template <int MODE, int U, bool WIN, bool NT = false>
 __global__ void __launch_bounds__(WIN ? KS2_WIN / 2 : 256) k_ks2_stage(
     int32_t n, int32_t nslices, const int32_t *__restrict__ moff, const int32_t *__restrict__ wmid,
     const int32_t *__restrict__ smid, const int32_t *__restrict__ umid, const int32_t *__restrict__ emid,
-    const int32_t *__restrict__ cmid, const qh_kmeta2 *__restrict__ meta, int32_t nval,
+    const int32_t *__restrict__ cmid, const int32_t *__restrict__ xmid, const qh_sepblk *__restrict__ sepb,
+    int32_t ngh, const double2 *__restrict__ gh, const qh_kmeta2 *__restrict__ meta, int32_t nval,
                                                    const float4 *__restrict__ val, int32_t nsval,
                                                    const float2 *__restrict__ sval, int32_t nsval1,
                                                    const float *__restrict__ sval1, const float2 *xin, float2 *kout,
@@ -464,7 +537,7 @@ __global__ void __launch_bounds__(WIN ? KS2_WIN / 2 : 256) k_ks2_stage(
   const int lane = threadIdx.x & 63;
   const int32_t r = s * 128 + 2 * lane;
   const int32_t k0 = moff[s], kw = wmid[s], ku = smid[s], km = umid[s], ke = emid[s], kc = cmid[s],
-                kend = moff[s + 1];
+                kx = xmid[s], kend = moff[s + 1];
   const __amdgpu_buffer_rsrc_t vr = ks_rsrc(val, (int64_t)nval * 16);
   const __amdgpu_buffer_rsrc_t sr = ks_rsrc(sval, (int64_t)nsval * 8);
   const __amdgpu_buffer_rsrc_t s1r = ks_rsrc(sval1, (int64_t)nsval1 * 4);
@@ -480,7 +553,9 @@ __global__ void __launch_bounds__(WIN ? KS2_WIN / 2 : 256) k_ks2_stage(
   ks_run<U, true>(Ks2Ops<KS_UNI>{meta + ku, r8, xr, vr}, km - ku, acc);
   ks_run<U, true>(Ks2Ops<KS_SCA, false, AUX>{meta + km, r8, xr, sr}, ke - km, acc);
   ks_run<U, true>(Ks2Ops<KS_SC1, false, AUX>{meta + ke, r8, xr, s1r}, kc - ke, acc);
-  ks_run<U, true>(Ks2Ops<KS_CPX, false, AUX>{meta + kc, r8, xr, vr}, kend - kc, acc);
+  ks_run<U, true>(Ks2Ops<KS_CPX, false, AUX>{meta + kc, r8, xr, vr}, kx - kc, acc);
+  if (kend > kx)
+    ks_run<U, true>(Ks2OpsSep{meta + kx, r8, (int64_t)r, xr, ks_rsrc(gh, (int64_t)ngh * 16), sepb}, kend - kx, acc);
   if (r >= n) return;
   const bool two = r + 1 < n;
   const float4 y = make_float4((float)acc.r0, (float)acc.i0, (float)acc.r1, (float)acc.i1);
@@ -519,6 +594,84 @@ __global__ void __launch_bounds__(WIN ? KS2_WIN / 2 : 256) k_ks2_stage(
 
 }  // namespace
 
+// the diagonal entry of every row (NaN where the row has none)
+__global__ void k_ks2_diag(int32_t n, const int32_t *__restrict__ rp, const int32_t *__restrict__ col,
+                           const double2 *__restrict__ val, double2 *__restrict__ d) {
+  const int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  double2 v = make_double2(__builtin_nan(""), __builtin_nan(""));
+  for (int32_t k = rp[r]; k < rp[r + 1]; k++)
+    if (col[k] == r) v = val[k];
+  d[r] = v;
+}
+
+// The blocks of A (the instances of a batch, else A itself) whose diagonal
+// separates: an N x N density-matrix block (N even, starting at an even row)
+// whose diagonal entry at row i, column j is g_i + h_j (g_i = D(i, 0),
+// h_j = D(0, j) - D(0, 0), every entry checked to 1e-12 relative).  Fills the
+// block parameters, their g and h, and for every slice the block that holds it
+// whole (-1: none).
+static int ks2_sep_prepare(qh_mat *A, int32_t ns, std::vector<qh_sepblk> &blks, std::vector<double2> &gh,
+                           std::vector<int32_t> &slice_blk) {
+  const int64_t n = A->rows;
+  std::vector<int64_t> bo = A->blk_off;
+  if (bo.size() < 2 || bo.back() != n) bo = {0, n};
+  double2 *dd = nullptr;
+  int rc = qh_alloc_t(&dd, (size_t)n);
+  if (rc) return rc;
+  hipStream_t s = qh_stream();
+  hipLaunchKernelGGL(k_ks2_diag, dim3(qh_blocks(n, 256)), dim3(256), 0, s, (int32_t)n, A->rowptr, A->col, A->val, dd);
+  std::vector<double2> d((size_t)n);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(d.data(), dd, sizeof(double2) * n, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  qh_free_ptr(dd);
+  QH_HIP(e);
+  std::vector<int32_t> blk_of((size_t)bo.size() - 1, -1);
+  for (size_t b = 0; b + 1 < bo.size(); b++) {
+    const int64_t off = bo[b], rows = bo[b + 1] - bo[b];
+    int64_t N = (int64_t)std::llround(std::sqrt((double)rows));
+    if (N < 2 || N * N != rows || (N & 1) || (off & 1) || N > 46340) continue;
+    auto D = [&](int64_t i, int64_t j) { return d[(size_t)(off + N * j + i)]; };
+    bool ok = true;
+    const double2 d00 = D(0, 0);
+    for (int64_t j = 0; j < N && ok; j++) {
+      const double2 d0j = D(0, j);
+      const double hr = d0j.x - d00.x, hi = d0j.y - d00.y;
+      for (int64_t i = 0; i < N && ok; i++) {
+        const double2 di0 = D(i, 0), dij = D(i, j);
+        const double er = dij.x - (di0.x + hr), ei = dij.y - (di0.y + hi);
+        const double sc = std::fabs(dij.x) + std::fabs(dij.y) + std::fabs(di0.x) + std::fabs(di0.y) +
+                          std::fabs(hr) + std::fabs(hi);
+        ok = std::isfinite(dij.x) && std::isfinite(dij.y) && std::fabs(er) + std::fabs(ei) <= 1e-12 * sc;
+      }
+    }
+    if (!ok || gh.size() + 2 * (size_t)N >= ((size_t)1 << 27)) continue;
+    qh_sepblk q;
+    q.off = off;
+    q.n = (uint32_t)N;
+    uint32_t sh = 0;
+    while (sh < 32 && ((uint64_t)1 << sh) < (uint64_t)N) sh++;
+    q.m = (uint32_t)((((uint64_t)1 << 32) * (((uint64_t)1 << sh) - (uint64_t)N)) / (uint64_t)N + 1);
+    q.sh = sh;
+    q.gbase = (uint32_t)gh.size();
+    q.hbase = (uint32_t)(gh.size() + N);
+    q.pad = 0;
+    for (int64_t i = 0; i < N; i++) gh.push_back(D(i, 0));
+    for (int64_t j = 0; j < N; j++) gh.push_back(make_double2(D(0, j).x - d00.x, D(0, j).y - d00.y));
+    blk_of[b] = (int32_t)blks.size();
+    blks.push_back(q);
+  }
+  slice_blk.assign((size_t)ns, -1);
+  size_t b = 0;
+  for (int32_t sl = 0; sl < ns; sl++) {
+    const int64_t lo = (int64_t)sl * 128, hi = std::min<int64_t>(lo + 128, n);
+    while (b + 1 < bo.size() && bo[b + 1] <= lo) b++;
+    if (b + 1 < bo.size() && hi <= bo[b + 1]) slice_blk[sl] = blk_of[b];
+  }
+  return QH_OK;
+}
+
 // Build the row-pair layout of A (FP32 values) once; ks2_state -1 when it does
 // not apply (fewer than 2 rows, a row longer than KS2_MAXROW, or byte offsets
 // beyond the 2^31 reach of the buffer loads).
@@ -530,28 +683,35 @@ int ks2_build(qh_mat *A) {
     return QH_OK;
   }
   const int32_t ns = (int32_t)((A->rows + 127) / 128);
-  const int NC = 9;  // per-slice counts of k_ks2_build
-  int32_t *cnt = nullptr, *voff = nullptr, *svoff = nullptr, *s1off = nullptr;
+  const int NC = 10;  // per-slice counts of k_ks2_build
+  int32_t *cnt = nullptr, *voff = nullptr, *svoff = nullptr, *s1off = nullptr, *sblk = nullptr;
   int *ovf = nullptr;
-  int rc = qh_alloc_t(&cnt, (size_t)NC * ns);
+  std::vector<qh_sepblk> blks;
+  std::vector<double2> gh;
+  std::vector<int32_t> hsb;
+  int rc = ks2_sep_prepare(A, ns, blks, gh, hsb);
+  if (!rc) rc = qh_alloc_t(&cnt, (size_t)NC * ns);
   if (!rc) rc = qh_alloc_t(&voff, (size_t)ns + 1);
   if (!rc) rc = qh_alloc_t(&svoff, (size_t)ns + 1);
   if (!rc) rc = qh_alloc_t(&s1off, (size_t)ns + 1);
+  if (!rc) rc = qh_alloc_t(&sblk, (size_t)ns);
   if (!rc) rc = qh_alloc_t(&ovf, 1);
   auto cleanup = [&]() {
     qh_free_ptr(cnt);
     qh_free_ptr(voff);
     qh_free_ptr(svoff);
     qh_free_ptr(s1off);
+    qh_free_ptr(sblk);
     qh_free_ptr(ovf);
   };
   if (rc) {
     cleanup();
     return rc;
   }
+  QH_HIP(hipMemcpyAsync(sblk, hsb.data(), sizeof(int32_t) * ns, hipMemcpyHostToDevice, s));
   QH_HIP(hipMemsetAsync(ovf, 0, sizeof(int), s));
   hipLaunchKernelGGL((k_ks2_build<false>), dim3(ns), dim3(64), 0, s, (int32_t)A->rows, A->rowptr, A->col, A->val,
-                     cnt, ns, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, ovf);
+                     cnt, ns, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, sblk, ovf);
   QH_HIP(hipGetLastError());
   int hovf = 0;
   std::vector<int32_t> hc((size_t)NC * ns);
@@ -559,9 +719,10 @@ int ks2_build(qh_mat *A) {
   QH_HIP(hipMemcpyAsync(hc.data(), cnt, sizeof(int32_t) * NC * ns, hipMemcpyDeviceToHost, s));
   QH_HIP(hipStreamSynchronize(s));
   const int32_t *hw = hc.data(), *hn = hw + ns, *hu = hn + ns, *hs = hu + ns, *hsv = hs + ns, *hst = hsv + ns,
-                *hwn = hst + ns, *hs1 = hwn + ns, *hs1v = hs1 + ns;
-  std::vector<int32_t> ho(ns + 1), hv(ns + 1), hso(ns + 1), hs1o(ns + 1), hm(ns), hem(ns), hcm(ns), hsm(ns), hwm(ns);
-  int64_t tot = 0, totv = 0, tots = 0, tots1 = 0;
+                *hwn = hst + ns, *hs1 = hwn + ns, *hs1v = hs1 + ns, *hxs = hs1v + ns;
+  std::vector<int32_t> ho(ns + 1), hv(ns + 1), hso(ns + 1), hs1o(ns + 1), hm(ns), hem(ns), hcm(ns), hsm(ns), hwm(ns),
+      hxm(ns);
+  int64_t tot = 0, totv = 0, tots = 0, tots1 = 0, totx = 0;
   for (int32_t i = 0; i < ns; i++) {
     ho[i] = (int32_t)tot;
     hv[i] = (int32_t)totv;
@@ -572,6 +733,8 @@ int ks2_build(qh_mat *A) {
     hm[i] = (int32_t)(tot + hu[i]);
     hem[i] = (int32_t)(tot + hu[i] + hs[i]);
     hcm[i] = (int32_t)(tot + hu[i] + hs[i] + hs1[i]);
+    hxm[i] = (int32_t)(tot + hw[i] - hxs[i]);
+    totx += hxs[i];
     tot += hw[i];
     totv += hn[i];
     tots += hsv[i];
@@ -593,6 +756,9 @@ int ks2_build(qh_mat *A) {
   if (!rc) rc = qh_alloc_t(&A->ks2_umid, (size_t)ns);
   if (!rc) rc = qh_alloc_t(&A->ks2_emid, (size_t)ns);
   if (!rc) rc = qh_alloc_t(&A->ks2_cmid, (size_t)ns);
+  if (!rc) rc = qh_alloc_t(&A->ks2_xmid, (size_t)ns);
+  if (!rc) rc = qh_alloc_t(&A->ks2_sep, blks.empty() ? 1 : blks.size());
+  if (!rc) rc = qh_alloc_t(&A->ks2_gh, gh.empty() ? 1 : gh.size());
   if (!rc) rc = qh_alloc_t(&A->ks2_meta, (size_t)(tot > 0 ? tot : 1));
   if (!rc) rc = qh_alloc_t(&A->ks2_val, (size_t)(totv > 0 ? totv : 1));
   if (!rc) rc = qh_alloc_t(&A->ks2_sval, (size_t)(tots > 0 ? tots : 1));
@@ -607,12 +773,16 @@ int ks2_build(qh_mat *A) {
   QH_HIP(hipMemcpyAsync(A->ks2_umid, hm.data(), sizeof(int32_t) * ns, hipMemcpyHostToDevice, s));
   QH_HIP(hipMemcpyAsync(A->ks2_emid, hem.data(), sizeof(int32_t) * ns, hipMemcpyHostToDevice, s));
   QH_HIP(hipMemcpyAsync(A->ks2_cmid, hcm.data(), sizeof(int32_t) * ns, hipMemcpyHostToDevice, s));
+  QH_HIP(hipMemcpyAsync(A->ks2_xmid, hxm.data(), sizeof(int32_t) * ns, hipMemcpyHostToDevice, s));
+  if (!blks.empty())
+    QH_HIP(hipMemcpyAsync(A->ks2_sep, blks.data(), sizeof(qh_sepblk) * blks.size(), hipMemcpyHostToDevice, s));
+  if (!gh.empty()) QH_HIP(hipMemcpyAsync(A->ks2_gh, gh.data(), sizeof(double2) * gh.size(), hipMemcpyHostToDevice, s));
   QH_HIP(hipMemcpyAsync(voff, hv.data(), sizeof(int32_t) * (ns + 1), hipMemcpyHostToDevice, s));
   QH_HIP(hipMemcpyAsync(svoff, hso.data(), sizeof(int32_t) * (ns + 1), hipMemcpyHostToDevice, s));
   QH_HIP(hipMemcpyAsync(s1off, hs1o.data(), sizeof(int32_t) * (ns + 1), hipMemcpyHostToDevice, s));
   hipLaunchKernelGGL((k_ks2_build<true>), dim3(ns), dim3(64), 0, s, (int32_t)A->rows, A->rowptr, A->col, A->val,
                      cnt, ns, A->ks2_moff, voff, svoff, s1off, A->ks2_meta, A->ks2_val, A->ks2_sval, A->ks2_sval1,
-                     ovf);
+                     sblk, ovf);
   QH_HIP(hipGetLastError());
   QH_HIP(hipStreamSynchronize(s));
   cleanup();
@@ -621,6 +791,9 @@ int ks2_build(qh_mat *A) {
   A->ks2_nval = totv;
   A->ks2_nsval = tots;
   A->ks2_nsval1 = tots1;
+  A->ks2_nsep = totx;
+  A->ks2_ngh = (int64_t)gh.size();
+  A->ks2_nsepblk = (int64_t)blks.size();
   A->ks2_state = 1;
   return QH_OK;
 }
@@ -640,7 +813,8 @@ int ks2_launch(int G, int mode, hipStream_t s, const qh_mat *A, const void *xin,
   const dim3 grid(qh_blocks(ns, win ? KS2_WIN / 128 : 4)), blk(win ? KS2_WIN / 2 : 256);
 #define QH_LK2(MM, UU, WW, NN)                                                                                   \
   hipLaunchKernelGGL((k_ks2_stage<MM, UU, WW, NN>), grid, blk, 0, s, n, ns, A->ks2_moff, A->ks2_wmid, A->ks2_smid,  \
-                     A->ks2_umid, A->ks2_emid, A->ks2_cmid, A->ks2_meta, (int32_t)A->ks2_nval, A->ks2_val,       \
+                     A->ks2_umid, A->ks2_emid, A->ks2_cmid, A->ks2_xmid, A->ks2_sep, (int32_t)A->ks2_ngh,      \
+                     A->ks2_gh, A->ks2_meta, (int32_t)A->ks2_nval, A->ks2_val,                                    \
                      (int32_t)A->ks2_nsval, A->ks2_sval, (int32_t)A->ks2_nsval1, A->ks2_sval1,                  \
                      (const float2 *)xin, (float2 *)kout, (float2 *)ynext, cn, (const float2 *)x,                \
                      (const float2 *)k1, (const float2 *)k2, (const float2 *)k3, (float2 *)xout, h, xlo)

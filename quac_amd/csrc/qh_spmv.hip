@@ -1602,7 +1602,8 @@ extern "C" int qh_mat_layout_info(qh_mat *m, int64_t *slices, int64_t *slots, in
 }
 
 extern "C" int qh_mat_pair_layout_info(qh_mat *m, int64_t *slices, int64_t *slots, int64_t *stored_pairs,
-                                       int64_t *stored_real_pairs, int64_t *mbytes, int64_t *stored_real_singles) {
+                                       int64_t *stored_real_pairs, int64_t *mbytes, int64_t *stored_real_singles,
+                                       int64_t *sep_slots) {
   QH_REQUIRE(m, QH_EINVAL, "null matrix");
   int rc = ks2_build(m);
   if (rc) return rc;
@@ -1612,7 +1613,11 @@ extern "C" int qh_mat_pair_layout_info(qh_mat *m, int64_t *slices, int64_t *slot
   if (stored_pairs) *stored_pairs = ok ? m->ks2_nval : 0;
   if (stored_real_pairs) *stored_real_pairs = ok ? m->ks2_nsval : 0;
   if (stored_real_singles) *stored_real_singles = ok ? m->ks2_nsval1 : 0;
-  if (mbytes) *mbytes = ok ? (int64_t)(m->ks2_slots * sizeof(qh_kmeta2) + 16 * m->ks2_nslices + 4) : 0;
+  if (sep_slots) *sep_slots = ok ? m->ks2_nsep : 0;
+  if (mbytes)
+    *mbytes = ok ? (int64_t)(m->ks2_slots * sizeof(qh_kmeta2) + 28 * m->ks2_nslices + 4 +
+                             m->ks2_nsepblk * sizeof(qh_sepblk) + m->ks2_ngh * sizeof(double2))
+                 : 0;
   return QH_OK;
 }
 

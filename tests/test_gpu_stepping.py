@@ -221,7 +221,38 @@ def test_row_pair_layout_fp32_spmv(name):
         k.destroy()
 
 
-def _pair_slot_classes(rp, col, val):
+def _sep_slices(rp, col, val, blocks):
+    """per 128-row slice: True when a block of `blocks` (row offsets, then the row
+    count) holds the whole slice and its diagonal separates -- an N x N density
+    matrix block, N even, starting at an even row, whose diagonal entry at row i,
+    column j equals D(i, 0) + D(0, j) - D(0, 0) to 1e-12 relative"""
+    n = len(rp) - 1
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    d = np.full(n, np.nan + 0j)
+    on = col == rows
+    d[rows[on]] = val[on]
+    ok_blk = []
+    for b in range(len(blocks) - 1):
+        off, m = blocks[b], blocks[b + 1] - blocks[b]
+        N = int(round(np.sqrt(m)))
+        ok = N >= 2 and N * N == m and N % 2 == 0 and off % 2 == 0
+        if ok:
+            D = d[off:off + m].reshape(N, N, order="F")        # D[i, j]: rho row i, column j
+            g, h = D[:, 0], D[0, :] - D[0, 0]
+            err = np.abs((D - (g[:, None] + h[None, :])).real) + np.abs((D - (g[:, None] + h[None, :])).imag)
+            sc = (np.abs(D.real) + np.abs(D.imag) + (np.abs(g.real) + np.abs(g.imag))[:, None]
+                  + (np.abs(h.real) + np.abs(h.imag))[None, :])
+            ok = bool(np.all(np.isfinite(D)) and np.all(err <= 1e-12 * sc))
+        ok_blk.append(ok)
+    out = []
+    for s0 in range(0, n, 128):
+        s1 = min(s0 + 128, n)
+        b = int(np.searchsorted(blocks, s0, side="right")) - 1
+        out.append(ok_blk[b] and s1 <= blocks[b + 1])
+    return out
+
+
+def _pair_slot_classes(rp, col, val, blocks=None):
     """The row-pair layout restated in Python: per lane (rows 2l, 2l+1) each entry
     of row 2l at column c pairs with an unused entry of row 2l+1 at c+1 (load
     x[c], x[c+1], halves (0, 1)), else c-1 (load from c-1, halves (1, 0)), else c
@@ -232,13 +263,19 @@ def _pair_slot_classes(rp, col, val):
     lane holds bit-identical (value row 2l, value row 2l+1) (zero for an absent
     row), scalar when all those are real or all imaginary, else complex; a
     scalar slot stores one real per lane when every lane's two rows hold the same
-    FP32 value (an absent row holds zero), else a real pair per lane.
-    Returns (slots, complex pairs stored, real pairs stored, single reals stored)."""
+    FP32 value (an absent row holds zero), else a real pair per lane.  In a
+    slice inside a block with a separable diagonal (_sep_slices) the slot
+    (0, 0, 1, 0) stores nothing when it is not uniform and every active lane holds
+    the diagonal entries of both its rows.
+    Returns (slots, complex pairs stored, real pairs stored, single reals stored,
+    separable-diagonal slots)."""
     n = len(rp) - 1
-    slots = ncpx = nreal = nsingle = 0
+    slots = ncpx = nreal = nsingle = nsep = 0
     z = np.complex128(0)
+    sep_ok = _sep_slices(rp, col, val, blocks if blocks is not None else [0, n])
     for s0 in range(0, n, 128):
         per_slot = {}
+        diag_lane = {}
         for r0 in range(s0, min(n, s0 + 128), 2):
             r1 = r0 + 1
             e0 = list(range(rp[r0], rp[r0 + 1]))
@@ -278,10 +315,15 @@ def _pair_slot_classes(rp, col, val):
                 seen[k] = rank + 1
                 v = (val[a] if a is not None else z, val[b] if b is not None else z)
                 per_slot.setdefault(k + (rank,), []).append(v)
-        for vs in per_slot.values():
+                dg = a is not None and b is not None and int(col[a]) == r0 and int(col[b]) == r1
+                diag_lane.setdefault(k + (rank,), []).append(dg)
+        for key, vs in per_slot.items():
             slots += 1
             bits = {(v0.real.tobytes(), v0.imag.tobytes(), v1.real.tobytes(), v1.imag.tobytes()) for v0, v1 in vs}
             if len(bits) == 1:
+                continue
+            if sep_ok[s0 // 128] and key == (0, 0, 1, 0) and all(diag_lane[key]):
+                nsep += 1
                 continue
             isre = all(v0.imag == 0 and v1.imag == 0 for v0, v1 in vs)
             if isre or all(v0.real == 0 and v1.real == 0 for v0, v1 in vs):
@@ -292,7 +334,7 @@ def _pair_slot_classes(rp, col, val):
                     nreal += len(vs)
             else:
                 ncpx += len(vs)
-    return slots, ncpx, nreal, nsingle
+    return slots, ncpx, nreal, nsingle, nsep
 
 
 @pytest.mark.parametrize("name", ["odd", "tc3"])
@@ -315,11 +357,15 @@ def test_row_pair_layout_counts(name):
         keep.append(b)
     rp, col, val = A.export()
     lay = A.pair_layout_info()
-    got = (lay["slots"], lay["stored_pairs"], lay["stored_real_pairs"], lay["stored_real_singles"])
-    want = _pair_slot_classes(rp, col, val)
+    got = (lay["slots"], lay["stored_pairs"], lay["stored_real_pairs"], lay["stored_real_singles"],
+           lay["sep_slots"])
+    blocks = None
+    if name == "tc3":
+        blocks = np.concatenate([[0], np.cumsum([t * t for t in tots])])
+    want = _pair_slot_classes(rp, col, val, blocks)
     assert got == want, (got, want)
     if name == "tc3":
-        assert want[3] > 0   # the Tavis-Cummings batch has equal-scalar slots
+        assert want[3] > 0 and want[4] > 0   # equal-scalar and separable-diagonal slots
     for k in keep:
         k.destroy()
 
