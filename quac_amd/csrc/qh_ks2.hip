@@ -554,8 +554,13 @@ __global__ void __launch_bounds__(WIN ? KS2_WIN / 2 : 256) k_ks2_stage(
   ks_run<U, true>(Ks2Ops<KS_SCA, false, AUX>{meta + km, r8, xr, sr}, ke - km, acc);
   ks_run<U, true>(Ks2Ops<KS_SC1, false, AUX>{meta + ke, r8, xr, s1r}, kc - ke, acc);
   ks_run<U, true>(Ks2Ops<KS_CPX, false, AUX>{meta + kc, r8, xr, vr}, kx - kc, acc);
-  if (kend > kx)
-    ks_run<U, true>(Ks2OpsSep{meta + kx, r8, (int64_t)r, xr, ks_rsrc(gh, (int64_t)ngh * 16), sepb}, kend - kx, acc);
+  if (kend > kx) {  // the one separable-diagonal slot of the slice, after the pipelined groups
+    Ks2OpsSep op{meta + kx, r8, (int64_t)r, xr, ks_rsrc(gh, (int64_t)ngh * 16), sepb};
+    Ks2OpsSep::Buf<1> sb;
+    op.issue<1>(sb);
+    op.fma<1>(sb, acc);
+    op.flush(acc);
+  }
   if (r >= n) return;
   const bool two = r + 1 < n;
   const float4 y = make_float4((float)acc.r0, (float)acc.i0, (float)acc.r1, (float)acc.i1);
