@@ -537,7 +537,8 @@ __global__ void __launch_bounds__(WIN ? KS2_WIN / 2 : 256) k_ks2_stage(
   const int lane = threadIdx.x & 63;
   const int32_t r = s * 128 + 2 * lane;
   const int32_t k0 = moff[s], kw = wmid[s], ku = smid[s], km = umid[s], ke = emid[s], kc = cmid[s],
-                kx = xmid[s], kend = moff[s + 1];
+                kend = moff[s + 1];
+  const int32_t kx = xmid ? xmid[s] : kend;  // xmid null: no separable-diagonal slots in this matrix
   const __amdgpu_buffer_rsrc_t vr = ks_rsrc(val, (int64_t)nval * 16);
   const __amdgpu_buffer_rsrc_t sr = ks_rsrc(sval, (int64_t)nsval * 8);
   const __amdgpu_buffer_rsrc_t s1r = ks_rsrc(sval1, (int64_t)nsval1 * 4);
@@ -818,7 +819,8 @@ int ks2_launch(int G, int mode, hipStream_t s, const qh_mat *A, const void *xin,
   const dim3 grid(qh_blocks(ns, win ? KS2_WIN / 128 : 4)), blk(win ? KS2_WIN / 2 : 256);
 #define QH_LK2(MM, UU, WW, NN)                                                                                   \
   hipLaunchKernelGGL((k_ks2_stage<MM, UU, WW, NN>), grid, blk, 0, s, n, ns, A->ks2_moff, A->ks2_wmid, A->ks2_smid,  \
-                     A->ks2_umid, A->ks2_emid, A->ks2_cmid, A->ks2_xmid, A->ks2_sep, (int32_t)A->ks2_ngh,      \
+                     A->ks2_umid, A->ks2_emid, A->ks2_cmid, A->ks2_nsep ? A->ks2_xmid : nullptr, A->ks2_sep,     \
+                     (int32_t)A->ks2_ngh,                                                                        \
                      A->ks2_gh, A->ks2_meta, (int32_t)A->ks2_nval, A->ks2_val,                                    \
                      (int32_t)A->ks2_nsval, A->ks2_sval, (int32_t)A->ks2_nsval1, A->ks2_sval1,                  \
                      (const float2 *)xin, (float2 *)kout, (float2 *)ynext, cn, (const float2 *)x,                \
