@@ -503,7 +503,9 @@ __device__ inline void st_pair(__amdgpu_buffer_rsrc_t rs, uint32_t o, bool two, 
 // NT: the matrix values, the state words, K1..K3 and the K and low-word stores
 // are streamed with the non-temporal cache policy (the gathered stage input and
 // the next stage input Y keep the default policy)
-template <int MODE, int U, bool WIN, bool NT = false>
+// SEP: the matrix has separable-diagonal slots (the code for them is left out
+// of the kernels of matrices without)
+template <int MODE, int U, bool WIN, bool NT = false, bool SEP = false>
 __global__ void __launch_bounds__(WIN ? KS2_WIN / 2 : 256) k_ks2_stage(
     int32_t n, int32_t nslices, const int32_t *__restrict__ moff, const int32_t *__restrict__ wmid,
     const int32_t *__restrict__ smid, const int32_t *__restrict__ umid, const int32_t *__restrict__ emid,
@@ -538,7 +540,7 @@ __global__ void __launch_bounds__(WIN ? KS2_WIN / 2 : 256) k_ks2_stage(
   const int32_t r = s * 128 + 2 * lane;
   const int32_t k0 = moff[s], kw = wmid[s], ku = smid[s], km = umid[s], ke = emid[s], kc = cmid[s],
                 kend = moff[s + 1];
-  const int32_t kx = xmid ? xmid[s] : kend;  // xmid null: no separable-diagonal slots in this matrix
+  const int32_t kx = SEP ? xmid[s] : kend;
   const __amdgpu_buffer_rsrc_t vr = ks_rsrc(val, (int64_t)nval * 16);
   const __amdgpu_buffer_rsrc_t sr = ks_rsrc(sval, (int64_t)nsval * 8);
   const __amdgpu_buffer_rsrc_t s1r = ks_rsrc(sval1, (int64_t)nsval1 * 4);
@@ -555,7 +557,7 @@ __global__ void __launch_bounds__(WIN ? KS2_WIN / 2 : 256) k_ks2_stage(
   ks_run<U, true>(Ks2Ops<KS_SCA, false, AUX>{meta + km, r8, xr, sr}, ke - km, acc);
   ks_run<U, true>(Ks2Ops<KS_SC1, false, AUX>{meta + ke, r8, xr, s1r}, kc - ke, acc);
   ks_run<U, true>(Ks2Ops<KS_CPX, false, AUX>{meta + kc, r8, xr, vr}, kx - kc, acc);
-  if (kend > kx) {  // the one separable-diagonal slot of the slice, after the pipelined groups
+  if (SEP && kend > kx) {  // the one separable-diagonal slot of the slice, after the pipelined groups
     Ks2OpsSep op{meta + kx, r8, (int64_t)r, xr, ks_rsrc(gh, (int64_t)ngh * 16), sepb};
     Ks2OpsSep::Buf<1> sb;
     op.issue<1>(sb);
@@ -817,14 +819,21 @@ int ks2_launch(int G, int mode, hipStream_t s, const qh_mat *A, const void *xin,
   const int32_t n = (int32_t)A->rows, ns = A->ks2_nslices;
   const bool win = G == 142 || G == 144;
   const dim3 grid(qh_blocks(ns, win ? KS2_WIN / 128 : 4)), blk(win ? KS2_WIN / 2 : 256);
-#define QH_LK2(MM, UU, WW, NN)                                                                                   \
-  hipLaunchKernelGGL((k_ks2_stage<MM, UU, WW, NN>), grid, blk, 0, s, n, ns, A->ks2_moff, A->ks2_wmid, A->ks2_smid,  \
-                     A->ks2_umid, A->ks2_emid, A->ks2_cmid, A->ks2_nsep ? A->ks2_xmid : nullptr, A->ks2_sep,     \
+#define QH_LK2S(MM, UU, WW, NN, SS)                                                                              \
+  hipLaunchKernelGGL((k_ks2_stage<MM, UU, WW, NN, SS>), grid, blk, 0, s, n, ns, A->ks2_moff, A->ks2_wmid, A->ks2_smid,  \
+                     A->ks2_umid, A->ks2_emid, A->ks2_cmid, A->ks2_xmid, A->ks2_sep,                               \
                      (int32_t)A->ks2_ngh,                                                                        \
                      A->ks2_gh, A->ks2_meta, (int32_t)A->ks2_nval, A->ks2_val,                                    \
                      (int32_t)A->ks2_nsval, A->ks2_sval, (int32_t)A->ks2_nsval1, A->ks2_sval1,                  \
                      (const float2 *)xin, (float2 *)kout, (float2 *)ynext, cn, (const float2 *)x,                \
                      (const float2 *)k1, (const float2 *)k2, (const float2 *)k3, (float2 *)xout, h, xlo)
+#define QH_LK2(MM, UU, WW, NN)            \
+  do {                                    \
+    if (A->ks2_nsep)                      \
+      QH_LK2S(MM, UU, WW, NN, true);      \
+    else                                  \
+      QH_LK2S(MM, UU, WW, NN, false);     \
+  } while (0)
   if (!QH_IS_KS2(G) || mode < MODE_SPMV || mode > MODE_FINAL) {
     qh_set_error("unsupported row-pair variant %d / mode %d", G, mode);
     return QH_EINVAL;
@@ -845,6 +854,7 @@ int ks2_launch(int G, int mode, hipStream_t s, const qh_mat *A, const void *xin,
     QH_LK2G(MODE_FINAL)
   }
 #undef QH_LK2G
+#undef QH_LK2S
 #undef QH_LK2
   QH_HIP(hipGetLastError());
   return QH_OK;
