@@ -177,7 +177,7 @@ def _odd_recipe():
                       ("lin", 0.1, [(1, "lower")]), ("lin", 0.05, [(0, "lower")])], "mcwf": False}
 
 
-@pytest.mark.parametrize("name", ["chain10", "tc_batch", "odd"])
+@pytest.mark.parametrize("name", ["chain10", "tc_batch", "odd", "tc_single"])
 def test_row_pair_layout_fp32_spmv(name):
     """FP32 SpMV on the row-pair sliced layout (140 .. 144) against the FP64 CSR
     product: every row within 1e-6 of sum_k |a_k| |x_k| (complex64 values and
@@ -190,6 +190,12 @@ def test_row_pair_layout_fp32_spmv(name):
     elif name == "odd":
         qs, _, _ = build_system(_odd_recipe())
         A, keep = qs.matrix(), [qs]
+    elif name == "tc_single":
+        # one Tavis-Cummings system (rho 160 x 160): its diagonal separates, so the
+        # layout forms it from g_i + h_j (one block: the whole matrix)
+        qs, _, _ = build_system(recipes.tc_recipe(3, 20, 0.1, 0.05, 5e-3, 5e-3))
+        A, keep = qs.matrix(), [qs]
+        assert A.pair_layout_info()["sep_slots"] > 0
     else:
         mats, tots = [], []
         for p in recipes.tc_instances(40, 3):
