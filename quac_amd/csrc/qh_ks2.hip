@@ -506,7 +506,7 @@ __device__ inline void st_pair(__amdgpu_buffer_rsrc_t rs, uint32_t o, bool two, 
 // SEP: the matrix has separable-diagonal slots (the code for them is left out
 // of the kernels of matrices without)
 template <int MODE, int U, bool WIN, bool NT = false, bool SEP = false>
-__global__ void __launch_bounds__(WIN ? KS2_WIN / 2 : 256) k_ks2_stage(
+__global__ void __launch_bounds__(WIN ? KS2_WIN / 2 : 256, WIN ? 8 : 1) k_ks2_stage(
     int32_t n, int32_t nslices, const int32_t *__restrict__ moff, const int32_t *__restrict__ wmid,
     const int32_t *__restrict__ smid, const int32_t *__restrict__ umid, const int32_t *__restrict__ emid,
     const int32_t *__restrict__ cmid, const int32_t *__restrict__ xmid, const qh_sepblk *__restrict__ sepb,
@@ -522,36 +522,35 @@ __global__ void __launch_bounds__(WIN ? KS2_WIN / 2 : 256) k_ks2_stage(
   const int64_t vb = (int64_t)n * 8;
   const __amdgpu_buffer_rsrc_t xr = ks_rsrc(xin, vb);
   __shared__ float2 xs[WIN ? KS2_WIN + 2 : 2];
+  // WIN: the window rows [b KS2_WIN, (b+1) KS2_WIN), two per thread (zeros past
+  // n), are loaded first but written to LDS only after the slots read from
+  // global memory have been processed, so the copy's latency hides behind them;
+  // every wavefront of the workgroup reaches the barrier
+  float4 wv = make_float4(0.f, 0.f, 0.f, 0.f);
   if (WIN) {
-    // the window rows [b KS2_WIN, (b+1) KS2_WIN), two per thread (zeros past n)
     const int64_t w = (int64_t)b * KS2_WIN + 2 * threadIdx.x;
-    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
     if (w + 1 < n)
-      v = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(xr, (uint32_t)w * 8u, 0, 0));
+      wv = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(xr, (uint32_t)w * 8u, 0, 0));
     else if (w < n)
-      v = ld_pair(xr, (uint32_t)w * 8u, false);
-    xs[2 * threadIdx.x] = make_float2(v.x, v.y);
-    xs[2 * threadIdx.x + 1] = make_float2(v.z, v.w);
-    if (threadIdx.x == 0) xs[KS2_WIN] = xs[KS2_WIN + 1] = make_float2(0.f, 0.f);
-    __syncthreads();
+      wv = ld_pair(xr, (uint32_t)w * 8u, false);
+  } else if (s >= nslices) {
+    return;  // whole wavefront
   }
-  if (s >= nslices) return;  // whole wavefront (after the window copy)
+  const bool live = s < nslices;
   const int lane = threadIdx.x & 63;
   const int32_t r = s * 128 + 2 * lane;
-  const int32_t k0 = moff[s], kw = wmid[s], ku = smid[s], km = umid[s], ke = emid[s], kc = cmid[s],
-                kend = moff[s + 1];
-  const int32_t kx = SEP ? xmid[s] : kend;
+  const int32_t k0 = live ? moff[s] : 0, kw = live ? wmid[s] : 0, ku = live ? smid[s] : 0,
+                km = live ? umid[s] : 0, ke = live ? emid[s] : 0, kc = live ? cmid[s] : 0,
+                kend = live ? moff[s + 1] : 0;
+  const int32_t kx = (SEP && live) ? xmid[s] : kend;
   const __amdgpu_buffer_rsrc_t vr = ks_rsrc(val, (int64_t)nval * 16);
   const __amdgpu_buffer_rsrc_t sr = ks_rsrc(sval, (int64_t)nsval * 8);
   const __amdgpu_buffer_rsrc_t s1r = ks_rsrc(sval1, (int64_t)nsval1 * 4);
   const uint32_t r8 = (uint32_t)r * 8u;
   KsAcc2 acc{0.0, 0.0, 0.0, 0.0, 0.f, 0.f, 0.f, 0.f};
-  if (WIN) {
-    ks_run<U, true>(Ks2OpsWin{meta + k0, (uint32_t)(r - (int32_t)b * KS2_WIN) * 8u, xs}, kw - k0, acc);
-    ks_run<U, true>(Ks2Ops<KS_UNI, true>{meta + kw, r8, xr, vr}, ku - kw, acc);
-  } else {
-    ks_run<U, true>(Ks2Ops<KS_UNI, true>{meta + k0, r8, xr, vr}, ku - k0, acc);
-  }
+  // (a wavefront past the last slice has empty slot ranges: k0 = ... = kend = 0)
+  ks_run<U, true>(Ks2Ops<KS_UNI, true>{meta + kw, r8, xr, vr}, ku - kw, acc);
+  if (!WIN) ks_run<U, true>(Ks2Ops<KS_UNI, true>{meta + k0, r8, xr, vr}, kw - k0, acc);
   constexpr int AUX = NT ? 2 : 0;
   ks_run<U, true>(Ks2Ops<KS_UNI>{meta + ku, r8, xr, vr}, km - ku, acc);
   ks_run<U, true>(Ks2Ops<KS_SCA, false, AUX>{meta + km, r8, xr, sr}, ke - km, acc);
@@ -564,6 +563,14 @@ __global__ void __launch_bounds__(WIN ? KS2_WIN / 2 : 256) k_ks2_stage(
     op.fma<1>(sb, acc);
     op.flush(acc);
   }
+  if (WIN) {
+    xs[2 * threadIdx.x] = make_float2(wv.x, wv.y);
+    xs[2 * threadIdx.x + 1] = make_float2(wv.z, wv.w);
+    if (threadIdx.x == 0) xs[KS2_WIN] = xs[KS2_WIN + 1] = make_float2(0.f, 0.f);
+    __syncthreads();
+    ks_run<U, true>(Ks2OpsWin{meta + k0, (uint32_t)(r - (int32_t)b * KS2_WIN) * 8u, xs}, kw - k0, acc);
+  }
+  if (!live) return;
   if (r >= n) return;
   const bool two = r + 1 < n;
   const float4 y = make_float4((float)acc.r0, (float)acc.i0, (float)acc.r1, (float)acc.i1);
