@@ -1,6 +1,6 @@
 """Edge cases of the device path: zero steps, empty and minimal systems,
 single-instance batches, mismatched sizes and bad gate strides (errors are
-reported, nothing runs), and one MCWF trajectory."""
+reported, nothing runs), and the smallest MCWF ensemble."""
 import ctypes
 
 import numpy as np
@@ -95,16 +95,20 @@ def test_size_mismatch_and_bad_gate_strides_are_errors():
     qs.destroy()
 
 
-def test_one_mcwf_trajectory():
-    """an ensemble of one trajectory (a wavefront holding a single segment): a
-    decaying qubit with no drive has jumped to the ground state by t = 20 (the
-    chance it has not is e^-10), so the ensemble's excited population is zero"""
+def test_smallest_mcwf_ensemble():
+    """use_mcwf_solver refuses fewer than two trajectories, as the reference
+    does (qsystem.c:372-373, whose message says "more than 2"); the smallest
+    ensemble, two trajectories in one wavefront, of a decaying qubit with no
+    drive has jumped to the ground state by t = 20 (the chance one has not is
+    2 e^-10), so its excited population is zero"""
     _fixed_rk4()
     qs = qa.System()
     q = qs.create_op(2)
     qs.add_ham_term(1.0, q.n)
     qs.add_lin_term(0.5, q)
-    qs.use_mcwf_solver(1, 7)
+    with pytest.raises(qa.QuacError, match="more than 2"):
+        qs.use_mcwf_solver(1, 7)
+    qs.use_mcwf_solver(2, 7)
     qs.construct_matrix()
     x = qs.create_qvec()
     x.add(1.0, 1)
