@@ -241,7 +241,8 @@ def copy(args):
             "n_gpus": 1, "steps": reps, "warmup": 1, "ms_per_step": round(ms, 5), "higher_is_better": True,
             "scaling": "weak", "vs_baseline": None, "dtype": "f64 -> f32 pair", "data": "zeros",
             "config": {"workload": "k_split_d2 over 16 777 216 complex128 elements"},
-            "read_bytes_per_launch": rd, "write_bytes_per_launch": wr}
+            "read_bytes_per_launch": rd, "write_bytes_per_launch": wr,
+            "stream_copy_read_gbs_1gib": [round(v, 1) for v in qa.bench_stream(1 << 30, 10)]}
 
 
 def kerr(args):

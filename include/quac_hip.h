@@ -329,6 +329,10 @@ int qh_last_stage_ms(double *spmv_ms, int64_t *launches);
 int qh_set_graph_steps(int steps);
 /* mean duration (ms) of reps plain SpMV launches y = A x on the chosen precision */
 int qh_bench_spmv(qh_mat *A, int prec, int reps, double *avg_ms);
+/* achievable HBM bandwidth: reps one-pass streaming copies (bytes read + bytes
+ * written, 16 bytes per thread) and reps read-only passes over a fresh buffer of
+ * `bytes` (a multiple of 16, at least 4096); GB/s of each */
+int qh_bench_stream(int64_t bytes, int reps, double *copy_gbs, double *read_gbs);
 /* HIP event pair on the library stream around a caller's region; qh_timer_end
  * waits for the region to finish and returns its device time in ms */
 int qh_timer_begin(void);

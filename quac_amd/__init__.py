@@ -212,6 +212,7 @@ def _setup(L):
     sig("qh_last_stage_ms", i32, P(dbl), P(i64))
     sig("qh_set_graph_steps", i32, i32)
     sig("qh_bench_spmv", i32, vp, i32, i32, P(dbl))
+    sig("qh_bench_stream", i32, i64, i32, P(dbl), P(dbl))
     sig("qh_pc_create_hbasis", i32, vp, P(vp))
     sig("qh_pc_apply", i32, vp, vp, vp)
     sig("qh_pc_destroy", i32, vp)
@@ -808,6 +809,13 @@ def bench_spmv(mat, prec, reps=20):
     ms = ctypes.c_double()
     _qh(lib().qh_bench_spmv(mat.h, prec, reps, ctypes.byref(ms)))
     return ms.value
+
+
+def bench_stream(nbytes=1 << 30, reps=10):
+    """(copy GB/s, read GB/s) of one-pass streaming kernels over nbytes buffers"""
+    c, r = ctypes.c_double(), ctypes.c_double()
+    _qh(lib().qh_bench_stream(nbytes, reps, ctypes.byref(c), ctypes.byref(r)))
+    return c.value, r.value
 
 
 class StreamTimer:

@@ -1,6 +1,8 @@
 """Edge cases of the device path: zero steps, empty and minimal systems,
 single-instance batches, mismatched sizes and bad gate strides (errors are
-reported, nothing runs), and the smallest MCWF ensemble."""
+reported, nothing runs), the smallest MCWF ensemble, and the bandwidth
+calibration's argument checks and ordinary vector kernels on a size that is
+not a multiple of the block."""
 import ctypes
 
 import numpy as np
@@ -117,3 +119,23 @@ def test_smallest_mcwf_ensemble():
     e = x.expectation(q.n)
     assert abs(e.real) < 1e-9 and abs(e.imag) < 1e-12
     qs.destroy()
+
+
+def test_stream_calibration_and_ragged_vector_kernels():
+    c, r = qa.bench_stream(1 << 24, 2)
+    assert 100.0 < c < 20000.0 and 100.0 < r < 20000.0
+    with pytest.raises(qa.QuacError):
+        qa.bench_stream(1000, 2)          # below 4096 bytes and not a multiple of 16
+    # FP64 -> FP32 hi + lo split and back over one element per thread, with a
+    # last partial block
+    n = 256 * 37 + 5
+    rng = np.random.default_rng(4)
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    v64 = qa.DeviceVec(n, qa.FP64)
+    v64.set(x)
+    v32 = qa.DeviceVec(n, qa.FP32)
+    qa._qh(qa.lib().qh_vec_copy(v64.h, v32.h))
+    back = qa.DeviceVec(n, qa.FP64)
+    qa._qh(qa.lib().qh_vec_copy(v32.h, back.h))
+    y = back.get()
+    assert np.max(np.abs(y - x)) <= 1e-14 * np.max(np.abs(x))
