@@ -288,12 +288,26 @@ void quac_set_error_mode(int mode);
 const char *quac_last_error(void);   /* "" when the last call succeeded */
 void quac_clear_error(void);
 /* options normally given as PETSc command-line flags: -ts_rk_type {3bs,4},
- * -ts_adapt_type {basic,none}, -quac_precision {fp64,fp32}, -quac_renorm N,
+ * -ts_adapt_type {basic,none}, -quac_precision {fp64,fp32}, -quac_batch_precision {fp32,fp64}, -quac_renorm N,
  * -quac_kron_conj_a {0,1}, -ksp_rtol X, -ksp_gmres_restart N, -quac_jacobi {0,1} */
 void quac_set_option(const char *name, const char *value);
 const char *quac_get_option(const char *name);
-/* last time_step_sys: steps taken and final time */
+/* last time_step_sys or time_step_sys_batch: steps taken and final time */
 void quac_ts_stats(PetscInt *steps, PetscReal *t_final);
+/* Batched parameter sweeps (no counterpart in the reference, which steps one
+ * qsystem per time_step_sys call, qsystem.c:895-1098): n constructed
+ * density-matrix systems without time-dependent terms, circuits or MCWF,
+ * advanced together by fixed-step classic RK4 from init_time to time_max in
+ * steps of dt (the last one shortened to land on time_max; at most steps_max),
+ * states[i] in place.  One block-diagonal device matrix and one fused stage
+ * kernel per RK stage over every instance, replayed as a HIP graph; the batch is
+ * kept for the next call with the same matrices.  -quac_batch_precision
+ * {fp32 (default), fp64}; FP32 rescales every instance to unit trace every
+ * -quac_renorm steps (0: never).  Monitors are not called. */
+void time_step_sys_batch(PetscInt n, qsystem *systems, qvec *states, PetscReal init_time, PetscReal time_max,
+                         PetscReal dt, PetscInt steps_max);
+/* frees the batch kept by time_step_sys_batch (QuaC_finalize does too) */
+void time_step_sys_batch_release(void);
 /* TS accessors usable from monitors */
 void TSGetTime(TS, PetscReal *);
 void TSGetStepNumber(TS, PetscInt *);

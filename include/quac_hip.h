@@ -214,6 +214,13 @@ int qh_batch_info(const qh_batch *b, int64_t *rows, int64_t *nnz);
 qh_mat *qh_batch_matrix(qh_batch *b);
 qh_vec *qh_batch_state_vec(qh_batch *b);
 int64_t qh_batch_offset(qh_batch *b, int i);
+/* device-to-device copy of instance i's state from / into a vector of any
+ * precision with the instance's row count (no host round trip) */
+int qh_batch_set_state_vec(qh_batch *b, int i, const qh_vec *src);
+int qh_batch_get_state_vec(qh_batch *b, int i, qh_vec *dst);
+/* serial number of a device matrix: distinct for every matrix the process
+ * creates (an address can be reused after a destroy, a serial number is not) */
+uint64_t qh_mat_uid(const qh_mat *m);
 
 /* ------------------------------------------------------------ steady state */
 /*

@@ -127,6 +127,11 @@ def _setup(L):
     sig("construct_matrix", None, vp)
     sig("construct_op_matrix_wf_list", None, vp, Cx, P(vp), i64, P(OpPtr))
     sig("time_step_sys", None, vp, vp, dbl, dbl, dbl, i64)
+    sig("time_step_sys_batch", None, i64, P(vp), P(vp), dbl, dbl, dbl, i64)
+    sig("time_step_sys_batch_release", None)
+    sig("qh_batch_set_state_vec", i32, vp, i32, vp)
+    sig("qh_batch_get_state_vec", i32, vp, i32, vp)
+    sig("qh_mat_uid", ctypes.c_uint64, vp)
     sig("set_ts_monitor_sys", None, vp, MONITOR, vp)
     sig("quac_ts_stats", None, P(i64), P(dbl))
     sig("create_qvec_sys", None, vp, P(vp))
@@ -781,6 +786,19 @@ class Batch:
         if self.h:
             _qh(lib().qh_batch_destroy(self.h))
             self.h = None
+
+
+def time_step_batch(systems, states, t0, tmax, dt, steps_max):
+    """time_step_sys_batch: the systems advanced together (fixed-step RK4), each
+    state in place; returns (steps, final time) of quac_ts_stats"""
+    n = len(systems)
+    hs = (ctypes.c_void_p * n)(*[s.h for s in systems])
+    xs = (ctypes.c_void_p * n)(*[x.h for x in states])
+    lib().time_step_sys_batch(n, hs, xs, t0, tmax, dt, steps_max)
+    _check()
+    st, t = ctypes.c_int64(), ctypes.c_double()
+    lib().quac_ts_stats(ctypes.byref(st), ctypes.byref(t))
+    return st.value, t.value
 
 
 def rk4_device(mat, vec, dt, nsteps, tot=0, renorm_every=0, timed=False):

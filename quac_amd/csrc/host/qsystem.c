@@ -744,3 +744,103 @@ void time_step_sys(qsystem qsys, qvec x, PetscReal init_time, PetscReal time_max
   g_last_t = t;
   if (fabs(t - time_max) > 1e-8) PetscPrintf(0, "ERROR! Did not reach final time! Reason: %d\n", -1);
 }
+
+/* ------------------------------------------------------------ batched sweeps */
+/* The batch of the last time_step_sys_batch call, kept for the next call with
+ * the same matrices (by serial number) and precision. */
+static struct {
+  qh_batch *b;
+  PetscInt n;
+  uint64_t *uid;
+  int prec;
+} g_batch = {NULL, 0, NULL, 0};
+
+void time_step_sys_batch_release(void) {
+  if (g_batch.b) qh_batch_destroy(g_batch.b);
+  free(g_batch.uid);
+  g_batch.b = NULL;
+  g_batch.uid = NULL;
+  g_batch.n = 0;
+}
+
+/* n independent density-matrix systems advanced together by fixed-step classic
+ * RK4 from init_time to time_max (steps of dt, the last one shortened to land
+ * on time_max, at most steps_max): one block-diagonal device matrix, one fused
+ * stage kernel per RK stage over every instance, hipGraph replay.  The systems
+ * must be constructed density-matrix systems without time-dependent terms,
+ * circuits or the MCWF solver; x[i] is advanced in place.  Precision:
+ * -quac_batch_precision {fp32 (default), fp64}; FP32 rescales each instance to
+ * unit trace every -quac_renorm steps (0: never).  Monitors are not called. */
+void time_step_sys_batch(PetscInt n, qsystem *sys, qvec *x, PetscReal init_time, PetscReal time_max, PetscReal dt,
+                         PetscInt steps_max) {
+  if (n < 1 || !sys || !x) QUAC_FAIL(0, "time_step_sys_batch: no systems");
+  if (dt <= 0) QUAC_FAIL(0, "time_step_sys_batch: dt must be positive");
+  for (PetscInt i = 0; i < n; i++) {
+    qsystem s = sys[i];
+    if (!s || s->mat_allocated != 1)
+      QUAC_FAIL(0, "time_step_sys_batch: system %lld is not constructed (call construct_matrix)", (long long)i);
+    if (!s->dm_equations || s->mcwf_solver)
+      QUAC_FAIL(0, "time_step_sys_batch: system %lld is not a density-matrix system", (long long)i);
+    if (s->num_time_dep > 0 || s->num_circuits > 0)
+      QUAC_FAIL(0, "time_step_sys_batch: system %lld has time-dependent terms or circuits", (long long)i);
+    if (!x[i] || !x[i]->data || x[i]->n != s->dim)
+      QUAC_FAIL(0, "time_step_sys_batch: state %lld does not match its system", (long long)i);
+  }
+  const int prec = strcmp(quac_opt_str("quac_batch_precision", "fp32"), "fp64") ? QH_FP32 : QH_FP64;
+  const int renorm = prec == QH_FP32 ? quac_opt_int("quac_renorm", 0) : 0;
+  int same = g_batch.b && g_batch.n == n && g_batch.prec == prec;
+  for (PetscInt i = 0; same && i < n; i++) same = g_batch.uid[i] == qh_mat_uid(sys[i]->mat_A);
+  if (!same) {
+    time_step_sys_batch_release();
+    qh_mat **mats = (qh_mat **)malloc(sizeof(qh_mat *) * (size_t)n);
+    int64_t *tot = (int64_t *)malloc(sizeof(int64_t) * (size_t)n);
+    g_batch.uid = (uint64_t *)malloc(sizeof(uint64_t) * (size_t)n);
+    if (!mats || !tot || !g_batch.uid) {
+      free(mats);
+      free(tot);
+      time_step_sys_batch_release();
+      QUAC_FAIL(9, "time_step_sys_batch: out of host memory");
+    }
+    for (PetscInt i = 0; i < n; i++) {
+      mats[i] = sys[i]->mat_A;
+      tot[i] = sys[i]->total_levels;
+      g_batch.uid[i] = qh_mat_uid(sys[i]->mat_A);
+    }
+    const int rc = qh_batch_create((int)n, mats, tot, prec, &g_batch.b);
+    free(mats);
+    free(tot);
+    if (rc) {
+      g_batch.b = NULL;
+      time_step_sys_batch_release();
+      QUAC_FAIL(9, "time_step_sys_batch: %s", qh_last_error());
+    }
+    g_batch.n = n;
+    g_batch.prec = prec;
+  }
+  for (PetscInt i = 0; i < n; i++) QUAC_HIPCHK(qh_batch_set_state_vec(g_batch.b, (int)i, x[i]->data));
+  const PetscReal eps = 1e-12 * (fabs(time_max) + fabs(dt) + 1.0);
+  PetscReal t = init_time;
+  PetscInt steps = 0;
+  if (time_max > t + eps && steps_max > 0) {
+    PetscInt nfull = (PetscInt)floor((time_max - t) / dt + 1e-9);
+    if (nfull > steps_max) nfull = steps_max;
+    if (nfull > 0) {
+      QUAC_HIPCHK(qh_batch_rk4(g_batch.b, dt, nfull, renorm));
+      t += (double)nfull * dt;
+      steps = nfull;
+    }
+    if (steps < steps_max && time_max - t > eps) {
+      QUAC_HIPCHK(qh_batch_rk4(g_batch.b, time_max - t, 1, 0));
+      t = time_max;
+      steps++;
+    }
+  }
+  for (PetscInt i = 0; i < n; i++) {
+    QUAC_HIPCHK(qh_batch_get_state_vec(g_batch.b, (int)i, x[i]->data));
+    sys[i]->time_step_called = PETSC_TRUE;
+    sys[i]->current_time = t;
+  }
+  QUAC_HIPCHK(qh_synchronize());
+  g_last_steps = steps;
+  g_last_t = t;
+}

@@ -126,6 +126,7 @@ void QuaC_initialize(int argc, char **args) {
 void QuaC_clear(void) { qh_synchronize(); }
 
 void QuaC_finalize(void) {
+  time_step_sys_batch_release();
   if (g_initialized) qh_finalize();
   g_initialized = 0;
 }

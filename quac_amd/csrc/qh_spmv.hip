@@ -1995,6 +1995,40 @@ extern "C" int qh_batch_get_state(qh_batch *b, int i, double *host) {
   return rc;
 }
 
+// the batch state slice of instance i as a qh_vec view (owns nothing)
+static qh_vec batch_slice(qh_batch *b, int i) {
+  qh_vec slice;
+  slice.n = b->rows[i];
+  slice.prec = b->x->prec;
+  slice.d = (char *)b->x->d + b->off[i] * (b->x->prec == QH_FP64 ? 16 : 8);
+  slice.lo = b->x->lo ? b->x->lo + b->off[i] : nullptr;
+  return slice;
+}
+
+extern "C" int qh_batch_set_state_vec(qh_batch *b, int i, const qh_vec *src) {
+  QH_REQUIRE(b && src && i >= 0 && i < b->n, QH_EINVAL, "qh_batch_set_state_vec: bad instance");
+  QH_REQUIRE(src->n == b->rows[i], QH_EINVAL, "qh_batch_set_state_vec: state size %lld, instance %d has %lld rows",
+             (long long)src->n, i, (long long)b->rows[i]);
+  qh_vec slice = batch_slice(b, i);
+  const int rc = qh_vec_copy(src, &slice);
+  slice.d = nullptr;
+  slice.lo = nullptr;
+  return rc;
+}
+
+extern "C" int qh_batch_get_state_vec(qh_batch *b, int i, qh_vec *dst) {
+  QH_REQUIRE(b && dst && i >= 0 && i < b->n, QH_EINVAL, "qh_batch_get_state_vec: bad instance");
+  QH_REQUIRE(dst->n == b->rows[i], QH_EINVAL, "qh_batch_get_state_vec: state size %lld, instance %d has %lld rows",
+             (long long)dst->n, i, (long long)b->rows[i]);
+  qh_vec slice = batch_slice(b, i);
+  const int rc = qh_vec_copy(&slice, dst);
+  slice.d = nullptr;
+  slice.lo = nullptr;
+  return rc;
+}
+
+extern "C" uint64_t qh_mat_uid(const qh_mat *m) { return m ? m->uid : 0; }
+
 extern "C" int qh_batch_rk4(qh_batch *b, double dt, int64_t nsteps, int renorm_every) {
   QH_REQUIRE(b, QH_EINVAL, "null batch");
   QH_REQUIRE(renorm_every == 0 || b->x->prec == QH_FP32, QH_EINVAL, "batch renormalisation needs FP32");

@@ -169,3 +169,32 @@ def test_legacy_dm_utilities_closed_forms(tmp_path):
     want = np.zeros((4, 4))
     want[2, 2], want[3, 3] = 0.1, 0.9
     assert np.abs(got - want).max() < 1e-14, got
+
+
+def test_c_sweep_through_time_step_sys_batch(tmp_path):
+    """tests/c/jc_sweep_batch.c: eight Jaynes-Cummings systems of 4 .. 11 cavity
+    levels advanced together by time_step_sys_batch (FP32 batch, unit trace every
+    100 steps) against time_step_sys on each alone (FP64 RK4), and the first
+    system's single run against the oracle"""
+    exe = _build(tmp_path, "jc_sweep_batch")
+    out = _run(exe, "-ts_rk_type", "4", "-ts_adapt_type", "none", "-quac_renorm", "100")
+    rows = {}
+    for l in out.splitlines():
+        p = l.split()
+        if len(p) == 5 and p[0] in ("batch", "single"):
+            rows[(p[0], int(p[1]))] = [float(v) for v in p[2:]]
+    assert len(rows) == 16
+    for i in range(8):
+        b, s = rows[("batch", i)], rows[("single", i)]
+        assert abs(b[0] - s[0]) < 1e-6 and abs(b[1] - s[1]) < 1e-6, (i, b, s)
+        assert abs(b[2] - 1.0) < 1e-6 and abs(s[2] - 1.0) < 1e-12
+    r = recipes.jc_recipe(levels=4, wc=1.0 + 0.02 * (0 - 3.5), kappa=2e-2)
+    tot, terms = recipes.oracle_terms(r)
+    m = oracle.construct(tot, terms, dm=True)
+    x = np.zeros(tot * tot, np.complex128)
+    x[4 * tot + 4] = 1.0
+    x = m.rk4(x, 0.01, 1000)
+    pe = oracle.expect_dm(tot, x, [(0, 2, 1, -1)]).real
+    pc = oracle.expect_dm(tot, x, [(0, 4, 2, -1)]).real
+    s = rows[("single", 0)]
+    assert abs(s[0] - pe) < 1e-10 and abs(s[1] - pc) < 1e-10
