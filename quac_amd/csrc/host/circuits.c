@@ -281,17 +281,18 @@ int qc_apply_gate(struct quantum_gate_struct *g, qvec x, int superop, qsystem sy
  * 127-132, 146-151), otherwise on the state itself */
 static int apply_on_members(struct quantum_gate_struct *g, qvec state, int superop, qsystem sys) {
   if (state->my_type != WF_ENSEMBLE || !state->ens_spawned) return qc_apply_gate(g, state, superop, sys);
-  struct qvec member = *state;
-  member.my_type = WAVEFUNCTION;
-  for (PetscInt i = 0; i < state->n_ensemble; i++) {
-    Vec v = NULL;
-    if (qh_vec_view(state->ens_datas[0], i * state->n, state->n, &v)) return quac_fail(9, "%s", qh_last_error());
-    member.data = v;
-    const int rc = qc_apply_gate(g, &member, superop, sys);
-    qh_vec_destroy(v);
-    if (rc) return rc;
-  }
-  return 0;
+  /* the trajectories are consecutive blocks of N = total_levels entries, and a
+   * qubit's stride s has 2s dividing N, so one launch over the whole ensemble as
+   * a single vector of M N entries pairs the same entries as M launches would */
+  struct qvec all = *state;
+  all.my_type = WAVEFUNCTION;
+  all.total_levels = state->n * state->n_ensemble;
+  Vec v = NULL;
+  if (qh_vec_view(state->ens_datas[0], 0, all.total_levels, &v)) return quac_fail(9, "%s", qh_last_error());
+  all.data = v;
+  const int rc = qc_apply_gate(g, &all, superop, sys);
+  qh_vec_destroy(v);
+  return rc;
 }
 
 /* quantum_circuits.c:257-314: the circuit's matrix G_n ... G_1 on the Hilbert
@@ -385,27 +386,28 @@ int qc_next_event_time(qsystem sys, PetscReal *t) {
 }
 
 /* _sys_QC_PostEventFunction, quantum_circuits.c:58-120 */
-void qc_apply_events(qsystem sys, PetscReal t, qvec x) {
+int qc_apply_events(qsystem sys, PetscReal t, qvec x) {
   (void)t;
-  if (sys->current_circuit >= sys->num_circuits) return;
+  if (sys->current_circuit >= sys->num_circuits) return 0;
   circuit *c = &sys->circuit_list[sys->current_circuit];
   if (c->num_layers > 0) {
     PetscReal lt = c->layer_list[c->current_layer].time;
     while (c->current_layer < c->num_layers && c->layer_list[c->current_layer].time == lt) {
       struct gate_layer_struct *L = &c->layer_list[c->current_layer];
       for (PetscInt i = 0; i < L->num_gates; i++)
-        if (qc_apply_gate(&L->gate_list[i], x, 1, sys)) return;
+        if (apply_on_members(&L->gate_list[i], x, 1, sys)) return 1;
       c->current_layer++;
     }
     if (c->current_layer >= c->num_layers) sys->current_circuit++;
   } else {
     PetscReal gt = c->gate_list[c->current_gate].time;
     while (c->current_gate < c->num_gates && c->gate_list[c->current_gate].time == gt) {
-      if (qc_apply_gate(&c->gate_list[c->current_gate], x, 0, sys)) return;
+      if (apply_on_members(&c->gate_list[c->current_gate], x, 0, sys)) return 1;
       c->current_gate++;
     }
     if (c->current_gate >= c->num_gates) sys->current_circuit++;
   }
+  return 0;
 }
 
 /* ------------------------------------------- non-variadic forms for FFI */

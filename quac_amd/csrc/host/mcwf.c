@@ -279,7 +279,8 @@ static int jump_in_step(qsystem s, Vec pre, Vec block, int64_t i, int64_t N, dou
 }
 
 int quac_mcwf_time_step(qsystem s, qvec x, PetscReal t0, PetscReal tmax, PetscReal dt, PetscInt steps_max,
-                        int (*monitor)(qsystem, PetscInt, PetscReal, qvec), PetscInt *nsteps) {
+                        int (*monitor)(qsystem, PetscInt, PetscReal, qvec), PetscInt *nsteps, PetscInt step0,
+                        int call_initial) {
   if (x->my_type != WF_ENSEMBLE) return quac_fail(0, "time_step_sys with use_mcwf_solver needs a qvec from create_qvec_sys");
   if (build_jumps(s) || spawn(s, x)) return 9;
   const int64_t M = x->n_ensemble, N = x->n;
@@ -291,7 +292,7 @@ int quac_mcwf_time_step(qsystem s, qvec x, PetscReal t0, PetscReal tmax, PetscRe
   PetscInt step = 0;
   const PetscReal eps = 1e-12 * (fabs(tmax) + fabs(dt) + 1.0);
   int rc = 0;
-  if (monitor) rc = monitor(s, 0, t, x);
+  if (monitor && call_initial) rc = monitor(s, step0, t, x);
   if (!monitor && fast_path(s, N)) {
     /* the step sizes of the loop below: dt, the last one cut at tmax */
     PetscReal hlast = dt;
@@ -318,7 +319,7 @@ int quac_mcwf_time_step(qsystem s, qvec x, PetscReal t0, PetscReal tmax, PetscRe
       if (nrm[i] < s->rand_number[i]) rc = jump_in_step(s, pre, block, i, N, h, work);
     t += h;
     step++;
-    if (!rc && monitor) rc = monitor(s, step, t, x);
+    if (!rc && monitor) rc = monitor(s, step0 + step, t, x);
   }
   qh_vec_destroy(pre);
   free(nrm);

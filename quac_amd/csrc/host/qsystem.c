@@ -581,12 +581,32 @@ void time_step_sys(qsystem qsys, qvec x, PetscReal init_time, PetscReal time_max
                                    (long long)x->n, (long long)qsys->dim);
   if (dt <= 0) QUAC_FAIL(0, "time_step_sys: dt must be positive");
   if (qsys->mcwf_solver) {
-    /* trajectories of the ensemble (qsystem.c:1024-1079) */
-    PetscInt nsteps = 0;
-    if (quac_mcwf_time_step(qsys, x, init_time, time_max, dt, steps_max, qsys->ts_monitor ? mcwf_monitor : NULL,
-                            &nsteps))
-      QUAC_FAIL(9, "time_step_sys (MCWF) failed");
-    g_last_steps = nsteps;
+    /* trajectories of the ensemble (qsystem.c:1024-1079); a circuit's gates
+     * are events (qsystem.c:1006-1019, quantum_circuits.c:17-120): the ensemble
+     * runs to each gate time, where every trajectory takes the gate, and then
+     * on (steps of dt again from the event) */
+    const PetscReal eps = 1e-12 * (fabs(time_max) + fabs(dt) + 1.0);
+    PetscReal t = init_time;
+    PetscInt total = 0;
+    int first = 1;
+    for (;;) {
+      PetscReal t_ev = 0.0, tend = time_max;
+      const int have_ev = qsys->num_circuits > 0 && qc_next_event_time(qsys, &t_ev) && t_ev > t + eps &&
+                          t_ev < time_max + eps;
+      if (have_ev && t_ev < tend) tend = t_ev;
+      PetscInt nsteps = 0;
+      if (quac_mcwf_time_step(qsys, x, t, tend, dt, steps_max - total, qsys->ts_monitor ? mcwf_monitor : NULL,
+                              &nsteps, total, first))
+        QUAC_FAIL(9, "time_step_sys (MCWF) failed");
+      first = 0;
+      total += nsteps;
+      t = qsys->current_time;
+      if (!(have_ev && fabs(t - t_ev) <= 1e-9)) break;
+      t = t_ev;
+      if (qc_apply_events(qsys, t, x)) return;
+      if (t >= time_max - eps || total >= steps_max) break;
+    }
+    g_last_steps = total;
     g_last_t = qsys->current_time;
     return;
   }
@@ -664,8 +684,9 @@ void time_step_sys(qsystem qsys, qvec x, PetscReal init_time, PetscReal time_max
          * kernel computes in FP64 and splits the result again) */
         Vec keep = x->data;
         x->data = work;
-        qc_apply_events(qsys, t, x);
+        const int bad = qc_apply_events(qsys, t, x);
         x->data = keep;
+        if (bad) break;
       }
       if (qsys->ts_monitor) {
         if (fp32) QUAC_HIPCHK(qh_vec_copy(w32, x->data));
@@ -727,7 +748,7 @@ void time_step_sys(qsystem qsys, qvec x, PetscReal init_time, PetscReal time_max
         h = hnew;
       }
       if (!rc && events && have_ev && fabs(t - t_ev) <= 1e-9) {
-        qc_apply_events(qsys, t, x);
+        if (qc_apply_events(qsys, t, x)) rc = 1;
         have_k1 = 0;
       }
       if (!rc && qsys->ts_monitor) {

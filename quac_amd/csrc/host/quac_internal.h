@@ -44,8 +44,12 @@ void quac_rng_seed_state(unsigned long long *q, uint64_t seed);
 double quac_rng_uniform_state(unsigned long long *q);
 double quac_rng_uniform(qsystem s);
 void quac_mcwf_free(qsystem s);
+/* the ensemble from t0 to tmax (steps of dt, the last one cut at tmax, at most
+ * steps_max); the monitor sees step numbers from step0, and the state at t0
+ * only when call_initial */
 int quac_mcwf_time_step(qsystem s, qvec x, PetscReal t0, PetscReal tmax, PetscReal dt, PetscInt steps_max,
-                        int (*monitor)(qsystem, PetscInt, PetscReal, qvec), PetscInt *nsteps);
+                        int (*monitor)(qsystem, PetscInt, PetscReal, qvec), PetscInt *nsteps, PetscInt step0,
+                        int call_initial);
 
 /* Wootters' concurrence of a two-qubit density matrix (4 x 4, column-major) */
 int quac_concurrence_2q(const double _Complex *rho, double *conc);
@@ -76,7 +80,8 @@ operator qc_dag(operator op);
 
 /* circuit events during time stepping (circuits.c) */
 int qc_next_event_time(qsystem sys, PetscReal *t_event);
-void qc_apply_events(qsystem sys, PetscReal t, qvec x);
+/* applies the gates (or layer) due at the current event; nonzero on failure */
+int qc_apply_events(qsystem sys, PetscReal t, qvec x);
 /* device gate matrix and strides for a gate; superop=1: qubit q -> subsystem q
  * (stride n_after), else qubit q -> stride prod(dims[0..q-1]) */
 int qc_apply_gate(struct quantum_gate_struct *g, qvec x, int superop, qsystem sys);
