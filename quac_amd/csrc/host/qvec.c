@@ -265,7 +265,45 @@ void get_wf_element_qvec(qvec wf, PetscInt i, PetscScalar *val) {
 }
 
 /* qvec_utilities.c:1682-1717 (DM: Re diag(rho); WF: |psi_i|^2), arrays malloc'd */
+/* a spawned MCWF ensemble (qvec_utilities.c:2041-2086): the mean over the
+ * trajectories of |psi_e,i|^2 / <psi_e|psi_e>, and its sample variance */
+static void bitstring_probs_ens(qvec state, PetscInt *nloc, PetscReal **probs, PetscReal **vars) {
+  const PetscInt N = state->n, M = state->n_ensemble;
+  double *h = (double *)malloc(sizeof(double) * 2 * (size_t)(N * M));
+  double *nrm = (double *)malloc(sizeof(double) * (size_t)M);
+  if (!h || !nrm || qh_vec_get(state->ens_datas[0], h)) {
+    free(h);
+    free(nrm);
+    QUAC_FAIL(9, "get_bitstring_probs: %s", qh_last_error());
+  }
+  *nloc = N;
+  *probs = (PetscReal *)calloc((size_t)N, sizeof(PetscReal));
+  *vars = (PetscReal *)calloc((size_t)N, sizeof(PetscReal));
+  for (PetscInt e = 0; e < M; e++) {
+    const double *p = h + 2 * (size_t)(e * N);
+    double s = 0.0;
+    for (PetscInt i = 0; i < N; i++) s += p[2 * i] * p[2 * i] + p[2 * i + 1] * p[2 * i + 1];
+    nrm[e] = s;
+    for (PetscInt i = 0; i < N; i++) (*probs)[i] += (p[2 * i] * p[2 * i] + p[2 * i + 1] * p[2 * i + 1]) / s;
+  }
+  for (PetscInt i = 0; i < N; i++) (*probs)[i] /= (double)M;
+  for (PetscInt e = 0; e < M; e++) {
+    const double *p = h + 2 * (size_t)(e * N);
+    for (PetscInt i = 0; i < N; i++) {
+      const double d = (p[2 * i] * p[2 * i] + p[2 * i + 1] * p[2 * i + 1]) / nrm[e] - (*probs)[i];
+      (*vars)[i] += d * d;
+    }
+  }
+  for (PetscInt i = 0; i < N; i++) (*vars)[i] /= (double)(M > 1 ? M - 1 : 1);
+  free(h);
+  free(nrm);
+}
+
 void get_bitstring_probs(qvec state, PetscInt *nloc, PetscReal **probs, PetscReal **vars) {
+  if (state->my_type == WF_ENSEMBLE && state->ens_spawned) {
+    bitstring_probs_ens(state, nloc, probs, vars);
+    return;
+  }
   PetscInt N = state->total_levels;
   double *h = (double *)malloc(sizeof(double) * 2 * state->n);
   if (qh_vec_get(state->data, h)) {
