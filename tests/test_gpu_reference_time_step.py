@@ -16,7 +16,8 @@ test_gpu_stepping.py / test_gpu_mcwf.py):
 * :678-750  circuits applied to the states ten times between runs from t = 0:
             fidelity(dm, wf) against fidelity(wf, ensemble).
 (The reference does not run :566-676, gamma = 1, whose two systems differ in
-their dephasing terms.)"""
+their dephasing terms.)  Also the one test tests/time_dep_tests.c runs (:142-213):
+a time-dependent drive on a wavefunction checked at every monitored step."""
 import ctypes
 import math
 
@@ -214,3 +215,39 @@ def test_circuit_restarts_ensemble_against_density_matrix(opts):
         assert abs(_fidelity(dm, wf) - _fidelity(wf, wf_ens)) < 1 / math.sqrt(n)
     for s in (qd, qw, q3):
         s.destroy()
+
+
+def test_time_dependent_wavefunction_every_step(opts):
+    """time_dep_tests.c:142-213 (the test that file runs): H(t) = 0.5 omega
+    e^{-alpha t} (a + a^dag) on a wavefunction from |0>, omega = 2 pi,
+    alpha = 0.1, dt = 0.01 to t = 20; at every monitored step <sx> = 0,
+    <sy> = -sin(theta), <sz> = cos(theta) with theta = omega (1 - e^{-alpha t}) /
+    alpha, within the reference's 1e-4"""
+    om, al = 2 * math.pi, 0.1
+    qs = qa.System()
+    o = qs.create_op(2)
+    env = lambda t: math.exp(-al * t)
+    qs.add_ham_term_time_dep(0.5 * om, env, o)
+    qs.add_ham_term_time_dep(0.5 * om, env, o.dag)
+    qs.construct_matrix()
+    x = qs.create_wf()
+    x.add_loc(1.0, 0)
+    worst = [0.0, 0.0, 0.0]
+    count = [0]
+
+    def mon(step, t, v):
+        p = v.get()
+        c = p[0].conjugate() * p[1]
+        th = om * (1 - math.exp(-al * t)) / al
+        got = (2 * c.real, 2 * c.imag, abs(p[0]) ** 2 - abs(p[1]) ** 2)
+        want = (0.0, -math.sin(th), math.cos(th))
+        for k in range(3):
+            worst[k] = max(worst[k], abs(got[k] - want[k]))
+        count[0] += 1
+
+    qs.set_monitor(mon)
+    qs.time_step(x, 0.0, 20.0, 0.01, 10 ** 6)
+    qs.set_monitor(None)
+    assert count[0] > 20
+    assert max(worst) < 1e-4, worst
+    qs.destroy()
