@@ -198,3 +198,15 @@ def test_c_sweep_through_time_step_sys_batch(tmp_path):
     pc = oracle.expect_dm(tot, x, [(0, 4, 2, -1)]).real
     s = rows[("single", 0)]
     assert abs(s[0] - pe) < 1e-10 and abs(s[1] - pc) < 1e-10
+
+
+def test_c_qvec_suite_mirrors_the_reference(tmp_path):
+    """tests/c/qvec_suite.c: the state-container and observable cases of the
+    reference's qvec_utilities_tests.c (creation and dimensions, expectation
+    values, the random wavefunction fixture, bitstring probabilities of a
+    wavefunction, a density matrix and an MCWF ensemble) through the C API"""
+    exe = _build(tmp_path, "qvec_suite")
+    out = _run(exe, os.path.join(ROOT, "tests", "golden", "pristine_matrices", "rand_wf.dat"))
+    assert "FAIL" not in out, out
+    last = [l for l in out.splitlines() if l.startswith("checks")][-1].split()
+    assert int(last[1]) >= 90 and int(last[3]) == 0, out
