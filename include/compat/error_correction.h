@@ -1,0 +1,3 @@
+/* src/error_correction.h of the reference: outside this library (a program that includes it
+ * compiles; its functions are not provided). */
+#include "quac.h"
