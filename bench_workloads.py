@@ -35,7 +35,7 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 import quac_amd as qa  # noqa: E402
 import recipes  # noqa: E402
-from qbuild import build_system, subsystem_layout  # noqa: E402
+from qbuild import build_system, populations_np, subsystem_layout  # noqa: E402
 
 
 def _pops(vec, subs):
@@ -211,6 +211,25 @@ def jc(args):
             trace_pts.append(_pops(v, subs))
         qa.synchronize()
         out[prec] = (time.perf_counter() - t, np.array(trace_pts))
+    # the default integrator of time_step_sys (adaptive RK3BS, FP64) to t = 100:
+    # one workgroup with the controller on the device, and the host-driven loop
+    # that a monitor keeps (one error read-back per trial step)
+    adaptive = {}
+    for mode in ("one_workgroup", "host_loop"):
+        qa.set_option("ts_rk_type", "3bs")
+        qa.set_option("ts_adapt_type", "basic")
+        x = qs.create_qvec()
+        x.add_loc(1.0, loc)
+        if mode == "host_loop":
+            qs.set_monitor(lambda step, tt, v: None)
+        qa.synchronize()
+        t = time.perf_counter()
+        steps, tend = qs.time_step(x, 0.0, 100.0, 0.01, 10 ** 7)
+        qa.synchronize()
+        el = time.perf_counter() - t
+        qs.set_monitor(None)
+        adaptive[mode] = {"steps": steps, "seconds": round(el, 5), "steps_per_s": round(steps / el, 1),
+                          "populations": [float(p) for p in populations_np(x.get(), recipe)]}
     qs.destroy()
     el64, p64 = out[qa.FP64]
     el32, p32 = out[qa.FP32]
@@ -222,7 +241,8 @@ def jc(args):
             "config": {"workload": "Jaynes-Cummings 2x10 (400-row superoperator), RK4 dt=0.01 to t=100"},
             "fp32_steps_per_s": round(10000 / el32, 2),
             "max_pop_err_fp32_vs_fp64": float(np.abs(p32 - p64).max()),
-            "final_populations_fp64": [float(x) for x in p64[-1]]}
+            "final_populations_fp64": [float(x) for x in p64[-1]],
+            "default_integrator_rk3bs_to_t100": adaptive}
 
 
 def copy(args):

@@ -158,6 +158,17 @@ int qh_rk3bs_err(const qh_vec *x, const qh_vec *xn, const qh_vec *k1, const qh_v
  * three fused launches. */
 int qh_rk3bs_step(qh_mat *A, const qh_vec *x, qh_vec *xn, qh_vec *k1, qh_vec *k2, qh_vec *k3, qh_vec *k4,
                   qh_vec *y, double h, double atol, double rtol, int have_k1, double *err);
+/* The whole Bogacki-Shampine integration of dx/dt = A x from t0 to tmax for a
+ * generator of at most 640 rows (FP64 x in place) in one workgroup, the step
+ * size controller included (the time_step_sys loop without monitor, events or
+ * time-dependent terms, qsystem.c:995-1005,1078): first trial step h0; adaptive
+ * != 0: fac = 0.9 err^(-1/3) in [0.1, 10], next step in [hmin, hmax], accepted
+ * when err <= 1 or h <= hmin; adaptive == 0: fixed steps of h0, the last cut at
+ * tmax.  At most steps_max accepted steps; returns the final time, the accepted
+ * steps and the next step size. */
+int qh_rk3bs_small(qh_mat *A, qh_vec *x, double t0, double tmax, double h0, int64_t steps_max, double atol,
+                   double rtol, double hmin, double hmax, int adaptive, double *t_out, int64_t *steps_out,
+                   double *h_out);
 
 /* ------------------------------------------------------------- observables */
 /* Tr(ops rho) (qvec_utilities.c:1197-1281), FP64 accumulation on the device */

@@ -711,6 +711,21 @@ void time_step_sys(qsystem qsys, qvec x, PetscReal init_time, PetscReal time_max
     const double hmin = quac_opt_double("ts_adapt_dt_min", 0.01), hmax = quac_opt_double("ts_adapt_dt_max", 1.0);
     PetscReal h = dt;
     int have_k1 = 0, rc = 0;
+    int64_t vn = 0;
+    int vprec = 0;
+    qh_vec_size(x->data, &vn, &vprec);
+    if (!td && !events && !qsys->ts_monitor && vprec == QH_FP64 && qsys->dim <= 640) {
+      /* a small generator: the whole integration in one workgroup, controller
+       * included (same formulas as the loop below) */
+      double tf = t, hl = h;
+      int64_t ns = 0;
+      rc = qh_rk3bs_small(qsys->mat_A, x->data, t, time_max, dt, steps_max, atol, rtol, hmin, hmax, !fixed, &tf, &ns,
+                          &hl);
+      if (!rc) {
+        t = tf;
+        step = ns;
+      }
+    }
     while (t < time_max - eps && step < steps_max && !rc) {
       PetscReal t_ev = 0.0;
       int have_ev = events && qc_next_event_time(qsys, &t_ev) && t_ev > t + eps;

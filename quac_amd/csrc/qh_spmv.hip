@@ -1371,6 +1371,192 @@ __global__ void __launch_bounds__(1024) k_rk4_small(int32_t n, const int32_t *__
   }
 }
 
+// The adaptive Bogacki-Shampine integration of time_step_sys (TSRK3BS with
+// TSAdapt basic, qsystem.c:995-1005) in one workgroup for a generator of at most
+// QH_SMALL_BS3_ROWS rows: x, the candidate and K1..K4 in LDS (6 x 16 B per row),
+// the step-size controller on the device (thread 0, broadcast through LDS), so a
+// trial step costs five barriers and no host round trip.  The same stage
+// formulas, error weights and controller as the host-driven loop: K2 = A(x +
+// h/2 K1), K3 = A(x + 3h/4 K2), xn = x + h(2/9 K1 + 1/3 K2 + 4/9 K3), K4 = A xn,
+// err = RMS of h sum (b_i - b^_i) K_i over atol + rtol max(|x|, |xn|);
+// fac = 0.9 err^(-1/3) in [0.1, 10], h' = h fac in [hmin, hmax]; accepted when
+// err <= 1 or h <= hmin (1 + 1e-12), with K1 of the next step = K4 (FSAL).
+// adaptive == 0: fixed steps of h0 (the last cut at tmax), K1 recomputed.
+// out: final time, accepted steps, the next step size.
+#define QH_SMALL_BS3_ROWS 640
+
+template <int MAXNZ>
+__global__ void __launch_bounds__(QH_SMALL_BS3_ROWS) k_rk3bs_small(int32_t n, const int32_t *__restrict__ rp,
+                                                      const int32_t *__restrict__ col,
+                                                      const double2 *__restrict__ val, double2 *xg, double t0,
+                                                      double tmax, double h0, int64_t steps_max, double atol,
+                                                      double rtol, double hmin, double hmax, int adaptive, double eps,
+                                                      double *out) {
+  extern __shared__ double2 sm[];
+  double2 *xa = sm, *xn = sm + n, *k1 = sm + 2 * n, *k2 = sm + 3 * n, *k3 = sm + 4 * n, *k4 = sm + 5 * n;
+  __shared__ double red[QH_SMALL_BS3_ROWS / 64];
+  __shared__ double s_hnew;
+  __shared__ int s_acc;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  for (int r = tid; r < n; r += nt) xa[r] = xg[r];
+  int32_t rc_[MAXNZ > 0 ? MAXNZ : 1];
+  double2 rv_[MAXNZ > 0 ? MAXNZ : 1];
+  int rlen = 0;
+  if (MAXNZ > 0 && tid < n) {
+    const int32_t b = rp[tid];
+    rlen = rp[tid + 1] - b;
+#pragma unroll
+    for (int k = 0; k < (MAXNZ > 0 ? MAXNZ : 1); k++) {
+      rc_[k] = k < rlen ? col[b + k] : 0;
+      rv_[k] = k < rlen ? val[b + k] : make_double2(0.0, 0.0);
+    }
+  }
+  __syncthreads();
+  auto rowdot = [&](int r, const double2 *xs, const double2 *kin, double c) {
+    double are = 0.0, aim = 0.0;
+    if (MAXNZ > 0) {
+#pragma unroll
+      for (int k = 0; k < (MAXNZ > 0 ? MAXNZ : 1); k++) {
+        if (k < rlen) {
+          double2 v = xs[rc_[k]];
+          if (kin) {
+            v.x += c * kin[rc_[k]].x;
+            v.y += c * kin[rc_[k]].y;
+          }
+          const double2 a = rv_[k];
+          are = fma(a.x, v.x, fma(-a.y, v.y, are));
+          aim = fma(a.x, v.y, fma(a.y, v.x, aim));
+        }
+      }
+      return make_double2(are, aim);
+    }
+    for (int32_t k = rp[r]; k < rp[r + 1]; k++) {
+      const int32_t cc = col[k];
+      double2 v = xs[cc];
+      if (kin) {
+        v.x += c * kin[cc].x;
+        v.y += c * kin[cc].y;
+      }
+      const double2 a = val[k];
+      are = fma(a.x, v.x, fma(-a.y, v.y, are));
+      aim = fma(a.x, v.y, fma(a.y, v.x, aim));
+    }
+    return make_double2(are, aim);
+  };
+  const double c1 = 2.0 / 9.0 - 7.0 / 24.0, c2 = 1.0 / 3.0 - 1.0 / 4.0, c3 = 4.0 / 9.0 - 1.0 / 3.0, c4 = -1.0 / 8.0;
+  double t = t0, h = h0;
+  int64_t step = 0;
+  bool have_k1 = false;
+  while (t < tmax - eps && step < steps_max) {
+    const double hh = (t + h > tmax) ? tmax - t : h;
+    if (!have_k1) {
+      for (int r = tid; r < n; r += nt) k1[r] = rowdot(r, xa, nullptr, 0.0);
+      __syncthreads();
+    }
+    for (int r = tid; r < n; r += nt) k2[r] = rowdot(r, xa, k1, 0.5 * hh);
+    __syncthreads();
+    for (int r = tid; r < n; r += nt) k3[r] = rowdot(r, xa, k2, 0.75 * hh);
+    __syncthreads();
+    for (int r = tid; r < n; r += nt) {
+      const double2 xo = xa[r], a = k1[r], bq = k2[r], cq = k3[r];
+      xn[r] = make_double2(xo.x + hh * (2.0 / 9.0) * a.x + hh / 3.0 * bq.x + hh * (4.0 / 9.0) * cq.x,
+                           xo.y + hh * (2.0 / 9.0) * a.y + hh / 3.0 * bq.y + hh * (4.0 / 9.0) * cq.y);
+    }
+    __syncthreads();
+    double v = 0.0;
+    for (int r = tid; r < n; r += nt) {
+      const double2 d = rowdot(r, xn, nullptr, 0.0), a = k1[r], bq = k2[r], cq = k3[r], xo = xa[r], xw = xn[r];
+      k4[r] = d;
+      const double er = hh * (c1 * a.x + c2 * bq.x + c3 * cq.x + c4 * d.x);
+      const double ei = hh * (c1 * a.y + c2 * bq.y + c3 * cq.y + c4 * d.y);
+      const double ax = sqrt(xo.x * xo.x + xo.y * xo.y), an = sqrt(xw.x * xw.x + xw.y * xw.y);
+      const double sc = atol + rtol * (ax > an ? ax : an);
+      v += (er * er + ei * ei) / (sc * sc);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((tid & 63) == 0) red[tid >> 6] = v;
+    __syncthreads();
+    if (tid == 0) {
+      double sum = 0.0;
+      for (int w = 0; w < (nt + 63) / 64; w++) sum += red[w];
+      const double err = sqrt(sum / (double)n);
+      double fac = 0.9 * pow(err > 1e-300 ? err : 1e-300, -1.0 / 3.0);
+      fac = fac < 0.1 ? 0.1 : (fac > 10.0 ? 10.0 : fac);
+      double hn = hh * fac;
+      hn = hn < hmin ? hmin : (hn > hmax ? hmax : hn);
+      s_hnew = hn;
+      s_acc = (!adaptive || err <= 1.0 || hh <= hmin * (1 + 1e-12)) ? 1 : 0;
+    }
+    __syncthreads();
+    const int acc = s_acc;
+    if (acc) {
+      double2 *tp = xa;
+      xa = xn;
+      xn = tp;
+      t += hh;
+      step++;
+    }
+    if (adaptive) {
+      if (acc) {
+        double2 *tk = k1;
+        k1 = k4;
+        k4 = tk;
+      }
+      have_k1 = true;
+      h = s_hnew;
+    }
+    // the next trial writes K2 (and K1 at fixed steps) only after every thread
+    // has read s_acc / s_hnew: the barrier after its first stage orders that
+    __syncthreads();
+  }
+  for (int r = tid; r < n; r += nt) xg[r] = xa[r];
+  if (tid == 0) {
+    out[0] = t;
+    out[1] = (double)step;
+    out[2] = h;
+  }
+}
+
+extern "C" int qh_rk3bs_small(qh_mat *A, qh_vec *x, double t0, double tmax, double h0, int64_t steps_max, double atol,
+                              double rtol, double hmin, double hmax, int adaptive, double *t_out, int64_t *steps_out,
+                              double *h_out) {
+  QH_REQUIRE(A && x && t_out && steps_out && h_out && h0 > 0.0, QH_EINVAL, "qh_rk3bs_small: bad arguments");
+  QH_REQUIRE(x->prec == QH_FP64 && x->n == A->rows, QH_EINVAL, "qh_rk3bs_small: an FP64 vector of the matrix size");
+  QH_REQUIRE(A->rows >= 1 && A->rows <= QH_SMALL_BS3_ROWS, QH_EINVAL,
+             "qh_rk3bs_small: %lld rows (the one-workgroup path takes at most %d)", (long long)A->rows,
+             QH_SMALL_BS3_ROWS);
+  int rc = qh_mat_max_row(A);
+  if (rc) return rc;
+  double *dout = nullptr, *hout = nullptr;
+  rc = qh_scratch(3 * sizeof(double), (void **)&dout, 3 * sizeof(double), (void **)&hout);
+  if (rc) return rc;
+  const int32_t n = (int32_t)A->rows;
+  int bt = ((n + 63) / 64) * 64;
+  const size_t lds = (size_t)6 * n * sizeof(double2);
+  const double eps = 1e-12 * (fabs(tmax) + fabs(h0) + 1.0);
+  hipStream_t s = qh_stream();
+  // rows of at most 8 entries in registers (one row per thread); longer rows read
+  // from the CSR, which the caches hold for a generator this small (register
+  // copies of 16 or 32 entries would spill)
+  const int mz = A->max_row <= 8 ? 8 : 0;
+#define QH_LB(Z)                                                                                                \
+  hipLaunchKernelGGL((k_rk3bs_small<Z>), dim3(1), dim3(bt), lds, s, n, A->rowptr, A->col, A->val,             \
+                     (double2 *)x->d, t0, tmax, h0, steps_max, atol, rtol, hmin, hmax, adaptive, eps, dout)
+  if (mz == 8)
+    QH_LB(8);
+  else
+    QH_LB(0);
+#undef QH_LB
+  QH_HIP(hipGetLastError());
+  QH_HIP(hipMemcpyAsync(hout, dout, 3 * sizeof(double), hipMemcpyDeviceToHost, s));
+  QH_HIP(hipStreamSynchronize(s));
+  *t_out = hout[0];
+  *steps_out = (int64_t)hout[1];
+  *h_out = hout[2];
+  return QH_OK;
+}
+
 // longest row of a CSR (one block, grid-stride)
 __global__ void k_max_row(const int32_t *__restrict__ rp, int64_t n, int32_t *out) {
   __shared__ int32_t red[256];

@@ -676,3 +676,37 @@ def test_small_system_single_workgroup_matches_regular_kernels():
     qa.rk4_device(A, f, 0.01, 300, tot=N, renorm_every=50)
     assert np.abs(populations_np(f.get(), r) - populations_np(ref, r)).max() < 1e-6
     qs.destroy()
+
+
+@pytest.mark.parametrize("adapt", ["basic", "none"])
+def test_small_system_rk3bs_in_one_workgroup_matches_host_loop(adapt):
+    """the default integrator on a generator of at most 640 rows runs in one
+    workgroup with the step-size controller on the device; a monitor keeps the
+    host-driven loop (one trial step per launch sequence and an error
+    read-back), which must take the same steps to the same state"""
+    qa.set_option("ts_rk_type", "3bs")
+    qa.set_option("ts_adapt_type", adapt)
+    r = recipes.jc_recipe()
+    N, loc = recipes.jc_init()
+    qs, _, _ = build_system(r)
+    xs = qs.create_qvec()
+    xs.add_loc(1.0, loc)
+    s1, t1 = qs.time_step(xs, 0.0, 20.0, 0.05, 100000)
+    xh = qs.create_qvec()
+    xh.add_loc(1.0, loc)
+    qs.set_monitor(lambda step, t, v: None)
+    s2, t2 = qs.time_step(xh, 0.0, 20.0, 0.05, 100000)
+    qs.set_monitor(None)
+    assert abs(t1 - 20.0) < 1e-9 and abs(t2 - 20.0) < 1e-9
+    assert s1 == s2, (s1, s2)
+    assert np.abs(xs.get() - xh.get()).max() < 1e-10
+    # against the oracle's fixed-step RK4 at a fine step (the adaptive error
+    # control holds the local error near atol = rtol = 5e-7)
+    tot, terms = recipes.oracle_terms(r)
+    m = oracle.construct(tot, terms, dm=True)
+    x0 = np.zeros(N * N, np.complex128)
+    x0[loc] = 1.0
+    ref = m.rk4(x0, 0.005, 4000)
+    assert np.abs(populations_np(xs.get(), r) - populations_np(ref, r)).max() < 1e-4
+    _adaptive()
+    qs.destroy()
