@@ -213,21 +213,20 @@ def jc(args):
         out[prec] = (time.perf_counter() - t, np.array(trace_pts))
     # the default integrator of time_step_sys (adaptive RK3BS, FP64) to t = 100:
     # one workgroup with the controller on the device, and the host-driven loop
-    # that a monitor keeps (one error read-back per trial step)
+    # (-quac_small_path 0: one error read-back per trial step)
     adaptive = {}
     for mode in ("one_workgroup", "host_loop"):
         qa.set_option("ts_rk_type", "3bs")
         qa.set_option("ts_adapt_type", "basic")
         x = qs.create_qvec()
         x.add_loc(1.0, loc)
-        if mode == "host_loop":
-            qs.set_monitor(lambda step, tt, v: None)
+        qa.set_option("quac_small_path", "0" if mode == "host_loop" else "1")
         qa.synchronize()
         t = time.perf_counter()
         steps, tend = qs.time_step(x, 0.0, 100.0, 0.01, 10 ** 7)
         qa.synchronize()
         el = time.perf_counter() - t
-        qs.set_monitor(None)
+        qa.set_option("quac_small_path", "1")
         adaptive[mode] = {"steps": steps, "seconds": round(el, 5), "steps_per_s": round(steps / el, 1),
                           "populations": [float(p) for p in populations_np(x.get(), recipe)]}
     qs.destroy()

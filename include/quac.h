@@ -289,7 +289,9 @@ const char *quac_last_error(void);   /* "" when the last call succeeded */
 void quac_clear_error(void);
 /* options normally given as PETSc command-line flags: -ts_rk_type {3bs,4},
  * -ts_adapt_type {basic,none}, -quac_precision {fp64,fp32}, -quac_batch_precision {fp32,fp64}, -quac_renorm N,
- * -quac_kron_conj_a {0,1}, -ksp_rtol X, -ksp_gmres_restart N, -quac_jacobi {0,1} */
+ * -quac_kron_conj_a {0,1}, -ksp_rtol X, -ksp_gmres_restart N, -quac_jacobi {0,1},
+ * -quac_small_path {1,0} (0: generators of at most 640 rows take the host-driven
+ * RK3BS loop instead of the one-workgroup integration) */
 void quac_set_option(const char *name, const char *value);
 const char *quac_get_option(const char *name);
 /* last time_step_sys or time_step_sys_batch: steps taken and final time */

@@ -714,16 +714,35 @@ void time_step_sys(qsystem qsys, qvec x, PetscReal init_time, PetscReal time_max
     int64_t vn = 0;
     int vprec = 0;
     qh_vec_size(x->data, &vn, &vprec);
-    if (!td && !events && !qsys->ts_monitor && vprec == QH_FP64 && qsys->dim <= 640) {
-      /* a small generator: the whole integration in one workgroup, controller
-       * included (same formulas as the loop below) */
-      double tf = t, hl = h;
-      int64_t ns = 0;
-      rc = qh_rk3bs_small(qsys->mat_A, x->data, t, time_max, dt, steps_max, atol, rtol, hmin, hmax, !fixed, &tf, &ns,
-                          &hl);
-      if (!rc) {
+    if (!td && vprec == QH_FP64 && qsys->dim <= 640 && quac_opt_int("quac_small_path", 1)) {
+      /* a small generator: the integration in one workgroup, controller
+       * included (same formulas as the loop below); up to the next event, and
+       * with a monitor one accepted step per launch (the monitor is called after
+       * accepted steps, as PETSc's TSMonitor is) */
+      PetscReal hcur = dt;
+      while (t < time_max - eps && step < steps_max && !rc) {
+        PetscReal t_ev = 0.0, tend = time_max;
+        const int have_ev = events && qc_next_event_time(qsys, &t_ev) && t_ev > t + eps;
+        if (have_ev && t_ev < tend) tend = t_ev;
+        double tf = t, hl = hcur;
+        int64_t ns = 0;
+        rc = qh_rk3bs_small(qsys->mat_A, x->data, t, tend, hcur, qsys->ts_monitor ? 1 : steps_max - step, atol, rtol,
+                            hmin, hmax, !fixed, &tf, &ns, &hl);
+        if (rc) break;
         t = tf;
-        step = ns;
+        step += ns;
+        hcur = fixed ? dt : hl;
+        const int at_ev = have_ev && fabs(t - t_ev) <= 1e-9;
+        if (at_ev) {
+          t = t_ev;
+          if (qc_apply_events(qsys, t, x)) rc = 1;
+        }
+        if (!rc && qsys->ts_monitor) {
+          ts.t = t;
+          ts.step = step;
+          if (call_monitor(qsys, &ts, x)) rc = 9;
+        }
+        if (ns == 0 && !at_ev) break;
       }
     }
     while (t < time_max - eps && step < steps_max && !rc) {

@@ -1504,7 +1504,9 @@ __global__ void __launch_bounds__(QH_SMALL_BS3_ROWS) k_rk3bs_small(int32_t n, co
         k4 = tk;
       }
       have_k1 = true;
-      h = s_hnew;
+      // a step cut to land on tmax (the end, or an event the caller stops at)
+      // leaves the step size as it was, as the host loop does after an event
+      if (!(acc && hh < h && t >= tmax - eps)) h = s_hnew;
     }
     // the next trial writes K2 (and K1 at fixed steps) only after every thread
     // has read s_acc / s_hnew: the barrier after its first stage orders that

@@ -681,8 +681,8 @@ def test_small_system_single_workgroup_matches_regular_kernels():
 @pytest.mark.parametrize("adapt", ["basic", "none"])
 def test_small_system_rk3bs_in_one_workgroup_matches_host_loop(adapt):
     """the default integrator on a generator of at most 640 rows runs in one
-    workgroup with the step-size controller on the device; a monitor keeps the
-    host-driven loop (one trial step per launch sequence and an error
+    workgroup with the step-size controller on the device; -quac_small_path 0
+    keeps the host-driven loop (one trial step per launch sequence and an error
     read-back), which must take the same steps to the same state"""
     qa.set_option("ts_rk_type", "3bs")
     qa.set_option("ts_adapt_type", adapt)
@@ -694,9 +694,11 @@ def test_small_system_rk3bs_in_one_workgroup_matches_host_loop(adapt):
     s1, t1 = qs.time_step(xs, 0.0, 20.0, 0.05, 100000)
     xh = qs.create_qvec()
     xh.add_loc(1.0, loc)
-    qs.set_monitor(lambda step, t, v: None)
-    s2, t2 = qs.time_step(xh, 0.0, 20.0, 0.05, 100000)
-    qs.set_monitor(None)
+    qa.set_option("quac_small_path", "0")
+    try:
+        s2, t2 = qs.time_step(xh, 0.0, 20.0, 0.05, 100000)
+    finally:
+        qa.set_option("quac_small_path", "1")
     assert abs(t1 - 20.0) < 1e-9 and abs(t2 - 20.0) < 1e-9
     assert s1 == s2, (s1, s2)
     assert np.abs(xs.get() - xh.get()).max() < 1e-10
@@ -710,3 +712,45 @@ def test_small_system_rk3bs_in_one_workgroup_matches_host_loop(adapt):
     assert np.abs(populations_np(xs.get(), r) - populations_np(ref, r)).max() < 1e-4
     _adaptive()
     qs.destroy()
+
+
+def test_small_system_rk3bs_with_monitor_and_events_matches_host_loop():
+    """the one-workgroup integration stops at each gate time and, with a
+    monitor, after every accepted step: the cascade-and-circuit case of
+    time_step_tests.c:337-396 gives the host loop's state, and the monitor sees
+    increasing accepted step numbers ending at t = 1.1"""
+    _adaptive()
+
+    def run(small):
+        qa.set_option("quac_small_path", "1" if small else "0")
+        qs = qa.System()
+        o2 = qs.create_op(2)
+        o3 = qs.create_op(2)
+        qs.add_ham_term(1.0, o2.n)
+        qs.add_lin_term(1.0, o2)
+        qs.add_lin_term(1.0, o3)
+        x = qs.create_qvec()
+        x.add(1.0, 0, 0)
+        qs.construct_matrix()
+        c = qa.Circuit(5)
+        c.add_gate(0.0, "SIGMAX", [0])
+        c.add_gate(0.1, "SIGMAX", [1])
+        qs.apply_circuit(c, 0.1)
+        seen = []
+        qs.set_monitor(lambda step, t, v: seen.append((step, t)))
+        qs.time_step(x, 0.0, 1.1, 0.05, 200)
+        qs.set_monitor(None)
+        out = (x.get(), x.expectation(o2.n).real, x.expectation(o3.n).real, seen)
+        qs.destroy()
+        return out
+
+    try:
+        a, a2, a3, seen = run(True)
+        b, b2, b3, _ = run(False)
+    finally:
+        qa.set_option("quac_small_path", "1")
+    assert np.abs(a - b).max() < 1e-9
+    assert abs(a2 - math.exp(-0.9)) < 5e-4 and abs(a3 - math.exp(-1.0)) < 5e-4
+    steps = [s for s, _ in seen]
+    assert steps[0] == 0 and all(q > p for p, q in zip(steps, steps[1:]))
+    assert abs(seen[-1][1] - 1.1) < 1e-9
