@@ -129,6 +129,41 @@ def tc512(args):
             "max_pop_err_fp32_batch_vs_fp64": worst, "assembly_s": round(asm_s, 3)}
 
 
+def sweep_eager(args):
+    """the 512-instance sweep's FP32 stage kernels launched one by one (no step
+    graph), for counter collection: the batch is built and the variant chosen
+    as in the headline run (or --fp32-variant), W steps, then K timed steps with
+    an event pair around every stage launch"""
+    insts = recipes.tc_instances(args.instances, 2)
+    mats, tots, x0s, systems = [], [], [], []
+    for p in insts:
+        qs, _, _ = build_system(recipes.tc_recipe(*p))
+        N, loc = recipes.tc_init(p[0], p[1])
+        x0 = np.zeros(N * N, np.complex128)
+        x0[loc] = 1.0
+        systems.append(qs)
+        mats.append(qs.matrix())
+        tots.append(N)
+        x0s.append(x0)
+    b = qa.Batch(mats, tots, prec=qa.FP32)
+    for i, x0 in enumerate(x0s):
+        b.set_state(i, x0)
+    bA, bx = b.matrix(), b.state()
+    if args.fp32_variant:
+        bA.set_lanes(0, args.fp32_variant)
+    qa.rk4_prepare(bA, bx, 0.01)
+    qa.rk4_device(bA, bx, 0.01, max(args.warmup, 1), timed=True)
+    ms = qa.rk4_device(bA, bx, 0.01, args.steps, timed=True)
+    lanes = bA.lanes()
+    b.destroy()
+    for qs in systems:
+        qs.destroy()
+    return {"metric": "sweep stage kernel, eager launches", "value": round(ms, 5), "unit": "ms per stage launch",
+            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": False,
+            "config": {"workload": f"{args.instances} Tavis-Cummings instances (seed 2), one FP32 batch"},
+            "lanes": lanes}
+
+
 def circuit12(args):
     """the circuit through QuaC's circuit API: add_gate_to_circuit_sys (run time
     0.1 per gate), schedule_circuit_layers, apply_circuit_to_sys, and one
