@@ -726,8 +726,10 @@ void time_step_sys(qsystem qsys, qvec x, PetscReal init_time, PetscReal time_max
         if (have_ev && t_ev < tend) tend = t_ev;
         double tf = t, hl = hcur;
         int64_t ns = 0;
-        rc = qh_rk3bs_small(qsys->mat_A, x->data, t, tend, hcur, qsys->ts_monitor ? 1 : steps_max - step, atol, rtol,
-                            hmin, hmax, !fixed, &tf, &ns, &hl);
+        /* at most 65 536 accepted steps per launch, so that no launch runs for long */
+        const int64_t room = steps_max - step < 65536 ? steps_max - step : 65536;
+        rc = qh_rk3bs_small(qsys->mat_A, x->data, t, tend, hcur, qsys->ts_monitor ? 1 : room, atol, rtol, hmin, hmax,
+                            !fixed, &tf, &ns, &hl);
         if (rc) break;
         t = tf;
         step += ns;
@@ -761,6 +763,11 @@ void time_step_sys(qsystem qsys, qvec x, PetscReal init_time, PetscReal time_max
         double err = 0.0;
         rc = rk3bs_try(qsys, x->data, t, hh, &w, have_k1, atol, rtol, &err);
         if (rc) break;
+        if (!(err <= 1.7976931348623157e308)) {
+          /* NaN or infinite: no step size can be accepted */
+          work_free(&w);
+          QUAC_FAIL(9, "time_step_sys: non-finite error estimate at t = %g (the state is not finite)", t);
+        }
         double fac = 0.9 * pow(err > 1e-300 ? err : 1e-300, -1.0 / 3.0);
         if (fac < 0.1) fac = 0.1;
         if (fac > 10.0) fac = 10.0;

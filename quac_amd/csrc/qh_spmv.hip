@@ -1396,7 +1396,7 @@ __global__ void __launch_bounds__(QH_SMALL_BS3_ROWS) k_rk3bs_small(int32_t n, co
   double2 *xa = sm, *xn = sm + n, *k1 = sm + 2 * n, *k2 = sm + 3 * n, *k3 = sm + 4 * n, *k4 = sm + 5 * n;
   __shared__ double red[QH_SMALL_BS3_ROWS / 64];
   __shared__ double s_hnew;
-  __shared__ int s_acc;
+  __shared__ int s_acc, s_bad;
   const int tid = threadIdx.x, nt = blockDim.x;
   for (int r = tid; r < n; r += nt) xa[r] = xg[r];
   int32_t rc_[MAXNZ > 0 ? MAXNZ : 1];
@@ -1447,7 +1447,14 @@ __global__ void __launch_bounds__(QH_SMALL_BS3_ROWS) k_rk3bs_small(int32_t n, co
   double t = t0, h = h0;
   int64_t step = 0;
   bool have_k1 = false;
-  while (t < tmax - eps && step < steps_max) {
+  if (tid == 0) s_bad = 0;
+  __syncthreads();
+  // a bound on the trial steps: a rejection shrinks the step by at least 0.9
+  // and a step at hmin is always accepted, so fewer than 64 trials per accepted
+  // step are possible while the state is finite; the loop ends in every case
+  const int64_t trial_cap = 64 * (steps_max < ((int64_t)1 << 40) ? steps_max : ((int64_t)1 << 40)) + 64;
+  int64_t trials = 0;
+  while (t < tmax - eps && step < steps_max && trials++ < trial_cap) {
     const double hh = (t + h > tmax) ? tmax - t : h;
     if (!have_k1) {
       for (int r = tid; r < n; r += nt) k1[r] = rowdot(r, xa, nullptr, 0.0);
@@ -1481,6 +1488,9 @@ __global__ void __launch_bounds__(QH_SMALL_BS3_ROWS) k_rk3bs_small(int32_t n, co
       double sum = 0.0;
       for (int w = 0; w < (nt + 63) / 64; w++) sum += red[w];
       const double err = sqrt(sum / (double)n);
+      // a non-finite error estimate (a NaN or infinite state) ends the
+      // integration: the controller could not leave the step otherwise
+      s_bad = !(err <= 1.7976931348623157e308);
       double fac = 0.9 * pow(err > 1e-300 ? err : 1e-300, -1.0 / 3.0);
       fac = fac < 0.1 ? 0.1 : (fac > 10.0 ? 10.0 : fac);
       double hn = hh * fac;
@@ -1489,6 +1499,7 @@ __global__ void __launch_bounds__(QH_SMALL_BS3_ROWS) k_rk3bs_small(int32_t n, co
       s_acc = (!adaptive || err <= 1.0 || hh <= hmin * (1 + 1e-12)) ? 1 : 0;
     }
     __syncthreads();
+    if (s_bad) break;
     const int acc = s_acc;
     if (acc) {
       double2 *tp = xa;
@@ -1517,6 +1528,7 @@ __global__ void __launch_bounds__(QH_SMALL_BS3_ROWS) k_rk3bs_small(int32_t n, co
     out[0] = t;
     out[1] = (double)step;
     out[2] = h;
+    out[3] = s_bad ? 1.0 : 0.0;
   }
 }
 
@@ -1531,7 +1543,7 @@ extern "C" int qh_rk3bs_small(qh_mat *A, qh_vec *x, double t0, double tmax, doub
   int rc = qh_mat_max_row(A);
   if (rc) return rc;
   double *dout = nullptr, *hout = nullptr;
-  rc = qh_scratch(3 * sizeof(double), (void **)&dout, 3 * sizeof(double), (void **)&hout);
+  rc = qh_scratch(4 * sizeof(double), (void **)&dout, 4 * sizeof(double), (void **)&hout);
   if (rc) return rc;
   const int32_t n = (int32_t)A->rows;
   int bt = ((n + 63) / 64) * 64;
@@ -1551,11 +1563,13 @@ extern "C" int qh_rk3bs_small(qh_mat *A, qh_vec *x, double t0, double tmax, doub
     QH_LB(0);
 #undef QH_LB
   QH_HIP(hipGetLastError());
-  QH_HIP(hipMemcpyAsync(hout, dout, 3 * sizeof(double), hipMemcpyDeviceToHost, s));
+  QH_HIP(hipMemcpyAsync(hout, dout, 4 * sizeof(double), hipMemcpyDeviceToHost, s));
   QH_HIP(hipStreamSynchronize(s));
   *t_out = hout[0];
   *steps_out = (int64_t)hout[1];
   *h_out = hout[2];
+  QH_REQUIRE(hout[3] == 0.0, QH_EINVAL, "qh_rk3bs_small: non-finite error estimate at t = %g (the state is not finite)",
+             hout[0]);
   return QH_OK;
 }
 

@@ -754,3 +754,22 @@ def test_small_system_rk3bs_with_monitor_and_events_matches_host_loop():
     steps = [s for s, _ in seen]
     assert steps[0] == 0 and all(q > p for p, q in zip(steps, steps[1:]))
     assert abs(seen[-1][1] - 1.1) < 1e-9
+
+
+@pytest.mark.parametrize("small", ["1", "0"])
+def test_adaptive_integrator_reports_a_non_finite_state(small):
+    """a NaN in the state makes the error estimate NaN, which no step size can
+    bring below 1: both the one-workgroup integration and the host loop stop at
+    the first trial step with an error instead of retrying"""
+    _adaptive()
+    qa.set_option("quac_small_path", small)
+    try:
+        qs, _, _ = build_system(recipes.jc_recipe(levels=3))
+        x = qs.create_qvec()
+        x.add_loc(1.0, 0)
+        x.add_loc(float("nan"), 7)
+        with pytest.raises(qa.QuacError, match="non-finite"):
+            qs.time_step(x, 0.0, 1.0, 0.05, 1000)
+        qs.destroy()
+    finally:
+        qa.set_option("quac_small_path", "1")
