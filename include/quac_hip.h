@@ -28,6 +28,8 @@ int qh_finalize(void);
 const char *qh_last_error(void);
 int qh_device_count(void);
 int qh_synchronize(void);
+/* free and total device memory of the current device (after the library stream drains) */
+int qh_mem_info(size_t *free_bytes, size_t *total_bytes);
 /* allocate / free raw device memory (plumbing for the host layer and tests) */
 int qh_dmalloc(void **p, size_t bytes);
 int qh_dfree(void *p);

@@ -218,6 +218,7 @@ def _setup(L):
     sig("qh_set_graph_steps", i32, i32)
     sig("qh_bench_spmv", i32, vp, i32, i32, P(dbl))
     sig("qh_bench_stream", i32, i64, i32, P(dbl), P(dbl))
+    sig("qh_mem_info", i32, P(ctypes.c_size_t), P(ctypes.c_size_t))
     sig("qh_pc_create_hbasis", i32, vp, P(vp))
     sig("qh_pc_apply", i32, vp, vp, vp)
     sig("qh_pc_destroy", i32, vp)
@@ -827,6 +828,13 @@ def bench_spmv(mat, prec, reps=20):
     ms = ctypes.c_double()
     _qh(lib().qh_bench_spmv(mat.h, prec, reps, ctypes.byref(ms)))
     return ms.value
+
+
+def mem_info():
+    """(free, total) device bytes"""
+    f, t = ctypes.c_size_t(), ctypes.c_size_t()
+    _qh(lib().qh_mem_info(ctypes.byref(f), ctypes.byref(t)))
+    return f.value, t.value
 
 
 def bench_stream(nbytes=1 << 30, reps=10):

@@ -121,6 +121,15 @@ extern "C" int qh_finalize(void) {
   return QH_OK;
 }
 
+extern "C" int qh_mem_info(size_t *free_bytes, size_t *total_bytes) {
+  int rc = qh_ensure_init();
+  if (rc) return rc;
+  QH_REQUIRE(free_bytes && total_bytes, QH_EINVAL, "qh_mem_info: null argument");
+  QH_HIP(hipStreamSynchronize(g_stream));
+  QH_HIP(hipMemGetInfo(free_bytes, total_bytes));
+  return QH_OK;
+}
+
 extern "C" int qh_synchronize(void) {
   int rc = qh_ensure_init();
   if (rc) return rc;

@@ -1017,8 +1017,12 @@ int autotune(qh_mat *A, int *out_g) {
         for (int u = 140; u <= 144; u++) cands.push_back(u);
     }
   }
-  for (int g : {2, 4, 8, 16, 32, 64})
-    if (g <= 4 * avg + 4) cands.push_back(g);
+  // the CSR kernels only when no sliced layout exists or the matrix is small:
+  // on a large one they take 5-30x the sliced step and are never chosen
+  const bool sliced = A->ks_state == 1 || A->ks2_state == 1;
+  if (!sliced || A->rows < (1 << 16))
+    for (int g : {2, 4, 8, 16, 32, 64})
+      if (g <= 4 * avg + 4) cands.push_back(g);
   if (cands.empty()) cands.push_back(2);
   float best = 1e30f;
   int bestg = cands[0];
