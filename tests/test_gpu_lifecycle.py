@@ -1,9 +1,10 @@
 """Device memory across repeated use: systems built, stepped on every path
 (fixed-step RK4 in FP64 and FP32, the adaptive integrator, the batch entry
 point, an MCWF ensemble, the steady-state solve) and destroyed, twenty times
-over, must give their device memory back (what the library keeps on purpose,
-its reduction scratch and the batch kept between calls, is allocated by the
-first cycle or released explicitly)."""
+over, with their states destroyed as the C API requires (destroy_qvec), must
+give their device memory back (what the library keeps on purpose, its
+reduction scratch and the batch kept between calls, is allocated by the first
+cycle or released explicitly)."""
 import numpy as np
 import pytest
 
@@ -25,14 +26,17 @@ def _cycle():
         x = qs.create_qvec()
         x.add_loc(1.0, loc)
         qs.time_step(x, 0.0, 0.5, 0.01, 1000)
+        x.destroy()
     qa.set_option("quac_precision", "fp64")
     qa.set_option("ts_rk_type", "3bs")
     qa.set_option("ts_adapt_type", "basic")
     x = qs.create_qvec()
     x.add_loc(1.0, loc)
     qs.time_step(x, 0.0, 0.5, 0.05, 1000)
+    x.destroy()
     y = qs.create_qvec()
     qs.steady_state(y, tol=1e-8)
+    y.destroy()
     # the batch entry point on three instances, then its kept batch released
     insts = [recipes.tc_recipe(*p) for p in recipes.tc_instances(3, 5)]
     systems = [build_system(rr)[0] for rr in insts]
@@ -43,6 +47,8 @@ def _cycle():
         states.append(v)
     qa.time_step_batch(systems, states, 0.0, 0.2, 0.01, 1000)
     qa.lib().time_step_sys_batch_release()
+    for v in states:
+        v.destroy()
     for qb in systems:
         qb.destroy()
     # an MCWF ensemble
@@ -55,6 +61,7 @@ def _cycle():
     w.add(1.0, 2)
     qm.construct_matrix()
     qm.time_step(w, 0.0, 0.5, 0.01, 1000)
+    w.destroy()
     qm.destroy()
     qs.destroy()
 
