@@ -620,62 +620,155 @@ __global__ void k_ks2_diag(int32_t n, const int32_t *__restrict__ rp, const int3
   d[r] = v;
 }
 
+// Separability of the diagonal of each candidate block (N x N density matrix
+// at block offset off): row r = off + N j + i is bad when its diagonal entry
+// D(i, j) is not finite or differs from D(i, 0) + D(0, j) - D(0, 0) by more than
+// 1e-12 of the magnitudes involved.  The block of r by binary search over boff.
+__global__ void k_ks2_sep_check(int32_t n, const double2 *__restrict__ d, int32_t nb,
+                                const int64_t *__restrict__ boff, const int32_t *__restrict__ bn,
+                                int32_t *__restrict__ bad) {
+  const int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  int lo = 0, hi = nb - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) / 2;
+    if (boff[mid] <= r) lo = mid;
+    else hi = mid - 1;
+  }
+  const int32_t N = bn[lo];
+  if (N == 0) return;
+  const int64_t off = boff[lo];
+  const uint32_t t = (uint32_t)(r - off), j = t / (uint32_t)N, i = t - j * (uint32_t)N;
+  const double2 dij = d[r], di0 = d[off + i], d0j = d[off + (int64_t)N * j], d00 = d[off];
+  const double hr = d0j.x - d00.x, hi_ = d0j.y - d00.y;
+  const double er = dij.x - (di0.x + hr), ei = dij.y - (di0.y + hi_);
+  const double sc = fabs(dij.x) + fabs(dij.y) + fabs(di0.x) + fabs(di0.y) + fabs(hr) + fabs(hi_);
+  const bool ok = isfinite(dij.x) && isfinite(dij.y) && fabs(er) + fabs(ei) <= 1e-12 * sc;
+  if (!ok) bad[lo] = 1;
+}
+
+// g_i = D(i, 0) and h_j = D(0, j) - D(0, 0) of one separable block into gh
+__global__ void k_ks2_sep_gather(const double2 *__restrict__ d, int64_t off, int32_t N, double2 *__restrict__ g,
+                                 double2 *__restrict__ h) {
+  const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= N) return;
+  g[k] = d[off + k];
+  const double2 a = d[off + (int64_t)N * k], z = d[off];
+  h[k] = make_double2(a.x - z.x, a.y - z.y);
+}
+
 // The blocks of A (the instances of a batch, else A itself) whose diagonal
 // separates: an N x N density-matrix block (N even, starting at an even row)
 // whose diagonal entry at row i, column j is g_i + h_j (g_i = D(i, 0),
-// h_j = D(0, j) - D(0, 0), every entry checked to 1e-12 relative).  Fills the
-// block parameters, their g and h, and for every slice the block that holds it
-// whole (-1: none).
-static int ks2_sep_prepare(qh_mat *A, int32_t ns, std::vector<qh_sepblk> &blks, std::vector<double2> &gh,
+// h_j = D(0, j) - D(0, 0), every entry checked to 1e-12 relative on the device).
+// Fills the block parameters, their g and h (device, *gh_out of *ngh_out
+// entries), and for every slice the block that holds it whole (-1: none).
+static int ks2_sep_prepare(qh_mat *A, int32_t ns, std::vector<qh_sepblk> &blks, double2 **gh_out, int64_t *ngh_out,
                            std::vector<int32_t> &slice_blk) {
   const int64_t n = A->rows;
   std::vector<int64_t> bo = A->blk_off;
   if (bo.size() < 2 || bo.back() != n) bo = {0, n};
-  double2 *dd = nullptr;
-  int rc = qh_alloc_t(&dd, (size_t)n);
-  if (rc) return rc;
-  hipStream_t s = qh_stream();
-  hipLaunchKernelGGL(k_ks2_diag, dim3(qh_blocks(n, 256)), dim3(256), 0, s, (int32_t)n, A->rowptr, A->col, A->val, dd);
-  std::vector<double2> d((size_t)n);
-  hipError_t e = hipGetLastError();
-  if (e == hipSuccess) e = hipMemcpyAsync(d.data(), dd, sizeof(double2) * n, hipMemcpyDeviceToHost, s);
-  if (e == hipSuccess) e = hipStreamSynchronize(s);
-  qh_free_ptr(dd);
-  QH_HIP(e);
-  std::vector<int32_t> blk_of((size_t)bo.size() - 1, -1);
-  for (size_t b = 0; b + 1 < bo.size(); b++) {
+  const int32_t nb = (int32_t)bo.size() - 1;
+  std::vector<int32_t> bn((size_t)nb, 0);
+  int ncand = 0;
+  for (int32_t b = 0; b < nb; b++) {
     const int64_t off = bo[b], rows = bo[b + 1] - bo[b];
-    int64_t N = (int64_t)std::llround(std::sqrt((double)rows));
+    const int64_t N = (int64_t)std::llround(std::sqrt((double)rows));
     if (N < 2 || N * N != rows || (N & 1) || (off & 1) || N > 46340) continue;
-    auto D = [&](int64_t i, int64_t j) { return d[(size_t)(off + N * j + i)]; };
-    bool ok = true;
-    const double2 d00 = D(0, 0);
-    for (int64_t j = 0; j < N && ok; j++) {
-      const double2 d0j = D(0, j);
-      const double hr = d0j.x - d00.x, hi = d0j.y - d00.y;
-      for (int64_t i = 0; i < N && ok; i++) {
-        const double2 di0 = D(i, 0), dij = D(i, j);
-        const double er = dij.x - (di0.x + hr), ei = dij.y - (di0.y + hi);
-        const double sc = std::fabs(dij.x) + std::fabs(dij.y) + std::fabs(di0.x) + std::fabs(di0.y) +
-                          std::fabs(hr) + std::fabs(hi);
-        ok = std::isfinite(dij.x) && std::isfinite(dij.y) && std::fabs(er) + std::fabs(ei) <= 1e-12 * sc;
+    bn[b] = (int32_t)N;
+    ncand++;
+  }
+  *gh_out = nullptr;
+  *ngh_out = 0;
+  std::vector<int32_t> blk_of((size_t)nb, -1);
+  double2 *dd = nullptr, *gh = nullptr;
+  int64_t *dboff = nullptr;
+  int32_t *dbn = nullptr, *dbad = nullptr;
+  hipStream_t s = qh_stream();
+  int rc = QH_OK;
+  if (ncand > 0) {
+    rc = qh_alloc_t(&dd, (size_t)n);
+    if (!rc) rc = qh_alloc_t(&dboff, bo.size());
+    if (!rc) rc = qh_alloc_t(&dbn, (size_t)nb);
+    if (!rc) rc = qh_alloc_t(&dbad, (size_t)nb);
+    std::vector<int32_t> hbad((size_t)nb, 0);
+    hipError_t e = hipSuccess;
+    if (!rc) {
+      hipLaunchKernelGGL(k_ks2_diag, dim3(qh_blocks(n, 256)), dim3(256), 0, s, (int32_t)n, A->rowptr, A->col, A->val,
+                         dd);
+      e = hipGetLastError();
+      if (e == hipSuccess) e = hipMemcpyAsync(dboff, bo.data(), sizeof(int64_t) * bo.size(), hipMemcpyHostToDevice, s);
+      if (e == hipSuccess) e = hipMemcpyAsync(dbn, bn.data(), sizeof(int32_t) * nb, hipMemcpyHostToDevice, s);
+      if (e == hipSuccess) e = hipMemsetAsync(dbad, 0, sizeof(int32_t) * nb, s);
+      if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_ks2_sep_check, dim3(qh_blocks(n, 256)), dim3(256), 0, s, (int32_t)n, dd, nb, dboff, dbn,
+                           dbad);
+        e = hipGetLastError();
+      }
+      if (e == hipSuccess) e = hipMemcpyAsync(hbad.data(), dbad, sizeof(int32_t) * nb, hipMemcpyDeviceToHost, s);
+      if (e == hipSuccess) e = hipStreamSynchronize(s);
+      if (e != hipSuccess) {
+        qh_set_error("separable-diagonal check: %s", hipGetErrorString(e));
+        rc = QH_EHIP;
       }
     }
-    if (!ok || gh.size() + 2 * (size_t)N >= ((size_t)1 << 27)) continue;
-    qh_sepblk q;
-    q.off = off;
-    q.n = (uint32_t)N;
-    uint32_t sh = 0;
-    while (sh < 32 && ((uint64_t)1 << sh) < (uint64_t)N) sh++;
-    q.m = (uint32_t)((((uint64_t)1 << 32) * (((uint64_t)1 << sh) - (uint64_t)N)) / (uint64_t)N + 1);
-    q.sh = sh;
-    q.gbase = (uint32_t)gh.size();
-    q.hbase = (uint32_t)(gh.size() + N);
-    q.pad = 0;
-    for (int64_t i = 0; i < N; i++) gh.push_back(D(i, 0));
-    for (int64_t j = 0; j < N; j++) gh.push_back(make_double2(D(0, j).x - d00.x, D(0, j).y - d00.y));
-    blk_of[b] = (int32_t)blks.size();
-    blks.push_back(q);
+    int64_t total = 0;
+    for (int32_t b = 0; !rc && b < nb; b++) {
+      if (bn[b] == 0 || hbad[b]) continue;
+      const int64_t N = bn[b];
+      if (total + 2 * N >= ((int64_t)1 << 27)) continue;
+      qh_sepblk q;
+      q.off = bo[b];
+      q.n = (uint32_t)N;
+      uint32_t sh = 0;
+      while (sh < 32 && ((uint64_t)1 << sh) < (uint64_t)N) sh++;
+      q.m = (uint32_t)((((uint64_t)1 << 32) * (((uint64_t)1 << sh) - (uint64_t)N)) / (uint64_t)N + 1);
+      q.sh = sh;
+      q.gbase = (uint32_t)total;
+      q.hbase = (uint32_t)(total + N);
+      q.pad = 0;
+      blk_of[b] = (int32_t)blks.size();
+      blks.push_back(q);
+      total += 2 * N;
+    }
+    if (!rc && total > 0) {
+      rc = qh_alloc_t(&gh, (size_t)total);
+      for (size_t k = 0; !rc && k < blks.size(); k++) {
+        const qh_sepblk &q = blks[k];
+        hipLaunchKernelGGL(k_ks2_sep_gather, dim3(qh_blocks(q.n, 256)), dim3(256), 0, s, dd, q.off, (int32_t)q.n,
+                           gh + q.gbase, gh + q.hbase);
+      }
+      if (!rc) {
+        const hipError_t e2 = hipGetLastError();
+        if (e2 == hipSuccess) {
+          *gh_out = gh;
+          *ngh_out = total;
+          gh = nullptr;
+        } else {
+          qh_set_error("separable-diagonal gather: %s", hipGetErrorString(e2));
+          rc = QH_EHIP;
+        }
+      }
+    }
+    if (!rc) {
+      const hipError_t e3 = hipStreamSynchronize(s);  // dd is freed below
+      if (e3 != hipSuccess) {
+        qh_set_error("separable-diagonal gather: %s", hipGetErrorString(e3));
+        rc = QH_EHIP;
+      }
+    }
+  }
+  qh_free_ptr(dd);
+  qh_free_ptr(gh);
+  qh_free_ptr(dboff);
+  qh_free_ptr(dbn);
+  qh_free_ptr(dbad);
+  if (rc) {
+    blks.clear();
+    qh_free_ptr(*gh_out);
+    *gh_out = nullptr;
+    *ngh_out = 0;
+    return rc;
   }
   slice_blk.assign((size_t)ns, -1);
   size_t b = 0;
@@ -702,9 +795,10 @@ int ks2_build(qh_mat *A) {
   int32_t *cnt = nullptr, *voff = nullptr, *svoff = nullptr, *s1off = nullptr, *sblk = nullptr;
   int *ovf = nullptr;
   std::vector<qh_sepblk> blks;
-  std::vector<double2> gh;
+  double2 *ghd = nullptr;  // device g/h of the separable blocks (owned by A once built)
+  int64_t ngh = 0;
   std::vector<int32_t> hsb;
-  int rc = ks2_sep_prepare(A, ns, blks, gh, hsb);
+  int rc = ks2_sep_prepare(A, ns, blks, &ghd, &ngh, hsb);
   if (!rc) rc = qh_alloc_t(&cnt, (size_t)NC * ns);
   if (!rc) rc = qh_alloc_t(&voff, (size_t)ns + 1);
   if (!rc) rc = qh_alloc_t(&svoff, (size_t)ns + 1);
@@ -718,6 +812,8 @@ int ks2_build(qh_mat *A) {
     qh_free_ptr(s1off);
     qh_free_ptr(sblk);
     qh_free_ptr(ovf);
+    qh_free_ptr(ghd);  // null once A owns it
+    ghd = nullptr;
   };
   if (rc) {
     cleanup();
@@ -773,7 +869,14 @@ int ks2_build(qh_mat *A) {
   if (!rc) rc = qh_alloc_t(&A->ks2_cmid, (size_t)ns);
   if (!rc) rc = qh_alloc_t(&A->ks2_xmid, (size_t)ns);
   if (!rc) rc = qh_alloc_t(&A->ks2_sep, blks.empty() ? 1 : blks.size());
-  if (!rc) rc = qh_alloc_t(&A->ks2_gh, gh.empty() ? 1 : gh.size());
+  if (!rc) {
+    if (ghd) {
+      A->ks2_gh = ghd;
+      ghd = nullptr;
+    } else {
+      rc = qh_alloc_t(&A->ks2_gh, 1);
+    }
+  }
   if (!rc) rc = qh_alloc_t(&A->ks2_meta, (size_t)(tot > 0 ? tot : 1));
   if (!rc) rc = qh_alloc_t(&A->ks2_val, (size_t)(totv > 0 ? totv : 1));
   if (!rc) rc = qh_alloc_t(&A->ks2_sval, (size_t)(tots > 0 ? tots : 1));
@@ -791,7 +894,6 @@ int ks2_build(qh_mat *A) {
   QH_HIP(hipMemcpyAsync(A->ks2_xmid, hxm.data(), sizeof(int32_t) * ns, hipMemcpyHostToDevice, s));
   if (!blks.empty())
     QH_HIP(hipMemcpyAsync(A->ks2_sep, blks.data(), sizeof(qh_sepblk) * blks.size(), hipMemcpyHostToDevice, s));
-  if (!gh.empty()) QH_HIP(hipMemcpyAsync(A->ks2_gh, gh.data(), sizeof(double2) * gh.size(), hipMemcpyHostToDevice, s));
   QH_HIP(hipMemcpyAsync(voff, hv.data(), sizeof(int32_t) * (ns + 1), hipMemcpyHostToDevice, s));
   QH_HIP(hipMemcpyAsync(svoff, hso.data(), sizeof(int32_t) * (ns + 1), hipMemcpyHostToDevice, s));
   QH_HIP(hipMemcpyAsync(s1off, hs1o.data(), sizeof(int32_t) * (ns + 1), hipMemcpyHostToDevice, s));
@@ -807,7 +909,7 @@ int ks2_build(qh_mat *A) {
   A->ks2_nsval = tots;
   A->ks2_nsval1 = tots1;
   A->ks2_nsep = totx;
-  A->ks2_ngh = (int64_t)gh.size();
+  A->ks2_ngh = ngh;
   A->ks2_nsepblk = (int64_t)blks.size();
   A->ks2_state = 1;
   return QH_OK;
