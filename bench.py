@@ -369,7 +369,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU time per thread count of the sweep sample")
     ap.add_argument("--renorm", type=int, default=10, help="FP32 trace renormalisation period (steps)")
     ap.add_argument("--workload", default="headline",
-                    choices=["headline", "tc512", "circuit12", "jc", "copy", "kerr", "mcwf", "sweep_eager"])
+                    choices=["headline", "tc512", "circuit12", "jc", "copy", "kerr", "mcwf", "sweep_eager",
+                             "timedep"])
     ap.add_argument("--instances", type=int, default=512)
     ap.add_argument("--levels", type=int, default=8)
     ap.add_argument("--train", type=int, default=2000)
