@@ -1596,18 +1596,20 @@ extern "C" int qh_rk3bs_small(qh_mat *A, qh_vec *x, double t0, double tmax, doub
   return QH_OK;
 }
 
-// longest row of a CSR (one block, grid-stride)
+// longest row of a CSR: per-block maxima over a grid-stride range, combined
+// with an atomic max into out[0] (zeroed by the caller)
 __global__ void k_max_row(const int32_t *__restrict__ rp, int64_t n, int32_t *out) {
   __shared__ int32_t red[256];
   int32_t m = 0;
-  for (int64_t r = threadIdx.x; r < n; r += 256) m = max(m, rp[r + 1] - rp[r]);
+  for (int64_t r = blockIdx.x * 256 + threadIdx.x; r < n; r += (int64_t)gridDim.x * 256)
+    m = max(m, rp[r + 1] - rp[r]);
   red[threadIdx.x] = m;
   __syncthreads();
   for (int o = 128; o > 0; o >>= 1) {
     if (threadIdx.x < o) red[threadIdx.x] = max(red[threadIdx.x], red[threadIdx.x + o]);
     __syncthreads();
   }
-  if (threadIdx.x == 0) out[0] = red[0];
+  if (threadIdx.x == 0) atomicMax(out, red[0]);
 }
 
 int qh_mat_max_row(qh_mat *A) {
@@ -1616,7 +1618,10 @@ int qh_mat_max_row(qh_mat *A) {
   int rc = qh_alloc_t(&d, 1);
   if (rc) return rc;
   hipStream_t s = qh_stream();
-  hipLaunchKernelGGL(k_max_row, dim3(1), dim3(256), 0, s, A->rowptr, A->rows, d);
+  QH_HIP(hipMemsetAsync(d, 0, sizeof(int32_t), s));
+  const unsigned nb = (unsigned)std::min<int64_t>(qh_blocks(A->rows, 256), 1024);
+  hipLaunchKernelGGL(k_max_row, dim3(nb > 0 ? nb : 1), dim3(256), 0, s, A->rowptr, A->rows, d);
+  QH_HIP(hipGetLastError());
   int32_t h = 0;
   QH_HIP(hipMemcpyAsync(&h, d, sizeof(int32_t), hipMemcpyDeviceToHost, s));
   QH_HIP(hipStreamSynchronize(s));
