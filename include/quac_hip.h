@@ -133,19 +133,34 @@ int qh_rk4_timed(qh_mat *A, qh_vec *x, double dt, int64_t nsteps, int64_t tot, i
  * captured and instantiated (the TSSetUp of qsystem.c:937-1023) */
 int qh_rk4_prepare(qh_mat *A, qh_vec *x, double dt, int64_t tot, int renorm_every);
 /*
- * Time-dependent RHS (qsystem.c:1288-1330): y = A x + sum_k s_k A_k x in one
- * kernel (A's CSR and a merged CSR of the A_k, built once and cached on A),
- * s_k = s_re_im[2k] + i s_re_im[2k+1] given by the host, |s_k| <= 1e-10 skipped
- * (qsystem.c:34,1320).  ntd <= 150.  FP64 or FP32 vectors (x, y one precision).
+ * Time-dependent RHS (qsystem.c:1288-1330): y = A x + sum_k s_k A_k x, with the
+ * entries of all the A_k merged into one CSR (built once and cached on A; a
+ * dictionary of the distinct (term, value) pairs in place of per-entry values
+ * where there are few).  A's part runs on its autotuned sliced layout followed
+ * by one pass over the merged A_k, or, where A has none, fused with it in one
+ * CSR kernel.  s_k = s_re_im[2k] + i s_re_im[2k+1] given by the host,
+ * |s_k| <= 1e-10 skipped (qsystem.c:34,1320).  ntd <= 150.  FP64 or FP32
+ * vectors (x, y one precision, distinct).
  */
 int qh_spmv_td(const qh_mat *A, int ntd, qh_mat *const *Ak, const double *s_re_im,
                const qh_vec *x, qh_vec *y);
 /* One classic RK4 step of size h of dx/dt = A x + sum_k s_k(t) A_k x, in place
- * (FP64, or the FP32 hi+lo state): four fused stage launches; s4_re_im holds the
- * coefficients of the stage times t, t+h/2, t+h/2, t+h (4 x ntd complex).  The
- * RHS evaluations of time_step_sys with time-dependent terms (qsystem.c:946-950,
- * 1288-1330); the stage buffers are kept with the cached merged structure. */
+ * (FP64, or the FP32 hi+lo state): per stage A's sliced stage kernel and the A_k
+ * pass (or one fused CSR kernel); s4_re_im holds the coefficients of the stage
+ * times t, t+h/2, t+h/2, t+h (4 x ntd complex).  The RHS evaluations of
+ * time_step_sys with time-dependent terms (qsystem.c:946-950, 1288-1330); the
+ * stage buffers are kept with the cached merged structure. */
 int qh_rk4_td_step(qh_mat *A, int ntd, qh_mat *const *Ak, const double *s4_re_im, qh_vec *x, double h);
+/* Switches of the two functions above (no counterpart in the reference; for
+ * tuning and for tests that compare the forms): dict_max = the largest
+ * (term, value) dictionary (0: every entry stores its term and value; default
+ * 4096, at most 8192; applies from the next merged build), sliced = 1 runs A's
+ * part on its sliced layout where it has one (default), 0 the fused CSR kernel;
+ * lanes = the A_k pass after a sliced stage: 0 or 1 = one thread per row over the
+ * sliced (64-row, entry-major) form of the dictionary entries where it exists,
+ * 4, 8, 16 = lanes per row over their CSR (0 falls back to 4 or 8 by the rows'
+ * average length). */
+int qh_td_config(int dict_max, int sliced, int lanes);
 /* generic stage helpers for host-driven integrators (RK3BS, td RK4) */
 int qh_axpy_to(qh_vec *y, const qh_vec *x, double c_re, double c_im, const qh_vec *k); /* y = x + c k */
 int qh_lincomb(qh_vec *y, int n, const double *c_re_im, qh_vec *const *v);             /* y = sum c_i v_i */

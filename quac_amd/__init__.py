@@ -219,6 +219,7 @@ def _setup(L):
     sig("qh_bench_spmv", i32, vp, i32, i32, P(dbl))
     sig("qh_bench_stream", i32, i64, i32, P(dbl), P(dbl))
     sig("qh_mem_info", i32, P(ctypes.c_size_t), P(ctypes.c_size_t))
+    sig("qh_td_config", i32, i32, i32, i32)
     sig("qh_pc_create_hbasis", i32, vp, P(vp))
     sig("qh_pc_apply", i32, vp, vp, vp)
     sig("qh_pc_destroy", i32, vp)
@@ -835,6 +836,13 @@ def mem_info():
     f, t = ctypes.c_size_t(), ctypes.c_size_t()
     _qh(lib().qh_mem_info(ctypes.byref(f), ctypes.byref(t)))
     return f.value, t.value
+
+
+def td_config(dict_max=4096, sliced=1, lanes=0):
+    """switches of the time-dependent RHS (qh_td_config): the (term, value)
+    dictionary limit (0 = none), A's part on its sliced layout, lanes per row of
+    the A_k pass (0 = auto)"""
+    _qh(lib().qh_td_config(dict_max, sliced, lanes))
 
 
 def bench_stream(nbytes=1 << 30, reps=10):

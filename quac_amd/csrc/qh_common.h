@@ -172,6 +172,17 @@ int ks2_launch(int G, int mode, hipStream_t s, const qh_mat *A, const void *xin,
 // longest row of A into A->max_row (computed once per matrix)
 int qh_mat_max_row(qh_mat *A);
 
+// A's part of the time-dependent RK stages on A's autotuned variant when that is
+// a sliced layout (qh_spmv.hip): *G = the variant, or 0 where A runs on its CSR
+// (a generator of at most 800 rows, or a CSR variant chosen)
+int qh_td_sliced_prepare(qh_mat *A, int prec, int *G);
+// one stage of y' = A y on variant G: mode 0 kout = A xin; 1 also ynext = x +
+// cn kout; 2 xout = x + h/6 (k1 + 2 k2 + 2 k3 + A xin) (x and the low word xlo of
+// an FP32 state may be updated in place, xout = x)
+int qh_td_sliced_stage(int G, const qh_mat *A, int prec, int mode, const void *xin, void *kout, void *ynext,
+                       double cn, const void *x, const void *k1, const void *k2, const void *k3, void *xout,
+                       double h, float2 *xlo);
+
 // device allocation helpers (return QH_* codes)
 int qh_alloc(void **p, size_t bytes);
 void qh_free_ptr(void *p);

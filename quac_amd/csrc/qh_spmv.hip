@@ -2283,3 +2283,33 @@ extern "C" int qh_batch_rk4(qh_batch *b, double dt, int64_t nsteps, int renorm_e
 extern "C" qh_vec *qh_batch_state_vec(qh_batch *b) { return b ? b->x : nullptr; }
 extern "C" qh_mat *qh_batch_matrix(qh_batch *b) { return b ? b->A : nullptr; }
 extern "C" int64_t qh_batch_offset(qh_batch *b, int i) { return (b && i >= 0 && i < b->n) ? b->off[i] : -1; }
+
+// ------------------------------------------------------------ time-dependent stages
+int qh_td_sliced_prepare(qh_mat *A, int prec, int *G) {
+  *G = 0;
+  if (A->rows <= QH_SMALL_ROWS && !A->lanes_forced) return QH_OK;
+  int g = 0;
+  const int rc = lanes_for(A, prec, &g);
+  if (rc) return rc;
+  if (QH_IS_KSELL(g) || QH_IS_KS2(g)) *G = g;
+  return QH_OK;
+}
+
+int qh_td_sliced_stage(int G, const qh_mat *A, int prec, int mode, const void *xin, void *kout, void *ynext,
+                       double cn, const void *x, const void *k1, const void *k2, const void *k3, void *xout,
+                       double h, float2 *xlo) {
+  hipStream_t s = qh_stream();
+  int rc;
+#define QH_TS(R, M) \
+  launch_sliced<R, M>(G, s, A, xin, kout, ynext, (R)cn, x, k1, k2, k3, xout, (R)h, xlo)
+  if (prec == QH_FP64)
+    rc = mode == MODE_SPMV ? QH_TS(double, MODE_SPMV) : mode == MODE_STAGE ? QH_TS(double, MODE_STAGE)
+                                                                          : QH_TS(double, MODE_FINAL);
+  else
+    rc = mode == MODE_SPMV ? QH_TS(float, MODE_SPMV) : mode == MODE_STAGE ? QH_TS(float, MODE_STAGE)
+                                                                         : QH_TS(float, MODE_FINAL);
+#undef QH_TS
+  if (rc) return rc;
+  QH_HIP(hipGetLastError());
+  return QH_OK;
+}

@@ -1,17 +1,26 @@
-// qh_td.hip -- the time-dependent right-hand side as one fused kernel.
+// qh_td.hip -- the time-dependent right-hand side.
 //
 // The reference evaluates y = A x + sum_k s_k(t) A_k x with K + 1 MatMults and K
 // axpys per RHS (qsystem.c:1288-1330; s_k = a_k f_k(t), times -i for a
 // wavefunction, terms with |s_k| <= 1e-10 skipped), where A's pattern already
-// holds every A_k's entries (seeded with a = 0, qsystem.c:645-653).  Here one
-// kernel per RK stage reads A's CSR and a second, merged CSR of all the A_k
-// entries (column, term index, value) and forms the row sum of both in FP64,
-// with the RK4 stage epilogue fused (K_s and the next stage input, or the
-// update).  The s_k of the four stages of a step are uploaded together once per
-// step; the stage buffers live with the merged structure, which is built once
-// per (A, A_1..A_K) and cached on A, so a step allocates nothing.
+// holds every A_k's entries (seeded with a = 0, qsystem.c:645-653).  Here the
+// entries of all the A_k are merged into one CSR (built once per (A, A_1..A_K),
+// cached on A), and an RK stage is either
+//   - A's own stage kernel on its autotuned sliced layout, followed by one pass
+//     over the merged CSR that adds sum_k s_k A_k y to the stage's K, its next
+//     stage input or the state (k_td_add), or
+//   - one fused kernel over A's CSR and the merged CSR (k_td_stage), where A
+//     has no sliced layout (small generators, CSR variants).
+// The merged entries carry a column and either a 2-byte index into a dictionary
+// of the distinct (term, value) pairs -- whose coefficients s_k(t) v are formed
+// on the host per stage, a few per term for the usual drives -- or, past the
+// dictionary's size limit, the term index and the value.  The coefficients of
+// the four stages of a step are uploaded together once per step; the stage
+// buffers live with the merged structure, so a step allocates nothing.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <unordered_map>
 #include <vector>
 
 #include "qh_common.h"
@@ -21,32 +30,57 @@ namespace {
 
 const int TD_MAX = 150;  // time-dependent terms (the reference allocates 150)
 
-// row sum of A x + sum_e s[k_e] v_e x[c_e] for one row, lanes of a G-group striding
-template <typename T, int G>
+// the merged A_k entries of one row, lanes of a G-group striding: with the
+// dictionary (DICT) entry e adds cf[tidx_e] x[c_e], cf = the stage's s_k v table;
+// without it s[k_e] v_e x[c_e]
+template <typename T, int G, bool DICT>
+__device__ inline void td_part(int32_t row, int lane, const int32_t *__restrict__ trp,
+                               const int32_t *__restrict__ tcol, const int32_t *__restrict__ tk,
+                               const T *__restrict__ tval, const uint16_t *__restrict__ tidx,
+                               const double2 *__restrict__ s, const T *__restrict__ x, double &re, double &im) {
+  for (int32_t k = trp[row] + lane; k < trp[row + 1]; k += G) {
+    const T v = x[tcol[k]];
+    double cr, ci;
+    if (DICT) {
+      const double2 c = s[tidx[k]];
+      cr = c.x;
+      ci = c.y;
+    } else {
+      const T a = tval[k];
+      const double2 c = s[tk[k]];
+      cr = c.x * (double)a.x - c.y * (double)a.y;
+      ci = c.x * (double)a.y + c.y * (double)a.x;
+    }
+    re += cr * (double)v.x - ci * (double)v.y;
+    im += cr * (double)v.y + ci * (double)v.x;
+  }
+}
+
+template <int G>
+__device__ inline void group_sum(double &re, double &im) {
+#pragma unroll
+  for (int off = G / 2; off > 0; off >>= 1) {
+    re += __shfl_xor(re, off, G);
+    im += __shfl_xor(im, off, G);
+  }
+}
+
+// row sum of A x + the merged A_k entries for one row
+template <typename T, int G, bool DICT>
 __device__ inline double2 td_rowsum(int32_t row, int lane, const int32_t *__restrict__ rp,
                                     const int32_t *__restrict__ col, const T *__restrict__ val,
                                     const int32_t *__restrict__ trp, const int32_t *__restrict__ tcol,
                                     const int32_t *__restrict__ tk, const T *__restrict__ tval,
-                                    const double2 *__restrict__ s, const T *__restrict__ x) {
+                                    const uint16_t *__restrict__ tidx, const double2 *__restrict__ s,
+                                    const T *__restrict__ x) {
   double re = 0.0, im = 0.0;
   for (int32_t k = rp[row] + lane; k < rp[row + 1]; k += G) {
     const T a = val[k], v = x[col[k]];
     re += (double)a.x * (double)v.x - (double)a.y * (double)v.y;
     im += (double)a.x * (double)v.y + (double)a.y * (double)v.x;
   }
-  for (int32_t k = trp[row] + lane; k < trp[row + 1]; k += G) {
-    const T a = tval[k], v = x[tcol[k]];
-    const double2 c = s[tk[k]];
-    // (c * a) * v with c = s_k(t)
-    const double ar = c.x * (double)a.x - c.y * (double)a.y, ai = c.x * (double)a.y + c.y * (double)a.x;
-    re += ar * (double)v.x - ai * (double)v.y;
-    im += ar * (double)v.y + ai * (double)v.x;
-  }
-#pragma unroll
-  for (int off = G / 2; off > 0; off >>= 1) {
-    re += __shfl_xor(re, off, G);
-    im += __shfl_xor(im, off, G);
-  }
+  td_part<T, G, DICT>(row, lane, trp, tcol, tk, tval, tidx, s, x, re, im);
+  group_sum<G>(re, im);
   return make_double2(re, im);
 }
 
@@ -71,11 +105,12 @@ __device__ inline void st_state(T *x, float2 *lo, int32_t i, double re, double i
 //   MODE_STAGE: kout as above;  ynext = x + cn kout                (stages 1-3)
 //   MODE_FINAL: x = x + h/6 (K1 + 2 K2 + 2 K3 + kout), in place    (stage 4)
 // T = double2 (FP64) or float2 (FP32: x = hi + lo, lo != null)
-template <typename T, int G, int MODE>
+template <typename T, int G, int MODE, bool DICT>
 __global__ void __launch_bounds__(256) k_td_stage(int32_t nrows, const int32_t *__restrict__ rp,
                                                   const int32_t *__restrict__ col, const T *__restrict__ val,
                                                   const int32_t *__restrict__ trp, const int32_t *__restrict__ tcol,
                                                   const int32_t *__restrict__ tk, const T *__restrict__ tval,
+                                                  const uint16_t *__restrict__ tidx,
                                                   const double2 *__restrict__ s, const T *xin, T *kout, T *ynext,
                                                   double cn, T *x, float2 *lo, const T *k1, const T *k2,
                                                   const T *k3, double h) {
@@ -83,7 +118,7 @@ __global__ void __launch_bounds__(256) k_td_stage(int32_t nrows, const int32_t *
   const int32_t row = (int32_t)(tid / G);
   const int lane = (int)(tid % G);
   if (row >= nrows) return;  // G divides 256: a group never straddles the bound
-  const double2 y = td_rowsum<T, G>(row, lane, rp, col, val, trp, tcol, tk, tval, s, xin);
+  const double2 y = td_rowsum<T, G, DICT>(row, lane, rp, col, val, trp, tcol, tk, tval, tidx, s, xin);
   if (lane != 0) return;
   if (MODE == MODE_SPMV) {
     T o;
@@ -111,12 +146,101 @@ __global__ void __launch_bounds__(256) k_td_stage(int32_t nrows, const int32_t *
   st_state(x, lo, row, nr, ni);
 }
 
+// the A_k pass after A's own stage kernel has run on its sliced layout:
+//   MODE_SPMV : kout += d                      with d = sum_k s_k A_k xin
+//   MODE_STAGE: kout += d;  ynext += cn d      (stages 1-3)
+//   MODE_FINAL: x += h/6 d, in place           (stage 4; FP32: x = hi + lo)
+template <typename T, int G, int MODE, bool DICT>
+__global__ void __launch_bounds__(256) k_td_add(int32_t nrows, const int32_t *__restrict__ trp,
+                                                const int32_t *__restrict__ tcol, const int32_t *__restrict__ tk,
+                                                const T *__restrict__ tval, const uint16_t *__restrict__ tidx,
+                                                const double2 *__restrict__ s, const T *__restrict__ xin,
+                                                T *kout, T *ynext, double cn, T *x, float2 *lo, double h) {
+  const int64_t tid = (int64_t)qh_xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+  const int32_t row = (int32_t)(tid / G);
+  const int lane = (int)(tid % G);
+  if (row >= nrows) return;  // G divides 256: a group never straddles the bound
+  double re = 0.0, im = 0.0;
+  td_part<T, G, DICT>(row, lane, trp, tcol, tk, tval, tidx, s, xin, re, im);
+  group_sum<G>(re, im);
+  if (lane != 0) return;
+  if (MODE == MODE_SPMV || MODE == MODE_STAGE) {
+    T k = kout[row];
+    const double kr = (double)k.x + re, ki = (double)k.y + im;
+    k.x = kr;
+    k.y = ki;
+    kout[row] = k;
+    if (MODE == MODE_STAGE) {
+      T yn = ynext[row];
+      yn.x = (double)yn.x + cn * re;
+      yn.y = (double)yn.y + cn * im;
+      ynext[row] = yn;
+    }
+    return;
+  }
+  const double2 xo = ld_state(x, lo, row);
+  const double s6 = h * (1.0 / 6.0);
+  st_state(x, lo, row, xo.x + s6 * re, xo.y + s6 * im);
+}
+
+// The A_k pass over the sliced form of the dictionary entries: slices of 64
+// rows, slice s holding w_s entries per row (the longest of its rows; shorter
+// rows padded with column = the row itself and the dictionary's zero entry),
+// entry i of row 64 s + l at soff_s + 64 i + l, so the one thread per row reads
+// its columns and indices in fully coalesced wavefront loads.  Epilogues as
+// k_td_add.
+template <typename T, int MODE>
+__global__ void __launch_bounds__(256) k_td_ell(int32_t nrows, const int64_t *__restrict__ soff,
+                                                const int32_t *__restrict__ sw, const int32_t *__restrict__ ecol,
+                                                const uint16_t *__restrict__ eidx, const double2 *__restrict__ cf,
+                                                const T *__restrict__ xin, T *kout, T *ynext, double cn, T *x,
+                                                float2 *lo, double h) {
+  const int32_t row = (int32_t)((int64_t)qh_xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x);
+  if (row >= nrows) return;
+  const int32_t sl = row >> 6;
+  const int32_t w = sw[sl];
+  const int64_t base = soff[sl] + (row & 63);
+  double re = 0.0, im = 0.0;
+#pragma unroll 4
+  for (int32_t i = 0; i < w; i++) {
+    const int64_t e = base + 64 * (int64_t)i;
+    const T v = xin[ecol[e]];
+    const double2 c = cf[eidx[e]];
+    re += c.x * (double)v.x - c.y * (double)v.y;
+    im += c.x * (double)v.y + c.y * (double)v.x;
+  }
+  if (MODE == MODE_SPMV || MODE == MODE_STAGE) {
+    T k = kout[row];
+    k.x = (double)k.x + re;
+    k.y = (double)k.y + im;
+    kout[row] = k;
+    if (MODE == MODE_STAGE) {
+      T yn = ynext[row];
+      yn.x = (double)yn.x + cn * re;
+      yn.y = (double)yn.y + cn * im;
+      ynext[row] = yn;
+    }
+    return;
+  }
+  const double2 xo = ld_state(x, lo, row);
+  const double s6 = h * (1.0 / 6.0);
+  st_state(x, lo, row, xo.x + s6 * re, xo.y + s6 * im);
+}
+
 __global__ void k_narrow_td(const double2 *a, float2 *b, int64_t n) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     b[i] = make_float2((float)a[i].x, (float)a[i].y);
 }
 
 }  // namespace
+
+// switches of qh_td_config: the dictionary's size limit (0: no dictionary), A's
+// part on its sliced layout, the lanes per row of the A_k pass (0: from the
+// merged rows' average length)
+static int g_dict_max = 4096;
+static const int TD_DICT_CAP = 8192;
+static int g_td_sliced = 1;
+static int g_td_lanes = 0;
 
 // The merged structure of A_1..A_K and the stage buffers of one precision.
 struct qh_td_cache {
@@ -125,13 +249,23 @@ struct qh_td_cache {
   std::vector<uint64_t> akid;      // and their serial numbers
   int64_t rows = 0, tnnz = 0;
   int32_t max_row = 0;
+  int dict_max = 0;                 // the limit it was built under
   int32_t *trp = nullptr, *tcol = nullptr, *tk = nullptr;
   double2 *tval = nullptr;
   float2 *tval32 = nullptr;
-  double2 *s = nullptr;       // 4 stages x TD_MAX coefficients (device)
-  int prec = 0;               // precision of the stage buffers
+  uint16_t *tidx = nullptr;         // dictionary form: index of the (term, value) pair
+  int64_t *soff = nullptr;          // its sliced form (slices of 64 rows): slice offsets,
+  int32_t *sw = nullptr;            // entries per row of each slice,
+  int32_t *ecol = nullptr;          // columns
+  uint16_t *eidx = nullptr;         // and dictionary indices (the zero entry: index dv.size() - 1)
+  std::vector<int> dk;              // the dictionary's terms
+  std::vector<double2> dv;          // and values (host)
+  double2 *s = nullptr;             // 4 stages x (TD_MAX coefficients, or the dictionary's s_k v)
+  int prec = 0;                     // precision of the stage buffers
   void *k[3] = {nullptr, nullptr, nullptr};
   void *y[2] = {nullptr, nullptr};
+  bool dict() const { return tidx != nullptr; }
+  int stride() const { return dict() ? (int)dv.size() : TD_MAX; }  // coefficients per stage
 };
 
 static void td_free_bufs(qh_td_cache *c) {
@@ -154,6 +288,11 @@ void qh_td_cache_free(qh_td_cache *c) {
   qh_free_ptr(c->tk);
   qh_free_ptr(c->tval);
   qh_free_ptr(c->tval32);
+  qh_free_ptr(c->tidx);
+  qh_free_ptr(c->soff);
+  qh_free_ptr(c->sw);
+  qh_free_ptr(c->ecol);
+  qh_free_ptr(c->eidx);
   qh_free_ptr(c->s);
   td_free_bufs(c);
   delete c;
@@ -164,9 +303,25 @@ void qh_td_cache_free(qh_td_cache *c) {
 // as exact zeros (every A_k carries the explicit zero diagonal of
 // construct_matrix, qsystem.c:656-662) contribute s_k * 0 * x = 0 for a finite
 // state and are left out: a tenth to a third of the entries for one-body drives.
+// The distinct (term, value) pairs form the dictionary when there are at most
+// g_dict_max of them.
+struct td_key {
+  int k;
+  uint64_t re, im;
+  bool operator==(const td_key &o) const { return k == o.k && re == o.re && im == o.im; }
+};
+struct td_key_hash {
+  size_t operator()(const td_key &a) const {
+    uint64_t h = a.re * 0x9E3779B97F4A7C15ull;
+    h ^= (a.im + 0x632BE59BD9B4E019ull) + (h << 6) + (h >> 2);
+    h ^= (uint64_t)(uint32_t)a.k * 0xC2B2AE3D27D4EB4Full;
+    return (size_t)(h ^ (h >> 31));
+  }
+};
+
 static int td_cache_get(qh_mat *A, int ntd, qh_mat *const *Ak, qh_td_cache **out) {
   qh_td_cache *c = A->td_cache;
-  bool ok = c && c->ntd == ntd;
+  bool ok = c && c->ntd == ntd && c->dict_max == g_dict_max;
   for (int k = 0; ok && k < ntd; k++) ok = c->ak[k] == Ak[k] && Ak[k] && c->akid[k] == Ak[k]->uid;
   if (ok) {
     *out = c;
@@ -212,9 +367,83 @@ static int td_cache_get(qh_mat *A, int ntd, qh_mat *const *Ak, qh_td_cache **out
   }
   trp[n] = (int32_t)e;
   tot = e;
-  tcol.resize(e > 0 ? e : 1);
-  tk.resize(e > 0 ? e : 1);
-  tval.resize(e > 0 ? e : 1);
+  // the dictionary of distinct (term, value) pairs (bit patterns: -0.0 and 0.0
+  // are two values, both exact)
+  std::vector<uint16_t> tidx;
+  std::vector<int> dk;
+  std::vector<double2> dv;
+  if (g_dict_max > 0 && tot > 0) {
+    std::unordered_map<td_key, int, td_key_hash> idx;
+    tidx.resize(tot);
+    td_key last{-1, 0, 0};
+    int last_i = -1;
+    bool over = false;
+    for (int64_t q = 0; q < tot && !over; q++) {
+      td_key key{tk[q], 0, 0};
+      std::memcpy(&key.re, &tval[q].x, 8);
+      std::memcpy(&key.im, &tval[q].y, 8);
+      if (!(key == last)) {
+        auto it = idx.find(key);
+        if (it == idx.end()) {
+          if ((int)dk.size() >= g_dict_max) {
+            over = true;
+            break;
+          }
+          it = idx.emplace(key, (int)dk.size()).first;
+          dk.push_back(key.k);
+          dv.push_back(tval[q]);
+        }
+        last = key;
+        last_i = it->second;
+      }
+      tidx[q] = (uint16_t)last_i;
+    }
+    if (over) {
+      tidx.clear();
+      dk.clear();
+      dv.clear();
+    } else {
+      // the zero entry, for the padding of the sliced form
+      dk.push_back(-1);
+      dv.push_back(make_double2(0.0, 0.0));
+    }
+  }
+  // the sliced form of the dictionary entries (skipped where padding would more
+  // than double them)
+  const int64_t nsl = (n + 63) / 64;
+  std::vector<int64_t> soff;
+  std::vector<int32_t> sw, ecol;
+  std::vector<uint16_t> eidx;
+  if (!dk.empty()) {
+    soff.resize(nsl + 1);
+    sw.resize(nsl > 0 ? nsl : 1);
+    int64_t te = 0;
+    for (int64_t q = 0; q < nsl; q++) {
+      int32_t w = 0;
+      for (int64_t r = 64 * q; r < std::min<int64_t>(n, 64 * q + 64); r++) w = std::max(w, trp[r + 1] - trp[r]);
+      sw[q] = w;
+      soff[q] = te;
+      te += 64 * (int64_t)w;
+    }
+    soff[nsl] = te;
+    if (te <= 2 * tot + 64 * nsl) {
+      const uint16_t zero = (uint16_t)(dk.size() - 1);
+      ecol.resize(te > 0 ? te : 1);
+      eidx.resize(te > 0 ? te : 1);
+      for (int64_t q = 0; q < nsl; q++)
+        for (int64_t l = 0; l < 64; l++) {
+          const int64_t r = 64 * q + l;
+          const int32_t len = r < n ? trp[r + 1] - trp[r] : 0;
+          for (int32_t i = 0; i < sw[q]; i++) {
+            const int64_t e = soff[q] + 64 * (int64_t)i + l;
+            ecol[e] = i < len ? tcol[trp[r] + i] : (int32_t)(r < n ? r : 0);
+            eidx[e] = i < len ? tidx[trp[r] + i] : zero;
+          }
+        }
+    } else {
+      soff.clear();
+    }
+  }
   c = new qh_td_cache();
   c->ntd = ntd;
   c->ak.assign(Ak, Ak + ntd);
@@ -223,24 +452,51 @@ static int td_cache_get(qh_mat *A, int ntd, qh_mat *const *Ak, qh_td_cache **out
   c->rows = n;
   c->tnnz = tot;
   c->max_row = mr;
+  c->dict_max = g_dict_max;
+  const size_t ne = tot > 0 ? (size_t)tot : 1;
   rc = qh_alloc_t(&c->trp, (size_t)n + 1);
-  if (!rc) rc = qh_alloc_t(&c->tcol, tcol.size());
-  if (!rc) rc = qh_alloc_t(&c->tk, tk.size());
-  if (!rc) rc = qh_alloc_t(&c->tval, tval.size());
-  if (!rc) rc = qh_alloc_t(&c->tval32, tval.size());
-  if (!rc) rc = qh_alloc_t(&c->s, (size_t)4 * TD_MAX);
+  if (!rc) rc = qh_alloc_t(&c->tcol, ne);
+  const bool dict = !dk.empty();
+  if (dict) {
+    c->dk = dk;
+    c->dv = dv;
+    if (!rc) rc = qh_alloc_t(&c->tidx, ne);
+    if (!rc) rc = qh_alloc_t(&c->s, (size_t)4 * dv.size());
+    if (!soff.empty()) {
+      if (!rc) rc = qh_alloc_t(&c->soff, soff.size());
+      if (!rc) rc = qh_alloc_t(&c->sw, sw.size());
+      if (!rc) rc = qh_alloc_t(&c->ecol, ecol.size());
+      if (!rc) rc = qh_alloc_t(&c->eidx, eidx.size());
+    }
+  } else {
+    if (!rc) rc = qh_alloc_t(&c->tk, ne);
+    if (!rc) rc = qh_alloc_t(&c->tval, ne);
+    if (!rc) rc = qh_alloc_t(&c->tval32, ne);
+    if (!rc) rc = qh_alloc_t(&c->s, (size_t)4 * TD_MAX);
+  }
   if (rc) {
     qh_td_cache_free(c);
     return rc;
   }
   hipStream_t s = qh_stream();
   QH_HIP(hipMemcpyAsync(c->trp, trp.data(), sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice, s));
-  QH_HIP(hipMemcpyAsync(c->tcol, tcol.data(), sizeof(int32_t) * tcol.size(), hipMemcpyHostToDevice, s));
-  QH_HIP(hipMemcpyAsync(c->tk, tk.data(), sizeof(int32_t) * tk.size(), hipMemcpyHostToDevice, s));
-  QH_HIP(hipMemcpyAsync(c->tval, tval.data(), sizeof(double2) * tval.size(), hipMemcpyHostToDevice, s));
-  hipLaunchKernelGGL(k_narrow_td, dim3(qh_blocks((int64_t)tval.size(), 256)), dim3(256), 0, s, c->tval,
-                     c->tval32, (int64_t)tval.size());
-  QH_HIP(hipGetLastError());
+  if (c->soff) {
+    QH_HIP(hipMemcpyAsync(c->soff, soff.data(), sizeof(int64_t) * soff.size(), hipMemcpyHostToDevice, s));
+    QH_HIP(hipMemcpyAsync(c->sw, sw.data(), sizeof(int32_t) * sw.size(), hipMemcpyHostToDevice, s));
+    QH_HIP(hipMemcpyAsync(c->ecol, ecol.data(), sizeof(int32_t) * ecol.size(), hipMemcpyHostToDevice, s));
+    QH_HIP(hipMemcpyAsync(c->eidx, eidx.data(), sizeof(uint16_t) * eidx.size(), hipMemcpyHostToDevice, s));
+  }
+  if (tot > 0) {
+    QH_HIP(hipMemcpyAsync(c->tcol, tcol.data(), sizeof(int32_t) * tot, hipMemcpyHostToDevice, s));
+    if (dict) {
+      QH_HIP(hipMemcpyAsync(c->tidx, tidx.data(), sizeof(uint16_t) * tot, hipMemcpyHostToDevice, s));
+    } else {
+      QH_HIP(hipMemcpyAsync(c->tk, tk.data(), sizeof(int32_t) * tot, hipMemcpyHostToDevice, s));
+      QH_HIP(hipMemcpyAsync(c->tval, tval.data(), sizeof(double2) * tot, hipMemcpyHostToDevice, s));
+      hipLaunchKernelGGL(k_narrow_td, dim3(qh_blocks(tot, 256)), dim3(256), 0, s, c->tval, c->tval32, tot);
+      QH_HIP(hipGetLastError());
+    }
+  }
   QH_HIP(hipStreamSynchronize(s));
   A->td_cache = c;
   *out = c;
@@ -255,6 +511,25 @@ static void td_coefs(int ntd, const double *sri, double2 *dst) {
   }
 }
 
+// the coefficient table of one stage: s_k per term, or s_k v per dictionary entry
+static void td_table(const qh_td_cache *c, int ntd, const double *sri, double2 *dst) {
+  double2 sk[TD_MAX];
+  td_coefs(ntd, sri, sk);
+  if (!c->dict()) {
+    for (int k = 0; k < ntd; k++) dst[k] = sk[k];
+    return;
+  }
+  for (size_t d = 0; d < c->dv.size(); d++) {
+    if (c->dk[d] < 0) {  // the padding's zero entry
+      dst[d] = make_double2(0.0, 0.0);
+      continue;
+    }
+    const double2 a = sk[c->dk[d]], v = c->dv[d];
+    dst[d] = make_double2(a.x * v.x - a.y * v.y, a.x * v.y + a.y * v.x);
+  }
+}
+
+// the fused kernel: A's CSR and the merged A_k in one pass
 template <int MODE>
 static int td_launch(const qh_mat *A, const qh_td_cache *c, int prec, const double2 *s, const void *xin, void *kout,
                      void *ynext, double cn, void *x, float2 *lo, const void *k1, const void *k2, const void *k3,
@@ -264,17 +539,72 @@ static int td_launch(const qh_mat *A, const qh_td_cache *c, int prec, const doub
   const int G = longest > 24 ? 16 : 4;
   const dim3 grid(qh_blocks(n * G, 256));
   hipStream_t st = qh_stream();
-#define QH_TDL(T, GG, V, TV)                                                                                          \
-  hipLaunchKernelGGL((k_td_stage<T, GG, MODE>), grid, dim3(256), 0, st, (int32_t)n, A->rowptr, A->col, (const T *)V, \
-                     c->trp, c->tcol, c->tk, (const T *)TV, s, (const T *)xin, (T *)kout, (T *)ynext, cn, (T *)x,   \
-                     lo, (const T *)k1, (const T *)k2, (const T *)k3, h)
+#define QH_TDL(T, GG, V, TV, D)                                                                                     \
+  hipLaunchKernelGGL((k_td_stage<T, GG, MODE, D>), grid, dim3(256), 0, st, (int32_t)n, A->rowptr, A->col,          \
+                     (const T *)V, c->trp, c->tcol, c->tk, (const T *)TV, c->tidx, s, (const T *)xin, (T *)kout,   \
+                     (T *)ynext, cn, (T *)x, lo, (const T *)k1, (const T *)k2, (const T *)k3, h)
+#define QH_TDD(T, GG, V, TV) \
+  if (c->dict()) QH_TDL(T, GG, V, TV, true); else QH_TDL(T, GG, V, TV, false)
   if (prec == QH_FP64) {
-    if (G == 16) QH_TDL(double2, 16, A->val, c->tval); else QH_TDL(double2, 4, A->val, c->tval);
+    if (G == 16) { QH_TDD(double2, 16, A->val, c->tval); } else { QH_TDD(double2, 4, A->val, c->tval); }
   } else {
-    if (G == 16) QH_TDL(float2, 16, A->val32, c->tval32); else QH_TDL(float2, 4, A->val32, c->tval32);
+    if (G == 16) { QH_TDD(float2, 16, A->val32, c->tval32); } else { QH_TDD(float2, 4, A->val32, c->tval32); }
   }
+#undef QH_TDD
 #undef QH_TDL
   QH_HIP(hipGetLastError());
+  return QH_OK;
+}
+
+// the A_k pass after A's sliced stage kernel
+template <int MODE>
+static int td_add_launch(const qh_td_cache *c, int prec, const double2 *s, const void *xin, void *kout, void *ynext,
+                         double cn, void *x, float2 *lo, double h) {
+  const int64_t n = c->rows;
+  hipStream_t st = qh_stream();
+  if (c->soff && g_td_lanes <= 1) {  // the sliced form, one thread per row
+    const dim3 grid(qh_blocks(n, 256));
+    if (prec == QH_FP64)
+      hipLaunchKernelGGL((k_td_ell<double2, MODE>), grid, dim3(256), 0, st, (int32_t)n, c->soff, c->sw, c->ecol,
+                         c->eidx, s, (const double2 *)xin, (double2 *)kout, (double2 *)ynext, cn, (double2 *)x, lo, h);
+    else
+      hipLaunchKernelGGL((k_td_ell<float2, MODE>), grid, dim3(256), 0, st, (int32_t)n, c->soff, c->sw, c->ecol,
+                         c->eidx, s, (const float2 *)xin, (float2 *)kout, (float2 *)ynext, cn, (float2 *)x, lo, h);
+    QH_HIP(hipGetLastError());
+    return QH_OK;
+  }
+  int G = g_td_lanes > 1 ? g_td_lanes : 0;
+  if (G == 0) G = (double)c->tnnz <= 48.0 * (double)(n > 0 ? n : 1) ? 4 : 8;
+  const dim3 grid(qh_blocks(n * G, 256));
+#define QH_TDA(T, GG, TV, D)                                                                                        \
+  hipLaunchKernelGGL((k_td_add<T, GG, MODE, D>), grid, dim3(256), 0, st, (int32_t)n, c->trp, c->tcol, c->tk,        \
+                     (const T *)TV, c->tidx, s, (const T *)xin, (T *)kout, (T *)ynext, cn, (T *)x, lo, h)
+#define QH_TDG(T, TV, D)                 \
+  switch (G) {                           \
+    case 4: QH_TDA(T, 4, TV, D); break;  \
+    case 8: QH_TDA(T, 8, TV, D); break;  \
+    default: QH_TDA(T, 16, TV, D); break; \
+  }
+  if (prec == QH_FP64) {
+    if (c->dict()) { QH_TDG(double2, c->tval, true) } else { QH_TDG(double2, c->tval, false) }
+  } else {
+    if (c->dict()) { QH_TDG(float2, c->tval32, true) } else { QH_TDG(float2, c->tval32, false) }
+  }
+#undef QH_TDG
+#undef QH_TDA
+  QH_HIP(hipGetLastError());
+  return QH_OK;
+}
+
+extern "C" int qh_td_config(int dict_max, int sliced, int lanes) {
+  QH_REQUIRE(dict_max >= 0 && dict_max <= TD_DICT_CAP, QH_EINVAL, "qh_td_config: dictionary limit %d outside 0..%d",
+             dict_max, TD_DICT_CAP);
+  QH_REQUIRE(sliced == 0 || sliced == 1, QH_EINVAL, "qh_td_config: sliced must be 0 or 1");
+  QH_REQUIRE(lanes == 0 || lanes == 1 || lanes == 4 || lanes == 8 || lanes == 16, QH_EINVAL,
+             "qh_td_config: lanes per row must be 0, 1, 4, 8 or 16");
+  g_dict_max = dict_max;
+  g_td_sliced = sliced;
+  g_td_lanes = lanes;
   return QH_OK;
 }
 
@@ -286,12 +616,25 @@ extern "C" int qh_spmv_td(const qh_mat *A, int ntd, qh_mat *const *Ak, const dou
   QH_REQUIRE(x->n == A->rows && y->n == A->rows && x->prec == y->prec, QH_EINVAL,
              "qh_spmv_td: vectors of the matrix dimension and one precision required");
   QH_REQUIRE(x->prec == QH_FP64 || A->val32, QH_EINVAL, "qh_spmv_td: FP32 needs the complex64 values (qh_mat_make_fp32)");
+  QH_REQUIRE(x->d != y->d, QH_EINVAL, "qh_spmv_td: the input and output vectors must differ");
+  qh_mat *Am = const_cast<qh_mat *>(A);
   qh_td_cache *c = nullptr;
-  rc = td_cache_get(const_cast<qh_mat *>(A), ntd, Ak, &c);
+  rc = td_cache_get(Am, ntd, Ak, &c);
   if (rc) return rc;
-  double2 hs[TD_MAX];
-  td_coefs(ntd, s_re_im, hs);
-  QH_HIP(hipMemcpyAsync(c->s, hs, sizeof(double2) * (ntd > 0 ? ntd : 1), hipMemcpyHostToDevice, qh_stream()));
+  int G = 0;
+  if (g_td_sliced) {
+    rc = qh_td_sliced_prepare(Am, x->prec, &G);
+    if (rc) return rc;
+  }
+  std::vector<double2> hs((size_t)c->stride() + 1);
+  td_table(c, ntd, s_re_im, hs.data());
+  QH_HIP(hipMemcpyAsync(c->s, hs.data(), sizeof(double2) * c->stride(), hipMemcpyHostToDevice, qh_stream()));
+  if (G) {
+    rc = qh_td_sliced_stage(G, A, x->prec, MODE_SPMV, x->d, y->d, nullptr, 0.0, nullptr, nullptr, nullptr, nullptr,
+                            nullptr, 0.0, nullptr);
+    if (rc) return rc;
+    return td_add_launch<MODE_SPMV>(c, x->prec, c->s, x->d, y->d, nullptr, 0.0, nullptr, nullptr, 0.0);
+  }
   return td_launch<MODE_SPMV>(A, c, x->prec, c->s, x->d, y->d, nullptr, 0.0, nullptr, nullptr, nullptr, nullptr,
                               nullptr, 0.0);
 }
@@ -308,6 +651,11 @@ extern "C" int qh_rk4_td_step(qh_mat *A, int ntd, qh_mat *const *Ak, const doubl
   qh_td_cache *c = nullptr;
   rc = td_cache_get(A, ntd, Ak, &c);
   if (rc) return rc;
+  int G = 0;
+  if (g_td_sliced) {
+    rc = qh_td_sliced_prepare(A, x->prec, &G);
+    if (rc) return rc;
+  }
   if (c->prec != x->prec) {
     td_free_bufs(c);
     const size_t es = x->prec == QH_FP64 ? 16 : 8;
@@ -321,21 +669,38 @@ extern "C" int qh_rk4_td_step(qh_mat *A, int ntd, qh_mat *const *Ak, const doubl
     }
     c->prec = x->prec;
   }
-  // the four stages' coefficients in one upload: stage j at s + j * TD_MAX
-  double2 hs[4 * TD_MAX];
-  for (int j = 0; j < 4; j++) td_coefs(ntd, s4_re_im + 2 * (size_t)j * ntd, hs + j * TD_MAX);
-  QH_HIP(hipMemcpyAsync(c->s, hs, sizeof(double2) * 4 * TD_MAX, hipMemcpyHostToDevice, qh_stream()));
+  // the four stages' coefficient tables in one upload: stage j at s + j * stride
+  const int st = c->stride();
+  std::vector<double2> hs((size_t)4 * st);
+  for (int j = 0; j < 4; j++) td_table(c, ntd, s4_re_im + 2 * (size_t)j * ntd, hs.data() + (size_t)j * st);
+  QH_HIP(hipMemcpyAsync(c->s, hs.data(), sizeof(double2) * 4 * st, hipMemcpyHostToDevice, qh_stream()));
   float2 *lo = x->prec == QH_FP32 ? x->lo : nullptr;
   // K1 = f(t, x); Ya = x + h/2 K1; K2 = f(t + h/2, Ya); Yb = x + h/2 K2; K3 = f(t + h/2, Yb);
   // Ya = x + h K3; x += h/6 (K1 + 2 K2 + 2 K3 + f(t + h, Ya))
   const double cs[3] = {0.5 * h, 0.5 * h, h};
   const void *ins[3] = {x->d, c->y[0], c->y[1]};
   void *outs[3] = {c->y[0], c->y[1], c->y[0]};
+  if (G) {
+    // A's part on its sliced layout, then the A_k pass adds into the stage's outputs
+    for (int j = 0; j < 3 && !rc; j++) {
+      rc = qh_td_sliced_stage(G, A, x->prec, MODE_STAGE, ins[j], c->k[j], outs[j], cs[j], x->d, nullptr, nullptr,
+                              nullptr, nullptr, h, lo);
+      if (!rc)
+        rc = td_add_launch<MODE_STAGE>(c, x->prec, c->s + (size_t)j * st, ins[j], c->k[j], outs[j], cs[j], nullptr,
+                                       nullptr, 0.0);
+    }
+    if (!rc)
+      rc = qh_td_sliced_stage(G, A, x->prec, MODE_FINAL, c->y[0], nullptr, nullptr, 0.0, x->d, c->k[0], c->k[1],
+                              c->k[2], x->d, h, lo);
+    if (!rc)
+      rc = td_add_launch<MODE_FINAL>(c, x->prec, c->s + (size_t)3 * st, c->y[0], nullptr, nullptr, 0.0, x->d, lo, h);
+    return rc;
+  }
   for (int j = 0; j < 3 && !rc; j++)
-    rc = td_launch<MODE_STAGE>(A, c, x->prec, c->s + j * TD_MAX, ins[j], c->k[j], outs[j], cs[j], x->d, lo, nullptr,
-                               nullptr, nullptr, 0.0);
+    rc = td_launch<MODE_STAGE>(A, c, x->prec, c->s + (size_t)j * st, ins[j], c->k[j], outs[j], cs[j], x->d, lo,
+                               nullptr, nullptr, nullptr, 0.0);
   if (!rc)
-    rc = td_launch<MODE_FINAL>(A, c, x->prec, c->s + 3 * TD_MAX, c->y[0], nullptr, nullptr, 0.0, x->d, lo, c->k[0],
-                               c->k[1], c->k[2], h);
+    rc = td_launch<MODE_FINAL>(A, c, x->prec, c->s + (size_t)3 * st, c->y[0], nullptr, nullptr, 0.0, x->d, lo,
+                               c->k[0], c->k[1], c->k[2], h);
   return rc;
 }

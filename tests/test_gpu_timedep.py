@@ -21,12 +21,12 @@ F = [lambda t: math.cos(1.3 * t), lambda t: cmath.exp(0.4j * t) * math.sin(0.7 *
      lambda t: math.exp(-0.2 * t)]
 
 
-def _recipe(dm):
+def _recipe(dm, nc=3):
     terms = [("ham", 0.4, [(1, "n")]), ("td_ham", 0.3 + 0.2j, [(0, "raise"), (1, "lower")]),
              ("td_ham", 0.3 - 0.2j, [(0, "lower"), (1, "raise")])]
     if dm:
         terms += [("td_lin", 0.05, [(1, "lower")]), ("lin", 0.25, [(0, "lower")])]
-    return {"subs": [("op", 2), ("op", 3)], "terms": terms, "dm": dm}
+    return {"subs": [("op", 2), ("op", nc)], "terms": terms, "dm": dm}
 
 
 def _product(recipe, prec):
@@ -35,7 +35,7 @@ def _product(recipe, prec):
     qa.set_option("quac_precision", prec)
     qs = qa.System()
     q = qs.create_op(2)
-    c = qs.create_op(3)
+    c = qs.create_op(recipe["subs"][1][1])
     op = {(0, "raise"): q.dag, (0, "lower"): q, (1, "raise"): c.dag, (1, "lower"): c, (1, "n"): c.n}
     k = 0
     for (kind, a, ops) in recipe["terms"]:
@@ -107,12 +107,17 @@ def test_td_rk4_fp32_vs_oracle():
         qs.destroy()
 
 
-def test_spmv_td_matches_dense_sum():
-    """qh_spmv_td against A + sum_k s_k A_k from the oracle's matrices; a
+@pytest.mark.parametrize("form", ["fused_dict", "fused_full", "sliced_dict", "sliced_full", "sliced_dict_l4"])
+def test_spmv_td_matches_dense_sum(form):
+    """qh_spmv_td against A + sum_k s_k A_k from the oracle's matrices, each form
+    of the RHS (A forced onto sliced variant 134 for the sliced forms); a
     coefficient of magnitude <= 1e-10 is skipped (qsystem.c:34,1320)"""
     recipe = _recipe(True)
+    qa.td_config(*FORMS[form])
     qs, q, c = _product(recipe, "fp64")
     try:
+        if FORMS[form][1]:
+            qs.matrix().set_lanes(134, 0)
         tot, terms = recipes.oracle_terms(recipe)
         m = oracle.construct(tot, terms, dm=True)
         idx = [k for k, t in enumerate(terms) if t[0] < 0]
@@ -143,7 +148,91 @@ def test_spmv_td_matches_dense_sum():
         qa._qh(L.qh_spmv_td(qs.matrix().h, 3, mats, sc.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), xv.h, yv.h))
         assert np.abs(yv.get() - want).max() < 1e-12 * np.abs(want).max()
     finally:
+        qa.td_config()
         qs.destroy()
+
+
+# the forms of the RHS: (dictionary limit, A's part on a sliced layout, lanes of the A_k pass)
+# (lanes 0 or 1 after a sliced stage: the sliced 64-row form of the dictionary
+# entries; a limit of 3 is below these systems' distinct (term, value) pairs, so
+# the build falls back to per-entry values)
+FORMS = {"sliced_dict": (4096, 1, 0), "sliced_full": (0, 1, 0), "sliced_dict_l4": (4096, 1, 4),
+         "sliced_full_l16": (0, 1, 16), "sliced_dict_l8": (4096, 1, 8), "sliced_over": (3, 1, 1),
+         "fused_dict": (4096, 0, 0), "fused_full": (0, 0, 0)}
+# forced sliced variants of A (FP64 / FP32): one-row layouts 128, 134, 136; row-pair 142, 143
+VARIANTS = {"fp64": [134, 128, 136], "fp32": [142, 143, 134]}
+
+
+def _forms_case(prec, form, variant, nc, steps):
+    """a qubit and an nc-level cavity (density matrix), stepped under one form of
+    the time-dependent RHS, against the oracle from the same random state"""
+    qa.td_config(*FORMS[form])
+    recipe = _recipe(True, nc)
+    qs, q, c = _product(recipe, prec)
+    try:
+        if variant:
+            qs.matrix().set_lanes(variant if prec == "fp64" else 0, variant if prec == "fp32" else 0)
+        N = 2 * nc
+        rng = np.random.default_rng(nc + steps)
+        a = rng.normal(size=(N, N)) + 1j * rng.normal(size=(N, N))
+        rho = a @ a.conj().T
+        x0 = (rho / np.trace(rho)).flatten(order="F")
+        x = qs.create_qvec()
+        x.set(x0)
+        print(f"td forms: {prec} {form} variant {variant} rows {N * N} steps {steps}", flush=True)
+        qs.time_step(x, 0.1, 0.1 + 0.01 * steps, 0.01, 10 ** 6)
+        ref, _, _, _ = _oracle(recipe, x0, 0.1, 0.01, steps)
+        got = x.get()
+        if prec == "fp64":
+            assert np.abs(got - ref).max() < 1e-10
+        else:
+            assert np.abs(populations_np(got, recipe) - populations_np(ref, recipe)).max() < 1e-6
+            assert np.abs(got - ref).max() < 2e-6
+    finally:
+        qa.td_config()
+        qa.set_option("quac_precision", "fp64")
+        qs.destroy()
+
+
+@pytest.mark.parametrize("form", list(FORMS))
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_td_rk4_forms_vs_oracle(prec, form):
+    """every form of the RHS on a qubit and a 16-level cavity (1 024 rows, A on
+    its first forced sliced variant where the form runs A sliced), 150 RK4 steps"""
+    variant = VARIANTS[prec][0] if FORMS[form][1] else 0
+    _forms_case(prec, form, variant, 16, 150)
+
+
+@pytest.mark.parametrize("prec,variant", [(p, v) for p in VARIANTS for v in VARIANTS[p][1:]])
+def test_td_rk4_sliced_variants_vs_oracle(prec, variant):
+    """A's part on the other sliced variants: 12-level cavity (576 rows, sliced
+    only because forced), 100 steps"""
+    _forms_case(prec, "sliced_dict", variant, 12, 100)
+
+
+def test_td_rk4_sliced_autotuned_vs_fused():
+    """a 24-level cavity (2 304 rows): the autotuned variant of A (sliced or CSR,
+    whichever the tuner picks) against the fused CSR kernel, FP64 and FP32, with
+    the same coefficients; FP64 to 1e-12 relative"""
+    recipe = _recipe(True, 24)
+    out = {}
+    for prec in ("fp64", "fp32"):
+        for sliced in (1, 0):
+            qa.td_config(4096, sliced, 0)
+            qs, q, c = _product(recipe, prec)
+            try:
+                x = qs.create_qvec()
+                x.add_loc(1.0, 24 * 48 + 24)  # |1,0><1,0|
+                qs.time_step(x, 0.0, 0.5, 0.005, 10 ** 6)
+                out[(prec, sliced)] = x.get()
+            finally:
+                qa.td_config()
+                qa.set_option("quac_precision", "fp64")
+                qs.destroy()
+    a, b = out[("fp64", 1)], out[("fp64", 0)]
+    assert np.abs(a - b).max() < 1e-12 * np.abs(b).max()
+    a, b = out[("fp32", 1)], out[("fp32", 0)]
+    assert np.abs(populations_np(a, recipe) - populations_np(b, recipe)).max() < 1e-6
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
