@@ -164,14 +164,30 @@ def sweep_eager(args):
             "lanes": lanes}
 
 
+def _td_ell_entries(qs, nterms, rows):
+    """entries of the sliced 64-row form of the merged A_k (exact zeros left
+    out, every row of a slice padded to the slice's longest)"""
+    per_row = np.zeros(rows, np.int64)
+    for k in range(nterms):
+        rp, _, val = qs.td_matrix(k).export()
+        owner = np.repeat(np.arange(rows), np.diff(rp))
+        per_row += np.bincount(owner[val != 0], minlength=rows)
+    pad = (-rows) % 64
+    w = np.concatenate([per_row, np.zeros(pad, np.int64)]).reshape(-1, 64).max(axis=1)
+    return int(per_row.sum()), int(64 * w.sum())
+
+
 def timedep(args):
     """Time-dependent terms (qsystem.c:1288-1330) on the 10-qubit dissipative
     chain: its time-independent generator plus one driven sigma_x term per qubit,
     H_d(t) = sum_q 0.2 cos(w_q t) sigma_x^q (w_q = 1 + 0.1 q), fixed-step RK4
-    through time_step_sys (dt = 0.005): every stage one fused kernel over A's CSR
-    and the merged CSR of the ten A_k, the envelopes evaluated on the host once
-    per stage.  FP64 and the FP32 hi+lo state; populations of both after the
-    same steps."""
+    through time_step_sys (dt = 0.005).  Every stage is A's stage kernel on its
+    autotuned sliced layout and one pass over the merged A_k (a column and a
+    2-byte (term, value) dictionary index per entry, sliced 64-row entry-major),
+    the envelopes evaluated on the host once per stage.  FP64 and the FP32 hi+lo
+    state, populations of both after the same steps; and, for comparison, FP64
+    with every stage one fused kernel over A's CSR and the merged CSR carrying the
+    term index and value per entry."""
     import math
     nq = 10
     recipe = recipes.spin_chain_recipe(nq, seed=1)
@@ -189,9 +205,11 @@ def timedep(args):
     asm_s = time.perf_counter() - t0
     rows, nnz = qs.matrix().info()
     td_nnz = sum(qs.td_matrix(k).info()[1] for k in range(nq))
+    td_nonzero, td_ell = _td_ell_entries(qs, nq, rows)
     out = {}
-    for prec in ("fp64", "fp32"):
-        qa.set_option("quac_precision", prec)
+    for prec, cfg in (("fp64", (4096, 1, 0)), ("fp32", (4096, 1, 0)), ("fp64_fused_csr", (0, 0, 0))):
+        qa.td_config(*cfg)
+        qa.set_option("quac_precision", prec[:4])
         x = qs.create_qvec()
         x.add_loc(1.0, loc)
         qs.time_step(x, 0.0, 0.01, 0.005, 10)        # set-up: merged structure, stage buffers
@@ -202,13 +220,16 @@ def timedep(args):
         el = time.perf_counter() - t
         out[prec] = (steps / el, populations_np(x.get(), recipe))
         x.destroy()
+    qa.td_config()
+    lanes = qs.matrix().lanes()
     qa.set_option("quac_precision", "fp64")
     qs.destroy()
-    # bytes of one fused stage launch in the CSR model: A's and the merged td CSR
-    # (value + column, + term index for the td entries), row pointers of both,
-    # and three vector passes
+    # bytes of one FP64 A_k pass (stages 1-3): the sliced form's column and index
+    # per entry, the slice offsets and widths, K and the next stage input read
+    # and written, the stage input gathered at least once
     S = 16
-    stage_bytes = nnz * (S + 4) + td_nnz * (S + 8) + 2 * 4 * (rows + 1) + 3 * rows * S
+    nsl = (rows + 63) // 64
+    pass_bytes = td_ell * (4 + 2) + nsl * (8 + 4) + 4 * rows * S + rows * S
     return {"metric": "batched Lindblad RK4 steps/s at 1e-6 pop err vs plain FP64; SpMV % HBM peak",
             "value": round(out["fp64"][0], 2), "unit": "RK4 steps/s (FP64, time-dependent terms)", "n_gpus": 1,
             "steps": K, "warmup": 2, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
@@ -216,8 +237,11 @@ def timedep(args):
             "config": {"workload": f"10-qubit dissipative chain + 10 driven sigma_x terms (cos envelopes), "
                                    f"{rows} rows, {nnz} nnz + {td_nnz} time-dependent entries, RK4 dt=0.005"},
             "fp32_steps_per_s": round(out["fp32"][0], 2),
+            "fp64_steps_per_s_fused_csr_form": round(out["fp64_fused_csr"][0], 2),
             "max_pop_err_fp32_vs_fp64": float(np.abs(out["fp32"][1] - out["fp64"][1]).max()),
-            "stage_bytes_csr_model_fp64": stage_bytes, "assembly_s": round(asm_s, 3)}
+            "max_pop_err_fused_vs_sliced_fp64": float(np.abs(out["fp64_fused_csr"][1] - out["fp64"][1]).max()),
+            "a_variants_fp64_fp32": list(lanes), "td_nonzero_entries": td_nonzero, "td_sliced_form_entries": td_ell,
+            "td_pass_bytes_fp64": pass_bytes, "assembly_s": round(asm_s, 3)}
 
 
 def circuit12(args):
