@@ -156,10 +156,12 @@ int qh_rk4_td_step(qh_mat *A, int ntd, qh_mat *const *Ak, const double *s4_re_im
  * (term, value) dictionary (0: every entry stores its term and value; default
  * 4096, at most 8192; applies from the next merged build), sliced = 1 runs A's
  * part on its sliced layout where it has one (default), 0 the fused CSR kernel;
- * lanes = the A_k pass after a sliced stage: 0 or 1 = one thread per row over the
- * sliced (64-row, entry-major) form of the dictionary entries where it exists,
- * 4, 8, 16 = lanes per row over their CSR (0 falls back to 4 or 8 by the rows'
- * average length). */
+ * lanes = how the A_k entries join a sliced stage: 0 = summed inside A's stage
+ * kernel where its variant allows (128, 134, 136, 140, 142, 143), else as 1; 1 = a
+ * pass of their own, one thread per row over the sliced (64-row, entry-major)
+ * form of the dictionary entries; 4, 8, 16 = a pass with that many lanes per row
+ * over their CSR (0 and 1 fall back to 4 or 8 lanes, by the rows' average
+ * length, where the sliced form does not exist). */
 int qh_td_config(int dict_max, int sliced, int lanes);
 /* generic stage helpers for host-driven integrators (RK3BS, td RK4) */
 int qh_axpy_to(qh_vec *y, const qh_vec *x, double c_re, double c_im, const qh_vec *k); /* y = x + c k */

@@ -164,13 +164,28 @@ struct qh_vec {
 };
 
 // FP32 row-pair sliced layout (qh_ks2.hip): build once per matrix, launch one
-// stage (mode 0 SpMV, 1 stages 1-3, 2 stage 4) with variant G = 140 / 141
+// stage (mode 0 SpMV, 1 stages 1-3, 2 stage 4) with variant G = 140 .. 144; td
+// (may be null): the merged A_k of a time-dependent RHS summed in the same kernel
+// (variants 140, 142, 143)
+struct qh_td_ell;
 int ks2_build(qh_mat *A);
 int ks2_launch(int G, int mode, hipStream_t s, const qh_mat *A, const void *xin, void *kout, void *ynext, float cn,
-               const void *x, const void *k1, const void *k2, const void *k3, void *xout, float h, float2 *xlo);
+               const void *x, const void *k1, const void *k2, const void *k3, void *xout, float h, float2 *xlo,
+               const qh_td_ell *td = nullptr);
 
 // longest row of A into A->max_row (computed once per matrix)
 int qh_mat_max_row(qh_mat *A);
+
+// the merged A_k entries of a time-dependent RHS in their sliced form (slices of
+// 64 rows, entry i of row 64 s + l at off[s] + 64 i + l, w[s] entries per row)
+// with the stage's coefficient table cf (s_k(t) v per dictionary index)
+struct qh_td_ell {
+  const int64_t *off;
+  const int32_t *w;
+  const int32_t *col;
+  const uint16_t *idx;
+  const double2 *cf;
+};
 
 // A's part of the time-dependent RK stages on A's autotuned variant when that is
 // a sliced layout (qh_spmv.hip): *G = the variant, or 0 where A runs on its CSR
@@ -179,9 +194,13 @@ int qh_td_sliced_prepare(qh_mat *A, int prec, int *G);
 // one stage of y' = A y on variant G: mode 0 kout = A xin; 1 also ynext = x +
 // cn kout; 2 xout = x + h/6 (k1 + 2 k2 + 2 k3 + A xin) (x and the low word xlo of
 // an FP32 state may be updated in place, xout = x)
+// td (optional, may be null): the merged A_k to add inside A's stage kernel,
+// done where the variant is one of the one-row sliced variants 128, 134, 136 or
+// the row-pair variants 140, 142, 143 (*fused = 1); otherwise A's part alone
+// (*fused = 0)
 int qh_td_sliced_stage(int G, const qh_mat *A, int prec, int mode, const void *xin, void *kout, void *ynext,
                        double cn, const void *x, const void *k1, const void *k2, const void *k3, void *xout,
-                       double h, float2 *xlo);
+                       double h, float2 *xlo, const qh_td_ell *td, int *fused);
 
 // device allocation helpers (return QH_* codes)
 int qh_alloc(void **p, size_t bytes);

@@ -505,7 +505,10 @@ __device__ inline void st_pair(__amdgpu_buffer_rsrc_t rs, uint32_t o, bool two, 
 // the next stage input Y keep the default policy)
 // SEP: the matrix has separable-diagonal slots (the code for them is left out
 // of the kernels of matrices without)
-template <int MODE, int U, bool WIN, bool NT = false, bool SEP = false>
+// TDP: each row also sums its merged A_k entries of a time-dependent RHS (the
+// sliced form over 64-row slices: a lane's two rows are adjacent entries of one
+// slice), before the epilogue
+template <int MODE, int U, bool WIN, bool NT = false, bool SEP = false, bool TDP = false>
 __global__ void __launch_bounds__(WIN ? KS2_WIN / 2 : 256, WIN ? 8 : 1) k_ks2_stage(
     int32_t n, int32_t nslices, const int32_t *__restrict__ moff, const int32_t *__restrict__ wmid,
     const int32_t *__restrict__ smid, const int32_t *__restrict__ umid, const int32_t *__restrict__ emid,
@@ -516,7 +519,7 @@ __global__ void __launch_bounds__(WIN ? KS2_WIN / 2 : 256, WIN ? 8 : 1) k_ks2_st
                                                    const float *__restrict__ sval1, const float2 *xin, float2 *kout,
                                                    float2 *ynext, float cn, const float2 *x, const float2 *k1,
                                                    const float2 *k2, const float2 *k3, float2 *xout, float h,
-                                                   float2 *xlo) {
+                                                   float2 *xlo, qh_td_ell td) {
   const unsigned b = qh_xcd_block(blockIdx.x, gridDim.x);
   const int s = (int)(b * (WIN ? KS2_WIN / 128 : 4) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6));
   const int64_t vb = (int64_t)n * 8;
@@ -573,7 +576,28 @@ __global__ void __launch_bounds__(WIN ? KS2_WIN / 2 : 256, WIN ? 8 : 1) k_ks2_st
   if (!live) return;
   if (r >= n) return;
   const bool two = r + 1 < n;
-  const float4 y = make_float4((float)acc.r0, (float)acc.i0, (float)acc.r1, (float)acc.i1);
+  double a0r = acc.r0, a0i = acc.i0, a1r = acc.r1, a1i = acc.i1;
+  if constexpr (TDP) {
+    // r is even and below n, so its 64-row slice exists and the pair's columns
+    // and indices are adjacent (8- and 4-byte aligned); a row r + 1 >= n reads
+    // padding (column 0, coefficient 0)
+    const int32_t sl = r >> 6;
+    const int32_t w = td.w[sl];
+    const int64_t base = td.off[sl] + (r & 63);
+#pragma unroll 2
+    for (int32_t i = 0; i < w; i++) {
+      const int64_t e = base + 64 * (int64_t)i;
+      const int2 c2 = *reinterpret_cast<const int2 *>(td.col + e);
+      const uint32_t i2 = *reinterpret_cast<const uint32_t *>(td.idx + e);
+      const float2 v0 = xin[c2.x], v1 = xin[c2.y];
+      const double2 f0 = td.cf[i2 & 0xffffu], f1 = td.cf[i2 >> 16];
+      a0r += f0.x * (double)v0.x - f0.y * (double)v0.y;
+      a0i += f0.x * (double)v0.y + f0.y * (double)v0.x;
+      a1r += f1.x * (double)v1.x - f1.y * (double)v1.y;
+      a1i += f1.x * (double)v1.y + f1.y * (double)v1.x;
+    }
+  }
+  const float4 y = make_float4((float)a0r, (float)a0i, (float)a1r, (float)a1i);
   if (MODE == MODE_SPMV) {
     st_pair(ks_rsrc(kout, vb), r8, two, y);
     return;
@@ -586,8 +610,8 @@ __global__ void __launch_bounds__(WIN ? KS2_WIN / 2 : 256, WIN ? 8 : 1) k_ks2_st
     // uses both)
     st_pair<AUX>(ks_rsrc(kout, vb), r8, two, y);
     const double c = (double)cn;
-    const float4 yn = make_float4((float)((double)xo.x + c * acc.r0), (float)((double)xo.y + c * acc.i0),
-                                  (float)((double)xo.z + c * acc.r1), (float)((double)xo.w + c * acc.i1));
+    const float4 yn = make_float4((float)((double)xo.x + c * a0r), (float)((double)xo.y + c * a0i),
+                                  (float)((double)xo.z + c * a1r), (float)((double)xo.w + c * a1i));
     st_pair(ks_rsrc(ynext, vb), r8, two, yn);
     return;
   }
@@ -596,10 +620,10 @@ __global__ void __launch_bounds__(WIN ? KS2_WIN / 2 : 256, WIN ? 8 : 1) k_ks2_st
   const float4 a = ld_pair<AUX>(ks_rsrc(k1, vb), r8, two), bq = ld_pair<AUX>(ks_rsrc(k2, vb), r8, two),
                cq = ld_pair<AUX>(ks_rsrc(k3, vb), r8, two);
   const double s6 = (double)h * (1.0 / 6.0), s3 = (double)h * (1.0 / 3.0);
-  const double n0r = ((double)xo.x + (double)lo.x) + (s6 * (double)a.x + s3 * (double)bq.x + s3 * (double)cq.x + s6 * acc.r0);
-  const double n0i = ((double)xo.y + (double)lo.y) + (s6 * (double)a.y + s3 * (double)bq.y + s3 * (double)cq.y + s6 * acc.i0);
-  const double n1r = ((double)xo.z + (double)lo.z) + (s6 * (double)a.z + s3 * (double)bq.z + s3 * (double)cq.z + s6 * acc.r1);
-  const double n1i = ((double)xo.w + (double)lo.w) + (s6 * (double)a.w + s3 * (double)bq.w + s3 * (double)cq.w + s6 * acc.i1);
+  const double n0r = ((double)xo.x + (double)lo.x) + (s6 * (double)a.x + s3 * (double)bq.x + s3 * (double)cq.x + s6 * a0r);
+  const double n0i = ((double)xo.y + (double)lo.y) + (s6 * (double)a.y + s3 * (double)bq.y + s3 * (double)cq.y + s6 * a0i);
+  const double n1r = ((double)xo.z + (double)lo.z) + (s6 * (double)a.z + s3 * (double)bq.z + s3 * (double)cq.z + s6 * a1r);
+  const double n1i = ((double)xo.w + (double)lo.w) + (s6 * (double)a.w + s3 * (double)bq.w + s3 * (double)cq.w + s6 * a1i);
   const float4 o = make_float4((float)n0r, (float)n0i, (float)n1r, (float)n1i);
   st_pair(ks_rsrc(xout, vb), r8, two, o);
   st_pair<AUX>(lr, r8, two,
@@ -920,7 +944,8 @@ int ks2_build(qh_mat *A) {
 // the non-temporal cache policy on the streamed data);
 // mode as in the one-row-per-lane kernels.
 int ks2_launch(int G, int mode, hipStream_t s, const qh_mat *A, const void *xin, void *kout, void *ynext, float cn,
-               const void *x, const void *k1, const void *k2, const void *k3, void *xout, float h, float2 *xlo) {
+               const void *x, const void *k1, const void *k2, const void *k3, void *xout, float h, float2 *xlo,
+               const qh_td_ell *td) {
   if (A->ks2_state != 1) {
     qh_set_error("row-pair sliced layout not built for this matrix");
     return QH_EINVAL;
@@ -928,14 +953,17 @@ int ks2_launch(int G, int mode, hipStream_t s, const qh_mat *A, const void *xin,
   const int32_t n = (int32_t)A->rows, ns = A->ks2_nslices;
   const bool win = G == 142 || G == 144;
   const dim3 grid(qh_blocks(ns, win ? KS2_WIN / 128 : 4)), blk(win ? KS2_WIN / 2 : 256);
-#define QH_LK2S(MM, UU, WW, NN, SS)                                                                              \
-  hipLaunchKernelGGL((k_ks2_stage<MM, UU, WW, NN, SS>), grid, blk, 0, s, n, ns, A->ks2_moff, A->ks2_wmid, A->ks2_smid,  \
+  const qh_td_ell none{nullptr, nullptr, nullptr, nullptr, nullptr};
+#define QH_LK2T(MM, UU, WW, NN, SS, TT, TD)                                                                      \
+  hipLaunchKernelGGL((k_ks2_stage<MM, UU, WW, NN, SS, TT>), grid, blk, 0, s, n, ns, A->ks2_moff, A->ks2_wmid,       \
+                     A->ks2_smid,                                                                                \
                      A->ks2_umid, A->ks2_emid, A->ks2_cmid, A->ks2_xmid, A->ks2_sep,                               \
                      (int32_t)A->ks2_ngh,                                                                        \
                      A->ks2_gh, A->ks2_meta, (int32_t)A->ks2_nval, A->ks2_val,                                    \
                      (int32_t)A->ks2_nsval, A->ks2_sval, (int32_t)A->ks2_nsval1, A->ks2_sval1,                  \
                      (const float2 *)xin, (float2 *)kout, (float2 *)ynext, cn, (const float2 *)x,                \
-                     (const float2 *)k1, (const float2 *)k2, (const float2 *)k3, (float2 *)xout, h, xlo)
+                     (const float2 *)k1, (const float2 *)k2, (const float2 *)k3, (float2 *)xout, h, xlo, TD)
+#define QH_LK2S(MM, UU, WW, NN, SS) QH_LK2T(MM, UU, WW, NN, SS, false, none)
 #define QH_LK2(MM, UU, WW, NN)            \
   do {                                    \
     if (A->ks2_nsep)                      \
@@ -946,6 +974,34 @@ int ks2_launch(int G, int mode, hipStream_t s, const qh_mat *A, const void *xin,
   if (!QH_IS_KS2(G) || mode < MODE_SPMV || mode > MODE_FINAL) {
     qh_set_error("unsupported row-pair variant %d / mode %d", G, mode);
     return QH_EINVAL;
+  }
+  if (td) {  // the time-dependent entries summed in the same kernel (variants 140, 142, 143)
+#define QH_LK2D(MM, UU, WW, NN)                                                             \
+  do {                                                                                      \
+    if (A->ks2_nsep)                                                                        \
+      QH_LK2T(MM, UU, WW, NN, true, true, *td);                                            \
+    else                                                                                    \
+      QH_LK2T(MM, UU, WW, NN, false, true, *td);                                           \
+  } while (0)
+#define QH_LK2DG(MM)                                                                        \
+  switch (G) {                                                                              \
+    case 140: QH_LK2D(MM, 2, false, false); break;                                          \
+    case 142: QH_LK2D(MM, 2, true, false); break;                                           \
+    case 143: QH_LK2D(MM, 2, false, true); break;                                           \
+    default:                                                                                \
+      qh_set_error("row-pair variant %d has no time-dependent form", G);                    \
+      return QH_EINVAL;                                                                     \
+  }
+    if (mode == MODE_SPMV) {
+      QH_LK2DG(MODE_SPMV)
+    } else if (mode == MODE_STAGE) {
+      QH_LK2DG(MODE_STAGE)
+    } else {
+      QH_LK2DG(MODE_FINAL)
+    }
+#undef QH_LK2DG
+#undef QH_LK2D
+    return QH_OK;
   }
 #define QH_LK2G(MM)                      \
   switch (G) {                           \
@@ -964,6 +1020,7 @@ int ks2_launch(int G, int mode, hipStream_t s, const qh_mat *A, const void *xin,
   }
 #undef QH_LK2G
 #undef QH_LK2S
+#undef QH_LK2T
 #undef QH_LK2
   QH_HIP(hipGetLastError());
   return QH_OK;

@@ -475,7 +475,9 @@ __device__ inline void st_nt(typename C2<R>::T *p, int64_t r, typename C2<R>::T 
 // NT: the matrix values, the state words, K1..K3 and the K and low-word stores
 // of the RK4 stages under the non-temporal cache policy (the gathered input and
 // the next stage input keep the default one)
-template <typename R, int MODE, int U, bool VV, bool PIPE, bool NT = false>
+// TDP: the row also sums its merged A_k entries of a time-dependent RHS (the
+// sliced form over the same 64-row slices), before the epilogue
+template <typename R, int MODE, int U, bool VV, bool PIPE, bool NT = false, bool TDP = false>
 __global__ void __launch_bounds__(256) k_ks_stage(
     int32_t nrows, int32_t nslices, const int32_t *__restrict__ moff, const int32_t *__restrict__ umid,
     const int32_t *__restrict__ cmid, const qh_kmeta *__restrict__ meta, const qh_kmeta32 *__restrict__ meta32,
@@ -485,7 +487,7 @@ __global__ void __launch_bounds__(256) k_ks_stage(
     typename C2<R>::T *__restrict__ ynext, R cn, const typename C2<R>::T *__restrict__ x,
     const typename C2<R>::T *__restrict__ k1, const typename C2<R>::T *__restrict__ k2,
     const typename C2<R>::T *__restrict__ k3, typename C2<R>::T *__restrict__ xout, R h,
-    float2 *__restrict__ xlo, double atol, double rtol, double *__restrict__ eacc) {
+    float2 *__restrict__ xlo, double atol, double rtol, double *__restrict__ eacc, qh_td_ell td) {
   typedef typename C2<R>::T T;
   const unsigned b = qh_xcd_block(blockIdx.x, gridDim.x);
   const int s = (int)(b * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6));
@@ -509,7 +511,20 @@ __global__ void __launch_bounds__(256) k_ks_stage(
     ks_group<R, KS_SCA, U, VV, PIPE, AUX>(meta, km, kc, r8, xr, sr, mr, acc);
     ks_group<R, KS_CPX, U, VV, PIPE, AUX>(meta, kc, kend, r8, xr, vr, mr, acc);
   }
-  const double are = acc.re, aim = acc.im;
+  double are = acc.re, aim = acc.im;
+  if constexpr (TDP) {
+    // (rows past nrows in the last slice read padding: column 0, coefficient 0)
+    const int32_t w = td.w[s];
+    const int64_t base = td.off[s] + lane;
+#pragma unroll 4
+    for (int32_t i = 0; i < w; i++) {
+      const int64_t e = base + 64 * (int64_t)i;
+      const T v = xin[td.col[e]];
+      const double2 c = td.cf[td.idx[e]];
+      are += c.x * (double)v.x - c.y * (double)v.y;
+      aim += c.x * (double)v.y + c.y * (double)v.x;
+    }
+  }
   if (MODE == MODE_BS3E) {
     // Bogacki-Shampine error stage: k4 = A xn and the weighted squared error of
     // h (b1 - b1^) k1 + ... + h (b4 - b4^) k4 with the weights atol + rtol max(|x|, |xn|)
@@ -864,18 +879,35 @@ int ks_make_fp32(qh_mat *A) {
 template <typename R, int MODE>
 int launch_ks(int G, hipStream_t s, const qh_mat *A, const void *xin, void *kout, void *ynext, R cn, const void *x,
               const void *k1, const void *k2, const void *k3, void *xout, R h, float2 *xlo, double atol = 0.0,
-              double rtol = 0.0, double *eacc = nullptr) {
+              double rtol = 0.0, double *eacc = nullptr, const qh_td_ell *td = nullptr) {
   typedef typename C2<R>::T T;
   const T *kv = sizeof(R) == 8 ? (const T *)A->ks_val : (const T *)A->ks_val32;
   const R *sv = sizeof(R) == 8 ? (const R *)A->ks_sval : (const R *)A->ks_sval32;
   const dim3 grid(qh_blocks(A->nslices, 4)), blk(256);
   const int32_t n = (int32_t)A->rows, ns = A->nslices;
-#define QH_LK(UU, VV, PP, NN)                                                                                 \
-  hipLaunchKernelGGL((k_ks_stage<R, MODE, UU, VV, PP, NN>), grid, blk, 0, s, n, ns, A->ks_moff, A->ks_umid,          \
+  const qh_td_ell none{nullptr, nullptr, nullptr, nullptr, nullptr};
+#define QH_LKT(UU, VV, PP, NN, TT, TD)                                                                             \
+  hipLaunchKernelGGL((k_ks_stage<R, MODE, UU, VV, PP, NN, TT>), grid, blk, 0, s, n, ns, A->ks_moff, A->ks_umid,     \
                      A->ks_cmid,                                                                              \
                      A->ks_meta, A->ks_meta32, (int32_t)(A->ks_slots + QH_KS_PAD), (int32_t)A->ks_nval, kv,    \
                      (int32_t)A->ks_nsval, sv, (const T *)xin, (T *)kout, (T *)ynext, cn, (const T *)x,       \
-                     (const T *)k1, (const T *)k2, (const T *)k3, (T *)xout, h, xlo, atol, rtol, eacc)
+                     (const T *)k1, (const T *)k2, (const T *)k3, (T *)xout, h, xlo, atol, rtol, eacc, TD)
+#define QH_LK(UU, VV, PP, NN) QH_LKT(UU, VV, PP, NN, false, none)
+  if constexpr (MODE == MODE_SPMV || MODE == MODE_STAGE || MODE == MODE_FINAL) {
+    if (td) {  // the time-dependent entries summed in the same kernel (variants 128, 134, 136)
+      switch (G) {
+        case 128: QH_LKT(2, false, false, false, true, *td); return QH_OK;
+        case 134: QH_LKT(4, false, true, false, true, *td); return QH_OK;
+        case 136: QH_LKT(4, true, false, true, true, *td); return QH_OK;
+        default:
+          qh_set_error("variant %d has no time-dependent form", G);
+          return QH_EINVAL;
+      }
+    }
+  } else if (td) {
+    qh_set_error("no time-dependent form of this stage");
+    return QH_EINVAL;
+  }
   switch (G) {
     case 128: QH_LK(2, false, false, false); break;
     case 129: QH_LK(4, false, false, false); break;
@@ -891,6 +923,7 @@ int launch_ks(int G, hipStream_t s, const qh_mat *A, const void *xin, void *kout
       return QH_EINVAL;
   }
 #undef QH_LK
+#undef QH_LKT
   return QH_OK;
 }
 
@@ -2297,18 +2330,35 @@ int qh_td_sliced_prepare(qh_mat *A, int prec, int *G) {
 
 int qh_td_sliced_stage(int G, const qh_mat *A, int prec, int mode, const void *xin, void *kout, void *ynext,
                        double cn, const void *x, const void *k1, const void *k2, const void *k3, void *xout,
-                       double h, float2 *xlo) {
+                       double h, float2 *xlo, const qh_td_ell *td, int *fused) {
   hipStream_t s = qh_stream();
+  *fused = 0;
   int rc;
+  if (td && prec == QH_FP32 && (G == 140 || G == 142 || G == 143) && A->ks2_state == 1) {
+    *fused = 1;
+    rc = ks2_launch(G, mode, s, A, xin, kout, ynext, (float)cn, x, k1, k2, k3, xout, (float)h, xlo, td);
+  } else if (td && (G == 128 || G == 134 || G == 136) && A->ks_state == 1) {
+    *fused = 1;
+#define QH_TF(R, M) \
+  launch_ks<R, M>(G, s, A, xin, kout, ynext, (R)cn, x, k1, k2, k3, xout, (R)h, xlo, 0.0, 0.0, nullptr, td)
+    if (prec == QH_FP64)
+      rc = mode == MODE_SPMV ? QH_TF(double, MODE_SPMV) : mode == MODE_STAGE ? QH_TF(double, MODE_STAGE)
+                                                                            : QH_TF(double, MODE_FINAL);
+    else
+      rc = mode == MODE_SPMV ? QH_TF(float, MODE_SPMV) : mode == MODE_STAGE ? QH_TF(float, MODE_STAGE)
+                                                                           : QH_TF(float, MODE_FINAL);
+#undef QH_TF
+  } else {
 #define QH_TS(R, M) \
   launch_sliced<R, M>(G, s, A, xin, kout, ynext, (R)cn, x, k1, k2, k3, xout, (R)h, xlo)
-  if (prec == QH_FP64)
-    rc = mode == MODE_SPMV ? QH_TS(double, MODE_SPMV) : mode == MODE_STAGE ? QH_TS(double, MODE_STAGE)
-                                                                          : QH_TS(double, MODE_FINAL);
-  else
-    rc = mode == MODE_SPMV ? QH_TS(float, MODE_SPMV) : mode == MODE_STAGE ? QH_TS(float, MODE_STAGE)
-                                                                         : QH_TS(float, MODE_FINAL);
+    if (prec == QH_FP64)
+      rc = mode == MODE_SPMV ? QH_TS(double, MODE_SPMV) : mode == MODE_STAGE ? QH_TS(double, MODE_STAGE)
+                                                                            : QH_TS(double, MODE_FINAL);
+    else
+      rc = mode == MODE_SPMV ? QH_TS(float, MODE_SPMV) : mode == MODE_STAGE ? QH_TS(float, MODE_STAGE)
+                                                                           : QH_TS(float, MODE_FINAL);
 #undef QH_TS
+  }
   if (rc) return rc;
   QH_HIP(hipGetLastError());
   return QH_OK;

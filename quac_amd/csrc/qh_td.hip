@@ -6,9 +6,12 @@
 // holds every A_k's entries (seeded with a = 0, qsystem.c:645-653).  Here the
 // entries of all the A_k are merged into one CSR (built once per (A, A_1..A_K),
 // cached on A), and an RK stage is either
-//   - A's own stage kernel on its autotuned sliced layout, followed by one pass
-//     over the merged CSR that adds sum_k s_k A_k y to the stage's K, its next
-//     stage input or the state (k_td_add), or
+//   - A's own stage kernel on its autotuned sliced layout, summing the merged
+//     entries of each row in the same kernel where the variant allows (the
+//     one-row layouts share the 64-row slices; a row-pair lane's two rows are
+//     adjacent in one of them), or followed by one pass over
+//     the merged entries that adds sum_k s_k A_k y to the stage's K, its next
+//     stage input or the state (k_td_ell, k_td_add), or
 //   - one fused kernel over A's CSR and the merged CSR (k_td_stage), where A
 //     has no sliced layout (small generators, CSR variants).
 // The merged entries carry a column and either a 2-byte index into a dictionary
@@ -562,7 +565,7 @@ static int td_add_launch(const qh_td_cache *c, int prec, const double2 *s, const
                          double cn, void *x, float2 *lo, double h) {
   const int64_t n = c->rows;
   hipStream_t st = qh_stream();
-  if (c->soff && g_td_lanes <= 1) {  // the sliced form, one thread per row
+  if (c->soff && g_td_lanes <= 1) {  // the sliced form, one thread per row (a pass of its own)
     const dim3 grid(qh_blocks(n, 256));
     if (prec == QH_FP64)
       hipLaunchKernelGGL((k_td_ell<double2, MODE>), grid, dim3(256), 0, st, (int32_t)n, c->soff, c->sw, c->ecol,
@@ -594,6 +597,14 @@ static int td_add_launch(const qh_td_cache *c, int prec, const double2 *s, const
 #undef QH_TDA
   QH_HIP(hipGetLastError());
   return QH_OK;
+}
+
+// the sliced form of the merged A_k for A's stage kernel to sum in the same pass
+// (lanes 0: wherever A's variant has that form), or null
+static const qh_td_ell *td_fusable(const qh_td_cache *c, const double2 *cf, qh_td_ell *buf) {
+  if (!c->soff || g_td_lanes != 0) return nullptr;
+  *buf = qh_td_ell{c->soff, c->sw, c->ecol, c->eidx, cf};
+  return buf;
 }
 
 extern "C" int qh_td_config(int dict_max, int sliced, int lanes) {
@@ -630,9 +641,11 @@ extern "C" int qh_spmv_td(const qh_mat *A, int ntd, qh_mat *const *Ak, const dou
   td_table(c, ntd, s_re_im, hs.data());
   QH_HIP(hipMemcpyAsync(c->s, hs.data(), sizeof(double2) * c->stride(), hipMemcpyHostToDevice, qh_stream()));
   if (G) {
+    qh_td_ell tb;
+    int fused = 0;
     rc = qh_td_sliced_stage(G, A, x->prec, MODE_SPMV, x->d, y->d, nullptr, 0.0, nullptr, nullptr, nullptr, nullptr,
-                            nullptr, 0.0, nullptr);
-    if (rc) return rc;
+                            nullptr, 0.0, nullptr, td_fusable(c, c->s, &tb), &fused);
+    if (rc || fused) return rc;
     return td_add_launch<MODE_SPMV>(c, x->prec, c->s, x->d, y->d, nullptr, 0.0, nullptr, nullptr, 0.0);
   }
   return td_launch<MODE_SPMV>(A, c, x->prec, c->s, x->d, y->d, nullptr, 0.0, nullptr, nullptr, nullptr, nullptr,
@@ -681,19 +694,24 @@ extern "C" int qh_rk4_td_step(qh_mat *A, int ntd, qh_mat *const *Ak, const doubl
   const void *ins[3] = {x->d, c->y[0], c->y[1]};
   void *outs[3] = {c->y[0], c->y[1], c->y[0]};
   if (G) {
-    // A's part on its sliced layout, then the A_k pass adds into the stage's outputs
+    // A's part on its sliced layout with the A_k entries summed in the same
+    // kernel where its variant allows, else followed by the A_k pass that adds
+    // into the stage's outputs
+    qh_td_ell tb;
+    int fused = 0;
     for (int j = 0; j < 3 && !rc; j++) {
+      const double2 *cf = c->s + (size_t)j * st;
       rc = qh_td_sliced_stage(G, A, x->prec, MODE_STAGE, ins[j], c->k[j], outs[j], cs[j], x->d, nullptr, nullptr,
-                              nullptr, nullptr, h, lo);
-      if (!rc)
-        rc = td_add_launch<MODE_STAGE>(c, x->prec, c->s + (size_t)j * st, ins[j], c->k[j], outs[j], cs[j], nullptr,
-                                       nullptr, 0.0);
+                              nullptr, nullptr, h, lo, td_fusable(c, cf, &tb), &fused);
+      if (!rc && !fused)
+        rc = td_add_launch<MODE_STAGE>(c, x->prec, cf, ins[j], c->k[j], outs[j], cs[j], nullptr, nullptr, 0.0);
     }
+    const double2 *cf = c->s + (size_t)3 * st;
     if (!rc)
       rc = qh_td_sliced_stage(G, A, x->prec, MODE_FINAL, c->y[0], nullptr, nullptr, 0.0, x->d, c->k[0], c->k[1],
-                              c->k[2], x->d, h, lo);
-    if (!rc)
-      rc = td_add_launch<MODE_FINAL>(c, x->prec, c->s + (size_t)3 * st, c->y[0], nullptr, nullptr, 0.0, x->d, lo, h);
+                              c->k[2], x->d, h, lo, td_fusable(c, cf, &tb), &fused);
+    if (!rc && !fused)
+      rc = td_add_launch<MODE_FINAL>(c, x->prec, cf, c->y[0], nullptr, nullptr, 0.0, x->d, lo, h);
     return rc;
   }
   for (int j = 0; j < 3 && !rc; j++)

@@ -107,7 +107,8 @@ def test_td_rk4_fp32_vs_oracle():
         qs.destroy()
 
 
-@pytest.mark.parametrize("form", ["fused_dict", "fused_full", "sliced_dict", "sliced_full", "sliced_dict_l4"])
+@pytest.mark.parametrize("form", ["fused_dict", "fused_full", "sliced_dict", "sliced_dict_pass", "sliced_full",
+                                  "sliced_dict_l4"])
 def test_spmv_td_matches_dense_sum(form):
     """qh_spmv_td against A + sum_k s_k A_k from the oracle's matrices, each form
     of the RHS (A forced onto sliced variant 134 for the sliced forms); a
@@ -153,10 +154,12 @@ def test_spmv_td_matches_dense_sum(form):
 
 
 # the forms of the RHS: (dictionary limit, A's part on a sliced layout, lanes of the A_k pass)
-# (lanes 0 or 1 after a sliced stage: the sliced 64-row form of the dictionary
-# entries; a limit of 3 is below these systems' distinct (term, value) pairs, so
-# the build falls back to per-entry values)
-FORMS = {"sliced_dict": (4096, 1, 0), "sliced_full": (0, 1, 0), "sliced_dict_l4": (4096, 1, 4),
+# (lanes 0: the sliced 64-row form of the dictionary entries summed inside A's
+# stage kernel for the one-row variants 128, 134, 136, else in a pass of its own;
+# lanes 1: always that pass; a limit of 3 is below these systems' distinct
+# (term, value) pairs, so the build falls back to per-entry values)
+FORMS = {"sliced_dict": (4096, 1, 0), "sliced_dict_pass": (4096, 1, 1), "sliced_full": (0, 1, 0),
+         "sliced_dict_l4": (4096, 1, 4),
          "sliced_full_l16": (0, 1, 16), "sliced_dict_l8": (4096, 1, 8), "sliced_over": (3, 1, 1),
          "fused_dict": (4096, 0, 0), "fused_full": (0, 0, 0)}
 # forced sliced variants of A (FP64 / FP32): one-row layouts 128, 134, 136; row-pair 142, 143
