@@ -225,6 +225,23 @@ def timedep(args):
     qa.td_config()
     lanes = qs.matrix().lanes()
     qa.set_option("quac_precision", "fp64")
+    # the default integrator (adaptive RK3BS, the reference's TS defaults) over
+    # the same interval: every trial step four RHS evaluations through
+    # qh_spmv_td, the combinations and the error norm on the device
+    qa.set_option("ts_rk_type", "3bs")
+    qa.set_option("ts_adapt_type", "basic")
+    x = qs.create_qvec()
+    x.add_loc(1.0, loc)
+    qs.time_step(x, 0.0, 0.01, 0.005, 10)
+    qa.synchronize()
+    t = time.perf_counter()
+    rk3_steps, _ = qs.time_step(x, 0.01, 0.01 + K * 0.005, 0.005, 10 ** 7)
+    qa.synchronize()
+    rk3_s = time.perf_counter() - t
+    rk3_err = float(np.abs(populations_np(x.get(), recipe) - out["fp64"][1]).max())
+    x.destroy()
+    qa.set_option("ts_rk_type", "4")
+    qa.set_option("ts_adapt_type", "none")
     qs.destroy()
     # bytes the A_k entries add to an FP64 stage summed in A's kernel: the sliced
     # form's column and index per entry and the slice offsets and widths (as a
@@ -244,6 +261,9 @@ def timedep(args):
             "fp64_steps_per_s_fused_csr_form": round(out["fp64_fused_csr"][0], 2),
             "max_pop_err_fp32_vs_fp64": float(np.abs(out["fp32"][1] - out["fp64"][1]).max()),
             "max_pop_err_fused_vs_sliced_fp64": float(np.abs(out["fp64_fused_csr"][1] - out["fp64"][1]).max()),
+            "default_integrator_rk3bs": {"steps": int(rk3_steps), "seconds": round(rk3_s, 4),
+                                         "steps_per_s": round(rk3_steps / rk3_s, 2),
+                                         "max_pop_diff_vs_rk4": rk3_err},
             "a_variants_fp64_fp32": list(lanes), "td_nonzero_entries": td_nonzero, "td_sliced_form_entries": td_ell,
             "td_entry_bytes_per_stage": pass_bytes, "assembly_s": round(asm_s, 3)}
 
