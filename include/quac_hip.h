@@ -177,6 +177,16 @@ int qh_rk3bs_err(const qh_vec *x, const qh_vec *xn, const qh_vec *k1, const qh_v
  * three fused launches. */
 int qh_rk3bs_step(qh_mat *A, const qh_vec *x, qh_vec *xn, qh_vec *k1, qh_vec *k2, qh_vec *k3, qh_vec *k4,
                   qh_vec *y, double h, double atol, double rtol, int have_k1, double *err);
+/* The same trial step of dx/dt = A x + sum_k s_k(t) A_k x (the default
+ * integrator of time_step_sys with time-dependent terms, qsystem.c:1003-1005,
+ * 1288-1330): s4_re_im holds the coefficients of the stage times t, t + h/2,
+ * t + 3h/4, t + h (4 x ntd complex; the first unused when have_k1).  Where A has
+ * a one-row sliced FP64 variant (128, 134, 136) the merged entries are summed in
+ * the four stage launches of qh_rk3bs_step; otherwise four qh_spmv_td
+ * evaluations and the combinations. */
+int qh_rk3bs_td_step(qh_mat *A, int ntd, qh_mat *const *Ak, const double *s4_re_im, const qh_vec *x, qh_vec *xn,
+                     qh_vec *k1, qh_vec *k2, qh_vec *k3, qh_vec *k4, qh_vec *y, double h, double atol, double rtol,
+                     int have_k1, double *err);
 /* The whole Bogacki-Shampine integration of dx/dt = A x from t0 to tmax for a
  * generator of at most 640 rows (FP64 x in place) in one workgroup, the step
  * size controller included (the time_step_sys loop without monitor, events or

@@ -501,14 +501,6 @@ static void td_coefs(qsystem s, PetscReal t, double *sc, Mat *mats) {
   }
 }
 
-/* time-dependent RHS y = A x + sum_k s_k(t) A_k x (qsystem.c:1300-1330): one fused kernel */
-static int td_rhs(qsystem s, PetscReal t, Vec x, Vec y) {
-  double sc[2 * NUM_INIT_ALLOC];
-  Mat mats[NUM_INIT_ALLOC];
-  td_coefs(s, t, sc, mats);
-  return qh_spmv_td(s->mat_A, (int)s->num_time_dep, mats, sc, x, y);
-}
-
 /* rho <- rho / Tr(rho) (FP64 trace of the FP32 state) */
 static int renorm_trace(Vec x, PetscInt tot) {
   double re = 0.0, im = 0.0;
@@ -556,18 +548,21 @@ static int rk3bs_try(qsystem s, Vec x, PetscReal t, PetscReal h, work_t *w, int 
   int td = s->num_time_dep > 0;
   if (!td) /* time-independent RHS: the fused device step */
     return qh_rk3bs_step(s->mat_A, x, w->xn, w->k[0], w->k[1], w->k[2], w->k[3], w->y, h, atol, rtol, have_k1, err);
-  if (!have_k1) rc = td ? td_rhs(s, t, x, w->k[0]) : qh_spmv(s->mat_A, x, w->k[0]);
-  if (!rc) rc = qh_axpy_to(w->y, x, 0.5 * h, 0.0, w->k[0]);
-  if (!rc) rc = td ? td_rhs(s, t + 0.5 * h, w->y, w->k[1]) : qh_spmv(s->mat_A, w->y, w->k[1]);
-  if (!rc) rc = qh_axpy_to(w->y, x, 0.75 * h, 0.0, w->k[1]);
-  if (!rc) rc = td ? td_rhs(s, t + 0.75 * h, w->y, w->k[2]) : qh_spmv(s->mat_A, w->y, w->k[2]);
-  if (!rc) {
-    double c[8] = {1.0, 0.0, h * 2.0 / 9.0, 0.0, h / 3.0, 0.0, h * 4.0 / 9.0, 0.0};
-    Vec v[4] = {x, w->k[0], w->k[1], w->k[2]};
-    rc = qh_lincomb(w->xn, 4, c, v);
+  /* the envelopes at the four stage times, each evaluated once per stage as the
+   * reference's RHS callback is (the first not at all under FSAL) */
+  const int n = (int)s->num_time_dep;
+  double sc[4 * 2 * NUM_INIT_ALLOC];
+  Mat mats[NUM_INIT_ALLOC];
+  const PetscReal ts[4] = {t, t + 0.5 * h, t + 0.75 * h, t + h};
+  for (int j = 0; j < 4; j++) {
+    if (j == 0 && have_k1) {
+      memset(sc, 0, sizeof(double) * 2 * (size_t)n);
+      continue;
+    }
+    td_coefs(s, ts[j], sc + 2 * (size_t)j * n, mats);
   }
-  if (!rc) rc = td ? td_rhs(s, t + h, w->xn, w->k[3]) : qh_spmv(s->mat_A, w->xn, w->k[3]);
-  if (!rc) rc = qh_rk3bs_err(x, w->xn, w->k[0], w->k[1], w->k[2], w->k[3], h, atol, rtol, err);
+  rc = qh_rk3bs_td_step(s->mat_A, n, mats, sc, x, w->xn, w->k[0], w->k[1], w->k[2], w->k[3], w->y, h, atol, rtol,
+                        have_k1, err);
   return rc;
 }
 
@@ -783,9 +778,10 @@ void time_step_sys(qsystem qsys, qvec x, PetscReal init_time, PetscReal time_max
           step++;
           if (hh < h && have_ev && fabs(t - t_ev) <= 1e-9) hnew = h; /* shortened for the event */
         }
-        /* FSAL: k[0] now holds A x for the current x (accepted: A x_{n+1},
-         * rejected: unchanged); only valid for a time-independent RHS */
-        have_k1 = !td;
+        /* FSAL: k[0] now holds f(t, x) for the current t and x (accepted: the
+         * last stage f(t + h, x_{n+1}), its time-dependent coefficients taken at
+         * t + h; rejected: unchanged) */
+        have_k1 = 1;
         h = hnew;
       }
       if (!rc && events && have_ev && fabs(t - t_ev) <= 1e-9) {

@@ -173,6 +173,15 @@ int ks2_launch(int G, int mode, hipStream_t s, const qh_mat *A, const void *xin,
                const void *x, const void *k1, const void *k2, const void *k3, void *xout, float h, float2 *xlo,
                const qh_td_ell *td = nullptr);
 
+// the Bogacki-Shampine trial step of qh_rk3bs_step with the merged
+// time-dependent entries summed in A's stage kernels (td4: the tables of the
+// stage times t, t + h/2, t + 3h/4, t + h), where A's FP64 variant is one of the
+// one-row sliced variants 128, 134, 136 (*done = 1); otherwise nothing is
+// launched (*done = 0)
+int qh_rk3bs_td_sliced(qh_mat *A, const qh_vec *x, qh_vec *xn, qh_vec *k1, qh_vec *k2, qh_vec *k3, qh_vec *k4,
+                       qh_vec *y, double h, double atol, double rtol, int have_k1, double *err,
+                       const qh_td_ell *td4, int *done);
+
 // longest row of A into A->max_row (computed once per matrix)
 int qh_mat_max_row(qh_mat *A);
 

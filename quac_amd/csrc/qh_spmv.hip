@@ -893,7 +893,8 @@ int launch_ks(int G, hipStream_t s, const qh_mat *A, const void *xin, void *kout
                      (int32_t)A->ks_nsval, sv, (const T *)xin, (T *)kout, (T *)ynext, cn, (const T *)x,       \
                      (const T *)k1, (const T *)k2, (const T *)k3, (T *)xout, h, xlo, atol, rtol, eacc, TD)
 #define QH_LK(UU, VV, PP, NN) QH_LKT(UU, VV, PP, NN, false, none)
-  if constexpr (MODE == MODE_SPMV || MODE == MODE_STAGE || MODE == MODE_FINAL) {
+  if constexpr (MODE == MODE_SPMV || MODE == MODE_STAGE || MODE == MODE_FINAL ||
+                ((MODE == MODE_BS3 || MODE == MODE_BS3E) && sizeof(R) == 8)) {
     if (td) {  // the time-dependent entries summed in the same kernel (variants 128, 134, 136)
       switch (G) {
         case 128: QH_LKT(2, false, false, false, true, *td); return QH_OK;
@@ -2024,15 +2025,56 @@ __global__ void __launch_bounds__(1024) k_sum_partials(const double *p, int32_t 
 // candidate xn and the error sums are formed in the epilogues; y holds Y2, k4
 // holds Y3 until the last stage); on the CSR kernels the same sequence runs as
 // separate launches.
-extern "C" int qh_rk3bs_step(qh_mat *A, const qh_vec *x, qh_vec *xn, qh_vec *k1, qh_vec *k2, qh_vec *k3, qh_vec *k4,
-                             qh_vec *y, double h, double atol, double rtol, int have_k1, double *err) {
-  QH_REQUIRE(A && x && xn && k1 && k2 && k3 && k4 && y && err, QH_EINVAL, "qh_rk3bs_step: null argument");
+// The Bogacki-Shampine trial step on a one-row sliced variant G: four stage
+// launches (k1 = f(x) unless have_k1; y = x + h/2 k1; k2 = f(y), Y = x + 3h/4 k2;
+// k3 = f(Y), xn = x + h (2/9 k1 + 1/3 k2 + 4/9 k3); k4 = f(xn) and the weighted
+// error norm), f = A plus, with td (four stage tables, may be null), the merged
+// time-dependent entries summed in the same kernels
+static int rk3bs_ks(int G, qh_mat *A, const qh_vec *x, qh_vec *xn, qh_vec *k1, qh_vec *k2, qh_vec *k3, qh_vec *k4,
+                    qh_vec *y, double h, double atol, double rtol, int have_k1, double *err, const qh_td_ell *td) {
+  hipStream_t s = qh_stream();
+  // per-slice error partial sums, then their total at dacc[nslices]
+  double *dacc = nullptr, *hacc = nullptr;
+  int rc = qh_scratch(sizeof(double) * ((size_t)A->nslices + 1), (void **)&dacc, sizeof(double), (void **)&hacc);
+  if (rc) return rc;
+  if (!have_k1)
+    rc = launch_ks<double, MODE_SPMV>(G, s, A, x->d, k1->d, nullptr, 0.0, nullptr, nullptr, nullptr, nullptr,
+                                      nullptr, 0.0, nullptr, 0.0, 0.0, nullptr, td ? &td[0] : nullptr);
+  if (!rc) rc = qh_axpy_to(y, x, 0.5 * h, 0.0, k1);
+  if (!rc)
+    rc = launch_ks<double, MODE_STAGE>(G, s, A, y->d, k2->d, k4->d, 0.75 * h, x->d, nullptr, nullptr, nullptr,
+                                       nullptr, h, nullptr, 0.0, 0.0, nullptr, td ? &td[1] : nullptr);
+  if (!rc)
+    rc = launch_ks<double, MODE_BS3>(G, s, A, k4->d, k3->d, nullptr, 0.0, x->d, k1->d, k2->d, nullptr, xn->d, h,
+                                     nullptr, 0.0, 0.0, nullptr, td ? &td[2] : nullptr);
+  if (rc) return rc;
+  rc = launch_ks<double, MODE_BS3E>(G, s, A, xn->d, k4->d, nullptr, 0.0, x->d, k1->d, k2->d, k3->d, xn->d, h,
+                                    nullptr, atol, rtol, dacc, td ? &td[3] : nullptr);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, s, dacc, (int32_t)A->nslices, dacc + A->nslices);
+  QH_HIP(hipGetLastError());
+  QH_HIP(hipMemcpyAsync(hacc, dacc + A->nslices, sizeof(double), hipMemcpyDeviceToHost, s));
+  QH_HIP(hipStreamSynchronize(s));
+  *err = sqrt(hacc[0] / (double)A->rows);
+  return QH_OK;
+}
+
+static int rk3bs_check(qh_mat *A, const qh_vec *x, qh_vec *xn, qh_vec *k1, qh_vec *k2, qh_vec *k3, qh_vec *k4,
+                       qh_vec *y, double *err, const char *who) {
+  QH_REQUIRE(A && x && xn && k1 && k2 && k3 && k4 && y && err, QH_EINVAL, "%s: null argument", who);
   const qh_vec *vs[7] = {x, xn, k1, k2, k3, k4, y};
   for (const qh_vec *v : vs)
-    QH_REQUIRE(v->prec == QH_FP64 && v->n == A->rows, QH_EINVAL,
-               "qh_rk3bs_step: FP64 vectors of the matrix dimension required");
+    QH_REQUIRE(v->prec == QH_FP64 && v->n == A->rows, QH_EINVAL, "%s: FP64 vectors of the matrix dimension required",
+               who);
+  return QH_OK;
+}
+
+extern "C" int qh_rk3bs_step(qh_mat *A, const qh_vec *x, qh_vec *xn, qh_vec *k1, qh_vec *k2, qh_vec *k3, qh_vec *k4,
+                             qh_vec *y, double h, double atol, double rtol, int have_k1, double *err) {
+  int rc = rk3bs_check(A, x, xn, k1, k2, k3, k4, y, err, "qh_rk3bs_step");
+  if (rc) return rc;
   int G = 0;
-  int rc = lanes_for(A, QH_FP64, &G);
+  rc = lanes_for(A, QH_FP64, &G);
   if (rc) return rc;
   if (!QH_IS_KSELL(G)) {
     if (!have_k1) rc = qh_spmv(A, x, k1);
@@ -2049,31 +2091,22 @@ extern "C" int qh_rk3bs_step(qh_mat *A, const qh_vec *x, qh_vec *xn, qh_vec *k1,
     if (!rc) rc = qh_rk3bs_err(x, xn, k1, k2, k3, k4, h, atol, rtol, err);
     return rc;
   }
-  hipStream_t s = qh_stream();
-  // per-slice error partial sums, then their total at dacc[nslices]
-  double *dacc = nullptr, *hacc = nullptr;
-  rc = qh_scratch(sizeof(double) * ((size_t)A->nslices + 1), (void **)&dacc, sizeof(double), (void **)&hacc);
+  return rk3bs_ks(G, A, x, xn, k1, k2, k3, k4, y, h, atol, rtol, have_k1, err, nullptr);
+}
+
+int qh_rk3bs_td_sliced(qh_mat *A, const qh_vec *x, qh_vec *xn, qh_vec *k1, qh_vec *k2, qh_vec *k3, qh_vec *k4,
+                       qh_vec *y, double h, double atol, double rtol, int have_k1, double *err,
+                       const qh_td_ell *td4, int *done) {
+  *done = 0;
+  int rc = rk3bs_check(A, x, xn, k1, k2, k3, k4, y, err, "qh_rk3bs_td_step");
   if (rc) return rc;
-  if (!have_k1)
-    rc = launch_ks<double, MODE_SPMV>(G, s, A, x->d, k1->d, nullptr, 0.0, nullptr, nullptr, nullptr, nullptr,
-                                      nullptr, 0.0, nullptr);
-  if (!rc) rc = qh_axpy_to(y, x, 0.5 * h, 0.0, k1);
-  if (!rc)
-    rc = launch_ks<double, MODE_STAGE>(G, s, A, y->d, k2->d, k4->d, 0.75 * h, x->d, nullptr, nullptr, nullptr,
-                                       nullptr, h, nullptr);
-  if (!rc)
-    rc = launch_ks<double, MODE_BS3>(G, s, A, k4->d, k3->d, nullptr, 0.0, x->d, k1->d, k2->d, nullptr, xn->d, h,
-                                     nullptr);
+  if (A->rows <= QH_SMALL_ROWS && !A->lanes_forced) return QH_OK;
+  int G = 0;
+  rc = lanes_for(A, QH_FP64, &G);
   if (rc) return rc;
-  rc = launch_ks<double, MODE_BS3E>(G, s, A, xn->d, k4->d, nullptr, 0.0, x->d, k1->d, k2->d, k3->d, xn->d, h,
-                                    nullptr, atol, rtol, dacc);
-  if (rc) return rc;
-  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, s, dacc, (int32_t)A->nslices, dacc + A->nslices);
-  QH_HIP(hipGetLastError());
-  QH_HIP(hipMemcpyAsync(hacc, dacc + A->nslices, sizeof(double), hipMemcpyDeviceToHost, s));
-  QH_HIP(hipStreamSynchronize(s));
-  *err = sqrt(hacc[0] / (double)A->rows);
-  return QH_OK;
+  if (!(G == 128 || G == 134 || G == 136) || A->ks_state != 1) return QH_OK;
+  *done = 1;
+  return rk3bs_ks(G, A, x, xn, k1, k2, k3, k4, y, h, atol, rtol, have_k1, err, td4);
 }
 
 extern "C" int qh_axpy_to(qh_vec *y, const qh_vec *x, double cr, double ci, const qh_vec *k) {

@@ -722,3 +722,44 @@ extern "C" int qh_rk4_td_step(qh_mat *A, int ntd, qh_mat *const *Ak, const doubl
                                c->k[0], c->k[1], c->k[2], h);
   return rc;
 }
+
+extern "C" int qh_rk3bs_td_step(qh_mat *A, int ntd, qh_mat *const *Ak, const double *s4_re_im, const qh_vec *x,
+                                qh_vec *xn, qh_vec *k1, qh_vec *k2, qh_vec *k3, qh_vec *k4, qh_vec *y, double h,
+                                double atol, double rtol, int have_k1, double *err) {
+  int rc = qh_ensure_init();
+  if (rc) return rc;
+  QH_REQUIRE(A && x && (ntd == 0 || (Ak && s4_re_im)), QH_EINVAL, "qh_rk3bs_td_step: null argument");
+  QH_REQUIRE(x->n == A->rows && x->prec == QH_FP64, QH_EINVAL,
+             "qh_rk3bs_td_step: FP64 state of the matrix dimension required");
+  qh_td_cache *c = nullptr;
+  rc = td_cache_get(A, ntd, Ak, &c);
+  if (rc) return rc;
+  if (g_td_sliced && c->soff && g_td_lanes == 0) {
+    // the four stage tables in one upload, the entries summed in A's stage kernels
+    const int st = c->stride();
+    std::vector<double2> hs((size_t)4 * st);
+    for (int j = 0; j < 4; j++) td_table(c, ntd, s4_re_im + 2 * (size_t)j * ntd, hs.data() + (size_t)j * st);
+    QH_HIP(hipMemcpyAsync(c->s, hs.data(), sizeof(double2) * 4 * st, hipMemcpyHostToDevice, qh_stream()));
+    qh_td_ell t4[4];
+    for (int j = 0; j < 4; j++) t4[j] = qh_td_ell{c->soff, c->sw, c->ecol, c->eidx, c->s + (size_t)j * st};
+    int done = 0;
+    rc = qh_rk3bs_td_sliced(A, x, xn, k1, k2, k3, k4, y, h, atol, rtol, have_k1, err, t4, &done);
+    if (rc || done) return rc;
+  }
+  // four RHS evaluations and the combinations
+  const double *s0 = s4_re_im, *s1 = s4_re_im + 2 * (size_t)ntd, *s2 = s4_re_im + 4 * (size_t)ntd,
+               *s3 = s4_re_im + 6 * (size_t)ntd;
+  if (!have_k1) rc = qh_spmv_td(A, ntd, Ak, s0, x, k1);
+  if (!rc) rc = qh_axpy_to(y, x, 0.5 * h, 0.0, k1);
+  if (!rc) rc = qh_spmv_td(A, ntd, Ak, s1, y, k2);
+  if (!rc) rc = qh_axpy_to(y, x, 0.75 * h, 0.0, k2);
+  if (!rc) rc = qh_spmv_td(A, ntd, Ak, s2, y, k3);
+  if (!rc) {
+    const double cf[8] = {1.0, 0.0, h * 2.0 / 9.0, 0.0, h / 3.0, 0.0, h * 4.0 / 9.0, 0.0};
+    qh_vec *v[4] = {const_cast<qh_vec *>(x), k1, k2, k3};
+    rc = qh_lincomb(xn, 4, cf, v);
+  }
+  if (!rc) rc = qh_spmv_td(A, ntd, Ak, s3, xn, k4);
+  if (!rc) rc = qh_rk3bs_err(x, xn, k1, k2, k3, k4, h, atol, rtol, err);
+  return rc;
+}
