@@ -575,6 +575,9 @@ void time_step_sys(qsystem qsys, qvec x, PetscReal init_time, PetscReal time_max
   if (x->n != qsys->dim) QUAC_FAIL(0, "time_step_sys: state size %lld does not match the system dimension %lld",
                                    (long long)x->n, (long long)qsys->dim);
   if (dt <= 0) QUAC_FAIL(0, "time_step_sys: dt must be positive");
+  if (qsys->mcwf_solver && qsys->num_time_dep > 0)
+    QUAC_FAIL(9, "time_step_sys: time-dependent terms with use_mcwf_solver are not supported (the trajectories "
+                 "step with the time-independent effective Hamiltonian on the device)");
   if (qsys->mcwf_solver) {
     /* trajectories of the ensemble (qsystem.c:1024-1079); a circuit's gates
      * are events (qsystem.c:1006-1019, quantum_circuits.c:17-120): the ensemble

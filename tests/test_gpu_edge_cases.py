@@ -121,6 +121,25 @@ def test_smallest_mcwf_ensemble():
     qs.destroy()
 
 
+def test_mcwf_with_time_dependent_terms_is_refused():
+    """the MCWF trajectories step with the time-independent effective
+    Hamiltonian on the device; time_step_sys refuses an ensemble whose system has
+    time-dependent terms with an error, not a silent result without them"""
+    _fixed_rk4()
+    qs = qa.System()
+    q = qs.create_op(2)
+    qs.add_ham_term(1.0, q.n)
+    qs.add_lin_term(0.5, q)
+    qs.add_ham_term_time_dep(0.3, lambda t: 1.0, q)
+    qs.use_mcwf_solver(4, 7)
+    qs.construct_matrix()
+    x = qs.create_qvec()
+    x.add(1.0, 1)
+    with pytest.raises(qa.QuacError, match="time-dependent"):
+        qs.time_step(x, 0.0, 0.1, 0.01, 100)
+    qs.destroy()
+
+
 def test_stream_calibration_and_ragged_vector_kernels():
     c, r = qa.bench_stream(1 << 24, 2)
     assert 100.0 < c < 20000.0 and 100.0 < r < 20000.0
