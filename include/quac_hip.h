@@ -151,6 +151,14 @@ int qh_spmv_td(const qh_mat *A, int ntd, qh_mat *const *Ak, const double *s_re_i
  * time_step_sys with time-dependent terms (qsystem.c:946-950, 1288-1330); the
  * stage buffers are kept with the cached merged structure. */
 int qh_rk4_td_step(qh_mat *A, int ntd, qh_mat *const *Ak, const double *s4_re_im, qh_vec *x, double h);
+/* nsteps consecutive RK4 steps of size h (the run of fixed steps of
+ * time_step_sys between monitor calls and events): s_re_im holds, for step j,
+ * the coefficients of its four stage times t_j, t_j + h/2, t_j + h/2, t_j + h
+ * (nsteps x 4 x ntd complex).  A generator of at most 800 rows (no variant
+ * forced, dictionary form) runs the whole run in one workgroup with the state in
+ * LDS; otherwise qh_rk4_td_step per step. */
+int qh_rk4_td_steps(qh_mat *A, int ntd, qh_mat *const *Ak, const double *s_re_im, int64_t nsteps, qh_vec *x,
+                    double h);
 /* Switches of the two functions above (no counterpart in the reference; for
  * tuning and for tests that compare the forms): dict_max = the largest
  * (term, value) dictionary (0: every entry stores its term and value; default

@@ -521,6 +521,30 @@ static int rk4_td_step(qsystem s, Vec x, PetscReal t, PetscReal h) {
   return qh_rk4_td_step(s->mat_A, n, mats, sc, x, h);
 }
 
+/* nfull RK4 steps of length h from time t with the time-dependent terms: the
+ * envelopes at the four stage times of every step evaluated here (in step
+ * order, once per stage), the steps run in chunks of up to 256 by
+ * qh_rk4_td_steps (one workgroup for a generator of at most 800 rows) */
+static int rk4_td_run(qsystem s, Vec x, PetscReal t, PetscReal h, PetscInt nfull) {
+  const int n = (int)s->num_time_dep;
+  const PetscInt chunk = 256;
+  double *sc = malloc(sizeof(double) * 8 * (size_t)n * (size_t)chunk + sizeof(double));
+  Mat mats[NUM_INIT_ALLOC];
+  if (!sc) return 1;
+  int rc = 0;
+  for (PetscInt j0 = 0; j0 < nfull && !rc; j0 += chunk) {
+    const PetscInt m = nfull - j0 < chunk ? nfull - j0 : chunk;
+    for (PetscInt j = 0; j < m; j++) {
+      const PetscReal tj = t + (double)(j0 + j) * h;
+      const PetscReal ts[4] = {tj, tj + 0.5 * h, tj + 0.5 * h, tj + h};
+      for (int q = 0; q < 4; q++) td_coefs(s, ts[q], sc + 2 * (size_t)n * (4 * (size_t)j + q), mats);
+    }
+    rc = qh_rk4_td_steps(s->mat_A, n, mats, sc, m, x, h);
+  }
+  free(sc);
+  return rc;
+}
+
 typedef struct {
   Vec k[4], y, xn;
 } work_t;
@@ -660,7 +684,8 @@ void time_step_sys(qsystem qsys, qvec x, PetscReal init_time, PetscReal time_max
       if (td) {
         const PetscReal hs = (nfull == 1 && h != dt) ? h : dt;
         int bad = 0;
-        for (PetscInt j = 0; j < nfull && !bad; j++) {
+        if (!renorm_ok) bad = rk4_td_run(qsys, work, t, hs, nfull);
+        for (PetscInt j = 0; j < nfull && !bad && renorm_ok; j++) {
           bad = rk4_td_step(qsys, work, t + (double)j * hs, hs);
           /* FP32: trace renormalisation every quac_renorm steps */
           if (!bad && renorm_ok && (step + j + 1) % renorm == 0) bad = renorm_trace(work, x->total_levels);

@@ -230,6 +230,90 @@ __global__ void __launch_bounds__(256) k_td_ell(int32_t nrows, const int64_t *__
   st_state(x, lo, row, xo.x + s6 * re, xo.y + s6 * im);
 }
 
+// nsteps classic RK4 steps of a generator of at most 800 rows with the
+// time-dependent entries (dictionary form) in one workgroup: the state and the
+// stage derivatives in LDS (FP64; an FP32 state is widened from hi + lo and
+// split again at the end), stage s of step j taking the coefficient table
+// cf + (4 j + s) st.  Each stage input is formed on the fly from the previous
+// stage (x + c k), as in the one-workgroup RK4 of a time-independent generator.
+template <bool F32>
+__global__ void __launch_bounds__(1024) k_rk4_td_small(int32_t n, const int32_t *__restrict__ rp,
+                                                       const int32_t *__restrict__ col,
+                                                       const double2 *__restrict__ val,
+                                                       const int32_t *__restrict__ trp,
+                                                       const int32_t *__restrict__ tcol,
+                                                       const uint16_t *__restrict__ tidx,
+                                                       const double2 *__restrict__ cf, int32_t st, void *xg,
+                                                       float2 *xlo, double h, int64_t nsteps) {
+  extern __shared__ double2 sm[];
+  double2 *xa = sm, *xb = sm + n, *k1 = sm + 2 * n, *k2 = sm + 3 * n, *k3 = sm + 4 * n;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  for (int r = tid; r < n; r += nt) {
+    if (F32) {
+      const float2 hi = ((const float2 *)xg)[r], lo = xlo[r];
+      xa[r] = make_double2((double)hi.x + (double)lo.x, (double)hi.y + (double)lo.y);
+    } else {
+      xa[r] = ((const double2 *)xg)[r];
+    }
+  }
+  __syncthreads();
+  auto rowdot = [&](int r, const double2 *xs, const double2 *kin, double c, const double2 *tab) {
+    double are = 0.0, aim = 0.0;
+    for (int32_t k = rp[r]; k < rp[r + 1]; k++) {
+      const int32_t cc = col[k];
+      double2 v = xs[cc];
+      if (kin) {
+        v.x += c * kin[cc].x;
+        v.y += c * kin[cc].y;
+      }
+      const double2 a = val[k];
+      are = fma(a.x, v.x, fma(-a.y, v.y, are));
+      aim = fma(a.x, v.y, fma(a.y, v.x, aim));
+    }
+    for (int32_t k = trp[r]; k < trp[r + 1]; k++) {
+      const int32_t cc = tcol[k];
+      double2 v = xs[cc];
+      if (kin) {
+        v.x += c * kin[cc].x;
+        v.y += c * kin[cc].y;
+      }
+      const double2 a = tab[tidx[k]];
+      are = fma(a.x, v.x, fma(-a.y, v.y, are));
+      aim = fma(a.x, v.y, fma(a.y, v.x, aim));
+    }
+    return make_double2(are, aim);
+  };
+  const double s6 = h * (1.0 / 6.0), s3 = h * (1.0 / 3.0);
+  for (int64_t j = 0; j < nsteps; j++) {
+    const double2 *tb = cf + 4 * j * (int64_t)st;
+    for (int r = tid; r < n; r += nt) k1[r] = rowdot(r, xa, nullptr, 0.0, tb);
+    __syncthreads();
+    for (int r = tid; r < n; r += nt) k2[r] = rowdot(r, xa, k1, 0.5 * h, tb + st);
+    __syncthreads();
+    for (int r = tid; r < n; r += nt) k3[r] = rowdot(r, xa, k2, 0.5 * h, tb + 2 * st);
+    __syncthreads();
+    for (int r = tid; r < n; r += nt) {
+      const double2 k4 = rowdot(r, xa, k3, h, tb + 3 * st), a = k1[r], b = k2[r], c = k3[r], xo = xa[r];
+      xb[r] = make_double2(xo.x + s6 * a.x + s3 * b.x + s3 * c.x + s6 * k4.x,
+                           xo.y + s6 * a.y + s3 * b.y + s3 * c.y + s6 * k4.y);
+    }
+    __syncthreads();
+    double2 *t = xa;
+    xa = xb;
+    xb = t;
+  }
+  for (int r = tid; r < n; r += nt) {
+    const double2 v = xa[r];
+    if (F32) {
+      const float2 hi = make_float2((float)v.x, (float)v.y);
+      ((float2 *)xg)[r] = hi;
+      xlo[r] = make_float2((float)(v.x - (double)hi.x), (float)(v.y - (double)hi.y));
+    } else {
+      ((double2 *)xg)[r] = v;
+    }
+  }
+}
+
 __global__ void k_narrow_td(const double2 *a, float2 *b, int64_t n) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     b[i] = make_float2((float)a[i].x, (float)a[i].y);
@@ -242,6 +326,9 @@ __global__ void k_narrow_td(const double2 *a, float2 *b, int64_t n) {
 // merged rows' average length)
 static int g_dict_max = 4096;
 static const int TD_DICT_CAP = 8192;
+// generators of at most this many rows take a run of RK4 steps in one workgroup
+// (five state-sized FP64 arrays in LDS: 64 KB at 800 rows)
+#define QH_TD_SMALL_ROWS 800
 static int g_td_sliced = 1;
 static int g_td_lanes = 0;
 
@@ -264,6 +351,8 @@ struct qh_td_cache {
   std::vector<int> dk;              // the dictionary's terms
   std::vector<double2> dv;          // and values (host)
   double2 *s = nullptr;             // 4 stages x (TD_MAX coefficients, or the dictionary's s_k v)
+  double2 *steps_cf = nullptr;      // the stage tables of a run of steps (one-workgroup path)
+  size_t steps_cap = 0;             // its capacity (tables)
   int prec = 0;                     // precision of the stage buffers
   void *k[3] = {nullptr, nullptr, nullptr};
   void *y[2] = {nullptr, nullptr};
@@ -297,6 +386,7 @@ void qh_td_cache_free(qh_td_cache *c) {
   qh_free_ptr(c->ecol);
   qh_free_ptr(c->eidx);
   qh_free_ptr(c->s);
+  qh_free_ptr(c->steps_cf);
   td_free_bufs(c);
   delete c;
 }
@@ -762,4 +852,52 @@ extern "C" int qh_rk3bs_td_step(qh_mat *A, int ntd, qh_mat *const *Ak, const dou
   if (!rc) rc = qh_spmv_td(A, ntd, Ak, s3, xn, k4);
   if (!rc) rc = qh_rk3bs_err(x, xn, k1, k2, k3, k4, h, atol, rtol, err);
   return rc;
+}
+
+extern "C" int qh_rk4_td_steps(qh_mat *A, int ntd, qh_mat *const *Ak, const double *s_re_im, int64_t nsteps,
+                               qh_vec *x, double h) {
+  int rc = qh_ensure_init();
+  if (rc) return rc;
+  QH_REQUIRE(A && x && (ntd == 0 || (Ak && s_re_im)), QH_EINVAL, "qh_rk4_td_steps: null argument");
+  QH_REQUIRE(x->n == A->rows && nsteps >= 0, QH_EINVAL, "qh_rk4_td_steps: state of the matrix dimension required");
+  if (nsteps == 0) return QH_OK;
+  qh_td_cache *c = nullptr;
+  rc = td_cache_get(A, ntd, Ak, &c);
+  if (rc) return rc;
+  const int64_t n = A->rows;
+  if (n > QH_TD_SMALL_ROWS || A->lanes_forced || !c->dict()) {
+    // step by step
+    for (int64_t j = 0; j < nsteps && !rc; j++)
+      rc = qh_rk4_td_step(A, ntd, Ak, s_re_im + 8 * (size_t)ntd * j, x, h);
+    return rc;
+  }
+  // the stage tables of every step in one upload, then one workgroup
+  const int st = c->stride();
+  const size_t need = (size_t)4 * st * nsteps;
+  if (c->steps_cap < need) {
+    qh_free_ptr(c->steps_cf);
+    c->steps_cf = nullptr;
+    c->steps_cap = 0;
+    rc = qh_alloc_t(&c->steps_cf, need);
+    if (rc) return rc;
+    c->steps_cap = need;
+  }
+  std::vector<double2> hs(need);
+  for (int64_t j = 0; j < nsteps; j++)
+    for (int q = 0; q < 4; q++)
+      td_table(c, ntd, s_re_im + 2 * (size_t)ntd * (4 * j + q), hs.data() + (size_t)st * (4 * j + q));
+  hipStream_t sm = qh_stream();
+  QH_HIP(hipMemcpyAsync(c->steps_cf, hs.data(), sizeof(double2) * need, hipMemcpyHostToDevice, sm));
+  const int bt = (int)std::min<int64_t>(1024, ((n + 63) / 64) * 64);
+  const size_t lds = sizeof(double2) * 5 * (size_t)n;
+  if (x->prec == QH_FP32)
+    hipLaunchKernelGGL((k_rk4_td_small<true>), dim3(1), dim3(bt), lds, sm, (int32_t)n, A->rowptr, A->col, A->val,
+                       c->trp, c->tcol, c->tidx, c->steps_cf, st, x->d, x->lo, h, nsteps);
+  else
+    hipLaunchKernelGGL((k_rk4_td_small<false>), dim3(1), dim3(bt), lds, sm, (int32_t)n, A->rowptr, A->col, A->val,
+                       c->trp, c->tcol, c->tidx, c->steps_cf, st, x->d, nullptr, h, nsteps);
+  QH_HIP(hipGetLastError());
+  // the tables are read by the kernel: keep the host copy alive until it ends
+  QH_HIP(hipStreamSynchronize(sm));
+  return QH_OK;
 }

@@ -238,6 +238,43 @@ def test_td_rk4_sliced_autotuned_vs_fused():
     assert np.abs(populations_np(a, recipe) - populations_np(b, recipe)).max() < 1e-6
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_td_rk4_one_workgroup_run_matches_stepwise_and_oracle(prec):
+    """a qubit and a 12-level cavity (576 rows, under the 800-row limit): the
+    fixed RK4 steps of time_step_sys run in one workgroup per chunk of steps
+    (qh_rk4_td_steps, stage tables of every step computed ahead) against the
+    same steps launched one at a time (per-entry values force that path) and
+    against the oracle; 300 steps, more than one chunk of 256"""
+    recipe = _recipe(True, 12)
+    N = 24
+    rng = np.random.default_rng(33)
+    a = rng.normal(size=(N, N)) + 1j * rng.normal(size=(N, N))
+    rho = a @ a.conj().T
+    x0 = (rho / np.trace(rho)).flatten(order="F")
+    out = {}
+    for form, cfg in (("run", (4096, 1, 0)), ("stepwise", (0, 1, 0))):
+        qa.td_config(*cfg)
+        qs, q, c = _product(recipe, prec)
+        try:
+            x = qs.create_qvec()
+            x.set(x0)
+            print(f"td small run: {prec} {form}", flush=True)
+            steps, tend = qs.time_step(x, 0.1, 3.1, 0.01, 10 ** 6)
+            assert steps == 300 and abs(tend - 3.1) < 1e-9
+            out[form] = x.get()
+        finally:
+            qa.td_config()
+            qa.set_option("quac_precision", "fp64")
+            qs.destroy()
+    ref, _, _, _ = _oracle(recipe, x0, 0.1, 0.01, 300)
+    if prec == "fp64":
+        assert np.abs(out["run"] - out["stepwise"]).max() < 1e-13
+        assert np.abs(out["run"] - ref).max() < 1e-10
+    else:
+        assert np.abs(populations_np(out["run"], recipe) - populations_np(ref, recipe)).max() < 1e-6
+        assert np.abs(populations_np(out["stepwise"], recipe) - populations_np(ref, recipe)).max() < 1e-6
+
+
 def test_td_rk3bs_forms_agree_and_match_fine_rk4():
     """the default integrator (adaptive RK3BS) with time-dependent terms on a
     qubit and a 16-level cavity (1 024 rows, A forced onto variant 134): the trial
