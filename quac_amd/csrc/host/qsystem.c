@@ -527,7 +527,7 @@ static int rk4_td_step(qsystem s, Vec x, PetscReal t, PetscReal h) {
  * qh_rk4_td_steps (one workgroup for a generator of at most 800 rows) */
 static int rk4_td_run(qsystem s, Vec x, PetscReal t, PetscReal h, PetscInt nfull) {
   const int n = (int)s->num_time_dep;
-  const PetscInt chunk = 256;
+  const PetscInt chunk = nfull < 256 ? nfull : 256;
   double *sc = malloc(sizeof(double) * 8 * (size_t)n * (size_t)chunk + sizeof(double));
   Mat mats[NUM_INIT_ALLOC];
   if (!sc) return 1;
