@@ -1,8 +1,12 @@
-"""Time-dependent terms through the fused RHS kernel (qsystem.c:1288-1330):
-y = A x + sum_k s_k(t) A_k x in one launch per RK stage (qh_spmv_td,
-qh_rk4_td_step), against the oracle's qo_rk4_td (pinned by the analytic Rabi
-drive and by qo_rk4 on the summed matrix) and the closed form of
-time_dep_tests.c:142-213: FP64 to 1e-10, the FP32 state to 1e-6 on populations."""
+"""Time-dependent terms (qsystem.c:1288-1330): y = A x + sum_k s_k(t) A_k x per
+RK stage in each of its forms -- the merged A_k entries summed in A's sliced
+stage kernel, as a pass of their own (sliced form or CSR lanes), or one fused
+CSR kernel; dictionary or per-entry values; one-workgroup runs of steps for
+small generators; the adaptive RK3BS trial step (qh_spmv_td, qh_rk4_td_step,
+qh_rk4_td_steps, qh_rk3bs_td_step) -- against the oracle's qo_rk4_td (pinned by
+the analytic Rabi drive and by qo_rk4 on the summed matrix) and the closed form
+of time_dep_tests.c:142-213: FP64 to 1e-10, the FP32 state to 1e-6 on
+populations."""
 import cmath
 import ctypes
 import math
