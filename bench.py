@@ -20,6 +20,10 @@ Headline (default workload):
     compared with the CPU oracle after the same W + K steps.
   * hbm_copy: the achievable bandwidth of this GPU (a one-pass streaming copy of
     1 GiB, and the same read alone), beside the 8 TB/s peak.
+  * timedep: the same chain with ten driven sigma_x terms through time_step_sys
+    (300 fixed RK4 steps of 0.005, FP64 and FP32; FP64 also with the A_k entries
+    as a pass of their own and in the fused CSR form; the default adaptive
+    integrator over the same interval), as --workload timedep reports it.
   * cpu_baseline: the C oracle's FP64 RK4 (same CSR, row-ordered MatMult, same RK4)
     on this host's cores, on a bounded sample of the sweep (every 16th instance),
     median of 3 runs at 1 thread and at all cores of the GPU's CPU share.
@@ -414,6 +418,17 @@ def main():
     ch = run_chain(qa, recipes, build_system, subsystem_layout, K, W, args.renorm, args.nq, args.fp32_variant)
     log("copy bandwidth")
     cp = measure_copy(qa)
+    log("time-dependent terms on the chain, 300 steps")
+    import bench_workloads
+    td = bench_workloads.timedep(argparse.Namespace(steps=300))
+    qa.set_option("ts_rk_type", "4")
+    qa.set_option("ts_adapt_type", "none")
+    td_keys = ("value", "fp32_steps_per_s", "fp64_steps_per_s_separate_pass", "fp64_steps_per_s_fused_csr_form",
+               "max_pop_err_fp32_vs_fp64", "max_pop_err_fused_vs_sliced_fp64", "default_integrator_rk3bs",
+               "a_variants_fp64_fp32", "td_entry_bytes_per_stage")
+    timedep = {"fp64_steps_per_s" if k == "value" else k: td[k] for k in td_keys}
+    timedep["steps"] = td["steps"]
+    timedep["workload"] = td["config"]["workload"]
 
     cpu = None
     if not args.no_cpu:
@@ -461,6 +476,7 @@ def main():
                     "roofline_sweep": sw["roofline"], "sweep_layout": sw["layout"]})
     out["chain10"] = ch["out"]
     out["hbm_copy"] = cp
+    out["timedep"] = timedep
     print(json.dumps(out), flush=True)
 
 
