@@ -76,6 +76,19 @@ def test_loads_and_fails_loudly_without_device():
         qa.System()
 
 
+def test_td_config_validates_its_switches():
+    """qh_td_config needs no device: out-of-range switches are errors, valid
+    ones are taken (and the defaults restored)"""
+    import quac_amd as qa
+    qa.lib()
+    for bad in ((-1, 1, 0), (8193, 1, 0), (4096, 2, 0), (4096, 1, 3), (4096, 1, 32)):
+        with pytest.raises(qa.QuacError):
+            qa.td_config(*bad)
+    for ok in ((0, 0, 0), (8192, 1, 1), (4096, 1, 4), (4096, 0, 16)):
+        qa.td_config(*ok)
+    qa.td_config()
+
+
 def test_exit_semantics_match_reference_in_a_child():
     """Misuse prints ERROR! and exits (reference: exit(0) in create_op_sys after
     construct_matrix, qsystem.c:158-162) -- default error mode, run in a child."""
